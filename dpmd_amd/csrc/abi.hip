@@ -1,0 +1,879 @@
+// C ABI (include/dpmd/dpmd.h): context lifecycle, fields, validation, and the host-side
+// orchestration of the particle / integrator / routine calls.
+#include "internal.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+using namespace dpmd;
+
+namespace dpmd {
+void fire_recompute_update(dpmd_ctx* c);
+void rigid_compute_forces(dpmd_ctx* c, int mode);
+void rigid_update_positions(dpmd_ctx* c, const double* scale, double scale2);
+void rigid_update_velocities(dpmd_ctx* c, const double* scale, double scale2);
+void rigid_scale_velocities(dpmd_ctx* c, const double* scale);
+void rigid_scale_positions(dpmd_ctx* c, const double* scale);
+void rigid_mix(dpmd_ctx* c, const double* alpha);
+void rigid_stress(dpmd_ctx* c);
+void rigid_fire_drift(dpmd_ctx* c, const int* active);
+void rigid_fire_force(dpmd_ctx* c, const int* active, int kick, int do_conv, int pred, double pa, double pb);
+void rigid_vv_drift(dpmd_ctx* c, const int* active);
+void rigid_vv_force(dpmd_ctx* c, const int* active);
+}  // namespace dpmd
+
+namespace {
+
+std::string g_global_err;
+
+template <class F>
+int guard(dpmd_ctx* c, F&& f) {
+    try {
+        f();
+        return DPMD_OK;
+    } catch (const ArgError& e) {
+        (c ? c->err : g_global_err) = e.what();
+        return DPMD_ERR_ARGUMENT;
+    } catch (const StateError& e) {
+        (c ? c->err : g_global_err) = e.what();
+        return DPMD_ERR_STATE;
+    } catch (const OverflowError& e) {
+        (c ? c->err : g_global_err) = e.what();
+        return DPMD_ERR_OVERFLOW;
+    } catch (const std::exception& e) {
+        (c ? c->err : g_global_err) = e.what();
+        return DPMD_ERR_HIP;
+    }
+}
+
+void need(bool ok, const std::string& msg) { if (!ok) throw ArgError(msg); }
+void need_state(bool ok, const std::string& msg) { if (!ok) throw StateError(msg); }
+
+struct FieldRef {
+    void* p; int64_t n; int elem;   // elem: 8 = double, 4 = int
+    int domain;                     // 0 particle, 1 vertex, 2 system, 3 other
+};
+
+template <class T>
+FieldRef ref(dpmd::DVec<T>& v, int domain) { return FieldRef{(void*)v.p, v.n, (int)sizeof(T), domain}; }
+
+FieldRef field_ref(dpmd_ctx* c, int f) {
+    switch (f) {
+        case DPMD_F_POS_X: return ref(c->x, 0);
+        case DPMD_F_POS_Y: return ref(c->y, 0);
+        case DPMD_F_VEL_X: return ref(c->vx, 0);
+        case DPMD_F_VEL_Y: return ref(c->vy, 0);
+        case DPMD_F_FORCE_X: return ref(c->fx, 0);
+        case DPMD_F_FORCE_Y: return ref(c->fy, 0);
+        case DPMD_F_PE: return ref(c->pe, 0);
+        case DPMD_F_KE: return ref(c->ke, 0);
+        case DPMD_F_RAD: return ref(c->rad, 0);
+        case DPMD_F_MASS: return ref(c->mass, 0);
+        case DPMD_F_AREA: return ref(c->area, 0);
+        case DPMD_F_LAST_POS_X: return ref(c->lx, 0);
+        case DPMD_F_LAST_POS_Y: return ref(c->ly, 0);
+        case DPMD_F_ANGLE: return ref(c->angle, 0);
+        case DPMD_F_ANGULAR_VEL: return ref(c->avel, 0);
+        case DPMD_F_TORQUE: return ref(c->torque, 0);
+        case DPMD_F_MOMENT_INERTIA: return ref(c->inertia, 0);
+        case DPMD_F_STRESS_XX: return ref(c->sxx, 0);
+        case DPMD_F_STRESS_XY: return ref(c->sxy, 0);
+        case DPMD_F_STRESS_YX: return ref(c->syx, 0);
+        case DPMD_F_STRESS_YY: return ref(c->syy, 0);
+        case DPMD_F_BOX_SIZE_X: return ref(c->Lx, 2);
+        case DPMD_F_BOX_SIZE_Y: return ref(c->Ly, 2);
+        case DPMD_F_BOX_INV_X: return ref(c->iLx, 2);
+        case DPMD_F_BOX_INV_Y: return ref(c->iLy, 2);
+        case DPMD_F_E_INTERACTION: return ref(c->e, 2);
+        case DPMD_F_NEIGHBOR_CUTOFF: return ref(c->cutoff, 2);
+        case DPMD_F_THRESH2: return ref(c->thresh2, 2);
+        case DPMD_F_PE_TOTAL: return ref(c->pe_tot, 2);
+        case DPMD_F_KE_TOTAL: return ref(c->ke_tot, 2);
+        case DPMD_F_FPOWER_TOTAL: return ref(c->pw_tot, 2);
+        case DPMD_F_PRESSURE: return ref(c->press, 2);
+        case DPMD_F_TEMPERATURE: return ref(c->temp, 2);
+        case DPMD_F_PACKING_FRACTION: return ref(c->phi, 2);
+        case DPMD_F_MAX_FORCE2: return ref(c->maxf2, 2);
+        case DPMD_F_STRESS_TOTAL_XX: return ref(c->st_xx, 2);
+        case DPMD_F_STRESS_TOTAL_XY: return ref(c->st_xy, 2);
+        case DPMD_F_STRESS_TOTAL_YX: return ref(c->st_yx, 2);
+        case DPMD_F_STRESS_TOTAL_YY: return ref(c->st_yy, 2);
+        case DPMD_F_SYSTEM_OFFSET: return ref(c->off, 3);
+        case DPMD_F_SYSTEM_SIZE: return ref(c->size, 2);
+        case DPMD_F_CELL_DIM_X: return ref(c->cdx, 2);
+        case DPMD_F_CELL_DIM_Y: return ref(c->cdy, 2);
+        case DPMD_F_CELL_SYSTEM_START: return ref(c->cstart, 3);
+        case DPMD_F_BOUNDARY: return ref(c->boundary, 2);
+        case DPMD_F_N_DOF: return ref(c->ndof, 2);
+        case DPMD_F_REBUILD_FLAG: return ref(c->flag, 2);
+        case DPMD_F_SYSTEM_ID: return ref(c->sid, 0);
+        case DPMD_F_STATIC_INDEX: return ref(c->static_index, c->kind == DPMD_RIGID_BUMPY ? 1 : 0);
+        case DPMD_F_CELL_ID: return ref(c->cell_id, c->kind == DPMD_RIGID_BUMPY ? 1 : 0);
+        case DPMD_F_NEIGHBOR_COUNT: return ref(c->nn, c->kind == DPMD_RIGID_BUMPY ? 1 : 0);
+        case DPMD_F_VERTEX_POS_X: return ref(c->vpx, 1);
+        case DPMD_F_VERTEX_POS_Y: return ref(c->vpy, 1);
+        case DPMD_F_VERTEX_FORCE_X: return ref(c->vfx, 1);
+        case DPMD_F_VERTEX_FORCE_Y: return ref(c->vfy, 1);
+        case DPMD_F_VERTEX_PE: return ref(c->vpe, 1);
+        case DPMD_F_VERTEX_RAD: return ref(c->vrad, 1);
+        case DPMD_F_VERTEX_MASS: return ref(c->vmass, 1);
+        case DPMD_F_VERTEX_PARTICLE_ID: return ref(c->vpid, 1);
+        case DPMD_F_PARTICLE_OFFSET: return ref(c->poff, 3);
+        case DPMD_F_N_VERTICES_PER_PARTICLE: return ref(c->nvpp, 0);
+        case DPMD_F_VERTEX_SYSTEM_ID: return ref(c->vsid, 1);
+        case DPMD_F_VERTEX_SYSTEM_OFFSET: return ref(c->voff, 3);
+    }
+    throw ArgError("unknown field id " + std::to_string(f));
+}
+
+template <class T>
+std::vector<T> dl(dpmd_ctx* c, const dpmd::DVec<T>& v) {
+    std::vector<T> h(v.n);
+    if (v.n) {
+        HIP_OK(hipMemcpyAsync(h.data(), v.p, v.n * sizeof(T), hipMemcpyDeviceToHost, c->st));
+        HIP_OK(hipStreamSynchronize(c->st));
+    }
+    return h;
+}
+
+const int* active_one(dpmd_ctx* c) { return c->one.p; }
+
+bool rigid(dpmd_ctx* c) { return c->kind == DPMD_RIGID_BUMPY; }
+
+// Cell layout (init_cell_sizes, base_particle.hpp:552-592).  A zero cell dimension is
+// derived from the box: the most cells whose side still covers 2*r_max + cutoff.
+void setup_cells(dpmd_ctx* c, bool validate) {
+    const int S = c->S;
+    auto cdx = dl(c, c->cdx), cdy = dl(c, c->cdy);
+    auto Lx = dl(c, c->Lx), Ly = dl(c, c->Ly), cut = dl(c, c->cutoff);
+    auto rad = rigid(c) ? dl(c, c->vrad) : dl(c, c->rad);
+    const std::vector<int>& off = rigid(c) ? c->h_voff : c->h_off;
+    bool changed = false;
+    for (int s = 0; s < S; ++s) {
+        double rmax = 0.0;
+        for (int i = off[s]; i < off[s + 1]; ++i) rmax = std::max(rmax, rad[i]);
+        const double need_sz = 2.0 * rmax + cut[s];
+        if (cdx[s] <= 0 || cdy[s] <= 0 || (validate && (Lx[s] / cdx[s] < need_sz || Ly[s] / cdy[s] < need_sz))) {
+            const int nx = (int)std::floor(Lx[s] / need_sz), ny = (int)std::floor(Ly[s] / need_sz);
+            if (validate || cdx[s] <= 0 || cdy[s] <= 0) {
+                cdx[s] = std::max(3, nx); cdy[s] = std::max(3, ny);
+                changed = true;
+            }
+        }
+        if (validate) {
+            need(cdx[s] >= 3 && cdy[s] >= 3,
+                 "cell grid must be at least 3x3 per system (system " + std::to_string(s) +
+                     "): the periodic 3x3 stencil would visit a cell twice");
+            need(Lx[s] / cdx[s] >= need_sz && Ly[s] / cdy[s] >= need_sz,
+                 "cell side must cover 2*max radius + neighbor cutoff (system " + std::to_string(s) +
+                     "); the box is too small for a 3x3 grid of such cells");
+        }
+    }
+    if (changed) {
+        c->cdx.upload(cdx.data(), S, c->st);
+        c->cdy.upload(cdy.data(), S, c->st);
+    }
+    std::vector<int> cs(S + 1, 0);
+    for (int s = 0; s < S; ++s) cs[s + 1] = cs[s] + std::max(0, cdx[s]) * std::max(0, cdy[s]);
+    c->cstart.upload(cs.data(), S + 1, c->st);
+    c->C = cs[S];
+    std::vector<int> csys(std::max(1, c->C), 0);
+    for (int s = 0; s < S; ++s) for (int k = cs[s]; k < cs[s + 1]; ++k) csys[k] = s;
+    c->cell_sys.upload(csys.data(), csys.size(), c->st);
+    c->cell_count.resize(std::max(1, c->C)); c->cell_count.zero(c->st);
+    c->cell_start.resize(c->C + 1); c->cell_start.zero(c->st);
+    c->cell_wptr.resize(std::max(1, c->C));
+    c->scan_part.resize(std::max(1, (c->C + 2047) / 2048));
+}
+
+void alloc_obj_buffers(dpmd_ctx* c) {
+    const int n = c->n_obj();
+    c->cell_id.resize(std::max(1, n));
+    c->order.resize(std::max(1, n));
+    c->order_inv.resize(std::max(1, n));
+    c->tmp_d.resize((int64_t)std::max(1, n) * 10);
+    c->tmp_i.resize((int64_t)std::max(1, n) * 2);
+    c->nn.resize(std::max(1, n)); c->nn.zero(c->st);
+    c->nbr.resize((int64_t)std::max(1, n) * c->K);
+}
+
+void sync_impl(dpmd_ctx* c) {
+    need(c->S > 0, "dpmd_sync: allocate_systems first");
+    need(c->N > 0, "dpmd_sync: allocate_particles first");
+    const int S = c->S, N = c->N;
+    c->h_off = dl(c, c->off);
+    need(c->h_off[0] == 0 && c->h_off[S] == N, "system_offset must start at 0 and end at n_particles");
+    for (int s = 0; s < S; ++s)
+        need(c->h_off[s + 1] > c->h_off[s], "every system needs at least one particle (system_offset must increase)");
+    std::vector<int> sid(N), size(S), ndof(S);
+    for (int s = 0; s < S; ++s) {
+        size[s] = c->h_off[s + 1] - c->h_off[s];
+        for (int i = c->h_off[s]; i < c->h_off[s + 1]; ++i) sid[i] = s;
+    }
+    c->sid.upload(sid.data(), N, c->st);
+    c->size.upload(size.data(), S, c->st);
+    if (rigid(c)) {
+        need(c->Nv > 0, "dpmd_sync: allocate_vertices first");
+        auto nvpp = dl(c, c->nvpp);
+        std::vector<int> poff(N + 1, 0), vpid(c->Nv), vsid(c->Nv);
+        for (int i = 0; i < N; ++i) { need(nvpp[i] > 0, "every rigid particle needs at least one vertex"); poff[i + 1] = poff[i] + nvpp[i]; }
+        need(poff[N] == c->Nv, "sum of n_vertices_per_particle must equal n_vertices");
+        c->h_voff.assign(S + 1, 0);
+        for (int s = 0; s < S; ++s) c->h_voff[s + 1] = poff[c->h_off[s + 1]];
+        for (int i = 0; i < N; ++i)
+            for (int v = poff[i]; v < poff[i + 1]; ++v) { vpid[v] = i; vsid[v] = sid[i]; }
+        c->poff.upload(poff.data(), N + 1, c->st);
+        c->vpid.upload(vpid.data(), c->Nv, c->st);
+        c->vsid.upload(vsid.data(), c->Nv, c->st);
+        c->voff.upload(c->h_voff.data(), S + 1, c->st);
+        auto inertia = dl(c, c->inertia);
+        for (int s = 0; s < S; ++s) {   // set_n_dof_kernel (rigid_bumpy.cu:743-757)
+            int d = 0;
+            for (int i = c->h_off[s]; i < c->h_off[s + 1]; ++i) d += 2 + (inertia[i] > 0.0 ? 1 : 0);
+            ndof[s] = d;
+        }
+    } else {
+        for (int s = 0; s < S; ++s) ndof[s] = 2 * size[s];   // Disk::set_n_dof_impl (disk.cu:1250-1253)
+    }
+    c->ndof.upload(ndof.data(), S, c->st);
+    box_inv(c);
+    build_chunks(c);
+    if (c->method == DPMD_NEIGHBOR_CELL) setup_cells(c, false);
+    alloc_obj_buffers(c);
+    c->synced = true;
+    c->nbr_ready = false;
+}
+
+void zero_scale(dpmd_ctx* c) { fill_f64(c->st, c->scr_c.p, c->S, 0.0); }
+
+void compute_forces_impl(dpmd_ctx* c, int mode) {
+    need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
+    if (rigid(c)) rigid_compute_forces(c, mode); else disk_compute_forces(c, mode);
+}
+
+void check_neighbors_impl(dpmd_ctx* c, bool force) {
+    if (c->method != DPMD_NEIGHBOR_CELL) return;
+    need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
+    rebuild_neighbors(c, active_one(c), force);
+}
+
+void init_neighbors_impl(dpmd_ctx* c) {
+    need_state(c->synced, "call dpmd_sync before init_neighbors");
+    const int n = c->n_obj();
+    HIP_OK(hipMemsetAsync(c->overflow.p, 0, sizeof(int), c->st));
+    HIP_OK(hipMemsetAsync(c->max_count.p, 0, sizeof(int), c->st));
+    if (c->method == DPMD_NEIGHBOR_NAIVE) {
+        const std::vector<int>& off = rigid(c) ? c->h_voff : c->h_off;
+        int mx = 1;
+        for (int s = 0; s < c->S; ++s) mx = std::max(mx, off[s + 1] - off[s] - 1);
+        c->K = mx;
+        alloc_obj_buffers(c);
+        naive_neighbors(c);
+    } else {
+        setup_cells(c, true);
+        alloc_obj_buffers(c);
+        std::vector<int> iota(n);
+        std::iota(iota.begin(), iota.end(), 0);
+        c->static_index.upload(iota.data(), n, c->st);
+        if (!rigid(c)) {
+            HIP_OK(hipMemcpyAsync(c->lx.p, c->x.p, c->N * 8, hipMemcpyDeviceToDevice, c->st));
+            HIP_OK(hipMemcpyAsync(c->ly.p, c->y.p, c->N * 8, hipMemcpyDeviceToDevice, c->st));
+        }
+        c->nbr_ready = true;
+        rebuild_neighbors(c, active_one(c), true);
+    }
+    c->nbr_ready = true;
+    check_overflow(c);
+}
+
+void update_positions_impl(dpmd_ctx* c, const double* sc, double s2) {
+    if (rigid(c)) rigid_update_positions(c, sc, s2); else disk_update_positions(c, sc, s2);
+}
+void update_velocities_impl(dpmd_ctx* c, const double* sc, double s2) {
+    if (rigid(c)) rigid_update_velocities(c, sc, s2); else disk_update_velocities(c, sc, s2);
+}
+void scale_velocities_impl(dpmd_ctx* c, const double* sc) {
+    if (rigid(c)) rigid_scale_velocities(c, sc); else disk_scale_velocities(c, sc);
+}
+void scale_positions_impl(dpmd_ctx* c, const double* sc) {
+    if (rigid(c)) rigid_scale_positions(c, sc); else disk_scale_positions(c, sc);
+}
+
+void set_average_velocity_impl(dpmd_ctx* c, const double* ax, const double* ay) {
+    sys_reduce(c, RED_VX, c->scr_a.p);
+    sys_reduce(c, RED_VY, c->scr_b.p);
+    subtract_mean_velocity(c, c->scr_a.p, c->scr_b.p, ax, ay);
+}
+
+void fire_init_impl(dpmd_ctx* c, const double* dt, int mode) {
+    need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
+    const int S = c->S;
+    std::vector<double> h(S);
+    HIP_OK(hipMemcpyAsync(h.data(), dt, S * 8, hipMemcpyDeviceToHost, c->st));
+    HIP_OK(hipStreamSynchronize(c->st));
+    std::vector<double> mx(S), mn(S), al(S, 0.1), pp(S, 1e9);
+    std::vector<int> z(S, 0), stopped(S, 0);
+    int running = 0;
+    for (int s = 0; s < S; ++s) {          // fire.cuh:136-147 define_arrays
+        mx[s] = h[s] * 10.0; mn[s] = h[s] * 0.001;
+        stopped[s] = h[s] == 0.0 ? 1 : 0;
+        running += stopped[s] ? 0 : 1;
+    }
+    c->f_dt.upload(h.data(), S, c->st); c->f_dtmax.upload(mx.data(), S, c->st); c->f_dtmin.upload(mn.data(), S, c->st);
+    c->f_alpha.upload(al.data(), S, c->st); c->f_peprev.upload(pp.data(), S, c->st);
+    c->f_ngood.upload(z.data(), S, c->st); c->f_nbad.upload(z.data(), S, c->st);
+    c->f_stopped.upload(stopped.data(), S, c->st);
+    int run[8] = {0, 0, running, 1, 0, 0, 0, 0};
+    c->run.upload(run, 8, c->st);
+    c->fire_mode = mode;
+    // init_impl (fire.cuh:90-96): v *= 0, then forces; the first update sees F.v = 0.
+    zero_scale(c);
+    scale_velocities_impl(c, c->scr_c.p);
+    if (rigid(c)) rigid_fire_force(c, active_one(c), 0, 0, 0, 0.0, 0.0);
+    else disk_fire_force(c, active_one(c), 0, 0, 0, 0.0, 0.0);
+    c->fire_ready = true;
+}
+
+void fire_step_body(dpmd_ctx* c, const int* active, int do_conv, int pred, double a, double b) {
+    if (rigid(c)) rigid_fire_drift(c, active); else disk_fire_drift(c, active);
+    if (c->method == DPMD_NEIGHBOR_CELL) rebuild_neighbors(c, active, false);
+    if (rigid(c)) rigid_fire_force(c, active, 1, do_conv, pred, a, b);
+    else disk_fire_force(c, active, 1, do_conv, pred, a, b);
+}
+
+void vv_step_body(dpmd_ctx* c, const int* active) {
+    if (rigid(c)) rigid_vv_drift(c, active); else disk_vv_drift(c, active);
+    if (c->method == DPMD_NEIGHBOR_CELL) rebuild_neighbors(c, active, false);
+    if (rigid(c)) rigid_vv_force(c, active); else disk_vv_force(c, active);
+}
+
+int read_running(dpmd_ctx* c, int* steps) {
+    int run[4];
+    HIP_OK(hipMemcpyAsync(run, c->run.p, sizeof(run), hipMemcpyDeviceToHost, c->st));
+    HIP_OK(hipStreamSynchronize(c->st));
+    if (steps) *steps = run[0];
+    return run[2];
+}
+
+hipGraphExec_t capture(dpmd_ctx* c, int steps_per_graph, bool fire, int pred, double a, double b) {
+    hipGraph_t g = nullptr;
+    HIP_OK(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
+    try {
+        for (int k = 0; k < steps_per_graph; ++k) {
+            run_begin_step(c);
+            if (fire) fire_step_body(c, c->run.p + 3, 1, pred, a, b);
+            else vv_step_body(c, c->run.p + 3);
+        }
+    } catch (...) {
+        (void)hipStreamEndCapture(c->st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    HIP_OK(hipStreamEndCapture(c->st, &g));
+    hipGraphExec_t ex = nullptr;
+    HIP_OK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    HIP_OK(hipGraphDestroy(g));
+    return ex;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dpmd_version(void) { return "dpmd_amd 0.1 (gfx950, fp64)"; }
+
+const char* dpmd_last_error(const dpmd_ctx* c) { return c ? c->err.c_str() : g_global_err.c_str(); }
+
+int dpmd_ctx_create(dpmd_ctx** out, int kind, void* stream) {
+    return guard(nullptr, [&] {
+        need(out != nullptr, "dpmd_ctx_create: out is NULL");
+        need(kind == DPMD_DISK || kind == DPMD_RIGID_BUMPY, "dpmd_ctx_create: unknown kind");
+        dpmd_ctx* c = new dpmd_ctx();
+        c->kind = kind;
+        if (kind == DPMD_RIGID_BUMPY) c->K = 48;
+        try {
+            if (stream) c->st = (hipStream_t)stream;
+            else HIP_OK(hipStreamCreate(&c->st));
+            c->one.resize(1);
+            int one = 1;
+            c->one.upload(&one, 1, c->st);
+            c->run.resize(8); c->run.zero(c->st);
+            c->overflow.resize(1); c->overflow.zero(c->st);
+            c->max_count.resize(1); c->max_count.zero(c->st);
+        } catch (...) { delete c; throw; }
+        c->err.clear();
+        *out = c;
+    });
+}
+
+int dpmd_ctx_destroy(dpmd_ctx* c) {
+    if (!c) return DPMD_OK;
+    (void)hipStreamSynchronize(c->st);
+    if (c->g_fire) (void)hipGraphExecDestroy(c->g_fire);
+    if (c->g_vv) (void)hipGraphExecDestroy(c->g_vv);
+    // All DVec members free in their destructors' absence: release explicitly.
+    dpmd::DVec<double>* dv[] = {&c->Lx, &c->Ly, &c->iLx, &c->iLy, &c->e, &c->cutoff, &c->thresh2, &c->pe_tot,
+                                &c->ke_tot, &c->pw_tot, &c->press, &c->temp, &c->phi, &c->maxf2, &c->st_xx,
+                                &c->st_xy, &c->st_yx, &c->st_yy, &c->x, &c->y, &c->vx, &c->vy, &c->fx, &c->fy,
+                                &c->pe, &c->ke, &c->rad, &c->mass, &c->area, &c->lx, &c->ly, &c->angle, &c->avel,
+                                &c->torque, &c->inertia, &c->sxx, &c->sxy, &c->syx, &c->syy, &c->vpx, &c->vpy,
+                                &c->vfx, &c->vfy, &c->vpe, &c->vrad, &c->vmass, &c->tmp_d, &c->part, &c->f_dt,
+                                &c->f_dtmax, &c->f_dtmin, &c->f_alpha, &c->f_peprev, &c->s_dt, &c->s_alpha,
+                                &c->s_dtrev, &c->s_vscale, &c->vv_dt, &c->vv_dth, &c->sv_x, &c->sv_y, &c->sv_rad,
+                                &c->sv_mass, &c->sv_Lx, &c->sv_Ly, &c->sv_angle, &c->sv_vx, &c->sv_vy, &c->scr_a,
+                                &c->scr_b, &c->scr_c};
+    for (auto* v : dv) v->release();
+    dpmd::DVec<int>* iv[] = {&c->off, &c->size, &c->cdx, &c->cdy, &c->cstart, &c->boundary, &c->ndof, &c->voff,
+                             &c->sid, &c->static_index, &c->vpid, &c->poff, &c->nvpp, &c->vsid, &c->cell_id,
+                             &c->cell_count, &c->cell_start, &c->cell_wptr, &c->cell_sys, &c->order,
+                             &c->order_inv, &c->tmp_i, &c->nn, &c->nbr, &c->overflow, &c->max_count,
+                             &c->scan_part, &c->chunk_sys, &c->chunk_beg, &c->sys_chunk, &c->f_ngood,
+                             &c->f_nbad, &c->f_stopped, &c->s_ngood, &c->s_nbad, &c->run, &c->one, &c->sv_static};
+    for (auto* v : iv) v->release();
+    c->flag.release();
+    c->sys_counter.release();
+    delete c;
+    return DPMD_OK;
+}
+
+int dpmd_allocate_systems(dpmd_ctx* c, int S) {
+    return guard(c, [&] {
+        need(S > 0, "allocate_systems: n_systems must be positive");
+        c->S = S;
+        dpmd::DVec<double>* dv[] = {&c->Lx, &c->Ly, &c->iLx, &c->iLy, &c->e, &c->cutoff, &c->thresh2, &c->pe_tot,
+                                    &c->ke_tot, &c->pw_tot, &c->press, &c->temp, &c->phi, &c->maxf2, &c->st_xx,
+                                    &c->st_xy, &c->st_yx, &c->st_yy, &c->scr_a, &c->scr_b, &c->scr_c,
+                                    &c->f_dt, &c->f_dtmax, &c->f_dtmin, &c->f_alpha, &c->f_peprev, &c->s_dt,
+                                    &c->s_alpha, &c->s_dtrev, &c->s_vscale, &c->vv_dt, &c->vv_dth};
+        for (auto* v : dv) { v->resize(S); v->zero(c->st); }
+        dpmd::DVec<int>* iv[] = {&c->size, &c->cdx, &c->cdy, &c->boundary, &c->ndof, &c->f_ngood, &c->f_nbad,
+                                 &c->f_stopped, &c->s_ngood, &c->s_nbad};
+        for (auto* v : iv) { v->resize(S); v->zero(c->st); }
+        c->off.resize(S + 1); c->off.zero(c->st);
+        c->cstart.resize(S + 1); c->cstart.zero(c->st);
+        c->voff.resize(S + 1); c->voff.zero(c->st);
+        c->flag.resize(S); c->flag.zero(c->st);
+        c->synced = false; c->nbr_ready = false; c->fire_ready = false; c->vv_ready = false;
+    });
+}
+
+int dpmd_allocate_particles(dpmd_ctx* c, int N) {
+    return guard(c, [&] {
+        need(N > 0, "allocate_particles: n_particles must be positive");
+        c->N = N;
+        dpmd::DVec<double>* dv[] = {&c->x, &c->y, &c->vx, &c->vy, &c->fx, &c->fy, &c->pe, &c->ke, &c->rad,
+                                    &c->mass, &c->area, &c->lx, &c->ly, &c->angle, &c->avel, &c->torque,
+                                    &c->inertia, &c->sxx, &c->sxy, &c->syx, &c->syy};
+        for (auto* v : dv) { v->resize(N); v->zero(c->st); }
+        c->sid.resize(N); c->sid.zero(c->st);
+        c->nvpp.resize(N); c->nvpp.zero(c->st);
+        c->poff.resize(N + 1); c->poff.zero(c->st);
+        if (!rigid(c)) {
+            std::vector<int> iota(N);
+            std::iota(iota.begin(), iota.end(), 0);
+            c->static_index.upload(iota.data(), N, c->st);
+        }
+        c->synced = false; c->nbr_ready = false; c->fire_ready = false; c->vv_ready = false;
+    });
+}
+
+int dpmd_allocate_vertices(dpmd_ctx* c, int Nv) {
+    return guard(c, [&] {
+        need(rigid(c), "allocate_vertices: only rigid bumpy contexts have vertices");
+        need(Nv > 0, "allocate_vertices: n_vertices must be positive");
+        c->Nv = Nv;
+        dpmd::DVec<double>* dv[] = {&c->vpx, &c->vpy, &c->vfx, &c->vfy, &c->vpe, &c->vrad, &c->vmass};
+        for (auto* v : dv) { v->resize(Nv); v->zero(c->st); }
+        c->vpid.resize(Nv); c->vpid.zero(c->st);
+        c->vsid.resize(Nv); c->vsid.zero(c->st);
+        std::vector<int> iota(Nv);
+        std::iota(iota.begin(), iota.end(), 0);
+        c->static_index.upload(iota.data(), Nv, c->st);
+        c->synced = false; c->nbr_ready = false;
+    });
+}
+
+int dpmd_set_neighbor_method(dpmd_ctx* c, int m) {
+    return guard(c, [&] {
+        need(m == DPMD_NEIGHBOR_NAIVE || m == DPMD_NEIGHBOR_CELL, "unknown neighbor method");
+        c->method = m;
+        c->nbr_ready = false;
+    });
+}
+
+int dpmd_set_neighbor_capacity(dpmd_ctx* c, int k) {
+    return guard(c, [&] {
+        need(k > 0 && k <= 4096, "neighbor capacity must be in [1, 4096]");
+        c->K = k;
+        if (c->synced) alloc_obj_buffers(c);
+        c->nbr_ready = false;
+    });
+}
+
+int dpmd_sizes(const dpmd_ctx* c, int* S, int* N, int* Nv, int* C) {
+    if (!c) return DPMD_ERR_ARGUMENT;
+    if (S) *S = c->S;
+    if (N) *N = c->N;
+    if (Nv) *Nv = c->Nv;
+    if (C) *C = c->C;
+    return DPMD_OK;
+}
+
+int dpmd_field(dpmd_ctx* c, int f, void** p, int64_t* n) {
+    return guard(c, [&] {
+        FieldRef r = field_ref(c, f);
+        if (p) *p = r.p;
+        if (n) *n = r.n;
+    });
+}
+
+int dpmd_set_field(dpmd_ctx* c, int f, const void* host, int64_t count) {
+    return guard(c, [&] {
+        FieldRef r = field_ref(c, f);
+        need(count == r.n, "set_field: count " + std::to_string(count) + " != field length " + std::to_string(r.n));
+        if (count) HIP_OK(hipMemcpyAsync(r.p, host, count * r.elem, hipMemcpyHostToDevice, c->st));
+        HIP_OK(hipStreamSynchronize(c->st));
+        if (f == DPMD_F_SYSTEM_OFFSET || f == DPMD_F_CELL_DIM_X || f == DPMD_F_CELL_DIM_Y ||
+            f == DPMD_F_N_VERTICES_PER_PARTICLE)
+            c->synced = false;
+    });
+}
+
+int dpmd_get_field(dpmd_ctx* c, int f, void* host, int64_t count, int order) {
+    return guard(c, [&] {
+        FieldRef r = field_ref(c, f);
+        need(count == r.n, "get_field: count " + std::to_string(count) + " != field length " + std::to_string(r.n));
+        HIP_OK(hipStreamSynchronize(c->st));
+        if (count == 0) return;
+        const bool reordered = (r.domain == 0 && !rigid(c)) || (r.domain == 1 && rigid(c));
+        if (order == DPMD_ORDER_CANONICAL && reordered && c->method == DPMD_NEIGHBOR_CELL &&
+            f != DPMD_F_STATIC_INDEX && f != DPMD_F_SYSTEM_ID && f != DPMD_F_VERTEX_SYSTEM_ID) {
+            std::vector<char> cur(count * r.elem);
+            HIP_OK(hipMemcpy(cur.data(), r.p, count * r.elem, hipMemcpyDeviceToHost));
+            auto st = dl(c, c->static_index);
+            char* out = (char*)host;
+            for (int64_t k = 0; k < count; ++k) std::memcpy(out + k * r.elem, cur.data() + (int64_t)st[k] * r.elem, r.elem);
+        } else {
+            HIP_OK(hipMemcpy(host, r.p, count * r.elem, hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int dpmd_sync(dpmd_ctx* c) { return guard(c, [&] { sync_impl(c); }); }
+
+int dpmd_init_neighbors(dpmd_ctx* c) { return guard(c, [&] { init_neighbors_impl(c); }); }
+
+int dpmd_update_neighbors(dpmd_ctx* c) {
+    return guard(c, [&] {
+        if (c->method == DPMD_NEIGHBOR_NAIVE) { naive_neighbors(c); return; }
+        check_neighbors_impl(c, true);
+    });
+}
+
+int dpmd_check_neighbors(dpmd_ctx* c, int force) { return guard(c, [&] { check_neighbors_impl(c, force != 0); }); }
+
+int dpmd_neighbor_info(dpmd_ctx* c, int* capacity, int* max_count, int64_t* total) {
+    return guard(c, [&] {
+        need_state(c->nbr_ready, "neighbors are not initialised");
+        if (capacity) *capacity = c->K;
+        if (max_count) {
+            HIP_OK(hipMemcpyAsync(max_count, c->max_count.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+            HIP_OK(hipStreamSynchronize(c->st));
+        }
+        if (total) {
+            auto nn = dl(c, c->nn);
+            int64_t t = 0;
+            for (int k = 0; k < c->n_obj(); ++k) t += nn[k];
+            *total = t;
+        }
+    });
+}
+
+int dpmd_get_neighbors_csr(dpmd_ctx* c, int* start, int* ids, int64_t ids_len) {
+    return guard(c, [&] {
+        need_state(c->nbr_ready, "neighbors are not initialised");
+        check_overflow(c);
+        const int n = c->n_obj();
+        auto nn = dl(c, c->nn);
+        auto nbr = dl(c, c->nbr);
+        int64_t w = 0;
+        for (int i = 0; i < n; ++i) {
+            start[i] = (int)w;
+            for (int k = 0; k < nn[i]; ++k) {
+                need(w < ids_len, "get_neighbors_csr: ids buffer too small");
+                ids[w++] = nbr[(size_t)k * n + i];
+            }
+        }
+        start[n] = (int)w;
+    });
+}
+
+int dpmd_compute_forces(dpmd_ctx* c, int mode) { return guard(c, [&] { compute_forces_impl(c, mode); }); }
+int dpmd_update_positions(dpmd_ctx* c, const double* sc, double s2) {
+    return guard(c, [&] { update_positions_impl(c, sc, s2); });
+}
+int dpmd_update_velocities(dpmd_ctx* c, const double* sc, double s2) {
+    return guard(c, [&] { update_velocities_impl(c, sc, s2); });
+}
+int dpmd_scale_velocities(dpmd_ctx* c, const double* sc) { return guard(c, [&] { scale_velocities_impl(c, sc); }); }
+int dpmd_scale_positions(dpmd_ctx* c, const double* sc) { return guard(c, [&] { scale_positions_impl(c, sc); }); }
+int dpmd_mix_velocities_and_forces(dpmd_ctx* c, const double* a) {
+    return guard(c, [&] { if (rigid(c)) rigid_mix(c, a); else disk_mix(c, a); });
+}
+int dpmd_zero_velocities(dpmd_ctx* c) {
+    return guard(c, [&] {
+        fill_f64(c->st, c->vx.p, c->N, 0.0);
+        fill_f64(c->st, c->vy.p, c->N, 0.0);
+        if (rigid(c)) fill_f64(c->st, c->avel.p, c->N, 0.0);
+    });
+}
+int dpmd_compute_ke(dpmd_ctx* c) { return guard(c, [&] { sys_reduce(c, RED_KE, c->ke_tot.p); }); }
+int dpmd_compute_ke_total(dpmd_ctx* c) { return guard(c, [&] { sys_reduce(c, RED_KE, c->ke_tot.p); }); }
+int dpmd_compute_pe_total(dpmd_ctx* c) { return guard(c, [&] { sys_reduce(c, RED_PE, c->pe_tot.p); }); }
+int dpmd_compute_fpower_total(dpmd_ctx* c) { return guard(c, [&] { sys_reduce(c, RED_POWER, c->pw_tot.p); }); }
+int dpmd_compute_temperature(dpmd_ctx* c) { return guard(c, [&] { per_system_temperature(c); }); }
+int dpmd_compute_packing_fraction(dpmd_ctx* c) { return guard(c, [&] { sys_reduce(c, RED_PHI, c->phi.p); }); }
+int dpmd_compute_max_force(dpmd_ctx* c) { return guard(c, [&] { sys_reduce(c, RED_MAXF2, c->maxf2.p); }); }
+int dpmd_compute_stress_tensor(dpmd_ctx* c) {
+    return guard(c, [&] {
+        need_state(c->nbr_ready, "neighbors are not initialised");
+        if (rigid(c)) rigid_stress(c); else disk_stress(c);
+    });
+}
+int dpmd_compute_pressure(dpmd_ctx* c) {
+    return guard(c, [&] {
+        need_state(c->nbr_ready, "neighbors are not initialised");
+        if (rigid(c)) rigid_stress(c); else disk_stress(c);
+        per_system_pressure(c);
+    });
+}
+int dpmd_set_average_velocity(dpmd_ctx* c, const double* ax, const double* ay) {
+    return guard(c, [&] { set_average_velocity_impl(c, ax, ay); });
+}
+int dpmd_set_temperature(dpmd_ctx* c, const double* T, int rm) {
+    return guard(c, [&] {
+        per_system_temperature(c);
+        temperature_scale(c, T, c->scr_c.p);
+        if (rm) set_average_velocity_impl(c, nullptr, nullptr);
+        scale_velocities_impl(c, c->scr_c.p);
+    });
+}
+int dpmd_increment_packing_fraction(dpmd_ctx* c, const double* inc) {
+    return guard(c, [&] {
+        sys_reduce(c, RED_PHI, c->phi.p);
+        scale_box_increment(c, inc, c->scr_c.p);
+        scale_positions_impl(c, c->scr_c.p);
+        box_inv(c);
+        if (c->method == DPMD_NEIGHBOR_CELL) {
+            setup_cells(c, true);
+            alloc_obj_buffers(c);
+            // the cell grid may have changed: every system is rebuilt below
+        }
+        check_neighbors_impl(c, true);
+        sys_reduce(c, RED_PHI, c->phi.p);
+    });
+}
+int dpmd_save_state(dpmd_ctx* c, const int* flag, int tv) {
+    return guard(c, [&] {
+        const int N = c->N, S = c->S;
+        if (!c->saved) {
+            dpmd::DVec<double>* dv[] = {&c->sv_x, &c->sv_y, &c->sv_rad, &c->sv_mass, &c->sv_angle};
+            for (auto* v : dv) { v->resize(N); v->zero(c->st); }
+            c->sv_static.resize(N); c->sv_static.zero(c->st);
+            c->sv_Lx.resize(S); c->sv_Ly.resize(S);
+            c->saved = true;
+        }
+        save_restore(c, flag, tv, false);
+    });
+}
+int dpmd_restore_state(dpmd_ctx* c, const int* flag, int tv) {
+    return guard(c, [&] {
+        need_state(c->saved, "restore_state: nothing has been saved");
+        need_state(!rigid(c), "restore_state: rigid bodies restore is not supported");
+        save_restore(c, flag, tv, true);
+        box_inv(c);
+        check_neighbors_impl(c, true);
+    });
+}
+
+int dpmd_rand_uniform_2d(dpmd_ctx* c, int fx, int fy, uint64_t seed, uint64_t call, double lox, double hix,
+                         double loy, double hiy) {
+    return guard(c, [&] {
+        FieldRef a = field_ref(c, fx), b = field_ref(c, fy);
+        need(a.elem == 8 && b.elem == 8 && a.n == b.n, "rand_uniform_2d: two equal-length double fields");
+        rand_uniform_2d(c->st, (double*)a.p, (double*)b.p, (int)a.n, seed, call, lox, hix, loy, hiy);
+    });
+}
+int dpmd_rand_uniform_1d(dpmd_ctx* c, int f, uint64_t seed, uint64_t call, double lo, double hi) {
+    return guard(c, [&] {
+        FieldRef a = field_ref(c, f);
+        need(a.elem == 8, "rand_uniform_1d: double field");
+        rand_uniform_1d(c->st, (double*)a.p, (int)a.n, seed, call, lo, hi);
+    });
+}
+int dpmd_rand_normal_2d(dpmd_ctx* c, int fx, int fy, uint64_t seed, uint64_t call, double mx, double sx, double my,
+                        double sy) {
+    return guard(c, [&] {
+        FieldRef a = field_ref(c, fx), b = field_ref(c, fy);
+        need(a.elem == 8 && b.elem == 8 && a.n == b.n, "rand_normal_2d: two equal-length double fields");
+        rand_normal_2d(c->st, (double*)a.p, (double*)b.p, (int)a.n, seed, call, mx, sx, my, sy);
+    });
+}
+
+int dpmd_vv_init(dpmd_ctx* c, const double* dt, int mode) {
+    return guard(c, [&] {
+        const int S = c->S;
+        std::vector<double> h(S), hh(S);
+        HIP_OK(hipMemcpyAsync(h.data(), dt, S * 8, hipMemcpyDeviceToHost, c->st));
+        HIP_OK(hipStreamSynchronize(c->st));
+        for (int s = 0; s < S; ++s) hh[s] = h[s] * 0.5;   // dt_half = 0.5 * dt (velocity_verlet.cuh:28-29)
+        c->vv_dt.upload(h.data(), S, c->st);
+        c->vv_dth.upload(hh.data(), S, c->st);
+        c->vv_mode = mode;
+        compute_forces_impl(c, mode);
+        c->vv_ready = true;
+    });
+}
+
+int dpmd_vv_step(dpmd_ctx* c, int n) {
+    return guard(c, [&] {
+        need_state(c->vv_ready, "vv_step: call vv_init first");
+        for (int k = 0; k < n; ++k) vv_step_body(c, active_one(c));
+    });
+}
+
+int dpmd_vv_run(dpmd_ctx* c, int n, int use_graph) {
+    return guard(c, [&] {
+        need_state(c->vv_ready, "vv_run: call vv_init first");
+        if (n <= 0) return;
+        int run[8] = {0, n, 1, 0, 0, 0, 0, 0};
+        c->run.upload(run, 8, c->st);
+        if (use_graph) {
+            const int G = std::min(n, 20);
+            if (!c->g_vv || c->g_vv_key != G) {
+                if (c->g_vv) HIP_OK(hipGraphExecDestroy(c->g_vv));
+                c->g_vv = capture(c, G, false, 0, 0, 0);
+                c->g_vv_key = G;
+            }
+            for (int k = 0; k < n; k += G) HIP_OK(hipGraphLaunch(c->g_vv, c->st));
+        } else {
+            for (int k = 0; k < n; ++k) { run_begin_step(c); vv_step_body(c, c->run.p + 3); }
+        }
+        check_overflow(c);
+    });
+}
+
+int dpmd_fire_init(dpmd_ctx* c, const double* dt, int mode) { return guard(c, [&] { fire_init_impl(c, dt, mode); }); }
+
+int dpmd_fire_step(dpmd_ctx* c) {
+    return guard(c, [&] {
+        need_state(c->fire_ready, "fire_step: call fire_init first");
+        // compute_fpower_total + fire_update_kernel (fire.cuh:62-72) from the live state
+        sys_reduce(c, RED_POWER, c->pw_tot.p);
+        fire_recompute_update(c);
+        fire_step_body(c, active_one(c), 0, 0, 0.0, 0.0);
+    });
+}
+
+int dpmd_fire_converged(dpmd_ctx* c, double diff_target, double pe_target, int* all) {
+    return guard(c, [&] {
+        need_state(c->fire_ready, "fire_converged: call fire_init first");
+        fire_apply_convergence(c, DPMD_FIRE_PRED_REFERENCE, diff_target, pe_target);
+        check_overflow(c);
+        const int running = read_running(c, nullptr);
+        if (all) *all = running == 0 ? 1 : 0;
+    });
+}
+
+int dpmd_fire_converged_max_force(dpmd_ctx* c, double tol, int* all) {
+    return guard(c, [&] {
+        need_state(c->fire_ready, "fire_converged: call fire_init first");
+        fire_apply_convergence(c, DPMD_FIRE_PRED_MAX_FORCE, tol, 0.0);
+        check_overflow(c);
+        const int running = read_running(c, nullptr);
+        if (all) *all = running == 0 ? 1 : 0;
+    });
+}
+
+int dpmd_fire_get_dt(dpmd_ctx* c, double* out) {
+    return guard(c, [&] {
+        HIP_OK(hipMemcpyAsync(out, c->f_dt.p, c->S * 8, hipMemcpyDeviceToDevice, c->st));
+    });
+}
+
+int dpmd_fire_state(dpmd_ctx* c, double* dt, double* alpha, int* ng, int* nb) {
+    return guard(c, [&] {
+        if (dt) { auto h = dl(c, c->f_dt); std::copy(h.begin(), h.end(), dt); }
+        if (alpha) { auto h = dl(c, c->f_alpha); std::copy(h.begin(), h.end(), alpha); }
+        if (ng) { auto h = dl(c, c->f_ngood); std::copy(h.begin(), h.end(), ng); }
+        if (nb) { auto h = dl(c, c->f_nbad); std::copy(h.begin(), h.end(), nb); }
+    });
+}
+
+int dpmd_fire_minimize(dpmd_ctx* c, const double* dt, int max_steps, int pred, double a, double b, int mode,
+                       int use_graph, int poll, int* steps_taken, int* all_conv) {
+    return guard(c, [&] {
+        need(pred == DPMD_FIRE_PRED_REFERENCE || pred == DPMD_FIRE_PRED_MAX_FORCE, "unknown FIRE predicate");
+        fire_init_impl(c, dt, mode);
+        int run[8] = {0, max_steps, 0, 0, 0, 0, 0, 0};
+        HIP_OK(hipMemcpyAsync(&run[2], c->run.p + 2, sizeof(int), hipMemcpyDeviceToHost, c->st));
+        HIP_OK(hipStreamSynchronize(c->st));
+        c->run.upload(run, 8, c->st);
+        if (poll <= 0) poll = 100;
+        int steps = 0, running = run[2];
+        if (use_graph) {
+            const int G = std::min(std::max(1, poll), 50);
+            const int per_poll = std::max(1, poll / G);
+            hipGraphExec_t ex = capture(c, G, true, pred, a, b);
+            int64_t launches = 0;
+            while (running > 0 && steps < max_steps) {
+                HIP_OK(hipGraphLaunch(ex, c->st));
+                ++launches;
+                if (launches % per_poll == 0 || launches * G >= max_steps) running = read_running(c, &steps);
+            }
+            HIP_OK(hipGraphExecDestroy(ex));
+        } else {
+            int k = 0;
+            while (running > 0 && steps < max_steps) {
+                run_begin_step(c);
+                fire_step_body(c, c->run.p + 3, 1, pred, a, b);
+                ++k;
+                if (k % poll == 0 || k >= max_steps) running = read_running(c, &steps);
+            }
+        }
+        running = read_running(c, &steps);
+        check_overflow(c);
+        if (steps_taken) *steps_taken = steps;
+        if (all_conv) *all_conv = running == 0 ? 1 : 0;
+    });
+}
+
+int dpmd_malloc(void** p, int64_t bytes) {
+    return guard(nullptr, [&] {
+        need(p != nullptr && bytes >= 0, "dpmd_malloc: bad arguments");
+        HIP_OK(hipMalloc(p, bytes > 0 ? bytes : 1));
+    });
+}
+int dpmd_free(void* p) { return guard(nullptr, [&] { if (p) HIP_OK(hipFree(p)); }); }
+int dpmd_memcpy(void* dst, const void* src, int64_t bytes, int kind) {
+    return guard(nullptr, [&] {
+        need(kind >= 0 && kind <= 2, "dpmd_memcpy: kind must be 0, 1 or 2");
+        const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+        if (bytes > 0) HIP_OK(hipMemcpy(dst, src, bytes, k));
+    });
+}
+int dpmd_fill_f64(void* p, int64_t n, double v) {
+    return guard(nullptr, [&] { fill_f64(nullptr, (double*)p, n, v); HIP_OK(hipDeviceSynchronize()); });
+}
+int dpmd_fill_i32(void* p, int64_t n, int32_t v) {
+    return guard(nullptr, [&] { fill_i32(nullptr, (int*)p, n, v); HIP_OK(hipDeviceSynchronize()); });
+}
+int dpmd_synchronize(dpmd_ctx* c) {
+    return guard(c, [&] {
+        HIP_OK(hipStreamSynchronize(c->st));
+        if (c->nbr_ready) check_overflow(c);
+    });
+}
+
+}  // extern "C"

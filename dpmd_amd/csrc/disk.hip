@@ -1,0 +1,501 @@
+// Disk kernels: harmonic contact forces (periodic and walled boxes), the velocity-Verlet
+// and FIRE particle primitives, and the fused per-step kernels.
+//
+// Reference: src/particles/disk.cu (kernels :20-335, :702-769), src/integrators/fire.cu,
+// include/integrators/{fire,velocity_verlet}.cuh.  The arithmetic of every primitive is the
+// reference's expression for expression (operand order kept, no contraction: the library
+// is built with -ffp-contract=off), so a fused kernel produces the same bits as the
+// reference's sequence of primitives.
+#include "internal.hpp"
+
+namespace dpmd {
+namespace {
+
+struct DiskArgs {
+    double* x; double* y; double* vx; double* vy; double* fx; double* fy; double* pe;
+    const double* rad; const double* mass;
+    const int* sid; const int* off;
+    const double* Lx; const double* Ly; const double* iLx; const double* iLy; const double* e;
+    const double* thresh2; const int* boundary;
+    double* lx; double* ly; unsigned* flag;
+    const int* nn; const int* nbr;
+    int n; int cell;
+};
+
+struct FireArgs {
+    double* dt; double* dtmax; double* dtmin; double* alpha; double* peprev;
+    int* ngood; int* nbad; int* stopped;
+    double* s_dt; double* s_alpha; double* s_dtrev; double* s_vscale; int* s_ngood; int* s_nbad;
+    int* run;
+};
+
+struct RedArgs {
+    const int* chunk_sys; const int* chunk_beg; const int* sys_chunk;
+    unsigned* counter; double* part; int nch;
+    double* pw_tot; double* pe_tot; double* maxf2;
+};
+
+DiskArgs disk_args(dpmd_ctx* c) {
+    DiskArgs a;
+    a.x = c->x.p; a.y = c->y.p; a.vx = c->vx.p; a.vy = c->vy.p; a.fx = c->fx.p; a.fy = c->fy.p; a.pe = c->pe.p;
+    a.rad = c->rad.p; a.mass = c->mass.p; a.sid = c->sid.p; a.off = c->off.p;
+    a.Lx = c->Lx.p; a.Ly = c->Ly.p; a.iLx = c->iLx.p; a.iLy = c->iLy.p; a.e = c->e.p;
+    a.thresh2 = c->thresh2.p; a.boundary = c->boundary.p;
+    a.lx = c->lx.p; a.ly = c->ly.p; a.flag = c->flag.p;
+    a.nn = c->nn.p; a.nbr = c->nbr.p; a.n = c->N; a.cell = c->method == DPMD_NEIGHBOR_CELL;
+    return a;
+}
+
+FireArgs fire_args(dpmd_ctx* c) {
+    FireArgs f;
+    f.dt = c->f_dt.p; f.dtmax = c->f_dtmax.p; f.dtmin = c->f_dtmin.p; f.alpha = c->f_alpha.p;
+    f.peprev = c->f_peprev.p; f.ngood = c->f_ngood.p; f.nbad = c->f_nbad.p; f.stopped = c->f_stopped.p;
+    f.s_dt = c->s_dt.p; f.s_alpha = c->s_alpha.p; f.s_dtrev = c->s_dtrev.p; f.s_vscale = c->s_vscale.p;
+    f.s_ngood = c->s_ngood.p; f.s_nbad = c->s_nbad.p; f.run = c->run.p;
+    return f;
+}
+
+RedArgs red_args(dpmd_ctx* c) {
+    RedArgs r;
+    r.chunk_sys = c->chunk_sys.p; r.chunk_beg = c->chunk_beg.p; r.sys_chunk = c->sys_chunk.p;
+    r.counter = c->sys_counter.p; r.part = c->part.p; r.nch = c->nchunks;
+    r.pw_tot = c->pw_tot.p; r.pe_tot = c->pe_tot.p; r.maxf2 = c->maxf2.p;
+    return r;
+}
+
+// compute_pair_forces_kernel (disk.cu:103-160) and compute_wall_forces_kernel
+// (disk.cu:163-246).  In walled boxes the pair displacement is xi - xj (repulsive,
+// consistent with the 0.5 k delta^2 energy); the reference writes xj - xi there (:218).
+template <bool WALL>
+__device__ __forceinline__ void disk_force_at(const DiskArgs& a, int i, int s, double& fxo, double& fyo, double& peo) {
+    const double ei = a.e[s];
+    const double xi = a.x[i], yi = a.y[i], ri = a.rad[i];
+    double fxi = 0.0, fyi = 0.0, pei = 0.0;
+    double Lx = a.Lx[s], Ly = a.Ly[s];
+    if (WALL) {
+        if (xi < ri) { const double d = ri - xi; fxi += ei * d; pei += (0.5 * ei * d * d); }
+        if (xi > Lx - ri) { const double d = ri - (Lx - xi); fxi -= ei * d; pei += (0.5 * ei * d * d); }
+        if (yi < ri) { const double d = ri - yi; fyi += ei * d; pei += (0.5 * ei * d * d); }
+        if (yi > Ly - ri) { const double d = ri - (Ly - yi); fyi -= ei * d; pei += (0.5 * ei * d * d); }
+    }
+    const double iLx = a.iLx[s], iLy = a.iLy[s];
+    const int cnt = a.nn[i];
+    const int* __restrict__ col = a.nbr + i;
+    for (int k = 0; k < cnt; ++k) {
+        const int j = col[(size_t)k * a.n];
+        double dx = xi - a.x[j], dy = yi - a.y[j];
+        if (!WALL) { dx = min_image(dx, Lx, iLx); dy = min_image(dy, Ly, iLy); }
+        const double r2 = dx * dx + dy * dy;
+        const double radsum = ri + a.rad[j];
+        if (r2 >= radsum * radsum) continue;
+        const double r = sqrt(r2);
+        const double inv = 1.0 / r;
+        const double nx = dx * inv, ny = dy * inv;
+        const double delta = radsum - r;
+        const double fmag = ei * delta;
+        fxi += fmag * nx;
+        fyi += fmag * ny;
+        pei += (0.5 * ei * delta * delta) * 0.5;
+    }
+    fxo = fxi; fyo = fyi; peo = pei;
+}
+
+template <int MODE>
+__device__ __forceinline__ void disk_force_mode(const DiskArgs& a, int i, int s, double& fx, double& fy, double& pe) {
+    if (MODE == DPMD_FORCE_PBC) disk_force_at<false>(a, i, s, fx, fy, pe);
+    else if (MODE == DPMD_FORCE_WALL) disk_force_at<true>(a, i, s, fx, fy, pe);
+    else {
+        if (a.boundary[s]) disk_force_at<true>(a, i, s, fx, fy, pe);
+        else disk_force_at<false>(a, i, s, fx, fy, pe);
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(DPMD_BLOCK) k_force(DiskArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double fx, fy, pe;
+    disk_force_mode<MODE>(a, i, a.sid[i], fx, fy, pe);
+    a.fx[i] = fx; a.fy[i] = fy; a.pe[i] = pe;
+}
+
+__device__ __forceinline__ void skin_check(const DiskArgs& a, int i, int s, double x, double y) {
+    const double dx = x - a.lx[i], dy = y - a.ly[i];
+    const double d2 = dx * dx + dy * dy;
+    if (d2 > a.thresh2[s]) atomicOr(&a.flag[s], 1u);
+}
+
+// update_positions_kernel_cell / _naive (disk.cu:20-75)
+__global__ void k_update_positions(DiskArgs a, const double* __restrict__ scale, double scale2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int s = a.sid[i];
+    const double sc = scale[s] * scale2;
+    double x = a.x[i], y = a.y[i];
+    x += a.vx[i] * sc;
+    y += a.vy[i] * sc;
+    a.x[i] = x; a.y[i] = y;
+    if (a.cell) skin_check(a, i, s, x, y);
+}
+
+// update_velocities_kernel (disk.cu:78-100)
+__global__ void k_update_velocities(DiskArgs a, const double* __restrict__ scale, double scale2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double smi = scale[a.sid[i]] * scale2 / a.mass[i];
+    a.vx[i] = a.vx[i] + a.fx[i] * smi;
+    a.vy[i] = a.vy[i] + a.fy[i] * smi;
+}
+
+__global__ void k_scale_velocities(DiskArgs a, const double* __restrict__ scale) {   // disk.cu:278-289
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double sc = scale[a.sid[i]];
+    a.vx[i] *= sc; a.vy[i] *= sc;
+}
+
+__global__ void k_scale_positions(DiskArgs a, const double* __restrict__ scale) {    // disk.cu:292-303
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double sc = scale[a.sid[i]];
+    a.x[i] *= sc; a.y[i] *= sc;
+}
+
+// mix_velocities_and_forces_kernel (disk.cu:307-335): per-particle |v_i|/|F_i| mixing.
+__device__ __forceinline__ void disk_mix_one(double& vx, double& vy, double fx, double fy, double a) {
+    const double fn = sqrt(fx * fx + fy * fy);
+    const double vn = sqrt(vx * vx + vy * vy);
+    double ratio = 0.0;
+    if (fn > 1e-16) ratio = vn / fn * a;
+    else { vx = 0.0; vy = 0.0; ratio = 0.0; }
+    vx = vx * (1 - a) + fx * ratio;
+    vy = vy * (1 - a) + fy * ratio;
+}
+
+__global__ void k_mix(DiskArgs a, const double* __restrict__ alpha) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double vx = a.vx[i], vy = a.vy[i];
+    disk_mix_one(vx, vy, a.fx[i], a.fy[i], alpha[a.sid[i]]);
+    a.vx[i] = vx; a.vy[i] = vy;
+}
+
+// compute_stress_tensor_kernel (disk.cu:702-769)
+__global__ void k_stress(DiskArgs a, double* sxx, double* sxy, double* syx, double* syy) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int s = a.sid[i];
+    const double ei = a.e[s], Lx = a.Lx[s], Ly = a.Ly[s], iLx = a.iLx[s], iLy = a.iLy[s];
+    const double xi = a.x[i], yi = a.y[i], vxi = a.vx[i], vyi = a.vy[i], m = a.mass[i], ri = a.rad[i];
+    double axx = m * vxi * vxi, axy = m * vxi * vyi;
+    double ayx = axy, ayy = m * vyi * vyi;
+    const double area_box = Lx * Ly;
+    const int cnt = a.nn[i];
+    for (int k = 0; k < cnt; ++k) {
+        const int j = a.nbr[(size_t)k * a.n + i];
+        const double dx = min_image(xi - a.x[j], Lx, iLx);
+        const double dy = min_image(yi - a.y[j], Ly, iLy);
+        const double r2 = dx * dx + dy * dy;
+        const double radsum = ri + a.rad[j];
+        if (r2 >= radsum * radsum) continue;
+        const double r = sqrt(r2);
+        const double inv = 1.0 / r;
+        const double nx = dx * inv, ny = dy * inv;
+        const double delta = radsum - r;
+        const double fmag = ei * delta;
+        const double fx = fmag * nx, fy = fmag * ny;
+        axx += dx * fx / 2.0; axy += dx * fy / 2.0;
+        ayx += dy * fx / 2.0; ayy += dy * fy / 2.0;
+    }
+    sxx[i] = axx / area_box; sxy[i] = axy / area_box;
+    syx[i] = ayx / area_box; syy[i] = ayy / area_box;
+}
+
+// ---------------------------------------------------------------------------------------
+// FIRE, fused.  One step of BaseFIRE::step_impl (fire.cuh:59-88) is:
+//   k_fire_drift   fire_update result -> reverse drift, vscale, half kick, mixing, vscale,
+//                  half drift, skin check (all per particle, in the reference's order);
+//   rebuild        (device-side conditional, cells.hip);
+//   k_fire_force   forces, half kick, per-system F.v / pe / max|F|^2 partials; the last
+//                  workgroup of each system folds them (fixed tree), applies the stopping
+//                  test (fire.cu:59-78) and evaluates fire_update (fire.cu:7-57) for the
+//                  next step into the s_* ("next-step") scalars.
+// k_fire_drift commits s_* into the live scalars, so the live dt/alpha/N_good/N_bad are
+// always what the reference holds after the same number of steps.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void fire_update_sys(const FireArgs& f, int s, double power) {
+    double dt = f.dt[s], alpha = f.alpha[s];
+    int ng = f.ngood[s], nb = f.nbad[s];
+    double dtrev = 0.0, vscale = 1.0;
+    if (dt == 0.0) {
+        vscale = 0.0; alpha = 0.0;
+    } else if (power > 0) {
+        ng++; nb = 0;
+        if (ng > 5) { dt = fmin(dt * 1.1, f.dtmax[s]); alpha *= 0.99; }
+    } else {
+        ng = 0; nb++;
+        if (nb > 10) {
+            dt = 0.0;
+        } else {
+            dt = fmax(dt * 0.5, f.dtmin[s]);
+            alpha = 0.1;
+            dtrev = -dt * 0.5;
+            vscale = 0.0;
+        }
+    }
+    f.s_dt[s] = dt; f.s_alpha[s] = alpha; f.s_ngood[s] = ng; f.s_nbad[s] = nb;
+    f.s_dtrev[s] = dtrev; f.s_vscale[s] = vscale;
+}
+
+__device__ __forceinline__ void fire_convergence_sys(const FireArgs& f, const int* off, int s, double pe_total,
+                                                     double maxf2, int pred, double a, double b) {
+    const int dof = off[s + 1] - off[s];
+    const double avg = pe_total / dof;
+    const double avg_prev = f.peprev[s] / dof;
+    const double diff = avg_prev == 0 ? 0 : fabs(avg / avg_prev - 1);
+    if (pred == DPMD_FIRE_PRED_REFERENCE) {
+        if (diff < a || avg < b) f.dt[s] = 0.0;
+    } else {
+        if (sqrt(maxf2) < a) f.dt[s] = 0.0;
+    }
+    f.peprev[s] = avg;
+}
+
+__device__ __forceinline__ void fire_count_stopped(const FireArgs& f, int s) {
+    if (f.dt[s] == 0.0 && !f.stopped[s]) {
+        f.stopped[s] = 1;
+        atomicSub(&f.run[2], 1);
+    }
+}
+
+__global__ void __launch_bounds__(DPMD_BLOCK) k_fire_drift(DiskArgs a, FireArgs f, const int* active) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n || !*active) return;
+    const int s = a.sid[i];
+    const double dt = f.s_dt[s], alpha = f.s_alpha[s], dtrev = f.s_dtrev[s], vscale = f.s_vscale[s];
+    const double m = a.mass[i];
+    const double fx = a.fx[i], fy = a.fy[i];
+    double x = a.x[i], y = a.y[i], vx = a.vx[i], vy = a.vy[i];
+    // update_positions(dt_reverse, 1.0)
+    const double s1 = dtrev * 1.0;
+    x += vx * s1; y += vy * s1;
+    bool flag = false;
+    if (a.cell) {
+        const double dx = x - a.lx[i], dy = y - a.ly[i];
+        flag = (dx * dx + dy * dy) > a.thresh2[s];
+    }
+    // scale_velocities(velocity_scale)
+    vx *= vscale; vy *= vscale;
+    // update_velocities(dt, 0.5)
+    const double smi = dt * 0.5 / m;
+    vx = vx + fx * smi; vy = vy + fy * smi;
+    // mix_velocities_and_forces(alpha)
+    disk_mix_one(vx, vy, fx, fy, alpha);
+    // scale_velocities(velocity_scale)
+    vx *= vscale; vy *= vscale;
+    // update_positions(dt, 0.5)
+    const double s2 = dt * 0.5;
+    x += vx * s2; y += vy * s2;
+    if (a.cell) {
+        const double dx = x - a.lx[i], dy = y - a.ly[i];
+        flag = flag || ((dx * dx + dy * dy) > a.thresh2[s]);
+        if (flag) atomicOr(&a.flag[s], 1u);
+    }
+    a.x[i] = x; a.y[i] = y; a.vx[i] = vx; a.vy[i] = vy;
+    if (i == a.off[s]) {
+        f.dt[s] = dt; f.alpha[s] = alpha; f.ngood[s] = f.s_ngood[s]; f.nbad[s] = f.s_nbad[s];
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, int do_conv, int do_update, int pred, double pa,
+             double pb, const int* active) {
+    __shared__ double lds[4];
+    __shared__ int last;
+    if (!*active) return;
+    const int c = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = r.chunk_sys[c];
+    const int beg = r.chunk_beg[c];
+    const int end = min(beg + DPMD_BLOCK, a.off[s + 1]);
+    const int i = beg + threadIdx.x;
+    double vp = 0.0, vpe = 0.0, vf2 = 0.0;
+    if (i < end) {
+        double fx, fy, pe;
+        disk_force_mode<MODE>(a, i, s, fx, fy, pe);
+        a.fx[i] = fx; a.fy[i] = fy; a.pe[i] = pe;
+        double vx = a.vx[i], vy = a.vy[i];
+        if (kick) {
+            const double smi = f.dt[s] * 0.5 / a.mass[i];
+            vx = vx + fx * smi; vy = vy + fy * smi;
+            a.vx[i] = vx; a.vy[i] = vy;
+        }
+        vp = fx * vx + fy * vy;
+        vpe = pe;
+        vf2 = fx * fx + fy * fy;
+    }
+    vp = block_sum256(vp, lds);
+    vpe = block_sum256(vpe, lds);
+    vf2 = block_max256(vf2, lds);
+    if (threadIdx.x == 0) {
+        r.part[c] = vp; r.part[r.nch + c] = vpe; r.part[2 * r.nch + c] = vf2;
+        __threadfence();
+        const int cb = r.sys_chunk[s], ce = r.sys_chunk[s + 1];
+        const unsigned prev = atomicAdd(&r.counter[s], 1u);
+        last = (prev == (unsigned)(ce - cb - 1));
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const int cb = r.sys_chunk[s], ce = r.sys_chunk[s + 1];
+    double ap = 0.0, ape = 0.0, af = 0.0;
+    for (int k = cb + threadIdx.x; k < ce; k += DPMD_BLOCK) {
+        ap = ap + load_agent(&r.part[k]);
+        ape = ape + load_agent(&r.part[r.nch + k]);
+        af = fmax(af, load_agent(&r.part[2 * r.nch + k]));
+    }
+    ap = block_sum256(ap, lds);
+    ape = block_sum256(ape, lds);
+    af = block_max256(af, lds);
+    if (threadIdx.x == 0) {
+        r.counter[s] = 0u;
+        r.pw_tot[s] = ap; r.pe_tot[s] = ape; r.maxf2[s] = af;
+        if (do_conv) fire_convergence_sys(f, a.off, s, ape, af, pred, pa, pb);
+        if (do_conv) fire_count_stopped(f, s);
+        if (do_update) fire_update_sys(f, s, ap);
+    }
+}
+
+// Stopping test + next-step update for the step()/converged() call pattern (fire.cu:59-78).
+__global__ void k_fire_converge(int S, const int* off, FireArgs f, const double* pe_tot, const double* maxf2,
+                                const double* pw_tot, int pred, double pa, double pb) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    fire_convergence_sys(f, off, s, pe_tot[s], maxf2[s], pred, pa, pb);
+    fire_count_stopped(f, s);
+    fire_update_sys(f, s, pw_tot[s]);
+}
+
+__global__ void k_fire_update(int S, FireArgs f, const double* pw_tot) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    fire_update_sys(f, s, pw_tot[s]);
+}
+
+// ---------------------------------------------------------------------------------------
+// Velocity Verlet, fused (BaseVelocityVerlet::step_impl, velocity_verlet.cuh:34-41):
+//   k_vv_drift: update_velocities(dt_half) + update_positions(dt) + skin check;
+//   rebuild;  k_vv_force: compute_forces + update_velocities(dt_half).
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_vv_drift(DiskArgs a, const double* __restrict__ dt, const double* __restrict__ dth, const int* active) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n || !*active) return;
+    const int s = a.sid[i];
+    const double smi = dth[s] * 1.0 / a.mass[i];
+    double vx = a.vx[i] + a.fx[i] * smi;
+    double vy = a.vy[i] + a.fy[i] * smi;
+    const double sc = dt[s] * 1.0;
+    double x = a.x[i], y = a.y[i];
+    x += vx * sc; y += vy * sc;
+    a.x[i] = x; a.y[i] = y; a.vx[i] = vx; a.vy[i] = vy;
+    if (a.cell) skin_check(a, i, s, x, y);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_vv_force(DiskArgs a, const double* __restrict__ dth, const int* active) {
+    if (!*active) return;
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * DPMD_BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const int s = a.sid[i];
+    double fx, fy, pe;
+    disk_force_mode<MODE>(a, i, s, fx, fy, pe);
+    a.fx[i] = fx; a.fy[i] = fy; a.pe[i] = pe;
+    const double smi = dth[s] * 1.0 / a.mass[i];
+    a.vx[i] = a.vx[i] + fx * smi;
+    a.vy[i] = a.vy[i] + fy * smi;
+}
+
+template <class F>
+void by_mode(int mode, F&& f) {
+    switch (mode) {
+        case DPMD_FORCE_PBC: f(std::integral_constant<int, DPMD_FORCE_PBC>{}); break;
+        case DPMD_FORCE_WALL: f(std::integral_constant<int, DPMD_FORCE_WALL>{}); break;
+        default: f(std::integral_constant<int, DPMD_FORCE_PER_SYSTEM>{}); break;
+    }
+}
+
+}  // namespace
+
+void disk_compute_forces(dpmd_ctx* c, int mode) {
+    DiskArgs a = disk_args(c);
+    by_mode(mode, [&](auto m) { k_force<decltype(m)::value><<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(a); });
+    HIP_OK(hipGetLastError());
+}
+void disk_update_positions(dpmd_ctx* c, const double* scale, double scale2) {
+    k_update_positions<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), scale, scale2);
+    HIP_OK(hipGetLastError());
+}
+void disk_update_velocities(dpmd_ctx* c, const double* scale, double scale2) {
+    k_update_velocities<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), scale, scale2);
+    HIP_OK(hipGetLastError());
+}
+void disk_scale_velocities(dpmd_ctx* c, const double* scale) {
+    k_scale_velocities<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), scale);
+    HIP_OK(hipGetLastError());
+}
+void disk_scale_positions(dpmd_ctx* c, const double* scale) {
+    k_scale_positions<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), scale);
+    HIP_OK(hipGetLastError());
+}
+void disk_mix(dpmd_ctx* c, const double* alpha) {
+    k_mix<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), alpha);
+    HIP_OK(hipGetLastError());
+}
+void disk_stress(dpmd_ctx* c) {
+    k_stress<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), c->sxx.p, c->sxy.p, c->syx.p, c->syy.p);
+    HIP_OK(hipGetLastError());
+}
+
+void disk_fire_drift(dpmd_ctx* c, const int* active) {
+    k_fire_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), fire_args(c), active);
+    HIP_OK(hipGetLastError());
+}
+
+void disk_fire_force(dpmd_ctx* c, const int* active, int kick, int do_conv, int pred, double pa, double pb) {
+    DiskArgs a = disk_args(c);
+    FireArgs f = fire_args(c);
+    RedArgs r = red_args(c);
+    by_mode(c->fire_mode, [&](auto m) {
+        k_fire_force<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(a, f, r, kick, do_conv, 1, pred,
+                                                                               pa, pb, active);
+    });
+    HIP_OK(hipGetLastError());
+}
+
+void disk_vv_drift(dpmd_ctx* c, const int* active) {
+    k_vv_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), c->vv_dt.p, c->vv_dth.p, active);
+    HIP_OK(hipGetLastError());
+}
+
+void disk_vv_force(dpmd_ctx* c, const int* active) {
+    DiskArgs a = disk_args(c);
+    by_mode(c->vv_mode, [&](auto m) {
+        k_vv_force<decltype(m)::value><<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(a, c->vv_dth.p, active);
+    });
+    HIP_OK(hipGetLastError());
+}
+
+void fire_apply_convergence(dpmd_ctx* c, int pred, double a, double b) {
+    k_fire_converge<<<blocks_for(c->S), DPMD_BLOCK, 0, c->st>>>(c->S, c->off.p, fire_args(c), c->pe_tot.p,
+                                                               c->maxf2.p, c->pw_tot.p, pred, a, b);
+    HIP_OK(hipGetLastError());
+}
+
+void fire_recompute_update(dpmd_ctx* c) {
+    k_fire_update<<<blocks_for(c->S), DPMD_BLOCK, 0, c->st>>>(c->S, fire_args(c), c->pw_tot.p);
+    HIP_OK(hipGetLastError());
+}
+
+}  // namespace dpmd

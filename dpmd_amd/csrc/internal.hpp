@@ -1,0 +1,153 @@
+// Context layout and launcher declarations shared by the library's translation units.
+#pragma once
+#include "common.hpp"
+#include "../../include/dpmd/dpmd.h"
+
+namespace dpmd {
+
+// Object view for the cell / neighbor pipeline.  Objects are disks, or the vertices of
+// rigid bodies (then pid gives the owning particle, whose other vertices are excluded;
+// reference: base_poly_particle_kernels.cu:78-195).
+struct ObjArgs {
+    const double* x; const double* y; const double* rad;
+    const int* sid;      // object -> system
+    const int* off;      // per-system object offsets (S+1)
+    const int* pid;      // object -> particle (nullptr for disks)
+    int n;
+};
+
+struct CellArgs {
+    const int* cdx; const int* cdy; const int* cstart;        // S, S, S+1
+    const double* Lx; const double* Ly; const double* iLx; const double* iLy;
+    const double* cutoff;
+    int* cell_id; int* cell_count; int* cell_start; int* cell_wptr; const int* cell_sys;
+    int C;
+    unsigned* flag;
+};
+
+// Arrays permuted by a rebuild (gathered by the new order, then copied back).
+struct PermArgs {
+    double* d[12]; int nd;
+    int* i[4]; int ni;
+    double* tmp_d; int* tmp_i;
+};
+
+struct NbrArgs {
+    int* nn; int* nbr; int K;      // ELL, column-major: nbr[k * n + i]
+    int* overflow; int* max_count;
+};
+
+}  // namespace dpmd
+
+struct dpmd_ctx {
+    int kind = DPMD_DISK;
+    hipStream_t st = nullptr;
+    std::string err;
+    int S = 0, N = 0, Nv = 0;
+    int method = DPMD_NEIGHBOR_CELL;
+    bool synced = false, nbr_ready = false;
+    int K = 32;                 // neighbor capacity per object
+    int C = 0;                  // total cells
+
+    // ---- per system
+    dpmd::DVec<int> off, size, cdx, cdy, cstart, boundary, ndof, voff;
+    dpmd::DVec<unsigned> flag;
+    dpmd::DVec<double> Lx, Ly, iLx, iLy, e, cutoff, thresh2;
+    dpmd::DVec<double> pe_tot, ke_tot, pw_tot, press, temp, phi, maxf2;
+    dpmd::DVec<double> st_xx, st_xy, st_yx, st_yy;
+    std::vector<int> h_off, h_voff;
+
+    // ---- per particle
+    dpmd::DVec<double> x, y, vx, vy, fx, fy, pe, ke, rad, mass, area, lx, ly;
+    dpmd::DVec<double> angle, avel, torque, inertia;
+    dpmd::DVec<double> sxx, sxy, syx, syy;
+    dpmd::DVec<int> sid, static_index;
+
+    // ---- per vertex (rigid bumpy)
+    dpmd::DVec<double> vpx, vpy, vfx, vfy, vpe, vrad, vmass;
+    dpmd::DVec<int> vpid, poff, nvpp, vsid;
+
+    // ---- cells and neighbors (over objects: particles for disks, vertices for rigid)
+    dpmd::DVec<int> cell_id, cell_count, cell_start, cell_wptr, cell_sys, order, order_inv;
+    dpmd::DVec<int> tmp_i; dpmd::DVec<double> tmp_d;
+    dpmd::DVec<int> nn, nbr, overflow, max_count;
+    dpmd::DVec<int> scan_part;
+
+    // ---- system-aligned 256-wide chunks over particles (deterministic reductions)
+    int nchunks = 0;
+    dpmd::DVec<int> chunk_sys, chunk_beg, sys_chunk;   // sys_chunk: S+1
+    dpmd::DVec<unsigned> sys_counter;
+    dpmd::DVec<double> part;                            // 4 * nchunks
+
+    // ---- FIRE (fire.cuh:116-147): committed and next-step ("shadow") scalars
+    dpmd::DVec<double> f_dt, f_dtmax, f_dtmin, f_alpha, f_peprev;
+    dpmd::DVec<int> f_ngood, f_nbad, f_stopped;
+    dpmd::DVec<double> s_dt, s_alpha, s_dtrev, s_vscale;
+    dpmd::DVec<int> s_ngood, s_nbad;
+    int fire_mode = 0; bool fire_ready = false;
+    // ---- VV
+    dpmd::DVec<double> vv_dt, vv_dth;
+    int vv_mode = 0; bool vv_ready = false;
+    // ---- run control: [0] steps_done [1] max_steps [2] n_running [3] active
+    dpmd::DVec<int> run, one;
+    // ---- saved state (save_state/restore_state)
+    dpmd::DVec<double> sv_x, sv_y, sv_rad, sv_mass, sv_Lx, sv_Ly, sv_angle, sv_vx, sv_vy;
+    dpmd::DVec<int> sv_static;
+    bool saved = false;
+    // scratch per-system
+    dpmd::DVec<double> scr_a, scr_b, scr_c;
+    // graphs
+    hipGraphExec_t g_fire = nullptr, g_vv = nullptr;
+    int g_fire_key[4] = {0, 0, 0, 0};
+    double g_fire_keyd[2] = {0, 0};
+    int g_vv_key = -1;
+
+    int n_obj() const { return kind == DPMD_RIGID_BUMPY ? Nv : N; }
+};
+
+namespace dpmd {
+
+// cells.hip
+void build_chunks(dpmd_ctx* c);
+void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all);
+void clear_flags(dpmd_ctx* c, const int* active);
+void naive_neighbors(dpmd_ctx* c);
+void check_overflow(dpmd_ctx* c);
+
+// disk.hip
+void disk_compute_forces(dpmd_ctx* c, int mode);
+void disk_update_positions(dpmd_ctx* c, const double* scale, double scale2);
+void disk_update_velocities(dpmd_ctx* c, const double* scale, double scale2);
+void disk_scale_velocities(dpmd_ctx* c, const double* scale);
+void disk_scale_positions(dpmd_ctx* c, const double* scale);
+void disk_mix(dpmd_ctx* c, const double* alpha);
+void disk_stress(dpmd_ctx* c);
+void disk_fire_force(dpmd_ctx* c, const int* active, int kick, int do_conv, int pred, double a, double b);
+void disk_fire_drift(dpmd_ctx* c, const int* active);
+void disk_vv_drift(dpmd_ctx* c, const int* active);
+void disk_vv_force(dpmd_ctx* c, const int* active);
+
+// reduce.hip
+enum RedKind { RED_PE = 0, RED_KE, RED_POWER, RED_VX, RED_VY, RED_PHI, RED_SXX, RED_SXY, RED_SYX, RED_SYY, RED_MAXF2 };
+void sys_reduce(dpmd_ctx* c, int kind, double* out);
+void fire_apply_convergence(dpmd_ctx* c, int pred, double a, double b);
+void fire_setup(dpmd_ctx* c, const double* dt);
+void run_begin_step(dpmd_ctx* c);
+void per_system_temperature(dpmd_ctx* c);
+void per_system_pressure(dpmd_ctx* c);
+void temperature_scale(dpmd_ctx* c, const double* target, double* scale_out);
+void subtract_mean_velocity(dpmd_ctx* c, const double* cur_x, const double* cur_y, const double* want_x, const double* want_y);
+void box_inv(dpmd_ctx* c);
+void scale_box_increment(dpmd_ctx* c, const double* inc, double* scale_out);
+void save_restore(dpmd_ctx* c, const int* flag, int tv, bool restore);
+
+// rng.hip
+void rand_uniform_2d(hipStream_t st, double* ox, double* oy, int n, uint64_t seed, uint64_t call,
+                     double lox, double hix, double loy, double hiy);
+void rand_uniform_1d(hipStream_t st, double* o, int n, uint64_t seed, uint64_t call, double lo, double hi);
+void rand_normal_2d(hipStream_t st, double* ox, double* oy, int n, uint64_t seed, uint64_t call,
+                    double mx, double sx, double my, double sy);
+void fill_f64(hipStream_t st, double* p, int64_t n, double v);
+void fill_i32(hipStream_t st, int* p, int64_t n, int v);
+
+}  // namespace dpmd

@@ -1,0 +1,294 @@
+"""Python host mirror of the reference particle / integrator / routine API over the C ABI.
+
+Names follow the reference: ``Disk`` (md::disk::Disk, include/particles/disk.cuh:22),
+``RigidBumpy`` (md::rigid_bumpy::RigidBumpy, include/particles/rigid_bumpy.cuh:24),
+``VelocityVerlet`` / ``FIRE`` (include/integrators/{velocity_verlet,fire}.cuh) and
+``minimize_fire`` (include/routines/minimizers.cuh:10-31).  Every call goes through
+libdpmd_amd.so; nothing here computes physics on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import F, check, lib
+
+
+class DeviceArray:
+    """A small device buffer (the role of df::DeviceField1D for per-system parameters)."""
+
+    def __init__(self, values, dtype=np.float64):
+        a = np.ascontiguousarray(np.asarray(values, dtype=dtype))
+        self.dtype = a.dtype
+        self.n = a.size
+        p = C.c_void_p()
+        check(lib().dpmd_malloc(C.byref(p), max(1, a.nbytes)))
+        self.ptr = p.value
+        if a.nbytes:
+            check(lib().dpmd_memcpy(self.ptr, a.ctypes.data, a.nbytes, 0))
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        if out.nbytes:
+            check(lib().dpmd_memcpy(out.ctypes.data, self.ptr, out.nbytes, 1))
+        return out
+
+    def fill(self, v) -> None:
+        a = np.full(self.n, v, self.dtype)
+        check(lib().dpmd_memcpy(self.ptr, a.ctypes.data, a.nbytes, 0))
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None):
+                lib().dpmd_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+def _per_system(x, S: int, dtype=np.float64) -> DeviceArray:
+    if isinstance(x, DeviceArray):
+        return x
+    a = np.asarray(x, dtype=dtype)
+    if a.ndim == 0:
+        a = np.full(S, a, dtype)
+    if a.size != S:
+        raise ValueError(f"expected {S} per-system values, got {a.size}")
+    return DeviceArray(a, dtype)
+
+
+class _Particles:
+    kind = L.DISK
+
+    def __init__(self, stream: Optional[int] = None):
+        p = C.c_void_p()
+        check(lib().dpmd_ctx_create(C.byref(p), self.kind, stream))
+        self.ctx = p.value
+        self._keep = []
+
+    def __del__(self):
+        try:
+            if getattr(self, "ctx", None):
+                lib().dpmd_ctx_destroy(self.ctx)
+                self.ctx = None
+        except Exception:
+            pass
+
+    def _c(self, status: int) -> None:
+        check(status, self.ctx)
+
+    # ---- allocation / fields ------------------------------------------------------------
+    def allocate_systems(self, S: int):
+        self._c(lib().dpmd_allocate_systems(self.ctx, S))
+
+    def allocate_particles(self, N: int):
+        self._c(lib().dpmd_allocate_particles(self.ctx, N))
+
+    def allocate_vertices(self, Nv: int):
+        self._c(lib().dpmd_allocate_vertices(self.ctx, Nv))
+
+    def set_neighbor_method(self, m: int):
+        self._c(lib().dpmd_set_neighbor_method(self.ctx, m))
+
+    def field_len(self, name: str) -> int:
+        n = C.c_int64()
+        self._c(lib().dpmd_field(self.ctx, F[name], None, C.byref(n)))
+        return n.value
+
+    def field_ptr(self, name: str) -> int:
+        p = C.c_void_p()
+        self._c(lib().dpmd_field(self.ctx, F[name], C.byref(p), None))
+        return p.value
+
+    def set(self, name: str, values) -> None:
+        dt = np.int32 if name in L.INT_FIELDS else np.float64
+        a = np.ascontiguousarray(np.asarray(values, dtype=dt))
+        self._c(lib().dpmd_set_field(self.ctx, F[name], a.ctypes.data, a.size))
+
+    def get(self, name: str, order: int = L.ORDER_CURRENT) -> np.ndarray:
+        dt = np.int32 if name in L.INT_FIELDS else np.float64
+        out = np.empty(self.field_len(name), dt)
+        self._c(lib().dpmd_get_field(self.ctx, F[name], out.ctypes.data, out.size, order))
+        return out
+
+    def sizes(self):
+        S, N, Nv, Cc = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self._c(lib().dpmd_sizes(self.ctx, C.byref(S), C.byref(N), C.byref(Nv), C.byref(Cc)))
+        return S.value, N.value, Nv.value, Cc.value
+
+    def n_systems(self): return self.sizes()[0]
+    def n_particles(self): return self.sizes()[1]
+    def n_vertices(self): return self.sizes()[2]
+    def n_cells(self): return self.sizes()[3]
+
+    def sync(self):
+        self._c(lib().dpmd_sync(self.ctx))
+
+    # ---- neighbors ------------------------------------------------------------------------
+    def init_neighbors(self): self._c(lib().dpmd_init_neighbors(self.ctx))
+    def update_neighbors(self): self._c(lib().dpmd_update_neighbors(self.ctx))
+    def check_neighbors(self, force_update: bool = False):
+        self._c(lib().dpmd_check_neighbors(self.ctx, int(force_update)))
+
+    def set_neighbor_capacity(self, k: int): self._c(lib().dpmd_set_neighbor_capacity(self.ctx, k))
+
+    def neighbor_info(self):
+        cap, mx, tot = C.c_int(), C.c_int(), C.c_int64()
+        self._c(lib().dpmd_neighbor_info(self.ctx, C.byref(cap), C.byref(mx), C.byref(tot)))
+        return cap.value, mx.value, tot.value
+
+    def n_neighbors(self) -> int: return self.neighbor_info()[2]
+
+    def neighbors_csr(self):
+        tot = self.n_neighbors()
+        n = self.n_vertices() if self.kind == L.RIGID_BUMPY else self.n_particles()
+        start = np.empty(n + 1, np.int32)
+        ids = np.empty(max(1, tot), np.int32)
+        self._c(lib().dpmd_get_neighbors_csr(self.ctx, start.ctypes.data, ids.ctypes.data, ids.size))
+        return start, ids[:tot]
+
+    # ---- primitives -------------------------------------------------------------------------
+    def _ps(self, x, dtype=np.float64):
+        d = _per_system(x, self.n_systems(), dtype)
+        self._keep = [d]
+        return d
+
+    def compute_forces(self): self._c(lib().dpmd_compute_forces(self.ctx, L.FORCE_PBC))
+    def compute_wall_forces(self): self._c(lib().dpmd_compute_forces(self.ctx, L.FORCE_WALL))
+    def compute_forces_mode(self, mode: int): self._c(lib().dpmd_compute_forces(self.ctx, mode))
+
+    def update_positions(self, scale, scale2: float = 1.0):
+        self._c(lib().dpmd_update_positions(self.ctx, self._ps(scale).ptr, scale2))
+
+    def update_velocities(self, scale, scale2: float = 1.0):
+        self._c(lib().dpmd_update_velocities(self.ctx, self._ps(scale).ptr, scale2))
+
+    def scale_velocities(self, scale): self._c(lib().dpmd_scale_velocities(self.ctx, self._ps(scale).ptr))
+    def scale_positions(self, scale): self._c(lib().dpmd_scale_positions(self.ctx, self._ps(scale).ptr))
+
+    def mix_velocities_and_forces(self, alpha):
+        self._c(lib().dpmd_mix_velocities_and_forces(self.ctx, self._ps(alpha).ptr))
+
+    def zero_velocities(self): self._c(lib().dpmd_zero_velocities(self.ctx))
+
+    def compute_ke_total(self): self._c(lib().dpmd_compute_ke_total(self.ctx))
+    def compute_pe_total(self): self._c(lib().dpmd_compute_pe_total(self.ctx))
+    def compute_fpower_total(self): self._c(lib().dpmd_compute_fpower_total(self.ctx))
+    def compute_temperature(self): self._c(lib().dpmd_compute_temperature(self.ctx))
+    def compute_packing_fraction(self): self._c(lib().dpmd_compute_packing_fraction(self.ctx))
+    def compute_stress_tensor(self): self._c(lib().dpmd_compute_stress_tensor(self.ctx))
+    def compute_pressure(self): self._c(lib().dpmd_compute_pressure(self.ctx))
+    def compute_max_force(self): self._c(lib().dpmd_compute_max_force(self.ctx))
+
+    def set_average_velocity(self, vx=0.0, vy=0.0):
+        ax, ay = self._ps(vx), _per_system(vy, self.n_systems())
+        self._keep = [ax, ay]
+        self._c(lib().dpmd_set_average_velocity(self.ctx, ax.ptr, ay.ptr))
+
+    def set_temperature(self, T, remove_average_velocity: bool = True):
+        self._c(lib().dpmd_set_temperature(self.ctx, self._ps(T).ptr, int(remove_average_velocity)))
+
+    def increment_packing_fraction(self, inc):
+        self._c(lib().dpmd_increment_packing_fraction(self.ctx, self._ps(inc).ptr))
+
+    def save_state(self, flag=None, true_val: int = 1):
+        f = self._ps(1 if flag is None else flag, np.int32)
+        self._c(lib().dpmd_save_state(self.ctx, f.ptr, true_val))
+
+    def restore_state(self, flag=None, true_val: int = 1):
+        f = self._ps(1 if flag is None else flag, np.int32)
+        self._c(lib().dpmd_restore_state(self.ctx, f.ptr, true_val))
+
+    def rand_uniform_2d(self, fx: str, fy: str, seed: int, call: int, lox, hix, loy, hiy):
+        self._c(lib().dpmd_rand_uniform_2d(self.ctx, F[fx], F[fy], seed, call, lox, hix, loy, hiy))
+
+    def rand_normal_2d(self, fx: str, fy: str, seed: int, call: int, mx=0.0, sx=1.0, my=0.0, sy=1.0):
+        self._c(lib().dpmd_rand_normal_2d(self.ctx, F[fx], F[fy], seed, call, mx, sx, my, sy))
+
+    def synchronize(self): self._c(lib().dpmd_synchronize(self.ctx))
+
+
+class Disk(_Particles):
+    kind = L.DISK
+
+
+class RigidBumpy(_Particles):
+    kind = L.RIGID_BUMPY
+
+
+class VelocityVerlet:
+    """VelocityVerlet / VelocityVerletWall (include/integrators/velocity_verlet.cuh:53-78)."""
+
+    def __init__(self, P: _Particles, dt, wall: bool = False, mode: Optional[int] = None):
+        self.P = P
+        self.mode = mode if mode is not None else (L.FORCE_WALL if wall else L.FORCE_PBC)
+        self.dt = _per_system(dt, P.n_systems())
+
+    def init(self): self.P._c(lib().dpmd_vv_init(self.P.ctx, self.dt.ptr, self.mode))
+    def step(self, n: int = 1): self.P._c(lib().dpmd_vv_step(self.P.ctx, n))
+    def run(self, n: int, use_graph: bool = True): self.P._c(lib().dpmd_vv_run(self.P.ctx, n, int(use_graph)))
+
+
+class FIRE:
+    """FIRE / FIREWall (include/integrators/fire.cuh:152-178)."""
+
+    def __init__(self, P: _Particles, dt, wall: bool = False, mode: Optional[int] = None):
+        self.P = P
+        self.mode = mode if mode is not None else (L.FORCE_WALL if wall else L.FORCE_PBC)
+        self.dt = _per_system(dt, P.n_systems())
+
+    def init(self): self.P._c(lib().dpmd_fire_init(self.P.ctx, self.dt.ptr, self.mode))
+    def step(self): self.P._c(lib().dpmd_fire_step(self.P.ctx))
+
+    def converged(self, avg_pe_diff_target: float, avg_pe_target: float) -> bool:
+        a = C.c_int()
+        self.P._c(lib().dpmd_fire_converged(self.P.ctx, avg_pe_diff_target, avg_pe_target, C.byref(a)))
+        return bool(a.value)
+
+    def converged_max_force(self, tol: float) -> bool:
+        a = C.c_int()
+        self.P._c(lib().dpmd_fire_converged_max_force(self.P.ctx, tol, C.byref(a)))
+        return bool(a.value)
+
+    def get_dt(self) -> np.ndarray:
+        out = DeviceArray(np.zeros(self.P.n_systems()))
+        self.P._c(lib().dpmd_fire_get_dt(self.P.ctx, out.ptr))
+        return out.to_host()
+
+    def state(self):
+        S = self.P.n_systems()
+        dt, al = np.empty(S), np.empty(S)
+        ng, nb = np.empty(S, np.int32), np.empty(S, np.int32)
+        self.P._c(lib().dpmd_fire_state(self.P.ctx, dt.ctypes.data, al.ctypes.data, ng.ctypes.data, nb.ctypes.data))
+        return dt, al, ng, nb
+
+
+def minimize_fire(P: _Particles, dt, max_steps: int, avg_pe_target: float, avg_pe_diff_target: float,
+                  wall: bool = False) -> int:
+    """minimize_fire[_wall] (include/routines/minimizers.cuh:10-31, 68-88), host-driven.
+
+    The reference passes (avg_pe_target, avg_pe_diff_target) to converged(diff, target):
+    the two are swapped at that call (minimizers.cuh:25 vs fire.cuh:98); kept as written.
+    Returns the number of steps taken.
+    """
+    fire = FIRE(P, dt, wall=wall)
+    fire.init()
+    step, conv = 0, False
+    while step < max_steps and not conv:
+        fire.step()
+        conv = fire.converged(avg_pe_target, avg_pe_diff_target)
+        step += 1
+    return step
+
+
+def run_fire(P: _Particles, dt, max_steps: int, predicate: int, a: float, b: float = 0.0,
+             mode: int = L.FORCE_PBC, use_graph: bool = True, poll: int = 200):
+    """Device-driven FIRE minimisation (dpmd_fire_minimize). Returns (steps, converged)."""
+    d = _per_system(dt, P.n_systems())
+    steps, conv = C.c_int(), C.c_int()
+    P._c(lib().dpmd_fire_minimize(P.ctx, d.ptr, max_steps, predicate, a, b, mode, int(use_graph), poll,
+                                  C.byref(steps), C.byref(conv)))
+    return steps.value, bool(conv.value)

@@ -1,0 +1,199 @@
+/*
+ * dpmd.h -- C ABI of the MI355X-native (gfx950, HIP) batched 2D granular-MD hot path.
+ *
+ * This is the drop-in boundary for the reference's particle / integrator / routine API
+ * (mmccraw98/dpmd).  Every entry point below replaces one reference call; the citation
+ * on each names the reference interface (file:line) it stands in for.  The C++ facade in
+ * include/dpmd/*.hpp re-exposes these under the reference's own class and method names.
+ *
+ * Conventions
+ *   - All entry points return 0 on success and a nonzero dpmd_status otherwise; the
+ *     message is available from dpmd_last_error(ctx) (or dpmd_last_error(NULL) for
+ *     failures that have no context yet).  No C++ exception crosses this boundary.
+ *   - Floating point is IEEE FP64 throughout.
+ *   - Per-system parameter arrays (dt, alpha, scale, flags, ...) are DEVICE pointers of
+ *     length n_systems, exactly as the reference passes df::DeviceField1D<double>.
+ *   - Particle arrays live in "current" order: cell rebuilds reorder particles so that
+ *     each cell's particles are contiguous (reference: disk.cu:1102-1125).  The field
+ *     DPMD_F_STATIC_INDEX maps canonical (input) index -> current index, as the
+ *     reference's static_index does (base_particle_kernels.cu:172-180).
+ *   - Every call is stream-ordered on the stream given at creation (NULL = the default
+ *     stream).  Calls on one context must come from one host thread at a time.
+ */
+#ifndef DPMD_DPMD_H
+#define DPMD_DPMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dpmd_ctx dpmd_ctx;
+
+enum dpmd_status {
+    DPMD_OK = 0,
+    DPMD_ERR_ARGUMENT = 1,   /* bad argument or precondition (std::invalid_argument)    */
+    DPMD_ERR_STATE = 2,      /* call out of order (std::runtime_error in the reference) */
+    DPMD_ERR_HIP = 3,        /* HIP runtime error                                        */
+    DPMD_ERR_OVERFLOW = 4    /* neighbor list capacity exceeded                          */
+};
+
+enum dpmd_kind { DPMD_DISK = 0, DPMD_RIGID_BUMPY = 1 };
+
+/* reference: enum class md::NeighborMethod (base_particle.hpp:24) */
+enum dpmd_neighbor_method { DPMD_NEIGHBOR_NAIVE = 0, DPMD_NEIGHBOR_CELL = 1 };
+
+/* PBC = compute_forces(), WALL = compute_wall_forces() (base_particle.hpp:380-383);
+ * PER_SYSTEM picks per system from DPMD_F_BOUNDARY (0 = periodic, 1 = walls). */
+enum dpmd_force_mode { DPMD_FORCE_PBC = 0, DPMD_FORCE_WALL = 1, DPMD_FORCE_PER_SYSTEM = 2 };
+
+enum dpmd_order { DPMD_ORDER_CURRENT = 0, DPMD_ORDER_CANONICAL = 1 };
+
+/* FIRE stopping predicate: REFERENCE = fire_convergence_kernel (fire.cu:59-78);
+ * MAX_FORCE = every particle's |F_i| below a tolerance (per system). */
+enum dpmd_fire_predicate { DPMD_FIRE_PRED_REFERENCE = 0, DPMD_FIRE_PRED_MAX_FORCE = 1 };
+
+/* Fields (reference: public DeviceField members, base_particle.hpp:31-103,
+ * base_point_particle.hpp:86-89, base_poly_particle.hpp:104-120, rigid_bumpy.cuh:31-45).
+ * Type and length are given per field: P = n_particles, V = n_vertices, S = n_systems. */
+enum dpmd_field {
+    /* double, P */
+    DPMD_F_POS_X = 0, DPMD_F_POS_Y, DPMD_F_VEL_X, DPMD_F_VEL_Y, DPMD_F_FORCE_X, DPMD_F_FORCE_Y,
+    DPMD_F_PE, DPMD_F_KE, DPMD_F_RAD, DPMD_F_MASS, DPMD_F_AREA, DPMD_F_LAST_POS_X, DPMD_F_LAST_POS_Y,
+    DPMD_F_ANGLE, DPMD_F_ANGULAR_VEL, DPMD_F_TORQUE, DPMD_F_MOMENT_INERTIA,
+    DPMD_F_STRESS_XX, DPMD_F_STRESS_XY, DPMD_F_STRESS_YX, DPMD_F_STRESS_YY,
+    /* double, S */
+    DPMD_F_BOX_SIZE_X = 32, DPMD_F_BOX_SIZE_Y, DPMD_F_BOX_INV_X, DPMD_F_BOX_INV_Y,
+    DPMD_F_E_INTERACTION, DPMD_F_NEIGHBOR_CUTOFF, DPMD_F_THRESH2,
+    DPMD_F_PE_TOTAL, DPMD_F_KE_TOTAL, DPMD_F_FPOWER_TOTAL, DPMD_F_PRESSURE, DPMD_F_TEMPERATURE,
+    DPMD_F_PACKING_FRACTION, DPMD_F_MAX_FORCE2,
+    DPMD_F_STRESS_TOTAL_XX, DPMD_F_STRESS_TOTAL_XY, DPMD_F_STRESS_TOTAL_YX, DPMD_F_STRESS_TOTAL_YY,
+    /* int, S (+1 where noted) */
+    DPMD_F_SYSTEM_OFFSET = 64 /* S+1 */, DPMD_F_SYSTEM_SIZE, DPMD_F_CELL_DIM_X, DPMD_F_CELL_DIM_Y,
+    DPMD_F_CELL_SYSTEM_START /* S+1 */, DPMD_F_BOUNDARY, DPMD_F_N_DOF, DPMD_F_REBUILD_FLAG,
+    /* int, P (or V for rigid bodies: the neighbor / cell objects are vertices) */
+    DPMD_F_SYSTEM_ID = 80, DPMD_F_STATIC_INDEX, DPMD_F_CELL_ID, DPMD_F_NEIGHBOR_COUNT,
+    /* rigid bumpy (base_poly_particle.hpp:104-120) */
+    DPMD_F_VERTEX_POS_X = 96, DPMD_F_VERTEX_POS_Y, DPMD_F_VERTEX_FORCE_X, DPMD_F_VERTEX_FORCE_Y,
+    DPMD_F_VERTEX_PE, DPMD_F_VERTEX_RAD, DPMD_F_VERTEX_MASS,              /* double, V */
+    DPMD_F_VERTEX_PARTICLE_ID = 112, DPMD_F_PARTICLE_OFFSET /* P+1 */,
+    DPMD_F_N_VERTICES_PER_PARTICLE, DPMD_F_VERTEX_SYSTEM_ID,
+    DPMD_F_VERTEX_SYSTEM_OFFSET /* S+1 */                                 /* int */
+};
+
+/* ---- context lifecycle ------------------------------------------------------------ */
+/* Replaces constructing md::disk::Disk / md::rigid_bumpy::RigidBumpy (disk.cuh:22,
+ * rigid_bumpy.cuh:24).  hip_stream is a hipStream_t (NULL = default stream). */
+int dpmd_ctx_create(dpmd_ctx** out, int kind, void* hip_stream);
+int dpmd_ctx_destroy(dpmd_ctx* ctx);
+const char* dpmd_last_error(const dpmd_ctx* ctx);
+
+/* allocate_systems / allocate_particles / allocate_vertices (base_particle.hpp:315-338) */
+int dpmd_allocate_systems(dpmd_ctx* ctx, int n_systems);
+int dpmd_allocate_particles(dpmd_ctx* ctx, int n_particles);
+int dpmd_allocate_vertices(dpmd_ctx* ctx, int n_vertices);
+/* set_neighbor_method (base_particle.hpp:260) */
+int dpmd_set_neighbor_method(dpmd_ctx* ctx, int method);
+/* n_particles / n_systems / n_vertices / n_cells (base_particle.hpp:291-312) */
+int dpmd_sizes(const dpmd_ctx* ctx, int* n_systems, int* n_particles, int* n_vertices, int* n_cells);
+
+/* ---- fields ----------------------------------------------------------------------- */
+/* Device pointer and element count of a context-owned field (DeviceField::ptr(),
+ * device_fields.cuh:303-463). */
+int dpmd_field(dpmd_ctx* ctx, int field, void** device_ptr, int64_t* count);
+/* from_host (device_fields.cuh:426-429): count must equal the field length. */
+int dpmd_set_field(dpmd_ctx* ctx, int field, const void* host, int64_t count);
+/* to_host, in current or canonical order (canonical applies to P- and V-length fields). */
+int dpmd_get_field(dpmd_ctx* ctx, int field, void* host, int64_t count, int order);
+
+/* sync_box / sync_system / sync_cells / sync_class_constants (base_particle.hpp:346-368):
+ * validates the inputs and derives box inverses, system ids, cell layout. */
+int dpmd_sync(dpmd_ctx* ctx);
+
+/* ---- neighbors -------------------------------------------------------------------- */
+int dpmd_init_neighbors(dpmd_ctx* ctx);                        /* base_particle.hpp:265-272 */
+int dpmd_update_neighbors(dpmd_ctx* ctx);                      /* base_particle.hpp:275-280 */
+int dpmd_check_neighbors(dpmd_ctx* ctx, int force_update);     /* base_particle.hpp:283-288 */
+/* n_neighbors (base_particle.hpp:298-310) plus the fixed per-object capacity used here. */
+int dpmd_neighbor_info(dpmd_ctx* ctx, int* capacity, int* max_count, int64_t* total);
+/* neighbor_start / neighbor_ids in the reference's CSR layout, current order. */
+int dpmd_get_neighbors_csr(dpmd_ctx* ctx, int* start, int* ids, int64_t ids_len);
+/* Neighbor-list capacity per object (default 32 for disks, 48 for vertices). */
+int dpmd_set_neighbor_capacity(dpmd_ctx* ctx, int capacity);
+
+/* ---- particle primitives (base_particle.hpp:380-420) -------------------------------- */
+int dpmd_compute_forces(dpmd_ctx* ctx, int mode);
+int dpmd_update_positions(dpmd_ctx* ctx, const double* scale, double scale2);
+int dpmd_update_velocities(dpmd_ctx* ctx, const double* scale, double scale2);
+int dpmd_scale_velocities(dpmd_ctx* ctx, const double* scale);
+int dpmd_scale_positions(dpmd_ctx* ctx, const double* scale);
+int dpmd_mix_velocities_and_forces(dpmd_ctx* ctx, const double* alpha);
+int dpmd_zero_velocities(dpmd_ctx* ctx);
+int dpmd_compute_ke(dpmd_ctx* ctx);
+int dpmd_compute_ke_total(dpmd_ctx* ctx);            /* base_particle.hpp:601-606 */
+int dpmd_compute_pe_total(dpmd_ctx* ctx);            /* base_particle.hpp:595-598 */
+int dpmd_compute_fpower_total(dpmd_ctx* ctx);        /* disk.cu:941-960            */
+int dpmd_compute_temperature(dpmd_ctx* ctx);         /* base_particle.hpp:609-615  */
+int dpmd_compute_packing_fraction(dpmd_ctx* ctx);    /* base_particle.hpp:627-647  */
+int dpmd_compute_stress_tensor(dpmd_ctx* ctx);       /* disk.cu:702-769            */
+int dpmd_compute_pressure(dpmd_ctx* ctx);            /* base_particle.hpp:682-696  */
+int dpmd_compute_max_force(dpmd_ctx* ctx);           /* new: max_i |F_i|^2 per system */
+/* set_average_velocity (disk.cu:771-811) with per-system targets (device, S each). */
+int dpmd_set_average_velocity(dpmd_ctx* ctx, const double* avg_x, const double* avg_y);
+/* set_temperature (base_particle.hpp:449-462) */
+int dpmd_set_temperature(dpmd_ctx* ctx, const double* temperature_target, int remove_average_velocity);
+/* increment_packing_fraction (base_particle.hpp:505-529) */
+int dpmd_increment_packing_fraction(dpmd_ctx* ctx, const double* increment);
+/* save_state / restore_state (base_particle.hpp:708-736) */
+int dpmd_save_state(dpmd_ctx* ctx, const int* flag, int true_val);
+int dpmd_restore_state(dpmd_ctx* ctx, const int* flag, int true_val);
+
+/* ---- stateless Philox4x32-10 (device_fields.cuh:219-298) ------------------------------ */
+int dpmd_rand_uniform_2d(dpmd_ctx* ctx, int field_x, int field_y, uint64_t seed, uint64_t call,
+                         double lo_x, double hi_x, double lo_y, double hi_y);
+int dpmd_rand_uniform_1d(dpmd_ctx* ctx, int field, uint64_t seed, uint64_t call, double lo, double hi);
+int dpmd_rand_normal_2d(dpmd_ctx* ctx, int field_x, int field_y, uint64_t seed, uint64_t call,
+                        double mean_x, double sigma_x, double mean_y, double sigma_y);
+
+/* ---- integrators (include/integrators/*.cuh) ---------------------------------------- */
+/* VelocityVerlet[Wall](P, dt).init() / .step() (velocity_verlet.cuh:17-47). */
+int dpmd_vv_init(dpmd_ctx* ctx, const double* dt, int mode);
+int dpmd_vv_step(dpmd_ctx* ctx, int n_steps);
+/* FIRE[Wall](P, dt).init() / .step() / .converged() / .get_dt() (fire.cuh:44-109). */
+int dpmd_fire_init(dpmd_ctx* ctx, const double* dt, int mode);
+int dpmd_fire_step(dpmd_ctx* ctx);
+int dpmd_fire_converged(dpmd_ctx* ctx, double avg_pe_diff_target, double avg_pe_target, int* all_converged);
+int dpmd_fire_converged_max_force(dpmd_ctx* ctx, double force_tol, int* all_converged);
+int dpmd_fire_get_dt(dpmd_ctx* ctx, double* dt_out);
+/* FIRE scalar state per system (host copies, length S each; any pointer may be NULL). */
+int dpmd_fire_state(dpmd_ctx* ctx, double* dt, double* alpha, int* n_good, int* n_bad);
+
+/* ---- fused, device-driven runs ------------------------------------------------------ */
+/* minimize_fire[_wall] (minimizers.cuh:10-121) run on the device: every step's stopping
+ * test is evaluated per system on the GPU, stopped systems are masked, and the host only
+ * polls every poll_interval steps.  With use_graph the step is captured once into a
+ * hipGraph and replayed.  predicate: dpmd_fire_predicate; (a, b) = (avg_pe_diff_target,
+ * avg_pe_target) for REFERENCE, (force_tol, unused) for MAX_FORCE. */
+int dpmd_fire_minimize(dpmd_ctx* ctx, const double* dt, int max_steps, int predicate, double a,
+                       double b, int mode, int use_graph, int poll_interval, int* steps_taken,
+                       int* all_converged);
+/* n_steps of VelocityVerlet after dpmd_vv_init, device-driven (graph when use_graph). */
+int dpmd_vv_run(dpmd_ctx* ctx, int n_steps, int use_graph);
+
+/* ---- memory helpers so C++ callers need no HIP headers ------------------------------- */
+int dpmd_malloc(void** device_ptr, int64_t bytes);
+int dpmd_free(void* device_ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device */
+int dpmd_memcpy(void* dst, const void* src, int64_t bytes, int kind);
+int dpmd_fill_f64(void* device_ptr, int64_t n, double value);
+int dpmd_fill_i32(void* device_ptr, int64_t n, int32_t value);
+int dpmd_synchronize(dpmd_ctx* ctx);
+/* Library version string. */
+const char* dpmd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DPMD_DPMD_H */

@@ -1,0 +1,746 @@
+// =====================================================================================
+//  dpmd oracle -- TEST INFRASTRUCTURE ONLY.
+//
+//  A plain C++17 FP64 host restatement of the reference (mmccraw98/dpmd) routines on the
+//  batched 2D granular-MD hot path.  Only tests/, __graft_entry__.smoke() and bench.py's
+//  cpu_baseline leg may load this library, and only as the checker / the timed CPU
+//  baseline.  The product (dpmd_amd/libdpmd_amd.so) never links, loads or calls it.
+//
+//  Every routine cites the reference file:line it restates.  Where the reference is
+//  nondeterministic (atomic bucket scatter, CUB segmented-sum order) this file fixes one
+//  deterministic rule and states it; the HIP library implements the same rule, so the
+//  two agree bit for bit:
+//    * in-cell order: stable by current index (the reference's Standard sort path,
+//      include/particles/base_point_particle.hpp:423-447, with a stable key sort);
+//    * neighbor order: 3x3 stencil (dy outer, dx inner) then in-cell order
+//      (src/kernels/base_point_particle_kernels.cu:72-94);
+//    * per-system sums: system-aligned 256-element chunks, each reduced by a
+//      64-lane xor butterfly per wave and ((w0+w1)+(w2+w3)) across 4 waves; chunk
+//      partials are folded by 256 strided sequential accumulators and the same tree.
+//  Compile with -ffp-contract=off: no a*b+c is ever fused (the HIP side is built the same
+//  way).
+//
+//  Pinning (what fixes this restatement independently of the HIP code): Random123
+//  Philox4x32-10 published known-answer vectors; reference unit-test asserts ported as
+//  known-answer tests (tests/test_base_particle.cu:52-91, test_base_point_particle.cu,
+//  test_disk.cu); closed-form two/three-disk forces and energies; brute-force O(N^2)
+//  neighbor sets; Newton's third law; the harmonic-oscillator VV error order; FIRE state
+//  machine branch tables; rigid-body closed forms (I = m R^2, torque = r x F).  See
+//  tests/test_oracle_*.py.
+// =====================================================================================
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <string>
+#include <omp.h>
+
+namespace {
+
+// ------------------------------------------------------------------------------------
+// Deterministic per-system reduction tree (the HIP kernels implement the same tree).
+// ------------------------------------------------------------------------------------
+double wave_tree(const double* v) {
+    double a[64], b[64];
+    for (int l = 0; l < 64; ++l) a[l] = v[l];
+    for (int off = 32; off >= 1; off >>= 1) {
+        for (int l = 0; l < 64; ++l) b[l] = a[l] + a[l ^ off];
+        for (int l = 0; l < 64; ++l) a[l] = b[l];
+    }
+    return a[0];
+}
+
+double block_tree(const double* v256) {
+    double w0 = wave_tree(v256), w1 = wave_tree(v256 + 64);
+    double w2 = wave_tree(v256 + 128), w3 = wave_tree(v256 + 192);
+    return (w0 + w1) + (w2 + w3);
+}
+
+// Sum of vals[0..n) with the chunked tree.
+double sys_sum(const double* vals, int n) {
+    const int nch = (n + 255) / 256;
+    std::vector<double> part(nch > 0 ? nch : 1, 0.0);
+    double buf[256];
+    for (int c = 0; c < nch; ++c) {
+        for (int t = 0; t < 256; ++t) {
+            const int idx = c * 256 + t;
+            buf[t] = idx < n ? vals[idx] : 0.0;
+        }
+        part[c] = block_tree(buf);
+    }
+    for (int t = 0; t < 256; ++t) {
+        double acc = 0.0;
+        for (int c = t; c < nch; c += 256) acc = acc + part[c];
+        buf[t] = acc;
+    }
+    return block_tree(buf);
+}
+
+// ------------------------------------------------------------------------------------
+// Geometry helpers -- include/kernels/base_particle_kernels.cuh:117-179
+// ------------------------------------------------------------------------------------
+inline double wrap01(double u) {            // base_particle_kernels.cuh:117-120
+    u -= std::floor(u);
+    return (u >= 1.0) ? 0.0 : u;
+}
+
+inline double disp_pbc_L(double xi, double yi, double xj, double yj, double Lx, double Ly,
+                         double ix, double iy, double& dx, double& dy) {   // :169-179
+    dx = xi - xj; dy = yi - yj;
+    dx = (Lx > 0.0) ? (dx - Lx * std::nearbyint(dx * ix)) : dx;
+    dy = (Ly > 0.0) ? (dy - Ly * std::nearbyint(dy * iy)) : dy;
+    return dx * dx + dy * dy;
+}
+
+// ------------------------------------------------------------------------------------
+// Philox4x32-10 -- include/utils/device_fields.cuh:150-207 (Random123 constants)
+// ------------------------------------------------------------------------------------
+void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c[2];
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+}
+
+inline double u01d(uint32_t hi, uint32_t lo) {  // device_fields.cuh:203-207
+    const uint64_t bits = ((uint64_t)hi << 21) | ((uint64_t)lo >> 11);
+    return ((double)bits + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// ------------------------------------------------------------------------------------
+// Disk system state (SoA, current order) -- include/particles/base_particle.hpp:31-103,
+// include/particles/base_point_particle.hpp:86-89
+// ------------------------------------------------------------------------------------
+struct FireState {   // include/integrators/fire.cuh:116-147
+    std::vector<double> dt, dt_max, dt_min, dt_rev, vscale, alpha, pe_prev;
+    std::vector<int> n_good, n_bad;
+    double alpha_init = 0.1, f_inc = 1.1, f_dec = 0.5, f_alpha = 0.99;
+    int n_min = 5, n_bad_max = 10;
+    double dt_max_scale = 10.0, dt_min_scale = 0.001;
+};
+
+struct Disk {
+    int S = 0, N = 0, K = 0;       // K = neighbor capacity per particle
+    int method = 1;                 // 0 naive, 1 cell
+    std::vector<int> off, nx, ny, cstart, boundary, sid, static_index, cell_id;
+    std::vector<double> Lx, Ly, ix, iy, e, cutoff, thresh2;
+    std::vector<double> x, y, vx, vy, fx, fy, pe, rad, mass, area, lastx, lasty;
+    std::vector<unsigned> flag;
+    std::vector<std::vector<int>> nbr;
+    int max_count = 0;
+    FireState fire;
+    std::vector<double> vv_dt, vv_dt_half;
+    // saved state (base_particle.hpp:708-736; disk.cu:348-432)
+    std::vector<double> sx, sy, srad, smass, sLx, sLy;
+    std::vector<int> sstatic;
+
+    void sync_box() {   // base_particle_kernels.cu:50-65
+        ix.resize(S); iy.resize(S);
+        for (int s = 0; s < S; ++s) {
+            ix[s] = Lx[s] > 0.0 ? 1.0 / Lx[s] : 0.0;
+            iy[s] = Ly[s] > 0.0 ? 1.0 / Ly[s] : 0.0;
+        }
+    }
+    void sync_cells() {  // base_particle.hpp:552-592 (cell_system_start = exclusive scan)
+        cstart.assign(S + 1, 0);
+        for (int s = 0; s < S; ++s) cstart[s + 1] = cstart[s] + nx[s] * ny[s];
+    }
+
+    int hash_local(int i) const {   // base_particle_kernels.cu:205-233
+        const int s = sid[i];
+        const double u = wrap01(x[i] * ix[s]);
+        const double v = wrap01(y[i] * iy[s]);
+        int cx = (int)std::floor(u * nx[s]);
+        int cy = (int)std::floor(v * ny[s]);
+        cx = cx < nx[s] ? (cx > 0 ? cx : 0) : nx[s] - 1;   // guard: u*nx may round up to nx
+        cy = cy < ny[s] ? (cy > 0 ? cy : 0) : ny[s] - 1;
+        return cx + cy * nx[s];
+    }
+
+    // Cell rebuild of one system: hash, stable counting sort, reorder, static index,
+    // reset displacements, neighbor fill.  base_point_particle.hpp:180-201,401-447;
+    // disk.cu:1097-1125; base_particle_kernels.cu:172-180.
+    void rebuild_system(int s) {
+        const int b = off[s], en = off[s + 1], n = en - b;
+        const int C = nx[s] * ny[s];
+        std::vector<int> cid(n), cnt(C + 1, 0), order(n), inv(n);
+        for (int t = 0; t < n; ++t) { cid[t] = hash_local(b + t); cnt[cid[t] + 1]++; }
+        for (int c = 0; c < C; ++c) cnt[c + 1] += cnt[c];
+        std::vector<int> cs(cnt.begin(), cnt.end());   // cell starts (local)
+        std::vector<int> wp(cnt.begin(), cnt.end() - 1);
+        for (int t = 0; t < n; ++t) order[wp[cid[t]]++] = t;   // stable by current index
+        for (int p = 0; p < n; ++p) inv[order[p]] = p;
+        auto gather = [&](std::vector<double>& a) {
+            std::vector<double> tmp(n);
+            for (int p = 0; p < n; ++p) tmp[p] = a[b + order[p]];
+            for (int p = 0; p < n; ++p) a[b + p] = tmp[p];
+        };
+        gather(x); gather(y); gather(vx); gather(vy); gather(fx); gather(fy);
+        gather(rad); gather(mass); gather(pe); gather(area);
+        for (int c = b; c < en; ++c) static_index[c] = b + inv[static_index[c] - b];
+        for (int p = 0; p < n; ++p) {
+            cell_id[b + p] = cstart[s] + cid[order[p]];
+            lastx[b + p] = x[b + p]; lasty[b + p] = y[b + p];
+        }
+        // neighbor fill -- base_point_particle_kernels.cu:98-157
+        for (int p = 0; p < n; ++p) {
+            const int i = b + p;
+            std::vector<int>& L = nbr[i];
+            L.clear();
+            const int loc = cid[order[p]];
+            const int cxi = loc % nx[s], cyi = loc / nx[s];
+            for (int dy = -1; dy <= 1; ++dy) {
+                int yy = cyi + dy; if (yy < 0) yy += ny[s]; else if (yy >= ny[s]) yy -= ny[s];
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int xx = cxi + dx; if (xx < 0) xx += nx[s]; else if (xx >= nx[s]) xx -= nx[s];
+                    const int c = yy * nx[s] + xx;
+                    for (int q = cs[c]; q < cs[c + 1]; ++q) {
+                        const int j = b + q;
+                        if (j == i) continue;
+                        double ddx, ddy;
+                        const double r2 = disp_pbc_L(x[i], y[i], x[j], y[j], Lx[s], Ly[s], ix[s], iy[s], ddx, ddy);
+                        const double cut = rad[i] + rad[j] + cutoff[s];
+                        if (r2 < cut * cut) L.push_back(j);
+                    }
+                }
+            }
+            max_count = std::max(max_count, (int)L.size());
+        }
+        flag[s] = 0u;
+    }
+
+    void build_naive() {   // base_point_particle_kernels.cu:6-35
+        for (int s = 0; s < S; ++s)
+            for (int i = off[s]; i < off[s + 1]; ++i) {
+                nbr[i].clear();
+                for (int j = off[s]; j < off[s + 1]; ++j) if (j != i) nbr[i].push_back(j);
+                max_count = std::max(max_count, (int)nbr[i].size());
+            }
+    }
+
+    void check_neighbors(bool force) {  // base_particle.hpp:283-288,428; disk.cu:1127-1138
+        if (method == 0) return;
+        for (int s = 0; s < S; ++s) if (force || flag[s]) rebuild_system(s);
+    }
+
+    void init_neighbors() {   // base_particle.hpp:265-272
+        nbr.assign(N, {});
+        cell_id.assign(N, 0);
+        lastx = x; lasty = y;
+        flag.assign(S, 0u);
+        if (method == 0) build_naive(); else check_neighbors(true);
+    }
+
+    // Harmonic PBC contact -- src/particles/disk.cu:103-160
+    void force_pbc_particle(int i) {
+        const int s = sid[i];
+        const double ei = e[s];
+        const double xi = x[i], yi = y[i], ri = rad[i];
+        double fxi = 0.0, fyi = 0.0, pei = 0.0;
+        for (int j : nbr[i]) {
+            double dx, dy;
+            const double r2 = disp_pbc_L(xi, yi, x[j], y[j], Lx[s], Ly[s], ix[s], iy[s], dx, dy);
+            const double radsum = ri + rad[j];
+            if (r2 >= radsum * radsum) continue;
+            const double r = std::sqrt(r2);
+            const double inv = 1.0 / r;
+            const double nxv = dx * inv, nyv = dy * inv;
+            const double delta = radsum - r;
+            const double fmag = ei * delta;
+            fxi += fmag * nxv;
+            fyi += fmag * nyv;
+            pei += (0.5 * ei * delta * delta) * 0.5;
+        }
+        fx[i] = fxi; fy[i] = fyi; pe[i] = pei;
+    }
+
+    // Harmonic walls + non-periodic repulsive pair term -- src/particles/disk.cu:163-246.
+    // Deviation (documented): the pair displacement is xi - xj (repulsive, consistent with
+    // the energy 0.5 k delta^2); the reference writes xj - xi there (:218).
+    void force_wall_particle(int i) {
+        const int s = sid[i];
+        const double ei = e[s];
+        const double bx = Lx[s], by = Ly[s];
+        const double xi = x[i], yi = y[i], ri = rad[i];
+        double fxi = 0.0, fyi = 0.0, pei = 0.0;
+        if (xi < ri) { const double d = ri - xi; fxi += ei * d; pei += (0.5 * ei * d * d); }
+        if (xi > bx - ri) { const double d = ri - (bx - xi); fxi -= ei * d; pei += (0.5 * ei * d * d); }
+        if (yi < ri) { const double d = ri - yi; fyi += ei * d; pei += (0.5 * ei * d * d); }
+        if (yi > by - ri) { const double d = ri - (by - yi); fyi -= ei * d; pei += (0.5 * ei * d * d); }
+        for (int j : nbr[i]) {
+            const double dx = xi - x[j], dy = yi - y[j];
+            const double r2 = dx * dx + dy * dy;
+            const double radsum = ri + rad[j];
+            if (r2 >= radsum * radsum) continue;
+            const double r = std::sqrt(r2);
+            const double inv = 1.0 / r;
+            const double nxv = dx * inv, nyv = dy * inv;
+            const double delta = radsum - r;
+            const double fmag = ei * delta;
+            fxi += fmag * nxv;
+            fyi += fmag * nyv;
+            pei += (0.5 * ei * delta * delta) * 0.5;
+        }
+        fx[i] = fxi; fy[i] = fyi; pe[i] = pei;
+    }
+
+    // mode 0: PBC kernel for every system (compute_forces), 1: wall kernel for every
+    // system (compute_wall_forces), 2: per-system boundary flag.
+    void compute_forces(int mode) {
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            const bool wall = mode == 1 || (mode == 2 && boundary[sid[i]] != 0);
+            if (wall) force_wall_particle(i); else force_pbc_particle(i);
+        }
+    }
+
+    // disk.cu:20-57 (cell) / :60-75 (naive)
+    void update_positions(const double* scale, double scale2) {
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            const int s = sid[i];
+            const double sc = scale[s] * scale2;
+            x[i] += vx[i] * sc;
+            y[i] += vy[i] * sc;
+        }
+        if (method == 1) {
+            for (int i = 0; i < N; ++i) {
+                const double dx = x[i] - lastx[i], dy = y[i] - lasty[i];
+                const double d2 = dx * dx + dy * dy;
+                if (d2 > thresh2[sid[i]]) flag[sid[i]] = 1u;
+            }
+        }
+    }
+    void update_velocities(const double* scale, double scale2) {   // disk.cu:78-100
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            const double smi = scale[sid[i]] * scale2 / mass[i];
+            vx[i] = vx[i] + fx[i] * smi;
+            vy[i] = vy[i] + fy[i] * smi;
+        }
+    }
+    void scale_velocities(const double* scale) {   // disk.cu:278-289
+        for (int i = 0; i < N; ++i) { vx[i] *= scale[sid[i]]; vy[i] *= scale[sid[i]]; }
+    }
+    void scale_positions(const double* scale) {    // disk.cu:292-303
+        for (int i = 0; i < N; ++i) { x[i] *= scale[sid[i]]; y[i] *= scale[sid[i]]; }
+    }
+    void mix(const double* alpha) {   // disk.cu:307-335
+        for (int i = 0; i < N; ++i) {
+            const double a = alpha[sid[i]];
+            double vxi = vx[i], vyi = vy[i];
+            const double fxi = fx[i], fyi = fy[i];
+            const double fn = std::sqrt(fxi * fxi + fyi * fyi);
+            const double vn = std::sqrt(vxi * vxi + vyi * vyi);
+            double ratio = 0.0;
+            if (fn > 1e-16) ratio = vn / fn * a;
+            else { vxi = 0.0; vyi = 0.0; ratio = 0.0; }
+            vx[i] = vxi * (1 - a) + fxi * ratio;
+            vy[i] = vyi * (1 - a) + fyi * ratio;
+        }
+    }
+
+    template <class F>
+    void sys_reduce(F f, double* out) {
+        std::vector<double> v(N);
+        for (int i = 0; i < N; ++i) v[i] = f(i);
+        for (int s = 0; s < S; ++s) out[s] = sys_sum(v.data() + off[s], off[s + 1] - off[s]);
+    }
+    void pe_total(double* out) { sys_reduce([&](int i) { return pe[i]; }, out); }
+    void ke_total(double* out) {   // disk.cu:266-275
+        sys_reduce([&](int i) { return 0.5 * (vx[i] * vx[i] + vy[i] * vy[i]) * mass[i]; }, out);
+    }
+    void fpower_total(double* out) {   // disk.cu:338-346
+        sys_reduce([&](int i) { return fx[i] * vx[i] + fy[i] * vy[i]; }, out);
+    }
+    void max_force2(double* out) {
+        for (int s = 0; s < S; ++s) {
+            double m = 0.0;
+            for (int i = off[s]; i < off[s + 1]; ++i) m = std::max(m, fx[i] * fx[i] + fy[i] * fy[i]);
+            out[s] = m;
+        }
+    }
+    void temperature(double* out) {   // base_particle_kernels.cu:115-124, disk.cu:1251-1254
+        ke_total(out);
+        for (int s = 0; s < S; ++s) out[s] = out[s] * 2.0 / (2 * (off[s + 1] - off[s]));
+    }
+    void packing_fraction(double* out) {   // base_particle_kernels.cu:102-113
+        sys_reduce([&](int i) { return area[i] / (Lx[sid[i]] * Ly[sid[i]]); }, out);
+    }
+    // Stress tensor (disk.cu:702-769) and pressure 0.5*(sxx+syy) (base_particle_kernels.cu:126-135)
+    void stress(std::vector<double>& sxx, std::vector<double>& sxy, std::vector<double>& syx, std::vector<double>& syy) {
+        sxx.assign(N, 0); sxy.assign(N, 0); syx.assign(N, 0); syy.assign(N, 0);
+        for (int i = 0; i < N; ++i) {
+            const int s = sid[i];
+            const double ei = e[s], m = mass[i];
+            double axx = m * vx[i] * vx[i], axy = m * vx[i] * vy[i];
+            double ayx = axy, ayy = m * vy[i] * vy[i];
+            const double area_box = Lx[s] * Ly[s];
+            for (int j : nbr[i]) {
+                double dx, dy;
+                const double r2 = disp_pbc_L(x[i], y[i], x[j], y[j], Lx[s], Ly[s], ix[s], iy[s], dx, dy);
+                const double radsum = rad[i] + rad[j];
+                if (r2 >= radsum * radsum) continue;
+                const double r = std::sqrt(r2);
+                const double inv = 1.0 / r;
+                const double nxv = dx * inv, nyv = dy * inv;
+                const double delta = radsum - r;
+                const double fmag = ei * delta;
+                const double fxv = fmag * nxv, fyv = fmag * nyv;
+                axx += dx * fxv / 2.0; axy += dx * fyv / 2.0;
+                ayx += dy * fxv / 2.0; ayy += dy * fyv / 2.0;
+            }
+            sxx[i] = axx / area_box; sxy[i] = axy / area_box;
+            syx[i] = ayx / area_box; syy[i] = ayy / area_box;
+        }
+    }
+    void pressure(double* out) {
+        std::vector<double> a, b, c, d;
+        stress(a, b, c, d);
+        std::vector<double> txx(S), tyy(S);
+        for (int s = 0; s < S; ++s) {
+            txx[s] = sys_sum(a.data() + off[s], off[s + 1] - off[s]);
+            tyy[s] = sys_sum(d.data() + off[s], off[s + 1] - off[s]);
+            out[s] = 0.5 * (txx[s] + tyy[s]);
+        }
+    }
+    // set_average_velocity (disk.cu:771-811): mean by the chunked tree, then v -= mean.
+    void remove_mean_velocity() {
+        std::vector<double> mx(S), my(S);
+        sys_reduce([&](int i) { return vx[i]; }, mx.data());
+        sys_reduce([&](int i) { return vy[i]; }, my.data());
+        for (int s = 0; s < S; ++s) {
+            const int n = off[s + 1] - off[s];
+            const double ax = mx[s] / n, ay = my[s] / n;
+            for (int i = off[s]; i < off[s + 1]; ++i) { vx[i] -= ax; vy[i] -= ay; }
+        }
+    }
+    // set_temperature (base_particle.hpp:449-462): scale from the current T, then remove
+    // the mean (if asked), then scale.
+    void set_temperature(const double* T, int remove_avg) {
+        std::vector<double> cur(S), sc(S);
+        temperature(cur.data());
+        for (int s = 0; s < S; ++s) sc[s] = std::sqrt(T[s] / cur[s]);
+        if (remove_avg) remove_mean_velocity();
+        scale_velocities(sc.data());
+    }
+
+    // ---------------- FIRE (include/integrators/fire.cuh, src/integrators/fire.cu) ------
+    void fire_init(const double* dt0, int mode) {   // fire.cuh:44-56,90-96,136-147
+        FireState& F = fire;
+        F.dt.assign(dt0, dt0 + S);
+        F.dt_max.resize(S); F.dt_min.resize(S);
+        for (int s = 0; s < S; ++s) { F.dt_max[s] = F.dt[s] * F.dt_max_scale; F.dt_min[s] = F.dt[s] * F.dt_min_scale; }
+        F.dt_rev.assign(S, 0.0); F.vscale.assign(S, 0.0); F.alpha.assign(S, F.alpha_init);
+        F.n_good.assign(S, 0); F.n_bad.assign(S, 0); F.pe_prev.assign(S, 1e9);
+        scale_velocities(F.vscale.data());
+        F.vscale.assign(S, 1.0);
+        compute_forces(mode);
+    }
+    void fire_update(const double* power) {   // fire.cu:7-57
+        FireState& F = fire;
+        for (int s = 0; s < S; ++s) {
+            F.dt_rev[s] = 0; F.vscale[s] = 1.0;
+            if (F.dt[s] == 0.0) { F.vscale[s] = 0.0; F.alpha[s] = 0.0; continue; }
+            if (power[s] > 0) {
+                F.n_good[s]++; F.n_bad[s] = 0;
+                if (F.n_good[s] > F.n_min) {
+                    F.dt[s] = std::fmin(F.dt[s] * F.f_inc, F.dt_max[s]);
+                    F.alpha[s] *= F.f_alpha;
+                }
+            } else {
+                F.n_good[s] = 0; F.n_bad[s]++;
+                if (F.n_bad[s] > F.n_bad_max) { F.dt[s] = 0.0; continue; }
+                F.dt[s] = std::fmax(F.dt[s] * F.f_dec, F.dt_min[s]);
+                F.alpha[s] = F.alpha_init;
+                F.dt_rev[s] = -F.dt[s] * 0.5;
+                F.vscale[s] = 0.0;
+            }
+        }
+    }
+    void fire_step(int mode) {   // fire.cuh:59-88
+        std::vector<double> P(S);
+        fpower_total(P.data());
+        fire_update(P.data());
+        update_positions(fire.dt_rev.data(), 1.0);
+        scale_velocities(fire.vscale.data());
+        update_velocities(fire.dt.data(), 0.5);
+        mix(fire.alpha.data());
+        scale_velocities(fire.vscale.data());
+        update_positions(fire.dt.data(), 0.5);
+        check_neighbors(false);
+        compute_forces(mode);
+        update_velocities(fire.dt.data(), 0.5);
+    }
+    // fire.cuh:98-109 with fire.cu:59-78.  pred 0: the reference's <pe> test (note the
+    // double division of pe_prev, kept as written); pred 1: max|F_i| < tol (added for the
+    // max|F| stopping rule; pe_prev is still updated the same way).
+    int fire_converged(double diff_target, double pe_target, int pred, double ftol) {
+        FireState& F = fire;
+        std::vector<double> pt(S), mf(S);
+        pe_total(pt.data());
+        if (pred == 1) max_force2(mf.data());
+        bool all = true;
+        for (int s = 0; s < S; ++s) {
+            const int dof = off[s + 1] - off[s];
+            const double avg = pt[s] / dof;
+            const double avg_prev = F.pe_prev[s] / dof;
+            const double diff = avg_prev == 0 ? 0 : std::abs(avg / avg_prev - 1);
+            if (pred == 0) {
+                if (diff < diff_target || avg < pe_target) F.dt[s] = 0.0;
+            } else {
+                if (std::sqrt(mf[s]) < ftol) F.dt[s] = 0.0;
+            }
+            F.pe_prev[s] = avg;
+            if (F.dt[s] != 0.0) all = false;
+        }
+        return all ? 1 : 0;
+    }
+
+    // ---------------- velocity Verlet (include/integrators/velocity_verlet.cuh:34-41) ---
+    void vv_init(const double* dt, int mode) {
+        vv_dt.assign(dt, dt + S);
+        vv_dt_half.resize(S);
+        for (int s = 0; s < S; ++s) vv_dt_half[s] = dt[s] * 0.5;
+        compute_forces(mode);
+    }
+    void vv_step(int mode) {
+        update_velocities(vv_dt_half.data(), 1.0);
+        update_positions(vv_dt.data(), 1.0);
+        check_neighbors(false);
+        compute_forces(mode);
+        update_velocities(vv_dt_half.data(), 1.0);
+    }
+
+    // increment_packing_fraction -- base_particle.hpp:505-529, kernels.cu:148-170.
+    // Guard (T7): when the shrunken cell no longer covers 2*r_max + cutoff the cell grid
+    // is re-derived with the same rule the library uses.
+    void increment_packing_fraction(const double* inc) {
+        std::vector<double> phi(S), sc(S);
+        packing_fraction(phi.data());
+        for (int s = 0; s < S; ++s) {
+            const double nphi = phi[s] + inc[s];
+            sc[s] = std::sqrt(phi[s] / nphi);
+            Lx[s] = Lx[s] * sc[s]; Ly[s] = Ly[s] * sc[s];
+        }
+        scale_positions(sc.data());
+        sync_box();
+        regrid_if_needed();
+        check_neighbors(true);
+    }
+    void regrid_if_needed() {
+        if (method != 1) return;
+        bool changed = false;
+        for (int s = 0; s < S; ++s) {
+            double rmax = 0.0;
+            for (int i = off[s]; i < off[s + 1]; ++i) rmax = std::max(rmax, rad[i]);
+            const double need = 2.0 * rmax + cutoff[s];
+            if (nx[s] <= 0 || ny[s] <= 0 || Lx[s] / nx[s] < need || Ly[s] / ny[s] < need) {
+                nx[s] = std::max(3, (int)std::floor(Lx[s] / need));
+                ny[s] = std::max(3, (int)std::floor(Ly[s] / need));
+                changed = true;
+            }
+        }
+        if (changed) sync_cells();
+    }
+
+    // save/restore state (disk.cu:348-432,1158-1225; base_particle.hpp:708-736)
+    void save_state(const int* flg, int tv) {
+        if ((int)sx.size() != N) { sx = x; sy = y; srad = rad; smass = mass; sstatic = static_index; sLx = Lx; sLy = Ly; }
+        for (int i = 0; i < N; ++i) if (flg[sid[i]] == tv) {
+            sx[i] = x[i]; sy[i] = y[i]; srad[i] = rad[i]; smass[i] = mass[i]; sstatic[i] = static_index[i];
+        }
+        for (int s = 0; s < S; ++s) if (flg[s] == tv) { sLx[s] = Lx[s]; sLy[s] = Ly[s]; }
+    }
+    void restore_state(const int* flg, int tv) {
+        for (int i = 0; i < N; ++i) if (flg[sid[i]] == tv) {
+            x[i] = sx[i]; y[i] = sy[i]; rad[i] = srad[i]; mass[i] = smass[i]; static_index[i] = sstatic[i];
+        }
+        for (int s = 0; s < S; ++s) if (flg[s] == tv) { Lx[s] = sLx[s]; Ly[s] = sLy[s]; }
+        sync_box();
+        check_neighbors(true);
+    }
+};
+
+std::string g_err;
+
+}  // namespace
+
+// =====================================================================================
+//  extern "C" surface for the Python test harness (ctypes)
+// =====================================================================================
+extern "C" {
+
+const char* or_last_error() { return g_err.c_str(); }
+
+void or_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    philox10(c, key[0], key[1]);
+    for (int k = 0; k < 4; ++k) out[k] = c[k];
+}
+
+// device_fields.cuh:256-278 (double branch)
+void or_rand_uniform_2d(int n, uint64_t seed, uint64_t call, double lox, double hix, double loy,
+                        double hiy, double* ox, double* oy) {
+    for (int i = 0; i < n; ++i) {
+        uint32_t c[4] = {(uint32_t)i, (uint32_t)call, 0u, 0u};
+        philox10(c, (uint32_t)(seed & 0xFFFFFFFFull), (uint32_t)(seed >> 32));
+        double ux = u01d(c[0], c[1]); if (ux == 1.0) ux = 0.0;
+        double uy = u01d(c[2], c[3]); if (uy == 1.0) uy = 0.0;
+        ox[i] = lox + (hix - lox) * ux;
+        oy[i] = loy + (hiy - loy) * uy;
+    }
+}
+// device_fields.cuh:219-236 (double branch)
+void or_rand_uniform_1d(int n, uint64_t seed, uint64_t call, double lo, double hi, double* o) {
+    for (int i = 0; i < n; ++i) {
+        uint32_t c[4] = {(uint32_t)i, (uint32_t)call, 0u, 0u};
+        philox10(c, (uint32_t)(seed & 0xFFFFFFFFull), (uint32_t)(seed >> 32));
+        double u = u01d(c[0], c[1]); if (u == 1.0) u = 0.0;
+        o[i] = lo + (hi - lo) * u;
+    }
+}
+
+double or_sys_sum(const double* v, int n) { return sys_sum(v, n); }
+
+void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
+void* or_disk_create(int S, int N, int K, int method, const int* off, const double* Lx, const double* Ly,
+                     const double* e, const double* cutoff, const double* thresh2, const int* nx,
+                     const int* ny, const int* boundary, const double* x, const double* y,
+                     const double* rad, const double* mass, const double* area) {
+    Disk* d = new Disk();
+    d->S = S; d->N = N; d->K = K; d->method = method;
+    d->off.assign(off, off + S + 1);
+    d->Lx.assign(Lx, Lx + S); d->Ly.assign(Ly, Ly + S);
+    d->e.assign(e, e + S); d->cutoff.assign(cutoff, cutoff + S); d->thresh2.assign(thresh2, thresh2 + S);
+    d->nx.assign(nx, nx + S); d->ny.assign(ny, ny + S); d->boundary.assign(boundary, boundary + S);
+    d->x.assign(x, x + N); d->y.assign(y, y + N); d->rad.assign(rad, rad + N); d->mass.assign(mass, mass + N);
+    d->area.assign(area, area + N);
+    d->vx.assign(N, 0.0); d->vy.assign(N, 0.0); d->fx.assign(N, 0.0); d->fy.assign(N, 0.0); d->pe.assign(N, 0.0);
+    d->sid.resize(N);
+    for (int s = 0; s < S; ++s) for (int i = off[s]; i < off[s + 1]; ++i) d->sid[i] = s;
+    d->static_index.resize(N);
+    for (int i = 0; i < N; ++i) d->static_index[i] = i;
+    d->sync_box();
+    d->regrid_if_needed();
+    d->sync_cells();
+    d->init_neighbors();
+    return d;
+}
+void or_disk_destroy(void* h) { delete (Disk*)h; }
+
+// field ids: 0 x,1 y,2 vx,3 vy,4 fx,5 fy,6 pe,7 rad,8 mass,9 area,10 lastx,11 lasty
+static std::vector<double>* dfield(Disk* d, int f) {
+    switch (f) {
+        case 0: return &d->x; case 1: return &d->y; case 2: return &d->vx; case 3: return &d->vy;
+        case 4: return &d->fx; case 5: return &d->fy; case 6: return &d->pe; case 7: return &d->rad;
+        case 8: return &d->mass; case 9: return &d->area; case 10: return &d->lastx; case 11: return &d->lasty;
+        case 12: return &d->Lx; case 13: return &d->Ly;
+    }
+    return nullptr;
+}
+void or_disk_get(void* h, int f, double* out) { auto* v = dfield((Disk*)h, f); std::memcpy(out, v->data(), v->size() * 8); }
+void or_disk_set(void* h, int f, const double* in) { auto* v = dfield((Disk*)h, f); std::memcpy(v->data(), in, v->size() * 8); }
+void or_disk_get_static_index(void* h, int* out) { Disk* d = (Disk*)h; std::memcpy(out, d->static_index.data(), d->N * 4); }
+void or_disk_get_cells(void* h, int* nx, int* ny) {
+    Disk* d = (Disk*)h;
+    for (int s = 0; s < d->S; ++s) { nx[s] = d->nx[s]; ny[s] = d->ny[s]; }
+}
+int or_disk_max_count(void* h) { return ((Disk*)h)->max_count; }
+int or_disk_n_neighbors(void* h) { Disk* d = (Disk*)h; long t = 0; for (auto& L : d->nbr) t += L.size(); return (int)t; }
+// CSR export: start[N+1], ids[total]
+void or_disk_get_neighbors(void* h, int* start, int* ids) {
+    Disk* d = (Disk*)h;
+    int w = 0;
+    for (int i = 0; i < d->N; ++i) { start[i] = w; for (int j : d->nbr[i]) ids[w++] = j; }
+    start[d->N] = w;
+}
+void or_disk_flags(void* h, unsigned* out) { Disk* d = (Disk*)h; for (int s = 0; s < d->S; ++s) out[s] = d->flag[s]; }
+void or_disk_check_neighbors(void* h, int force) { ((Disk*)h)->check_neighbors(force != 0); }
+void or_disk_compute_forces(void* h, int mode) { ((Disk*)h)->compute_forces(mode); }
+void or_disk_update_positions(void* h, const double* sc, double s2) { ((Disk*)h)->update_positions(sc, s2); }
+void or_disk_update_velocities(void* h, const double* sc, double s2) { ((Disk*)h)->update_velocities(sc, s2); }
+void or_disk_scale_velocities(void* h, const double* sc) { ((Disk*)h)->scale_velocities(sc); }
+void or_disk_scale_positions(void* h, const double* sc) { ((Disk*)h)->scale_positions(sc); }
+void or_disk_mix(void* h, const double* a) { ((Disk*)h)->mix(a); }
+void or_disk_pe_total(void* h, double* o) { ((Disk*)h)->pe_total(o); }
+void or_disk_ke_total(void* h, double* o) { ((Disk*)h)->ke_total(o); }
+void or_disk_fpower_total(void* h, double* o) { ((Disk*)h)->fpower_total(o); }
+void or_disk_temperature(void* h, double* o) { ((Disk*)h)->temperature(o); }
+void or_disk_pressure(void* h, double* o) { ((Disk*)h)->pressure(o); }
+void or_disk_packing_fraction(void* h, double* o) { ((Disk*)h)->packing_fraction(o); }
+void or_disk_max_force2(void* h, double* o) { ((Disk*)h)->max_force2(o); }
+void or_disk_stress(void* h, double* xx, double* xy, double* yx, double* yy) {
+    Disk* d = (Disk*)h;
+    std::vector<double> a, b, c, e;
+    d->stress(a, b, c, e);
+    std::memcpy(xx, a.data(), d->N * 8); std::memcpy(xy, b.data(), d->N * 8);
+    std::memcpy(yx, c.data(), d->N * 8); std::memcpy(yy, e.data(), d->N * 8);
+}
+void or_disk_set_temperature(void* h, const double* T, int rm) { ((Disk*)h)->set_temperature(T, rm); }
+void or_disk_remove_mean_velocity(void* h) { ((Disk*)h)->remove_mean_velocity(); }
+void or_disk_increment_packing_fraction(void* h, const double* inc) { ((Disk*)h)->increment_packing_fraction(inc); }
+void or_disk_save_state(void* h, const int* f, int tv) { ((Disk*)h)->save_state(f, tv); }
+void or_disk_restore_state(void* h, const int* f, int tv) { ((Disk*)h)->restore_state(f, tv); }
+
+void or_disk_fire_init(void* h, const double* dt, int mode) { ((Disk*)h)->fire_init(dt, mode); }
+void or_disk_fire_step(void* h, int mode) { ((Disk*)h)->fire_step(mode); }
+int or_disk_fire_converged(void* h, double diff, double tgt, int pred, double ftol) {
+    return ((Disk*)h)->fire_converged(diff, tgt, pred, ftol);
+}
+void or_disk_fire_get(void* h, double* dt, double* alpha, int* ngood, int* nbad) {
+    Disk* d = (Disk*)h;
+    for (int s = 0; s < d->S; ++s) { dt[s] = d->fire.dt[s]; alpha[s] = d->fire.alpha[s]; ngood[s] = d->fire.n_good[s]; nbad[s] = d->fire.n_bad[s]; }
+}
+// FIRE scalar state machine on its own (fire.cu:7-57), for the branch-table tests.
+void or_fire_update_scalar(double* dt, double* alpha, int* ngood, int* nbad, double power, double dt_max,
+                           double dt_min, double* dt_rev, double* vscale) {
+    Disk d; d.S = 1;
+    d.fire.dt = {*dt}; d.fire.alpha = {*alpha}; d.fire.n_good = {*ngood}; d.fire.n_bad = {*nbad};
+    d.fire.dt_max = {dt_max}; d.fire.dt_min = {dt_min}; d.fire.dt_rev = {0}; d.fire.vscale = {1};
+    double p = power;
+    d.fire_update(&p);
+    *dt = d.fire.dt[0]; *alpha = d.fire.alpha[0]; *ngood = d.fire.n_good[0]; *nbad = d.fire.n_bad[0];
+    *dt_rev = d.fire.dt_rev[0]; *vscale = d.fire.vscale[0];
+}
+// minimize_fire (include/routines/minimizers.cuh:10-31): returns steps taken; *conv = 1 if
+// every system stopped.
+int or_disk_fire_minimize(void* h, const double* dt, int max_steps, double diff, double tgt, int pred,
+                          double ftol, int mode, int* conv) {
+    Disk* d = (Disk*)h;
+    d->fire_init(dt, mode);
+    int step = 0; int c = 0;
+    while (step < max_steps && !c) {
+        d->fire_step(mode);
+        c = d->fire_converged(diff, tgt, pred, ftol);
+        ++step;
+    }
+    *conv = c;
+    return step;
+}
+void or_disk_vv_init(void* h, const double* dt, int mode) { ((Disk*)h)->vv_init(dt, mode); }
+void or_disk_vv_run(void* h, int n, int mode) { Disk* d = (Disk*)h; for (int k = 0; k < n; ++k) d->vv_step(mode); }
+
+// Brute-force O(N^2) min-image neighbor set of particle i (for pinning the cell list).
+int or_disk_brute_neighbors(void* h, int i, int* out) {
+    Disk* d = (Disk*)h;
+    const int s = d->sid[i];
+    int w = 0;
+    for (int j = d->off[s]; j < d->off[s + 1]; ++j) {
+        if (j == i) continue;
+        double dx, dy;
+        const double r2 = disp_pbc_L(d->x[i], d->y[i], d->x[j], d->y[j], d->Lx[s], d->Ly[s], d->ix[s], d->iy[s], dx, dy);
+        const double cut = d->rad[i] + d->rad[j] + d->cutoff[s];
+        if (r2 < cut * cut) out[w++] = j;
+    }
+    return w;
+}
+
+}  // extern "C"

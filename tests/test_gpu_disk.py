@@ -1,0 +1,166 @@
+"""GPU parity tests of the disk hot path: HIP library (through the C ABI) vs the oracle.
+
+Bar: neighbor sets bit-exact (as sorted canonical sets, and here also in order); forces,
+positions and velocities bit-identical to the oracle's FP64 restatement (the library and the
+oracle evaluate the same expressions in the same order without contraction); tolerances
+written below where a result is compared to something other than the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from dpmd_amd import _lib as L
+from dpmd_amd import workloads as W
+from dpmd_amd.system import FIRE, Disk, VelocityVerlet, run_fire
+
+pytestmark = pytest.mark.gpu
+
+NAMES = {"x": "POS_X", "y": "POS_Y", "vx": "VEL_X", "vy": "VEL_Y", "fx": "FORCE_X", "fy": "FORCE_Y",
+         "pe": "PE", "rad": "RAD"}
+
+
+def make(b, method=L.NEIGHBOR_CELL):
+    P = Disk()
+    W.load_into(P, b, method)
+    P.init_neighbors()
+    return P, orc.OracleDisk(b, method=method)
+
+
+def assert_same(P, ref, fields=("x", "y", "vx", "vy", "fx", "fy", "pe"), what=""):
+    for f in fields:
+        g, r = P.get(NAMES[f]), ref.get(f)
+        if not np.array_equal(g, r):
+            d = np.max(np.abs(g - r))
+            raise AssertionError(f"{what}: field {f} differs, max |diff| = {d:.3e}")
+
+
+def batch_small():
+    return W.disk_batch([64, 200, 97, 333, 150, 64], [0.80, 0.86, 0.83, 0.85, 0.84, 0.81],
+                        [0, 1, 0, 1, 0, 1], seed=11)
+
+
+def test_philox_device_matches_host():
+    b = W.single_disk_system(5000, 0.85, seed=3)
+    P = Disk()
+    W.load_into(P, b)
+    P.rand_uniform_2d("POS_X", "POS_Y", 123456789012345, 4, 0.0, 1.0, -2.0, 3.0)
+    ox, oy = orc.rand_uniform_2d(5000, 123456789012345, 4, 0.0, 1.0, -2.0, 3.0)
+    assert np.array_equal(P.get("POS_X"), ox)
+    assert np.array_equal(P.get("POS_Y"), oy)
+
+
+@pytest.mark.parametrize("which", ["single", "batch"])
+def test_neighbor_lists_match_oracle(which):
+    b = W.single_disk_system(1024, 0.85, seed=1) if which == "single" else batch_small()
+    P, ref = make(b)
+    gs, gi = P.neighbors_csr()
+    rs, ri = ref.neighbors_csr()
+    assert np.array_equal(P.get("STATIC_INDEX"), ref.static_index())
+    assert np.array_equal(gs, rs) and np.array_equal(gi, ri)
+    # as canonical sorted sets, against brute force O(N^2)
+    st = P.get("STATIC_INDEX")
+    inv = np.empty_like(st); inv[st] = np.arange(len(st))
+    for i in range(0, b.N, 7):
+        cur = st[i]
+        got = sorted(inv[gi[gs[cur]:gs[cur + 1]]])
+        brute = sorted(inv[ref.brute(cur)])
+        assert got == brute
+
+
+@pytest.mark.parametrize("mode", [L.FORCE_PBC, L.FORCE_WALL, L.FORCE_PER_SYSTEM])
+def test_forces_match_oracle(mode):
+    b = batch_small()
+    P, ref = make(b)
+    P.compute_forces_mode(mode)
+    ref.compute_forces(mode)
+    assert_same(P, ref, ("fx", "fy", "pe"), "forces")
+
+
+def test_forces_naive_equal_cell():
+    b = batch_small()
+    P, _ = make(b, L.NEIGHBOR_CELL)
+    Q, _ = make(b, L.NEIGHBOR_NAIVE)
+    P.compute_forces(); Q.compute_forces()
+    fc = P.get("FORCE_X", L.ORDER_CANONICAL)
+    fq = Q.get("FORCE_X")
+    # test_disk.cu:241-263: cell and naive forces agree to feq (~2e-13 relative)
+    np.testing.assert_allclose(fc, fq, rtol=2e-13, atol=1e-14)
+
+
+def test_fire_api_path_bit_identical():
+    b = batch_small()
+    P, ref = make(b)
+    fire = FIRE(P, 1e-2, mode=L.FORCE_PER_SYSTEM)
+    fire.init()
+    ref.fire_init(1e-2, 2)
+    for k in range(300):
+        fire.step()
+        c1 = fire.converged(1e-16, 1e-16)
+        ref.fire_step(2)
+        c2 = ref.fire_converged(1e-16, 1e-16)
+        assert c1 == c2
+    assert_same(P, ref, what="FIRE 300 steps")
+    gdt, gal, gng, gnb = fire.state()
+    rdt, ral, rng_, rnb = ref.fire_state()
+    assert np.array_equal(gdt, rdt) and np.array_equal(gal, ral)
+    assert np.array_equal(gng, rng_) and np.array_equal(gnb, rnb)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_fire_fused_run_bit_identical(graph):
+    b = batch_small()
+    P, ref = make(b)
+    steps, conv = run_fire(P, 1e-2, 700, L.FIRE_PRED_MAX_FORCE, 1e-9, mode=L.FORCE_PER_SYSTEM,
+                           use_graph=graph, poll=50)
+    rsteps, rconv = ref.fire_minimize(1e-2, 700, pred=1, ftol=1e-9, mode=2)
+    assert (steps, conv) == (rsteps, rconv)
+    assert_same(P, ref, ("x", "y", "fx", "fy", "pe"), "fused FIRE")
+
+
+def test_vv_bit_identical_and_graph():
+    b = W.single_disk_system(1024, 0.85, seed=1)
+    P, ref = make(b)
+    P.rand_normal_2d("VEL_X", "VEL_Y", 1, 1, 0.0, 1e-2, 0.0, 1e-2)
+    ref.set("vx", P.get("VEL_X")); ref.set("vy", P.get("VEL_Y"))
+    vv = VelocityVerlet(P, 1e-2)
+    vv.init()
+    vv.step(200)
+    vv.run(300, use_graph=True)
+    ref.vv_init(1e-2)
+    ref.vv_run(500)
+    assert_same(P, ref, what="VV 500 steps")
+
+
+def test_energies_pressure_temperature():
+    b = batch_small()
+    P, ref = make(b)
+    P.rand_normal_2d("VEL_X", "VEL_Y", 5, 1)
+    ref.set("vx", P.get("VEL_X")); ref.set("vy", P.get("VEL_Y"))
+    P.compute_forces_mode(L.FORCE_PER_SYSTEM); ref.compute_forces(2)
+    P.compute_pe_total(); P.compute_ke_total(); P.compute_pressure(); P.compute_temperature()
+    P.compute_packing_fraction(); P.compute_fpower_total()
+    assert np.array_equal(P.get("PE_TOTAL"), ref.pe_total())
+    assert np.array_equal(P.get("KE_TOTAL"), ref.ke_total())
+    assert np.array_equal(P.get("PRESSURE"), ref.pressure())
+    assert np.array_equal(P.get("TEMPERATURE"), ref.temperature())
+    assert np.array_equal(P.get("PACKING_FRACTION"), ref.packing_fraction())
+    assert np.array_equal(P.get("FPOWER_TOTAL"), ref.fpower_total())
+    np.testing.assert_allclose(P.get("PACKING_FRACTION"), [0.80, 0.86, 0.83, 0.85, 0.84, 0.81], rtol=1e-12)
+    T = np.array([1e-4, 2e-4, 3e-4, 4e-4, 5e-4, 6e-4])
+    P.set_temperature(T); ref.set_temperature(T)
+    assert_same(P, ref, ("vx", "vy"), "set_temperature")
+    P.compute_temperature()
+    np.testing.assert_allclose(P.get("TEMPERATURE"), T, rtol=1e-12)
+
+
+def test_increment_packing_fraction_and_restore():
+    b = batch_small()
+    P, ref = make(b)
+    P.save_state(); ref.save_state()
+    P.increment_packing_fraction(1e-3); ref.increment_packing_fraction(1e-3)
+    assert_same(P, ref, ("x", "y"), "compress")
+    assert np.array_equal(P.get("BOX_SIZE_X"), ref.get("Lx"))
+    P.compute_forces(); ref.compute_forces(0)
+    assert_same(P, ref, ("fx", "fy", "pe"), "compress forces")
+    P.restore_state(); ref.restore_state()
+    assert_same(P, ref, ("x", "y"), "restore")
