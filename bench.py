@@ -1,0 +1,162 @@
+"""Benchmark: particle-steps/s of the disk hot path on one MI355X.
+
+Workload (one GPU's share at its real shape): a single system of N = 2**20 bidisperse disks
+(radii 0.5 / 0.7, 50:50), phi = 0.84, periodic box, FP64, neighbor skin 0.3; FIRE steps then
+velocity-Verlet (NVE) steps at dt = 1e-2 from Gaussian velocities at T = 1e-4.  The default
+schedule is the workload's 2000 FIRE + 5000 NVE steps; --steps K keeps that 2:5 split.
+
+value = N * (FIRE steps + NVE steps) / wall time of the timed region (inputs resident in
+HBM).  The dominant kernel's roofline figure comes from HIP events around every force
+kernel launch of the NVE phase; cpu_baseline times the oracle (the C++ FP64 restatement)
+on a bounded sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=7000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    return ap.parse_args()
+
+
+def cpu_baseline(n: int, steps: int):
+    """Oracle (C++ FP64 restatement of the reference routines) on this host, bounded sample:
+    `steps` FIRE steps + `steps` velocity-Verlet steps of the same N-particle system."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import oracle as orc
+    from dpmd_amd import workloads as W
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    orc.lib().or_set_threads(threads)
+    b = W.single_disk_system(n, 0.84, seed=1)
+    d = orc.OracleDisk(b)
+    t0 = time.perf_counter()
+    d.fire_init(1e-2)
+    for _ in range(steps):
+        d.fire_step()
+        d.fire_converged(0.0, 0.0, 1, 1e-12)
+    d.vv_init(1e-2)
+    d.vv_run(steps)
+    t = time.perf_counter() - t0
+    return {"value": n * 2 * steps / t, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle, N={n} phi=0.84 PBC: {steps} FIRE + {steps} NVE steps, {t:.1f} s, "
+                      f"{threads} OpenMP threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(systems shard across GPUs with no communication, see DESIGN.md)"}))
+        sys.exit(2)
+    import numpy as np
+
+    from dpmd_amd import _lib as L
+    from dpmd_amd import workloads as W
+    from dpmd_amd.system import Disk, VelocityVerlet, run_fire
+
+    lib = L.lib()
+    N = args.n
+    K = max(7, args.steps)
+    n_fire = round(K * 2 / 7)
+    n_nve = K - n_fire
+    b = W.single_disk_system(N, 0.84, seed=1)
+
+    P = Disk()
+    W.load_into(P, b)
+    P.init_neighbors()
+    # warmup on the same context (graph capture, code paths), then restore the input
+    if args.warmup > 0:
+        run_fire(P, 1e-2, args.warmup, L.FIRE_PRED_MAX_FORCE, 1e-12, use_graph=True)
+    P.set("POS_X", b.x); P.set("POS_Y", b.y); P.set("RAD", b.rad); P.set("MASS", b.mass); P.set("AREA", b.area)
+    P.sync()
+    P.init_neighbors()
+    P.synchronize()
+
+
+    t0 = time.perf_counter()
+    steps_fire, _ = run_fire(P, 1e-2, n_fire, L.FIRE_PRED_MAX_FORCE, 1e-12, use_graph=True, poll=n_fire)
+    P.rand_normal_2d("VEL_X", "VEL_Y", 1, 1)
+    P.set_average_velocity(0.0, 0.0)
+    P.set_temperature(1e-4)
+    vv = VelocityVerlet(P, 1e-2)
+    vv.init()
+    lib.dpmd_profile(P.ctx, 1)
+    vv.step(n_nve)
+    P.synchronize()
+
+    t = time.perf_counter() - t0
+
+    tot_ms, cnt = C.c_double(), C.c_int()
+    lib.dpmd_profile_read(P.ctx, C.byref(tot_ms), C.byref(cnt))
+    lib.dpmd_profile(P.ctx, 0)
+    cap, mx, nnb = P.neighbor_info()
+    k = nnb / N
+    # k_vv_force per particle: reads x,y,rad (24), sid (4), count (4), k ids (4k), vx,vy,mass (24);
+    # writes fx,fy,pe,vx,vy (40)  -> 96 + 4k bytes (neighbor coordinate gathers excluded).
+    bytes_per_launch = N * (96 + 4 * k)
+    avg_ms = tot_ms.value / max(1, cnt.value)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+
+    # measured copy ceiling (plain copy kernel, 2 x 128 MiB per launch)
+    ncopy = 1 << 24
+    src, dst = C.c_void_p(), C.c_void_p()
+    L.check(lib.dpmd_malloc(C.byref(src), ncopy * 8)); L.check(lib.dpmd_malloc(C.byref(dst), ncopy * 8))
+    lib.dpmd_fill_f64(src, ncopy, 1.0)
+    ms_copy = C.c_double()
+    L.check(lib.dpmd_stream_copy(dst, src, ncopy, 20, C.byref(ms_copy)))
+    copy_gbs = 2 * ncopy * 8 / (ms_copy.value * 1e-3) / 1e9
+    lib.dpmd_free(src); lib.dpmd_free(dst)
+
+    P.compute_pe_total()
+    total_steps = steps_fire + n_nve
+    value = N * total_steps / t
+    out = {
+        "metric": "particle-steps/sec (FIRE + NVE, single large system)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": total_steps,
+        "warmup": args.warmup,
+        "ms_per_step": t * 1e3 / total_steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "single large disk system: N=2^20 bidisperse (0.5/0.7, 50:50), phi=0.84, "
+                               "PBC, FP64, skin 0.3; FIRE then NVE dt=1e-2 (T=1e-4)",
+                   "n_particles": N, "fire_steps": steps_fire, "nve_steps": n_nve,
+                   "mean_neighbors": k, "neighbor_capacity": cap, "max_neighbors": mx},
+        "roofline": {"bound": "hbm", "kernel": "k_vv_force (force + half kick, NVE)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_us": avg_ms * 1e3, "launches": cnt.value,
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "copy_kernel_gbs": copy_gbs},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(N, args.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

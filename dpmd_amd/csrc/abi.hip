@@ -356,9 +356,10 @@ int read_running(dpmd_ctx* c, int* steps) {
     return run[2];
 }
 
-hipGraphExec_t capture(dpmd_ctx* c, int steps_per_graph, bool fire, int pred, double a, double b) {
+hipGraphExec_t capture_once(dpmd_ctx* c, int steps_per_graph, bool fire, int pred, double a, double b) {
     hipGraph_t g = nullptr;
     HIP_OK(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
+    c->capturing = true;
     try {
         for (int k = 0; k < steps_per_graph; ++k) {
             run_begin_step(c);
@@ -366,15 +367,33 @@ hipGraphExec_t capture(dpmd_ctx* c, int steps_per_graph, bool fire, int pred, do
             else vv_step_body(c, c->run.p + 3);
         }
     } catch (...) {
+        c->capturing = false;
         (void)hipStreamEndCapture(c->st, &g);
         if (g) (void)hipGraphDestroy(g);
         throw;
     }
+    c->capturing = false;
     HIP_OK(hipStreamEndCapture(c->st, &g));
     hipGraphExec_t ex = nullptr;
-    HIP_OK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-    HIP_OK(hipGraphDestroy(g));
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) throw Error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     return ex;
+}
+
+// Captures one graph of `steps_per_graph` steps.  If the cooperative rebuild launch cannot be
+// captured, the rebuild inside graphs falls back to its multi-launch form (same phases).
+hipGraphExec_t capture(dpmd_ctx* c, int steps_per_graph, bool fire, int pred, double a, double b) {
+    if (c->use_coop && c->coop_in_graph) {
+        try {
+            return capture_once(c, steps_per_graph, fire, pred, a, b);
+        } catch (const Error&) {
+            (void)hipGetLastError();
+            HIP_OK(hipStreamSynchronize(c->st));
+            c->coop_in_graph = false;
+        }
+    }
+    return capture_once(c, steps_per_graph, fire, pred, a, b);
 }
 
 }  // namespace
@@ -869,6 +888,68 @@ int dpmd_fill_f64(void* p, int64_t n, double v) {
 int dpmd_fill_i32(void* p, int64_t n, int32_t v) {
     return guard(nullptr, [&] { fill_i32(nullptr, (int*)p, n, v); HIP_OK(hipDeviceSynchronize()); });
 }
+int dpmd_set_option(dpmd_ctx* c, int key, int value) {
+    return guard(c, [&] {
+        switch (key) {
+            case DPMD_OPT_COOP_REBUILD: c->use_coop = value != 0; break;
+            case DPMD_OPT_COOP_IN_GRAPH: c->coop_in_graph = value != 0; break;
+            default: throw ArgError("unknown option key");
+        }
+    });
+}
+
+int dpmd_get_option(dpmd_ctx* c, int key, int* value) {
+    return guard(c, [&] {
+        switch (key) {
+            case DPMD_OPT_COOP_REBUILD: *value = c->use_coop ? 1 : 0; break;
+            case DPMD_OPT_COOP_IN_GRAPH: *value = c->coop_in_graph ? 1 : 0; break;
+            default: throw ArgError("unknown option key");
+        }
+    });
+}
+
+int dpmd_profile(dpmd_ctx* c, int enable) {
+    return guard(c, [&] {
+        HIP_OK(hipStreamSynchronize(c->st));
+        for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+        c->prof_ev.clear();
+        c->prof = enable != 0;
+    });
+}
+
+int dpmd_profile_read(dpmd_ctx* c, double* total_ms, int* count) {
+    return guard(c, [&] {
+        HIP_OK(hipStreamSynchronize(c->st));
+        double t = 0.0;
+        for (size_t k = 0; k + 1 < c->prof_ev.size(); k += 2) {
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, c->prof_ev[k], c->prof_ev[k + 1]));
+            t += ms;
+        }
+        if (total_ms) *total_ms = t;
+        if (count) *count = (int)(c->prof_ev.size() / 2);
+        for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+        c->prof_ev.clear();
+    });
+}
+
+int dpmd_stream_copy(double* dst, const double* src, int64_t n, int repeats, double* ms_per_copy) {
+    return guard(nullptr, [&] {
+        need(n > 0 && repeats > 0, "dpmd_stream_copy: n and repeats must be positive");
+        hipEvent_t a, b;
+        HIP_OK(hipEventCreate(&a)); HIP_OK(hipEventCreate(&b));
+        copy_f64(nullptr, dst, src, n);
+        HIP_OK(hipEventRecord(a, nullptr));
+        for (int k = 0; k < repeats; ++k) copy_f64(nullptr, dst, src, n);
+        HIP_OK(hipEventRecord(b, nullptr));
+        HIP_OK(hipEventSynchronize(b));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, a, b));
+        (void)hipEventDestroy(a); (void)hipEventDestroy(b);
+        if (ms_per_copy) *ms_per_copy = ms / repeats;
+    });
+}
+
 int dpmd_synchronize(dpmd_ctx* c) {
     return guard(c, [&] {
         HIP_OK(hipStreamSynchronize(c->st));

@@ -8,52 +8,34 @@
 //  * only systems whose rebuild flag is set are rebuilt (the reference rebuilds every
 //    system when any flag is set, disk.cu:1127-1138); the decision is taken on the device,
 //    so no host round trip happens per step;
+//  * the whole rebuild is ONE cooperative launch with grid-wide barriers between its
+//    phases; when no system is flagged every workgroup leaves at once, so the per-step
+//    check costs a single (empty) launch.  A multi-launch form of the same phases is kept
+//    for graph capture where cooperative launches cannot be captured;
 //  * in-cell order is deterministic (ascending current index), by an atomic scatter
 //    followed by a per-cell insertion sort; the reference's atomic scatter order is
 //    nondeterministic (base_particle_kernels.cu:246-259);
 //  * the neighbor list is a fixed-capacity column-major ELL array nbr[k*n+i] (coalesced
 //    for one-thread-per-object force kernels), with the reference's stencil order.
 #include "internal.hpp"
+#include <hip/hip_cooperative_groups.h>
+
+namespace cg = cooperative_groups;
 
 namespace dpmd {
 namespace {
 
-__global__ void k_set_flags(int S, unsigned* flag, const int* active) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S || !*active) return;
-    flag[s] = 1u;
-}
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = DPMD_BLOCK * SCAN_ITEMS;
 
-__global__ void k_clear_flags(int S, unsigned* flag, const int* active) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S || !*active) return;
-    flag[s] = 0u;
-}
-
-__global__ void k_cell_zero(int C, const int* __restrict__ cell_sys, const unsigned* __restrict__ flag,
-                            int* __restrict__ count, const int* active) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= C || !*active) return;
-    if (flag[cell_sys[c]]) count[c] = 0;
-}
-
-// assign_cell_ids_kernel (base_particle_kernels.cu:205-233) + count_cells_kernel (:235-244)
-__global__ void k_cell_hash_count(ObjArgs o, CellArgs ca, const int* active) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= o.n || !*active) return;
-    const int s = o.sid[i];
-    if (!ca.flag[s]) return;
-    const double u = wrap01(o.x[i] * ca.iLx[s]);
-    const double v = wrap01(o.y[i] * ca.iLy[s]);
-    const int nx = ca.cdx[s], ny = ca.cdy[s];
-    int cx = (int)floor(u * nx);
-    int cy = (int)floor(v * ny);
-    cx = cx < nx ? (cx > 0 ? cx : 0) : nx - 1;   // guard: u*nx may round up to nx
-    cy = cy < ny ? (cy > 0 ? cy : 0) : ny - 1;
-    const int cell = ca.cstart[s] + cx + cy * nx;
-    ca.cell_id[i] = cell;
-    atomicAdd(&ca.cell_count[cell], 1);
-}
+struct RebuildArgs {
+    ObjArgs o; CellArgs ca; PermArgs pa; NbrArgs nb;
+    int* order; int* order_inv; int* static_index;
+    double* lx; double* ly;           // disks: displacement reference, reset per object
+    // rigid bodies: per-particle reference = first vertex of each particle
+    int N; const int* psid; const int* poff; double* plx; double* ply;
+    int* tile_sum; int nt; int S;
+};
 
 __device__ __forceinline__ int block_excl_scan_int(int v, int* lds4, int& total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -72,115 +54,122 @@ __device__ __forceinline__ int block_excl_scan_int(int v, int* lds4, int& total)
     return pre + inc - v;
 }
 
-constexpr int SCAN_ITEMS = 8;
-constexpr int SCAN_TILE = DPMD_BLOCK * SCAN_ITEMS;
+// ---- per-element phase bodies -------------------------------------------------------
 
-__global__ void k_scan_tiles(int C, const int* __restrict__ count, int* __restrict__ tile_sum, const int* active) {
-    if (!*active) return;
-    __shared__ int lds[4];
-    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
-    int s = 0;
-#pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) s += (base + k < C) ? count[base + k] : 0;
-    int total;
-    (void)block_excl_scan_int(s, lds, total);
-    if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+__device__ __forceinline__ void ph_zero(const RebuildArgs& r, int c) {
+    if (r.ca.flag[r.ca.cell_sys[c]]) r.ca.cell_count[c] = 0;
 }
 
-__global__ void k_scan_top(int nt, int* __restrict__ tile_sum, const int* active) {
-    if (!*active) return;
-    __shared__ int lds[4];
+// assign_cell_ids_kernel (base_particle_kernels.cu:205-233) + count_cells_kernel (:235-244)
+__device__ __forceinline__ void ph_hash(const RebuildArgs& r, int i) {
+    const ObjArgs& o = r.o; const CellArgs& ca = r.ca;
+    const int s = o.sid[i];
+    if (!ca.flag[s]) return;
+    const double u = wrap01(o.x[i] * ca.iLx[s]);
+    const double v = wrap01(o.y[i] * ca.iLy[s]);
+    const int nx = ca.cdx[s], ny = ca.cdy[s];
+    int cx = (int)floor(u * nx);
+    int cy = (int)floor(v * ny);
+    cx = cx < nx ? (cx > 0 ? cx : 0) : nx - 1;   // guard: u*nx may round up to nx
+    cy = cy < ny ? (cy > 0 ? cy : 0) : ny - 1;
+    const int cell = ca.cstart[s] + cx + cy * nx;
+    ca.cell_id[i] = cell;
+    atomicAdd(&ca.cell_count[cell], 1);
+}
+
+// Block-level: sum of one tile of cell counts.
+__device__ __forceinline__ void ph_tile_sum(const RebuildArgs& r, int t, int* lds) {
+    const int C = r.ca.C;
+    const int base = t * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) s += (base + k < C) ? r.ca.cell_count[base + k] : 0;
+    int total;
+    (void)block_excl_scan_int(s, lds, total);
+    if (threadIdx.x == 0) r.tile_sum[t] = total;
+}
+
+// Block-level: exclusive scan of the tile sums (one block).
+__device__ __forceinline__ void ph_top_scan(const RebuildArgs& r, int* lds) {
     int carry = 0;
-    for (int b = 0; b < nt; b += DPMD_BLOCK) {
+    for (int b = 0; b < r.nt; b += DPMD_BLOCK) {
         const int t = b + threadIdx.x;
-        const int v = t < nt ? tile_sum[t] : 0;
+        const int v = t < r.nt ? r.tile_sum[t] : 0;
         int total;
         const int ex = block_excl_scan_int(v, lds, total);
-        if (t < nt) tile_sum[t] = carry + ex;
+        if (t < r.nt) r.tile_sum[t] = carry + ex;
         carry += total;
     }
 }
 
-__global__ void k_scan_down(int C, const int* __restrict__ count, const int* __restrict__ tile_sum,
-                            int* __restrict__ cell_start, int* __restrict__ wptr,
-                            const int* __restrict__ cell_sys, const unsigned* __restrict__ flag,
-                            const int* active) {
-    if (!*active) return;
-    __shared__ int lds[4];
-    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+// Block-level: cell_start for one tile; write pointers for flagged systems' cells.  Counts
+// of systems that are not rebuilt are unchanged since their last rebuild, so their starts
+// are recomputed to the same values.
+__device__ __forceinline__ void ph_tile_down(const RebuildArgs& r, int t, int* lds) {
+    const int C = r.ca.C;
+    const int base = t * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS];
     int s = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = (base + k < C) ? count[base + k] : 0; s += v[k]; }
+    for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = (base + k < C) ? r.ca.cell_count[base + k] : 0; s += v[k]; }
     int total;
-    int run = block_excl_scan_int(s, lds, total) + tile_sum[blockIdx.x];
+    int run = block_excl_scan_int(s, lds, total) + r.tile_sum[t];
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         const int c = base + k;
         if (c < C) {
-            cell_start[c] = run;
-            if (flag[cell_sys[c]]) wptr[c] = run;
-            if (c == C - 1) cell_start[C] = run + v[k];
+            r.ca.cell_start[c] = run;
+            if (r.ca.flag[r.ca.cell_sys[c]]) r.ca.cell_wptr[c] = run;
+            if (c == C - 1) r.ca.cell_start[C] = run + v[k];
         }
         run += v[k];
     }
 }
 
-__global__ void k_scatter(ObjArgs o, CellArgs ca, int* __restrict__ order, const int* active) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= o.n || !*active) return;
-    if (!ca.flag[o.sid[i]]) return;
-    const int slot = atomicAdd(&ca.cell_wptr[ca.cell_id[i]], 1);
-    order[slot] = i;
+__device__ __forceinline__ void ph_scatter(const RebuildArgs& r, int i) {
+    if (!r.ca.flag[r.o.sid[i]]) return;
+    const int slot = atomicAdd(&r.ca.cell_wptr[r.ca.cell_id[i]], 1);
+    r.order[slot] = i;
 }
 
-// Fix the in-cell order: ascending current index (stable sort by cell id).
-__global__ void k_cell_sort(CellArgs ca, int* __restrict__ order, const int* active) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ca.C || !*active) return;
-    if (!ca.flag[ca.cell_sys[c]]) return;
-    const int b = ca.cell_start[c], e = ca.cell_start[c + 1];
+// In-cell order: ascending current index (a stable sort by cell id).
+__device__ __forceinline__ void ph_sort(const RebuildArgs& r, int c) {
+    if (!r.ca.flag[r.ca.cell_sys[c]]) return;
+    const int b = r.ca.cell_start[c], e = r.ca.cell_start[c + 1];
+    int* ord = r.order;
     for (int k = b + 1; k < e; ++k) {
-        const int key = order[k];
+        const int key = ord[k];
         int m = k - 1;
-        while (m >= b && order[m] > key) { order[m + 1] = order[m]; --m; }
-        order[m + 1] = key;
+        while (m >= b && ord[m] > key) { ord[m + 1] = ord[m]; --m; }
+        ord[m + 1] = key;
     }
 }
 
 // reorder_particles_impl (disk.cu:1102-1125): gather into scratch by the new order.
-__global__ void k_gather(int n, const int* __restrict__ sid, const unsigned* __restrict__ flag,
-                         const int* __restrict__ order, int* __restrict__ order_inv, PermArgs pa,
-                         const int* active) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || !*active) return;
-    if (!flag[sid[p]]) return;
-    const int o = order[p];
-    for (int k = 0; k < pa.nd; ++k) pa.tmp_d[(size_t)k * n + p] = pa.d[k][o];
-    for (int k = 0; k < pa.ni; ++k) pa.tmp_i[(size_t)k * n + p] = pa.i[k][o];
-    order_inv[o] = p;
+__device__ __forceinline__ void ph_gather(const RebuildArgs& r, int p) {
+    if (!r.ca.flag[r.o.sid[p]]) return;
+    const int n = r.o.n;
+    const int o = r.order[p];
+    for (int k = 0; k < r.pa.nd; ++k) r.pa.tmp_d[(size_t)k * n + p] = r.pa.d[k][o];
+    for (int k = 0; k < r.pa.ni; ++k) r.pa.tmp_i[(size_t)k * n + p] = r.pa.i[k][o];
+    r.order_inv[o] = p;
 }
 
 // Copy back, update static_index (update_static_index_kernel, base_particle_kernels.cu:
 // 172-180) and reset the displacement reference (reset_displacements_impl, disk.cu:1097).
-__global__ void k_copy_back(int n, const int* __restrict__ sid, const unsigned* __restrict__ flag,
-                            const int* __restrict__ order_inv, int* __restrict__ static_index, PermArgs pa,
-                            double* __restrict__ lx, double* __restrict__ ly, const int* active) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || !*active) return;
-    if (!flag[sid[p]]) return;
-    for (int k = 0; k < pa.nd; ++k) pa.d[k][p] = pa.tmp_d[(size_t)k * n + p];
-    for (int k = 0; k < pa.ni; ++k) pa.i[k][p] = pa.tmp_i[(size_t)k * n + p];
-    static_index[p] = order_inv[static_index[p]];
-    if (lx) { lx[p] = pa.tmp_d[p]; ly[p] = pa.tmp_d[(size_t)n + p]; }
+__device__ __forceinline__ void ph_copy_back(const RebuildArgs& r, int p) {
+    if (!r.ca.flag[r.o.sid[p]]) return;
+    const int n = r.o.n;
+    for (int k = 0; k < r.pa.nd; ++k) r.pa.d[k][p] = r.pa.tmp_d[(size_t)k * n + p];
+    for (int k = 0; k < r.pa.ni; ++k) r.pa.i[k][p] = r.pa.tmp_i[(size_t)k * n + p];
+    r.static_index[p] = r.order_inv[r.static_index[p]];
+    if (r.lx) { r.lx[p] = r.pa.tmp_d[p]; r.ly[p] = r.pa.tmp_d[(size_t)n + p]; }
 }
 
 // count_cell_neighbors_kernel + fill_neighbors_cell_kernel (base_point_particle_kernels.cu:
 // 37-157) in one pass, writing a fixed-capacity ELL list.
-__global__ void __launch_bounds__(DPMD_BLOCK)
-k_neighbors(ObjArgs o, CellArgs ca, NbrArgs nb, const int* active) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= o.n || !*active) return;
+__device__ __forceinline__ void ph_neighbors(const RebuildArgs& r, int i) {
+    const ObjArgs& o = r.o; const CellArgs& ca = r.ca; const NbrArgs& nb = r.nb;
     const int s = o.sid[i];
     if (!ca.flag[s]) return;
     const int nx = ca.cdx[s], ny = ca.cdy[s], cs = ca.cstart[s];
@@ -215,6 +204,109 @@ k_neighbors(ObjArgs o, CellArgs ca, NbrArgs nb, const int* active) {
     nb.nn[i] = cnt < nb.K ? cnt : nb.K;
 }
 
+// reset_displacements_kernel for rigid bodies (rigid_bumpy.cu:1070-1085): the skin
+// reference point is the first vertex of each particle.
+__device__ __forceinline__ void ph_rigid_reset(const RebuildArgs& r, int i) {
+    if (!r.ca.flag[r.psid[i]]) return;
+    const int fv = r.static_index[r.poff[i]];
+    r.plx[i] = r.o.x[fv]; r.ply[i] = r.o.y[fv];
+}
+
+// ---- multi-launch form ----------------------------------------------------------------
+#define DPMD_EACH(name, count, body)                                                          \
+    __global__ void name(RebuildArgs r, const int* active) {                                  \
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;                                  \
+        if (i >= (count) || !*active) return;                                                 \
+        body(r, i);                                                                           \
+    }
+DPMD_EACH(k_zero, r.ca.C, ph_zero)
+DPMD_EACH(k_hash, r.o.n, ph_hash)
+DPMD_EACH(k_scatter, r.o.n, ph_scatter)
+DPMD_EACH(k_sort, r.ca.C, ph_sort)
+DPMD_EACH(k_gather, r.o.n, ph_gather)
+DPMD_EACH(k_copy_back, r.o.n, ph_copy_back)
+DPMD_EACH(k_rigid_reset, r.N, ph_rigid_reset)
+#undef DPMD_EACH
+
+__global__ void __launch_bounds__(DPMD_BLOCK) k_neighbors(RebuildArgs r, const int* active) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r.o.n || !*active) return;
+    ph_neighbors(r, i);
+}
+
+__global__ void k_tile_sum(RebuildArgs r, const int* active) {
+    __shared__ int lds[4];
+    if (!*active) return;
+    ph_tile_sum(r, blockIdx.x, lds);
+}
+__global__ void k_top_scan(RebuildArgs r, const int* active) {
+    __shared__ int lds[4];
+    if (!*active) return;
+    ph_top_scan(r, lds);
+}
+__global__ void k_tile_down(RebuildArgs r, const int* active) {
+    __shared__ int lds[4];
+    if (!*active) return;
+    ph_tile_down(r, blockIdx.x, lds);
+}
+
+__global__ void k_set_flags(int S, unsigned* flag, const int* active) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || !*active) return;
+    flag[s] = 1u;
+}
+
+__global__ void k_clear_flags(int S, unsigned* flag, const int* active) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || !*active) return;
+    flag[s] = 0u;
+}
+
+// ---- single cooperative launch -----------------------------------------------------------
+// Every workgroup runs every phase with a grid-stride loop; phases are separated by
+// grid-wide barriers.  All exits happen before the first barrier and depend only on values
+// that no workgroup writes during the launch (active, the rebuild flags), so every
+// workgroup takes the same path.
+__global__ void __launch_bounds__(DPMD_BLOCK) k_rebuild_coop(RebuildArgs r, const int* active) {
+    __shared__ int lds[4];
+    __shared__ int any;
+    if (!*active) return;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    for (int s = threadIdx.x; s < r.S; s += DPMD_BLOCK)
+        if (r.ca.flag[s]) any = 1;
+    __syncthreads();
+    if (!any) return;
+    cg::grid_group g = cg::this_grid();
+    const int gt = blockIdx.x * DPMD_BLOCK + threadIdx.x;
+    const int gs = gridDim.x * DPMD_BLOCK;
+    const int n = r.o.n, C = r.ca.C;
+    for (int c = gt; c < C; c += gs) ph_zero(r, c);
+    g.sync();
+    for (int i = gt; i < n; i += gs) ph_hash(r, i);
+    g.sync();
+    for (int t = blockIdx.x; t < r.nt; t += gridDim.x) ph_tile_sum(r, t, lds);
+    g.sync();
+    if (blockIdx.x == 0) ph_top_scan(r, lds);
+    g.sync();
+    for (int t = blockIdx.x; t < r.nt; t += gridDim.x) ph_tile_down(r, t, lds);
+    g.sync();
+    for (int i = gt; i < n; i += gs) ph_scatter(r, i);
+    g.sync();
+    for (int c = gt; c < C; c += gs) ph_sort(r, c);
+    g.sync();
+    for (int p = gt; p < n; p += gs) ph_gather(r, p);
+    g.sync();
+    for (int p = gt; p < n; p += gs) ph_copy_back(r, p);
+    g.sync();
+    for (int i = gt; i < n; i += gs) ph_neighbors(r, i);
+    if (r.plx)
+        for (int i = gt; i < r.N; i += gs) ph_rigid_reset(r, i);
+    g.sync();
+    if (blockIdx.x == 0)
+        for (int s = threadIdx.x; s < r.S; s += DPMD_BLOCK) r.ca.flag[s] = 0u;
+}
+
 // set_naive_neighbor_count + fill_naive_neighbor_list_kernel (base_point_particle_kernels.cu:
 // 6-35); vertex form excludes same-particle vertices (base_poly_particle_kernels.cu:18-55).
 __global__ void k_naive(ObjArgs o, NbrArgs nb) {
@@ -232,19 +324,6 @@ __global__ void k_naive(ObjArgs o, NbrArgs nb) {
     if (cnt > nb.K) atomicMax(nb.overflow, cnt);
     atomicMax(nb.max_count, cnt);
     nb.nn[i] = cnt < nb.K ? cnt : nb.K;
-}
-
-// reset_displacements_kernel for rigid bodies (rigid_bumpy.cu:1070-1085): the skin
-// reference point is the first vertex of each particle.
-__global__ void k_rigid_reset_disp(int N, const int* __restrict__ sid, const unsigned* __restrict__ flag,
-                                   const int* __restrict__ poff, const int* __restrict__ vstatic,
-                                   const double* __restrict__ vpx, const double* __restrict__ vpy,
-                                   double* __restrict__ lx, double* __restrict__ ly, const int* active) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N || !*active) return;
-    if (!flag[sid[i]]) return;
-    const int fv = vstatic[poff[i]];
-    lx[i] = vpx[fv]; ly[i] = vpy[fv];
 }
 
 ObjArgs obj_args(dpmd_ctx* c) {
@@ -291,6 +370,47 @@ PermArgs perm_args(dpmd_ctx* c) {
     return pa;
 }
 
+RebuildArgs rebuild_args(dpmd_ctx* c) {
+    RebuildArgs r;
+    r.o = obj_args(c); r.ca = cell_args(c); r.pa = perm_args(c); r.nb = nbr_args(c);
+    r.order = c->order.p; r.order_inv = c->order_inv.p; r.static_index = c->static_index.p;
+    const bool rigid = c->kind == DPMD_RIGID_BUMPY;
+    r.lx = rigid ? nullptr : c->lx.p; r.ly = rigid ? nullptr : c->ly.p;
+    r.N = c->N; r.psid = c->sid.p; r.poff = c->poff.p;
+    r.plx = rigid ? c->lx.p : nullptr; r.ply = rigid ? c->ly.p : nullptr;
+    r.tile_sum = c->scan_part.p; r.nt = (c->C + SCAN_TILE - 1) / SCAN_TILE; r.S = c->S;
+    return r;
+}
+
+void rebuild_multi(dpmd_ctx* c, const RebuildArgs& r, const int* active) {
+    hipStream_t st = c->st;
+    const int n = r.o.n, C = r.ca.C;
+    k_zero<<<blocks_for(C), DPMD_BLOCK, 0, st>>>(r, active);
+    k_hash<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
+    k_tile_sum<<<r.nt, DPMD_BLOCK, 0, st>>>(r, active);
+    k_top_scan<<<1, DPMD_BLOCK, 0, st>>>(r, active);
+    k_tile_down<<<r.nt, DPMD_BLOCK, 0, st>>>(r, active);
+    k_scatter<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
+    k_sort<<<blocks_for(C), DPMD_BLOCK, 0, st>>>(r, active);
+    k_gather<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
+    k_copy_back<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
+    k_neighbors<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
+    if (r.plx) k_rigid_reset<<<blocks_for(r.N), DPMD_BLOCK, 0, st>>>(r, active);
+    k_clear_flags<<<blocks_for(r.S), DPMD_BLOCK, 0, st>>>(r.S, r.ca.flag, active);
+}
+
+int coop_grid(dpmd_ctx* c) {
+    if (c->coop_blocks <= 0) {
+        int dev = 0, ncu = 0, per_cu = 0;
+        HIP_OK(hipGetDevice(&dev));
+        HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_rebuild_coop, DPMD_BLOCK, 0));
+        c->coop_blocks = std::max(1, ncu * std::max(1, per_cu));
+    }
+    const int need = std::max(blocks_for(c->n_obj()), blocks_for(c->C));
+    return std::max(1, std::min(c->coop_blocks, need));
+}
+
 }  // namespace
 
 void build_chunks(dpmd_ctx* c) {
@@ -309,34 +429,18 @@ void build_chunks(dpmd_ctx* c) {
 }
 
 void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all) {
-    const int n = c->n_obj();
     const int S = c->S;
     hipStream_t st = c->st;
     if (force_all) k_set_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, c->flag.p, active);
-    ObjArgs o = obj_args(c);
-    CellArgs ca = cell_args(c);
-    NbrArgs nb = nbr_args(c);
-    PermArgs pa = perm_args(c);
-    const int C = c->C;
-    const int nt = (C + SCAN_TILE - 1) / SCAN_TILE;
-    k_cell_zero<<<blocks_for(C), DPMD_BLOCK, 0, st>>>(C, c->cell_sys.p, c->flag.p, c->cell_count.p, active);
-    k_cell_hash_count<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(o, ca, active);
-    k_scan_tiles<<<nt, DPMD_BLOCK, 0, st>>>(C, c->cell_count.p, c->scan_part.p, active);
-    k_scan_top<<<1, DPMD_BLOCK, 0, st>>>(nt, c->scan_part.p, active);
-    k_scan_down<<<nt, DPMD_BLOCK, 0, st>>>(C, c->cell_count.p, c->scan_part.p, c->cell_start.p,
-                                          c->cell_wptr.p, c->cell_sys.p, c->flag.p, active);
-    k_scatter<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(o, ca, c->order.p, active);
-    k_cell_sort<<<blocks_for(C), DPMD_BLOCK, 0, st>>>(ca, c->order.p, active);
-    k_gather<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(n, o.sid, c->flag.p, c->order.p, c->order_inv.p, pa, active);
-    const bool rigid = c->kind == DPMD_RIGID_BUMPY;
-    k_copy_back<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(n, o.sid, c->flag.p, c->order_inv.p, c->static_index.p, pa,
-                                                     rigid ? nullptr : c->lx.p, rigid ? nullptr : c->ly.p, active);
-    k_neighbors<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(obj_args(c), ca, nb, active);
-    if (rigid)
-        k_rigid_reset_disp<<<blocks_for(c->N), DPMD_BLOCK, 0, st>>>(c->N, c->sid.p, c->flag.p, c->poff.p,
-                                                                    c->static_index.p, c->vpx.p, c->vpy.p,
-                                                                    c->lx.p, c->ly.p, active);
-    k_clear_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, c->flag.p, active);
+    RebuildArgs r = rebuild_args(c);
+    const bool coop = c->use_coop && !(c->capturing && !c->coop_in_graph);
+    if (coop) {
+        void* args[] = {(void*)&r, (void*)&active};
+        HIP_OK(hipLaunchCooperativeKernel((const void*)k_rebuild_coop, dim3(coop_grid(c)), dim3(DPMD_BLOCK), args, 0,
+                                          st));
+    } else {
+        rebuild_multi(c, r, active);
+    }
     HIP_OK(hipGetLastError());
 }
 
@@ -356,8 +460,8 @@ void check_overflow(dpmd_ctx* c) {
     HIP_OK(hipStreamSynchronize(c->st));
     if (ov > 0)
         throw OverflowError("neighbor list overflow: an object has " + std::to_string(ov) +
-                    " neighbors, capacity is " + std::to_string(c->K) +
-                    " (raise it with dpmd_set_neighbor_capacity)");
+                            " neighbors, capacity is " + std::to_string(c->K) +
+                            " (raise it with dpmd_set_neighbor_capacity)");
 }
 
 }  // namespace dpmd

@@ -309,10 +309,8 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_fire_drift(DiskArgs a, FireArgs 
 
 template <int MODE>
 __global__ void __launch_bounds__(DPMD_BLOCK)
-k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, int do_conv, int do_update, int pred, double pa,
-             double pb, const int* active) {
+k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, const int* active) {
     __shared__ double lds[4];
-    __shared__ int last;
     if (!*active) return;
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     const int s = r.chunk_sys[c];
@@ -339,28 +337,31 @@ k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, int do_conv, int do_up
     vf2 = block_max256(vf2, lds);
     if (threadIdx.x == 0) {
         r.part[c] = vp; r.part[r.nch + c] = vpe; r.part[2 * r.nch + c] = vf2;
-        __threadfence();
-        const int cb = r.sys_chunk[s], ce = r.sys_chunk[s + 1];
-        const unsigned prev = atomicAdd(&r.counter[s], 1u);
-        last = (prev == (unsigned)(ce - cb - 1));
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+}
+
+// One workgroup per system folds the chunk partials in a fixed order (strided sequential
+// accumulators, then the block tree), applies the stopping test and evaluates the next
+// step's FIRE update.  A separate launch orders it after every chunk without fences.
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_update, int pred, double pa, double pb,
+                const int* active) {
+    __shared__ double lds[4];
+    if (!*active) return;
+    const int s = blockIdx.x;
     const int cb = r.sys_chunk[s], ce = r.sys_chunk[s + 1];
     double ap = 0.0, ape = 0.0, af = 0.0;
     for (int k = cb + threadIdx.x; k < ce; k += DPMD_BLOCK) {
-        ap = ap + load_agent(&r.part[k]);
-        ape = ape + load_agent(&r.part[r.nch + k]);
-        af = fmax(af, load_agent(&r.part[2 * r.nch + k]));
+        ap = ap + r.part[k];
+        ape = ape + r.part[r.nch + k];
+        af = fmax(af, r.part[2 * r.nch + k]);
     }
     ap = block_sum256(ap, lds);
     ape = block_sum256(ape, lds);
     af = block_max256(af, lds);
     if (threadIdx.x == 0) {
-        r.counter[s] = 0u;
         r.pw_tot[s] = ap; r.pe_tot[s] = ape; r.maxf2[s] = af;
-        if (do_conv) fire_convergence_sys(f, a.off, s, ape, af, pred, pa, pb);
+        if (do_conv) fire_convergence_sys(f, off, s, ape, af, pred, pa, pb);
         if (do_conv) fire_count_stopped(f, s);
         if (do_update) fire_update_sys(f, s, ap);
     }
@@ -467,10 +468,11 @@ void disk_fire_force(dpmd_ctx* c, const int* active, int kick, int do_conv, int 
     DiskArgs a = disk_args(c);
     FireArgs f = fire_args(c);
     RedArgs r = red_args(c);
+    ProfScope prof(c);
     by_mode(c->fire_mode, [&](auto m) {
-        k_fire_force<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(a, f, r, kick, do_conv, 1, pred,
-                                                                               pa, pb, active);
+        k_fire_force<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(a, f, r, kick, active);
     });
+    k_fire_finalize<<<c->S, DPMD_BLOCK, 0, c->st>>>(c->off.p, f, r, do_conv, 1, pred, pa, pb, active);
     HIP_OK(hipGetLastError());
 }
 
@@ -481,6 +483,7 @@ void disk_vv_drift(dpmd_ctx* c, const int* active) {
 
 void disk_vv_force(dpmd_ctx* c, const int* active) {
     DiskArgs a = disk_args(c);
+    ProfScope prof(c);
     by_mode(c->vv_mode, [&](auto m) {
         k_vv_force<decltype(m)::value><<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(a, c->vv_dth.p, active);
     });

@@ -101,6 +101,12 @@ struct dpmd_ctx {
     int g_fire_key[4] = {0, 0, 0, 0};
     double g_fire_keyd[2] = {0, 0};
     int g_vv_key = -1;
+    // rebuild as one cooperative launch (use_coop); whether that launch may be captured
+    bool use_coop = true, coop_in_graph = true;
+    int coop_blocks = 0;
+    // kernel timing (force kernels; outside graph capture only)
+    bool prof = false, capturing = false;
+    std::vector<hipEvent_t> prof_ev;   // (begin, end) pairs
 
     int n_obj() const { return kind == DPMD_RIGID_BUMPY ? Nv : N; }
 };
@@ -149,5 +155,21 @@ void rand_normal_2d(hipStream_t st, double* ox, double* oy, int n, uint64_t seed
                     double mx, double sx, double my, double sy);
 void fill_f64(hipStream_t st, double* p, int64_t n, double v);
 void fill_i32(hipStream_t st, int* p, int64_t n, int v);
+void copy_f64(hipStream_t st, double* dst, const double* src, int64_t n);
+
+// Records a (begin, end) event pair around the enclosed launch when timing is on.
+struct ProfScope {
+    dpmd_ctx* c; hipEvent_t b = nullptr;
+    explicit ProfScope(dpmd_ctx* cc) : c(cc) {
+        if (c->prof && !c->capturing) { HIP_OK(hipEventCreate(&b)); HIP_OK(hipEventRecord(b, c->st)); }
+    }
+    ~ProfScope() {
+        if (!b) return;
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) == hipSuccess && hipEventRecord(e, c->st) == hipSuccess) {
+            c->prof_ev.push_back(b); c->prof_ev.push_back(e);
+        }
+    }
+};
 
 }  // namespace dpmd

@@ -67,7 +67,18 @@ __global__ void k_fill_i32(int* p, int64_t n, int v) {
     if (i < n) p[i] = v;
 }
 
+__global__ void k_copy_f64(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 }  // namespace
+
+void copy_f64(hipStream_t st, double* dst, const double* src, int64_t n) {
+    if (n <= 0) return;
+    k_copy_f64<<<(unsigned)((n + DPMD_BLOCK - 1) / DPMD_BLOCK), DPMD_BLOCK, 0, st>>>(dst, src, n);
+    HIP_OK(hipGetLastError());
+}
 
 void rand_uniform_2d(hipStream_t st, double* ox, double* oy, int n, uint64_t seed, uint64_t call, double lox,
                      double hix, double loy, double hiy) {

@@ -189,6 +189,25 @@ int dpmd_memcpy(void* dst, const void* src, int64_t bytes, int kind);
 int dpmd_fill_f64(void* device_ptr, int64_t n, double value);
 int dpmd_fill_i32(void* device_ptr, int64_t n, int32_t value);
 int dpmd_synchronize(dpmd_ctx* ctx);
+/* ---- instrumentation (no reference counterpart: the reference times with std::chrono,
+ * tests/test_disk_system_cell_parallelism.cu:92-102) -------------------------------------
+ * dpmd_profile(ctx, 1) starts recording a HIP event pair around every force-kernel launch
+ * issued outside graph capture; dpmd_profile_read returns their summed duration and count. */
+int dpmd_profile(dpmd_ctx* ctx, int enable);
+int dpmd_profile_read(dpmd_ctx* ctx, double* total_ms, int* count);
+/* A plain device copy kernel of n doubles (dst <- src), timed over `repeats` launches on the
+ * default stream: the measured copy ceiling, and the calibration kernel for HBM counters. */
+int dpmd_stream_copy(double* dst, const double* src, int64_t n, int repeats, double* ms_per_copy);
+
+/* Execution options (no reference counterpart).
+ *   DPMD_OPT_COOP_REBUILD: 1 (default) = the cell/neighbor rebuild is one cooperative launch
+ *     that exits at once when no system is flagged; 0 = one launch per rebuild phase.
+ *   DPMD_OPT_COOP_IN_GRAPH: 1 while cooperative launches may be captured into hipGraphs;
+ *     the library clears it by itself if capture of that launch fails. */
+enum dpmd_option { DPMD_OPT_COOP_REBUILD = 0, DPMD_OPT_COOP_IN_GRAPH = 1 };
+int dpmd_set_option(dpmd_ctx* ctx, int key, int value);
+int dpmd_get_option(dpmd_ctx* ctx, int key, int* value);
+
 /* Library version string. */
 const char* dpmd_version(void);
 
