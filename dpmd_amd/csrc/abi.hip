@@ -141,9 +141,13 @@ const int* active_one(dpmd_ctx* c) { return c->one.p; }
 
 bool rigid(dpmd_ctx* c) { return c->kind == DPMD_RIGID_BUMPY; }
 
-// Cell layout (init_cell_sizes, base_particle.hpp:552-592).  A zero cell dimension is
-// derived from the box: the most cells whose side still covers 2*r_max + cutoff.
-void setup_cells(dpmd_ctx* c, bool validate) {
+// Cell layout (init_cell_sizes, base_particle.hpp:552-592).
+// Grid rule (the oracle applies the same one): a zero dimension is derived as the most cells
+// whose side covers 2*r_max + cutoff; after a box change (CELLS_BOX_CHANGED) a grid whose
+// cells became smaller than that is re-derived too.  A user grid is kept at set-up, as the
+// reference does (its own one-particle-per-cell test uses cells smaller than the cutoff).
+enum { CELLS_SYNC = 0, CELLS_INIT = 1, CELLS_BOX_CHANGED = 2 };
+void setup_cells(dpmd_ctx* c, int mode) {
     const int S = c->S;
     auto cdx = dl(c, c->cdx), cdy = dl(c, c->cdy);
     auto Lx = dl(c, c->Lx), Ly = dl(c, c->Ly), cut = dl(c, c->cutoff);
@@ -154,21 +158,20 @@ void setup_cells(dpmd_ctx* c, bool validate) {
         double rmax = 0.0;
         for (int i = off[s]; i < off[s + 1]; ++i) rmax = std::max(rmax, rad[i]);
         const double need_sz = 2.0 * rmax + cut[s];
-        if (cdx[s] <= 0 || cdy[s] <= 0 || (validate && (Lx[s] / cdx[s] < need_sz || Ly[s] / cdy[s] < need_sz))) {
+        const bool too_small = Lx[s] / cdx[s] < need_sz || Ly[s] / cdy[s] < need_sz;
+        if (cdx[s] <= 0 || cdy[s] <= 0 || (mode == CELLS_BOX_CHANGED && too_small)) {
             const int nx = (int)std::floor(Lx[s] / need_sz), ny = (int)std::floor(Ly[s] / need_sz);
-            if (validate || cdx[s] <= 0 || cdy[s] <= 0) {
-                cdx[s] = std::max(3, nx); cdy[s] = std::max(3, ny);
-                changed = true;
-            }
+            if (mode != CELLS_SYNC)
+                need(nx >= 3 && ny >= 3, "system " + std::to_string(s) +
+                                             ": the box is too small for a 3x3 grid of cells covering "
+                                             "2*max radius + neighbor cutoff (use the naive method)");
+            cdx[s] = std::max(3, nx); cdy[s] = std::max(3, ny);
+            changed = true;
         }
-        if (validate) {
+        if (mode != CELLS_SYNC)
             need(cdx[s] >= 3 && cdy[s] >= 3,
                  "cell grid must be at least 3x3 per system (system " + std::to_string(s) +
                      "): the periodic 3x3 stencil would visit a cell twice");
-            need(Lx[s] / cdx[s] >= need_sz && Ly[s] / cdy[s] >= need_sz,
-                 "cell side must cover 2*max radius + neighbor cutoff (system " + std::to_string(s) +
-                     "); the box is too small for a 3x3 grid of such cells");
-        }
     }
     if (changed) {
         c->cdx.upload(cdx.data(), S, c->st);
@@ -239,7 +242,7 @@ void sync_impl(dpmd_ctx* c) {
     c->ndof.upload(ndof.data(), S, c->st);
     box_inv(c);
     build_chunks(c);
-    if (c->method == DPMD_NEIGHBOR_CELL) setup_cells(c, false);
+    if (c->method == DPMD_NEIGHBOR_CELL) setup_cells(c, CELLS_SYNC);
     alloc_obj_buffers(c);
     c->synced = true;
     c->nbr_ready = false;
@@ -271,7 +274,7 @@ void init_neighbors_impl(dpmd_ctx* c) {
         alloc_obj_buffers(c);
         naive_neighbors(c);
     } else {
-        setup_cells(c, true);
+        setup_cells(c, CELLS_INIT);
         alloc_obj_buffers(c);
         std::vector<int> iota(n);
         std::iota(iota.begin(), iota.end(), 0);
@@ -685,7 +688,7 @@ int dpmd_increment_packing_fraction(dpmd_ctx* c, const double* inc) {
         scale_positions_impl(c, c->scr_c.p);
         box_inv(c);
         if (c->method == DPMD_NEIGHBOR_CELL) {
-            setup_cells(c, true);
+            setup_cells(c, CELLS_BOX_CHANGED);
             alloc_obj_buffers(c);
             // the cell grid may have changed: every system is rebuilt below
         }

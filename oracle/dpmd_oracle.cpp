@@ -534,14 +534,18 @@ struct Disk {
         regrid_if_needed();
         check_neighbors(true);
     }
-    void regrid_if_needed() {
+    // Cell grid rule shared with the library: a zero dimension is derived (the most cells
+    // whose side covers 2*r_max + cutoff, at least 3); after a box change (box_changed) a
+    // grid whose cells became smaller than that is re-derived too.  User grids are kept at
+    // set-up (the reference's KATs use cells smaller than the cutoff on purpose).
+    void regrid_if_needed(bool box_changed = true) {
         if (method != 1) return;
         bool changed = false;
         for (int s = 0; s < S; ++s) {
             double rmax = 0.0;
             for (int i = off[s]; i < off[s + 1]; ++i) rmax = std::max(rmax, rad[i]);
             const double need = 2.0 * rmax + cutoff[s];
-            if (nx[s] <= 0 || ny[s] <= 0 || Lx[s] / nx[s] < need || Ly[s] / ny[s] < need) {
+            if (nx[s] <= 0 || ny[s] <= 0 || (box_changed && (Lx[s] / nx[s] < need || Ly[s] / ny[s] < need))) {
                 nx[s] = std::max(3, (int)std::floor(Lx[s] / need));
                 ny[s] = std::max(3, (int)std::floor(Ly[s] / need));
                 changed = true;
@@ -629,7 +633,7 @@ void* or_disk_create(int S, int N, int K, int method, const int* off, const doub
     d->static_index.resize(N);
     for (int i = 0; i < N; ++i) d->static_index[i] = i;
     d->sync_box();
-    d->regrid_if_needed();
+    d->regrid_if_needed(false);
     d->sync_cells();
     d->init_neighbors();
     return d;
@@ -654,6 +658,25 @@ void or_disk_get_cells(void* h, int* nx, int* ny) {
     for (int s = 0; s < d->S; ++s) { nx[s] = d->nx[s]; ny[s] = d->ny[s]; }
 }
 int or_disk_max_count(void* h) { return ((Disk*)h)->max_count; }
+void or_disk_get_cell_ids(void* h, int* out) { Disk* d = (Disk*)h; std::memcpy(out, d->cell_id.data(), d->N * 4); }
+void or_disk_get_cell_system_start(void* h, int* out) { Disk* d = (Disk*)h; for (int s = 0; s <= d->S; ++s) out[s] = d->cstart[s]; }
+
+// init_cell_sizes_kernel (base_particle_kernels.cu:68-99) + the exclusive scan of cells per
+// system (base_particle.hpp:583-591).
+void or_cell_layout(int S, const double* Lx, const double* Ly, const int* nx, const int* ny, double* csx,
+                    double* csy, double* invx, double* invy, int* start) {
+    start[0] = 0;
+    for (int s = 0; s < S; ++s) {
+        csx[s] = nx[s] > 0 ? Lx[s] / nx[s] : 0.0;
+        csy[s] = ny[s] > 0 ? Ly[s] / ny[s] : 0.0;
+        invx[s] = csx[s] > 0.0 ? 1.0 / csx[s] : 0.0;
+        invy[s] = csy[s] > 0.0 ? 1.0 / csy[s] : 0.0;
+        start[s + 1] = start[s] + nx[s] * ny[s];
+    }
+}
+void or_box_inv(int S, const double* Lx, const double* Ly, double* ix, double* iy) {   // kernels.cu:50-65
+    for (int s = 0; s < S; ++s) { ix[s] = Lx[s] > 0.0 ? 1.0 / Lx[s] : 0.0; iy[s] = Ly[s] > 0.0 ? 1.0 / Ly[s] : 0.0; }
+}
 int or_disk_n_neighbors(void* h) { Disk* d = (Disk*)h; long t = 0; for (auto& L : d->nbr) t += L.size(); return (int)t; }
 // CSR export: start[N+1], ids[total]
 void or_disk_get_neighbors(void* h, int* start, int* ids) {

@@ -61,7 +61,28 @@ _PROTOS = {
     "or_disk_vv_init": (None, [vp, vp, i32]),
     "or_disk_vv_run": (None, [vp, i32, i32]),
     "or_disk_brute_neighbors": (i32, [vp, i32, vp]),
+    "or_disk_get_cell_ids": (None, [vp, vp]),
+    "or_disk_get_cell_system_start": (None, [vp, vp]),
+    "or_cell_layout": (None, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "or_box_inv": (None, [i32, vp, vp, vp, vp]),
 }
+
+
+def cell_layout(Lx, Ly, nx, ny):
+    Lx, Ly = np.ascontiguousarray(Lx, np.float64), np.ascontiguousarray(Ly, np.float64)
+    nx, ny = np.ascontiguousarray(nx, np.int32), np.ascontiguousarray(ny, np.int32)
+    S = Lx.size
+    o = [np.empty(S) for _ in range(4)]
+    start = np.empty(S + 1, np.int32)
+    lib().or_cell_layout(S, _p(Lx), _p(Ly), _p(nx), _p(ny), *[_p(a) for a in o], _p(start))
+    return (*o, start)
+
+
+def box_inv(Lx, Ly):
+    Lx, Ly = np.ascontiguousarray(Lx, np.float64), np.ascontiguousarray(Ly, np.float64)
+    ix, iy = np.empty(Lx.size), np.empty(Lx.size)
+    lib().or_box_inv(Lx.size, _p(Lx), _p(Ly), _p(ix), _p(iy))
+    return ix, iy
 
 FIELDS = dict(x=0, y=1, vx=2, vy=3, fx=4, fy=5, pe=6, rad=7, mass=8, area=9, lastx=10, lasty=11, Lx=12, Ly=13)
 
@@ -160,6 +181,16 @@ class OracleDisk:
 
     def max_count(self):
         return lib().or_disk_max_count(self.h)
+
+    def cell_ids(self):
+        out = np.empty(self.N, np.int32)
+        lib().or_disk_get_cell_ids(self.h, _p(out))
+        return out
+
+    def cell_system_start(self):
+        out = np.empty(self.S + 1, np.int32)
+        lib().or_disk_get_cell_system_start(self.h, _p(out))
+        return out
 
     def brute(self, i):
         out = np.empty(self.N, np.int32)

@@ -1,0 +1,357 @@
+"""Pins the oracle (CPU, no GPU): golden vectors, reference unit-test asserts as known-answer
+tests, closed forms, brute force and invariants.  Each check is chosen so that a plausible
+mistake in the restatement (dropped term, wrong sign or index, transposed operand) fails it.
+"""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from dpmd_amd import workloads as W
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+# ---------------------------------------------------------------------- Philox (RNG row)
+def test_philox_random123_kat():
+    for v in json.loads((GOLD / "philox_kat.json").read_text()):
+        assert list(orc.philox(v["ctr"], v["key"])) == v["out"]
+
+
+def test_numpy_philox_equals_oracle_and_kat():
+    for v in json.loads((GOLD / "philox_kat.json").read_text()):
+        out = W.philox4x32_10(*v["ctr"], *v["key"])
+        assert [int(o) for o in out] == v["out"]
+    ux, uy = W.uniform_2d(4096, 0xDEADBEEF12345, 3)
+    ox, oy = orc.rand_uniform_2d(4096, 0xDEADBEEF12345, 3)
+    assert np.array_equal(ux, ox) and np.array_equal(uy, oy)
+    # device_fields.cuh test ("explicit counter == pseudo counter"): same call, same draws;
+    # different calls differ
+    ax, _ = orc.rand_uniform_2d(1000, 5555, 0)
+    bx, _ = orc.rand_uniform_2d(1000, 5555, 0)
+    cx, _ = orc.rand_uniform_2d(1000, 5555, 1)
+    assert np.array_equal(ax, bx) and not np.array_equal(ax, cx)
+    assert 0.0 <= ux.min() and ux.max() < 1.0 and abs(ux.mean() - 0.5) < 0.02
+
+
+# --------------------------------------------------------------- cells (reference KATs)
+def test_cell_sizing_kat():
+    k = json.loads((GOLD / "cell_sizing_kat.json").read_text())
+    csx, csy, ix, iy, start = orc.cell_layout(k["box_x"], k["box_y"], k["dim_x"], k["dim_y"])
+    np.testing.assert_allclose(csx, k["cell_size_x"], rtol=1e-15)
+    np.testing.assert_allclose(csy, k["cell_size_y"], rtol=1e-15)
+    np.testing.assert_allclose(ix, k["cell_inv_x"], rtol=1e-15)
+    np.testing.assert_allclose(iy, k["cell_inv_y"], rtol=1e-15)
+    assert list(start) == k["cell_system_start"]
+    bix, biy = orc.box_inv(k["box_x"], k["box_y"])
+    assert list(bix) == [1.0 / 10.0, 1.0 / 20.0] and list(biy) == [1.0 / 5.0, 1.0 / 5.0]
+    zx, _ = orc.box_inv([0.0], [1.0])
+    assert zx[0] == 0.0
+
+
+def grid_one_per_cell(n=6, cs=2.0):
+    """One particle at the centre of every cell of an n x n grid (test_base_point_particle.cu:147-178),
+    listed in reverse so the rebuild must reorder."""
+    L = n * cs
+    xs, ys = [], []
+    for cy in range(n):
+        for cx in range(n):
+            xs.append((cx + 0.5) * cs); ys.append((cy + 0.5) * cs)
+    x, y = np.array(xs[::-1]), np.array(ys[::-1])
+    N = n * n
+    # r_i + r_j + cutoff must cover the diagonal (sqrt(2)*cs) but not 2*cs
+    rad = np.full(N, 0.5)
+    b = W.DiskBatch(off=np.array([0, N], np.int32), rad=rad, mass=np.ones(N), area=np.pi * rad ** 2,
+                    Lx=np.array([L]), Ly=np.array([L]), e=np.array([1.0]), cutoff=np.array([1.9]),
+                    thresh2=np.array([0.01]), boundary=np.array([0], np.int32), x=x, y=y,
+                    cdx=np.array([n], np.int32), cdy=np.array([n], np.int32))
+    return b
+
+
+def test_one_particle_per_cell_kat():
+    b = grid_one_per_cell()
+    d = orc.OracleDisk(b)
+    N = b.N
+    cid = d.cell_ids()
+    assert list(cid) == list(range(N))                 # sorted, one per cell
+    st, ids = d.neighbors_csr()
+    assert np.all(np.diff(st) == 8)                    # 3x3 stencil, 8 neighbors each
+    si = d.static_index()
+    assert sorted(si) == list(range(N))                # a permutation
+    x = d.get("x")
+    assert np.array_equal(x[si], b.x)                  # canonical -> current is consistent
+    assert list(d.cell_system_start()) == [0, N]
+
+
+def test_naive_counts():
+    b = W.disk_batch([7, 12, 5], [0.5, 0.5, 0.5], [0, 0, 0], seed=2)
+    d = orc.OracleDisk(b, method=0)
+    st, ids = d.neighbors_csr()
+    for s in range(3):
+        for i in range(b.off[s], b.off[s + 1]):
+            L = ids[st[i]:st[i + 1]]
+            assert len(L) == b.off[s + 1] - b.off[s] - 1
+            assert i not in L and len(set(L)) == len(L)
+            assert all(b.off[s] <= j < b.off[s + 1] for j in L)
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_cell_list_equals_brute_force(boundary):
+    b = W.disk_batch([300, 517, 64], [0.84, 0.86, 0.80], [boundary] * 3, seed=5)
+    d = orc.OracleDisk(b)
+    st, ids = d.neighbors_csr()
+    for i in range(b.N):
+        assert sorted(ids[st[i]:st[i + 1]]) == sorted(d.brute(i))
+        assert len(set(ids[st[i]:st[i + 1]])) == st[i + 1] - st[i]
+
+
+def test_hash_guard_near_box_edge():
+    """A coordinate just below L must land in the last cell (u*n may round up to n)."""
+    L = 30.0
+    x = np.array([np.nextafter(L, 0.0), 0.0, 15.0, np.nextafter(L, 0.0)])
+    y = np.array([0.1, np.nextafter(L, 0.0), 15.0, np.nextafter(L, 0.0)])
+    rad = np.full(4, 0.5)
+    b = W.DiskBatch(off=np.array([0, 4], np.int32), rad=rad, mass=np.ones(4), area=np.pi * rad ** 2,
+                    Lx=np.array([L]), Ly=np.array([L]), e=np.array([1.0]), cutoff=np.array([0.3]),
+                    thresh2=np.array([0.0225]), boundary=np.array([0], np.int32), x=x, y=y,
+                    cdx=np.array([7], np.int32), cdy=np.array([7], np.int32))
+    d = orc.OracleDisk(b)
+    st, ids = d.neighbors_csr()
+    for i in range(4):
+        assert sorted(ids[st[i]:st[i + 1]]) == sorted(d.brute(i))
+    assert d.cell_ids().max() < 49
+
+
+# ------------------------------------------------------------- forces: closed forms
+def pair_batch(x, y, rad, L=20.0, boundary=0, k=1.0):
+    n = len(x)
+    rad = np.asarray(rad, float)
+    return W.DiskBatch(off=np.array([0, n], np.int32), rad=rad, mass=np.ones(n), area=np.pi * rad ** 2,
+                       Lx=np.array([L]), Ly=np.array([L]), e=np.array([k]), cutoff=np.array([0.3]),
+                       thresh2=np.array([0.0225]), boundary=np.array([boundary], np.int32),
+                       x=np.asarray(x, float), y=np.asarray(y, float), cdx=np.array([0], np.int32),
+                       cdy=np.array([0], np.int32))
+
+
+def canon(d, name):
+    v = d.get(name)
+    return v[d.static_index()]
+
+
+@pytest.mark.parametrize("across", [False, True])
+def test_two_disk_closed_form(across):
+    r1, r2, k, gap = 0.5, 0.7, 2.5, 1.05
+    L = 20.0
+    if across:   # the pair straddles the periodic boundary: min image must be used
+        x = [0.2, 0.2 - gap + L]
+    else:
+        x = [5.0, 5.0 - gap]
+    d = orc.OracleDisk(pair_batch(x, [3.0, 3.0], [r1, r2], L=L, k=k))
+    d.compute_forces(0)
+    delta = r1 + r2 - gap
+    fx, fy, pe = canon(d, "fx"), canon(d, "fy"), canon(d, "pe")
+    # disk 0 sits to the right of disk 1: pushed in +x
+    np.testing.assert_allclose(fx, [k * delta, -k * delta], rtol=1e-13)
+    np.testing.assert_allclose(fy, [0.0, 0.0], atol=1e-15)
+    np.testing.assert_allclose(pe, [0.25 * k * delta ** 2] * 2, rtol=1e-13)
+    np.testing.assert_allclose(d.pe_total(), [0.5 * k * delta ** 2], rtol=1e-13)
+    # pressure: P = sum over directed pairs of (r_ij . f_ij)/2 / (2A) = gap*k*delta / (2A)
+    np.testing.assert_allclose(d.pressure(), [gap * k * delta / (2 * L * L)], rtol=1e-12)
+
+
+def test_three_disk_superposition():
+    r = 0.5
+    x = [5.0, 5.9, 6.8]     # 0-1 and 1-2 overlap by 0.1, 0-2 do not touch
+    d = orc.OracleDisk(pair_batch(x, [4.0] * 3, [r] * 3))
+    d.compute_forces(0)
+    fx = canon(d, "fx")
+    np.testing.assert_allclose(fx, [-0.1, 0.0, 0.1], atol=1e-13)
+    np.testing.assert_allclose(canon(d, "pe"), [0.0025, 0.005, 0.0025], rtol=1e-10)
+
+
+def test_wall_force_closed_form_and_repulsive_pair():
+    """Walls: delta = r - x (fire.cu semantics at disk.cu:185-208); pair term repulsive
+    (documented deviation from disk.cu:218's xj - xi)."""
+    L, k = 10.0, 1.0
+    d = orc.OracleDisk(pair_batch([0.3, 5.0], [5.0, 9.8], [0.5, 0.5], L=L, boundary=1, k=k))
+    d.compute_forces(1)
+    np.testing.assert_allclose(canon(d, "fx"), [k * 0.2, 0.0], atol=1e-15)
+    np.testing.assert_allclose(canon(d, "fy"), [0.0, -k * 0.3], atol=1e-15)
+    np.testing.assert_allclose(canon(d, "pe"), [0.5 * k * 0.04, 0.5 * k * 0.09], rtol=1e-12)
+    d2 = orc.OracleDisk(pair_batch([5.0, 5.8], [5.0, 5.0], [0.5, 0.5], L=L, boundary=1))
+    d2.compute_forces(1)
+    fx = canon(d2, "fx")
+    assert fx[0] < 0 < fx[1]               # the pair pushes apart
+    np.testing.assert_allclose(fx, [-0.2, 0.2], rtol=1e-12)
+
+
+def test_wall_lists_use_min_image():
+    """T6: wall-box lists are built with the min-image test, like the reference."""
+    L = 10.0
+    d = orc.OracleDisk(pair_batch([0.2, 9.8], [5.0, 5.0], [0.5, 0.5], L=L, boundary=1))
+    st, ids = d.neighbors_csr()
+    assert st[1] - st[0] == 1            # periodic image pair is listed ...
+    d.compute_forces(1)
+    fx = canon(d, "fx")                  # ... but contributes nothing in a walled box
+    np.testing.assert_allclose(fx, [0.5 * 1.0 - 0.2, -(0.5 - 0.2)], atol=1e-15)
+
+
+def test_newtons_third_law_and_rotation():
+    b = W.disk_batch([400, 250], [0.86, 0.84], [0, 0], seed=9)
+    d = orc.OracleDisk(b)
+    d.compute_forces(0)
+    fx, fy = d.get("fx"), d.get("fy")
+    for s in range(2):
+        sl = slice(b.off[s], b.off[s + 1])
+        scale = np.abs(fx[sl]).sum() + np.abs(fy[sl]).sum()
+        assert abs(fx[sl].sum()) < 1e-13 * scale and abs(fy[sl].sum()) < 1e-13 * scale
+
+
+# ------------------------------------------------------------- reductions (tree order)
+def independent_tree(v):
+    """Lane 0 of a 64-lane xor butterfly, restated as repeated halving."""
+    v = list(v)
+    while len(v) > 1:
+        h = len(v) // 2
+        v = [v[i] + v[i + h] for i in range(h)]
+    return v[0]
+
+
+def independent_sys_sum(vals):
+    n = len(vals)
+    nch = (n + 255) // 256
+    parts = []
+    for c in range(nch):
+        buf = [vals[c * 256 + t] if c * 256 + t < n else 0.0 for t in range(256)]
+        w = [independent_tree(buf[64 * q:64 * q + 64]) for q in range(4)]
+        parts.append((w[0] + w[1]) + (w[2] + w[3]))
+    acc = []
+    for t in range(256):
+        a = 0.0
+        for c in range(t, nch, 256):
+            a = a + parts[c]
+        acc.append(a)
+    w = [independent_tree(acc[64 * q:64 * q + 64]) for q in range(4)]
+    return (w[0] + w[1]) + (w[2] + w[3])
+
+
+@pytest.mark.parametrize("n", [1, 5, 256, 700, 256 * 300 + 17])
+def test_reduction_tree_order(n):
+    rng = np.random.default_rng(n)
+    v = rng.standard_normal(n) * 10.0 ** rng.integers(-8, 9, n)   # order-sensitive magnitudes
+    got = orc.sys_sum(v)
+    assert got == independent_sys_sum(v)
+    assert abs(got - math.fsum(v)) <= n * 2.3e-16 * np.abs(v).sum()
+    if n > 64:
+        assert got != float(np.cumsum(v)[-1]) or n < 300   # not a left-to-right sum
+
+
+# ------------------------------------------------------------- integrators
+def test_fire_update_branch_table():
+    f = orc.fire_update_scalar
+    r = f(0.0, 0.1, 3, 2, 5.0, 1.0, 1e-3)                        # stopped system
+    assert (r["dt"], r["alpha"], r["vscale"], r["dt_rev"]) == (0.0, 0.0, 0.0, 0.0)
+    r = f(0.1, 0.1, 2, 4, 1.0, 1.0, 1e-4)                        # downhill, warm-up
+    assert (r["dt"], r["alpha"], r["n_good"], r["n_bad"], r["vscale"]) == (0.1, 0.1, 3, 0, 1.0)
+    r = f(0.1, 0.1, 5, 0, 1.0, 1.0, 1e-4)                        # downhill, N_good > 5
+    assert r["dt"] == 0.1 * 1.1 and r["alpha"] == 0.1 * 0.99 and r["n_good"] == 6
+    r = f(0.95, 0.1, 9, 0, 1.0, 1.0, 1e-4)                       # capped at dt_max
+    assert r["dt"] == 1.0
+    r = f(0.1, 0.05, 7, 3, -1.0, 1.0, 1e-4)                      # uphill
+    assert (r["dt"], r["alpha"], r["n_good"], r["n_bad"]) == (0.05, 0.1, 0, 4)
+    assert r["dt_rev"] == -0.025 and r["vscale"] == 0.0          # dt decreased first (T3)
+    r = f(1e-4, 0.05, 0, 3, 0.0, 1.0, 1e-3)                      # floored at dt_min; P=0 is uphill
+    assert r["dt"] == 1e-3
+    r = f(0.1, 0.05, 0, 10, -1.0, 1.0, 1e-4)                     # N_bad > 10 stops, alpha kept
+    assert (r["dt"], r["alpha"], r["vscale"], r["dt_rev"], r["n_bad"]) == (0.0, 0.05, 1.0, 0.0, 11)
+
+
+def test_fire_convergence_reference_quirk():
+    """fire.cu:72-76 divides the stored average by dof again: with the reference predicate and
+    a huge pe_diff target nothing stops unless avg pe is below the pe target."""
+    b = W.single_disk_system(64, 0.9, seed=4)
+    d = orc.OracleDisk(b)
+    d.fire_init(1e-2)
+    d.fire_step()
+    assert not d.fire_converged(1e-3, 0.0)          # |N*pe/pe_prev - 1| ~ N - 1 >> 1e-3
+    assert d.fire_converged(1e-3, 1e9)              # the pe target stops it
+
+
+def test_fire_mixing_identity():
+    b = pair_batch([5.0, 12.0], [5.0, 5.0], [0.5, 0.5])
+    d = orc.OracleDisk(b)
+    d.set("fx", np.array([3.0, 0.0])); d.set("fy", np.array([4.0, 0.0]))
+    d.set("vx", np.array([0.0, 1.0])); d.set("vy", np.array([2.0, 1.0]))
+    a = 0.25
+    d.mix(a)
+    vx, vy = d.get("vx"), d.get("vy")
+    # v' = (1-a) v + a |v| F/|F|
+    np.testing.assert_allclose([vx[0], vy[0]], [0.75 * 0.0 + 0.25 * 2.0 * 0.6, 0.75 * 2.0 + 0.25 * 2.0 * 0.8],
+                               rtol=1e-15)
+    assert vx[1] == 0.0 and vy[1] == 0.0             # |F| <= 1e-16 zeroes v (T2)
+
+
+def test_vv_energy_error_is_second_order():
+    """test_disk.cu:356-397 (log-log slope of the energy fluctuation vs dt = 2 +- 0.1), scaled
+    down: 20 systems of 20 disks at phi 0.5, same simulated time for every dt."""
+    sizes = [20] * 20
+    res = []
+    dts = [4e-2, 2e-2, 1e-2]
+    for dt in dts:
+        b = W.disk_batch(sizes, [0.5] * 20, [0] * 20, seed=1, r_small=0.5, r_large=0.5)
+        d = orc.OracleDisk(b)
+        d.fire_minimize(1e-2, 3000, pred=1, ftol=1e-10)
+        rng = np.random.default_rng(0)
+        d.set("vx", rng.standard_normal(b.N) * 0.05); d.set("vy", rng.standard_normal(b.N) * 0.05)
+        d.compute_forces(0)
+        d.vv_init(dt)
+        nsteps = int(round(8.0 / dt))
+        te = []
+        for _ in range(nsteps // 4):
+            d.vv_run(4)
+            te.append(d.pe_total().sum() + d.ke_total().sum())
+        te = np.array(te)
+        res.append(te.std() / abs(te.mean()))
+    slope = np.polyfit(np.log(dts), np.log(res), 1)[0]
+    assert abs(slope - 2.0) < 0.2, (res, slope)
+
+
+def test_fire_reaches_a_minimum():
+    b = W.disk_batch([128, 200], [0.86, 0.85], [0, 1], seed=3)
+    d = orc.OracleDisk(b)
+    d.compute_forces(2)
+    pe0 = d.pe_total()
+    steps, conv = d.fire_minimize(1e-2, 40000, pred=1, ftol=1e-10, mode=2)
+    assert conv
+    assert np.all(d.pe_total() < pe0)
+    assert np.all(np.sqrt(d.max_force2()) < 1e-10)
+
+
+def test_temperature_and_set_temperature_order():
+    b = W.disk_batch([50, 70], [0.5, 0.5], [0, 0], seed=8)
+    d = orc.OracleDisk(b)
+    rng = np.random.default_rng(1)
+    vx, vy = rng.standard_normal(b.N) + 0.3, rng.standard_normal(b.N)
+    d.set("vx", vx); d.set("vy", vy)
+    ke = np.array([0.5 * np.sum(vx[:50] ** 2 + vy[:50] ** 2), 0.5 * np.sum(vx[50:] ** 2 + vy[50:] ** 2)])
+    np.testing.assert_allclose(d.temperature(), ke * 2 / np.array([100, 140]), rtol=1e-13)
+    d.set_temperature([1e-3, 2e-3])                 # scale from T before the mean is removed (T13)
+    assert np.all(d.temperature() < [1e-3, 2e-3])
+    d.set_temperature([1e-3, 2e-3])                 # mean already removed: exact now
+    np.testing.assert_allclose(d.temperature(), [1e-3, 2e-3], rtol=1e-12)
+    v = d.get("vx")
+    assert abs(v[:50].sum()) < 1e-13 and abs(v[50:].sum()) < 1e-13
+
+
+def test_increment_packing_fraction():
+    b = W.disk_batch([300], [0.80], [0], seed=6)
+    d = orc.OracleDisk(b)
+    np.testing.assert_allclose(d.packing_fraction(), [0.80], rtol=1e-13)
+    d.increment_packing_fraction(0.01)
+    np.testing.assert_allclose(d.packing_fraction(), [0.81], rtol=1e-13)
+    st, ids = d.neighbors_csr()
+    for i in range(0, b.N, 5):
+        assert sorted(ids[st[i]:st[i + 1]]) == sorted(d.brute(i))
