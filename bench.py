@@ -26,6 +26,10 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,18 +96,26 @@ def main():
     P.synchronize()
 
 
+    # NVE: the bulk runs as lean graph batches; the last n_prof steps run one launch per
+    # kernel with HIP events around every force launch (the roofline measurement).
+    n_prof = min(500, max(1, n_nve // 5))
+    log(f"timed region: {n_fire} FIRE + {n_nve} NVE steps, N={N}")
     t0 = time.perf_counter()
-    steps_fire, _ = run_fire(P, 1e-2, n_fire, L.FIRE_PRED_MAX_FORCE, 1e-12, use_graph=True, poll=n_fire)
+    steps_fire, _ = run_fire(P, 1e-2, n_fire, L.FIRE_PRED_MAX_FORCE, 1e-12, use_graph=True)
+    P.synchronize()
+    t_fire = time.perf_counter() - t0
+    log(f"FIRE done: {steps_fire} steps, {t_fire:.3f} s")
     P.rand_normal_2d("VEL_X", "VEL_Y", 1, 1)
     P.set_average_velocity(0.0, 0.0)
     P.set_temperature(1e-4)
     vv = VelocityVerlet(P, 1e-2)
     vv.init()
+    vv.run(n_nve - n_prof, use_graph=True)
     lib.dpmd_profile(P.ctx, 1)
-    vv.step(n_nve)
+    vv.step(n_prof)
     P.synchronize()
-
     t = time.perf_counter() - t0
+    log(f"NVE done: {n_nve} steps, {t - t_fire:.3f} s")
 
     tot_ms, cnt = C.c_double(), C.c_int()
     lib.dpmd_profile_read(P.ctx, C.byref(tot_ms), C.byref(cnt))
@@ -145,6 +157,7 @@ def main():
         "config": {"workload": "single large disk system: N=2^20 bidisperse (0.5/0.7, 50:50), phi=0.84, "
                                "PBC, FP64, skin 0.3; FIRE then NVE dt=1e-2 (T=1e-4)",
                    "n_particles": N, "fire_steps": steps_fire, "nve_steps": n_nve,
+                   "fire_seconds": t_fire, "nve_seconds": t - t_fire, "nve_profiled_steps": n_prof,
                    "mean_neighbors": k, "neighbor_capacity": cap, "max_neighbors": mx},
         "roofline": {"bound": "hbm", "kernel": "k_vv_force (force + half kick, NVE)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -8,21 +8,6 @@
 
 using namespace dpmd;
 
-namespace dpmd {
-void fire_recompute_update(dpmd_ctx* c);
-void rigid_compute_forces(dpmd_ctx* c, int mode);
-void rigid_update_positions(dpmd_ctx* c, const double* scale, double scale2);
-void rigid_update_velocities(dpmd_ctx* c, const double* scale, double scale2);
-void rigid_scale_velocities(dpmd_ctx* c, const double* scale);
-void rigid_scale_positions(dpmd_ctx* c, const double* scale);
-void rigid_mix(dpmd_ctx* c, const double* alpha);
-void rigid_stress(dpmd_ctx* c);
-void rigid_fire_drift(dpmd_ctx* c, const int* active);
-void rigid_fire_force(dpmd_ctx* c, const int* active, int kick, int do_conv, int pred, double pa, double pb);
-void rigid_vv_drift(dpmd_ctx* c, const int* active);
-void rigid_vv_force(dpmd_ctx* c, const int* active);
-}  // namespace dpmd
-
 namespace {
 
 std::string g_global_err;
@@ -138,6 +123,7 @@ std::vector<T> dl(dpmd_ctx* c, const dpmd::DVec<T>& v) {
 }
 
 const int* active_one(dpmd_ctx* c) { return c->one.p; }
+Gate plain(dpmd_ctx* c) { return Gate{c->one.p, nullptr, 0}; }
 
 bool rigid(dpmd_ctx* c) { return c->kind == DPMD_RIGID_BUMPY; }
 
@@ -327,48 +313,71 @@ void fire_init_impl(dpmd_ctx* c, const double* dt, int mode) {
     c->f_alpha.upload(al.data(), S, c->st); c->f_peprev.upload(pp.data(), S, c->st);
     c->f_ngood.upload(z.data(), S, c->st); c->f_nbad.upload(z.data(), S, c->st);
     c->f_stopped.upload(stopped.data(), S, c->st);
-    int run[8] = {0, 0, running, 1, 0, 0, 0, 0};
+    int run[8] = {0, 0, running, 1, 0, -1, -1, 0};
     c->run.upload(run, 8, c->st);
     c->fire_mode = mode;
     // init_impl (fire.cuh:90-96): v *= 0, then forces; the first update sees F.v = 0.
     zero_scale(c);
     scale_velocities_impl(c, c->scr_c.p);
-    if (rigid(c)) rigid_fire_force(c, active_one(c), 0, 0, 0, 0.0, 0.0);
-    else disk_fire_force(c, active_one(c), 0, 0, 0, 0.0, 0.0);
+    if (rigid(c)) rigid_fire_force(c, plain(c), 0, 0, 0, 0.0, 0.0);
+    else disk_fire_force(c, plain(c), 0, 0, 0, 0.0, 0.0);
     c->fire_ready = true;
 }
 
-void fire_step_body(dpmd_ctx* c, const int* active, int do_conv, int pred, double a, double b) {
-    if (rigid(c)) rigid_fire_drift(c, active); else disk_fire_drift(c, active);
-    if (c->method == DPMD_NEIGHBOR_CELL) rebuild_neighbors(c, active, false);
-    if (rigid(c)) rigid_fire_force(c, active, 1, do_conv, pred, a, b);
-    else disk_fire_force(c, active, 1, do_conv, pred, a, b);
+// One full step in the reference's order, every launch unconditional: drift, the
+// device-conditional rebuild chain, forces (+ FIRE bookkeeping).
+void fire_step_body(dpmd_ctx* c, int do_conv, int pred, double a, double b) {
+    if (rigid(c)) rigid_fire_drift(c, plain(c)); else disk_fire_drift(c, plain(c));
+    if (c->method == DPMD_NEIGHBOR_CELL) rebuild_neighbors(c, active_one(c), false);
+    if (rigid(c)) rigid_fire_force(c, plain(c), 1, do_conv, pred, a, b);
+    else disk_fire_force(c, plain(c), 1, do_conv, pred, a, b);
 }
 
-void vv_step_body(dpmd_ctx* c, const int* active) {
-    if (rigid(c)) rigid_vv_drift(c, active); else disk_vv_drift(c, active);
-    if (c->method == DPMD_NEIGHBOR_CELL) rebuild_neighbors(c, active, false);
-    if (rigid(c)) rigid_vv_force(c, active); else disk_vv_force(c, active);
+void vv_step_body(dpmd_ctx* c) {
+    if (rigid(c)) rigid_vv_drift(c, plain(c)); else disk_vv_drift(c, plain(c));
+    if (c->method == DPMD_NEIGHBOR_CELL) rebuild_neighbors(c, active_one(c), false);
+    if (rigid(c)) rigid_vv_force(c, plain(c)); else disk_vv_force(c, plain(c));
 }
 
-int read_running(dpmd_ctx* c, int* steps) {
-    int run[4];
+// ---------------------------------------------------------------------------------------
+// Lean fused runs.  A batch of m steps issues only drift + force (+ FIRE finalize) per
+// step.  A step whose drift raised a rebuild flag is halted by its force kernel (the rest
+// of the batch becomes no-ops); the host then finishes that step with the full chain
+// (rebuild, forces, bookkeeping) and continues.  Every step therefore does exactly what
+// the full step does, while steps without a rebuild cost two or three launches.  Batches
+// are captured once per size into hipGraphs when use_graph is set.
+// ---------------------------------------------------------------------------------------
+struct RunCtl {
+    int running, halt, halt_at, conv_at;
+};
+
+RunCtl read_run(dpmd_ctx* c) {
+    int run[8];
     HIP_OK(hipMemcpyAsync(run, c->run.p, sizeof(run), hipMemcpyDeviceToHost, c->st));
     HIP_OK(hipStreamSynchronize(c->st));
-    if (steps) *steps = run[0];
-    return run[2];
+    return RunCtl{run[RUN_RUNNING], run[RUN_HALT], run[RUN_HALT_AT], run[RUN_CONV_AT]};
 }
 
-hipGraphExec_t capture_once(dpmd_ctx* c, int steps_per_graph, bool fire, int pred, double a, double b) {
+void lean_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b) {
+    for (int k = 0; k < m; ++k) {
+        const Gate g = Gate{nullptr, c->run.p, k};
+        if (fire) {
+            if (rigid(c)) rigid_fire_drift(c, g); else disk_fire_drift(c, g);
+            if (rigid(c)) rigid_fire_force(c, g, 1, 1, pred, a, b);
+            else disk_fire_force(c, g, 1, 1, pred, a, b);
+        } else {
+            if (rigid(c)) rigid_vv_drift(c, g); else disk_vv_drift(c, g);
+            if (rigid(c)) rigid_vv_force(c, g); else disk_vv_force(c, g);
+        }
+    }
+}
+
+hipGraphExec_t capture_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b) {
     hipGraph_t g = nullptr;
     HIP_OK(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
     c->capturing = true;
     try {
-        for (int k = 0; k < steps_per_graph; ++k) {
-            run_begin_step(c);
-            if (fire) fire_step_body(c, c->run.p + 3, 1, pred, a, b);
-            else vv_step_body(c, c->run.p + 3);
-        }
+        lean_batch(c, m, fire, pred, a, b);
     } catch (...) {
         c->capturing = false;
         (void)hipStreamEndCapture(c->st, &g);
@@ -384,19 +393,56 @@ hipGraphExec_t capture_once(dpmd_ctx* c, int steps_per_graph, bool fire, int pre
     return ex;
 }
 
-// Captures one graph of `steps_per_graph` steps.  If the cooperative rebuild launch cannot be
-// captured, the rebuild inside graphs falls back to its multi-launch form (same phases).
-hipGraphExec_t capture(dpmd_ctx* c, int steps_per_graph, bool fire, int pred, double a, double b) {
-    if (c->use_coop && c->coop_in_graph) {
-        try {
-            return capture_once(c, steps_per_graph, fire, pred, a, b);
-        } catch (const Error&) {
-            (void)hipGetLastError();
-            HIP_OK(hipStreamSynchronize(c->st));
-            c->coop_in_graph = false;
+// Runs up to max_steps steps; returns the number taken and (FIRE) whether all systems stopped.
+int lean_run(dpmd_ctx* c, int max_steps, bool fire, int pred, double a, double b, bool use_graph, int* conv) {
+    std::vector<hipGraphExec_t> graphs(8, nullptr);   // batch sizes 1, 2, 4, ..., 128
+    int total = 0, m = 8, since_halt = 0;
+    bool done = false;
+    try {
+        while (!done && total < max_steps) {
+            int lg = 0;
+            while ((2 << lg) <= std::min(m, max_steps - total) && lg < 7) ++lg;
+            const int bm = 1 << lg;
+            if (use_graph) {
+                if (!graphs[lg]) graphs[lg] = capture_batch(c, bm, fire, pred, a, b);
+                HIP_OK(hipGraphLaunch(graphs[lg], c->st));
+            } else {
+                lean_batch(c, bm, fire, pred, a, b);
+            }
+            RunCtl r = read_run(c);
+            if (r.halt) {
+                total += r.halt_at + 1;
+                since_halt += r.halt_at;
+                // finish the halted step: rebuild, forces, bookkeeping
+                rebuild_neighbors(c, active_one(c), false);
+                if (fire) {
+                    if (rigid(c)) rigid_fire_force(c, plain(c), 1, 1, pred, a, b);
+                    else disk_fire_force(c, plain(c), 1, 1, pred, a, b);
+                } else {
+                    if (rigid(c)) rigid_vv_force(c, plain(c)); else disk_vv_force(c, plain(c));
+                }
+                const int clear[2] = {0, -1};
+                HIP_OK(hipMemcpyAsync(c->run.p + RUN_HALT, clear, sizeof(clear), hipMemcpyHostToDevice, c->st));
+                r = read_run(c);
+                if (fire && r.running <= 0) done = true;
+                m = std::max(1, std::min(128, since_halt));
+                since_halt = 0;
+            } else if (fire && r.running <= 0) {
+                total += r.conv_at + 1;
+                done = true;
+            } else {
+                total += bm;
+                since_halt += bm;
+                m = std::min(128, 2 * bm);
+            }
         }
+    } catch (...) {
+        for (auto g : graphs) if (g) (void)hipGraphExecDestroy(g);
+        throw;
     }
-    return capture_once(c, steps_per_graph, fire, pred, a, b);
+    for (auto g : graphs) if (g) HIP_OK(hipGraphExecDestroy(g));
+    if (conv) *conv = (fire && read_run(c).running <= 0) ? 1 : 0;
+    return total;
 }
 
 }  // namespace
@@ -761,7 +807,7 @@ int dpmd_vv_init(dpmd_ctx* c, const double* dt, int mode) {
 int dpmd_vv_step(dpmd_ctx* c, int n) {
     return guard(c, [&] {
         need_state(c->vv_ready, "vv_step: call vv_init first");
-        for (int k = 0; k < n; ++k) vv_step_body(c, active_one(c));
+        for (int k = 0; k < n; ++k) vv_step_body(c);
     });
 }
 
@@ -769,19 +815,12 @@ int dpmd_vv_run(dpmd_ctx* c, int n, int use_graph) {
     return guard(c, [&] {
         need_state(c->vv_ready, "vv_run: call vv_init first");
         if (n <= 0) return;
-        int run[8] = {0, n, 1, 0, 0, 0, 0, 0};
+        int any = 0;
+        HIP_OK(hipMemcpyAsync(&any, c->run.p + RUN_ANY, sizeof(int), hipMemcpyDeviceToHost, c->st));
+        HIP_OK(hipStreamSynchronize(c->st));
+        int run[8] = {0, n, 1, 0, 0, -1, -1, any};
         c->run.upload(run, 8, c->st);
-        if (use_graph) {
-            const int G = std::min(n, 20);
-            if (!c->g_vv || c->g_vv_key != G) {
-                if (c->g_vv) HIP_OK(hipGraphExecDestroy(c->g_vv));
-                c->g_vv = capture(c, G, false, 0, 0, 0);
-                c->g_vv_key = G;
-            }
-            for (int k = 0; k < n; k += G) HIP_OK(hipGraphLaunch(c->g_vv, c->st));
-        } else {
-            for (int k = 0; k < n; ++k) { run_begin_step(c); vv_step_body(c, c->run.p + 3); }
-        }
+        (void)lean_run(c, n, false, 0, 0.0, 0.0, use_graph != 0, nullptr);
         check_overflow(c);
     });
 }
@@ -794,7 +833,7 @@ int dpmd_fire_step(dpmd_ctx* c) {
         // compute_fpower_total + fire_update_kernel (fire.cuh:62-72) from the live state
         sys_reduce(c, RED_POWER, c->pw_tot.p);
         fire_recompute_update(c);
-        fire_step_body(c, active_one(c), 0, 0, 0.0, 0.0);
+        fire_step_body(c, 0, 0, 0.0, 0.0);
     });
 }
 
@@ -803,8 +842,7 @@ int dpmd_fire_converged(dpmd_ctx* c, double diff_target, double pe_target, int* 
         need_state(c->fire_ready, "fire_converged: call fire_init first");
         fire_apply_convergence(c, DPMD_FIRE_PRED_REFERENCE, diff_target, pe_target);
         check_overflow(c);
-        const int running = read_running(c, nullptr);
-        if (all) *all = running == 0 ? 1 : 0;
+        if (all) *all = read_run(c).running <= 0 ? 1 : 0;
     });
 }
 
@@ -813,8 +851,7 @@ int dpmd_fire_converged_max_force(dpmd_ctx* c, double tol, int* all) {
         need_state(c->fire_ready, "fire_converged: call fire_init first");
         fire_apply_convergence(c, DPMD_FIRE_PRED_MAX_FORCE, tol, 0.0);
         check_overflow(c);
-        const int running = read_running(c, nullptr);
-        if (all) *all = running == 0 ? 1 : 0;
+        if (all) *all = read_run(c).running <= 0 ? 1 : 0;
     });
 }
 
@@ -838,36 +875,13 @@ int dpmd_fire_minimize(dpmd_ctx* c, const double* dt, int max_steps, int pred, d
     return guard(c, [&] {
         need(pred == DPMD_FIRE_PRED_REFERENCE || pred == DPMD_FIRE_PRED_MAX_FORCE, "unknown FIRE predicate");
         fire_init_impl(c, dt, mode);
-        int run[8] = {0, max_steps, 0, 0, 0, 0, 0, 0};
-        HIP_OK(hipMemcpyAsync(&run[2], c->run.p + 2, sizeof(int), hipMemcpyDeviceToHost, c->st));
-        HIP_OK(hipStreamSynchronize(c->st));
-        c->run.upload(run, 8, c->st);
-        if (poll <= 0) poll = 100;
-        int steps = 0, running = run[2];
-        if (use_graph) {
-            const int G = std::min(std::max(1, poll), 50);
-            const int per_poll = std::max(1, poll / G);
-            hipGraphExec_t ex = capture(c, G, true, pred, a, b);
-            int64_t launches = 0;
-            while (running > 0 && steps < max_steps) {
-                HIP_OK(hipGraphLaunch(ex, c->st));
-                ++launches;
-                if (launches % per_poll == 0 || launches * G >= max_steps) running = read_running(c, &steps);
-            }
-            HIP_OK(hipGraphExecDestroy(ex));
-        } else {
-            int k = 0;
-            while (running > 0 && steps < max_steps) {
-                run_begin_step(c);
-                fire_step_body(c, c->run.p + 3, 1, pred, a, b);
-                ++k;
-                if (k % poll == 0 || k >= max_steps) running = read_running(c, &steps);
-            }
-        }
-        running = read_running(c, &steps);
+        (void)poll;   // batches adapt to the rebuild rate; the host reads state once per batch
+        int conv = 0;
+        const int running0 = read_run(c).running;
+        const int steps = running0 > 0 ? lean_run(c, max_steps, true, pred, a, b, use_graph != 0, &conv) : 0;
         check_overflow(c);
         if (steps_taken) *steps_taken = steps;
-        if (all_conv) *all_conv = running == 0 ? 1 : 0;
+        if (all_conv) *all_conv = running0 > 0 ? conv : 1;
     });
 }
 

@@ -35,6 +35,7 @@ struct RebuildArgs {
     // rigid bodies: per-particle reference = first vertex of each particle
     int N; const int* psid; const int* poff; double* plx; double* ply;
     int* tile_sum; int nt; int S;
+    int* any;                          // the run word "some rebuild flag is set"
 };
 
 __device__ __forceinline__ int block_excl_scan_int(int v, int* lds4, int& total) {
@@ -256,10 +257,11 @@ __global__ void k_set_flags(int S, unsigned* flag, const int* active) {
     flag[s] = 1u;
 }
 
-__global__ void k_clear_flags(int S, unsigned* flag, const int* active) {
+__global__ void k_clear_flags(int S, unsigned* flag, int* any, const int* active) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S || !*active) return;
     flag[s] = 0u;
+    if (s == 0) *any = 0;
 }
 
 // ---- single cooperative launch -----------------------------------------------------------
@@ -305,6 +307,7 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_rebuild_coop(RebuildArgs r, cons
     g.sync();
     if (blockIdx.x == 0)
         for (int s = threadIdx.x; s < r.S; s += DPMD_BLOCK) r.ca.flag[s] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *r.any = 0;
 }
 
 // set_naive_neighbor_count + fill_naive_neighbor_list_kernel (base_point_particle_kernels.cu:
@@ -379,6 +382,7 @@ RebuildArgs rebuild_args(dpmd_ctx* c) {
     r.N = c->N; r.psid = c->sid.p; r.poff = c->poff.p;
     r.plx = rigid ? c->lx.p : nullptr; r.ply = rigid ? c->ly.p : nullptr;
     r.tile_sum = c->scan_part.p; r.nt = (c->C + SCAN_TILE - 1) / SCAN_TILE; r.S = c->S;
+    r.any = c->run.p + RUN_ANY;
     return r;
 }
 
@@ -396,7 +400,7 @@ void rebuild_multi(dpmd_ctx* c, const RebuildArgs& r, const int* active) {
     k_copy_back<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     k_neighbors<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     if (r.plx) k_rigid_reset<<<blocks_for(r.N), DPMD_BLOCK, 0, st>>>(r, active);
-    k_clear_flags<<<blocks_for(r.S), DPMD_BLOCK, 0, st>>>(r.S, r.ca.flag, active);
+    k_clear_flags<<<blocks_for(r.S), DPMD_BLOCK, 0, st>>>(r.S, r.ca.flag, r.any, active);
 }
 
 int coop_grid(dpmd_ctx* c) {
@@ -445,7 +449,7 @@ void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all) {
 }
 
 void clear_flags(dpmd_ctx* c, const int* active) {
-    k_clear_flags<<<blocks_for(c->S), DPMD_BLOCK, 0, c->st>>>(c->S, c->flag.p, active);
+    k_clear_flags<<<blocks_for(c->S), DPMD_BLOCK, 0, c->st>>>(c->S, c->flag.p, c->run.p + RUN_ANY, active);
     HIP_OK(hipGetLastError());
 }
 

@@ -17,7 +17,7 @@ struct DiskArgs {
     const int* sid; const int* off;
     const double* Lx; const double* Ly; const double* iLx; const double* iLy; const double* e;
     const double* thresh2; const int* boundary;
-    double* lx; double* ly; unsigned* flag;
+    double* lx; double* ly; unsigned* flag; int* any;
     const int* nn; const int* nbr;
     int n; int cell;
 };
@@ -41,7 +41,7 @@ DiskArgs disk_args(dpmd_ctx* c) {
     a.rad = c->rad.p; a.mass = c->mass.p; a.sid = c->sid.p; a.off = c->off.p;
     a.Lx = c->Lx.p; a.Ly = c->Ly.p; a.iLx = c->iLx.p; a.iLy = c->iLy.p; a.e = c->e.p;
     a.thresh2 = c->thresh2.p; a.boundary = c->boundary.p;
-    a.lx = c->lx.p; a.ly = c->ly.p; a.flag = c->flag.p;
+    a.lx = c->lx.p; a.ly = c->ly.p; a.flag = c->flag.p; a.any = c->run.p + RUN_ANY;
     a.nn = c->nn.p; a.nbr = c->nbr.p; a.n = c->N; a.cell = c->method == DPMD_NEIGHBOR_CELL;
     return a;
 }
@@ -119,23 +119,24 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_force(DiskArgs a) {
     a.fx[i] = fx; a.fy[i] = fy; a.pe[i] = pe;
 }
 
-__device__ __forceinline__ void skin_check(const DiskArgs& a, int i, int s, double x, double y) {
-    const double dx = x - a.lx[i], dy = y - a.ly[i];
-    const double d2 = dx * dx + dy * dy;
-    if (d2 > a.thresh2[s]) atomicOr(&a.flag[s], 1u);
-}
-
 // update_positions_kernel_cell / _naive (disk.cu:20-75)
 __global__ void k_update_positions(DiskArgs a, const double* __restrict__ scale, double scale2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    const int s = a.sid[i];
-    const double sc = scale[s] * scale2;
-    double x = a.x[i], y = a.y[i];
-    x += a.vx[i] * sc;
-    y += a.vy[i] * sc;
-    a.x[i] = x; a.y[i] = y;
-    if (a.cell) skin_check(a, i, s, x, y);
+    const bool valid = i < a.n;
+    const int s = valid ? a.sid[i] : -1;
+    bool flag = false;
+    if (valid) {
+        const double sc = scale[s] * scale2;
+        double x = a.x[i], y = a.y[i];
+        x += a.vx[i] * sc;
+        y += a.vy[i] * sc;
+        a.x[i] = x; a.y[i] = y;
+        if (a.cell) {
+            const double dx = x - a.lx[i], dy = y - a.ly[i];
+            flag = (dx * dx + dy * dy) > a.thresh2[s];
+        }
+    }
+    if (a.cell) raise_flags(a.flag, a.any, s, flag);
 }
 
 // update_velocities_kernel (disk.cu:78-100)
@@ -261,17 +262,26 @@ __device__ __forceinline__ void fire_convergence_sys(const FireArgs& f, const in
     f.peprev[s] = avg;
 }
 
-__device__ __forceinline__ void fire_count_stopped(const FireArgs& f, int s) {
+__device__ __forceinline__ void fire_count_stopped(const FireArgs& f, int s, int k) {
     if (f.dt[s] == 0.0 && !f.stopped[s]) {
         f.stopped[s] = 1;
-        atomicSub(&f.run[2], 1);
+        if (atomicSub(&f.run[RUN_RUNNING], 1) == 1) f.run[RUN_CONV_AT] = k;
     }
 }
 
-__global__ void __launch_bounds__(DPMD_BLOCK) k_fire_drift(DiskArgs a, FireArgs f, const int* active) {
+__device__ __forceinline__ bool fire_drift_one(const DiskArgs& a, const FireArgs& f, int i, int s);
+
+__global__ void __launch_bounds__(DPMD_BLOCK) k_fire_drift(DiskArgs a, FireArgs f, Gate g) {
+    if (!gate_open(g)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n || !*active) return;
-    const int s = a.sid[i];
+    const bool valid = i < a.n;
+    const int s = valid ? a.sid[i] : -1;
+    bool flag = false;
+    if (valid) flag = fire_drift_one(a, f, i, s);
+    if (a.cell) raise_flags(a.flag, a.any, s, flag);
+}
+
+__device__ __forceinline__ bool fire_drift_one(const DiskArgs& a, const FireArgs& f, int i, int s) {
     const double dt = f.s_dt[s], alpha = f.s_alpha[s], dtrev = f.s_dtrev[s], vscale = f.s_vscale[s];
     const double m = a.mass[i];
     const double fx = a.fx[i], fy = a.fy[i];
@@ -299,19 +309,19 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_fire_drift(DiskArgs a, FireArgs 
     if (a.cell) {
         const double dx = x - a.lx[i], dy = y - a.ly[i];
         flag = flag || ((dx * dx + dy * dy) > a.thresh2[s]);
-        if (flag) atomicOr(&a.flag[s], 1u);
     }
     a.x[i] = x; a.y[i] = y; a.vx[i] = vx; a.vy[i] = vy;
     if (i == a.off[s]) {
         f.dt[s] = dt; f.alpha[s] = alpha; f.ngood[s] = f.s_ngood[s]; f.nbad[s] = f.s_nbad[s];
     }
+    return flag;
 }
 
 template <int MODE>
 __global__ void __launch_bounds__(DPMD_BLOCK)
-k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, const int* active) {
+k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, Gate g) {
     __shared__ double lds[4];
-    if (!*active) return;
+    if (!gate_force(g)) return;
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     const int s = r.chunk_sys[c];
     const int beg = r.chunk_beg[c];
@@ -345,9 +355,9 @@ k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, const int* active) {
 // step's FIRE update.  A separate launch orders it after every chunk without fences.
 __global__ void __launch_bounds__(DPMD_BLOCK)
 k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_update, int pred, double pa, double pb,
-                const int* active) {
+                Gate g) {
     __shared__ double lds[4];
-    if (!*active) return;
+    if (!gate_open(g)) return;
     const int s = blockIdx.x;
     const int cb = r.sys_chunk[s], ce = r.sys_chunk[s + 1];
     double ap = 0.0, ape = 0.0, af = 0.0;
@@ -362,7 +372,7 @@ k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_updat
     if (threadIdx.x == 0) {
         r.pw_tot[s] = ap; r.pe_tot[s] = ape; r.maxf2[s] = af;
         if (do_conv) fire_convergence_sys(f, off, s, ape, af, pred, pa, pb);
-        if (do_conv) fire_count_stopped(f, s);
+        if (do_conv) fire_count_stopped(f, s, g.k);
         if (do_update) fire_update_sys(f, s, ap);
     }
 }
@@ -373,7 +383,7 @@ __global__ void k_fire_converge(int S, const int* off, FireArgs f, const double*
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     fire_convergence_sys(f, off, s, pe_tot[s], maxf2[s], pred, pa, pb);
-    fire_count_stopped(f, s);
+    fire_count_stopped(f, s, 0);
     fire_update_sys(f, s, pw_tot[s]);
 }
 
@@ -389,24 +399,32 @@ __global__ void k_fire_update(int S, FireArgs f, const double* pw_tot) {
 //   rebuild;  k_vv_force: compute_forces + update_velocities(dt_half).
 // ---------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(DPMD_BLOCK)
-k_vv_drift(DiskArgs a, const double* __restrict__ dt, const double* __restrict__ dth, const int* active) {
+k_vv_drift(DiskArgs a, const double* __restrict__ dt, const double* __restrict__ dth, Gate g) {
+    if (!gate_open(g)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n || !*active) return;
-    const int s = a.sid[i];
-    const double smi = dth[s] * 1.0 / a.mass[i];
-    double vx = a.vx[i] + a.fx[i] * smi;
-    double vy = a.vy[i] + a.fy[i] * smi;
-    const double sc = dt[s] * 1.0;
-    double x = a.x[i], y = a.y[i];
-    x += vx * sc; y += vy * sc;
-    a.x[i] = x; a.y[i] = y; a.vx[i] = vx; a.vy[i] = vy;
-    if (a.cell) skin_check(a, i, s, x, y);
+    const bool valid = i < a.n;
+    const int s = valid ? a.sid[i] : -1;
+    bool flag = false;
+    if (valid) {
+        const double smi = dth[s] * 1.0 / a.mass[i];
+        double vx = a.vx[i] + a.fx[i] * smi;
+        double vy = a.vy[i] + a.fy[i] * smi;
+        const double sc = dt[s] * 1.0;
+        double x = a.x[i], y = a.y[i];
+        x += vx * sc; y += vy * sc;
+        a.x[i] = x; a.y[i] = y; a.vx[i] = vx; a.vy[i] = vy;
+        if (a.cell) {
+            const double dx = x - a.lx[i], dy = y - a.ly[i];
+            flag = (dx * dx + dy * dy) > a.thresh2[s];
+        }
+    }
+    if (a.cell) raise_flags(a.flag, a.any, s, flag);
 }
 
 template <int MODE>
 __global__ void __launch_bounds__(DPMD_BLOCK)
-k_vv_force(DiskArgs a, const double* __restrict__ dth, const int* active) {
-    if (!*active) return;
+k_vv_force(DiskArgs a, const double* __restrict__ dth, Gate g) {
+    if (!gate_force(g)) return;
     const int i = xcd_remap(blockIdx.x, gridDim.x) * DPMD_BLOCK + threadIdx.x;
     if (i >= a.n) return;
     const int s = a.sid[i];
@@ -459,34 +477,47 @@ void disk_stress(dpmd_ctx* c) {
     HIP_OK(hipGetLastError());
 }
 
-void disk_fire_drift(dpmd_ctx* c, const int* active) {
-    k_fire_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), fire_args(c), active);
+void disk_fire_drift(dpmd_ctx* c, Gate g) {
+    k_fire_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), fire_args(c), g);
     HIP_OK(hipGetLastError());
 }
 
-void disk_fire_force(dpmd_ctx* c, const int* active, int kick, int do_conv, int pred, double pa, double pb) {
+void disk_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, double pa, double pb) {
     DiskArgs a = disk_args(c);
     FireArgs f = fire_args(c);
     RedArgs r = red_args(c);
-    ProfScope prof(c);
-    by_mode(c->fire_mode, [&](auto m) {
-        k_fire_force<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(a, f, r, kick, active);
-    });
-    k_fire_finalize<<<c->S, DPMD_BLOCK, 0, c->st>>>(c->off.p, f, r, do_conv, 1, pred, pa, pb, active);
+    {
+        ProfScope prof(c);
+        by_mode(c->fire_mode, [&](auto m) {
+            k_fire_force<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(a, f, r, kick, g);
+        });
+    }
+    k_fire_finalize<<<c->S, DPMD_BLOCK, 0, c->st>>>(c->off.p, f, r, do_conv, 1, pred, pa, pb, g);
     HIP_OK(hipGetLastError());
 }
 
-void disk_vv_drift(dpmd_ctx* c, const int* active) {
-    k_vv_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), c->vv_dt.p, c->vv_dth.p, active);
+void disk_vv_drift(dpmd_ctx* c, Gate g) {
+    k_vv_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), c->vv_dt.p, c->vv_dth.p, g);
     HIP_OK(hipGetLastError());
 }
 
-void disk_vv_force(dpmd_ctx* c, const int* active) {
+void disk_vv_force(dpmd_ctx* c, Gate g) {
     DiskArgs a = disk_args(c);
     ProfScope prof(c);
     by_mode(c->vv_mode, [&](auto m) {
-        k_vv_force<decltype(m)::value><<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(a, c->vv_dth.p, active);
+        k_vv_force<decltype(m)::value><<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(a, c->vv_dth.p, g);
     });
+    HIP_OK(hipGetLastError());
+}
+
+__global__ void k_ke(DiskArgs a, double* ke) {   // compute_ke_kernel (disk.cu:266-275)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    ke[i] = 0.5 * (a.vx[i] * a.vx[i] + a.vy[i] * a.vy[i]) * a.mass[i];
+}
+
+void disk_ke(dpmd_ctx* c) {
+    k_ke<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), c->ke.p);
     HIP_OK(hipGetLastError());
 }
 

@@ -37,6 +37,54 @@ struct NbrArgs {
     int* overflow; int* max_count;
 };
 
+// Run-control words (int array "run" of the context):
+//   [1] max steps  [2] systems still running  [3] step active (device-conditional path)
+//   [4] halted     [5] index of the halted step in its batch
+//   [6] index of the step in its batch at which the last system stopped
+//   [7] "some rebuild flag is set" (raised by the drift kernels, cleared by the rebuild)
+enum { RUN_MAX = 1, RUN_RUNNING = 2, RUN_ACTIVE = 3, RUN_HALT = 4, RUN_HALT_AT = 5, RUN_CONV_AT = 6, RUN_ANY = 7 };
+
+// Launch gate.  Plain form (run == nullptr): the kernel works iff *active != 0.  Batch form
+// (lean fused runs): a step works while nothing halted and some system runs; the force
+// kernel halts the batch instead of computing with a list that needs a rebuild.  Every
+// workgroup of a launch reads the same words, so a whole launch takes one path.
+struct Gate {
+    const int* active;
+    int* run;
+    int k;
+};
+
+__device__ __forceinline__ bool gate_open(const Gate& g) {
+    if (!g.run) return *g.active != 0;
+    return g.run[RUN_HALT] == 0 && g.run[RUN_RUNNING] > 0;
+}
+
+__device__ __forceinline__ bool gate_force(const Gate& g) {
+    if (!g.run) return *g.active != 0;
+    if (g.run[RUN_HALT] != 0 || g.run[RUN_RUNNING] <= 0) return false;
+    if (g.run[RUN_ANY] != 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { g.run[RUN_HALT_AT] = g.k; g.run[RUN_HALT] = 1; }
+        return false;
+    }
+    return true;
+}
+
+// Raise rebuild flag s (and the any-flag word) for every lane with f set: one atomic per
+// wave when the whole wave belongs to one system, else one per flagged lane.  Every lane of
+// the wave must call (invalid lanes pass f = false, s = -1).
+__device__ __forceinline__ void raise_flags(unsigned* flag, int* any, int s, bool f) {
+    const unsigned long long m = __ballot(f);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int s0 = __shfl(s, 0, 64), s63 = __shfl(s, 63, 64);
+    if (s0 == s63 && s0 >= 0) {
+        if (lane == __ffsll((unsigned long long)m) - 1) { atomicOr(&flag[s], 1u); atomicOr(any, 1); }
+    } else if (f) {
+        atomicOr(&flag[s], 1u);
+        atomicOr(any, 1);
+    }
+}
+
 }  // namespace dpmd
 
 struct dpmd_ctx {
@@ -101,8 +149,9 @@ struct dpmd_ctx {
     int g_fire_key[4] = {0, 0, 0, 0};
     double g_fire_keyd[2] = {0, 0};
     int g_vv_key = -1;
-    // rebuild as one cooperative launch (use_coop); whether that launch may be captured
-    bool use_coop = true, coop_in_graph = true;
+    // rebuild as one cooperative launch (use_coop, off by default: a full-device grid of it
+    // did not finish in one run, cause not found); whether that launch may be captured
+    bool use_coop = false, coop_in_graph = false;
     int coop_blocks = 0;
     // kernel timing (force kernels; outside graph capture only)
     bool prof = false, capturing = false;
@@ -128,10 +177,24 @@ void disk_scale_velocities(dpmd_ctx* c, const double* scale);
 void disk_scale_positions(dpmd_ctx* c, const double* scale);
 void disk_mix(dpmd_ctx* c, const double* alpha);
 void disk_stress(dpmd_ctx* c);
-void disk_fire_force(dpmd_ctx* c, const int* active, int kick, int do_conv, int pred, double a, double b);
-void disk_fire_drift(dpmd_ctx* c, const int* active);
-void disk_vv_drift(dpmd_ctx* c, const int* active);
-void disk_vv_force(dpmd_ctx* c, const int* active);
+void disk_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, double a, double b);
+void disk_fire_drift(dpmd_ctx* c, Gate g);
+void disk_vv_drift(dpmd_ctx* c, Gate g);
+void disk_vv_force(dpmd_ctx* c, Gate g);
+void disk_ke(dpmd_ctx* c);
+void fire_recompute_update(dpmd_ctx* c);
+void rigid_compute_forces(dpmd_ctx* c, int mode);
+void rigid_update_positions(dpmd_ctx* c, const double* scale, double scale2);
+void rigid_update_velocities(dpmd_ctx* c, const double* scale, double scale2);
+void rigid_scale_velocities(dpmd_ctx* c, const double* scale);
+void rigid_scale_positions(dpmd_ctx* c, const double* scale);
+void rigid_mix(dpmd_ctx* c, const double* alpha);
+void rigid_stress(dpmd_ctx* c);
+void rigid_ke(dpmd_ctx* c);
+void rigid_fire_drift(dpmd_ctx* c, Gate g);
+void rigid_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, double a, double b);
+void rigid_vv_drift(dpmd_ctx* c, Gate g);
+void rigid_vv_force(dpmd_ctx* c, Gate g);
 
 // reduce.hip
 enum RedKind { RED_PE = 0, RED_KE, RED_POWER, RED_VX, RED_VY, RED_PHI, RED_SXX, RED_SXY, RED_SYX, RED_SYY, RED_MAXF2 };

@@ -10,8 +10,9 @@ void rigid_scale_velocities(dpmd_ctx*, const double*) { no_rigid(); }
 void rigid_scale_positions(dpmd_ctx*, const double*) { no_rigid(); }
 void rigid_mix(dpmd_ctx*, const double*) { no_rigid(); }
 void rigid_stress(dpmd_ctx*) { no_rigid(); }
-void rigid_fire_drift(dpmd_ctx*, const int*) { no_rigid(); }
-void rigid_fire_force(dpmd_ctx*, const int*, int, int, int, double, double) { no_rigid(); }
-void rigid_vv_drift(dpmd_ctx*, const int*) { no_rigid(); }
-void rigid_vv_force(dpmd_ctx*, const int*) { no_rigid(); }
+void rigid_ke(dpmd_ctx*) { no_rigid(); }
+void rigid_fire_drift(dpmd_ctx*, Gate) { no_rigid(); }
+void rigid_fire_force(dpmd_ctx*, Gate, int, int, int, double, double) { no_rigid(); }
+void rigid_vv_drift(dpmd_ctx*, Gate) { no_rigid(); }
+void rigid_vv_force(dpmd_ctx*, Gate) { no_rigid(); }
 }  // namespace dpmd
