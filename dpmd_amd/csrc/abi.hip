@@ -227,9 +227,19 @@ void sync_impl(dpmd_ctx* c) {
     }
     c->ndof.upload(ndof.data(), S, c->st);
     box_inv(c);
+    // A repeated sync with an unchanged layout (the reference calls sync_box / sync_system /
+    // sync_cells / ... in sequence, and again after editing a field) keeps the neighbor
+    // state; any layout change invalidates it.
+    std::vector<int> sig = {S, N, c->Nv, c->method, c->K};
+    sig.insert(sig.end(), c->h_off.begin(), c->h_off.end());
+    auto cdx = dl(c, c->cdx), cdy = dl(c, c->cdy);
+    sig.insert(sig.end(), cdx.begin(), cdx.end());
+    sig.insert(sig.end(), cdy.begin(), cdy.end());
+    if (c->synced && c->nbr_ready && sig == c->layout_sig) return;
     build_chunks(c);
     if (c->method == DPMD_NEIGHBOR_CELL) setup_cells(c, CELLS_SYNC);
     alloc_obj_buffers(c);
+    c->layout_sig = sig;
     c->synced = true;
     c->nbr_ready = false;
 }
@@ -696,7 +706,7 @@ int dpmd_zero_velocities(dpmd_ctx* c) {
         if (rigid(c)) fill_f64(c->st, c->avel.p, c->N, 0.0);
     });
 }
-int dpmd_compute_ke(dpmd_ctx* c) { return guard(c, [&] { sys_reduce(c, RED_KE, c->ke_tot.p); }); }
+int dpmd_compute_ke(dpmd_ctx* c) { return guard(c, [&] { if (rigid(c)) rigid_ke(c); else disk_ke(c); }); }
 int dpmd_compute_ke_total(dpmd_ctx* c) { return guard(c, [&] { sys_reduce(c, RED_KE, c->ke_tot.p); }); }
 int dpmd_compute_pe_total(dpmd_ctx* c) { return guard(c, [&] { sys_reduce(c, RED_PE, c->pe_tot.p); }); }
 int dpmd_compute_fpower_total(dpmd_ctx* c) { return guard(c, [&] { sys_reduce(c, RED_POWER, c->pw_tot.p); }); }

@@ -104,6 +104,7 @@ struct dpmd_ctx {
     dpmd::DVec<double> pe_tot, ke_tot, pw_tot, press, temp, phi, maxf2;
     dpmd::DVec<double> st_xx, st_xy, st_yx, st_yy;
     std::vector<int> h_off, h_voff;
+    std::vector<int> layout_sig;
 
     // ---- per particle
     dpmd::DVec<double> x, y, vx, vy, fx, fy, pe, ke, rad, mass, area, lx, ly;

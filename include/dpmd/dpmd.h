@@ -4,7 +4,8 @@
  * This is the drop-in boundary for the reference's particle / integrator / routine API
  * (mmccraw98/dpmd).  Every entry point below replaces one reference call; the citation
  * on each names the reference interface (file:line) it stands in for.  The C++ facade in
- * include/dpmd/*.hpp re-exposes these under the reference's own class and method names.
+ * include/dpmd/{device_fields,particles,integrators,routines}.hpp re-expose these under the
+ * reference's own class and method names.
  *
  * Conventions
  *   - All entry points return 0 on success and a nonzero dpmd_status otherwise; the
@@ -156,7 +157,7 @@ int dpmd_rand_uniform_1d(dpmd_ctx* ctx, int field, uint64_t seed, uint64_t call,
 int dpmd_rand_normal_2d(dpmd_ctx* ctx, int field_x, int field_y, uint64_t seed, uint64_t call,
                         double mean_x, double sigma_x, double mean_y, double sigma_y);
 
-/* ---- integrators (include/integrators/*.cuh) ---------------------------------------- */
+/* ---- integrators (include/integrators/{velocity_verlet,fire}.cuh) ---------------------- */
 /* VelocityVerlet[Wall](P, dt).init() / .step() (velocity_verlet.cuh:17-47). */
 int dpmd_vv_init(dpmd_ctx* ctx, const double* dt, int mode);
 int dpmd_vv_step(dpmd_ctx* ctx, int n_steps);

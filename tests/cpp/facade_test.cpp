@@ -1,0 +1,95 @@
+// Drop-in check of the C++ facade: a driver written against the reference's API
+// (md::disk::Disk, df::DeviceField1D/2D, md::integrators::*, md::routines::minimize_fire),
+// in the call order of the reference's tests (tests/test_disk.cu:120-207,
+// tests/fire_disk_pbc.cu).  It writes the final canonical state to a binary file so the
+// test harness can compare it with the same sequence issued through the Python binding.
+//
+// usage: facade_test <out.bin>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+#include "dpmd/routines.hpp"
+
+int main(int argc, char** argv) {
+    if (argc != 2) { std::cerr << "usage: facade_test <out.bin>\n"; return 2; }
+    const int S = 2, n_per = 128, N = S * n_per;
+    const double phi = 0.80;
+    std::vector<double> rad(N), mass(N, 1.0), area(N);
+    for (int s = 0; s < S; ++s)
+        for (int k = 0; k < n_per; ++k) rad[s * n_per + k] = k < n_per / 2 ? 0.5 : 0.7;
+    double a_sys = 0.0;
+    for (int k = 0; k < n_per; ++k) a_sys += M_PI * rad[k] * rad[k];
+    for (int i = 0; i < N; ++i) area[i] = M_PI * rad[i] * rad[i];
+    const double L = std::sqrt(a_sys / phi);
+    std::vector<int> off = {0, n_per, 2 * n_per}, sizes = {n_per, n_per}, sid(N);
+    for (int i = 0; i < N; ++i) sid[i] = i / n_per;
+
+    md::disk::Disk P;
+    P.set_neighbor_method(md::NeighborMethod::Cell);
+    P.allocate_systems(S);
+    P.allocate_particles(N);
+    P.neighbor_cutoff.from_host(std::vector<double>(S, 0.3));
+    P.thresh2.from_host(std::vector<double>(S, 0.15 * 0.15));
+    P.system_id.from_host(sid);
+    P.system_size.from_host(sizes);
+    P.system_offset.from_host(off);
+    P.cell_dim.from_host(std::vector<int>(S, 0), std::vector<int>(S, 0));   // derived
+    P.box_size.from_host(std::vector<double>(S, L), std::vector<double>(S, L));
+    P.e_interaction.from_host(std::vector<double>(S, 1.0));
+    P.rad.from_host(rad);
+    P.mass.from_host(mass);
+    P.area.from_host(area);
+    P.set_rng_seed(11);
+    P.stateless_rand_uniform_pos(0.0, L, 0.0, L, 0);
+    P.sync_box();
+    P.sync_system();
+    P.sync_neighbors();
+    P.sync_cells();
+    P.sync_class_constants();
+    P.init_neighbors();
+
+    P.compute_forces();
+    P.compute_pe_total();
+    std::vector<double> pe0;
+    P.pe_total.to_host(pe0);
+
+    df::DeviceField1D<double> dt(S);
+    dt.fill(1e-2);
+    const int steps = md::routines::minimize_fire(P, dt, 300, 1e-16, 1e-16);
+
+    P.set_rng_seed(3);
+    P.stateless_rand_normal_vel(0.0, 1e-2, 0.0, 1e-2, 1);
+    P.set_average_velocity(0.0, 0.0);
+    P.set_temperature(1e-4);
+    md::integrators::VelocityVerlet<md::disk::Disk> vv(P, 1e-2);
+    vv.init();
+    for (int k = 0; k < 50; ++k) vv.step();
+    vv.run(50);
+    P.compute_pe_total();
+    P.compute_ke_total();
+    P.compute_pressure();
+
+    std::vector<double> x, y, vx, vy, pe, ke, pr;
+    P.pos.to_host(x, y);
+    P.vel.to_host(vx, vy);
+    P.pe_total.to_host(pe);
+    P.ke_total.to_host(ke);
+    P.pressure.to_host(pr);
+    std::vector<int> si;
+    P.static_index.to_host(si);
+    std::ofstream f(argv[1], std::ios::binary);
+    auto put = [&](const std::vector<double>& v, bool canon) {
+        for (int c = 0; c < (int)v.size(); ++c) {
+            const double val = canon ? v[si[c]] : v[c];
+            f.write((const char*)&val, sizeof(double));
+        }
+    };
+    put(x, true); put(y, true); put(vx, true); put(vy, true);
+    put(pe0, false); put(pe, false); put(ke, false); put(pr, false);
+    std::cout << "facade_test: fire_steps=" << steps << " n_neighbors=" << P.n_neighbors()
+              << " n_cells=" << P.n_cells() << " pe=" << pe[0] << "," << pe[1] << std::endl;
+    return 0;
+}

@@ -152,6 +152,47 @@ def test_vv_bit_identical_and_graph():
     assert_same(P, ref, what="VV 500 steps")
 
 
+def test_single_system_fire_then_nve_full_size():
+    """N=1024, phi=0.85, PBC: FIRE until max|F_i| < 1e-12, then 1e4 velocity-Verlet steps at
+    dt=1e-2 from Gaussian velocities at T=1e-4 with zero net momentum.  Bar: converged FIRE
+    energy and pressure within 1e-10 relative of the oracle (bitwise here), same step count."""
+    b = W.single_disk_system(1024, 0.85, seed=1)
+    P, ref = make(b)
+    steps, conv = run_fire(P, 1e-2, 200000, L.FIRE_PRED_MAX_FORCE, 1e-12, use_graph=True)
+    rsteps, rconv = ref.fire_minimize(1e-2, 200000, pred=1, ftol=1e-12)
+    assert conv and rconv and steps == rsteps
+    P.compute_pe_total(); P.compute_pressure(); P.compute_max_force()
+    pe, pr = P.get("PE_TOTAL"), P.get("PRESSURE")
+    np.testing.assert_allclose(pe, ref.pe_total(), rtol=1e-10, atol=0)
+    np.testing.assert_allclose(pr, ref.pressure(), rtol=1e-10, atol=0)
+    assert np.sqrt(P.get("MAX_FORCE2")[0]) < 1e-12
+    P.rand_normal_2d("VEL_X", "VEL_Y", 1, 1)
+    ref.set("vx", P.get("VEL_X")); ref.set("vy", P.get("VEL_Y"))
+    P.set_average_velocity(0.0, 0.0); ref.remove_mean_velocity()
+    P.set_temperature(1e-4); ref.set_temperature(1e-4)
+    vv = VelocityVerlet(P, 1e-2)
+    vv.init()
+    vv.run(10000, use_graph=True)
+    ref.vv_init(1e-2)
+    ref.vv_run(10000)
+    P.compute_pe_total(); P.compute_ke_total()
+    np.testing.assert_allclose(P.get("PE_TOTAL"), ref.pe_total(), rtol=1e-10, atol=0)
+    np.testing.assert_allclose(P.get("KE_TOTAL"), ref.ke_total(), rtol=1e-10, atol=0)
+    assert_same(P, ref, ("x", "y", "vx", "vy"), "10^4 NVE steps")
+
+
+def test_uneven_batch_mixed_boundaries_fire():
+    b = W.uneven_batch(S=16, nmin=64, nmax=600, seed=5)
+    P, ref = make(b)
+    steps, conv = run_fire(P, 1e-2, 20000, L.FIRE_PRED_MAX_FORCE, 1e-10, mode=L.FORCE_PER_SYSTEM, use_graph=True)
+    rsteps, rconv = ref.fire_minimize(1e-2, 20000, pred=1, ftol=1e-10, mode=2)
+    assert (steps, conv) == (rsteps, rconv)
+    assert_same(P, ref, ("x", "y", "fx", "fy", "pe"), "uneven batch FIRE")
+    gdt, _, _, gnb = FIRE(P, 1e-2).state()
+    rdt, _, _, rnb = ref.fire_state()
+    assert np.array_equal(gdt, rdt) and np.array_equal(gnb, rnb)
+
+
 def test_energies_pressure_temperature():
     b = batch_small()
     P, ref = make(b)
