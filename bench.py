@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the lean step batches directly instead of replaying hipGraphs "
+                         "(same kernels; for profilers that cannot follow graph launches)")
     return ap.parse_args()
 
 
@@ -89,7 +92,7 @@ def main():
     P.init_neighbors()
     # warmup on the same context (graph capture, code paths), then restore the input
     if args.warmup > 0:
-        run_fire(P, 1e-2, args.warmup, L.FIRE_PRED_MAX_FORCE, 1e-12, use_graph=True)
+        run_fire(P, 1e-2, args.warmup, L.FIRE_PRED_MAX_FORCE, 1e-12, use_graph=not args.no_graph)
     P.set("POS_X", b.x); P.set("POS_Y", b.y); P.set("RAD", b.rad); P.set("MASS", b.mass); P.set("AREA", b.area)
     P.sync()
     P.init_neighbors()
@@ -101,7 +104,7 @@ def main():
     n_prof = min(500, max(1, n_nve // 5))
     log(f"timed region: {n_fire} FIRE + {n_nve} NVE steps, N={N}")
     t0 = time.perf_counter()
-    steps_fire, _ = run_fire(P, 1e-2, n_fire, L.FIRE_PRED_MAX_FORCE, 1e-12, use_graph=True)
+    steps_fire, _ = run_fire(P, 1e-2, n_fire, L.FIRE_PRED_MAX_FORCE, 1e-12, use_graph=not args.no_graph)
     P.synchronize()
     t_fire = time.perf_counter() - t0
     log(f"FIRE done: {steps_fire} steps, {t_fire:.3f} s")
@@ -110,7 +113,7 @@ def main():
     P.set_temperature(1e-4)
     vv = VelocityVerlet(P, 1e-2)
     vv.init()
-    vv.run(n_nve - n_prof, use_graph=True)
+    vv.run(n_nve - n_prof, use_graph=not args.no_graph)
     lib.dpmd_profile(P.ctx, 1)
     vv.step(n_prof)
     P.synchronize()

@@ -119,4 +119,40 @@ __device__ __forceinline__ double min_image(double d, double L, double Linv) {
     return (L > 0.0) ? (d - L * rint(d * Linv)) : d;
 }
 
+// sin and cos of a rotation increment (reference: sincos in rigid_bumpy.cu:51).  Cody-Waite
+// reduction by pi/2 with the fdlibm two-part constant, Taylor series to x^17 / x^18 on
+// |r| <= pi/4 in Horner form; within 1 ulp of libm on the tested ranges.  Written out
+// (no library call) so host and device produce the same bits.
+__device__ __forceinline__ void own_sincos(double x, double* sp, double* cp) {
+    const double pio2_1 = 1.57079632673412561417e+00;
+    const double pio2_1t = 6.07710050650619224932e-11;
+    const double two_over_pi = 6.36619772367581382433e-01;
+    const double k = rint(x * two_over_pi);
+    const double r = (x - k * pio2_1) - k * pio2_1t;
+    const double r2 = r * r;
+    double ps = 1.0 / 355687428096000.0;
+    ps = -1.0 / 1307674368000.0 + r2 * ps;
+    ps = 1.0 / 6227020800.0 + r2 * ps;
+    ps = -1.0 / 39916800.0 + r2 * ps;
+    ps = 1.0 / 362880.0 + r2 * ps;
+    ps = -1.0 / 5040.0 + r2 * ps;
+    ps = 1.0 / 120.0 + r2 * ps;
+    ps = -1.0 / 6.0 + r2 * ps;
+    const double s = r + (r * r2) * ps;
+    double pc = -1.0 / 6402373705728000.0;
+    pc = 1.0 / 20922789888000.0 + r2 * pc;
+    pc = -1.0 / 87178291200.0 + r2 * pc;
+    pc = 1.0 / 479001600.0 + r2 * pc;
+    pc = -1.0 / 3628800.0 + r2 * pc;
+    pc = 1.0 / 40320.0 + r2 * pc;
+    pc = -1.0 / 720.0 + r2 * pc;
+    pc = 1.0 / 24.0 + r2 * pc;
+    const double c = (1.0 - 0.5 * r2) + (r2 * r2) * pc;
+    const long long q = ((long long)k) & 3;
+    if (q == 0) { *sp = s; *cp = c; }
+    else if (q == 1) { *sp = c; *cp = -s; }
+    else if (q == 2) { *sp = -s; *cp = -c; }
+    else { *sp = -c; *cp = s; }
+}
+
 }  // namespace dpmd

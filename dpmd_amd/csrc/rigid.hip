@@ -1,18 +1,419 @@
-// Rigid bumpy polygons (placeholder until the vertex kernels land).
+// Rigid bumpy polygons: vertex contact forces, the vertex -> centre-of-mass reduction
+// (force, torque, energy), rigid-body drift with incremental vertex rotation, kicks with the
+// moment of inertia, rotational FIRE mixing, and the fused FIRE / velocity-Verlet steps.
+//
+// Reference: src/particles/rigid_bumpy.cu (kernels :20-370, :507-629, :968-1127).  Particle
+// arrays keep canonical order; vertex arrays are reordered by the cell rebuild and reached
+// from a particle through static_index[particle_offset[i] + k], as in the reference.  The
+// rotation uses own_sincos (common.hpp) so host and device agree to the bit.
 #include "internal.hpp"
+#include "fire.hpp"
 
 namespace dpmd {
-[[noreturn]] static void no_rigid() { throw StateError("rigid bumpy kernels are not built yet"); }
-void rigid_compute_forces(dpmd_ctx*, int) { no_rigid(); }
-void rigid_update_positions(dpmd_ctx*, const double*, double) { no_rigid(); }
-void rigid_update_velocities(dpmd_ctx*, const double*, double) { no_rigid(); }
-void rigid_scale_velocities(dpmd_ctx*, const double*) { no_rigid(); }
-void rigid_scale_positions(dpmd_ctx*, const double*) { no_rigid(); }
-void rigid_mix(dpmd_ctx*, const double*) { no_rigid(); }
-void rigid_stress(dpmd_ctx*) { no_rigid(); }
-void rigid_ke(dpmd_ctx*) { no_rigid(); }
-void rigid_fire_drift(dpmd_ctx*, Gate) { no_rigid(); }
-void rigid_fire_force(dpmd_ctx*, Gate, int, int, int, double, double) { no_rigid(); }
-void rigid_vv_drift(dpmd_ctx*, Gate) { no_rigid(); }
-void rigid_vv_force(dpmd_ctx*, Gate) { no_rigid(); }
+namespace {
+
+struct RigidArgs {
+    // particles
+    double* x; double* y; double* th; double* vx; double* vy; double* w;
+    double* fx; double* fy; double* tq; double* pe;
+    const double* mass; const double* inertia;
+    const int* sid; const int* off; const int* poff; double* lx; double* ly;
+    int n;
+    // vertices
+    double* vpx; double* vpy; double* vfx; double* vfy; double* vpe; const double* vrad;
+    const int* vsid; const int* vstatic; const int* nn; const int* nbr; int nv;
+    // systems
+    const double* Lx; const double* Ly; const double* iLx; const double* iLy; const double* e;
+    const double* thresh2; const int* boundary; unsigned* flag; int* any;
+    int cell;
+};
+
+RigidArgs rigid_args(dpmd_ctx* c) {
+    RigidArgs a;
+    a.x = c->x.p; a.y = c->y.p; a.th = c->angle.p; a.vx = c->vx.p; a.vy = c->vy.p; a.w = c->avel.p;
+    a.fx = c->fx.p; a.fy = c->fy.p; a.tq = c->torque.p; a.pe = c->pe.p;
+    a.mass = c->mass.p; a.inertia = c->inertia.p; a.sid = c->sid.p; a.off = c->off.p; a.poff = c->poff.p;
+    a.lx = c->lx.p; a.ly = c->ly.p; a.n = c->N;
+    a.vpx = c->vpx.p; a.vpy = c->vpy.p; a.vfx = c->vfx.p; a.vfy = c->vfy.p; a.vpe = c->vpe.p; a.vrad = c->vrad.p;
+    a.vsid = c->vsid.p; a.vstatic = c->static_index.p; a.nn = c->nn.p; a.nbr = c->nbr.p; a.nv = c->Nv;
+    a.Lx = c->Lx.p; a.Ly = c->Ly.p; a.iLx = c->iLx.p; a.iLy = c->iLy.p; a.e = c->e.p;
+    a.thresh2 = c->thresh2.p; a.boundary = c->boundary.p; a.flag = c->flag.p; a.any = c->run.p + RUN_ANY;
+    a.cell = c->method == DPMD_NEIGHBOR_CELL;
+    return a;
+}
+
+// compute_pair_forces_kernel (rigid_bumpy.cu:164-221) / compute_wall_forces_kernel (:224-307);
+// in walled boxes the pair displacement is xi - xj (repulsive), the reference writes xj - xi.
+template <bool WALL>
+__device__ __forceinline__ void vertex_force_at(const RigidArgs& a, int i, int s) {
+    const double ei = a.e[s];
+    const double xi = a.vpx[i], yi = a.vpy[i], ri = a.vrad[i];
+    double fxi = 0.0, fyi = 0.0, pei = 0.0;
+    const double Lx = a.Lx[s], Ly = a.Ly[s];
+    if (WALL) {
+        if (xi < ri) { const double d = ri - xi; fxi += ei * d; pei += (0.5 * ei * d * d); }
+        if (xi > Lx - ri) { const double d = ri - (Lx - xi); fxi -= ei * d; pei += (0.5 * ei * d * d); }
+        if (yi < ri) { const double d = ri - yi; fyi += ei * d; pei += (0.5 * ei * d * d); }
+        if (yi > Ly - ri) { const double d = ri - (Ly - yi); fyi -= ei * d; pei += (0.5 * ei * d * d); }
+    }
+    const double iLx = a.iLx[s], iLy = a.iLy[s];
+    const int cnt = a.nn[i];
+    for (int k = 0; k < cnt; ++k) {
+        const int j = a.nbr[(size_t)k * a.nv + i];
+        double dx = xi - a.vpx[j], dy = yi - a.vpy[j];
+        if (!WALL) { dx = min_image(dx, Lx, iLx); dy = min_image(dy, Ly, iLy); }
+        const double r2 = dx * dx + dy * dy;
+        const double radsum = ri + a.vrad[j];
+        if (r2 >= radsum * radsum) continue;
+        const double r = sqrt(r2);
+        const double inv = 1.0 / r;
+        const double nx = dx * inv, ny = dy * inv;
+        const double delta = radsum - r;
+        const double fmag = ei * delta;
+        fxi += fmag * nx;
+        fyi += fmag * ny;
+        pei += (0.5 * ei * delta * delta) * 0.5;
+    }
+    a.vfx[i] = fxi; a.vfy[i] = fyi; a.vpe[i] = pei;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(DPMD_BLOCK) k_vertex_force(RigidArgs a, Gate g) {
+    if (!gate_force(g)) return;
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * DPMD_BLOCK + threadIdx.x;
+    if (i >= a.nv) return;
+    const int s = a.vsid[i];
+    if (MODE == DPMD_FORCE_PBC) vertex_force_at<false>(a, i, s);
+    else if (MODE == DPMD_FORCE_WALL) vertex_force_at<true>(a, i, s);
+    else if (a.boundary[s]) vertex_force_at<true>(a, i, s);
+    else vertex_force_at<false>(a, i, s);
+}
+
+// compute_particle_forces_kernel (rigid_bumpy.cu:331-370): canonical vertex order.
+__device__ __forceinline__ void particle_force_one(const RigidArgs& a, int i, double& fxo, double& fyo, double& to,
+                                                   double& peo) {
+    const double px = a.x[i], py = a.y[i];
+    double fxi = 0.0, fyi = 0.0, t = 0.0, p = 0.0;
+    for (int j = a.poff[i]; j < a.poff[i + 1]; ++j) {
+        const int v = a.vstatic[j];
+        const double fvx = a.vfx[v], fvy = a.vfy[v];
+        fxi += fvx;
+        fyi += fvy;
+        p += a.vpe[v];
+        const double dx = a.vpx[v] - px, dy = a.vpy[v] - py;
+        t += fvy * dx - fvx * dy;
+    }
+    fxo = fxi; fyo = fyi; to = t; peo = p;
+}
+
+__global__ void k_particle_force(RigidArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double fx, fy, t, p;
+    particle_force_one(a, i, fx, fy, t, p);
+    a.fx[i] = fx; a.fy[i] = fy; a.tq[i] = t; a.pe[i] = p;
+}
+
+// update_velocities_kernel (rigid_bumpy.cu:130-161) for one particle
+__device__ __forceinline__ void kick_one(const RigidArgs& a, int i, double sc, double fx, double fy, double t,
+                                         double& vx, double& vy, double& w) {
+    const double I = a.inertia[i];
+    const double smi = sc / a.mass[i];
+    const double sii = I != 0.0 ? sc / I : 0.0;
+    vx = vx + fx * smi;
+    vy = vy + fy * smi;
+    w = w + t * sii;
+}
+
+// update_positions_kernel (rigid_bumpy.cu:20-127) for one particle: COM and angle drift,
+// vertices rotated by the angle increment about the old COM; returns the skin test of the
+// first vertex.
+__device__ __forceinline__ bool drift_one(const RigidArgs& a, int i, int s, double sc, double& x, double& y,
+                                          double& th, double vx, double vy, double w) {
+    const double x0 = x, y0 = y, a0 = th;
+    const double nx = x0 + vx * sc, ny = y0 + vy * sc, na = a0 + w * sc;
+    x = nx; y = ny; th = na;
+    double sn, cs;
+    own_sincos(na - a0, &sn, &cs);
+    const int b = a.poff[i], e = a.poff[i + 1];
+    for (int j = b; j < e; ++j) {
+        const int v = a.vstatic[j];
+        const double dx = a.vpx[v] - x0, dy = a.vpy[v] - y0;
+        const double rx = cs * dx - sn * dy, ry = sn * dx + cs * dy;
+        a.vpx[v] = nx + rx;
+        a.vpy[v] = ny + ry;
+    }
+    if (!a.cell) return false;
+    const int fv = a.vstatic[b];
+    const double dx = a.vpx[fv] - a.lx[i], dy = a.vpy[fv] - a.ly[i];
+    return (dx * dx + dy * dy) > a.thresh2[s];
+}
+
+__global__ void k_update_positions(RigidArgs a, const double* __restrict__ scale, double scale2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < a.n;
+    const int s = valid ? a.sid[i] : -1;
+    bool flag = false;
+    if (valid) {
+        double x = a.x[i], y = a.y[i], th = a.th[i];
+        flag = drift_one(a, i, s, scale[s] * scale2, x, y, th, a.vx[i], a.vy[i], a.w[i]);
+        a.x[i] = x; a.y[i] = y; a.th[i] = th;
+    }
+    if (a.cell) raise_flags(a.flag, a.any, s, flag);
+}
+
+__global__ void k_update_velocities(RigidArgs a, const double* __restrict__ scale, double scale2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double vx = a.vx[i], vy = a.vy[i], w = a.w[i];
+    kick_one(a, i, scale[a.sid[i]] * scale2, a.fx[i], a.fy[i], a.tq[i], vx, vy, w);
+    a.vx[i] = vx; a.vy[i] = vy; a.w[i] = w;
+}
+
+__global__ void k_scale_velocities(RigidArgs a, const double* __restrict__ scale) {   // :540-553
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double sc = scale[a.sid[i]];
+    a.vx[i] *= sc; a.vy[i] *= sc; a.w[i] *= sc;
+}
+
+__global__ void k_scale_positions(RigidArgs a, const double* __restrict__ scale) {    // :555-583
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double px = a.x[i], py = a.y[i];
+    const double sc = scale[a.sid[i]];
+    const double nx = px * sc, ny = py * sc;
+    a.x[i] = nx; a.y[i] = ny;
+    for (int j = a.poff[i]; j < a.poff[i + 1]; ++j) {
+        const int v = a.vstatic[j];
+        a.vpx[v] += (nx - px);
+        a.vpy[v] += (ny - py);
+    }
+}
+
+// mix_velocities_and_forces_kernel (rigid_bumpy.cu:586-629): no-op when alpha == 0.
+__device__ __forceinline__ void mix_one(double& vx, double& vy, double& w, double fx, double fy, double t, double a) {
+    if (a == 0.0) return;
+    double vxi = vx, vyi = vy, wi = w;
+    const double fn = sqrt(fx * fx + fy * fy), vn = sqrt(vxi * vxi + vyi * vyi);
+    double ratio = 0.0;
+    if (fn > 1e-16) ratio = vn / fn * a;
+    else { vxi = 0.0; vyi = 0.0; }
+    const double tn = fabs(t), wn = fabs(wi);
+    double tratio = 0.0;
+    if (tn > 1e-16) tratio = wn / tn * a;
+    else { wi = 0.0; tratio = 0.0; }
+    vx = vxi * (1 - a) + fx * ratio;
+    vy = vyi * (1 - a) + fy * ratio;
+    w = wi * (1 - a) + t * tratio;
+}
+
+__global__ void k_mix(RigidArgs a, const double* __restrict__ alpha) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double vx = a.vx[i], vy = a.vy[i], w = a.w[i];
+    mix_one(vx, vy, w, a.fx[i], a.fy[i], a.tq[i], alpha[a.sid[i]]);
+    a.vx[i] = vx; a.vy[i] = vy; a.w[i] = w;
+}
+
+__global__ void k_ke(RigidArgs a, double* ke) {   // compute_ke_kernel (rigid_bumpy.cu:507-524)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double vx = a.vx[i], vy = a.vy[i], w = a.w[i];
+    ke[i] = 0.5 * ((vx * vx + vy * vy) * a.mass[i] + w * w * a.inertia[i]);
+}
+
+// compute_stress_tensor_virial_kernel + _kinetic_kernel (rigid_bumpy.cu:968-1068): vertex
+// force times COM separation / 2, summed per particle in canonical vertex order (the
+// reference adds them with atomics), plus m v v, over the box area.
+__global__ void k_stress(RigidArgs a, const int* vpid, double* sxx, double* sxy, double* syx, double* syy) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int s = a.sid[i];
+    const double ei = a.e[s], Lx = a.Lx[s], Ly = a.Ly[s], iLx = a.iLx[s], iLy = a.iLy[s];
+    const double xi = a.x[i], yi = a.y[i];
+    double axx = 0.0, axy = 0.0, ayx = 0.0, ayy = 0.0;
+    for (int jc = a.poff[i]; jc < a.poff[i + 1]; ++jc) {
+        const int v = a.vstatic[jc];
+        const double vx_ = a.vpx[v], vy_ = a.vpy[v], rv = a.vrad[v];
+        double vxx = 0.0, vxy = 0.0, vyx = 0.0, vyy = 0.0;
+        const int cnt = a.nn[v];
+        for (int k = 0; k < cnt; ++k) {
+            const int u = a.nbr[(size_t)k * a.nv + v];
+            const double dx = min_image(vx_ - a.vpx[u], Lx, iLx);
+            const double dy = min_image(vy_ - a.vpy[u], Ly, iLy);
+            const int pu = vpid[u];
+            const double dxp = min_image(xi - a.x[pu], Lx, iLx);
+            const double dyp = min_image(yi - a.y[pu], Ly, iLy);
+            const double r2 = dx * dx + dy * dy;
+            const double radsum = rv + a.vrad[u];
+            if (r2 >= radsum * radsum) continue;
+            const double r = sqrt(r2);
+            const double inv = 1.0 / r;
+            const double delta = radsum - r;
+            const double fmag = ei * delta;
+            const double fxv = fmag * (dx * inv), fyv = fmag * (dy * inv);
+            vxx += dxp * fxv / 2.0; vxy += dxp * fyv / 2.0;
+            vyx += dyp * fxv / 2.0; vyy += dyp * fyv / 2.0;
+        }
+        axx += vxx; axy += vxy; ayx += vyx; ayy += vyy;
+    }
+    const double area_box = Lx * Ly, m = a.mass[i], vx = a.vx[i], vy = a.vy[i];
+    sxx[i] = (axx + m * vx * vx) / area_box;
+    sxy[i] = (axy + m * vx * vy) / area_box;
+    syx[i] = (ayx + m * vy * vx) / area_box;
+    syy[i] = (ayy + m * vy * vy) / area_box;
+}
+
+// ---- fused FIRE (fire.cuh:59-88) ---------------------------------------------------------
+__global__ void __launch_bounds__(DPMD_BLOCK) k_fire_drift(RigidArgs a, FireArgs f, Gate g) {
+    if (!gate_open(g)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < a.n;
+    const int s = valid ? a.sid[i] : -1;
+    bool flag = false;
+    if (valid) {
+        const double dt = f.s_dt[s], alpha = f.s_alpha[s], dtrev = f.s_dtrev[s], vscale = f.s_vscale[s];
+        double x = a.x[i], y = a.y[i], th = a.th[i], vx = a.vx[i], vy = a.vy[i], w = a.w[i];
+        const double fx = a.fx[i], fy = a.fy[i], t = a.tq[i];
+        flag = drift_one(a, i, s, dtrev * 1.0, x, y, th, vx, vy, w);            // update_positions(dt_rev, 1)
+        vx *= vscale; vy *= vscale; w *= vscale;                                  // scale_velocities
+        kick_one(a, i, dt * 0.5, fx, fy, t, vx, vy, w);                           // update_velocities(dt, 0.5)
+        mix_one(vx, vy, w, fx, fy, t, alpha);                                     // mix
+        vx *= vscale; vy *= vscale; w *= vscale;                                  // scale_velocities
+        flag = drift_one(a, i, s, dt * 0.5, x, y, th, vx, vy, w) || flag;       // update_positions(dt, 0.5)
+        a.x[i] = x; a.y[i] = y; a.th[i] = th; a.vx[i] = vx; a.vy[i] = vy; a.w[i] = w;
+        if (i == a.off[s]) {
+            f.dt[s] = dt; f.alpha[s] = alpha; f.ngood[s] = f.s_ngood[s]; f.nbad[s] = f.s_nbad[s];
+        }
+    }
+    if (a.cell) raise_flags(a.flag, a.any, s, flag);
+}
+
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_fire_particle(RigidArgs a, FireArgs f, RedArgs r, int kick, Gate g) {
+    __shared__ double lds[4];
+    if (!gate_open(g)) return;
+    const int c = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = r.chunk_sys[c];
+    const int beg = r.chunk_beg[c];
+    const int end = min(beg + DPMD_BLOCK, a.off[s + 1]);
+    const int i = beg + threadIdx.x;
+    double vp = 0.0, vpe = 0.0, vf2 = 0.0;
+    if (i < end) {
+        double fx, fy, t, p;
+        particle_force_one(a, i, fx, fy, t, p);
+        a.fx[i] = fx; a.fy[i] = fy; a.tq[i] = t; a.pe[i] = p;
+        double vx = a.vx[i], vy = a.vy[i], w = a.w[i];
+        if (kick) {
+            kick_one(a, i, f.dt[s] * 0.5, fx, fy, t, vx, vy, w);
+            a.vx[i] = vx; a.vy[i] = vy; a.w[i] = w;
+        }
+        vp = fx * vx + fy * vy + t * w;
+        vpe = p;
+        vf2 = fx * fx + fy * fy;
+    }
+    vp = block_sum256(vp, lds);
+    vpe = block_sum256(vpe, lds);
+    vf2 = block_max256(vf2, lds);
+    if (threadIdx.x == 0) { r.part[c] = vp; r.part[r.nch + c] = vpe; r.part[2 * r.nch + c] = vf2; }
+}
+
+// ---- fused velocity Verlet (velocity_verlet.cuh:34-41) -----------------------------------
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_vv_drift(RigidArgs a, const double* __restrict__ dt, const double* __restrict__ dth, Gate g) {
+    if (!gate_open(g)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < a.n;
+    const int s = valid ? a.sid[i] : -1;
+    bool flag = false;
+    if (valid) {
+        double x = a.x[i], y = a.y[i], th = a.th[i], vx = a.vx[i], vy = a.vy[i], w = a.w[i];
+        kick_one(a, i, dth[s] * 1.0, a.fx[i], a.fy[i], a.tq[i], vx, vy, w);
+        flag = drift_one(a, i, s, dt[s] * 1.0, x, y, th, vx, vy, w);
+        a.x[i] = x; a.y[i] = y; a.th[i] = th; a.vx[i] = vx; a.vy[i] = vy; a.w[i] = w;
+    }
+    if (a.cell) raise_flags(a.flag, a.any, s, flag);
+}
+
+__global__ void __launch_bounds__(DPMD_BLOCK) k_vv_particle(RigidArgs a, const double* __restrict__ dth, Gate g) {
+    if (!gate_open(g)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double fx, fy, t, p;
+    particle_force_one(a, i, fx, fy, t, p);
+    a.fx[i] = fx; a.fy[i] = fy; a.tq[i] = t; a.pe[i] = p;
+    double vx = a.vx[i], vy = a.vy[i], w = a.w[i];
+    kick_one(a, i, dth[a.sid[i]] * 1.0, fx, fy, t, vx, vy, w);
+    a.vx[i] = vx; a.vy[i] = vy; a.w[i] = w;
+}
+
+void launch_vertex_force(dpmd_ctx* c, int mode, Gate g) {
+    RigidArgs a = rigid_args(c);
+    const int G = blocks_for(c->Nv);
+    ProfScope prof(c);
+    switch (mode) {
+        case DPMD_FORCE_PBC: k_vertex_force<DPMD_FORCE_PBC><<<G, DPMD_BLOCK, 0, c->st>>>(a, g); break;
+        case DPMD_FORCE_WALL: k_vertex_force<DPMD_FORCE_WALL><<<G, DPMD_BLOCK, 0, c->st>>>(a, g); break;
+        default: k_vertex_force<DPMD_FORCE_PER_SYSTEM><<<G, DPMD_BLOCK, 0, c->st>>>(a, g); break;
+    }
+    HIP_OK(hipGetLastError());
+}
+
+}  // namespace
+
+void rigid_compute_forces(dpmd_ctx* c, int mode) {
+    launch_vertex_force(c, mode, Gate{c->one.p, nullptr, 0});
+    k_particle_force<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c));
+    HIP_OK(hipGetLastError());
+}
+void rigid_update_positions(dpmd_ctx* c, const double* scale, double scale2) {
+    k_update_positions<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), scale, scale2);
+    HIP_OK(hipGetLastError());
+}
+void rigid_update_velocities(dpmd_ctx* c, const double* scale, double scale2) {
+    k_update_velocities<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), scale, scale2);
+    HIP_OK(hipGetLastError());
+}
+void rigid_scale_velocities(dpmd_ctx* c, const double* scale) {
+    k_scale_velocities<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), scale);
+    HIP_OK(hipGetLastError());
+}
+void rigid_scale_positions(dpmd_ctx* c, const double* scale) {
+    k_scale_positions<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), scale);
+    HIP_OK(hipGetLastError());
+}
+void rigid_mix(dpmd_ctx* c, const double* alpha) {
+    k_mix<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), alpha);
+    HIP_OK(hipGetLastError());
+}
+void rigid_ke(dpmd_ctx* c) {
+    k_ke<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->ke.p);
+    HIP_OK(hipGetLastError());
+}
+void rigid_stress(dpmd_ctx* c) {
+    k_stress<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vpid.p, c->sxx.p, c->sxy.p, c->syx.p,
+                                                        c->syy.p);
+    HIP_OK(hipGetLastError());
+}
+void rigid_fire_drift(dpmd_ctx* c, Gate g) {
+    k_fire_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), fire_args(c), g);
+    HIP_OK(hipGetLastError());
+}
+void rigid_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, double pa, double pb) {
+    launch_vertex_force(c, c->fire_mode, g);
+    k_fire_particle<<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(rigid_args(c), fire_args(c), red_args(c), kick, g);
+    HIP_OK(hipGetLastError());
+    fire_finalize(c, g, do_conv, pred, pa, pb);
+}
+void rigid_vv_drift(dpmd_ctx* c, Gate g) {
+    k_vv_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vv_dt.p, c->vv_dth.p, g);
+    HIP_OK(hipGetLastError());
+}
+void rigid_vv_force(dpmd_ctx* c, Gate g) {
+    launch_vertex_force(c, c->vv_mode, g);
+    k_vv_particle<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vv_dth.p, g);
+    HIP_OK(hipGetLastError());
+}
+
 }  // namespace dpmd

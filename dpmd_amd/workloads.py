@@ -149,3 +149,94 @@ def load_into(P, b: DiskBatch, method: int = 1) -> None:
     for name, v in (("POS_X", b.x), ("POS_Y", b.y), ("RAD", b.rad), ("MASS", b.mass), ("AREA", b.area)):
         P.set(name, v)
     P.sync()
+
+
+@dataclass
+class RigidBatch:
+    """Host description of S rigid bumpy-polygon systems (canonical particle / vertex order)."""
+    off: np.ndarray          # S+1 particle offsets
+    nvpp: np.ndarray         # vertices per particle
+    x: np.ndarray
+    y: np.ndarray
+    angle: np.ndarray
+    mass: np.ndarray
+    inertia: np.ndarray
+    area: np.ndarray
+    vx: np.ndarray           # vertex positions
+    vy: np.ndarray
+    vrad: np.ndarray
+    vmass: np.ndarray
+    Lx: np.ndarray
+    Ly: np.ndarray
+    e: np.ndarray
+    cutoff: np.ndarray
+    thresh2: np.ndarray
+    boundary: np.ndarray
+    cdx: np.ndarray
+    cdy: np.ndarray
+
+    @property
+    def S(self): return len(self.Lx)
+
+    @property
+    def N(self): return int(self.off[-1])
+
+    @property
+    def Nv(self): return int(np.sum(self.nvpp))
+
+
+def rigid_batch(S: int = 256, n_per: int = 256, phi: float = 0.70, nv_min: int = 12, nv_max: int = 26,
+                radii=(0.5, 0.7), seed: int = 1, k: float = 1.0, skin: float = 0.2, boundary: int = 1,
+                mass: float = 1.0) -> RigidBatch:
+    """Rigid bumpy polygons.  Per particle: n_v = nv_min + floor(u * (nv_max - nv_min + 1))
+    (Philox call 4), circle radius R = radii[0] for the first half of each system and radii[1]
+    for the second, orientation theta = 2 pi u (call 5).  Vertices at R (cos, sin)(theta + 2 pi
+    k / n_v) around the centre; bump radius = half the vertex spacing = R sin(pi / n_v);
+    vertex mass m / n_v, so I = sum m_v |r_v|^2 = m R^2; area of the union of polygon and
+    bumps = n_v R^2 sin(2 pi / n_v) / 2 + n_v r_b^2 (pi + 2 pi / n_v) / 2.  Square box of side
+    sqrt(sum(area) / phi); centres U[0, L)^2 (call 0)."""
+    N = S * n_per
+    un = uniform_1d(N, seed, 4)
+    nvpp = (nv_min + np.floor(un * (nv_max - nv_min + 1))).astype(np.int32)
+    R = np.empty(N)
+    for s in range(S):
+        R[s * n_per:s * n_per + n_per // 2] = radii[0]
+        R[s * n_per + n_per // 2:(s + 1) * n_per] = radii[1]
+    theta = 2.0 * math.pi * uniform_1d(N, seed, 5)
+    rb = R * np.sin(math.pi / nvpp)
+    area = 0.5 * nvpp * R * R * np.sin(2 * math.pi / nvpp) + 0.5 * nvpp * rb * rb * (math.pi + 2 * math.pi / nvpp)
+    L = np.array([math.sqrt(float(np.sum(area[s * n_per:(s + 1) * n_per])) / phi) for s in range(S)])
+    ux, uy = uniform_2d(N, seed, 0)
+    sid = np.repeat(np.arange(S), n_per)
+    x, y = L[sid] * ux, L[sid] * uy
+    pid = np.repeat(np.arange(N), nvpp)
+    kk = np.arange(int(nvpp.sum())) - np.repeat(np.concatenate([[0], np.cumsum(nvpp)[:-1]]), nvpp)
+    ang = theta[pid] + 2.0 * math.pi * kk / nvpp[pid]
+    vx = x[pid] + R[pid] * np.cos(ang)
+    vy = y[pid] + R[pid] * np.sin(ang)
+    off = np.arange(0, N + 1, n_per, dtype=np.int32)
+    return RigidBatch(off=off, nvpp=nvpp, x=x, y=y, angle=theta, mass=np.full(N, mass), inertia=mass * R * R,
+                      area=area, vx=vx, vy=vy, vrad=rb[pid], vmass=(mass / nvpp)[pid], Lx=L.copy(), Ly=L.copy(),
+                      e=np.full(S, k), cutoff=np.full(S, skin), thresh2=np.full(S, (0.5 * skin) ** 2),
+                      boundary=np.full(S, boundary, np.int32), cdx=np.zeros(S, np.int32), cdy=np.zeros(S, np.int32))
+
+
+def load_rigid_into(P, b: RigidBatch, method: int = 1) -> None:
+    """Allocate and fill a RigidBumpy context from a RigidBatch, then sync."""
+    P.set_neighbor_method(method)
+    P.allocate_systems(b.S)
+    P.allocate_particles(b.N)
+    P.allocate_vertices(b.Nv)
+    P.set("SYSTEM_OFFSET", b.off)
+    P.set("N_VERTICES_PER_PARTICLE", b.nvpp)
+    for name, v in (("BOX_SIZE_X", b.Lx), ("BOX_SIZE_Y", b.Ly), ("E_INTERACTION", b.e),
+                    ("NEIGHBOR_CUTOFF", b.cutoff), ("THRESH2", b.thresh2)):
+        P.set(name, v)
+    P.set("BOUNDARY", b.boundary)
+    P.set("CELL_DIM_X", b.cdx)
+    P.set("CELL_DIM_Y", b.cdy)
+    for name, v in (("POS_X", b.x), ("POS_Y", b.y), ("ANGLE", b.angle), ("MASS", b.mass),
+                    ("MOMENT_INERTIA", b.inertia), ("AREA", b.area), ("VERTEX_POS_X", b.vx),
+                    ("VERTEX_POS_Y", b.vy), ("VERTEX_RAD", b.vrad), ("VERTEX_MASS", b.vmass)):
+        P.set(name, v)
+    P.sync()

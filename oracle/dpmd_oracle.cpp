@@ -574,7 +574,467 @@ struct Disk {
 
 std::string g_err;
 
+// ------------------------------------------------------------------------------------
+// sin/cos for the rigid-body rotation (rigid_bumpy.cu:51, sincos of the angle increment).
+// Method: Cody-Waite reduction by pi/2 with the fdlibm two-part constant (pio2_1 = first 33
+// bits of pi/2, pio2_1t = the remainder), then Taylor series to x^17 / x^18 on
+// |r| <= pi/4, Horner form, no contraction.  The HIP library evaluates the identical
+// expression sequence, so both produce the same bits; accuracy is checked against libm.
+// ------------------------------------------------------------------------------------
+void own_sincos(double x, double* sp, double* cp) {
+    const double pio2_1 = 1.57079632673412561417e+00;
+    const double pio2_1t = 6.07710050650619224932e-11;
+    const double two_over_pi = 6.36619772367581382433e-01;
+    const double k = std::nearbyint(x * two_over_pi);
+    const double r = (x - k * pio2_1) - k * pio2_1t;
+    const double r2 = r * r;
+    double ps = 1.0 / 355687428096000.0;            // 1/17!
+    ps = -1.0 / 1307674368000.0 + r2 * ps;          // 1/15!
+    ps = 1.0 / 6227020800.0 + r2 * ps;              // 1/13!
+    ps = -1.0 / 39916800.0 + r2 * ps;               // 1/11!
+    ps = 1.0 / 362880.0 + r2 * ps;                  // 1/9!
+    ps = -1.0 / 5040.0 + r2 * ps;                   // 1/7!
+    ps = 1.0 / 120.0 + r2 * ps;                     // 1/5!
+    ps = -1.0 / 6.0 + r2 * ps;                      // 1/3!
+    const double s = r + (r * r2) * ps;
+    double pc = -1.0 / 6402373705728000.0;          // 1/18!
+    pc = 1.0 / 20922789888000.0 + r2 * pc;          // 1/16!
+    pc = -1.0 / 87178291200.0 + r2 * pc;            // 1/14!
+    pc = 1.0 / 479001600.0 + r2 * pc;               // 1/12!
+    pc = -1.0 / 3628800.0 + r2 * pc;                // 1/10!
+    pc = 1.0 / 40320.0 + r2 * pc;                   // 1/8!
+    pc = -1.0 / 720.0 + r2 * pc;                    // 1/6!
+    pc = 1.0 / 24.0 + r2 * pc;                      // 1/4!
+    const double c = (1.0 - 0.5 * r2) + (r2 * r2) * pc;
+    const long q = ((long)k) & 3;
+    if (q == 0) { *sp = s; *cp = c; }
+    else if (q == 1) { *sp = c; *cp = -s; }
+    else if (q == 2) { *sp = -s; *cp = -c; }
+    else { *sp = -c; *cp = s; }
+}
+
+// ------------------------------------------------------------------------------------
+// Rigid bumpy polygons -- src/particles/rigid_bumpy.cu, include/particles/base_poly_particle.hpp.
+// Particle arrays (COM, angle, velocities, forces) keep canonical order; vertex arrays are
+// reordered by the cell rebuild; static_index maps canonical vertex -> current vertex.
+// ------------------------------------------------------------------------------------
+struct Rigid {
+    int S = 0, N = 0, Nv = 0;
+    int method = 1;
+    std::vector<int> off, voff, poff, nx, ny, cstart, boundary, sid, vsid, vpid, static_index;
+    std::vector<double> Lx, Ly, ix, iy, e, cutoff, thresh2;
+    std::vector<double> x, y, th, vx, vy, w, fx, fy, tq, pe, mass, inertia, area, lastx, lasty;
+    std::vector<double> vpx, vpy, vfx, vfy, vpe, vrad, vmass;
+    std::vector<unsigned> flag;
+    std::vector<std::vector<int>> nbr;
+    int max_count = 0;
+    FireState fire;
+    std::vector<double> vv_dt, vv_dt_half;
+
+    void sync_box() {
+        ix.resize(S); iy.resize(S);
+        for (int s = 0; s < S; ++s) { ix[s] = Lx[s] > 0.0 ? 1.0 / Lx[s] : 0.0; iy[s] = Ly[s] > 0.0 ? 1.0 / Ly[s] : 0.0; }
+    }
+    void sync_cells() {
+        cstart.assign(S + 1, 0);
+        for (int s = 0; s < S; ++s) cstart[s + 1] = cstart[s] + nx[s] * ny[s];
+    }
+    void regrid_if_needed(bool box_changed) {
+        for (int s = 0; s < S; ++s) {
+            double rmax = 0.0;
+            for (int v = voff[s]; v < voff[s + 1]; ++v) rmax = std::max(rmax, vrad[v]);
+            const double need = 2.0 * rmax + cutoff[s];
+            if (nx[s] <= 0 || ny[s] <= 0 || (box_changed && (Lx[s] / nx[s] < need || Ly[s] / ny[s] < need))) {
+                nx[s] = std::max(3, (int)std::floor(Lx[s] / need));
+                ny[s] = std::max(3, (int)std::floor(Ly[s] / need));
+            }
+        }
+        sync_cells();
+    }
+    int hash_local(int v) const {
+        const int s = vsid[v];
+        const double u = wrap01(vpx[v] * ix[s]);
+        const double q = wrap01(vpy[v] * iy[s]);
+        int cx = (int)std::floor(u * nx[s]);
+        int cy = (int)std::floor(q * ny[s]);
+        cx = cx < nx[s] ? (cx > 0 ? cx : 0) : nx[s] - 1;
+        cy = cy < ny[s] ? (cy > 0 ? cy : 0) : ny[s] - 1;
+        return cx + cy * nx[s];
+    }
+    // vertex pipeline: base_poly_particle.hpp:239-260, base_poly_particle_kernels.cu:78-195,
+    // reorder rigid_bumpy.cu:1471-1492, reset_displacements_kernel :1070-1085
+    void rebuild_system(int s) {
+        const int b = voff[s], en = voff[s + 1], n = en - b;
+        const int C = nx[s] * ny[s];
+        std::vector<int> cid(n), cnt(C + 1, 0), order(n), inv(n);
+        for (int t = 0; t < n; ++t) { cid[t] = hash_local(b + t); cnt[cid[t] + 1]++; }
+        for (int c = 0; c < C; ++c) cnt[c + 1] += cnt[c];
+        std::vector<int> cs(cnt.begin(), cnt.end()), wp(cnt.begin(), cnt.end() - 1);
+        for (int t = 0; t < n; ++t) order[wp[cid[t]]++] = t;
+        for (int p = 0; p < n; ++p) inv[order[p]] = p;
+        auto gather = [&](auto& a) {
+            auto tmp = a;
+            for (int p = 0; p < n; ++p) tmp[b + p] = a[b + order[p]];
+            a = tmp;
+        };
+        gather(vpx); gather(vpy); gather(vfx); gather(vfy); gather(vmass); gather(vrad); gather(vpe); gather(vpid);
+        for (int c = b; c < en; ++c) static_index[c] = b + inv[static_index[c] - b];
+        std::vector<int> pcid(n);
+        for (int p = 0; p < n; ++p) pcid[p] = cid[order[p]];
+        for (int p = 0; p < n; ++p) {
+            const int i = b + p;
+            std::vector<int>& L = nbr[i];
+            L.clear();
+            const int loc = pcid[p];
+            const int cxi = loc % nx[s], cyi = loc / nx[s];
+            for (int dy = -1; dy <= 1; ++dy) {
+                int yy = cyi + dy; if (yy < 0) yy += ny[s]; else if (yy >= ny[s]) yy -= ny[s];
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int xx = cxi + dx; if (xx < 0) xx += nx[s]; else if (xx >= nx[s]) xx -= nx[s];
+                    const int c = yy * nx[s] + xx;
+                    for (int q = cs[c]; q < cs[c + 1]; ++q) {
+                        const int j = b + q;
+                        if (j == i || vpid[j] == vpid[i]) continue;
+                        double ddx, ddy;
+                        const double r2 = disp_pbc_L(vpx[i], vpy[i], vpx[j], vpy[j], Lx[s], Ly[s], ix[s], iy[s], ddx, ddy);
+                        const double cut = vrad[i] + vrad[j] + cutoff[s];
+                        if (r2 < cut * cut) L.push_back(j);
+                    }
+                }
+            }
+            max_count = std::max(max_count, (int)L.size());
+        }
+        for (int i = off[s]; i < off[s + 1]; ++i) {
+            const int fv = static_index[poff[i]];
+            lastx[i] = vpx[fv]; lasty[i] = vpy[fv];
+        }
+        flag[s] = 0u;
+    }
+    void check_neighbors(bool force) { for (int s = 0; s < S; ++s) if (force || flag[s]) rebuild_system(s); }
+
+    // rigid_bumpy.cu:164-221 (PBC) / :224-307 (walls; pair term repulsive xi - xj)
+    void vertex_force(int i, bool wall) {
+        const int s = vsid[i];
+        const double ei = e[s];
+        const double xi = vpx[i], yi = vpy[i], ri = vrad[i];
+        double fxi = 0.0, fyi = 0.0, pei = 0.0;
+        if (wall) {
+            const double bx = Lx[s], by = Ly[s];
+            if (xi < ri) { const double d = ri - xi; fxi += ei * d; pei += (0.5 * ei * d * d); }
+            if (xi > bx - ri) { const double d = ri - (bx - xi); fxi -= ei * d; pei += (0.5 * ei * d * d); }
+            if (yi < ri) { const double d = ri - yi; fyi += ei * d; pei += (0.5 * ei * d * d); }
+            if (yi > by - ri) { const double d = ri - (by - yi); fyi -= ei * d; pei += (0.5 * ei * d * d); }
+        }
+        for (int j : nbr[i]) {
+            double dx = xi - vpx[j], dy = yi - vpy[j];
+            if (!wall) {
+                dx = (Lx[s] > 0.0) ? (dx - Lx[s] * std::nearbyint(dx * ix[s])) : dx;
+                dy = (Ly[s] > 0.0) ? (dy - Ly[s] * std::nearbyint(dy * iy[s])) : dy;
+            }
+            const double r2 = dx * dx + dy * dy;
+            const double radsum = ri + vrad[j];
+            if (r2 >= radsum * radsum) continue;
+            const double r = std::sqrt(r2);
+            const double inv = 1.0 / r;
+            const double nxv = dx * inv, nyv = dy * inv;
+            const double delta = radsum - r;
+            const double fmag = ei * delta;
+            fxi += fmag * nxv;
+            fyi += fmag * nyv;
+            pei += (0.5 * ei * delta * delta) * 0.5;
+        }
+        vfx[i] = fxi; vfy[i] = fyi; vpe[i] = pei;
+    }
+    // compute_particle_forces_kernel (rigid_bumpy.cu:331-370)
+    void particle_force(int i) {
+        const double px = x[i], py = y[i];
+        double fxi = 0.0, fyi = 0.0, t = 0.0, p = 0.0;
+        for (int j = poff[i]; j < poff[i + 1]; ++j) {
+            const int v = static_index[j];
+            const double fvx = vfx[v], fvy = vfy[v];
+            fxi += fvx; fyi += fvy;
+            p += vpe[v];
+            const double dx = vpx[v] - px, dy = vpy[v] - py;
+            t += fvy * dx - fvx * dy;
+        }
+        fx[i] = fxi; fy[i] = fyi; tq[i] = t; pe[i] = p;
+    }
+    void compute_forces(int mode) {
+        for (int v = 0; v < Nv; ++v) vertex_force(v, mode == 1 || (mode == 2 && boundary[vsid[v]] != 0));
+        for (int i = 0; i < N; ++i) particle_force(i);
+    }
+    // update_positions_kernel_naive/_cell (rigid_bumpy.cu:20-127)
+    void update_positions(const double* scale, double scale2) {
+        for (int i = 0; i < N; ++i) {
+            const int s = sid[i];
+            const double sc = scale[s] * scale2;
+            const double x0 = x[i], y0 = y[i], a0 = th[i];
+            const double nx_ = x0 + vx[i] * sc, ny_ = y0 + vy[i] * sc, na = a0 + w[i] * sc;
+            x[i] = nx_; y[i] = ny_; th[i] = na;
+            double sn, cs;
+            own_sincos(na - a0, &sn, &cs);
+            for (int j = poff[i]; j < poff[i + 1]; ++j) {
+                const int v = static_index[j];
+                const double dx = vpx[v] - x0, dy = vpy[v] - y0;
+                const double rx = cs * dx - sn * dy, ry = sn * dx + cs * dy;
+                vpx[v] = nx_ + rx; vpy[v] = ny_ + ry;
+            }
+            if (method == 1) {
+                const int fv = static_index[poff[i]];
+                const double dx = vpx[fv] - lastx[i], dy = vpy[fv] - lasty[i];
+                if (dx * dx + dy * dy > thresh2[s]) flag[s] = 1u;
+            }
+        }
+    }
+    // update_velocities_kernel (rigid_bumpy.cu:130-161)
+    void update_velocities(const double* scale, double scale2) {
+        for (int i = 0; i < N; ++i) {
+            const double sc = scale[sid[i]] * scale2;
+            const double smi = sc / mass[i];
+            const double sii = inertia[i] != 0.0 ? sc / inertia[i] : 0.0;
+            vx[i] = vx[i] + fx[i] * smi; vy[i] = vy[i] + fy[i] * smi; w[i] = w[i] + tq[i] * sii;
+        }
+    }
+    void scale_velocities(const double* sc) {   // rigid_bumpy.cu:540-553
+        for (int i = 0; i < N; ++i) { vx[i] *= sc[sid[i]]; vy[i] *= sc[sid[i]]; w[i] *= sc[sid[i]]; }
+    }
+    void scale_positions(const double* sc) {    // rigid_bumpy.cu:555-583
+        for (int i = 0; i < N; ++i) {
+            const double px = x[i], py = y[i];
+            const double nx_ = px * sc[sid[i]], ny_ = py * sc[sid[i]];
+            x[i] = nx_; y[i] = ny_;
+            for (int j = poff[i]; j < poff[i + 1]; ++j) {
+                const int v = static_index[j];
+                vpx[v] += (nx_ - px); vpy[v] += (ny_ - py);
+            }
+        }
+    }
+    void mix(const double* alpha) {   // rigid_bumpy.cu:586-629
+        for (int i = 0; i < N; ++i) {
+            const double a = alpha[sid[i]];
+            if (a == 0.0) continue;
+            double vxi = vx[i], vyi = vy[i], wi = w[i];
+            const double fxi = fx[i], fyi = fy[i], ti = tq[i];
+            const double fn = std::sqrt(fxi * fxi + fyi * fyi), vn = std::sqrt(vxi * vxi + vyi * vyi);
+            double ratio = 0.0;
+            if (fn > 1e-16) ratio = vn / fn * a; else { vxi = 0.0; vyi = 0.0; }
+            const double tn = std::fabs(ti), wn = std::fabs(wi);
+            double tratio = 0.0;
+            if (tn > 1e-16) tratio = wn / tn * a; else { wi = 0.0; tratio = 0.0; }
+            vx[i] = vxi * (1 - a) + fxi * ratio;
+            vy[i] = vyi * (1 - a) + fyi * ratio;
+            w[i] = wi * (1 - a) + ti * tratio;
+        }
+    }
+    template <class Fn>
+    void sys_reduce(Fn f, double* out) {
+        std::vector<double> v(N);
+        for (int i = 0; i < N; ++i) v[i] = f(i);
+        for (int s = 0; s < S; ++s) out[s] = sys_sum(v.data() + off[s], off[s + 1] - off[s]);
+    }
+    void pe_total(double* o) { sys_reduce([&](int i) { return pe[i]; }, o); }
+    void ke_total(double* o) {
+        sys_reduce([&](int i) { return 0.5 * ((vx[i] * vx[i] + vy[i] * vy[i]) * mass[i] + w[i] * w[i] * inertia[i]); }, o);
+    }
+    void fpower_total(double* o) {
+        sys_reduce([&](int i) { return fx[i] * vx[i] + fy[i] * vy[i] + tq[i] * w[i]; }, o);
+    }
+    void max_force2(double* o) {
+        for (int s = 0; s < S; ++s) {
+            double m = 0.0;
+            for (int i = off[s]; i < off[s + 1]; ++i) m = std::max(m, fx[i] * fx[i] + fy[i] * fy[i]);
+            o[s] = m;
+        }
+    }
+    // compute_stress_tensor_virial_kernel + _kinetic_kernel (rigid_bumpy.cu:968-1068): per
+    // particle, vertex contributions summed in canonical vertex order (the reference adds
+    // them with atomics in arbitrary order).
+    void pressure(double* o) {
+        std::vector<double> sxx(N), syy(N);
+        for (int i = 0; i < N; ++i) {
+            const int s = sid[i];
+            const double ei = e[s];
+            double axx = 0.0, ayy = 0.0;
+            for (int jc = poff[i]; jc < poff[i + 1]; ++jc) {
+                const int v = static_index[jc];
+                double vxx = 0.0, vyy = 0.0;
+                for (int u : nbr[v]) {
+                    double dx, dy, dxp, dyp;
+                    const double r2 = disp_pbc_L(vpx[v], vpy[v], vpx[u], vpy[u], Lx[s], Ly[s], ix[s], iy[s], dx, dy);
+                    disp_pbc_L(x[i], y[i], x[vpid[u]], y[vpid[u]], Lx[s], Ly[s], ix[s], iy[s], dxp, dyp);
+                    const double radsum = vrad[v] + vrad[u];
+                    if (r2 >= radsum * radsum) continue;
+                    const double r = std::sqrt(r2);
+                    const double inv = 1.0 / r;
+                    const double delta = radsum - r;
+                    const double fmag = ei * delta;
+                    const double fxv = fmag * (dx * inv), fyv = fmag * (dy * inv);
+                    vxx += dxp * fxv / 2.0;
+                    vyy += dyp * fyv / 2.0;
+                }
+                axx += vxx; ayy += vyy;
+            }
+            const double area_box = Lx[s] * Ly[s];
+            sxx[i] = (axx + mass[i] * vx[i] * vx[i]) / area_box;
+            syy[i] = (ayy + mass[i] * vy[i] * vy[i]) / area_box;
+        }
+        for (int s = 0; s < S; ++s) {
+            const double a = sys_sum(sxx.data() + off[s], off[s + 1] - off[s]);
+            const double b = sys_sum(syy.data() + off[s], off[s + 1] - off[s]);
+            o[s] = 0.5 * (a + b);
+        }
+    }
+    void fire_init(const double* dt0, int mode) {
+        FireState& F = fire;
+        F.dt.assign(dt0, dt0 + S);
+        F.dt_max.resize(S); F.dt_min.resize(S);
+        for (int s = 0; s < S; ++s) { F.dt_max[s] = F.dt[s] * F.dt_max_scale; F.dt_min[s] = F.dt[s] * F.dt_min_scale; }
+        F.dt_rev.assign(S, 0.0); F.vscale.assign(S, 0.0); F.alpha.assign(S, F.alpha_init);
+        F.n_good.assign(S, 0); F.n_bad.assign(S, 0); F.pe_prev.assign(S, 1e9);
+        scale_velocities(F.vscale.data());
+        F.vscale.assign(S, 1.0);
+        compute_forces(mode);
+    }
+    void fire_step(int mode) {
+        std::vector<double> P(S);
+        fpower_total(P.data());
+        Disk d; d.S = S; d.fire = fire;
+        d.fire_update(P.data());
+        fire = d.fire;
+        update_positions(fire.dt_rev.data(), 1.0);
+        scale_velocities(fire.vscale.data());
+        update_velocities(fire.dt.data(), 0.5);
+        mix(fire.alpha.data());
+        scale_velocities(fire.vscale.data());
+        update_positions(fire.dt.data(), 0.5);
+        check_neighbors(false);
+        compute_forces(mode);
+        update_velocities(fire.dt.data(), 0.5);
+    }
+    int fire_converged(double diff_target, double pe_target, int pred, double ftol) {
+        FireState& F = fire;
+        std::vector<double> pt(S), mf(S);
+        pe_total(pt.data());
+        max_force2(mf.data());
+        bool all = true;
+        for (int s = 0; s < S; ++s) {
+            const int dof = off[s + 1] - off[s];
+            const double avg = pt[s] / dof, avg_prev = F.pe_prev[s] / dof;
+            const double diff = avg_prev == 0 ? 0 : std::abs(avg / avg_prev - 1);
+            if (pred == 0) { if (diff < diff_target || avg < pe_target) F.dt[s] = 0.0; }
+            else { if (std::sqrt(mf[s]) < ftol) F.dt[s] = 0.0; }
+            F.pe_prev[s] = avg;
+            if (F.dt[s] != 0.0) all = false;
+        }
+        return all ? 1 : 0;
+    }
+    void vv_init(const double* dt, int mode) {
+        vv_dt.assign(dt, dt + S);
+        vv_dt_half.resize(S);
+        for (int s = 0; s < S; ++s) vv_dt_half[s] = dt[s] * 0.5;
+        compute_forces(mode);
+    }
+    void vv_step(int mode) {
+        update_velocities(vv_dt_half.data(), 1.0);
+        update_positions(vv_dt.data(), 1.0);
+        check_neighbors(false);
+        compute_forces(mode);
+        update_velocities(vv_dt_half.data(), 1.0);
+    }
+};
+
 }  // namespace
+
+extern "C" {
+void or_sincos(double x, double* s, double* c) { own_sincos(x, s, c); }
+
+// Rigid constructor: per system: off (S+1 particle offsets); per particle: nvpp, x, y, angle,
+// mass, inertia, area; per vertex (canonical order): vpx, vpy, vrad, vmass.
+void* or_rigid_create(int S, int N, int Nv, const int* off, const double* Lx, const double* Ly, const double* e,
+                      const double* cutoff, const double* thresh2, const int* nx, const int* ny,
+                      const int* boundary, const int* nvpp, const double* x, const double* y, const double* th,
+                      const double* mass, const double* inertia, const double* area, const double* vpx,
+                      const double* vpy, const double* vrad, const double* vmass) {
+    Rigid* r = new Rigid();
+    r->S = S; r->N = N; r->Nv = Nv;
+    r->off.assign(off, off + S + 1);
+    r->Lx.assign(Lx, Lx + S); r->Ly.assign(Ly, Ly + S); r->e.assign(e, e + S);
+    r->cutoff.assign(cutoff, cutoff + S); r->thresh2.assign(thresh2, thresh2 + S);
+    r->nx.assign(nx, nx + S); r->ny.assign(ny, ny + S); r->boundary.assign(boundary, boundary + S);
+    r->x.assign(x, x + N); r->y.assign(y, y + N); r->th.assign(th, th + N);
+    r->mass.assign(mass, mass + N); r->inertia.assign(inertia, inertia + N); r->area.assign(area, area + N);
+    r->vx.assign(N, 0.0); r->vy.assign(N, 0.0); r->w.assign(N, 0.0);
+    r->fx.assign(N, 0.0); r->fy.assign(N, 0.0); r->tq.assign(N, 0.0); r->pe.assign(N, 0.0);
+    r->lastx.assign(N, 0.0); r->lasty.assign(N, 0.0);
+    r->vpx.assign(vpx, vpx + Nv); r->vpy.assign(vpy, vpy + Nv); r->vrad.assign(vrad, vrad + Nv);
+    r->vmass.assign(vmass, vmass + Nv);
+    r->vfx.assign(Nv, 0.0); r->vfy.assign(Nv, 0.0); r->vpe.assign(Nv, 0.0);
+    r->sid.resize(N); r->poff.assign(N + 1, 0); r->vpid.resize(Nv); r->vsid.resize(Nv);
+    for (int s = 0; s < S; ++s) for (int i = off[s]; i < off[s + 1]; ++i) r->sid[i] = s;
+    for (int i = 0; i < N; ++i) r->poff[i + 1] = r->poff[i] + nvpp[i];
+    for (int i = 0; i < N; ++i) for (int v = r->poff[i]; v < r->poff[i + 1]; ++v) { r->vpid[v] = i; r->vsid[v] = r->sid[i]; }
+    r->voff.assign(S + 1, 0);
+    for (int s = 0; s < S; ++s) r->voff[s + 1] = r->poff[off[s + 1]];
+    r->static_index.resize(Nv);
+    for (int v = 0; v < Nv; ++v) r->static_index[v] = v;
+    r->sync_box();
+    r->regrid_if_needed(false);
+    r->nbr.assign(Nv, {});
+    r->flag.assign(S, 0u);
+    r->check_neighbors(true);
+    return r;
+}
+void or_rigid_destroy(void* h) { delete (Rigid*)h; }
+// field ids: particle 0 x 1 y 2 angle 3 vx 4 vy 5 w 6 fx 7 fy 8 torque 9 pe; vertex 20 vpx 21 vpy
+// 22 vfx 23 vfy 24 vpe 25 vrad
+static std::vector<double>* rfield(Rigid* r, int f) {
+    switch (f) {
+        case 0: return &r->x; case 1: return &r->y; case 2: return &r->th; case 3: return &r->vx;
+        case 4: return &r->vy; case 5: return &r->w; case 6: return &r->fx; case 7: return &r->fy;
+        case 8: return &r->tq; case 9: return &r->pe;
+        case 20: return &r->vpx; case 21: return &r->vpy; case 22: return &r->vfx; case 23: return &r->vfy;
+        case 24: return &r->vpe; case 25: return &r->vrad;
+    }
+    return nullptr;
+}
+void or_rigid_get(void* h, int f, double* out) { auto* v = rfield((Rigid*)h, f); std::memcpy(out, v->data(), v->size() * 8); }
+void or_rigid_set(void* h, int f, const double* in) { auto* v = rfield((Rigid*)h, f); std::memcpy(v->data(), in, v->size() * 8); }
+void or_rigid_get_static_index(void* h, int* out) { Rigid* r = (Rigid*)h; std::memcpy(out, r->static_index.data(), r->Nv * 4); }
+void or_rigid_get_cells(void* h, int* nx, int* ny) { Rigid* r = (Rigid*)h; for (int s = 0; s < r->S; ++s) { nx[s] = r->nx[s]; ny[s] = r->ny[s]; } }
+int or_rigid_n_neighbors(void* h) { Rigid* r = (Rigid*)h; long t = 0; for (auto& L : r->nbr) t += L.size(); return (int)t; }
+int or_rigid_max_count(void* h) { return ((Rigid*)h)->max_count; }
+void or_rigid_get_neighbors(void* h, int* start, int* ids) {
+    Rigid* r = (Rigid*)h;
+    int w = 0;
+    for (int i = 0; i < r->Nv; ++i) { start[i] = w; for (int j : r->nbr[i]) ids[w++] = j; }
+    start[r->Nv] = w;
+}
+void or_rigid_check_neighbors(void* h, int force) { ((Rigid*)h)->check_neighbors(force != 0); }
+void or_rigid_compute_forces(void* h, int mode) { ((Rigid*)h)->compute_forces(mode); }
+void or_rigid_update_positions(void* h, const double* sc, double s2) { ((Rigid*)h)->update_positions(sc, s2); }
+void or_rigid_update_velocities(void* h, const double* sc, double s2) { ((Rigid*)h)->update_velocities(sc, s2); }
+void or_rigid_scale_velocities(void* h, const double* sc) { ((Rigid*)h)->scale_velocities(sc); }
+void or_rigid_scale_positions(void* h, const double* sc) { ((Rigid*)h)->scale_positions(sc); }
+void or_rigid_mix(void* h, const double* a) { ((Rigid*)h)->mix(a); }
+void or_rigid_pe_total(void* h, double* o) { ((Rigid*)h)->pe_total(o); }
+void or_rigid_ke_total(void* h, double* o) { ((Rigid*)h)->ke_total(o); }
+void or_rigid_fpower_total(void* h, double* o) { ((Rigid*)h)->fpower_total(o); }
+void or_rigid_pressure(void* h, double* o) { ((Rigid*)h)->pressure(o); }
+void or_rigid_max_force2(void* h, double* o) { ((Rigid*)h)->max_force2(o); }
+void or_rigid_fire_init(void* h, const double* dt, int mode) { ((Rigid*)h)->fire_init(dt, mode); }
+void or_rigid_fire_step(void* h, int mode) { ((Rigid*)h)->fire_step(mode); }
+int or_rigid_fire_converged(void* h, double d, double t, int pred, double ftol) { return ((Rigid*)h)->fire_converged(d, t, pred, ftol); }
+int or_rigid_fire_minimize(void* h, const double* dt, int max_steps, double d, double t, int pred, double ftol, int mode, int* conv) {
+    Rigid* r = (Rigid*)h;
+    r->fire_init(dt, mode);
+    int step = 0, c = 0;
+    while (step < max_steps && !c) { r->fire_step(mode); c = r->fire_converged(d, t, pred, ftol); ++step; }
+    *conv = c;
+    return step;
+}
+void or_rigid_vv_init(void* h, const double* dt, int mode) { ((Rigid*)h)->vv_init(dt, mode); }
+void or_rigid_vv_run(void* h, int n, int mode) { Rigid* r = (Rigid*)h; for (int k = 0; k < n; ++k) r->vv_step(mode); }
+}  // extern "C"
 
 // =====================================================================================
 //  extern "C" surface for the Python test harness (ctypes)

@@ -290,3 +290,151 @@ def fire_update_scalar(dt, alpha, ngood, nbad, power, dt_max, dt_min):
     dr = np.zeros(1); vs = np.zeros(1)
     lib().or_fire_update_scalar(_p(d), _p(a), _p(g), _p(b), power, dt_max, dt_min, _p(dr), _p(vs))
     return dict(dt=d[0], alpha=a[0], n_good=int(g[0]), n_bad=int(b[0]), dt_rev=dr[0], vscale=vs[0])
+
+
+_RPROTOS = {
+    "or_sincos": (None, [f64, vp, vp]),
+    "or_rigid_create": (vp, [i32, i32, i32] + [vp] * 20),
+    "or_rigid_destroy": (None, [vp]),
+    "or_rigid_get": (None, [vp, i32, vp]),
+    "or_rigid_set": (None, [vp, i32, vp]),
+    "or_rigid_get_static_index": (None, [vp, vp]),
+    "or_rigid_get_cells": (None, [vp, vp, vp]),
+    "or_rigid_n_neighbors": (i32, [vp]),
+    "or_rigid_max_count": (i32, [vp]),
+    "or_rigid_get_neighbors": (None, [vp, vp, vp]),
+    "or_rigid_check_neighbors": (None, [vp, i32]),
+    "or_rigid_compute_forces": (None, [vp, i32]),
+    "or_rigid_update_positions": (None, [vp, vp, f64]),
+    "or_rigid_update_velocities": (None, [vp, vp, f64]),
+    "or_rigid_scale_velocities": (None, [vp, vp]),
+    "or_rigid_scale_positions": (None, [vp, vp]),
+    "or_rigid_mix": (None, [vp, vp]),
+    "or_rigid_pe_total": (None, [vp, vp]),
+    "or_rigid_ke_total": (None, [vp, vp]),
+    "or_rigid_fpower_total": (None, [vp, vp]),
+    "or_rigid_pressure": (None, [vp, vp]),
+    "or_rigid_max_force2": (None, [vp, vp]),
+    "or_rigid_fire_init": (None, [vp, vp, i32]),
+    "or_rigid_fire_step": (None, [vp, i32]),
+    "or_rigid_fire_converged": (i32, [vp, f64, f64, i32, f64]),
+    "or_rigid_fire_minimize": (i32, [vp, vp, i32, f64, f64, i32, f64, i32, vp]),
+    "or_rigid_vv_init": (None, [vp, vp, i32]),
+    "or_rigid_vv_run": (None, [vp, i32, i32]),
+}
+_PROTOS.update(_RPROTOS)
+
+RFIELDS = dict(x=0, y=1, angle=2, vx=3, vy=4, w=5, fx=6, fy=7, torque=8, pe=9,
+               vpx=20, vpy=21, vfx=22, vfy=23, vpe=24, vrad=25)
+
+
+def sincos(x):
+    s, c = np.zeros(1), np.zeros(1)
+    lib().or_sincos(float(x), _p(s), _p(c))
+    return s[0], c[0]
+
+
+class OracleRigid:
+    """Host restatement of a batch of rigid bumpy-polygon systems (see dpmd_oracle.cpp)."""
+
+    def __init__(self, b):
+        self.S, self.N, self.Nv = b.S, b.N, b.Nv
+        ints = [np.ascontiguousarray(a, np.int32) for a in (b.off,)]
+        arrs = [ints[0], b.Lx, b.Ly, b.e, b.cutoff, b.thresh2, np.ascontiguousarray(b.cdx, np.int32),
+                np.ascontiguousarray(b.cdy, np.int32), np.ascontiguousarray(b.boundary, np.int32),
+                np.ascontiguousarray(b.nvpp, np.int32), b.x, b.y, b.angle, b.mass, b.inertia, b.area, b.vx, b.vy,
+                b.vrad, b.vmass]
+        arrs = [np.ascontiguousarray(a, a.dtype if a.dtype == np.int32 else np.float64) for a in arrs]
+        self._arrs = arrs
+        self.h = lib().or_rigid_create(self.S, self.N, self.Nv, *[_p(a) for a in arrs])
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().or_rigid_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def get(self, name):
+        n = self.Nv if RFIELDS[name] >= 20 else self.N
+        out = np.empty(n)
+        lib().or_rigid_get(self.h, RFIELDS[name], _p(out))
+        return out
+
+    def set(self, name, v):
+        v = np.ascontiguousarray(v, np.float64)
+        lib().or_rigid_set(self.h, RFIELDS[name], _p(v))
+
+    def static_index(self):
+        out = np.empty(self.Nv, np.int32)
+        lib().or_rigid_get_static_index(self.h, _p(out))
+        return out
+
+    def cells(self):
+        nx, ny = np.empty(self.S, np.int32), np.empty(self.S, np.int32)
+        lib().or_rigid_get_cells(self.h, _p(nx), _p(ny))
+        return nx, ny
+
+    def neighbors_csr(self):
+        tot = lib().or_rigid_n_neighbors(self.h)
+        start = np.empty(self.Nv + 1, np.int32)
+        ids = np.empty(max(1, tot), np.int32)
+        lib().or_rigid_get_neighbors(self.h, _p(start), _p(ids))
+        return start, ids[:tot]
+
+    def max_count(self): return lib().or_rigid_max_count(self.h)
+
+    def _ps(self, v, dt=np.float64):
+        a = np.asarray(v, dt)
+        if a.ndim == 0:
+            a = np.full(self.S, a, dt)
+        return np.ascontiguousarray(a)
+
+    def _per_sys(self, fn):
+        out = np.empty(self.S)
+        fn(self.h, _p(out))
+        return out
+
+    def check_neighbors(self, force=False): lib().or_rigid_check_neighbors(self.h, int(force))
+    def compute_forces(self, mode=0): lib().or_rigid_compute_forces(self.h, mode)
+
+    def update_positions(self, sc, s2=1.0):
+        a = self._ps(sc); lib().or_rigid_update_positions(self.h, _p(a), s2)
+
+    def update_velocities(self, sc, s2=1.0):
+        a = self._ps(sc); lib().or_rigid_update_velocities(self.h, _p(a), s2)
+
+    def scale_velocities(self, sc):
+        a = self._ps(sc); lib().or_rigid_scale_velocities(self.h, _p(a))
+
+    def scale_positions(self, sc):
+        a = self._ps(sc); lib().or_rigid_scale_positions(self.h, _p(a))
+
+    def mix(self, alpha):
+        a = self._ps(alpha); lib().or_rigid_mix(self.h, _p(a))
+
+    def pe_total(self): return self._per_sys(lib().or_rigid_pe_total)
+    def ke_total(self): return self._per_sys(lib().or_rigid_ke_total)
+    def fpower_total(self): return self._per_sys(lib().or_rigid_fpower_total)
+    def pressure(self): return self._per_sys(lib().or_rigid_pressure)
+    def max_force2(self): return self._per_sys(lib().or_rigid_max_force2)
+
+    def fire_init(self, dt, mode=0):
+        a = self._ps(dt); lib().or_rigid_fire_init(self.h, _p(a), mode)
+
+    def fire_step(self, mode=0): lib().or_rigid_fire_step(self.h, mode)
+
+    def fire_converged(self, diff, tgt, pred=0, ftol=0.0):
+        return bool(lib().or_rigid_fire_converged(self.h, diff, tgt, pred, ftol))
+
+    def fire_minimize(self, dt, max_steps, diff=0.0, tgt=0.0, pred=0, ftol=0.0, mode=0):
+        a = self._ps(dt)
+        conv = np.zeros(1, np.int32)
+        steps = lib().or_rigid_fire_minimize(self.h, _p(a), max_steps, diff, tgt, pred, ftol, mode, _p(conv))
+        return steps, bool(conv[0])
+
+    def vv_init(self, dt, mode=0):
+        a = self._ps(dt); lib().or_rigid_vv_init(self.h, _p(a), mode)
+
+    def vv_run(self, n, mode=0): lib().or_rigid_vv_run(self.h, n, mode)
