@@ -38,6 +38,11 @@ def parse():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--workload", choices=("disk1m", "jam512"), default="disk1m",
+                    help="disk1m: one system of 2^20 disks, FIRE + NVE (default); jam512: the "
+                         "512-system uneven batch jammed by compression, batched vs one context per system")
+    ap.add_argument("--plain-sample", type=int, default=16,
+                    help="jam512: systems timed on the plain (one context per system) path")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the lean step batches directly instead of replaying hipGraphs "
                          "(same kernels; for profilers that cannot follow graph launches)")
@@ -68,12 +73,121 @@ def cpu_baseline(n: int, steps: int):
                       f"{threads} OpenMP threads"}
 
 
+JAM = dict(dt=1e-2, dphi=1e-3, p_target=1e-7, max_comp=100, fire_max=50000, ftol=1e-10)
+
+
+def cpu_baseline_jam(b, systems: int, steps: int):
+    """Oracle on this host, bounded sample of the jam workload: `steps` FIRE steps of the first
+    `systems` systems of the batch (the per-step work of a jamming round)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+    from dpmd_amd import workloads as W
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    orc.lib().or_set_threads(threads)
+    sub = W.systems_of(b, 0, systems)
+    d = orc.OracleDisk(sub)
+    t0 = time.perf_counter()
+    d.fire_init(JAM["dt"], 2)
+    for _ in range(steps):
+        d.fire_step(2)
+        d.fire_converged(0.0, 0.0, 1, JAM["ftol"])
+    t = time.perf_counter() - t0
+    n = int(sub.off[-1])
+    return {"value": n * steps / t, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle, first {systems} systems of the batch ({n} disks): {steps} FIRE steps, "
+                      f"{t:.1f} s, {threads} OpenMP threads"}
+
+
+def run_jam512(args):
+    """512 uneven systems (N_s in [64, 4096], phi_s in [0.80, 0.86], even systems periodic, odd
+    walled) jammed by dphi = 1e-3 compressions + FIRE (max|F| < 1e-10 per round) until
+    P_s >= 1e-7.  Batched: one context, fused FIRE rounds in hipGraphs, frozen systems skipped.
+    Plain: one context per system (the same driver on a batch of one), timed on a sample."""
+    import numpy as np
+
+    from dpmd_amd import _lib as L
+    from dpmd_amd import workloads as W
+    from dpmd_amd.system import Disk, jam_compress
+
+    b = W.uneven_batch(S=512, seed=1)
+    sizes = np.diff(b.off)
+    kw = dict(predicate=L.FIRE_PRED_MAX_FORCE, a=JAM["ftol"], mode=L.FORCE_PER_SYSTEM, use_graph=not args.no_graph)
+
+    def load(bb):
+        P = Disk()
+        W.load_into(P, bb)
+        P.init_neighbors()
+        P.synchronize()
+        return P
+
+    if args.warmup > 0:   # code paths and graph instantiation on a small batch of one system
+        jam_compress(load(W.system_of(b, 0)), JAM["dt"], JAM["dphi"], JAM["p_target"], 2, args.warmup, **kw)
+    P = load(b)
+    log(f"batched: S={b.S} N={b.N}")
+    t0 = time.perf_counter()
+    st, nc, work = jam_compress(P, JAM["dt"], JAM["dphi"], JAM["p_target"], JAM["max_comp"], JAM["fire_max"], **kw)
+    P.synchronize()
+    t_b = time.perf_counter() - t0
+    steps = P.get("FIRE_STEPS")
+    P.compute_pe_total(); P.compute_pressure()
+    pe_b, pr_b = P.get("PE_TOTAL"), P.get("PRESSURE")
+    rate_b = work / t_b
+    log(f"batched done: {t_b:.2f} s, work {work:.3e} particle-steps, {rate_b:.3e}/s; "
+        f"jammed {int((st == 1).sum())}, dropped {int((st == -1).sum())}, max steps/system {int(steps.max())}")
+
+    # plain path on a deterministic sample spread over the size range
+    order = np.argsort(sizes, kind="stable")
+    sample = sorted(order[np.linspace(0, b.S - 1, args.plain_sample).astype(int)].tolist())
+    work_p, t_p, worst, worst_abs = 0, 0.0, 0.0, 0.0
+    for s in sample:
+        Q = load(W.system_of(b, s))
+        t1 = time.perf_counter()
+        qst, qnc, qwork = jam_compress(Q, JAM["dt"], JAM["dphi"], JAM["p_target"], JAM["max_comp"], JAM["fire_max"],
+                                       **kw)
+        Q.synchronize()
+        t_p += time.perf_counter() - t1
+        work_p += qwork
+        Q.compute_pe_total(); Q.compute_pressure()
+        for got, ref in ((Q.get("PE_TOTAL")[0], pe_b[s]), (Q.get("PRESSURE")[0], pr_b[s])):
+            if st[s] == 1:   # converged and jammed: relative bar
+                worst = max(worst, abs(got - ref) / max(abs(ref), 1e-300))
+            else:            # P ~ 0 from cancelling terms: absolute difference
+                worst_abs = max(worst_abs, abs(got - ref))
+        if qst[0] != st[s] or qnc[0] != nc[s]:
+            worst = max(worst, float("inf"))
+    rate_p = work_p / t_p
+    log(f"plain sample: {len(sample)} systems, {t_p:.2f} s, {rate_p:.3e}/s; max rel energy diff {worst:.1e}")
+    out = {
+        "metric": "particle-steps/sec (batched FIRE jamming, 512 uneven systems)",
+        "value": rate_b, "unit": "particle-steps/s", "n_gpus": 1, "steps": int(steps.sum()),
+        "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "512 uneven disk systems (N_s 64..4096, phi 0.80..0.86, half PBC half walls), "
+                               "jammed by 1e-3 compressions + FIRE (max|F|<1e-10 per round, cap 50000 steps) "
+                               "until P >= 1e-7",
+                   "n_particles": int(b.N), "batched_seconds": t_b, "work_particle_steps": int(work),
+                   "jammed": int((st == 1).sum()), "dropped": int((st == -1).sum()),
+                   "compressions_max": int(nc.max()), "fire_steps_max": int(steps.max()),
+                   "plain_sample_systems": len(sample), "plain_seconds": t_p, "plain_rate": rate_p,
+                   "batched_over_plain": rate_b / rate_p,
+                   "plain_vs_batched_max_rel_diff_jammed": worst,
+                   "plain_vs_batched_max_abs_diff_unjammed": worst_abs},
+        "roofline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_jam(b, 32, 3000)
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                                    "(systems shard across GPUs with no communication, see DESIGN.md)"}))
         sys.exit(2)
+    if args.workload == "jam512":
+        run_jam512(args)
+        return
     import numpy as np
 
     from dpmd_amd import _lib as L

@@ -30,7 +30,7 @@ F = dict(
     PACKING_FRACTION=44, MAX_FORCE2=45, STRESS_TOTAL_XX=46, STRESS_TOTAL_XY=47, STRESS_TOTAL_YX=48,
     STRESS_TOTAL_YY=49,
     SYSTEM_OFFSET=64, SYSTEM_SIZE=65, CELL_DIM_X=66, CELL_DIM_Y=67, CELL_SYSTEM_START=68, BOUNDARY=69,
-    N_DOF=70, REBUILD_FLAG=71,
+    N_DOF=70, REBUILD_FLAG=71, FIRE_STEPS=72,
     SYSTEM_ID=80, STATIC_INDEX=81, CELL_ID=82, NEIGHBOR_COUNT=83,
     VERTEX_POS_X=96, VERTEX_POS_Y=97, VERTEX_FORCE_X=98, VERTEX_FORCE_Y=99, VERTEX_PE=100,
     VERTEX_RAD=101, VERTEX_MASS=102,
@@ -97,6 +97,8 @@ PROTOS = {
     "dpmd_fire_get_dt": (i32, [vp, vp]),
     "dpmd_fire_state": (i32, [vp, vp, vp, vp, vp]),
     "dpmd_fire_minimize": (i32, [vp, vp, i32, i32, f64, f64, i32, i32, i32, pi32, pi32]),
+    "dpmd_jam_compress": (i32, [vp, vp, f64, f64, i32, i32, i32, f64, f64, i32, i32, pi32, pi32,
+                                C.POINTER(C.c_int64)]),
     "dpmd_malloc": (i32, [C.POINTER(vp), i64]),
     "dpmd_free": (i32, [vp]),
     "dpmd_memcpy": (i32, [vp, vp, i64, i32]),

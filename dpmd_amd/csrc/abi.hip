@@ -92,6 +92,7 @@ FieldRef field_ref(dpmd_ctx* c, int f) {
         case DPMD_F_BOUNDARY: return ref(c->boundary, 2);
         case DPMD_F_N_DOF: return ref(c->ndof, 2);
         case DPMD_F_REBUILD_FLAG: return ref(c->flag, 2);
+        case DPMD_F_FIRE_STEPS: return ref(c->f_nsteps, 2);
         case DPMD_F_SYSTEM_ID: return ref(c->sid, 0);
         case DPMD_F_STATIC_INDEX: return ref(c->static_index, c->kind == DPMD_RIGID_BUMPY ? 1 : 0);
         case DPMD_F_CELL_ID: return ref(c->cell_id, c->kind == DPMD_RIGID_BUMPY ? 1 : 0);
@@ -305,6 +306,22 @@ void set_average_velocity_impl(dpmd_ctx* c, const double* ax, const double* ay) 
     subtract_mean_velocity(c, c->scr_a.p, c->scr_b.p, ax, ay);
 }
 
+// increment_packing_fraction (base_particle.hpp:505-529): s = sqrt(phi / (phi + inc)) per
+// system scales box and positions, then cells and lists are rebuilt.
+void increment_packing_fraction_impl(dpmd_ctx* c, const double* inc) {
+    sys_reduce(c, RED_PHI, c->phi.p);
+    scale_box_increment(c, inc, c->scr_c.p);
+    scale_positions_impl(c, c->scr_c.p);
+    box_inv(c);
+    if (c->method == DPMD_NEIGHBOR_CELL) {
+        setup_cells(c, CELLS_BOX_CHANGED);
+        alloc_obj_buffers(c);
+        // the cell grid may have changed: every system is rebuilt below
+    }
+    check_neighbors_impl(c, true);
+    sys_reduce(c, RED_PHI, c->phi.p);
+}
+
 void fire_init_impl(dpmd_ctx* c, const double* dt, int mode) {
     need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
     const int S = c->S;
@@ -323,6 +340,8 @@ void fire_init_impl(dpmd_ctx* c, const double* dt, int mode) {
     c->f_alpha.upload(al.data(), S, c->st); c->f_peprev.upload(pp.data(), S, c->st);
     c->f_ngood.upload(z.data(), S, c->st); c->f_nbad.upload(z.data(), S, c->st);
     c->f_stopped.upload(stopped.data(), S, c->st);
+    c->f_frozen.upload(stopped.data(), S, c->st);   // init zeroes v: a stopped system is final
+    c->f_nsteps.upload(z.data(), S, c->st);
     int run[8] = {0, 0, running, 1, 0, -1, -1, 0};
     c->run.upload(run, 8, c->st);
     c->fire_mode = mode;
@@ -359,6 +378,13 @@ void vv_step_body(dpmd_ctx* c) {
 // ---------------------------------------------------------------------------------------
 struct RunCtl {
     int running, halt, halt_at, conv_at;
+};
+
+// Turns on frozen-system skipping for the duration of one fused FIRE run.
+struct FrozenMask {
+    dpmd_ctx* c;
+    explicit FrozenMask(dpmd_ctx* ctx) : c(ctx) { c->mask_frozen = true; }
+    ~FrozenMask() { c->mask_frozen = false; }
 };
 
 RunCtl read_run(dpmd_ctx* c) {
@@ -507,7 +533,7 @@ int dpmd_ctx_destroy(dpmd_ctx* c) {
                              &c->cell_count, &c->cell_start, &c->cell_wptr, &c->cell_sys, &c->order,
                              &c->order_inv, &c->tmp_i, &c->nn, &c->nbr, &c->overflow, &c->max_count,
                              &c->scan_part, &c->chunk_sys, &c->chunk_beg, &c->sys_chunk, &c->f_ngood,
-                             &c->f_nbad, &c->f_stopped, &c->s_ngood, &c->s_nbad, &c->run, &c->one, &c->sv_static};
+                             &c->f_nbad, &c->f_stopped, &c->f_frozen, &c->f_nsteps, &c->s_ngood, &c->s_nbad, &c->run, &c->one, &c->sv_static};
     for (auto* v : iv) v->release();
     c->flag.release();
     c->sys_counter.release();
@@ -526,7 +552,7 @@ int dpmd_allocate_systems(dpmd_ctx* c, int S) {
                                     &c->s_alpha, &c->s_dtrev, &c->s_vscale, &c->vv_dt, &c->vv_dth};
         for (auto* v : dv) { v->resize(S); v->zero(c->st); }
         dpmd::DVec<int>* iv[] = {&c->size, &c->cdx, &c->cdy, &c->boundary, &c->ndof, &c->f_ngood, &c->f_nbad,
-                                 &c->f_stopped, &c->s_ngood, &c->s_nbad};
+                                 &c->f_stopped, &c->f_frozen, &c->f_nsteps, &c->s_ngood, &c->s_nbad};
         for (auto* v : iv) { v->resize(S); v->zero(c->st); }
         c->off.resize(S + 1); c->off.zero(c->st);
         c->cstart.resize(S + 1); c->cstart.zero(c->st);
@@ -738,19 +764,7 @@ int dpmd_set_temperature(dpmd_ctx* c, const double* T, int rm) {
     });
 }
 int dpmd_increment_packing_fraction(dpmd_ctx* c, const double* inc) {
-    return guard(c, [&] {
-        sys_reduce(c, RED_PHI, c->phi.p);
-        scale_box_increment(c, inc, c->scr_c.p);
-        scale_positions_impl(c, c->scr_c.p);
-        box_inv(c);
-        if (c->method == DPMD_NEIGHBOR_CELL) {
-            setup_cells(c, CELLS_BOX_CHANGED);
-            alloc_obj_buffers(c);
-            // the cell grid may have changed: every system is rebuilt below
-        }
-        check_neighbors_impl(c, true);
-        sys_reduce(c, RED_PHI, c->phi.p);
-    });
+    return guard(c, [&] { increment_packing_fraction_impl(c, inc); });
 }
 int dpmd_save_state(dpmd_ctx* c, const int* flag, int tv) {
     return guard(c, [&] {
@@ -888,10 +902,74 @@ int dpmd_fire_minimize(dpmd_ctx* c, const double* dt, int max_steps, int pred, d
         (void)poll;   // batches adapt to the rebuild rate; the host reads state once per batch
         int conv = 0;
         const int running0 = read_run(c).running;
-        const int steps = running0 > 0 ? lean_run(c, max_steps, true, pred, a, b, use_graph != 0, &conv) : 0;
+        int steps = 0;
+        if (running0 > 0) {
+            FrozenMask mask(c);
+            steps = lean_run(c, max_steps, true, pred, a, b, use_graph != 0, &conv);
+        }
         check_overflow(c);
         if (steps_taken) *steps_taken = steps;
         if (all_conv) *all_conv = running0 > 0 ? conv : 1;
+    });
+}
+
+int dpmd_jam_compress(dpmd_ctx* c, const double* dt, double dphi, double p_target, int max_comp, int fire_max,
+                      int pred, double a, double b, int mode, int use_graph, int* jam_state, int* n_comp,
+                      int64_t* work) {
+    return guard(c, [&] {
+        need(pred == DPMD_FIRE_PRED_REFERENCE || pred == DPMD_FIRE_PRED_MAX_FORCE, "unknown FIRE predicate");
+        need(dt != nullptr && max_comp >= 0 && fire_max > 0, "jam_compress: bad arguments");
+        need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
+        const int S = c->S;
+        std::vector<double> dt0(S), dts(S), inc(S), P(S);
+        HIP_OK(hipMemcpyAsync(dt0.data(), dt, S * 8, hipMemcpyDeviceToHost, c->st));
+        HIP_OK(hipStreamSynchronize(c->st));
+        // state: 0 compressing, 1 jammed, 2 budget used up, 3 not run, -1 dropped
+        std::vector<int> state(S), ncomp(S, 0), nsteps(S, 0), stopped(S), ns(S);
+        for (int s = 0; s < S; ++s) state[s] = dt0[s] == 0.0 ? 3 : 0;
+        int64_t total = 0;
+        for (;;) {
+            bool any = false;
+            for (int s = 0; s < S; ++s) {
+                dts[s] = state[s] == 0 ? dt0[s] : 0.0;
+                any = any || dts[s] != 0.0;
+            }
+            if (!any) break;
+            c->scr_a.upload(dts.data(), S, c->st);
+            fire_init_impl(c, c->scr_a.p, mode);
+            {
+                FrozenMask mask(c);
+                (void)lean_run(c, fire_max, true, pred, a, b, use_graph != 0, nullptr);
+            }
+            check_overflow(c);
+            if (rigid(c)) rigid_stress(c); else disk_stress(c);
+            per_system_pressure(c);
+            HIP_OK(hipMemcpyAsync(stopped.data(), c->f_stopped.p, S * 4, hipMemcpyDeviceToHost, c->st));
+            HIP_OK(hipMemcpyAsync(ns.data(), c->f_nsteps.p, S * 4, hipMemcpyDeviceToHost, c->st));
+            HIP_OK(hipMemcpyAsync(P.data(), c->press.p, S * 8, hipMemcpyDeviceToHost, c->st));
+            HIP_OK(hipStreamSynchronize(c->st));
+            bool grow = false;
+            for (int s = 0; s < S; ++s) {
+                inc[s] = 0.0;
+                if (dts[s] == 0.0) continue;
+                nsteps[s] += ns[s];
+                total += (int64_t)(c->h_off[s + 1] - c->h_off[s]) * ns[s];
+                if (!stopped[s]) state[s] = -1;
+                else if (P[s] >= p_target) state[s] = 1;
+                else if (ncomp[s] >= max_comp) state[s] = 2;
+                else { inc[s] = dphi; ncomp[s] += 1; grow = true; }
+            }
+            if (!grow) break;
+            c->scr_b.upload(inc.data(), S, c->st);
+            increment_packing_fraction_impl(c, c->scr_b.p);
+        }
+        c->f_nsteps.upload(nsteps.data(), S, c->st);
+        HIP_OK(hipStreamSynchronize(c->st));
+        for (int s = 0; s < S; ++s) {
+            if (jam_state) jam_state[s] = state[s] == 1 ? 1 : state[s] == -1 ? -1 : 0;
+            if (n_comp) n_comp[s] = ncomp[s];
+        }
+        if (work) *work = total;
     });
 }
 

@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_fire_drift(DiskArgs a, FireArgs 
     const bool valid = i < a.n;
     const int s = valid ? a.sid[i] : -1;
     bool flag = false;
-    if (valid) flag = fire_drift_one(a, f, i, s);
+    if (valid && !fire_frozen(f, s)) flag = fire_drift_one(a, f, i, s);
     if (a.cell) raise_flags(a.flag, a.any, s, flag);
 }
 
@@ -250,6 +250,7 @@ k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, Gate g) {
     if (!gate_force(g)) return;
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     const int s = r.chunk_sys[c];
+    if (fire_frozen(f, s)) return;   // whole workgroup: chunks never straddle systems
     const int beg = r.chunk_beg[c];
     const int end = min(beg + DPMD_BLOCK, a.off[s + 1]);
     const int i = beg + threadIdx.x;
@@ -285,6 +286,7 @@ k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_updat
     __shared__ double lds[4];
     if (!gate_open(g)) return;
     const int s = blockIdx.x;
+    if (fire_frozen(f, s)) return;
     const int cb = r.sys_chunk[s], ce = r.sys_chunk[s + 1];
     double ap = 0.0, ape = 0.0, af = 0.0;
     for (int k = cb + threadIdx.x; k < ce; k += DPMD_BLOCK) {
@@ -296,10 +298,15 @@ k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_updat
     ape = block_sum256(ape, lds);
     af = block_max256(af, lds);
     if (threadIdx.x == 0) {
+        // A system that had stopped before this step has now had its velocities zeroed
+        // (vscale = 0 after a stop), so every later step leaves it unchanged: freeze it.
+        const int was_stopped = f.stopped[s];
         r.pw_tot[s] = ap; r.pe_tot[s] = ape; r.maxf2[s] = af;
         if (do_conv) fire_convergence_sys(f, off, s, ape, af, pred, pa, pb);
         if (do_conv) fire_count_stopped(f, s, g.k);
         if (do_update) fire_update_sys(f, s, ap);
+        if (f.frozen && was_stopped) f.frozen[s] = 1;
+        if (do_conv) f.nsteps[s] += 1;
     }
 }
 

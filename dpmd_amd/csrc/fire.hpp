@@ -8,6 +8,8 @@ namespace dpmd {
 struct FireArgs {
     double* dt; double* dtmax; double* dtmin; double* alpha; double* peprev;
     int* ngood; int* nbad; int* stopped;
+    int* frozen;    // null unless the run skips frozen systems
+    int* nsteps;    // fused steps that did work, per system
     double* s_dt; double* s_alpha; double* s_dtrev; double* s_vscale; int* s_ngood; int* s_nbad;
     int* run;
 };
@@ -23,6 +25,8 @@ inline FireArgs fire_args(dpmd_ctx* c) {
     FireArgs f;
     f.dt = c->f_dt.p; f.dtmax = c->f_dtmax.p; f.dtmin = c->f_dtmin.p; f.alpha = c->f_alpha.p;
     f.peprev = c->f_peprev.p; f.ngood = c->f_ngood.p; f.nbad = c->f_nbad.p; f.stopped = c->f_stopped.p;
+    f.frozen = c->mask_frozen ? c->f_frozen.p : nullptr;
+    f.nsteps = c->f_nsteps.p;
     f.s_dt = c->s_dt.p; f.s_alpha = c->s_alpha.p; f.s_dtrev = c->s_dtrev.p; f.s_vscale = c->s_vscale.p;
     f.s_ngood = c->s_ngood.p; f.s_nbad = c->s_nbad.p; f.run = c->run.p;
     return f;
@@ -73,6 +77,8 @@ __device__ __forceinline__ void fire_convergence_sys(const FireArgs& f, const in
     }
     f.peprev[s] = avg;
 }
+
+__device__ __forceinline__ bool fire_frozen(const FireArgs& f, int s) { return f.frozen && f.frozen[s]; }
 
 __device__ __forceinline__ void fire_count_stopped(const FireArgs& f, int s, int k) {
     if (f.dt[s] == 0.0 && !f.stopped[s]) {

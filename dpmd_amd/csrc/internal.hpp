@@ -131,6 +131,11 @@ struct dpmd_ctx {
     // ---- FIRE (fire.cuh:116-147): committed and next-step ("shadow") scalars
     dpmd::DVec<double> f_dt, f_dtmax, f_dtmin, f_alpha, f_peprev;
     dpmd::DVec<int> f_ngood, f_nbad, f_stopped;
+    // frozen[s] = 1: system s stopped and its velocities were zeroed by the following step,
+    // so further FIRE steps leave its state bit-identical; fused runs skip its work.
+    dpmd::DVec<int> f_frozen;
+    dpmd::DVec<int> f_nsteps;   // fused steps that did work, per system
+    bool mask_frozen = false;   // honoured only inside fused minimisation runs
     dpmd::DVec<double> s_dt, s_alpha, s_dtrev, s_vscale;
     dpmd::DVec<int> s_ngood, s_nbad;
     int fire_mode = 0; bool fire_ready = false;

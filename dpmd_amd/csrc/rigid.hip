@@ -26,6 +26,7 @@ struct RigidArgs {
     const double* Lx; const double* Ly; const double* iLx; const double* iLy; const double* e;
     const double* thresh2; const int* boundary; unsigned* flag; int* any;
     int cell;
+    const int* frozen;   // systems whose FIRE state is final (skipped); null outside fused runs
 };
 
 RigidArgs rigid_args(dpmd_ctx* c) {
@@ -39,6 +40,7 @@ RigidArgs rigid_args(dpmd_ctx* c) {
     a.Lx = c->Lx.p; a.Ly = c->Ly.p; a.iLx = c->iLx.p; a.iLy = c->iLy.p; a.e = c->e.p;
     a.thresh2 = c->thresh2.p; a.boundary = c->boundary.p; a.flag = c->flag.p; a.any = c->run.p + RUN_ANY;
     a.cell = c->method == DPMD_NEIGHBOR_CELL;
+    a.frozen = c->mask_frozen ? c->f_frozen.p : nullptr;
     return a;
 }
 
@@ -83,6 +85,7 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_vertex_force(RigidArgs a, Gate g
     const int i = xcd_remap(blockIdx.x, gridDim.x) * DPMD_BLOCK + threadIdx.x;
     if (i >= a.nv) return;
     const int s = a.vsid[i];
+    if (a.frozen && a.frozen[s]) return;
     if (MODE == DPMD_FORCE_PBC) vertex_force_at<false>(a, i, s);
     else if (MODE == DPMD_FORCE_WALL) vertex_force_at<true>(a, i, s);
     else if (a.boundary[s]) vertex_force_at<true>(a, i, s);
@@ -272,7 +275,7 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_fire_drift(RigidArgs a, FireArgs
     const bool valid = i < a.n;
     const int s = valid ? a.sid[i] : -1;
     bool flag = false;
-    if (valid) {
+    if (valid && !fire_frozen(f, s)) {
         const double dt = f.s_dt[s], alpha = f.s_alpha[s], dtrev = f.s_dtrev[s], vscale = f.s_vscale[s];
         double x = a.x[i], y = a.y[i], th = a.th[i], vx = a.vx[i], vy = a.vy[i], w = a.w[i];
         const double fx = a.fx[i], fy = a.fy[i], t = a.tq[i];
@@ -297,6 +300,7 @@ k_fire_particle(RigidArgs a, FireArgs f, RedArgs r, int kick, Gate g) {
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     const int s = r.chunk_sys[c];
     const int beg = r.chunk_beg[c];
+    if (fire_frozen(f, s)) return;   // whole workgroup: chunks never straddle systems
     const int end = min(beg + DPMD_BLOCK, a.off[s + 1]);
     const int i = beg + threadIdx.x;
     double vp = 0.0, vpe = 0.0, vf2 = 0.0;

@@ -292,3 +292,23 @@ def run_fire(P: _Particles, dt, max_steps: int, predicate: int, a: float, b: flo
     P._c(lib().dpmd_fire_minimize(P.ctx, d.ptr, max_steps, predicate, a, b, mode, int(use_graph), poll,
                                   C.byref(steps), C.byref(conv)))
     return steps.value, bool(conv.value)
+
+
+def jam_compress(P: _Particles, dt, dphi: float, p_target: float, max_compressions: int, fire_max_steps: int,
+                 predicate: int = L.FIRE_PRED_MAX_FORCE, a: float = 1e-10, b: float = 0.0,
+                 mode: int = L.FORCE_PER_SYSTEM, use_graph: bool = True):
+    """Pressure-target jamming by compression (dpmd_jam_compress).
+
+    Returns (jam_state[S], n_compressions[S], work) where jam_state is 1 jammed, 0 never run
+    or below p_target after max_compressions, -1 dropped (a minimisation did not stop), and
+    work = sum of N_s x FIRE steps that did work.  P.get("FIRE_STEPS") holds the per-system
+    step counts."""
+    S = P.n_systems()
+    d = _per_system(dt, S)
+    st = np.zeros(S, np.int32)
+    nc = np.zeros(S, np.int32)
+    work = C.c_int64()
+    P._c(lib().dpmd_jam_compress(P.ctx, d.ptr, dphi, p_target, max_compressions, fire_max_steps, predicate, a, b,
+                                 mode, int(use_graph), st.ctypes.data_as(C.POINTER(C.c_int)),
+                                 nc.ctypes.data_as(C.POINTER(C.c_int)), C.byref(work)))
+    return st, nc, work.value

@@ -134,6 +134,22 @@ def uneven_batch(S: int = 512, nmin: int = 64, nmax: int = 4096, phi_lo: float =
     return disk_batch(sizes, phis, boundaries, seed=seed, skin=skin)
 
 
+def systems_of(b: DiskBatch, s0: int, s1: int) -> DiskBatch:
+    """Systems s0..s1-1 of a batch as a batch of their own (same particles, boxes and grids)."""
+    lo, hi = int(b.off[s0]), int(b.off[s1])
+    sl = lambda v: np.ascontiguousarray(v[s0:s1])
+    return DiskBatch(off=np.ascontiguousarray(b.off[s0:s1 + 1] - lo).astype(np.int32), rad=b.rad[lo:hi].copy(),
+                     mass=b.mass[lo:hi].copy(), area=b.area[lo:hi].copy(), Lx=sl(b.Lx), Ly=sl(b.Ly), e=sl(b.e),
+                     cutoff=sl(b.cutoff), thresh2=sl(b.thresh2), boundary=sl(b.boundary), x=b.x[lo:hi].copy(),
+                     y=b.y[lo:hi].copy(), cdx=None if b.cdx is None else sl(b.cdx),
+                     cdy=None if b.cdy is None else sl(b.cdy))
+
+
+def system_of(b: DiskBatch, s: int) -> DiskBatch:
+    """System s of a batch as a batch of one."""
+    return systems_of(b, s, s + 1)
+
+
 def load_into(P, b: DiskBatch, method: int = 1) -> None:
     """Allocate and fill a Disk context (dpmd_amd.system.Disk) from a DiskBatch, then sync."""
     P.set_neighbor_method(method)

@@ -73,6 +73,8 @@ enum dpmd_field {
     /* int, S (+1 where noted) */
     DPMD_F_SYSTEM_OFFSET = 64 /* S+1 */, DPMD_F_SYSTEM_SIZE, DPMD_F_CELL_DIM_X, DPMD_F_CELL_DIM_Y,
     DPMD_F_CELL_SYSTEM_START /* S+1 */, DPMD_F_BOUNDARY, DPMD_F_N_DOF, DPMD_F_REBUILD_FLAG,
+    DPMD_F_FIRE_STEPS /* steps that did work per system in the last dpmd_fire_minimize /
+                         dpmd_jam_compress: a system stops costing work one step after it stops */,
     /* int, P (or V for rigid bodies: the neighbor / cell objects are vertices) */
     DPMD_F_SYSTEM_ID = 80, DPMD_F_STATIC_INDEX, DPMD_F_CELL_ID, DPMD_F_NEIGHBOR_COUNT,
     /* rigid bumpy (base_poly_particle.hpp:104-120) */
@@ -179,6 +181,23 @@ int dpmd_fire_state(dpmd_ctx* ctx, double* dt, double* alpha, int* n_good, int* 
 int dpmd_fire_minimize(dpmd_ctx* ctx, const double* dt, int max_steps, int predicate, double a,
                        double b, int mode, int use_graph, int poll_interval, int* steps_taken,
                        int* all_converged);
+/* Pressure-target jamming by compression, per system.  New driver with no single reference
+ * function: the reference's jammer bisects on <pe> (jam_binary_search_pbc/_wall,
+ * jammers.cuh:60-256); this one is built from the same primitives, minimize_fire
+ * (minimizers.cuh:10-31) and increment_packing_fraction (base_particle.hpp:505-529).
+ * Each round: dpmd_fire_minimize over the systems still compressing (the others get dt 0,
+ * so they are masked and cost no work); P_s from dpmd_compute_pressure; a system with
+ * P_s >= p_target is jammed; the others grow phi_s by dphi and go on, up to
+ * max_compressions times.  jam_state (host, length S, may be NULL):
+ *   1 jammed; 0 never run (dt[s] == 0) or still below p_target after max_compressions;
+ *  -1 a minimisation did not stop within fire_max_steps (dropped, as jammers.cu:7-17 drops
+ *     systems whose minimisation failed).
+ * n_compressions (host, S, may be NULL): compressions applied per system.  work (host, may
+ * be NULL): sum over rounds and systems of N_s x (FIRE steps that did work).
+ * DPMD_F_FIRE_STEPS then holds each system's FIRE steps summed over all rounds. */
+int dpmd_jam_compress(dpmd_ctx* ctx, const double* dt, double dphi, double p_target, int max_compressions,
+                      int fire_max_steps, int predicate, double a, double b, int mode, int use_graph,
+                      int* jam_state, int* n_compressions, int64_t* work);
 /* n_steps of VelocityVerlet after dpmd_vv_init, device-driven (graph when use_graph). */
 int dpmd_vv_run(dpmd_ctx* ctx, int n_steps, int use_graph);
 

@@ -1208,6 +1208,76 @@ int or_disk_fire_minimize(void* h, const double* dt, int max_steps, double diff,
     *conv = c;
     return step;
 }
+// minimize_fire with per-system work counting: a system costs a step until the step after
+// the one that stopped it (that step zeroes its velocities; later steps leave it unchanged).
+// stopped[s] on return: 1 if system s stopped.
+static int fire_minimize_counted(Disk* d, const double* dt, int max_steps, int pred, double a, double b, int mode,
+                                 std::vector<int>& nsteps, std::vector<int>& stopped) {
+    const int S = d->S;
+    d->fire_init(dt, mode);
+    std::vector<int> frozen(S), prev(S);
+    for (int s = 0; s < S; ++s) { frozen[s] = dt[s] == 0.0; prev[s] = frozen[s]; }
+    nsteps.assign(S, 0);
+    int step = 0, c = 0;
+    bool all = true;
+    for (int s = 0; s < S; ++s) all = all && dt[s] == 0.0;
+    if (all) c = 1;
+    while (step < max_steps && !c) {
+        for (int s = 0; s < S; ++s) if (!frozen[s]) nsteps[s]++;
+        d->fire_step(mode);
+        c = pred == 0 ? d->fire_converged(a, b, 0, 0.0) : d->fire_converged(0.0, 0.0, 1, a);
+        for (int s = 0; s < S; ++s) {
+            if (prev[s]) frozen[s] = 1;
+            prev[s] = d->fire.dt[s] == 0.0;
+        }
+        ++step;
+    }
+    stopped.resize(S);
+    for (int s = 0; s < S; ++s) stopped[s] = d->fire.dt[s] == 0.0;
+    return step;
+}
+
+// Pressure-target jamming by compression, the same rounds as the library's
+// dpmd_jam_compress: minimise the systems still compressing (dt 0 for the rest), P_s;
+// jammed if P_s >= p_target, else phi_s += dphi (increment_packing_fraction,
+// base_particle.hpp:505-529), until max_comp compressions.  jam_state: 1 jammed, 0 never
+// run or budget used up, -1 minimisation did not stop.
+long long or_disk_jam_compress(void* h, const double* dt0, double dphi, double p_target, int max_comp, int fire_max,
+                               int pred, double a, double b, int mode, int* jam_state, int* n_comp, int* nsteps_out) {
+    Disk* d = (Disk*)h;
+    const int S = d->S;
+    std::vector<int> state(S), ncomp(S, 0), nsteps(S, 0), ns, stopped;
+    std::vector<double> dts(S), inc(S), P(S);
+    for (int s = 0; s < S; ++s) state[s] = dt0[s] == 0.0 ? 3 : 0;
+    long long work = 0;
+    for (;;) {
+        bool any = false;
+        for (int s = 0; s < S; ++s) { dts[s] = state[s] == 0 ? dt0[s] : 0.0; any = any || dts[s] != 0.0; }
+        if (!any) break;
+        fire_minimize_counted(d, dts.data(), fire_max, pred, a, b, mode, ns, stopped);
+        d->pressure(P.data());
+        bool grow = false;
+        for (int s = 0; s < S; ++s) {
+            inc[s] = 0.0;
+            if (dts[s] == 0.0) continue;
+            nsteps[s] += ns[s];
+            work += (long long)(d->off[s + 1] - d->off[s]) * ns[s];
+            if (!stopped[s]) state[s] = -1;
+            else if (P[s] >= p_target) state[s] = 1;
+            else if (ncomp[s] >= max_comp) state[s] = 2;
+            else { inc[s] = dphi; ncomp[s] += 1; grow = true; }
+        }
+        if (!grow) break;
+        d->increment_packing_fraction(inc.data());
+    }
+    for (int s = 0; s < S; ++s) {
+        jam_state[s] = state[s] == 1 ? 1 : state[s] == -1 ? -1 : 0;
+        n_comp[s] = ncomp[s];
+        nsteps_out[s] = nsteps[s];
+    }
+    return work;
+}
+
 void or_disk_vv_init(void* h, const double* dt, int mode) { ((Disk*)h)->vv_init(dt, mode); }
 void or_disk_vv_run(void* h, int n, int mode) { Disk* d = (Disk*)h; for (int k = 0; k < n; ++k) d->vv_step(mode); }
 

@@ -1,0 +1,108 @@
+"""Pins the oracle's pressure-target jamming driver (CPU): packing-fraction bookkeeping
+(phi_final = phi_0 + n_compressions * dphi), the jammed / unjammed decision against the
+final pressure, the per-system work count, and the whole driver against an independent
+loop over the oracle's pinned primitives (fire_minimize, pressure, increment_packing_fraction)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from dpmd_amd import workloads as W
+
+# A coarser compression step and force tolerance than the 512-system workload (1e-3, 1e-10)
+# keep the CPU cost to seconds; the logic under test is the same.
+DPHI, PT, FTOL, FMAX, CMAX = 4e-3, 1e-7, 1e-9, 20000, 30
+
+
+def small_batch(S=4, nmax=64, seed=2, phi_lo=0.78, phi_hi=0.81):
+    return W.uneven_batch(S=S, nmin=48, nmax=nmax, seed=seed, phi_lo=phi_lo, phi_hi=phi_hi)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def one_thread():
+    orc.lib().or_set_threads(1)          # systems of ~60 disks: threads only add overhead
+    yield
+    orc.lib().or_set_threads(os.cpu_count() or 1)
+
+
+@pytest.fixture(scope="module")
+def jammed():
+    b = small_batch()
+    o = orc.OracleDisk(b)
+    phi0 = o.packing_fraction()
+    st, nc, ns, work = o.jam_compress(1e-2, DPHI, PT, CMAX, FMAX, pred=1, a=FTOL, mode=2)
+    return b, o, phi0, st, nc, ns, work
+
+
+def test_phi_bookkeeping_and_decision(jammed):
+    b, o, phi0, st, nc, ns, work = jammed
+    phi = o.packing_fraction()
+    np.testing.assert_allclose(phi, phi0 + nc * DPHI, rtol=0, atol=1e-12)
+    P = o.pressure()
+    assert np.all(P[st == 1] >= PT)
+    assert np.all(P[(st == 0) & (nc == CMAX)] < PT)
+    # this batch holds a system that jams after compressions, one jammed from the start and
+    # one whose minimisation hits the step cap (dropped)
+    assert set(st.tolist()) == {-1, 1}
+    assert np.any((st == 1) & (nc > 0)) and np.any((st == 1) & (nc == 0))
+    sizes = np.diff(b.off)
+    assert work == int(np.sum(sizes.astype(np.int64) * ns))
+
+
+def test_driver_equals_loop_over_primitives(jammed):
+    """Re-run the rounds with the oracle's primitives from Python; the driver must agree bit for bit."""
+    b, o, phi0, st, nc, ns, work = jammed
+    q = orc.OracleDisk(small_batch())
+    S = q.S
+    state = np.zeros(S, int)
+    ncomp = np.zeros(S, int)
+    while True:
+        active = state == 0
+        if not active.any():
+            break
+        dts = np.where(active, 1e-2, 0.0)
+        q.fire_minimize(dts, FMAX, pred=1, ftol=FTOL, mode=2)
+        fdt = q.fire_state()[0]
+        P = q.pressure()
+        inc = np.zeros(S)
+        for s in np.flatnonzero(active):
+            if fdt[s] != 0.0:
+                state[s] = -1
+            elif P[s] >= PT:
+                state[s] = 1
+            elif ncomp[s] >= CMAX:
+                state[s] = 2
+            else:
+                inc[s] = DPHI
+                ncomp[s] += 1
+        if not inc.any():
+            break
+        q.increment_packing_fraction(inc)
+    assert np.array_equal(np.where(state == 2, 0, state), st)
+    assert np.array_equal(ncomp, nc)
+    for f in ("x", "y", "fx", "fy"):
+        assert np.array_equal(q.get(f), o.get(f)), f
+    assert np.array_equal(q.pressure(), o.pressure())
+
+
+def test_counted_steps_stop_one_after_stop():
+    """A system costs min(stop_step + 1, total) steps: one uneven pair of systems where one stops early."""
+    b = small_batch(S=2, nmax=80, seed=3)
+    o = orc.OracleDisk(b)
+    st, nc, ns, work = o.jam_compress(1e-2, DPHI, PT, 0, FMAX, pred=1, a=FTOL, mode=2)
+    q = orc.OracleDisk(b)
+    q.fire_init(1e-2, 2)
+    stop = [None, None]
+    k = 0
+    while not all(v is not None for v in stop) and k < FMAX:
+        q.fire_step(2)
+        q.fire_converged(0.0, 0.0, 1, FTOL)
+        k += 1
+        dt = q.fire_state()[0]
+        for s in range(2):
+            if stop[s] is None and dt[s] == 0.0:
+                stop[s] = k
+    assert all(v is not None for v in stop)
+    total = max(stop)
+    assert list(ns) == [min(v + 1, total) for v in stop]
