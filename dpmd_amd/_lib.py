@@ -97,6 +97,13 @@ PROTOS = {
     "dpmd_fire_get_dt": (i32, [vp, vp]),
     "dpmd_fire_state": (i32, [vp, vp, vp, vp, vp]),
     "dpmd_fire_minimize": (i32, [vp, vp, i32, i32, f64, f64, i32, i32, i32, pi32, pi32]),
+    "dpmd_adjoint_hvp": (i32, [vp, i32, vp, vp, vp, vp]),
+    "dpmd_adjoint_vjp": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "dpmd_adjoint_solve": (i32, [vp, i32, vp, vp, vp, vp, f64, i32, pi32, C.POINTER(C.c_double)]),
+    "dpmd_adjoint_pressure": (i32, [vp, i32, f64, i32, vp, vp, vp, pi32, C.POINTER(C.c_double)]),
+    "dpmd_newton_minimize": (i32, [vp, i32, i32, f64, f64, i32, C.POINTER(C.c_double)]),
+    "dpmd_vv_run_checkpointed": (i32, [vp, i32, i32, i32]),
+    "dpmd_vv_backprop": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32]),
     "dpmd_jam_compress": (i32, [vp, vp, f64, f64, i32, i32, i32, f64, f64, i32, i32, pi32, pi32,
                                 C.POINTER(C.c_int64)]),
     "dpmd_malloc": (i32, [C.POINTER(vp), i64]),

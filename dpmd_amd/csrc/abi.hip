@@ -342,8 +342,8 @@ void fire_init_impl(dpmd_ctx* c, const double* dt, int mode) {
     c->f_stopped.upload(stopped.data(), S, c->st);
     c->f_frozen.upload(stopped.data(), S, c->st);   // init zeroes v: a stopped system is final
     c->f_nsteps.upload(z.data(), S, c->st);
-    int run[8] = {0, 0, running, 1, 0, -1, -1, 0};
-    c->run.upload(run, 8, c->st);
+    int run[RUN_WORDS] = {0, 0, running, 1, 0, -1, -1, 0, 0};
+    c->run.upload(run, RUN_WORDS, c->st);
     c->fire_mode = mode;
     // init_impl (fire.cuh:90-96): v *= 0, then forces; the first update sees F.v = 0.
     zero_scale(c);
@@ -388,7 +388,7 @@ struct FrozenMask {
 };
 
 RunCtl read_run(dpmd_ctx* c) {
-    int run[8];
+    int run[RUN_WORDS];
     HIP_OK(hipMemcpyAsync(run, c->run.p, sizeof(run), hipMemcpyDeviceToHost, c->st));
     HIP_OK(hipStreamSynchronize(c->st));
     return RunCtl{run[RUN_RUNNING], run[RUN_HALT], run[RUN_HALT_AT], run[RUN_CONV_AT]};
@@ -403,6 +403,7 @@ void lean_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b) {
             else disk_fire_force(c, g, 1, 1, pred, a, b);
         } else {
             if (rigid(c)) rigid_vv_drift(c, g); else disk_vv_drift(c, g);
+            if (c->rec_seg) adj_record_positions(c, g);
             if (rigid(c)) rigid_vv_force(c, g); else disk_vv_force(c, g);
         }
     }
@@ -439,6 +440,8 @@ int lean_run(dpmd_ctx* c, int max_steps, bool fire, int pred, double a, double b
             int lg = 0;
             while ((2 << lg) <= std::min(m, max_steps - total) && lg < 7) ++lg;
             const int bm = 1 << lg;
+            if (c->rec_seg)   // recorded positions go to slots total + 1 ... of the segment buffer
+                HIP_OK(hipMemcpyAsync(c->run.p + RUN_REC, &total, sizeof(int), hipMemcpyHostToDevice, c->st));
             if (use_graph) {
                 if (!graphs[lg]) graphs[lg] = capture_batch(c, bm, fire, pred, a, b);
                 HIP_OK(hipGraphLaunch(graphs[lg], c->st));
@@ -481,6 +484,244 @@ int lean_run(dpmd_ctx* c, int max_steps, bool fire, int pred, double a, double b
     return total;
 }
 
+
+// ---------------------------------------------------------------------------------------
+// Adjoints (disks).  Particle vectors at the C ABI are canonical-order device arrays.
+// ---------------------------------------------------------------------------------------
+void need_disk_cells(dpmd_ctx* c) {
+    need(!rigid(c), "adjoints are implemented for disks");
+    need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
+    need(c->method == DPMD_NEIGHBOR_CELL, "adjoints need the cell neighbor method");
+}
+
+void vv_run_impl(dpmd_ctx* c, int n, bool use_graph) {
+    if (n <= 0) return;
+    int any = 0;
+    HIP_OK(hipMemcpyAsync(&any, c->run.p + RUN_ANY, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIP_OK(hipStreamSynchronize(c->st));
+    int run[RUN_WORDS] = {0, n, 1, 0, 0, -1, -1, any, 0};
+    c->run.upload(run, RUN_WORDS, c->st);
+    (void)lean_run(c, n, false, 0, 0.0, 0.0, use_graph, nullptr);
+    check_overflow(c);
+}
+
+void snapshot(dpmd_ctx* c, double* dst) {   // canonical x, y, vx, vy
+    const int N = c->N;
+    adj_to_canonical(c, c->x.p, dst);
+    adj_to_canonical(c, c->y.p, dst + N);
+    adj_to_canonical(c, c->vx.p, dst + 2 * (size_t)N);
+    adj_to_canonical(c, c->vy.p, dst + 3 * (size_t)N);
+}
+
+void restore_positions(dpmd_ctx* c, const double* src) {   // canonical x, y -> current; rebuild
+    const int N = c->N;
+    adj_to_current(c, src, c->x.p);
+    adj_to_current(c, src + N, c->y.p);
+    check_neighbors_impl(c, true);
+}
+
+void restore_snapshot(dpmd_ctx* c, const double* src) {
+    const int N = c->N;
+    adj_to_current(c, src + 2 * (size_t)N, c->vx.p);
+    adj_to_current(c, src + 3 * (size_t)N, c->vy.p);
+    restore_positions(c, src);
+    compute_forces_impl(c, c->vv_mode);
+}
+
+struct RecordTo {
+    dpmd_ctx* c;
+    RecordTo(dpmd_ctx* ctx, double* seg) : c(ctx) { c->rec_seg = seg; }
+    ~RecordTo() { c->rec_seg = nullptr; }
+};
+
+void vv_checkpointed_impl(dpmd_ctx* c, int n, int every, bool use_graph) {
+    need_disk_cells(c);
+    need_state(c->vv_ready, "vv_run_checkpointed: call vv_init first");
+    need(n >= 0 && every > 0, "vv_run_checkpointed: n_steps >= 0 and every > 0 required");
+    const int N = c->N;
+    const int J = (n + every - 1) / every;
+    c->ck_state.resize((int64_t)(J + 1) * 4 * N);
+    snapshot(c, c->ck_state.p);
+    for (int j = 0; j < J; ++j) {
+        const int len = std::min(every, n - j * every);
+        vv_run_impl(c, len, use_graph);
+        snapshot(c, c->ck_state.p + (size_t)(j + 1) * 4 * N);
+    }
+    c->ck_steps = n; c->ck_every = every; c->ck_count = J + 1;
+}
+
+// Reverse mode through the checkpointed run: cotangents of the final (x, y, vx, vy) ->
+// cotangents of the initial ones.  Per reverse step one Hessian-vector product:
+//   vbar += dt xbar;  z = H(x_k) (vbar / m);  xbar -= dt z  (dt / 2 at k = 0 and, before the
+//   loop, at k = T with vbar = the final cotangent).
+// Each segment is replayed from its checkpoint with positions recorded per step; the
+// Hessian products use one canonical-order contact list per segment, built at the segment's
+// first configuration with a cutoff covering every displacement in the segment.
+void vv_backprop_impl(dpmd_ctx* c, const double* gx, const double* gy, const double* gvx, const double* gvy, double* ox,
+                      double* oy, double* ovx, double* ovy, bool use_graph) {
+    need_disk_cells(c);
+    need_state(c->ck_count > 0, "vv_backprop: run vv_run_checkpointed first");
+    const int N = c->N, S = c->S, n = c->ck_steps, every = c->ck_every, J = c->ck_count - 1;
+    const int mode = c->vv_mode;
+    hipStream_t st = c->st;
+    c->adj_t0.resize(2 * (int64_t)N); c->adj_t1.resize(2 * (int64_t)N); c->adj_t2.resize(2 * (int64_t)N);
+    c->adj_t3.resize(2 * (int64_t)N); c->adj_t4.resize(N); c->adj_t5.resize(N); c->adj_bx.resize(N);
+    c->adj_u0.resize(S);
+    double *xb = c->adj_t0.p, *vb = c->adj_t1.p, *w_c = c->adj_t2.p, *z = c->adj_t3.p;
+    double *rad_c = c->adj_t4.p, *m_c = c->adj_t5.p;
+    auto in = [&](const double* src, double* dst) {
+        if (src) HIP_OK(hipMemcpyAsync(dst, src, (size_t)N * 8, hipMemcpyDeviceToDevice, st));
+        else fill_f64(st, dst, N, 0.0);
+    };
+    in(gx, xb); in(gy, xb + N); in(gvx, vb); in(gvy, vb + N);
+    adj_to_canonical(c, c->rad.p, rad_c);
+    adj_to_canonical(c, c->mass.p, m_c);
+    c->ck_seg.resize((int64_t)(every + 1) * 2 * N);
+    std::vector<double> cut0(S), d2(S), cut(S), Lx(S), Ly(S), rmax(S, 0.0);
+    HIP_OK(hipMemcpyAsync(cut0.data(), c->cutoff.p, (size_t)S * 8, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(Lx.data(), c->Lx.p, (size_t)S * 8, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(Ly.data(), c->Ly.p, (size_t)S * 8, hipMemcpyDeviceToHost, st));
+    {
+        const auto rad = dl(c, c->rad);
+        for (int s = 0; s < S; ++s)
+            for (int i = c->h_off[s]; i < c->h_off[s + 1]; ++i) rmax[s] = std::max(rmax[s], rad[i]);
+    }
+    bool first = true;
+    for (int j = J - 1; j >= 0; --j) {
+        const int k0 = j * every, k1 = std::min(k0 + every, n), len = k1 - k0;
+        const double* ck = c->ck_state.p + (size_t)j * 4 * N;
+        restore_snapshot(c, ck);
+        HIP_OK(hipMemcpyAsync(c->ck_seg.p, ck, (size_t)2 * N * 8, hipMemcpyDeviceToDevice, st));
+        {
+            RecordTo rec(c, c->ck_seg.p);
+            vv_run_impl(c, len, use_graph);
+        }
+        // Reverse sweep over the segment in windows [a, hi]: one canonical contact list per
+        // window, built at x_a with a cutoff covering every displacement in the window; a
+        // window whose enlarged cutoff would leave fewer than 3 cells per side, or exceed the
+        // cutoff by more than the largest radius (neighbor capacity), is halved.
+        int hi = k1;
+        while (hi > k0 || first) {
+            int w = hi - k0, a = hi - w;
+            bool enlarge = false;
+            for (;;) {
+                adj_seg_disp2(c, a - k0, hi - a, c->adj_bx.p);
+                sys_reduce_max(c, c->adj_bx.p, c->adj_u0.p);
+                HIP_OK(hipMemcpyAsync(d2.data(), c->adj_u0.p, (size_t)S * 8, hipMemcpyDeviceToHost, st));
+                HIP_OK(hipStreamSynchronize(st));
+                bool fits = true;
+                enlarge = false;
+                for (int s = 0; s < S; ++s) {
+                    const double span = 2.0 * std::sqrt(d2[s]);
+                    cut[s] = cut0[s];
+                    if (span >= cut0[s]) {
+                        cut[s] = span * (1.0 + 1e-9) + 1e-12;
+                        enlarge = true;
+                        const double need_sz = 2.0 * rmax[s] + cut[s];
+                        fits = fits && std::floor(Lx[s] / need_sz) >= 3 && std::floor(Ly[s] / need_sz) >= 3 &&
+                               cut[s] <= cut0[s] + rmax[s];
+                    }
+                }
+                if (fits || hi - a <= 1) break;
+                w = std::max(1, (hi - a) / 2);
+                a = hi - w;
+            }
+            if (enlarge) {
+                c->cutoff.upload(cut.data(), S, st);
+                setup_cells(c, CELLS_BOX_CHANGED);
+            }
+            restore_positions(c, c->ck_seg.p + (size_t)(a - k0) * 2 * N);
+            check_overflow(c);
+            adj_list_canonical(c, c->tmp_i.p);
+            if (enlarge) {
+                c->cutoff.upload(cut0.data(), S, st);
+                setup_cells(c, CELLS_BOX_CHANGED);
+            }
+            if (first) {
+                adj_rev_w(c, m_c, vb, w_c);
+                adj_hvp_segment(c, mode, hi - k0, rad_c, w_c, z);
+                adj_rev_x(c, 0.5, z, xb);
+                first = false;
+            }
+            for (int k = hi - 1; k >= a; --k) {
+                adj_rev_vw(c, m_c, xb, vb, w_c);
+                adj_hvp_segment(c, mode, k - k0, rad_c, w_c, z);
+                adj_rev_x(c, k > 0 ? 1.0 : 0.5, z, xb);
+            }
+            hi = a;
+        }
+    }
+    auto out = [&](const double* src, double* dst) {
+        if (dst) HIP_OK(hipMemcpyAsync(dst, src, (size_t)N * 8, hipMemcpyDeviceToDevice, st));
+    };
+    out(xb, ox); out(xb + N, oy); out(vb, ovx); out(vb + N, ovy);
+    restore_snapshot(c, c->ck_state.p + (size_t)J * 4 * N);   // leave the final state
+    HIP_OK(hipStreamSynchronize(st));
+}
+
+// d P_s / d(radii, box) through the minimum: dP = dP/dq + lambda^T dF/dq with H lambda = dP/dx.
+void adjoint_pressure_impl(dpmd_ctx* c, int mode, double tol, int max_iter, double* dr, double* dLx, double* dLy,
+                           int* iters, double* resid) {
+    need_disk_cells(c);
+    const int N = c->N, S = c->S;
+    hipStream_t st = c->st;
+    c->adj_t0.resize(2 * (int64_t)N); c->adj_t1.resize(N); c->adj_t2.resize(2 * (int64_t)N); c->adj_t3.resize(N);
+    for (auto* v : {&c->adj_u0, &c->adj_u1, &c->adj_u2, &c->adj_u3, &c->adj_u4}) v->resize(S);
+    fill_f64(st, c->adj_u0.p, S, 1.0);
+    double *b = c->adj_t0.p, *lam = c->adj_t2.p;
+    adj_vjp_current(c, mode, nullptr, c->adj_u0.p, nullptr, nullptr, b, b + N, c->adj_t1.p, c->adj_u1.p, c->adj_u2.p);
+    const int it = adj_cg_current(c, mode, b, b + N, lam, lam + N, tol, max_iter, resid);
+    adj_vjp_current(c, mode, nullptr, nullptr, lam, lam + N, nullptr, nullptr, c->adj_t3.p, c->adj_u3.p, c->adj_u4.p);
+    adj_add(c, N, c->adj_t1.p, c->adj_t3.p, c->adj_t3.p);
+    if (dr) adj_to_canonical(c, c->adj_t3.p, dr);
+    if (dLx) adj_add(c, S, c->adj_u1.p, c->adj_u3.p, dLx);
+    if (dLy) adj_add(c, S, c->adj_u2.p, c->adj_u4.p, dLy);
+    HIP_OK(hipStreamSynchronize(st));
+    if (iters) *iters = it;
+}
+
+// Newton iterations x <- x + H^-1 F (conjugate gradients) from a configuration near a
+// minimum, until max|F| <= ftol.  Steps are damped to a largest displacement of
+// kNewtonMaxStep; a Hessian solve that does not converge is an error (the configuration is
+// not near a minimum).  Returns the final max|F| over all particles.
+constexpr double kNewtonMaxStep = 0.05;
+double newton_impl(dpmd_ctx* c, int mode, int max_newton, double ftol, double cg_tol, int cg_max) {
+    need_disk_cells(c);
+    const int N = c->N, S = c->S;
+    c->adj_t0.resize(2 * (int64_t)N);
+    c->adj_bx.resize(N);
+    c->adj_u0.resize(S);
+    std::vector<double> mf(S);
+    auto maxf = [&]() {
+        compute_forces_impl(c, mode);
+        sys_reduce(c, RED_MAXF2, c->maxf2.p);
+        HIP_OK(hipMemcpyAsync(mf.data(), c->maxf2.p, (size_t)S * 8, hipMemcpyDeviceToHost, c->st));
+        HIP_OK(hipStreamSynchronize(c->st));
+        double m = 0.0;
+        for (double v : mf) m = std::max(m, v);
+        return std::sqrt(m);
+    };
+    double f = maxf();
+    for (int it = 0; it < max_newton && f > ftol; ++it) {
+        double res = 0.0;
+        (void)adj_cg_current(c, mode, c->fx.p, c->fy.p, c->adj_t0.p, c->adj_t0.p + N, cg_tol, cg_max, &res);
+        if (!(res <= 1e-6))
+            throw StateError("newton_minimize: the Hessian solve did not converge (relative residual " +
+                             std::to_string(res) + "); run FIRE closer to the minimum first");
+        adj_norm2(c, c->adj_t0.p, c->adj_t0.p + N, c->adj_bx.p);
+        sys_reduce_max(c, c->adj_bx.p, c->adj_u0.p);
+        HIP_OK(hipMemcpyAsync(mf.data(), c->adj_u0.p, (size_t)S * 8, hipMemcpyDeviceToHost, c->st));
+        HIP_OK(hipStreamSynchronize(c->st));
+        double step = 0.0;
+        for (double v : mf) step = std::max(step, std::sqrt(v));
+        const double scale = step > kNewtonMaxStep ? kNewtonMaxStep / step : 1.0;
+        adj_axpy2(c, scale, c->adj_t0.p, c->adj_t0.p + N, c->x.p, c->y.p);
+        check_neighbors_impl(c, true);
+        f = maxf();
+    }
+    return f;
+}
+
 }  // namespace
 
 extern "C" {
@@ -502,7 +743,7 @@ int dpmd_ctx_create(dpmd_ctx** out, int kind, void* stream) {
             c->one.resize(1);
             int one = 1;
             c->one.upload(&one, 1, c->st);
-            c->run.resize(8); c->run.zero(c->st);
+            c->run.resize(RUN_WORDS); c->run.zero(c->st);
             c->overflow.resize(1); c->overflow.zero(c->st);
             c->max_count.resize(1); c->max_count.zero(c->st);
         } catch (...) { delete c; throw; }
@@ -842,10 +1083,87 @@ int dpmd_vv_run(dpmd_ctx* c, int n, int use_graph) {
         int any = 0;
         HIP_OK(hipMemcpyAsync(&any, c->run.p + RUN_ANY, sizeof(int), hipMemcpyDeviceToHost, c->st));
         HIP_OK(hipStreamSynchronize(c->st));
-        int run[8] = {0, n, 1, 0, 0, -1, -1, any};
-        c->run.upload(run, 8, c->st);
+        int run[RUN_WORDS] = {0, n, 1, 0, 0, -1, -1, any, 0};
+        c->run.upload(run, RUN_WORDS, c->st);
         (void)lean_run(c, n, false, 0, 0.0, 0.0, use_graph != 0, nullptr);
         check_overflow(c);
+    });
+}
+
+int dpmd_adjoint_hvp(dpmd_ctx* c, int mode, const double* wx, const double* wy, double* yx, double* yy) {
+    return guard(c, [&] {
+        need_disk_cells(c);
+        need(wx && wy && yx && yy, "adjoint_hvp: null vector");
+        const int N = c->N;
+        c->adj_t0.resize(2 * (int64_t)N); c->adj_t1.resize(2 * (int64_t)N);
+        adj_to_current(c, wx, c->adj_t0.p); adj_to_current(c, wy, c->adj_t0.p + N);
+        adj_hvp_current(c, mode, c->adj_t0.p, c->adj_t0.p + N, c->adj_t1.p, c->adj_t1.p + N);
+        adj_to_canonical(c, c->adj_t1.p, yx); adj_to_canonical(c, c->adj_t1.p + N, yy);
+        HIP_OK(hipStreamSynchronize(c->st));
+    });
+}
+
+int dpmd_adjoint_vjp(dpmd_ctx* c, int mode, const double* gU, const double* gP, const double* gfx, const double* gfy,
+                     double* dx, double* dy, double* dr, double* dLx, double* dLy) {
+    return guard(c, [&] {
+        need_disk_cells(c);
+        need((gfx == nullptr) == (gfy == nullptr) && (dx == nullptr) == (dy == nullptr) &&
+                 (dLx == nullptr) == (dLy == nullptr), "adjoint_vjp: x and y arrays come in pairs");
+        const int N = c->N;
+        c->adj_t0.resize(2 * (int64_t)N); c->adj_t1.resize(2 * (int64_t)N); c->adj_t2.resize(N);
+        if (gfx) { adj_to_current(c, gfx, c->adj_t0.p); adj_to_current(c, gfy, c->adj_t0.p + N); }
+        adj_vjp_current(c, mode, gU, gP, gfx ? c->adj_t0.p : nullptr, gfx ? c->adj_t0.p + N : nullptr,
+                        dx ? c->adj_t1.p : nullptr, dx ? c->adj_t1.p + N : nullptr, dr ? c->adj_t2.p : nullptr, dLx,
+                        dLy);
+        if (dx) { adj_to_canonical(c, c->adj_t1.p, dx); adj_to_canonical(c, c->adj_t1.p + N, dy); }
+        if (dr) adj_to_canonical(c, c->adj_t2.p, dr);
+        HIP_OK(hipStreamSynchronize(c->st));
+    });
+}
+
+int dpmd_adjoint_solve(dpmd_ctx* c, int mode, const double* bx, const double* by, double* xx, double* xy, double tol,
+                       int max_iter, int* iters, double* rel_resid) {
+    return guard(c, [&] {
+        need_disk_cells(c);
+        need(bx && by && xx && xy && tol > 0.0 && max_iter > 0, "adjoint_solve: bad arguments");
+        const int N = c->N;
+        c->adj_t0.resize(2 * (int64_t)N); c->adj_t1.resize(2 * (int64_t)N);
+        adj_to_current(c, bx, c->adj_t0.p); adj_to_current(c, by, c->adj_t0.p + N);
+        const int it = adj_cg_current(c, mode, c->adj_t0.p, c->adj_t0.p + N, c->adj_t1.p, c->adj_t1.p + N, tol,
+                                      max_iter, rel_resid);
+        adj_to_canonical(c, c->adj_t1.p, xx); adj_to_canonical(c, c->adj_t1.p + N, xy);
+        HIP_OK(hipStreamSynchronize(c->st));
+        if (iters) *iters = it;
+    });
+}
+
+int dpmd_adjoint_pressure(dpmd_ctx* c, int mode, double tol, int max_iter, double* dr, double* dLx, double* dLy,
+                          int* iters, double* rel_resid) {
+    return guard(c, [&] {
+        need(tol > 0.0 && max_iter > 0 && (dLx == nullptr) == (dLy == nullptr), "adjoint_pressure: bad arguments");
+        adjoint_pressure_impl(c, mode, tol, max_iter, dr, dLx, dLy, iters, rel_resid);
+    });
+}
+
+int dpmd_newton_minimize(dpmd_ctx* c, int mode, int max_newton, double ftol, double cg_tol, int cg_max_iter,
+                         double* max_force) {
+    return guard(c, [&] {
+        need(max_newton >= 0 && cg_tol > 0.0 && cg_max_iter > 0, "newton_minimize: bad arguments");
+        const double f = newton_impl(c, mode, max_newton, ftol, cg_tol, cg_max_iter);
+        if (max_force) *max_force = f;
+    });
+}
+
+int dpmd_vv_run_checkpointed(dpmd_ctx* c, int n_steps, int every, int use_graph) {
+    return guard(c, [&] { vv_checkpointed_impl(c, n_steps, every, use_graph != 0); });
+}
+
+int dpmd_vv_backprop(dpmd_ctx* c, const double* gx, const double* gy, const double* gvx, const double* gvy, double* gx0,
+                     double* gy0, double* gvx0, double* gvy0, int use_graph) {
+    return guard(c, [&] {
+        need((gx == nullptr) == (gy == nullptr) && (gvx == nullptr) == (gvy == nullptr),
+             "vv_backprop: x and y arrays come in pairs");
+        vv_backprop_impl(c, gx, gy, gvx, gvy, gx0, gy0, gvx0, gvy0, use_graph != 0);
     });
 }
 

@@ -42,7 +42,9 @@ struct NbrArgs {
 //   [4] halted     [5] index of the halted step in its batch
 //   [6] index of the step in its batch at which the last system stopped
 //   [7] "some rebuild flag is set" (raised by the drift kernels, cleared by the rebuild)
-enum { RUN_MAX = 1, RUN_RUNNING = 2, RUN_ACTIVE = 3, RUN_HALT = 4, RUN_HALT_AT = 5, RUN_CONV_AT = 6, RUN_ANY = 7 };
+//   [8] base slot for recorded positions (reverse-mode replays)
+enum { RUN_MAX = 1, RUN_RUNNING = 2, RUN_ACTIVE = 3, RUN_HALT = 4, RUN_HALT_AT = 5, RUN_CONV_AT = 6, RUN_ANY = 7,
+       RUN_REC = 8, RUN_WORDS = 9 };
 
 // Launch gate.  Plain form (run == nullptr): the kernel works iff *active != 0.  Batch form
 // (lean fused runs): a step works while nothing halted and some system runs; the force
@@ -159,6 +161,18 @@ struct dpmd_ctx {
     // did not finish in one run, cause not found); whether that launch may be captured
     bool use_coop = false, coop_in_graph = false;
     int coop_blocks = 0;
+    // adjoints: scratch, CG vectors (current order), NVE checkpoints (canonical order)
+    dpmd::DVec<double> adj_bx, adj_by, adj_sa, adj_sb, adj_sc, adj_sd;
+    dpmd::DVec<double> adj_t0, adj_t1, adj_t2, adj_t3, adj_t4, adj_t5;
+    dpmd::DVec<double> adj_u0, adj_u1, adj_u2, adj_u3, adj_u4;   // per-system scratch
+    dpmd::DVec<double> cg_rx, cg_ry, cg_px, cg_py, cg_qx, cg_qy;
+    dpmd::DVec<double> cg_rr, cg_rrn, cg_pq, cg_bb, cg_beta;
+    dpmd::DVec<int> cg_active;
+    dpmd::DVec<double> ck_state;     // n_ck x 4N: x, y, vx, vy (canonical) at every checkpoint
+    dpmd::DVec<double> ck_seg;       // (every + 1) x 2N positions of one replayed segment
+    dpmd::DVec<int> ck_nn, ck_nbr;   // canonical-order contact list for a segment
+    int ck_steps = 0, ck_every = 0, ck_count = 0;
+    double* rec_seg = nullptr;       // when set, lean VV batches record positions per step
     // kernel timing (force kernels; outside graph capture only)
     bool prof = false, capturing = false;
     std::vector<hipEvent_t> prof_ev;   // (begin, end) pairs
@@ -174,6 +188,25 @@ void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all);
 void clear_flags(dpmd_ctx* c, const int* active);
 void naive_neighbors(dpmd_ctx* c);
 void check_overflow(dpmd_ctx* c);
+
+// adjoint.hip (disks; particle vectors in current order unless named canonical)
+void adj_to_current(dpmd_ctx* c, const double* canon, double* cur);
+void adj_to_canonical(dpmd_ctx* c, const double* cur, double* canon);
+void adj_hvp_current(dpmd_ctx* c, int mode, const double* wx, const double* wy, double* yx, double* yy);
+void adj_vjp_current(dpmd_ctx* c, int mode, const double* gU, const double* gP, const double* gfx, const double* gfy,
+                     double* dx, double* dy, double* dr, double* dLx, double* dLy);
+int adj_cg_current(dpmd_ctx* c, int mode, const double* bx, const double* by, double* xx, double* xy, double tol,
+                   int max_iter, double* rel_resid);
+void adj_record_positions(dpmd_ctx* c, Gate g);
+void adj_list_canonical(dpmd_ctx* c, int* inv_scratch);
+void adj_seg_disp2(dpmd_ctx* c, int base, int len, double* d2);
+void adj_rev_vw(dpmd_ctx* c, const double* m, double* xb, double* vb, double* w);
+void adj_rev_w(dpmd_ctx* c, const double* m, const double* vb, double* w);
+void adj_rev_x(dpmd_ctx* c, double coef, const double* z, double* xb);
+void adj_hvp_segment(dpmd_ctx* c, int mode, int slot, const double* rad_c, const double* w, double* z);
+void adj_add(dpmd_ctx* c, int64_t n, const double* a, const double* b, double* out);
+void adj_axpy2(dpmd_ctx* c, double a, const double* xx, const double* xy, double* yx, double* yy);
+void adj_norm2(dpmd_ctx* c, const double* ax, const double* ay, double* out);
 
 // disk.hip
 void disk_compute_forces(dpmd_ctx* c, int mode);
@@ -203,8 +236,12 @@ void rigid_vv_drift(dpmd_ctx* c, Gate g);
 void rigid_vv_force(dpmd_ctx* c, Gate g);
 
 // reduce.hip
-enum RedKind { RED_PE = 0, RED_KE, RED_POWER, RED_VX, RED_VY, RED_PHI, RED_SXX, RED_SXY, RED_SYX, RED_SYY, RED_MAXF2 };
+enum RedKind { RED_PE = 0, RED_KE, RED_POWER, RED_VX, RED_VY, RED_PHI, RED_SXX, RED_SXY, RED_SYX, RED_SYY, RED_MAXF2,
+               RED_ARR, RED_DOT, RED_MAXARR };
 void sys_reduce(dpmd_ctx* c, int kind, double* out);
+void sys_reduce_arr(dpmd_ctx* c, const double* a, double* out);
+void sys_reduce_max(dpmd_ctx* c, const double* a, double* out);
+void sys_reduce_dot(dpmd_ctx* c, const double* a, const double* b, const double* cc, const double* d, double* out);
 void fire_apply_convergence(dpmd_ctx* c, int pred, double a, double b);
 void fire_setup(dpmd_ctx* c, const double* dt);
 void run_begin_step(dpmd_ctx* c);

@@ -38,6 +38,8 @@ __device__ __forceinline__ double red_value(int kind, const RedIn& in, int i) {
             return in.a[i] / box_area;
         }
         case RED_MAXF2: { const double fx = in.a[i], fy = in.b[i]; return fx * fx + fy * fy; }
+        case RED_ARR: case RED_MAXARR: return in.a[i];
+        case RED_DOT: return in.a[i] * in.b[i] + in.c[i] * in.d[i];
     }
     return 0.0;
 }
@@ -52,7 +54,7 @@ k_red_chunks(int kind, RedIn in, const int* chunk_sys, const int* chunk_beg, con
     const int i = beg + threadIdx.x;
     double v = 0.0;
     if (i < end) v = red_value(kind, in, i);
-    v = (kind == RED_MAXF2) ? block_max256(v, lds) : block_sum256(v, lds);
+    v = (kind == RED_MAXF2 || kind == RED_MAXARR) ? block_max256(v, lds) : block_sum256(v, lds);
     if (threadIdx.x == 0) part[c] = v;
 }
 
@@ -62,7 +64,7 @@ k_red_final(int kind, const int* sys_chunk, const double* part, double* out) {
     const int s = blockIdx.x;
     const int cb = sys_chunk[s], ce = sys_chunk[s + 1];
     double acc = 0.0;
-    if (kind == RED_MAXF2) {
+    if (kind == RED_MAXF2 || kind == RED_MAXARR) {
         for (int k = cb + threadIdx.x; k < ce; k += DPMD_BLOCK) acc = fmax(acc, part[k]);
         acc = block_max256(acc, lds);
     } else {
@@ -182,6 +184,30 @@ void sys_reduce(dpmd_ctx* c, int kind, double* out) {
     }
     k_red_chunks<<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(kind, in, c->chunk_sys.p, c->chunk_beg.p, c->off.p, c->part.p);
     k_red_final<<<c->S, DPMD_BLOCK, 0, c->st>>>(kind, c->sys_chunk.p, c->part.p, out);
+    HIP_OK(hipGetLastError());
+}
+
+// Per-system sum of a per-particle array, and of a.b + c.d (same tree as every other sum).
+void sys_reduce_arr(dpmd_ctx* c, const double* a, double* out) {
+    RedIn in{};
+    in.a = a;
+    k_red_chunks<<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(RED_ARR, in, c->chunk_sys.p, c->chunk_beg.p, c->off.p, c->part.p);
+    k_red_final<<<c->S, DPMD_BLOCK, 0, c->st>>>(RED_ARR, c->sys_chunk.p, c->part.p, out);
+    HIP_OK(hipGetLastError());
+}
+void sys_reduce_max(dpmd_ctx* c, const double* a, double* out) {   // a >= 0
+    RedIn in{};
+    in.a = a;
+    k_red_chunks<<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(RED_MAXARR, in, c->chunk_sys.p, c->chunk_beg.p, c->off.p,
+                                                       c->part.p);
+    k_red_final<<<c->S, DPMD_BLOCK, 0, c->st>>>(RED_MAXARR, c->sys_chunk.p, c->part.p, out);
+    HIP_OK(hipGetLastError());
+}
+void sys_reduce_dot(dpmd_ctx* c, const double* a, const double* b, const double* cc, const double* d, double* out) {
+    RedIn in{};
+    in.a = a; in.b = b; in.c = cc; in.d = d;
+    k_red_chunks<<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(RED_DOT, in, c->chunk_sys.p, c->chunk_beg.p, c->off.p, c->part.p);
+    k_red_final<<<c->S, DPMD_BLOCK, 0, c->st>>>(RED_DOT, c->sys_chunk.p, c->part.p, out);
     HIP_OK(hipGetLastError());
 }
 

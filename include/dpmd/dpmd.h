@@ -198,6 +198,43 @@ int dpmd_fire_minimize(dpmd_ctx* ctx, const double* dt, int max_steps, int predi
 int dpmd_jam_compress(dpmd_ctx* ctx, const double* dt, double dphi, double p_target, int max_compressions,
                       int fire_max_steps, int predicate, double a, double b, int mode, int use_graph,
                       int* jam_state, int* n_compressions, int64_t* work);
+/* ---- adjoints (disks) --------------------------------------------------------------------
+ * New; the reference has no derivatives of its dynamics.  Its Hessian observable
+ * (compute_hessian, src/particles/disk.cu:565-661) fixes the pair block used here:
+ * H_ij = k n n^T - (k delta / d)(I - n n^T) for each contact (delta = r_i + r_j - d > 0), plus
+ * k on the diagonal per wall contact.  U = sum k delta^2 / 2 (+ walls); F = -dU/dx;
+ * P = (sum m v^2 + sum_pairs k delta d) / (2 Lx Ly) (compute_pressure, T14 definition).
+ * Box derivatives are taken at fixed reduced coordinates x / L (walls move with the box).
+ * Particle vectors are canonical-order DEVICE arrays of length N; per-system arrays are
+ * device arrays of length S.  mode: dpmd_force_mode. */
+/* y = H w. */
+int dpmd_adjoint_hvp(dpmd_ctx* ctx, int mode, const double* wx, const double* wy, double* yx, double* yy);
+/* Vector-Jacobian product of (U_s, P_s, F) at the current state: given cotangents gU[S],
+ * gP[S], gF[N] (each may be NULL = 0), returns d/dx, d/dr and d/dL of
+ * sum_s gU_s U_s + gP_s P_s + sum_i gF_i . F_i (any output may be NULL). */
+int dpmd_adjoint_vjp(dpmd_ctx* ctx, int mode, const double* gU, const double* gP, const double* gfx,
+                     const double* gfy, double* dx, double* dy, double* dr, double* dLx, double* dLy);
+/* Solves H x = b per system by conjugate gradients to |b - Hx| <= tol |b| (periodic systems
+ * in the complement of uniform translations).  rel_resid: max over systems of the true
+ * relative residual. */
+int dpmd_adjoint_solve(dpmd_ctx* ctx, int mode, const double* bx, const double* by, double* xx, double* xy,
+                       double tol, int max_iter, int* iters, double* rel_resid);
+/* Implicit derivative of each system's pressure through its energy minimum (call at a
+ * FIRE / Newton minimum): dP_s/dr_i and dP_s/dL_s = dP/dq + lambda^T dF/dq with
+ * H lambda = dP/dx solved to tol.  dr: [N]; dLx, dLy: [S]; any may be NULL. */
+int dpmd_adjoint_pressure(dpmd_ctx* ctx, int mode, double tol, int max_iter, double* dr, double* dLx,
+                          double* dLy, int* iters, double* rel_resid);
+/* Newton iterations x <- x + H^-1 F from near a minimum until max|F| <= ftol (polishes FIRE
+ * minima to rounding level).  max_force: final max |F_i|. */
+int dpmd_newton_minimize(dpmd_ctx* ctx, int mode, int max_newton, double ftol, double cg_tol, int cg_max_iter,
+                         double* max_force);
+/* n_steps of velocity Verlet (as dpmd_vv_run) keeping canonical (x, y, vx, vy) checkpoints
+ * every `every` steps, then reverse mode: cotangents of the final x, y, vx, vy (NULL = 0)
+ * -> cotangents of the initial ones.  Each segment is replayed once from its checkpoint;
+ * one Hessian-vector product per reverse step.  The context is left at the final state. */
+int dpmd_vv_run_checkpointed(dpmd_ctx* ctx, int n_steps, int every, int use_graph);
+int dpmd_vv_backprop(dpmd_ctx* ctx, const double* gx, const double* gy, const double* gvx, const double* gvy,
+                     double* gx0, double* gy0, double* gvx0, double* gvy0, int use_graph);
 /* n_steps of VelocityVerlet after dpmd_vv_init, device-driven (graph when use_graph). */
 int dpmd_vv_run(dpmd_ctx* ctx, int n_steps, int use_graph);
 
