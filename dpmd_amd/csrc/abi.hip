@@ -671,7 +671,10 @@ void adjoint_pressure_impl(dpmd_ctx* c, int mode, double tol, int max_iter, doub
     double *b = c->adj_t0.p, *lam = c->adj_t2.p;
     adj_vjp_current(c, mode, nullptr, c->adj_u0.p, nullptr, nullptr, b, b + N, c->adj_t1.p, c->adj_u1.p, c->adj_u2.p);
     const int it = adj_cg_current(c, mode, b, b + N, lam, lam + N, tol, max_iter, resid);
-    adj_vjp_current(c, mode, nullptr, nullptr, lam, lam + N, nullptr, nullptr, c->adj_t3.p, c->adj_u3.p, c->adj_u4.p);
+    // lambda lives on the backbone; contacts with non-backbone particles are not in H, so they
+    // are left out of dF/dq too
+    adj_vjp_current(c, mode, nullptr, nullptr, lam, lam + N, nullptr, nullptr, c->adj_t3.p, c->adj_u3.p, c->adj_u4.p,
+                    c->adj_live.p);
     adj_add(c, N, c->adj_t1.p, c->adj_t3.p, c->adj_t3.p);
     if (dr) adj_to_canonical(c, c->adj_t3.p, dr);
     if (dLx) adj_add(c, S, c->adj_u1.p, c->adj_u3.p, dLx);
@@ -680,11 +683,13 @@ void adjoint_pressure_impl(dpmd_ctx* c, int mode, double tol, int max_iter, doub
     if (iters) *iters = it;
 }
 
-// Newton iterations x <- x + H^-1 F (conjugate gradients) from a configuration near a
-// minimum, until max|F| <= ftol.  Steps are damped to a largest displacement of
-// kNewtonMaxStep; a Hessian solve that does not converge is an error (the configuration is
-// not near a minimum).  Returns the final max|F| over all particles.
+// Levenberg-Marquardt Newton iterations x <- x + (H + mu I)^-1 F (MINRES, every particle and
+// contact, mu = kNewtonShift * k) from a configuration near a minimum, until max|F| <= ftol.
+// The shift makes the solve definite (rattlers, particles touching at the FIRE tolerance)
+// while real modes converge quadratically.  Steps are damped to a largest displacement of
+// kNewtonMaxStep; a solve that does not converge is an error.  Returns the final max|F|.
 constexpr double kNewtonMaxStep = 0.05;
+constexpr double kNewtonShift = 1e-10;
 double newton_impl(dpmd_ctx* c, int mode, int max_newton, double ftol, double cg_tol, int cg_max) {
     need_disk_cells(c);
     const int N = c->N, S = c->S;
@@ -692,6 +697,11 @@ double newton_impl(dpmd_ctx* c, int mode, int max_newton, double ftol, double cg
     c->adj_bx.resize(N);
     c->adj_u0.resize(S);
     std::vector<double> mf(S);
+    std::vector<double> ek(S);
+    HIP_OK(hipMemcpyAsync(ek.data(), c->e.p, (size_t)S * 8, hipMemcpyDeviceToHost, c->st));
+    HIP_OK(hipStreamSynchronize(c->st));
+    double kmax = 0.0;
+    for (double v : ek) kmax = std::max(kmax, v);
     auto maxf = [&]() {
         compute_forces_impl(c, mode);
         sys_reduce(c, RED_MAXF2, c->maxf2.p);
@@ -704,7 +714,8 @@ double newton_impl(dpmd_ctx* c, int mode, int max_newton, double ftol, double cg
     double f = maxf();
     for (int it = 0; it < max_newton && f > ftol; ++it) {
         double res = 0.0;
-        (void)adj_cg_current(c, mode, c->fx.p, c->fy.p, c->adj_t0.p, c->adj_t0.p + N, cg_tol, cg_max, &res);
+        (void)adj_cg_current(c, mode, c->fx.p, c->fy.p, c->adj_t0.p, c->adj_t0.p + N, cg_tol, cg_max, &res, false,
+                             kNewtonShift * kmax);
         if (!(res <= 1e-6))
             throw StateError("newton_minimize: the Hessian solve did not converge (relative residual " +
                              std::to_string(res) + "); run FIRE closer to the minimum first");

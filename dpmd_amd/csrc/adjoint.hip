@@ -21,6 +21,7 @@ struct AdjGeom {
     const int* sid; const int* off;
     const double* Lx; const double* Ly; const double* iLx; const double* iLy; const double* e;
     const int* boundary; const int* nn; const int* nbr; int n;
+    const double* live;   // backbone mask (1 = in the rigid backbone); null = every particle
 };
 
 AdjGeom geom_current(dpmd_ctx* c) {
@@ -29,6 +30,7 @@ AdjGeom geom_current(dpmd_ctx* c) {
     g.sid = c->sid.p; g.off = c->off.p;
     g.Lx = c->Lx.p; g.Ly = c->Ly.p; g.iLx = c->iLx.p; g.iLy = c->iLy.p; g.e = c->e.p; g.boundary = c->boundary.p;
     g.nn = c->nn.p; g.nbr = c->nbr.p; g.n = c->N;
+    g.live = nullptr;
     return g;
 }
 
@@ -72,9 +74,10 @@ __device__ __forceinline__ void pair_block(double e, double nx, double ny, doubl
 // y = H w (skip: systems whose rows are left untouched, may be null).
 template <int MODE>
 __global__ void __launch_bounds__(DPMD_BLOCK)
-k_hvp(AdjGeom g, const double* __restrict__ wx, const double* __restrict__ wy, double* yx, double* yy) {
+k_hvp(AdjGeom g, const double* __restrict__ wx, const double* __restrict__ wy, double* yx, double* yy, double shift) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.n) return;
+    if (g.live && g.live[i] == 0.0) { yx[i] = 0.0; yy[i] = 0.0; return; }
     const int s = g.sid[i];
     const bool wall = walled<MODE>(g, s);
     const double e = g.e[s];
@@ -88,13 +91,14 @@ k_hvp(AdjGeom g, const double* __restrict__ wx, const double* __restrict__ wy, d
         if (yi > g.Ly[s] - ri) ay += e * wyi;
     }
     for_contacts(g, i, s, wall, [&](int j, double, double, double nx, double ny, double d, double delta) {
+        if (g.live && g.live[j] == 0.0) return;
         double kx, ky;
         pair_block(e, nx, ny, d, delta, wxi - wx[j], wyi - wy[j], kx, ky);
         ax += kx;
         ay += ky;
     });
-    yx[i] = ax;
-    yy[i] = ay;
+    yx[i] = ax + shift * wxi;
+    yy[i] = ay + shift * wyi;
 }
 
 struct VjpIn {
@@ -116,7 +120,8 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_vjp(AdjGeom g, VjpIn in, VjpOut 
     const double Lx = g.Lx[s], Ly = g.Ly[s];
     const double gU = in.gU ? in.gU[s] : 0.0;
     const double gPA = in.gP ? in.gP[s] / (2.0 * (Lx * Ly)) : 0.0;   // dP/dW = 1 / (2 A)
-    const double gfm_x = in.gfx ? in.gfx[m] : 0.0, gfm_y = in.gfy ? in.gfy[m] : 0.0;
+    const bool mlive = !g.live || g.live[m] != 0.0;
+    const double gfm_x = (in.gfx && mlive) ? in.gfx[m] : 0.0, gfm_y = (in.gfy && mlive) ? in.gfy[m] : 0.0;
     const double xm = g.x[m], ym = g.y[m], rm = g.rad[m];
     double Fx = 0.0, Fy = 0.0;          // force on m (energy term: dU/dx = -F)
     double hx = 0.0, hy = 0.0;          // (H gF)_m
@@ -167,7 +172,7 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_vjp(AdjGeom g, VjpIn in, VjpOut 
         const double cx = qx * qx / (d * Lx), cy = qy * qy / (d * Ly);
         bx += gU * (-0.5 * ed * cx);
         by += gU * (-0.5 * ed * cy);
-        if (in.gfx) {
+        if (in.gfx && mlive && (!g.live || g.live[j] != 0.0)) {
             const double dgx = gfm_x - in.gfx[j], dgy = gfm_y - in.gfy[j];
             double kx, ky;
             pair_block(e, nx, ny, d, delta, dgx, dgy, kx, ky);
@@ -210,53 +215,144 @@ __global__ void k_to_canonical(int n, const int* __restrict__ si, const double* 
     if (c < n) canon[c] = cur[si[c]];
 }
 
-// ---- batched conjugate gradient (one independent solve per system) -----------------------
-struct CgVec { double* x; double* y; };
+// ---- batched MINRES (one independent solve per system) ----------------------------------
+// Paige-Saunders MINRES without preconditioner.  The Hessian at a FIRE minimum can be
+// slightly indefinite (nearly collinear two-contact particles: compressive contacts have
+// negative transverse stiffness) and is singular (translations, rattlers), so CG is not
+// safe; MINRES needs only symmetry and converges to the minimum-norm solution of a
+// consistent system.  Per-system scalars live in mr[s * MR_STRIDE + field].
+enum { MR_BETA1 = 0, MR_BETA, MR_OLDB, MR_EPS, MR_OLDEPS, MR_DBAR, MR_CS, MR_SN, MR_PHIBAR, MR_PHI, MR_GAMMA,
+       MR_DELTA, MR_STOP, MR_STRIDE };
+struct V2 { double* x; double* y; };
 
-// x = alpha p + x; r = r - alpha q with alpha_s = rr_s / pq_s on active systems.
-__global__ void k_cg_xr(int n, const int* sid, const int* active, const double* rr, const double* pq, CgVec x, CgVec r,
-                        CgVec p, CgVec q) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int s = sid[i];
-    if (!active[s] || pq[s] == 0.0) return;
-    const double a = rr[s] / pq[s];
-    x.x[i] += a * p.x[i]; x.y[i] += a * p.y[i];
-    r.x[i] -= a * q.x[i]; r.y[i] -= a * q.y[i];
-}
-
-// beta_s = rr_new / rr; rr = rr_new; a system is active while |r| > tol |b|.
-__global__ void k_cg_beta(int S, int* active, double* rr, const double* rrn, const double* bb, double* beta, double tol2) {
+__global__ void k_mr_init(int S, const double* bb, double* mr, int* active) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
-    if (!active[s]) { beta[s] = 0.0; return; }
-    beta[s] = rr[s] != 0.0 ? rrn[s] / rr[s] : 0.0;
-    rr[s] = rrn[s];
-    if (!(rrn[s] > tol2 * bb[s])) active[s] = 0;
+    double* m = mr + (size_t)s * MR_STRIDE;
+    const double b1 = sqrt(bb[s]);
+    for (int k = 0; k < MR_STRIDE; ++k) m[k] = 0.0;
+    m[MR_BETA1] = b1; m[MR_BETA] = b1; m[MR_PHIBAR] = b1; m[MR_CS] = -1.0;
+    active[s] = b1 > 0.0 ? 1 : 0;
 }
 
-__global__ void k_cg_p(int n, const int* sid, const int* active, const double* beta, CgVec p, CgVec r) {
+// v = r2 / beta
+__global__ void k_mr_v(int n, const int* sid, const int* active, const double* mr, V2 y, V2 v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = sid[i];
     if (!active[s]) return;
-    const double b = beta[s];
-    p.x[i] = r.x[i] + b * p.x[i];
-    p.y[i] = r.y[i] + b * p.y[i];
+    const double b = mr[(size_t)s * MR_STRIDE + MR_BETA];
+    v.x[i] = y.x[i] / b; v.y[i] = y.y[i] / b;
 }
 
-// subtract per-system means (periodic systems only: their Hessian has the two uniform
-// translations as null space)
-__global__ void k_sub_mean_pbc(int n, const int* sid, const int* off, const int* boundary, int mode, const double* sx,
-                               const double* sy, double* vx, double* vy) {
+// y -= (beta / oldb) r1  (from the second iteration on)
+__global__ void k_mr_y1(int n, const int* sid, const int* active, const double* mr, V2 y, V2 r1) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = sid[i];
+    if (!active[s]) return;
+    const double* m = mr + (size_t)s * MR_STRIDE;
+    const double f = m[MR_BETA] / m[MR_OLDB];
+    y.x[i] -= f * r1.x[i]; y.y[i] -= f * r1.y[i];
+}
+
+// y -= (alpha / beta) r2
+__global__ void k_mr_y2(int n, const int* sid, const int* active, const double* mr, const double* alpha, V2 y, V2 r2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = sid[i];
+    if (!active[s]) return;
+    const double f = alpha[s] / mr[(size_t)s * MR_STRIDE + MR_BETA];
+    y.x[i] -= f * r2.x[i]; y.y[i] -= f * r2.y[i];
+}
+
+// Lanczos / Givens scalar recurrences; stop when the residual estimate phibar <= tol beta1.
+__global__ void k_mr_scalars(int S, const int* active, double* mr, const double* alpha, const double* bnew2, double tol) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || !active[s]) return;
+    double* m = mr + (size_t)s * MR_STRIDE;
+    const double a = alpha[s];
+    m[MR_OLDB] = m[MR_BETA];
+    const double beta = sqrt(bnew2[s]);
+    m[MR_BETA] = beta;
+    m[MR_OLDEPS] = m[MR_EPS];
+    const double cs = m[MR_CS], sn = m[MR_SN], dbar = m[MR_DBAR];
+    m[MR_DELTA] = cs * dbar + sn * a;
+    const double gbar = sn * dbar - cs * a;
+    m[MR_EPS] = sn * beta;
+    m[MR_DBAR] = -cs * beta;
+    double gamma = sqrt(gbar * gbar + beta * beta);
+    gamma = fmax(gamma, 1e-300);
+    m[MR_GAMMA] = gamma;
+    m[MR_CS] = gbar / gamma;
+    m[MR_SN] = beta / gamma;
+    m[MR_PHI] = m[MR_CS] * m[MR_PHIBAR];
+    m[MR_PHIBAR] = m[MR_SN] * m[MR_PHIBAR];
+    if (m[MR_PHIBAR] <= tol * m[MR_BETA1] || beta == 0.0) m[MR_STOP] = 1.0;
+}
+
+// w_new = (v - oldeps w2_old - delta w_old) / gamma (into the old w1 buffer); x += phi w_new
+__global__ void k_mr_w(int n, const int* sid, const int* active, const double* mr, V2 v, V2 w2, V2 w, V2 wnew, V2 x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = sid[i];
+    if (!active[s]) return;
+    const double* m = mr + (size_t)s * MR_STRIDE;
+    const double oe = m[MR_OLDEPS], de = m[MR_DELTA], g = m[MR_GAMMA], ph = m[MR_PHI];
+    const double nx = (v.x[i] - oe * w2.x[i] - de * w.x[i]) / g;
+    const double ny = (v.y[i] - oe * w2.y[i] - de * w.y[i]) / g;
+    wnew.x[i] = nx; wnew.y[i] = ny;
+    x.x[i] += ph * nx; x.y[i] += ph * ny;
+}
+
+__global__ void k_mr_commit(int S, const double* mr, int* active) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S && active[s] && mr[(size_t)s * MR_STRIDE + MR_STOP] != 0.0) active[s] = 0;
+}
+
+// One pass of rattler removal: a particle stays in the backbone while it has at least
+// d + 1 = 3 contacts (pair contacts with backbone particles, plus wall contacts).  Repeated
+// until nothing changes, this leaves the rigid backbone (the 3-core of the contact network);
+// the removed particles' coordinates are (near-)null directions of the Hessian.
+template <int MODE>
+__global__ void __launch_bounds__(DPMD_BLOCK) k_kcore(AdjGeom g, const double* lin, double* lout, int* changed) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    if (lin[i] == 0.0) { lout[i] = 0.0; return; }
+    const int s = g.sid[i];
+    const bool wall = walled<MODE>(g, s);
+    int cnt = 0;
+    if (wall) {
+        const double xi = g.x[i], yi = g.y[i], ri = g.rad[i];
+        cnt += (xi < ri) + (xi > g.Lx[s] - ri) + (yi < ri) + (yi > g.Ly[s] - ri);
+    }
+    for_contacts(g, i, s, wall, [&](int j, double, double, double, double, double, double) { cnt += lin[j] != 0.0; });
+    const bool keep = cnt >= 3;
+    lout[i] = keep ? 1.0 : 0.0;
+    if (!keep) atomicOr(changed, 1);
+}
+
+__global__ void k_norm2_masked(int n, const double* ax, const double* ay, const double* live, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = live[i] != 0.0 ? ax[i] * ax[i] + ay[i] * ay[i] : 0.0;
+}
+
+__global__ void k_mask_live(int n, const double* live, double* vx, double* vy) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && live[i] == 0.0) { vx[i] = 0.0; vy[i] = 0.0; }
+}
+
+// subtract the per-system mean over live particles (periodic systems only: a uniform
+// translation of the contact network is a null direction)
+__global__ void k_sub_mean_live(int n, const int* sid, const int* boundary, int mode, const double* live,
+                                const double* cnt, const double* sx, const double* sy, double* vx, double* vy) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || live[i] == 0.0) return;
+    const int s = sid[i];
     const bool wall = mode == DPMD_FORCE_WALL || (mode == DPMD_FORCE_PER_SYSTEM && boundary[s]);
-    if (wall) return;
-    const double cnt = (double)(off[s + 1] - off[s]);
-    vx[i] -= sx[s] / cnt;
-    vy[i] -= sy[s] / cnt;
+    if (wall || cnt[s] == 0.0) return;
+    vx[i] -= sx[s] / cnt[s];
+    vy[i] -= sy[s] / cnt[s];
 }
 
 __global__ void k_copy2(int n, const double* ax, const double* ay, double* bx, double* by) {
@@ -417,7 +513,7 @@ void adj_hvp_segment(dpmd_ctx* c, int mode, int slot, const double* rad_c, const
     g.nn = c->ck_nn.p;
     g.nbr = c->ck_nbr.p;
     by_mode(mode, [&](auto m) {
-        k_hvp<decltype(m)::value><<<blocks_for(N), DPMD_BLOCK, 0, c->st>>>(g, w, w + N, z, z + N);
+        k_hvp<decltype(m)::value><<<blocks_for(N), DPMD_BLOCK, 0, c->st>>>(g, w, w + N, z, z + N, 0.0);
     });
     HIP_OK(hipGetLastError());
 }
@@ -444,19 +540,61 @@ void adj_to_canonical(dpmd_ctx* c, const double* cur, double* canon) {
     HIP_OK(hipGetLastError());
 }
 
-void adj_hvp_current(dpmd_ctx* c, int mode, const double* wx, const double* wy, double* yx, double* yy) {
-    const AdjGeom g = geom_current(c);
+// The rigid backbone of the current contact network, into c->adj_live (1 / 0 per particle).
+const double* adj_backbone(dpmd_ctx* c, int mode) {
+    const int N = c->N;
+    hipStream_t st = c->st;
+    c->adj_live.resize(N); c->adj_live2.resize(N); c->adj_flag.resize(1);
+    fill_f64(st, c->adj_live.p, N, 1.0);
+    AdjGeom g = geom_current(c);
+    double* a = c->adj_live.p;
+    double* b = c->adj_live2.p;
+    int changed = 1;
+    for (int pass = 0; changed && pass < 100000; ++pass) {
+        fill_i32(st, c->adj_flag.p, 1, 0);
+        by_mode(mode, [&](auto m) {
+            k_kcore<decltype(m)::value><<<blocks_for(N), DPMD_BLOCK, 0, st>>>(g, a, b, c->adj_flag.p);
+        });
+        HIP_OK(hipGetLastError());
+        std::swap(a, b);
+        HIP_OK(hipMemcpyAsync(&changed, c->adj_flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+    }
+    if (a != c->adj_live.p) HIP_OK(hipMemcpyAsync(c->adj_live.p, a, (size_t)N * 8, hipMemcpyDeviceToDevice, st));
+    return c->adj_live.p;
+}
+
+double adj_backbone_max_force(dpmd_ctx* c, const double* live) {
+    const int N = c->N, S = c->S;
+    c->adj_bx.resize(N);
+    ensure(c->adj_sa, S, c->st);
+    k_norm2_masked<<<blocks_for(N), DPMD_BLOCK, 0, c->st>>>(N, c->fx.p, c->fy.p, live, c->adj_bx.p);
+    HIP_OK(hipGetLastError());
+    sys_reduce_max(c, c->adj_bx.p, c->adj_sa.p);
+    std::vector<double> m(S);
+    HIP_OK(hipMemcpyAsync(m.data(), c->adj_sa.p, (size_t)S * 8, hipMemcpyDeviceToHost, c->st));
+    HIP_OK(hipStreamSynchronize(c->st));
+    double w = 0.0;
+    for (double v : m) w = std::max(w, v);
+    return std::sqrt(w);
+}
+
+void adj_hvp_current(dpmd_ctx* c, int mode, const double* wx, const double* wy, double* yx, double* yy,
+                     const double* live, double shift) {
+    AdjGeom g = geom_current(c);
+    g.live = live;
     by_mode(mode, [&](auto m) {
-        k_hvp<decltype(m)::value><<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(g, wx, wy, yx, yy);
+        k_hvp<decltype(m)::value><<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(g, wx, wy, yx, yy, shift);
     });
     HIP_OK(hipGetLastError());
 }
 
 void adj_vjp_current(dpmd_ctx* c, int mode, const double* gU, const double* gP, const double* gfx, const double* gfy,
-                     double* dx, double* dy, double* dr, double* dLx, double* dLy) {
+                     double* dx, double* dy, double* dr, double* dLx, double* dLy, const double* live) {
     const int N = c->N, S = c->S;
     ensure(c->adj_bx, N, c->st); ensure(c->adj_by, N, c->st);
-    const AdjGeom g = geom_current(c);
+    AdjGeom g = geom_current(c);
+    g.live = live;
     VjpIn in{gU, gP, gfx, gfy};
     VjpOut out{dx, dy, dr, c->adj_bx.p, c->adj_by.p};
     by_mode(mode, [&](auto m) { k_vjp<decltype(m)::value><<<blocks_for(N), DPMD_BLOCK, 0, c->st>>>(g, in, out); });
@@ -477,47 +615,75 @@ void adj_vjp_current(dpmd_ctx* c, int mode, const double* gU, const double* gP, 
     }
 }
 
-// Solves H x = b per system by conjugate gradients (current order).  Periodic systems are
-// solved in the subspace orthogonal to uniform translations (b and the result are
-// projected).  Returns the iterations taken; *rel_resid = max_s |b - H x| / |b| (true
-// residual, recomputed at the end).
+// Solves (H + shift I) x = b per system by MINRES (current order).  With on_backbone the
+// Hessian is restricted to the rigid backbone (backbone particles and their mutual contacts)
+// and b and the result are projected onto the complement of its null directions
+// (non-backbone coordinates; translations of the backbone in periodic systems); the mask
+// stays in c->adj_live.  Otherwise every particle takes part (use shift > 0).  Returns the
+// iterations taken; *rel_resid = max_s |b - H x| / |b| (true residual, recomputed at the end).
 int adj_cg_current(dpmd_ctx* c, int mode, const double* bx, const double* by, double* xx, double* xy, double tol,
-                   int max_iter, double* rel_resid) {
+                   int max_iter, double* rel_resid, bool on_backbone, double shift) {
     const int N = c->N, S = c->S;
     hipStream_t st = c->st;
-    dpmd::DVec<double>* vs[] = {&c->cg_rx, &c->cg_ry, &c->cg_px, &c->cg_py, &c->cg_qx, &c->cg_qy};
-    for (auto* v : vs) ensure(*v, N, st);
-    dpmd::DVec<double>* ss[] = {&c->cg_rr, &c->cg_rrn, &c->cg_pq, &c->cg_bb, &c->cg_beta, &c->adj_sa, &c->adj_sb};
+    ensure(c->mr_vec, 12 * (int64_t)N, st);
+    dpmd::DVec<double>* ss[] = {&c->cg_rr, &c->cg_rrn, &c->cg_pq, &c->cg_bb, &c->adj_sa, &c->adj_sb};
     for (auto* v : ss) ensure(*v, S, st);
+    ensure(c->mr_s, (int64_t)S * MR_STRIDE, st);
     c->cg_active.resize(S);
     const int G = blocks_for(N), GS = blocks_for(S);
-    // r = b (projected), x = 0
-    k_copy2<<<G, DPMD_BLOCK, 0, st>>>(N, bx, by, c->cg_rx.p, c->cg_ry.p);
-    sys_reduce_arr(c, c->cg_rx.p, c->adj_sa.p);
-    sys_reduce_arr(c, c->cg_ry.p, c->adj_sb.p);
-    k_sub_mean_pbc<<<G, DPMD_BLOCK, 0, st>>>(N, c->sid.p, c->off.p, c->boundary.p, mode, c->adj_sa.p, c->adj_sb.p,
-                                            c->cg_rx.p, c->cg_ry.p);
-    fill_f64(st, xx, N, 0.0);
-    fill_f64(st, xy, N, 0.0);
-    k_copy2<<<G, DPMD_BLOCK, 0, st>>>(N, c->cg_rx.p, c->cg_ry.p, c->cg_px.p, c->cg_py.p);
-    sys_reduce_dot(c, c->cg_rx.p, c->cg_rx.p, c->cg_ry.p, c->cg_ry.p, c->cg_rr.p);
-    HIP_OK(hipMemcpyAsync(c->cg_bb.p, c->cg_rr.p, (size_t)S * 8, hipMemcpyDeviceToDevice, st));
-    k_init_active<<<GS, DPMD_BLOCK, 0, st>>>(S, c->cg_bb.p, c->cg_active.p);
+    auto vec = [&](int k) { return V2{c->mr_vec.p + (size_t)(2 * k) * N, c->mr_vec.p + (size_t)(2 * k + 1) * N}; };
+    // null directions: coordinates outside the rigid backbone and (periodic systems)
+    // translations of the backbone
+    ensure(c->cg_beta, S, st);
+    const double* live = nullptr;
+    if (on_backbone) {
+        live = adj_backbone(c, mode);
+    } else {   // every particle: with shift > 0 the shifted matrix is definite
+        c->adj_live.resize(N);
+        fill_f64(st, c->adj_live.p, N, 1.0);
+    }
+    const double* mask = c->adj_live.p;
+    double* nlive = c->cg_beta.p;
+    sys_reduce_arr(c, mask, nlive);
+    auto project = [&](double* vx, double* vy) {
+        k_mask_live<<<G, DPMD_BLOCK, 0, st>>>(N, mask, vx, vy);
+        sys_reduce_arr(c, vx, c->adj_sa.p);
+        sys_reduce_arr(c, vy, c->adj_sb.p);
+        k_sub_mean_live<<<G, DPMD_BLOCK, 0, st>>>(N, c->sid.p, c->boundary.p, mode, mask, nlive, c->adj_sa.p,
+                                                 c->adj_sb.p, vx, vy);
+    };
+    V2 r1 = vec(0), r2 = vec(1), y = vec(2), v = vec(3), w1 = vec(4), w2 = vec(5);
+    ensure(c->cg_qx, N, st); ensure(c->cg_qy, N, st);
+    V2 w{c->cg_qx.p, c->cg_qy.p};
+    const V2 X{xx, xy};
+    // b (projected) -> r1 = r2; x = w = w1 = w2 = 0
+    k_copy2<<<G, DPMD_BLOCK, 0, st>>>(N, bx, by, r1.x, r1.y);
+    project(r1.x, r1.y);
+    k_copy2<<<G, DPMD_BLOCK, 0, st>>>(N, r1.x, r1.y, r2.x, r2.y);
+    for (double* p : {xx, xy, w.x, w.y, w1.x, w1.y, w2.x, w2.y}) fill_f64(st, p, N, 0.0);
+    sys_reduce_dot(c, r1.x, r1.x, r1.y, r1.y, c->cg_bb.p);
+    k_mr_init<<<GS, DPMD_BLOCK, 0, st>>>(S, c->cg_bb.p, c->mr_s.p, c->cg_active.p);
     HIP_OK(hipGetLastError());
-    const CgVec X{xx, xy}, R{c->cg_rx.p, c->cg_ry.p}, Pv{c->cg_px.p, c->cg_py.p}, Q{c->cg_qx.p, c->cg_qy.p};
-    const double tol2 = tol * tol;
     int it = 0;
     std::vector<int> act(S);
     while (it < max_iter) {
-        adj_hvp_current(c, mode, Pv.x, Pv.y, Q.x, Q.y);
-        sys_reduce_dot(c, Pv.x, Q.x, Pv.y, Q.y, c->cg_pq.p);
-        k_cg_xr<<<G, DPMD_BLOCK, 0, st>>>(N, c->sid.p, c->cg_active.p, c->cg_rr.p, c->cg_pq.p, X, R, Pv, Q);
-        sys_reduce_dot(c, R.x, R.x, R.y, R.y, c->cg_rrn.p);
-        k_cg_beta<<<GS, DPMD_BLOCK, 0, st>>>(S, c->cg_active.p, c->cg_rr.p, c->cg_rrn.p, c->cg_bb.p, c->cg_beta.p,
-                                             tol2);
-        k_cg_p<<<G, DPMD_BLOCK, 0, st>>>(N, c->sid.p, c->cg_active.p, c->cg_beta.p, Pv, R);
-        HIP_OK(hipGetLastError());
         ++it;
+        k_mr_v<<<G, DPMD_BLOCK, 0, st>>>(N, c->sid.p, c->cg_active.p, c->mr_s.p, r2, v);
+        adj_hvp_current(c, mode, v.x, v.y, y.x, y.y, live, shift);
+        if (it >= 2) k_mr_y1<<<G, DPMD_BLOCK, 0, st>>>(N, c->sid.p, c->cg_active.p, c->mr_s.p, y, r1);
+        sys_reduce_dot(c, v.x, y.x, v.y, y.y, c->cg_pq.p);               // alpha
+        k_mr_y2<<<G, DPMD_BLOCK, 0, st>>>(N, c->sid.p, c->cg_active.p, c->mr_s.p, c->cg_pq.p, y, r2);
+        sys_reduce_dot(c, y.x, y.x, y.y, y.y, c->cg_rrn.p);              // beta_new^2
+        k_mr_scalars<<<GS, DPMD_BLOCK, 0, st>>>(S, c->cg_active.p, c->mr_s.p, c->cg_pq.p, c->cg_rrn.p, tol);
+        k_mr_w<<<G, DPMD_BLOCK, 0, st>>>(N, c->sid.p, c->cg_active.p, c->mr_s.p, v, w2, w, w1, X);
+        k_mr_commit<<<GS, DPMD_BLOCK, 0, st>>>(S, c->mr_s.p, c->cg_active.p);
+        HIP_OK(hipGetLastError());
+        // r1 <- r2, r2 <- y, y <- the old r1 buffer
+        const V2 old_r1 = r1;
+        r1 = r2; r2 = y; y = old_r1;
+        // w1 <- w2, w2 <- w, w <- the new direction (written into the old w1 buffer)
+        const V2 old_w1 = w1;
+        w1 = w2; w2 = w; w = old_w1;
         if (it % 16 == 0 || it == max_iter) {
             HIP_OK(hipMemcpyAsync(act.data(), c->cg_active.p, (size_t)S * 4, hipMemcpyDeviceToHost, st));
             HIP_OK(hipStreamSynchronize(st));
@@ -527,14 +693,11 @@ int adj_cg_current(dpmd_ctx* c, int mode, const double* bx, const double* by, do
         }
     }
     // project the solution, then the true residual b - H x
-    sys_reduce_arr(c, xx, c->adj_sa.p);
-    sys_reduce_arr(c, xy, c->adj_sb.p);
-    k_sub_mean_pbc<<<G, DPMD_BLOCK, 0, st>>>(N, c->sid.p, c->off.p, c->boundary.p, mode, c->adj_sa.p, c->adj_sb.p, xx, xy);
-    adj_hvp_current(c, mode, xx, xy, Q.x, Q.y);
+    V2 Q = vec(3), R = vec(2);
+    project(xx, xy);
+    adj_hvp_current(c, mode, xx, xy, Q.x, Q.y, live, shift);
     k_copy2<<<G, DPMD_BLOCK, 0, st>>>(N, bx, by, R.x, R.y);
-    sys_reduce_arr(c, R.x, c->adj_sa.p);
-    sys_reduce_arr(c, R.y, c->adj_sb.p);
-    k_sub_mean_pbc<<<G, DPMD_BLOCK, 0, st>>>(N, c->sid.p, c->off.p, c->boundary.p, mode, c->adj_sa.p, c->adj_sb.p, R.x, R.y);
+    project(R.x, R.y);
     k_axpy2<<<G, DPMD_BLOCK, 0, st>>>(N, -1.0, Q.x, Q.y, R.x, R.y);
     sys_reduce_dot(c, R.x, R.x, R.y, R.y, c->cg_rrn.p);
     HIP_OK(hipGetLastError());

@@ -70,12 +70,32 @@ def pressure(x, y, r, m, vx, vy, Lx, Ly, k, wall):
     return (np.sum(m * (vx * vx + vy * vy)) + W) / (2.0 * Lx * Ly)
 
 
-def hessian(x, y, r, Lx, Ly, k, wall):
-    """Dense 2n x 2n Hessian of U, ordering (x0, y0, x1, y1, ...)."""
+def backbone(x, y, r, Lx, Ly, wall):
+    """Rigid backbone: repeatedly drop particles with fewer than d + 1 = 3 contacts (pair
+    contacts with kept particles plus wall contacts); returns the boolean mask."""
+    n = len(x)
+    I, J, *_ = contacts(x, y, r, Lx, Ly, wall)
+    wc = np.zeros(n, int)
+    if wall:
+        wc = sum((o > 0).astype(int) for o in wall_gaps(x, y, r, Lx, Ly))
+    live = np.ones(n, bool)
+    while True:
+        cnt = wc + np.bincount(I[live[J]], minlength=n) + np.bincount(J[live[I]], minlength=n)
+        new = live & (cnt >= 3)
+        if np.array_equal(new, live):
+            return live
+        live = new
+
+
+def hessian(x, y, r, Lx, Ly, k, wall, live=None):
+    """Dense 2n x 2n Hessian of U, ordering (x0, y0, x1, y1, ...).  With a mask, restricted
+    to the masked particles and their mutual contacts."""
     n = len(x)
     H = np.zeros((2 * n, 2 * n))
     I, J, qx, qy, d, de = contacts(x, y, r, Lx, Ly, wall)
     for a, b, ux, uy, dd, dl in zip(I, J, qx / d, qy / d, d, de):
+        if live is not None and not (live[a] and live[b]):
+            continue
         nn = np.array([[ux * ux, ux * uy], [uy * ux, uy * uy]])
         K = k * nn - (k * dl / dd) * (np.eye(2) - nn)
         for p, q, sgn in ((a, a, 1), (b, b, 1), (a, b, -1), (b, a, -1)):
@@ -83,14 +103,17 @@ def hessian(x, y, r, Lx, Ly, k, wall):
     if wall:
         olx, ohx, oly, ohy = wall_gaps(x, y, r, Lx, Ly)
         for i in range(n):
+            if live is not None and not live[i]:
+                continue
             H[2 * i, 2 * i] += k * ((olx[i] > 0) + (ohx[i] > 0))
             H[2 * i + 1, 2 * i + 1] += k * ((oly[i] > 0) + (ohy[i] > 0))
     return H
 
 
-def grads(x, y, r, m, vx, vy, Lx, Ly, k, wall):
+def grads(x, y, r, m, vx, vy, Lx, Ly, k, wall, live=None):
     """Analytic first derivatives: dU/dx (2n, interleaved), dU/dr, dU/dL (2,), dP/dx, dP/dr,
-    dP/dL (box at fixed reduced coordinates, velocities fixed), dF/dr (2n x n), dF/dL (2n x 2)."""
+    dP/dL (box at fixed reduced coordinates, velocities fixed), dF/dr (2n x n), dF/dL (2n x 2).
+    With a mask, dF/dr and dF/dL keep only the masked particles' mutual contacts and walls."""
     n = len(x)
     A = Lx * Ly
     I, J, qx, qy, d, de = contacts(x, y, r, Lx, Ly, wall)
@@ -119,6 +142,8 @@ def grads(x, y, r, m, vx, vy, Lx, Ly, k, wall):
     dFdr = np.zeros((2 * n, n))
     dFdL = np.zeros((2 * n, 2))
     for a, b, ex, ey, qqx, qqy, dd, dl in zip(I, J, ux, uy, qx, qy, d, de):
+        if live is not None and not (live[a] and live[b]):
+            continue
         nn = np.array([[ex * ex, ex * ey], [ey * ex, ey * ey]])
         K = k * nn - (k * dl / dd) * (np.eye(2) - nn)
         for p, sgn in ((a, 1.0), (b, -1.0)):
@@ -132,6 +157,8 @@ def grads(x, y, r, m, vx, vy, Lx, Ly, k, wall):
         dUdL += np.array([np.sum(-k * olx * x / Lx) + np.sum(-k * ohx * (Lx - x) / Lx),
                           np.sum(-k * oly * y / Ly) + np.sum(-k * ohy * (Ly - y) / Ly)])
         for i in range(n):
+            if live is not None and not live[i]:
+                continue
             dFdr[2 * i, i] += k * (float(olx[i] > 0) - float(ohx[i] > 0))
             dFdr[2 * i + 1, i] += k * (float(oly[i] > 0) - float(ohy[i] > 0))
             dFdL[2 * i, 0] += -k * x[i] / Lx * (olx[i] > 0) + k * (Lx - x[i]) / Lx * (ohx[i] > 0)
@@ -142,11 +169,14 @@ def grads(x, y, r, m, vx, vy, Lx, Ly, k, wall):
     return dict(dUdx=dUdx, dUdr=dUdr, dUdL=dUdL, dPdx=dPdx, dPdr=dPdr, dPdL=dPdL, dFdr=dFdr, dFdL=dFdL, P=P)
 
 
-def implicit_pressure_grad(x, y, r, m, vx, vy, Lx, Ly, k, wall):
-    """dP/dr and dP/dL through the minimum: dP/dq + lambda^T dF/dq, H lambda = dP/dx
-    (minimum-norm least squares, which removes the translation null space)."""
-    g = grads(x, y, r, m, vx, vy, Lx, Ly, k, wall)
-    H = hessian(x, y, r, Lx, Ly, k, wall)
+def implicit_pressure_grad(x, y, r, m, vx, vy, Lx, Ly, k, wall, live=None):
+    """dP/dr and dP/dL through the minimum: dP/dq + lambda^T dF/dq, H lambda = dP/dx, on the
+    rigid backbone (or the given mask; minimum-norm least squares, which removes translations
+    and the coordinates outside the mask)."""
+    if live is None:
+        live = backbone(x, y, r, Lx, Ly, wall)
+    g = grads(x, y, r, m, vx, vy, Lx, Ly, k, wall, live)
+    H = hessian(x, y, r, Lx, Ly, k, wall, live)
     lam = np.linalg.lstsq(H, g["dPdx"], rcond=1e-13)[0]
     return g["dPdr"] + lam @ g["dFdr"], g["dPdL"] + lam @ g["dFdL"], lam
 

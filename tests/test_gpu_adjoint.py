@@ -96,6 +96,25 @@ def test_cg_solution_and_pressure_gradient_match_host(minimum):
         close(np.array([dLx[s], dLy[s]]), hdL, 1e-7)
 
 
+def test_pressure_gradient_full_size_systems_match_host():
+    """Systems of the differentiation workload's size (1024 disks at phi = 0.86, one periodic,
+    one walled): the GPU implicit gradient (MINRES on the rigid backbone) against the dense
+    host reference (least squares on the same backbone)."""
+    b = W.disk_batch([1024, 1024], [0.86, 0.86], [0, 1], seed=1)
+    P = make(b)
+    print(f"adjoint full-size S={b.S} N={b.N}", flush=True)
+    steps, conv = run_fire(P, 1e-2, 300000, L.FIRE_PRED_MAX_FORCE, 1e-12, mode=MODE)
+    assert conv
+    P.zero_velocities()
+    dr, dLx, dLy, it, res = A.pressure_grad(P, MODE, tol=1e-11, max_iter=50000)
+    assert res <= 1e-10, res
+    z = np.zeros(b.N)
+    for s, sl, args in per_system(b, P, z, z):
+        hdr, hdL, _ = R.implicit_pressure_grad(*args)
+        close(dr[sl], hdr, 1e-7)
+        close(np.array([dLx[s], dLy[s]]), hdL, 1e-7)
+
+
 def test_pressure_gradient_matches_gpu_finite_differences(minimum):
     """Perturb one radius by +-h, re-minimise on the GPU (Newton from the minimum), difference
     the pressures."""

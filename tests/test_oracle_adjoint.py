@@ -111,7 +111,8 @@ def test_ring_implicit_pressure_closed_form():
     T = k (2 sum r - Lx) / 3, and the virial W = sum k delta d = T Lx, so
     P = k (2 sum r - Lx) / (6 Ly):  dP/dr_m = k / (3 Ly) for every disk (relaxation included;
     the unrelaxed partial k (d_left + d_right) / (2 Lx Ly) differs for the middle disk),
-    dP/dLx = -k / (6 Ly) and dP/dLy = -P / Ly."""
+    dP/dLx = -k / (6 Ly) and dP/dLy = -P / Ly.  (A ring is a one-dimensional network: the 2-D
+    backbone rule, three contacts, would drop it, so the full network is used.)"""
     Lx, Ly, k = 3.0, 10.0, 1.0
     r = np.array([0.55, 0.6, 0.5])
     x = np.array([0.5, 1.55, 2.55]); y = np.full(3, 5.0); m = np.ones(3); z = np.zeros(3)
@@ -119,7 +120,8 @@ def test_ring_implicit_pressure_closed_form():
     assert np.max(np.abs(fx)) < 1e-14 and np.max(np.abs(fy)) == 0.0
     P = R.pressure(x, y, r, m, z, z, Lx, Ly, k, False)
     assert P == pytest.approx(k * (2 * r.sum() - Lx) / (6 * Ly), rel=1e-13)
-    dr, dL, lam = R.implicit_pressure_grad(x, y, r, m, z, z, Lx, Ly, k, False)
+    assert not R.backbone(x, y, r, Lx, Ly, False).any()
+    dr, dL, lam = R.implicit_pressure_grad(x, y, r, m, z, z, Lx, Ly, k, False, live=np.ones(3, bool))
     np.testing.assert_allclose(dr, [k / (3 * Ly)] * 3, rtol=1e-12)
     np.testing.assert_allclose(dL, [-k / (6 * Ly), -P / Ly], rtol=1e-12)
     g = R.grads(x, y, r, m, z, z, Lx, Ly, k, False)
