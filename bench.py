@@ -6,8 +6,8 @@ velocity-Verlet (NVE) steps at dt = 1e-2 from Gaussian velocities at T = 1e-4.  
 schedule is the workload's 2000 FIRE + 5000 NVE steps; --steps K keeps that 2:5 split.
 
 value = N * (FIRE steps + NVE steps) / wall time of the timed region (inputs resident in
-HBM).  The dominant kernel's roofline figure comes from HIP events around every force
-kernel launch of the NVE phase; cpu_baseline times the oracle (the C++ FP64 restatement)
+HBM).  The dominant kernel's roofline figure comes from HIP events around every fused
+velocity-Verlet launch of the last NVE steps; cpu_baseline times the oracle (the C++ FP64 restatement)
 on a bounded sample on this host.
 """
 from __future__ import annotations
@@ -24,6 +24,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_vv_fused launch on the default workload (N = 2^20, phi = 0.84), from the
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes in profiles/k_vv_fused_disk_N1M_fire_nve.md, each
+# scaled by what the counter reads for a copy kernel of known size (FETCH x 2.0, WRITE x 1.0).
+FUSED_TRAFFIC_N1M = (51687.0 * (131072.0 / 65545.5) + 57344.0) * 1024.0
 
 
 def log(msg: str) -> None:
@@ -341,8 +345,8 @@ def main():
     P.synchronize()
 
 
-    # NVE: the bulk runs as lean graph batches; the last n_prof steps run one launch per
-    # kernel with HIP events around every force launch (the roofline measurement).
+    # NVE: the bulk runs as lean graph batches; the last n_prof steps run the same batches
+    # launched directly, with HIP events around every fused launch (the roofline measurement).
     n_prof = min(500, max(1, n_nve // 5))
     log(f"timed region: {n_fire} FIRE + {n_nve} NVE steps, N={N}")
     t0 = time.perf_counter()
@@ -356,20 +360,22 @@ def main():
     vv = VelocityVerlet(P, 1e-2)
     vv.init()
     vv.run(n_nve - n_prof, use_graph=not args.no_graph)
-    lib.dpmd_profile(P.ctx, 1)
-    vv.step(n_prof)
+    lib.dpmd_profile(P.ctx, 2)                       # HIP events around every fused launch
+    vv.run(n_prof, use_graph=False)
     P.synchronize()
     t = time.perf_counter() - t0
     log(f"NVE done: {n_nve} steps, {t - t_fire:.3f} s")
 
+    # one HIP event pair around the fused launches of every batch that no rebuild halted
     tot_ms, cnt = C.c_double(), C.c_int()
-    lib.dpmd_profile_read(P.ctx, C.byref(tot_ms), C.byref(cnt))
+    L.check(lib.dpmd_profile_read(P.ctx, C.byref(tot_ms), C.byref(cnt)), P.ctx)
     lib.dpmd_profile(P.ctx, 0)
     cap, mx, nnb = P.neighbor_info()
     k = nnb / N
-    # k_vv_force per particle: reads x,y,rad (24), sid (4), count (4), k ids (4k), vx,vy,mass (24);
-    # writes fx,fy,pe,vx,vy (40)  -> 96 + 4k bytes (neighbor coordinate gathers excluded).
-    bytes_per_launch = N * (96 + 4 * k)
+    # k_vv_fused per particle: reads x,y,rad (24), sid (4), count (4), k ids (4k), vx,vy,mass (24),
+    # last-rebuild x,y (16); writes fx,fy,pe,vx,vy (40) and the new x,y (16)
+    # -> 128 + 4k bytes (neighbor coordinate gathers excluded).
+    bytes_per_launch = N * (128 + 4 * k)
     avg_ms = tot_ms.value / max(1, cnt.value)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
 
@@ -404,9 +410,11 @@ def main():
                    "n_particles": N, "fire_steps": steps_fire, "nve_steps": n_nve,
                    "fire_seconds": t_fire, "nve_seconds": t - t_fire, "nve_profiled_steps": n_prof,
                    "mean_neighbors": k, "neighbor_capacity": cap, "max_neighbors": mx},
-        "roofline": {"bound": "hbm", "kernel": "k_vv_force (force + half kick, NVE)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_vv_fused (NVE: forces, closing half kick, next opening half kick, drift, skin test)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": FUSED_TRAFFIC_N1M if N == 1 << 20 else None,
                      "avg_launch_us": avg_ms * 1e3, "launches": cnt.value,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "copy_kernel_gbs": copy_gbs},

@@ -17,7 +17,7 @@ NEIGHBOR_NAIVE, NEIGHBOR_CELL = 0, 1
 FORCE_PBC, FORCE_WALL, FORCE_PER_SYSTEM = 0, 1, 2
 ORDER_CURRENT, ORDER_CANONICAL = 0, 1
 FIRE_PRED_REFERENCE, FIRE_PRED_MAX_FORCE = 0, 1
-OPT_COOP_REBUILD, OPT_COOP_IN_GRAPH = 0, 1
+OPT_COOP_REBUILD, OPT_COOP_IN_GRAPH, OPT_FUSED_VV = 0, 1, 2
 
 STATUS = {0: "OK", 1: "ARGUMENT", 2: "STATE", 3: "HIP", 4: "OVERFLOW"}
 
@@ -113,6 +113,7 @@ PROTOS = {
     "dpmd_fill_i32": (i32, [vp, i64, C.c_int32]),
     "dpmd_synchronize": (i32, [vp]),
     "dpmd_set_option": (i32, [vp, i32, i32]),
+    "dpmd_profile_durations": (i32, [vp, vp, i32, pi32]),
     "dpmd_get_option": (i32, [vp, i32, pi32]),
     "dpmd_profile": (i32, [vp, i32]),
     "dpmd_profile_read": (i32, [vp, C.POINTER(f64), pi32]),

@@ -394,7 +394,40 @@ RunCtl read_run(dpmd_ctx* c) {
     return RunCtl{run[RUN_RUNNING], run[RUN_HALT], run[RUN_HALT_AT], run[RUN_CONV_AT]};
 }
 
+// Fused velocity-Verlet batches (disks, even batch sizes): drift of step 0 into the second
+// position buffer B, then m - 1 launches that each finish step q (forces at x_q, closing half
+// kick) and start step q + 1 (opening half kick, drift into the other buffer), then the
+// forces and closing half kick of step m - 1.  x_q lives in B for even q and in the primary
+// buffer A for odd q, so an even batch ends with the positions back in A.  The rebuild flags
+// of x_q go to run word W(q) (RUN_ANY for even q, RUN_ANY2 for odd q) and gate the launch
+// that computes forces at x_q.
+bool fused_vv_batch(dpmd_ctx* c, int m, bool fire) {
+    return !fire && !rigid(c) && c->fuse_vv && !c->rec_seg && m >= 2 && m % 2 == 0;
+}
+int any_word(int q) { return q % 2 == 0 ? RUN_ANY : RUN_ANY2; }
+
 void lean_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b) {
+    if (fused_vv_batch(c, m, fire)) {
+        double *Ax = c->x.p, *Ay = c->y.p, *Bx = c->x2.p, *By = c->y2.p;
+        disk_vv_drift_to(c, Gate{nullptr, c->run.p, 0}, Bx, By, any_word(0));
+        const bool prof = c->prof == 2 && !c->capturing;
+        if (prof) {   // one pair around the batch's fused launches; kept only if no rebuild halts it
+            HIP_OK(hipEventCreate(&c->pend_b)); HIP_OK(hipEventCreate(&c->pend_e));
+            HIP_OK(hipEventRecord(c->pend_b, c->st));
+            c->pend_n = m - 1;
+        }
+        for (int q = 0; q + 1 < m; ++q) {
+            Gate g{nullptr, c->run.p, q};
+            g.any = any_word(q);
+            const bool inB = q % 2 == 0;
+            disk_vv_fused(c, g, inB ? Bx : Ax, inB ? By : Ay, inB ? Ax : Bx, inB ? Ay : By, any_word(q + 1));
+        }
+        if (prof) HIP_OK(hipEventRecord(c->pend_e, c->st));
+        Gate gf{nullptr, c->run.p, m - 1};
+        gf.any = any_word(m - 1);
+        disk_vv_force(c, gf);
+        return;
+    }
     for (int k = 0; k < m; ++k) {
         const Gate g = Gate{nullptr, c->run.p, k};
         if (fire) {
@@ -442,6 +475,7 @@ int lean_run(dpmd_ctx* c, int max_steps, bool fire, int pred, double a, double b
             const int bm = 1 << lg;
             if (c->rec_seg)   // recorded positions go to slots total + 1 ... of the segment buffer
                 HIP_OK(hipMemcpyAsync(c->run.p + RUN_REC, &total, sizeof(int), hipMemcpyHostToDevice, c->st));
+            const bool fused = fused_vv_batch(c, bm, fire);
             if (use_graph) {
                 if (!graphs[lg]) graphs[lg] = capture_batch(c, bm, fire, pred, a, b);
                 HIP_OK(hipGraphLaunch(graphs[lg], c->st));
@@ -449,9 +483,21 @@ int lean_run(dpmd_ctx* c, int max_steps, bool fire, int pred, double a, double b
                 lean_batch(c, bm, fire, pred, a, b);
             }
             RunCtl r = read_run(c);
+            if (c->pend_b) {   // profiled fused batch: keep its event pair if it ran to the end
+                if (!r.halt) {
+                    c->prof_ev.push_back(c->pend_b); c->prof_ev.push_back(c->pend_e); c->prof_n.push_back(c->pend_n);
+                } else {
+                    (void)hipEventDestroy(c->pend_b); (void)hipEventDestroy(c->pend_e);
+                }
+                c->pend_b = c->pend_e = nullptr;
+            }
             if (r.halt) {
                 total += r.halt_at + 1;
                 since_halt += r.halt_at;
+                if (fused && r.halt_at % 2 == 0) {   // the halted step's positions are in buffer B
+                    HIP_OK(hipMemcpyAsync(c->x.p, c->x2.p, (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->st));
+                    HIP_OK(hipMemcpyAsync(c->y.p, c->y2.p, (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->st));
+                }
                 // finish the halted step: rebuild, forces, bookkeeping
                 rebuild_neighbors(c, active_one(c), false);
                 if (fire) {
@@ -462,6 +508,8 @@ int lean_run(dpmd_ctx* c, int max_steps, bool fire, int pred, double a, double b
                 }
                 const int clear[2] = {0, -1};
                 HIP_OK(hipMemcpyAsync(c->run.p + RUN_HALT, clear, sizeof(clear), hipMemcpyHostToDevice, c->st));
+                const int zero = 0;   // the rebuild cleared RUN_ANY; fused batches also use RUN_ANY2
+                HIP_OK(hipMemcpyAsync(c->run.p + RUN_ANY2, &zero, sizeof(int), hipMemcpyHostToDevice, c->st));
                 r = read_run(c);
                 if (fire && r.running <= 0) done = true;
                 m = std::max(1, std::min(128, since_halt));
@@ -773,7 +821,8 @@ int dpmd_ctx_destroy(dpmd_ctx* c) {
                                 &c->ke_tot, &c->pw_tot, &c->press, &c->temp, &c->phi, &c->maxf2, &c->st_xx,
                                 &c->st_xy, &c->st_yx, &c->st_yy, &c->x, &c->y, &c->vx, &c->vy, &c->fx, &c->fy,
                                 &c->pe, &c->ke, &c->rad, &c->mass, &c->area, &c->lx, &c->ly, &c->angle, &c->avel,
-                                &c->torque, &c->inertia, &c->sxx, &c->sxy, &c->syx, &c->syy, &c->vpx, &c->vpy,
+                                &c->torque, &c->inertia, &c->sxx, &c->sxy, &c->syx, &c->syy, &c->x2, &c->y2,
+                                &c->vpx, &c->vpy,
                                 &c->vfx, &c->vfy, &c->vpe, &c->vrad, &c->vmass, &c->tmp_d, &c->part, &c->f_dt,
                                 &c->f_dtmax, &c->f_dtmin, &c->f_alpha, &c->f_peprev, &c->s_dt, &c->s_alpha,
                                 &c->s_dtrev, &c->s_vscale, &c->vv_dt, &c->vv_dth, &c->sv_x, &c->sv_y, &c->sv_rad,
@@ -820,7 +869,7 @@ int dpmd_allocate_particles(dpmd_ctx* c, int N) {
         c->N = N;
         dpmd::DVec<double>* dv[] = {&c->x, &c->y, &c->vx, &c->vy, &c->fx, &c->fy, &c->pe, &c->ke, &c->rad,
                                     &c->mass, &c->area, &c->lx, &c->ly, &c->angle, &c->avel, &c->torque,
-                                    &c->inertia, &c->sxx, &c->sxy, &c->syx, &c->syy};
+                                    &c->inertia, &c->sxx, &c->sxy, &c->syx, &c->syy, &c->x2, &c->y2};
         for (auto* v : dv) { v->resize(N); v->zero(c->st); }
         c->sid.resize(N); c->sid.zero(c->st);
         c->nvpp.resize(N); c->nvpp.zero(c->st);
@@ -1327,6 +1376,7 @@ int dpmd_set_option(dpmd_ctx* c, int key, int value) {
         switch (key) {
             case DPMD_OPT_COOP_REBUILD: c->use_coop = value != 0; break;
             case DPMD_OPT_COOP_IN_GRAPH: c->coop_in_graph = value != 0; break;
+            case DPMD_OPT_FUSED_VV: c->fuse_vv = value != 0; break;
             default: throw ArgError("unknown option key");
         }
     });
@@ -1337,6 +1387,7 @@ int dpmd_get_option(dpmd_ctx* c, int key, int* value) {
         switch (key) {
             case DPMD_OPT_COOP_REBUILD: *value = c->use_coop ? 1 : 0; break;
             case DPMD_OPT_COOP_IN_GRAPH: *value = c->coop_in_graph ? 1 : 0; break;
+            case DPMD_OPT_FUSED_VV: *value = c->fuse_vv ? 1 : 0; break;
             default: throw ArgError("unknown option key");
         }
     });
@@ -1347,7 +1398,9 @@ int dpmd_profile(dpmd_ctx* c, int enable) {
         HIP_OK(hipStreamSynchronize(c->st));
         for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
         c->prof_ev.clear();
-        c->prof = enable != 0;
+        c->prof_n.clear();
+        need(enable >= 0 && enable <= 2, "dpmd_profile: enable must be 0, 1 or 2");
+        c->prof = enable;
     });
 }
 
@@ -1355,15 +1408,32 @@ int dpmd_profile_read(dpmd_ctx* c, double* total_ms, int* count) {
     return guard(c, [&] {
         HIP_OK(hipStreamSynchronize(c->st));
         double t = 0.0;
+        int n = 0;
         for (size_t k = 0; k + 1 < c->prof_ev.size(); k += 2) {
             float ms = 0.f;
             HIP_OK(hipEventElapsedTime(&ms, c->prof_ev[k], c->prof_ev[k + 1]));
             t += ms;
+            n += c->prof_n[k / 2];
         }
         if (total_ms) *total_ms = t;
-        if (count) *count = (int)(c->prof_ev.size() / 2);
+        if (count) *count = n;
         for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
         c->prof_ev.clear();
+        c->prof_n.clear();
+    });
+}
+
+int dpmd_profile_durations(dpmd_ctx* c, float* ms, int capacity, int* count) {
+    return guard(c, [&] {
+        need(capacity >= 0 && (ms != nullptr || capacity == 0), "dpmd_profile_durations: bad buffer");
+        HIP_OK(hipStreamSynchronize(c->st));
+        const int n = (int)(c->prof_ev.size() / 2);
+        for (int k = 0; k < n && k < capacity; ++k)
+            HIP_OK(hipEventElapsedTime(&ms[k], c->prof_ev[2 * k], c->prof_ev[2 * k + 1]));
+        if (count) *count = n;
+        for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+        c->prof_ev.clear();
+        c->prof_n.clear();
     });
 }
 

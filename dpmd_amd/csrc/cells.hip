@@ -168,11 +168,13 @@ __device__ __forceinline__ void ph_copy_back(const RebuildArgs& r, int p) {
 }
 
 // count_cell_neighbors_kernel + fill_neighbors_cell_kernel (base_point_particle_kernels.cu:
-// 37-157) in one pass, writing a fixed-capacity ELL list.
-__device__ __forceinline__ void ph_neighbors(const RebuildArgs& r, int i) {
+// 37-157) in one pass, writing a fixed-capacity ELL list.  Returns the neighbor count (0 for
+// objects of systems that are not rebuilt); the caller folds the largest count into
+// nb.max_count.
+__device__ __forceinline__ int ph_neighbors(const RebuildArgs& r, int i) {
     const ObjArgs& o = r.o; const CellArgs& ca = r.ca; const NbrArgs& nb = r.nb;
     const int s = o.sid[i];
-    if (!ca.flag[s]) return;
+    if (!ca.flag[s]) return 0;
     const int nx = ca.cdx[s], ny = ca.cdy[s], cs = ca.cstart[s];
     const int loc = ca.cell_id[i] - cs;
     const int cxi = loc % nx, cyi = loc / nx;
@@ -201,8 +203,8 @@ __device__ __forceinline__ void ph_neighbors(const RebuildArgs& r, int i) {
         }
     }
     if (cnt > nb.K) atomicMax(nb.overflow, cnt);
-    atomicMax(nb.max_count, cnt);
     nb.nn[i] = cnt < nb.K ? cnt : nb.K;
+    return cnt;
 }
 
 // reset_displacements_kernel for rigid bodies (rigid_bumpy.cu:1070-1085): the skin
@@ -229,10 +231,15 @@ DPMD_EACH(k_copy_back, r.o.n, ph_copy_back)
 DPMD_EACH(k_rigid_reset, r.N, ph_rigid_reset)
 #undef DPMD_EACH
 
+// One atomic per wavefront for the largest count (a per-object atomic on one word
+// serialises a million updates).
 __global__ void __launch_bounds__(DPMD_BLOCK) k_neighbors(RebuildArgs r, const int* active) {
+    if (!*active) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= r.o.n || !*active) return;
-    ph_neighbors(r, i);
+    int cnt = i < r.o.n ? ph_neighbors(r, i) : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cnt = max(cnt, __shfl_xor(cnt, off, 64));
+    if ((threadIdx.x & 63) == 0 && cnt > 0) atomicMax(r.nb.max_count, cnt);
 }
 
 __global__ void k_tile_sum(RebuildArgs r, const int* active) {
@@ -301,7 +308,10 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_rebuild_coop(RebuildArgs r, cons
     g.sync();
     for (int p = gt; p < n; p += gs) ph_copy_back(r, p);
     g.sync();
-    for (int i = gt; i < n; i += gs) ph_neighbors(r, i);
+    for (int i = gt; i < n; i += gs) {
+        const int cnt = ph_neighbors(r, i);
+        if (cnt > 0) atomicMax(r.nb.max_count, cnt);
+    }
     if (r.plx)
         for (int i = gt; i < r.N; i += gs) ph_rigid_reset(r, i);
     g.sync();

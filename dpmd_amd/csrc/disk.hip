@@ -52,21 +52,34 @@ __device__ __forceinline__ void disk_force_at(const DiskArgs& a, int i, int s, d
     const double iLx = a.iLx[s], iLy = a.iLy[s];
     const int cnt = a.nn[i];
     const int* __restrict__ col = a.nbr + i;
-    for (int k = 0; k < cnt; ++k) {
-        const int j = col[(size_t)k * a.n];
-        double dx = xi - a.x[j], dy = yi - a.y[j];
-        if (!WALL) { dx = min_image(dx, Lx, iLx); dy = min_image(dy, Ly, iLy); }
-        const double r2 = dx * dx + dy * dy;
-        const double radsum = ri + a.rad[j];
-        if (r2 >= radsum * radsum) continue;
-        const double r = sqrt(r2);
-        const double inv = 1.0 / r;
-        const double nx = dx * inv, ny = dy * inv;
-        const double delta = radsum - r;
-        const double fmag = ei * delta;
-        fxi += fmag * nx;
-        fyi += fmag * ny;
-        pei += (0.5 * ei * delta * delta) * 0.5;
+    // Neighbors in groups of four: every id and coordinate of a group is requested before
+    // any is used (four independent gather chains in flight per lane); the contributions
+    // are then added in list order, so the sums are those of the one-at-a-time loop.
+    constexpr int G = 4;
+    for (int k0 = 0; k0 < cnt; k0 += G) {
+        int jj[G];
+        double xj[G], yj[G], rj[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) jj[u] = (k0 + u < cnt) ? col[(size_t)(k0 + u) * a.n] : i;
+#pragma unroll
+        for (int u = 0; u < G; ++u) { xj[u] = a.x[jj[u]]; yj[u] = a.y[jj[u]]; rj[u] = a.rad[jj[u]]; }
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (k0 + u >= cnt) break;
+            double dx = xi - xj[u], dy = yi - yj[u];
+            if (!WALL) { dx = min_image(dx, Lx, iLx); dy = min_image(dy, Ly, iLy); }
+            const double r2 = dx * dx + dy * dy;
+            const double radsum = ri + rj[u];
+            if (r2 >= radsum * radsum) continue;
+            const double r = sqrt(r2);
+            const double inv = 1.0 / r;
+            const double nx = dx * inv, ny = dy * inv;
+            const double delta = radsum - r;
+            const double fmag = ei * delta;
+            fxi += fmag * nx;
+            fyi += fmag * ny;
+            pei += (0.5 * ei * delta * delta) * 0.5;
+        }
     }
     fxo = fxi; fyo = fyi; peo = pei;
 }
@@ -369,6 +382,67 @@ k_vv_force(DiskArgs a, const double* __restrict__ dth, Gate g) {
     a.vy[i] = a.vy[i] + fy * smi;
 }
 
+// The step's first half (update_velocities(dt_half) + update_positions(dt)) writing the new
+// positions to (xo, yo): the start of a fused run.
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_vv_drift_to(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, const double* __restrict__ dt,
+              const double* __restrict__ dth, Gate g) {
+    if (!gate_open(g)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < a.n;
+    const int s = valid ? a.sid[i] : -1;
+    bool flag = false;
+    if (valid) {
+        const double smi = dth[s] * 1.0 / a.mass[i];
+        const double vx = a.vx[i] + a.fx[i] * smi;
+        const double vy = a.vy[i] + a.fy[i] * smi;
+        const double sc = dt[s] * 1.0;
+        double x = a.x[i], y = a.y[i];
+        x += vx * sc; y += vy * sc;
+        xo[i] = x; yo[i] = y; a.vx[i] = vx; a.vy[i] = vy;
+        if (a.cell) {
+            const double dx = x - a.lx[i], dy = y - a.ly[i];
+            flag = (dx * dx + dy * dy) > a.thresh2[s];
+        }
+    }
+    if (a.cell) raise_flags(a.flag, a.any, s, flag);
+}
+
+// Velocity Verlet across a step boundary in one launch: forces at the positions (a.x, a.y)
+// and the step's closing half kick, then the next step's opening half kick, its drift into
+// (xo, yo) and the skin test of the new positions.  Every force must see its neighbors' old
+// positions, hence the second buffer.  Each expression is the one of k_vv_force / k_vv_drift,
+// in the same order, so a fused run equals the two-launch run bit for bit.
+template <int MODE>
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_vv_fused(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, const double* __restrict__ dt,
+           const double* __restrict__ dth, Gate g) {
+    if (!gate_force(g)) return;
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * DPMD_BLOCK + threadIdx.x;
+    const bool valid = i < a.n;
+    const int s = valid ? a.sid[i] : -1;
+    bool flag = false;
+    if (valid) {
+        double fx, fy, pe;
+        disk_force_mode<MODE>(a, i, s, fx, fy, pe);
+        a.fx[i] = fx; a.fy[i] = fy; a.pe[i] = pe;
+        const double smi = dth[s] * 1.0 / a.mass[i];
+        double vx = a.vx[i] + fx * smi;          // end of this step
+        double vy = a.vy[i] + fy * smi;
+        vx = vx + fx * smi;                      // next step: update_velocities(dt_half)
+        vy = vy + fy * smi;
+        const double sc = dt[s] * 1.0;           // update_positions(dt)
+        double x = a.x[i], y = a.y[i];
+        x += vx * sc; y += vy * sc;
+        xo[i] = x; yo[i] = y; a.vx[i] = vx; a.vy[i] = vy;
+        if (a.cell) {
+            const double dx = x - a.lx[i], dy = y - a.ly[i];
+            flag = (dx * dx + dy * dy) > a.thresh2[s];
+        }
+    }
+    if (a.cell) raise_flags(a.flag, a.any, s, flag);
+}
+
 template <class F>
 void by_mode(int mode, F&& f) {
     switch (mode) {
@@ -435,6 +509,25 @@ void fire_finalize(dpmd_ctx* c, Gate g, int do_conv, int pred, double pa, double
 
 void disk_vv_drift(dpmd_ctx* c, Gate g) {
     k_vv_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), c->vv_dt.p, c->vv_dth.p, g);
+    HIP_OK(hipGetLastError());
+}
+
+void disk_vv_drift_to(dpmd_ctx* c, Gate g, double* xo, double* yo, int any_word) {
+    DiskArgs a = disk_args(c);
+    a.any = c->run.p + any_word;
+    k_vv_drift_to<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(a, xo, yo, c->vv_dt.p, c->vv_dth.p, g);
+    HIP_OK(hipGetLastError());
+}
+
+void disk_vv_fused(dpmd_ctx* c, Gate g, const double* xs, const double* ys, double* xo, double* yo, int any_word) {
+    DiskArgs a = disk_args(c);
+    a.x = const_cast<double*>(xs);
+    a.y = const_cast<double*>(ys);
+    a.any = c->run.p + any_word;
+    by_mode(c->vv_mode, [&](auto m) {
+        k_vv_fused<decltype(m)::value><<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(a, xo, yo, c->vv_dt.p,
+                                                                                  c->vv_dth.p, g);
+    });
     HIP_OK(hipGetLastError());
 }
 

@@ -43,8 +43,11 @@ struct NbrArgs {
 //   [6] index of the step in its batch at which the last system stopped
 //   [7] "some rebuild flag is set" (raised by the drift kernels, cleared by the rebuild)
 //   [8] base slot for recorded positions (reverse-mode replays)
+//   [9] second "some rebuild flag is set" word: fused velocity-Verlet launches raise the
+//       flags of the positions they write into the word of the next step's parity, so a
+//       launch never reads a word that its own workgroups may be setting
 enum { RUN_MAX = 1, RUN_RUNNING = 2, RUN_ACTIVE = 3, RUN_HALT = 4, RUN_HALT_AT = 5, RUN_CONV_AT = 6, RUN_ANY = 7,
-       RUN_REC = 8, RUN_WORDS = 9 };
+       RUN_REC = 8, RUN_ANY2 = 9, RUN_WORDS = 10 };
 
 // Launch gate.  Plain form (run == nullptr): the kernel works iff *active != 0.  Batch form
 // (lean fused runs): a step works while nothing halted and some system runs; the force
@@ -54,6 +57,7 @@ struct Gate {
     const int* active;
     int* run;
     int k;
+    int any = RUN_ANY;   // the run word a batch force kernel checks for rebuild flags
 };
 
 __device__ __forceinline__ bool gate_open(const Gate& g) {
@@ -64,7 +68,7 @@ __device__ __forceinline__ bool gate_open(const Gate& g) {
 __device__ __forceinline__ bool gate_force(const Gate& g) {
     if (!g.run) return *g.active != 0;
     if (g.run[RUN_HALT] != 0 || g.run[RUN_RUNNING] <= 0) return false;
-    if (g.run[RUN_ANY] != 0) {
+    if (g.run[g.any] != 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { g.run[RUN_HALT_AT] = g.k; g.run[RUN_HALT] = 1; }
         return false;
     }
@@ -176,9 +180,16 @@ struct dpmd_ctx {
     dpmd::DVec<int> ck_nn, ck_nbr;   // canonical-order contact list for a segment
     int ck_steps = 0, ck_every = 0, ck_count = 0;
     double* rec_seg = nullptr;       // when set, lean VV batches record positions per step
+    // fused velocity Verlet (disks): second position buffer and the switch
+    dpmd::DVec<double> x2, y2;
+    bool fuse_vv = true;
     // kernel timing (force kernels; outside graph capture only)
-    bool prof = false, capturing = false;
+    int prof = 0;   // 0 off, 1 force kernels, 2 fused velocity-Verlet kernels
+    bool capturing = false;
     std::vector<hipEvent_t> prof_ev;   // (begin, end) pairs
+    std::vector<int> prof_n;           // launches covered by each pair
+    hipEvent_t pend_b = nullptr, pend_e = nullptr;   // pair of the batch in flight (kind 2)
+    int pend_n = 0;
 
     int n_obj() const { return kind == DPMD_RIGID_BUMPY ? Nv : N; }
 };
@@ -225,6 +236,8 @@ void disk_stress(dpmd_ctx* c);
 void disk_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, double a, double b);
 void disk_fire_drift(dpmd_ctx* c, Gate g);
 void disk_vv_drift(dpmd_ctx* c, Gate g);
+void disk_vv_drift_to(dpmd_ctx* c, Gate g, double* xo, double* yo, int any_word);
+void disk_vv_fused(dpmd_ctx* c, Gate g, const double* xs, const double* ys, double* xo, double* yo, int any_word);
 void disk_vv_force(dpmd_ctx* c, Gate g);
 void disk_ke(dpmd_ctx* c);
 void fire_recompute_update(dpmd_ctx* c);
@@ -270,16 +283,19 @@ void fill_i32(hipStream_t st, int* p, int64_t n, int v);
 void copy_f64(hipStream_t st, double* dst, const double* src, int64_t n);
 
 // Records a (begin, end) event pair around the enclosed launch when timing is on.
+// Records a HIP event pair around one launch when force-kernel profiling (kind 1) is on and
+// the stream is not capturing.  Fused velocity-Verlet profiling (kind 2) brackets the
+// consecutive fused launches of a batch instead (abi.hip).
 struct ProfScope {
     dpmd_ctx* c; hipEvent_t b = nullptr;
-    explicit ProfScope(dpmd_ctx* cc) : c(cc) {
-        if (c->prof && !c->capturing) { HIP_OK(hipEventCreate(&b)); HIP_OK(hipEventRecord(b, c->st)); }
+    explicit ProfScope(dpmd_ctx* cc, int kind = 1) : c(cc) {
+        if (c->prof == kind && !c->capturing) { HIP_OK(hipEventCreate(&b)); HIP_OK(hipEventRecord(b, c->st)); }
     }
     ~ProfScope() {
         if (!b) return;
         hipEvent_t e = nullptr;
         if (hipEventCreate(&e) == hipSuccess && hipEventRecord(e, c->st) == hipSuccess) {
-            c->prof_ev.push_back(b); c->prof_ev.push_back(e);
+            c->prof_ev.push_back(b); c->prof_ev.push_back(e); c->prof_n.push_back(1);
         }
     }
 };

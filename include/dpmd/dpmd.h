@@ -253,19 +253,27 @@ int dpmd_synchronize(dpmd_ctx* ctx);
 /* ---- instrumentation (no reference counterpart: the reference times with std::chrono,
  * tests/test_disk_system_cell_parallelism.cu:92-102) -------------------------------------
  * dpmd_profile(ctx, 1) starts recording a HIP event pair around every force-kernel launch
- * issued outside graph capture; dpmd_profile_read returns their summed duration and count. */
+ * issued outside graph capture; dpmd_profile(ctx, 2) records one pair around the consecutive
+ * fused velocity-Verlet launches of every directly launched batch that no rebuild halted;
+ * 0 stops.  dpmd_profile_read returns the summed duration and the number of launches covered. */
 int dpmd_profile(dpmd_ctx* ctx, int enable);
 int dpmd_profile_read(dpmd_ctx* ctx, double* total_ms, int* count);
+/* The recorded event pairs one by one (ms, in order, up to capacity); *count = the number of
+ * pairs.  Clears the record like dpmd_profile_read. */
+int dpmd_profile_durations(dpmd_ctx* ctx, float* ms, int capacity, int* count);
 /* A plain device copy kernel of n doubles (dst <- src), timed over `repeats` launches on the
  * default stream: the measured copy ceiling, and the calibration kernel for HBM counters. */
 int dpmd_stream_copy(double* dst, const double* src, int64_t n, int repeats, double* ms_per_copy);
 
 /* Execution options (no reference counterpart).
- *   DPMD_OPT_COOP_REBUILD: 1 (default) = the cell/neighbor rebuild is one cooperative launch
- *     that exits at once when no system is flagged; 0 = one launch per rebuild phase.
+ *   DPMD_OPT_COOP_REBUILD: 1 = the cell/neighbor rebuild is one cooperative launch that exits
+ *     at once when no system is flagged; 0 (default) = one launch per rebuild phase.
  *   DPMD_OPT_COOP_IN_GRAPH: 1 while cooperative launches may be captured into hipGraphs;
- *     the library clears it by itself if capture of that launch fails. */
-enum dpmd_option { DPMD_OPT_COOP_REBUILD = 0, DPMD_OPT_COOP_IN_GRAPH = 1 };
+ *     the library clears it by itself if capture of that launch fails.
+ *   DPMD_OPT_FUSED_VV: 1 (default) = device-driven velocity-Verlet runs of disks compute a
+ *     step's forces and the next step's kick and drift in one launch (positions ping-pong
+ *     between two buffers); 0 = one drift and one force launch per step.  Same results. */
+enum dpmd_option { DPMD_OPT_COOP_REBUILD = 0, DPMD_OPT_COOP_IN_GRAPH = 1, DPMD_OPT_FUSED_VV = 2 };
 int dpmd_set_option(dpmd_ctx* ctx, int key, int value);
 int dpmd_get_option(dpmd_ctx* ctx, int key, int* value);
 

@@ -152,6 +152,40 @@ def test_vv_bit_identical_and_graph():
     assert_same(P, ref, what="VV 500 steps")
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_fused_vv_equals_two_launch_vv(graph):
+    """Fused velocity-Verlet batches (forces of one step with the next step's kick and drift,
+    positions in two buffers) against one drift and one force launch per step and against the
+    oracle: hot enough that rebuild halts land on steps of both parities."""
+    b = batch_small()
+    rng = np.random.default_rng(12)
+    v = rng.standard_normal((2, b.N)) * 0.3
+    runs = []
+    for fused in (1, 0):
+        P, ref = make(b)
+        L.check(L.lib().dpmd_set_option(P.ctx, L.OPT_FUSED_VV, fused), P.ctx)
+        si = P.get("STATIC_INDEX")
+        cvx, cvy = np.empty(b.N), np.empty(b.N)
+        cvx[si], cvy[si] = v[0], v[1]
+        P.set("VEL_X", cvx); P.set("VEL_Y", cvy)
+        vv = VelocityVerlet(P, 1e-2, mode=L.FORCE_PER_SYSTEM)
+        vv.init()
+        vv.run(400, use_graph=graph)
+        runs.append(P)
+    ref = orc.OracleDisk(b)
+    si = ref.static_index()
+    cvx, cvy = np.empty(b.N), np.empty(b.N)
+    cvx[si], cvy[si] = v[0], v[1]
+    ref.set("vx", cvx); ref.set("vy", cvy)
+    ref.vv_init(1e-2, 2)
+    ref.vv_run(400, 2)
+    for f in ("x", "y", "vx", "vy", "fx", "fy", "pe"):
+        a0 = runs[0].get(NAMES[f], L.ORDER_CANONICAL)
+        a1 = runs[1].get(NAMES[f], L.ORDER_CANONICAL)
+        assert np.array_equal(a0, a1), f
+    assert_same(runs[0], ref, what="fused VV vs oracle")
+
+
 def test_single_system_fire_then_nve_full_size():
     """N=1024, phi=0.85, PBC: FIRE until max|F_i| < 1e-12, then 1e4 velocity-Verlet steps at
     dt=1e-2 from Gaussian velocities at T=1e-4 with zero net momentum.  Bar: converged FIRE
