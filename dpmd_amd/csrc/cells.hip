@@ -231,9 +231,88 @@ DPMD_EACH(k_copy_back, r.o.n, ph_copy_back)
 DPMD_EACH(k_rigid_reset, r.N, ph_rigid_reset)
 #undef DPMD_EACH
 
-// One atomic per wavefront for the largest count (a per-object atomic on one word
-// serialises a million updates).
+// ---- neighbor build, output staged in LDS -----------------------------------------------
+// Each object appends its neighbors to its own column of an LDS table nbr_l[k][thread]; the
+// workgroup then writes rows 0 .. (its largest count - 1) of the column-major ELL table as
+// whole 1 KB rows.  Written directly, each lane stores its k-th neighbor at a different time
+// of its divergent stencil loop, and rows become scattered partial cache lines (about 4x the
+// payload in HBM writes).  Objects of systems that are not rebuilt keep their lists.
 __global__ void __launch_bounds__(DPMD_BLOCK) k_neighbors(RebuildArgs r, const int* active) {
+    extern __shared__ int nbr_l[];   // K rows x DPMD_BLOCK
+    __shared__ int wmax[DPMD_BLOCK / 64];
+    if (!*active) return;
+    const ObjArgs& o = r.o; const CellArgs& ca = r.ca; const NbrArgs& nb = r.nb;
+    const int i = blockIdx.x * DPMD_BLOCK + threadIdx.x;
+    const int K = nb.K;
+    int cnt = 0;
+    bool rebuilt = false;
+    if (i < o.n) {
+        const int s = o.sid[i];
+        rebuilt = ca.flag[s] != 0;
+        if (rebuilt) {
+            const int nx = ca.cdx[s], ny = ca.cdy[s], cs = ca.cstart[s];
+            const int loc = ca.cell_id[i] - cs;
+            const int cxi = loc % nx, cyi = loc / nx;
+            const double Lx = ca.Lx[s], Ly = ca.Ly[s], iLx = ca.iLx[s], iLy = ca.iLy[s], cut0 = ca.cutoff[s];
+            const double xi = o.x[i], yi = o.y[i], ri = o.rad[i];
+            const int pi = o.pid ? o.pid[i] : -1;
+            // Each stencil row (cells cx-1, cx, cx+1) is one contiguous object range unless it
+            // wraps in x; scanning ranges in stencil order visits the same objects in the same
+            // order as the cell-by-cell loop.
+            auto scan = [&](int b, int e) {
+                for (int j = b; j < e; ++j) {
+                    if (j == i) continue;
+                    if (o.pid && o.pid[j] == pi) continue;
+                    const double ddx = min_image(xi - o.x[j], Lx, iLx);
+                    const double ddy = min_image(yi - o.y[j], Ly, iLy);
+                    const double r2 = ddx * ddx + ddy * ddy;
+                    const double cut = ri + o.rad[j] + cut0;
+                    if (r2 < cut * cut) {
+                        if (cnt < K) nbr_l[cnt * DPMD_BLOCK + threadIdx.x] = j;
+                        ++cnt;
+                    }
+                }
+            };
+            for (int dy = -1; dy <= 1; ++dy) {
+                int yy = cyi + dy; if (yy < 0) yy += ny; else if (yy >= ny) yy -= ny;
+                const int row = cs + yy * nx;
+                if (cxi == 0) {               // cells nx-1 | 0, 1
+                    scan(ca.cell_start[row + nx - 1], ca.cell_start[row + nx]);
+                    scan(ca.cell_start[row], ca.cell_start[row + 2]);
+                } else if (cxi == nx - 1) {   // cells nx-2, nx-1 | 0
+                    scan(ca.cell_start[row + nx - 2], ca.cell_start[row + nx]);
+                    scan(ca.cell_start[row], ca.cell_start[row + 1]);
+                } else {                      // cells cx-1 .. cx+1
+                    scan(ca.cell_start[row + cxi - 1], ca.cell_start[row + cxi + 2]);
+                }
+            }
+            if (cnt > K) atomicMax(nb.overflow, cnt);
+            nb.nn[i] = cnt < K ? cnt : K;
+        }
+    }
+    // largest stored count of the workgroup (rows to write) and of the wave (max_count)
+    int m = cnt;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) {
+        wmax[threadIdx.x >> 6] = m;
+        if (m > 0) atomicMax(nb.max_count, m);
+    }
+    __syncthreads();
+    int rows = 0;
+#pragma unroll
+    for (int w = 0; w < DPMD_BLOCK / 64; ++w) rows = max(rows, wmax[w]);
+    rows = min(rows, K);
+    const int mine = min(cnt, K);
+    if (rebuilt)
+        for (int k = 0; k < rows; ++k)
+            nb.nbr[(size_t)k * o.n + i] = k < mine ? nbr_l[k * DPMD_BLOCK + threadIdx.x] : -1;
+}
+
+// Capacities above kLdsCapacity: write the ELL table directly (one atomic per wave for the
+// largest count).
+constexpr int kLdsCapacity = 64;
+__global__ void __launch_bounds__(DPMD_BLOCK) k_neighbors_direct(RebuildArgs r, const int* active) {
     if (!*active) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int cnt = i < r.o.n ? ph_neighbors(r, i) : 0;
@@ -408,7 +487,10 @@ void rebuild_multi(dpmd_ctx* c, const RebuildArgs& r, const int* active) {
     k_sort<<<blocks_for(C), DPMD_BLOCK, 0, st>>>(r, active);
     k_gather<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     k_copy_back<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
-    k_neighbors<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
+    if (c->K <= kLdsCapacity)
+        k_neighbors<<<blocks_for(n), DPMD_BLOCK, (size_t)c->K * DPMD_BLOCK * sizeof(int), st>>>(r, active);
+    else
+        k_neighbors_direct<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     if (r.plx) k_rigid_reset<<<blocks_for(r.N), DPMD_BLOCK, 0, st>>>(r, active);
     k_clear_flags<<<blocks_for(r.S), DPMD_BLOCK, 0, st>>>(r.S, r.ca.flag, r.any, active);
 }

@@ -76,6 +76,22 @@ def test_forces_match_oracle(mode):
     assert_same(P, ref, ("fx", "fy", "pe"), "forces")
 
 
+def test_neighbor_capacity_paths_agree():
+    """Capacities up to 64 stage each workgroup's lists in LDS; larger ones write directly."""
+    b = batch_small()
+    lists = []
+    for cap in (32, 100):
+        P = Disk()
+        W.load_into(P, b)
+        P.set_neighbor_capacity(cap)
+        P.init_neighbors()
+        lists.append(P.neighbors_csr())
+    ref = orc.OracleDisk(b)
+    rs, ri = ref.neighbors_csr()
+    for gs, gi in lists:
+        assert np.array_equal(gs, rs) and np.array_equal(gi, ri)
+
+
 def test_forces_naive_equal_cell():
     b = batch_small()
     P, _ = make(b, L.NEIGHBOR_CELL)
