@@ -17,7 +17,8 @@ NEIGHBOR_NAIVE, NEIGHBOR_CELL = 0, 1
 FORCE_PBC, FORCE_WALL, FORCE_PER_SYSTEM = 0, 1, 2
 ORDER_CURRENT, ORDER_CANONICAL = 0, 1
 FIRE_PRED_REFERENCE, FIRE_PRED_MAX_FORCE = 0, 1
-OPT_COOP_REBUILD, OPT_COOP_IN_GRAPH, OPT_FUSED_VV = 0, 1, 2
+OPT_COOP_REBUILD, OPT_COOP_IN_GRAPH, OPT_FUSED_VV, OPT_BLOCK, OPT_MAX_BATCH = 0, 1, 2, 3, 4
+TUNE_FIRE, TUNE_NVE = 0, 1
 
 STATUS = {0: "OK", 1: "ARGUMENT", 2: "STATE", 3: "HIP", 4: "OVERFLOW"}
 
@@ -42,6 +43,21 @@ INT_FIELDS = {k for k, v in F.items() if v >= 64 and v < 96 or v >= 112}
 vp, i32, i64, f64, u64 = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_uint64
 pi32 = C.POINTER(C.c_int)
 pi64 = C.POINTER(C.c_int64)
+
+
+class TuneConfig(C.Structure):
+    """struct dpmd_tune_config"""
+    _fields_ = [("skin", C.c_double), ("cell_factor", C.c_double), ("block", C.c_int), ("max_batch", C.c_int),
+                ("fused_vv", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneTrial(C.Structure):
+    """struct dpmd_tune_trial"""
+    _fields_ = [("config", TuneConfig), ("rate", C.c_double), ("seconds", C.c_double)]
+
 
 # name: (restype, argtypes)
 PROTOS = {
@@ -118,6 +134,14 @@ PROTOS = {
     "dpmd_profile": (i32, [vp, i32]),
     "dpmd_profile_read": (i32, [vp, C.POINTER(f64), pi32]),
     "dpmd_stream_copy": (i32, [vp, vp, i64, i32, C.POINTER(f64)]),
+    "dpmd_cell_grid": (i32, [f64, f64, f64, f64, f64, pi32, pi32]),
+    "dpmd_tune_get": (i32, [vp, C.POINTER(TuneConfig)]),
+    "dpmd_tune_apply": (i32, [vp, C.POINTER(TuneConfig)]),
+    "dpmd_tune_signature": (i32, [vp, i32, i32, C.c_char_p, i32]),
+    "dpmd_autotune": (i32, [vp, i32, vp, i32, i32, C.c_char_p, C.POINTER(TuneConfig), C.POINTER(f64),
+                            C.POINTER(TuneTrial), i32, pi32]),
+    "dpmd_tune_cache_load": (i32, [C.c_char_p, C.c_char_p, C.POINTER(TuneConfig), C.POINTER(f64), pi32]),
+    "dpmd_tune_cache_store": (i32, [C.c_char_p, C.c_char_p, C.POINTER(TuneConfig), f64]),
     "dpmd_version": (C.c_char_p, []),
 }
 

@@ -10,31 +10,6 @@ using namespace dpmd;
 
 namespace {
 
-std::string g_global_err;
-
-template <class F>
-int guard(dpmd_ctx* c, F&& f) {
-    try {
-        f();
-        return DPMD_OK;
-    } catch (const ArgError& e) {
-        (c ? c->err : g_global_err) = e.what();
-        return DPMD_ERR_ARGUMENT;
-    } catch (const StateError& e) {
-        (c ? c->err : g_global_err) = e.what();
-        return DPMD_ERR_STATE;
-    } catch (const OverflowError& e) {
-        (c ? c->err : g_global_err) = e.what();
-        return DPMD_ERR_OVERFLOW;
-    } catch (const std::exception& e) {
-        (c ? c->err : g_global_err) = e.what();
-        return DPMD_ERR_HIP;
-    }
-}
-
-void need(bool ok, const std::string& msg) { if (!ok) throw ArgError(msg); }
-void need_state(bool ok, const std::string& msg) { if (!ok) throw StateError(msg); }
-
 struct FieldRef {
     void* p; int64_t n; int elem;   // elem: 8 = double, 4 = int
     int domain;                     // 0 particle, 1 vertex, 2 system, 3 other
@@ -113,15 +88,6 @@ FieldRef field_ref(dpmd_ctx* c, int f) {
     throw ArgError("unknown field id " + std::to_string(f));
 }
 
-template <class T>
-std::vector<T> dl(dpmd_ctx* c, const dpmd::DVec<T>& v) {
-    std::vector<T> h(v.n);
-    if (v.n) {
-        HIP_OK(hipMemcpyAsync(h.data(), v.p, v.n * sizeof(T), hipMemcpyDeviceToHost, c->st));
-        HIP_OK(hipStreamSynchronize(c->st));
-    }
-    return h;
-}
 
 const int* active_one(dpmd_ctx* c) { return c->one.p; }
 Gate plain(dpmd_ctx* c) { return Gate{c->one.p, nullptr, 0}; }
@@ -262,7 +228,6 @@ void init_neighbors_impl(dpmd_ctx* c) {
     need_state(c->synced, "call dpmd_sync before init_neighbors");
     const int n = c->n_obj();
     HIP_OK(hipMemsetAsync(c->overflow.p, 0, sizeof(int), c->st));
-    HIP_OK(hipMemsetAsync(c->max_count.p, 0, sizeof(int), c->st));
     if (c->method == DPMD_NEIGHBOR_NAIVE) {
         const std::vector<int>& off = rigid(c) ? c->h_voff : c->h_off;
         int mx = 1;
@@ -471,7 +436,7 @@ int lean_run(dpmd_ctx* c, int max_steps, bool fire, int pred, double a, double b
     try {
         while (!done && total < max_steps) {
             int lg = 0;
-            while ((2 << lg) <= std::min(m, max_steps - total) && lg < 7) ++lg;
+            while ((2 << lg) <= std::min(std::min(m, c->max_batch), max_steps - total) && lg < 7) ++lg;
             const int bm = 1 << lg;
             if (c->rec_seg)   // recorded positions go to slots total + 1 ... of the segment buffer
                 HIP_OK(hipMemcpyAsync(c->run.p + RUN_REC, &total, sizeof(int), hipMemcpyHostToDevice, c->st));
@@ -781,13 +746,130 @@ double newton_impl(dpmd_ctx* c, int mode, int max_newton, double ftol, double cg
     return f;
 }
 
+void vv_init_impl(dpmd_ctx* c, const double* dt, int mode) {
+    const int S = c->S;
+    std::vector<double> h(S), hh(S);
+    HIP_OK(hipMemcpyAsync(h.data(), dt, S * 8, hipMemcpyDeviceToHost, c->st));
+    HIP_OK(hipStreamSynchronize(c->st));
+    for (int s = 0; s < S; ++s) hh[s] = h[s] * 0.5;   // dt_half = 0.5 * dt (velocity_verlet.cuh:28-29)
+    c->vv_dt.upload(h.data(), S, c->st);
+    c->vv_dth.upload(hh.data(), S, c->st);
+    c->vv_mode = mode;
+    compute_forces_impl(c, mode);
+    c->vv_ready = true;
+}
+
 }  // namespace
+
+std::string& dpmd::global_error() {
+    static std::string e;
+    return e;
+}
+
+// ---------------------------------------------------------------------------------------
+// Autotuner engine.  A configuration sets, for every system s, the neighbor cutoff (the
+// Verlet skin) to skin * 2 r_min,s with the rebuild threshold (cutoff / 2)^2, and a cell
+// grid whose side is at least cell_factor * (2 r_max,s + cutoff) -- the two preconditions of
+// the periodic 3x3 stencil (a grid of at least 3 x 3 cells, cells no smaller than the
+// interaction range) are checked here for every system -- plus the execution options.
+// ---------------------------------------------------------------------------------------
+namespace dpmd {
+
+void tune_get(dpmd_ctx* c, dpmd_tune_config* t) {
+    need(!rigid(c) && c->method == DPMD_NEIGHBOR_CELL, "the autotuner handles disk systems with cell lists");
+    auto rad = dl(c, c->rad);
+    auto cut = dl(c, c->cutoff);
+    auto Lx = dl(c, c->Lx), Ly = dl(c, c->Ly);
+    auto cdx = dl(c, c->cdx), cdy = dl(c, c->cdy);
+    double skin = 0.0, factor = 1e300;
+    for (int s = 0; s < c->S; ++s) {
+        double rmin = 1e300, rmax = 0.0;
+        for (int i = c->h_off[s]; i < c->h_off[s + 1]; ++i) { rmin = std::min(rmin, rad[i]); rmax = std::max(rmax, rad[i]); }
+        skin = std::max(skin, cut[s] / (2.0 * rmin));
+        const double need_sz = 2.0 * rmax + cut[s];
+        factor = std::min(factor, std::min(Lx[s] / cdx[s], Ly[s] / cdy[s]) / need_sz);
+    }
+    t->skin = skin;
+    t->cell_factor = factor;
+    t->block = c->block;
+    t->max_batch = c->max_batch;
+    t->fused_vv = c->fuse_vv ? 1 : 0;
+}
+
+void tune_apply(dpmd_ctx* c, const dpmd_tune_config& t) {
+    need(!rigid(c) && c->method == DPMD_NEIGHBOR_CELL, "the autotuner handles disk systems with cell lists");
+    need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
+    need(t.skin > 0.0 && std::isfinite(t.skin), "tune config: skin must be positive");
+    need(t.cell_factor >= 1.0 && std::isfinite(t.cell_factor), "tune config: cell_factor must be >= 1");
+    need(t.block == 64 || t.block == 128 || t.block == 256 || t.block == 512, "tune config: block must be 64, 128, 256 or 512");
+    need(t.max_batch >= 1 && t.max_batch <= 128 && (t.max_batch & (t.max_batch - 1)) == 0,
+         "tune config: max_batch must be a power of two from 1 to 128");
+    const int S = c->S;
+    auto rad = dl(c, c->rad);
+    auto Lx = dl(c, c->Lx), Ly = dl(c, c->Ly);
+    std::vector<double> cut(S), th2(S);
+    std::vector<int> nx(S), ny(S);
+    for (int s = 0; s < S; ++s) {
+        double rmin = 1e300, rmax = 0.0;
+        for (int i = c->h_off[s]; i < c->h_off[s + 1]; ++i) { rmin = std::min(rmin, rad[i]); rmax = std::max(rmax, rad[i]); }
+        cut[s] = t.skin * 2.0 * rmin;
+        th2[s] = (0.5 * cut[s]) * (0.5 * cut[s]);
+        const int rc = dpmd_cell_grid(Lx[s], Ly[s], rmax, cut[s], t.cell_factor, &nx[s], &ny[s]);
+        if (rc != DPMD_OK)
+            throw ArgError("tune config: system " + std::to_string(s) + ": " + global_error());
+    }
+    c->cutoff.upload(cut.data(), S, c->st);
+    c->thresh2.upload(th2.data(), S, c->st);
+    c->cdx.upload(nx.data(), S, c->st);
+    c->cdy.upload(ny.data(), S, c->st);
+    setup_cells(c, CELLS_SYNC);
+    alloc_obj_buffers(c);
+    check_neighbors_impl(c, true);
+    check_overflow(c);
+    c->block = t.block;
+    c->max_batch = t.max_batch;
+    c->fuse_vv = t.fused_vv != 0;
+}
+
+void tune_snapshot(dpmd_ctx* c, double* dst) { snapshot(c, dst); }
+
+void tune_restore(dpmd_ctx* c, const double* src, int mode) {
+    const int N = c->N;
+    adj_to_current(c, src + 2 * (size_t)N, c->vx.p);
+    adj_to_current(c, src + 3 * (size_t)N, c->vy.p);
+    restore_positions(c, src);
+    compute_forces_impl(c, mode);
+}
+
+int64_t tune_run(dpmd_ctx* c, int integrator, const double* dt, int mode, int steps) {
+    need(integrator == DPMD_TUNE_FIRE || integrator == DPMD_TUNE_NVE, "unknown tuning integrator");
+    const int S = c->S;
+    std::vector<double> h(S);
+    HIP_OK(hipMemcpyAsync(h.data(), dt, S * 8, hipMemcpyDeviceToHost, c->st));
+    HIP_OK(hipStreamSynchronize(c->st));
+    int64_t live = 0;
+    for (int s = 0; s < S; ++s)
+        if (h[s] != 0.0) live += c->h_off[s + 1] - c->h_off[s];
+    need(live > 0, "tuning run: every dt is zero");
+    if (integrator == DPMD_TUNE_FIRE) {
+        fire_init_impl(c, dt, mode);
+        FrozenMask mask(c);
+        const int done = lean_run(c, steps, true, DPMD_FIRE_PRED_MAX_FORCE, 0.0, 0.0, true, nullptr);
+        check_overflow(c);
+        return live * done;
+    }
+    vv_init_impl(c, dt, mode);
+    vv_run_impl(c, steps, true);
+    return (int64_t)c->N * steps;
+}
+
+}  // namespace dpmd
 
 extern "C" {
 
 const char* dpmd_version(void) { return "dpmd_amd 0.1 (gfx950, fp64)"; }
 
-const char* dpmd_last_error(const dpmd_ctx* c) { return c ? c->err.c_str() : g_global_err.c_str(); }
+const char* dpmd_last_error(const dpmd_ctx* c) { return c ? c->err.c_str() : global_error().c_str(); }
 
 int dpmd_ctx_create(dpmd_ctx** out, int kind, void* stream) {
     return guard(nullptr, [&] {
@@ -804,7 +886,6 @@ int dpmd_ctx_create(dpmd_ctx** out, int kind, void* stream) {
             c->one.upload(&one, 1, c->st);
             c->run.resize(RUN_WORDS); c->run.zero(c->st);
             c->overflow.resize(1); c->overflow.zero(c->st);
-            c->max_count.resize(1); c->max_count.zero(c->st);
         } catch (...) { delete c; throw; }
         c->err.clear();
         *out = c;
@@ -832,7 +913,7 @@ int dpmd_ctx_destroy(dpmd_ctx* c) {
     dpmd::DVec<int>* iv[] = {&c->off, &c->size, &c->cdx, &c->cdy, &c->cstart, &c->boundary, &c->ndof, &c->voff,
                              &c->sid, &c->static_index, &c->vpid, &c->poff, &c->nvpp, &c->vsid, &c->cell_id,
                              &c->cell_count, &c->cell_start, &c->cell_wptr, &c->cell_sys, &c->order,
-                             &c->order_inv, &c->tmp_i, &c->nn, &c->nbr, &c->overflow, &c->max_count,
+                             &c->order_inv, &c->tmp_i, &c->nn, &c->nbr, &c->overflow,
                              &c->scan_part, &c->chunk_sys, &c->chunk_beg, &c->sys_chunk, &c->f_ngood,
                              &c->f_nbad, &c->f_stopped, &c->f_frozen, &c->f_nsteps, &c->s_ngood, &c->s_nbad, &c->run, &c->one, &c->sv_static};
     for (auto* v : iv) v->release();
@@ -982,15 +1063,15 @@ int dpmd_neighbor_info(dpmd_ctx* c, int* capacity, int* max_count, int64_t* tota
     return guard(c, [&] {
         need_state(c->nbr_ready, "neighbors are not initialised");
         if (capacity) *capacity = c->K;
-        if (max_count) {
-            HIP_OK(hipMemcpyAsync(max_count, c->max_count.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
-            HIP_OK(hipStreamSynchronize(c->st));
-        }
-        if (total) {
+        if (max_count || total) {
+            // A list longer than the capacity is an error at build time, so the stored counts
+            // are the true ones.
             auto nn = dl(c, c->nn);
             int64_t t = 0;
-            for (int k = 0; k < c->n_obj(); ++k) t += nn[k];
-            *total = t;
+            int mx = 0;
+            for (int k = 0; k < c->n_obj(); ++k) { t += nn[k]; mx = std::max(mx, nn[k]); }
+            if (max_count) *max_count = mx;
+            if (total) *total = t;
         }
     });
 }
@@ -1115,18 +1196,7 @@ int dpmd_rand_normal_2d(dpmd_ctx* c, int fx, int fy, uint64_t seed, uint64_t cal
 }
 
 int dpmd_vv_init(dpmd_ctx* c, const double* dt, int mode) {
-    return guard(c, [&] {
-        const int S = c->S;
-        std::vector<double> h(S), hh(S);
-        HIP_OK(hipMemcpyAsync(h.data(), dt, S * 8, hipMemcpyDeviceToHost, c->st));
-        HIP_OK(hipStreamSynchronize(c->st));
-        for (int s = 0; s < S; ++s) hh[s] = h[s] * 0.5;   // dt_half = 0.5 * dt (velocity_verlet.cuh:28-29)
-        c->vv_dt.upload(h.data(), S, c->st);
-        c->vv_dth.upload(hh.data(), S, c->st);
-        c->vv_mode = mode;
-        compute_forces_impl(c, mode);
-        c->vv_ready = true;
-    });
+    return guard(c, [&] { vv_init_impl(c, dt, mode); });
 }
 
 int dpmd_vv_step(dpmd_ctx* c, int n) {
@@ -1377,6 +1447,16 @@ int dpmd_set_option(dpmd_ctx* c, int key, int value) {
             case DPMD_OPT_COOP_REBUILD: c->use_coop = value != 0; break;
             case DPMD_OPT_COOP_IN_GRAPH: c->coop_in_graph = value != 0; break;
             case DPMD_OPT_FUSED_VV: c->fuse_vv = value != 0; break;
+            case DPMD_OPT_BLOCK:
+                need(value == 64 || value == 128 || value == 256 || value == 512,
+                     "DPMD_OPT_BLOCK must be 64, 128, 256 or 512");
+                c->block = value;
+                break;
+            case DPMD_OPT_MAX_BATCH:
+                need(value >= 1 && value <= 128 && (value & (value - 1)) == 0,
+                     "DPMD_OPT_MAX_BATCH must be a power of two from 1 to 128");
+                c->max_batch = value;
+                break;
             default: throw ArgError("unknown option key");
         }
     });
@@ -1388,6 +1468,8 @@ int dpmd_get_option(dpmd_ctx* c, int key, int* value) {
             case DPMD_OPT_COOP_REBUILD: *value = c->use_coop ? 1 : 0; break;
             case DPMD_OPT_COOP_IN_GRAPH: *value = c->coop_in_graph ? 1 : 0; break;
             case DPMD_OPT_FUSED_VV: *value = c->fuse_vv ? 1 : 0; break;
+            case DPMD_OPT_BLOCK: *value = c->block; break;
+            case DPMD_OPT_MAX_BATCH: *value = c->max_batch; break;
             default: throw ArgError("unknown option key");
         }
     });

@@ -168,13 +168,11 @@ __device__ __forceinline__ void ph_copy_back(const RebuildArgs& r, int p) {
 }
 
 // count_cell_neighbors_kernel + fill_neighbors_cell_kernel (base_point_particle_kernels.cu:
-// 37-157) in one pass, writing a fixed-capacity ELL list.  Returns the neighbor count (0 for
-// objects of systems that are not rebuilt); the caller folds the largest count into
-// nb.max_count.
-__device__ __forceinline__ int ph_neighbors(const RebuildArgs& r, int i) {
+// 37-157) in one pass, writing a fixed-capacity ELL list.
+__device__ __forceinline__ void ph_neighbors(const RebuildArgs& r, int i) {
     const ObjArgs& o = r.o; const CellArgs& ca = r.ca; const NbrArgs& nb = r.nb;
     const int s = o.sid[i];
-    if (!ca.flag[s]) return 0;
+    if (!ca.flag[s]) return;
     const int nx = ca.cdx[s], ny = ca.cdy[s], cs = ca.cstart[s];
     const int loc = ca.cell_id[i] - cs;
     const int cxi = loc % nx, cyi = loc / nx;
@@ -204,7 +202,6 @@ __device__ __forceinline__ int ph_neighbors(const RebuildArgs& r, int i) {
     }
     if (cnt > nb.K) atomicMax(nb.overflow, cnt);
     nb.nn[i] = cnt < nb.K ? cnt : nb.K;
-    return cnt;
 }
 
 // reset_displacements_kernel for rigid bodies (rigid_bumpy.cu:1070-1085): the skin
@@ -290,14 +287,12 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_neighbors(RebuildArgs r, const i
             nb.nn[i] = cnt < K ? cnt : K;
         }
     }
-    // largest stored count of the workgroup (rows to write) and of the wave (max_count)
+    // largest stored count of the workgroup: the rows to write.  (No global maximum is kept
+    // here: one same-address atomic per wave serialises the whole launch.)
     int m = cnt;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0) {
-        wmax[threadIdx.x >> 6] = m;
-        if (m > 0) atomicMax(nb.max_count, m);
-    }
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
     __syncthreads();
     int rows = 0;
 #pragma unroll
@@ -309,16 +304,12 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_neighbors(RebuildArgs r, const i
             nb.nbr[(size_t)k * o.n + i] = k < mine ? nbr_l[k * DPMD_BLOCK + threadIdx.x] : -1;
 }
 
-// Capacities above kLdsCapacity: write the ELL table directly (one atomic per wave for the
-// largest count).
+// Capacities above kLdsCapacity: write the ELL table directly.
 constexpr int kLdsCapacity = 64;
 __global__ void __launch_bounds__(DPMD_BLOCK) k_neighbors_direct(RebuildArgs r, const int* active) {
     if (!*active) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int cnt = i < r.o.n ? ph_neighbors(r, i) : 0;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) cnt = max(cnt, __shfl_xor(cnt, off, 64));
-    if ((threadIdx.x & 63) == 0 && cnt > 0) atomicMax(r.nb.max_count, cnt);
+    if (i < r.o.n) ph_neighbors(r, i);
 }
 
 __global__ void k_tile_sum(RebuildArgs r, const int* active) {
@@ -387,10 +378,7 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_rebuild_coop(RebuildArgs r, cons
     g.sync();
     for (int p = gt; p < n; p += gs) ph_copy_back(r, p);
     g.sync();
-    for (int i = gt; i < n; i += gs) {
-        const int cnt = ph_neighbors(r, i);
-        if (cnt > 0) atomicMax(r.nb.max_count, cnt);
-    }
+    for (int i = gt; i < n; i += gs) ph_neighbors(r, i);
     if (r.plx)
         for (int i = gt; i < r.N; i += gs) ph_rigid_reset(r, i);
     g.sync();
@@ -414,7 +402,6 @@ __global__ void k_naive(ObjArgs o, NbrArgs nb) {
         ++cnt;
     }
     if (cnt > nb.K) atomicMax(nb.overflow, cnt);
-    atomicMax(nb.max_count, cnt);
     nb.nn[i] = cnt < nb.K ? cnt : nb.K;
 }
 
@@ -441,7 +428,7 @@ CellArgs cell_args(dpmd_ctx* c) {
 
 NbrArgs nbr_args(dpmd_ctx* c) {
     NbrArgs nb;
-    nb.nn = c->nn.p; nb.nbr = c->nbr.p; nb.K = c->K; nb.overflow = c->overflow.p; nb.max_count = c->max_count.p;
+    nb.nn = c->nn.p; nb.nbr = c->nbr.p; nb.K = c->K; nb.overflow = c->overflow.p;
     return nb;
 }
 

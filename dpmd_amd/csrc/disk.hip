@@ -210,7 +210,8 @@ __global__ void k_stress(DiskArgs a, double* sxx, double* sxy, double* syx, doub
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ bool fire_drift_one(const DiskArgs& a, const FireArgs& f, int i, int s);
 
-__global__ void __launch_bounds__(DPMD_BLOCK) k_fire_drift(DiskArgs a, FireArgs f, Gate g) {
+template <int B>
+__global__ void __launch_bounds__(B) k_fire_drift(DiskArgs a, FireArgs f, Gate g) {
     if (!gate_open(g)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < a.n;
@@ -344,7 +345,8 @@ __global__ void k_fire_update(int S, FireArgs f, const double* pw_tot) {
 //   k_vv_drift: update_velocities(dt_half) + update_positions(dt) + skin check;
 //   rebuild;  k_vv_force: compute_forces + update_velocities(dt_half).
 // ---------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(DPMD_BLOCK)
+template <int B>
+__global__ void __launch_bounds__(B)
 k_vv_drift(DiskArgs a, const double* __restrict__ dt, const double* __restrict__ dth, Gate g) {
     if (!gate_open(g)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -367,11 +369,11 @@ k_vv_drift(DiskArgs a, const double* __restrict__ dt, const double* __restrict__
     if (a.cell) raise_flags(a.flag, a.any, s, flag);
 }
 
-template <int MODE>
-__global__ void __launch_bounds__(DPMD_BLOCK)
+template <int MODE, int B>
+__global__ void __launch_bounds__(B)
 k_vv_force(DiskArgs a, const double* __restrict__ dth, Gate g) {
     if (!gate_force(g)) return;
-    const int i = xcd_remap(blockIdx.x, gridDim.x) * DPMD_BLOCK + threadIdx.x;
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * B + threadIdx.x;
     if (i >= a.n) return;
     const int s = a.sid[i];
     double fx, fy, pe;
@@ -384,7 +386,8 @@ k_vv_force(DiskArgs a, const double* __restrict__ dth, Gate g) {
 
 // The step's first half (update_velocities(dt_half) + update_positions(dt)) writing the new
 // positions to (xo, yo): the start of a fused run.
-__global__ void __launch_bounds__(DPMD_BLOCK)
+template <int B>
+__global__ void __launch_bounds__(B)
 k_vv_drift_to(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, const double* __restrict__ dt,
               const double* __restrict__ dth, Gate g) {
     if (!gate_open(g)) return;
@@ -413,12 +416,12 @@ k_vv_drift_to(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, cons
 // (xo, yo) and the skin test of the new positions.  Every force must see its neighbors' old
 // positions, hence the second buffer.  Each expression is the one of k_vv_force / k_vv_drift,
 // in the same order, so a fused run equals the two-launch run bit for bit.
-template <int MODE>
-__global__ void __launch_bounds__(DPMD_BLOCK)
+template <int MODE, int B>
+__global__ void __launch_bounds__(B)
 k_vv_fused(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, const double* __restrict__ dt,
            const double* __restrict__ dth, Gate g) {
     if (!gate_force(g)) return;
-    const int i = xcd_remap(blockIdx.x, gridDim.x) * DPMD_BLOCK + threadIdx.x;
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * B + threadIdx.x;
     const bool valid = i < a.n;
     const int s = valid ? a.sid[i] : -1;
     bool flag = false;
@@ -441,6 +444,17 @@ k_vv_fused(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, const d
         }
     }
     if (a.cell) raise_flags(a.flag, a.any, s, flag);
+}
+
+// Workgroup size of the per-particle step kernels (the tunable block shape).
+template <class F>
+void by_block(int block, F&& f) {
+    switch (block) {
+        case 64: f(std::integral_constant<int, 64>{}); break;
+        case 128: f(std::integral_constant<int, 128>{}); break;
+        case 512: f(std::integral_constant<int, 512>{}); break;
+        default: f(std::integral_constant<int, 256>{}); break;
+    }
 }
 
 template <class F>
@@ -485,7 +499,10 @@ void disk_stress(dpmd_ctx* c) {
 }
 
 void disk_fire_drift(dpmd_ctx* c, Gate g) {
-    k_fire_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), fire_args(c), g);
+    by_block(c->block, [&](auto b) {
+        constexpr int B = decltype(b)::value;
+        k_fire_drift<B><<<blocks_for(c->N, B), B, 0, c->st>>>(disk_args(c), fire_args(c), g);
+    });
     HIP_OK(hipGetLastError());
 }
 
@@ -508,14 +525,20 @@ void fire_finalize(dpmd_ctx* c, Gate g, int do_conv, int pred, double pa, double
 }
 
 void disk_vv_drift(dpmd_ctx* c, Gate g) {
-    k_vv_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), c->vv_dt.p, c->vv_dth.p, g);
+    by_block(c->block, [&](auto b) {
+        constexpr int B = decltype(b)::value;
+        k_vv_drift<B><<<blocks_for(c->N, B), B, 0, c->st>>>(disk_args(c), c->vv_dt.p, c->vv_dth.p, g);
+    });
     HIP_OK(hipGetLastError());
 }
 
 void disk_vv_drift_to(dpmd_ctx* c, Gate g, double* xo, double* yo, int any_word) {
     DiskArgs a = disk_args(c);
     a.any = c->run.p + any_word;
-    k_vv_drift_to<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(a, xo, yo, c->vv_dt.p, c->vv_dth.p, g);
+    by_block(c->block, [&](auto b) {
+        constexpr int B = decltype(b)::value;
+        k_vv_drift_to<B><<<blocks_for(c->N, B), B, 0, c->st>>>(a, xo, yo, c->vv_dt.p, c->vv_dth.p, g);
+    });
     HIP_OK(hipGetLastError());
 }
 
@@ -525,8 +548,11 @@ void disk_vv_fused(dpmd_ctx* c, Gate g, const double* xs, const double* ys, doub
     a.y = const_cast<double*>(ys);
     a.any = c->run.p + any_word;
     by_mode(c->vv_mode, [&](auto m) {
-        k_vv_fused<decltype(m)::value><<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(a, xo, yo, c->vv_dt.p,
-                                                                                  c->vv_dth.p, g);
+        by_block(c->block, [&](auto b) {
+            constexpr int B = decltype(b)::value;
+            k_vv_fused<decltype(m)::value, B><<<blocks_for(c->N, B), B, 0, c->st>>>(a, xo, yo, c->vv_dt.p,
+                                                                                     c->vv_dth.p, g);
+        });
     });
     HIP_OK(hipGetLastError());
 }
@@ -535,7 +561,10 @@ void disk_vv_force(dpmd_ctx* c, Gate g) {
     DiskArgs a = disk_args(c);
     ProfScope prof(c);
     by_mode(c->vv_mode, [&](auto m) {
-        k_vv_force<decltype(m)::value><<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(a, c->vv_dth.p, g);
+        by_block(c->block, [&](auto b) {
+            constexpr int B = decltype(b)::value;
+            k_vv_force<decltype(m)::value, B><<<blocks_for(c->N, B), B, 0, c->st>>>(a, c->vv_dth.p, g);
+        });
     });
     HIP_OK(hipGetLastError());
 }

@@ -34,7 +34,7 @@ struct PermArgs {
 
 struct NbrArgs {
     int* nn; int* nbr; int K;      // ELL, column-major: nbr[k * n + i]
-    int* overflow; int* max_count;
+    int* overflow;
 };
 
 // Run-control words (int array "run" of the context):
@@ -125,7 +125,7 @@ struct dpmd_ctx {
     // ---- cells and neighbors (over objects: particles for disks, vertices for rigid)
     dpmd::DVec<int> cell_id, cell_count, cell_start, cell_wptr, cell_sys, order, order_inv;
     dpmd::DVec<int> tmp_i; dpmd::DVec<double> tmp_d;
-    dpmd::DVec<int> nn, nbr, overflow, max_count;
+    dpmd::DVec<int> nn, nbr, overflow;
     dpmd::DVec<int> scan_part;
 
     // ---- system-aligned 256-wide chunks over particles (deterministic reductions)
@@ -183,6 +183,10 @@ struct dpmd_ctx {
     // fused velocity Verlet (disks): second position buffer and the switch
     dpmd::DVec<double> x2, y2;
     bool fuse_vv = true;
+    // workgroup size of the disk step kernels (64, 128, 256, 512) and the largest number of
+    // steps captured into one hipGraph batch (a power of two, 1 .. 128)
+    int block = DPMD_BLOCK;
+    int max_batch = 128;
     // kernel timing (force kernels; outside graph capture only)
     int prof = 0;   // 0 off, 1 force kernels, 2 fused velocity-Verlet kernels
     bool capturing = false;
@@ -195,6 +199,52 @@ struct dpmd_ctx {
 };
 
 namespace dpmd {
+
+// Error plumbing of the C ABI: every entry point runs its body under guard(), which maps the
+// exception types to dpmd_status and keeps the message on the context (or globally when
+// there is no context yet).
+std::string& global_error();
+template <class F>
+int guard(dpmd_ctx* c, F&& f) {
+    try {
+        f();
+        return DPMD_OK;
+    } catch (const ArgError& e) {
+        (c ? c->err : global_error()) = e.what();
+        return DPMD_ERR_ARGUMENT;
+    } catch (const StateError& e) {
+        (c ? c->err : global_error()) = e.what();
+        return DPMD_ERR_STATE;
+    } catch (const OverflowError& e) {
+        (c ? c->err : global_error()) = e.what();
+        return DPMD_ERR_OVERFLOW;
+    } catch (const std::exception& e) {
+        (c ? c->err : global_error()) = e.what();
+        return DPMD_ERR_HIP;
+    }
+}
+inline void need(bool ok, const std::string& msg) { if (!ok) throw ArgError(msg); }
+inline void need_state(bool ok, const std::string& msg) { if (!ok) throw StateError(msg); }
+
+// Host copy of a device vector (synchronous on the context's stream).
+template <class T>
+std::vector<T> dl(dpmd_ctx* c, const dpmd::DVec<T>& v) {
+    std::vector<T> h(v.n);
+    if (v.n) {
+        HIP_OK(hipMemcpyAsync(h.data(), v.p, v.n * sizeof(T), hipMemcpyDeviceToHost, c->st));
+        HIP_OK(hipStreamSynchronize(c->st));
+    }
+    return h;
+}
+
+// abi.hip: the autotuner's engine (tune.hip holds the search and the cache)
+void tune_get(dpmd_ctx* c, dpmd_tune_config* t);
+void tune_apply(dpmd_ctx* c, const dpmd_tune_config& t);
+void tune_snapshot(dpmd_ctx* c, double* dst);                   // canonical x, y, vx, vy (4N)
+void tune_restore(dpmd_ctx* c, const double* src, int mode);    // and recompute forces
+// Runs `steps` steps of FIRE (no stopping: max |F| < 0) or velocity Verlet from the current
+// state through the device-driven path; returns particle-steps done (steps x running N).
+int64_t tune_run(dpmd_ctx* c, int integrator, const double* dt, int mode, int steps);
 
 // cells.hip
 void build_chunks(dpmd_ctx* c);

@@ -118,7 +118,8 @@ int dpmd_sync(dpmd_ctx* ctx);
 int dpmd_init_neighbors(dpmd_ctx* ctx);                        /* base_particle.hpp:265-272 */
 int dpmd_update_neighbors(dpmd_ctx* ctx);                      /* base_particle.hpp:275-280 */
 int dpmd_check_neighbors(dpmd_ctx* ctx, int force_update);     /* base_particle.hpp:283-288 */
-/* n_neighbors (base_particle.hpp:298-310) plus the fixed per-object capacity used here. */
+/* n_neighbors (base_particle.hpp:298-310) as *total, plus the fixed per-object capacity used
+ * here and the largest count in the current lists.  Any pointer may be NULL. */
 int dpmd_neighbor_info(dpmd_ctx* ctx, int* capacity, int* max_count, int64_t* total);
 /* neighbor_start / neighbor_ids in the reference's CSR layout, current order. */
 int dpmd_get_neighbors_csr(dpmd_ctx* ctx, int* start, int* ids, int64_t ids_len);
@@ -272,10 +273,69 @@ int dpmd_stream_copy(double* dst, const double* src, int64_t n, int repeats, dou
  *     the library clears it by itself if capture of that launch fails.
  *   DPMD_OPT_FUSED_VV: 1 (default) = device-driven velocity-Verlet runs of disks compute a
  *     step's forces and the next step's kick and drift in one launch (positions ping-pong
- *     between two buffers); 0 = one drift and one force launch per step.  Same results. */
-enum dpmd_option { DPMD_OPT_COOP_REBUILD = 0, DPMD_OPT_COOP_IN_GRAPH = 1, DPMD_OPT_FUSED_VV = 2 };
+ *     between two buffers); 0 = one drift and one force launch per step.  Same results.
+ *   DPMD_OPT_BLOCK: workgroup size of the disk step kernels (velocity-Verlet drift, force and
+ *     fused launches, FIRE drift): 64, 128, 256 (default) or 512.  Same results.
+ *   DPMD_OPT_MAX_BATCH: the largest number of steps captured into one hipGraph batch of a
+ *     device-driven run, a power of two from 1 to 128 (default 128).  Same results. */
+enum dpmd_option {
+    DPMD_OPT_COOP_REBUILD = 0, DPMD_OPT_COOP_IN_GRAPH = 1, DPMD_OPT_FUSED_VV = 2, DPMD_OPT_BLOCK = 3,
+    DPMD_OPT_MAX_BATCH = 4
+};
 int dpmd_set_option(dpmd_ctx* ctx, int key, int value);
 int dpmd_get_option(dpmd_ctx* ctx, int key, int* value);
+
+/* ---- autotuner (no reference counterpart; the reference's cell size, skin and launch
+ * shapes are fixed by the caller) -------------------------------------------------------
+ * A configuration: for every system s the neighbor cutoff (Verlet skin) is
+ * skin * 2 * r_min,s and the rebuild threshold is (cutoff / 2)^2; the cell grid is the
+ * largest whose cells are at least cell_factor * (2 r_max,s + cutoff) wide; block, max_batch
+ * and fused_vv are the DPMD_OPT_BLOCK / _MAX_BATCH / _FUSED_VV options.  None of these
+ * changes the physics: the neighbor lists always hold every pair within the interaction
+ * range, so trajectories differ only by the summation order of forces. */
+typedef struct dpmd_tune_config {
+    double skin;
+    double cell_factor;
+    int block;
+    int max_batch;
+    int fused_vv;
+} dpmd_tune_config;
+typedef struct dpmd_tune_trial {
+    dpmd_tune_config config;
+    double rate;      /* particle-steps per second */
+    double seconds;   /* timed seconds of the trial */
+} dpmd_tune_trial;
+enum dpmd_tune_integrator { DPMD_TUNE_FIRE = 0, DPMD_TUNE_NVE = 1 };
+/* The cell grid of one box for a configuration: nx = floor(Lx / (cell_factor * (2 r_max +
+ * cutoff))), likewise ny.  Fails (DPMD_ERR_ARGUMENT) unless both are >= 3: the periodic 3x3
+ * stencil visits a cell twice on a smaller grid, and cells narrower than 2 r_max + cutoff
+ * miss pairs (reference precondition, base_point_particle_kernels.cu:72-94, unchecked
+ * there).  Needs no context and no GPU. */
+int dpmd_cell_grid(double Lx, double Ly, double r_max, double cutoff, double cell_factor, int* nx, int* ny);
+/* The context's configuration (skin: largest cutoff / (2 r_min) over systems; cell_factor:
+ * smallest cell side / (2 r_max + cutoff)). */
+int dpmd_tune_get(dpmd_ctx* ctx, dpmd_tune_config* config);
+/* Applies a configuration (disks, cell lists, after init_neighbors) and rebuilds. */
+int dpmd_tune_apply(dpmd_ctx* ctx, const dpmd_tune_config* config);
+/* The workload signature that keys the cache: kind, systems, particles, size range, boxes,
+ * packing fraction, size ratio, integrator, force mode, device and library version. */
+int dpmd_tune_signature(dpmd_ctx* ctx, int integrator, int mode, char* buf, int capacity);
+/* Picks the configuration with the most particle-steps per second for `integrator` (FIRE
+ * without stopping, or velocity Verlet) with per-system dt (device, S; 0 = masked) and force
+ * mode, each trial timing `steps` device-driven, graph-captured steps from the current
+ * state after a short warm-up, one knob at a time (skin, cell_factor, block, fused_vv for
+ * NVE, max_batch).  Positions and velocities are restored after every trial, and the best
+ * configuration is applied at the end; integrator state must be initialised again
+ * afterwards.  With cache_path (JSON, may be NULL) a stored entry for this signature is
+ * applied without trials (*n_trials = 0), and a new result is stored.  trials (host, may be
+ * NULL): up to trial_capacity records of the trials run. */
+int dpmd_autotune(dpmd_ctx* ctx, int integrator, const double* dt, int mode, int steps, const char* cache_path,
+                  dpmd_tune_config* best, double* best_rate, dpmd_tune_trial* trials, int trial_capacity,
+                  int* n_trials);
+/* The JSON cache on its own (no context, no GPU): look up / store the entry for a signature. */
+int dpmd_tune_cache_load(const char* path, const char* signature, dpmd_tune_config* config, double* rate,
+                         int* found);
+int dpmd_tune_cache_store(const char* path, const char* signature, const dpmd_tune_config* config, double rate);
 
 /* Library version string. */
 const char* dpmd_version(void);
