@@ -50,6 +50,10 @@ def parse():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the lean step batches directly instead of replaying hipGraphs "
                          "(same kernels; for profilers that cannot follow graph launches)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="jam512: skip the autotuner (untimed) and run the library defaults")
+    ap.add_argument("--tune-cache", default=None,
+                    help="jam512: JSON autotune cache to read (and extend); default: tune afresh")
     return ap.parse_args()
 
 
@@ -128,6 +132,14 @@ def run_jam512(args):
         jam_compress(load(W.system_of(b, 0)), JAM["dt"], JAM["dphi"], JAM["p_target"], 2, args.warmup, **kw)
     P = load(b)
     log(f"batched: S={b.S} N={b.N}")
+    tune = None
+    if not args.no_tune:   # untimed, like the warm-up: picks skin, cell size, block, batch length
+        from dpmd_amd import autotune as T
+        t0 = time.perf_counter()
+        res = T.autotune(P, T.FIRE, JAM["dt"], mode=L.FORCE_PER_SYSTEM, steps=256, cache=args.tune_cache)
+        tune = {"config": res.config, "rate": res.rate, "trials": len(res.trials),
+                "seconds": time.perf_counter() - t0, "signature": res.signature}
+        log(f"autotune: {res.config} ({len(res.trials)} trials, {tune['seconds']:.2f} s)")
     t0 = time.perf_counter()
     st, nc, work = jam_compress(P, JAM["dt"], JAM["dphi"], JAM["p_target"], JAM["max_comp"], JAM["fire_max"], **kw)
     P.synchronize()
@@ -145,6 +157,9 @@ def run_jam512(args):
     work_p, t_p, worst, worst_abs = 0, 0.0, 0.0, 0.0
     for s in sample:
         Q = load(W.system_of(b, s))
+        if tune:   # the same per-system configuration, so both paths do the same arithmetic
+            from dpmd_amd import autotune as T
+            T.apply(Q, tune["config"])
         t1 = time.perf_counter()
         qst, qnc, qwork = jam_compress(Q, JAM["dt"], JAM["dphi"], JAM["p_target"], JAM["max_comp"], JAM["fire_max"],
                                        **kw)
@@ -175,7 +190,8 @@ def run_jam512(args):
                    "plain_sample_systems": len(sample), "plain_seconds": t_p, "plain_rate": rate_p,
                    "batched_over_plain": rate_b / rate_p,
                    "plain_vs_batched_max_rel_diff_jammed": worst,
-                   "plain_vs_batched_max_abs_diff_unjammed": worst_abs},
+                   "plain_vs_batched_max_abs_diff_unjammed": worst_abs,
+                   "autotune": tune},
         "roofline": None,
     }
     if not args.no_cpu:

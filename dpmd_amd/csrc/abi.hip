@@ -833,6 +833,15 @@ void tune_apply(dpmd_ctx* c, const dpmd_tune_config& t) {
 
 void tune_snapshot(dpmd_ctx* c, double* dst) { snapshot(c, dst); }
 
+void tune_restore_order(dpmd_ctx* c, const std::vector<int>& si0) {
+    const int n = c->N;
+    need((int)si0.size() == n, "tune_restore_order: wrong length");
+    const auto si = dl(c, c->static_index);
+    std::vector<int> order(n);
+    for (int k = 0; k < n; ++k) order[si0[k]] = si[k];
+    permute_particles(c, order.data());
+}
+
 void tune_restore(dpmd_ctx* c, const double* src, int mode) {
     const int N = c->N;
     adj_to_current(c, src + 2 * (size_t)N, c->vx.p);
@@ -1345,7 +1354,7 @@ int dpmd_fire_state(dpmd_ctx* c, double* dt, double* alpha, int* ng, int* nb) {
 int dpmd_fire_minimize(dpmd_ctx* c, const double* dt, int max_steps, int pred, double a, double b, int mode,
                        int use_graph, int poll, int* steps_taken, int* all_conv) {
     return guard(c, [&] {
-        need(pred == DPMD_FIRE_PRED_REFERENCE || pred == DPMD_FIRE_PRED_MAX_FORCE, "unknown FIRE predicate");
+        need(pred >= DPMD_FIRE_PRED_REFERENCE && pred <= DPMD_FIRE_PRED_FORCE_OR_PE, "unknown FIRE predicate");
         fire_init_impl(c, dt, mode);
         (void)poll;   // batches adapt to the rebuild rate; the host reads state once per batch
         int conv = 0;
@@ -1365,7 +1374,7 @@ int dpmd_jam_compress(dpmd_ctx* c, const double* dt, double dphi, double p_targe
                       int pred, double a, double b, int mode, int use_graph, int* jam_state, int* n_comp,
                       int64_t* work) {
     return guard(c, [&] {
-        need(pred == DPMD_FIRE_PRED_REFERENCE || pred == DPMD_FIRE_PRED_MAX_FORCE, "unknown FIRE predicate");
+        need(pred >= DPMD_FIRE_PRED_REFERENCE && pred <= DPMD_FIRE_PRED_FORCE_OR_PE, "unknown FIRE predicate");
         need(dt != nullptr && max_comp >= 0 && fire_max > 0, "jam_compress: bad arguments");
         need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
         const int S = c->S;

@@ -511,6 +511,23 @@ void build_chunks(dpmd_ctx* c) {
     c->part.resize(4 * (int64_t)std::max(1, c->nchunks));
 }
 
+// Moves every particle to a given current position: order[p] = the current index of the
+// particle that goes to p (a permutation within each system).  Uses the rebuild's gather and
+// copy-back phases, so every per-particle array, static_index and the displacement reference
+// follow; cells and neighbor lists are stale afterwards (the caller rebuilds).
+void permute_particles(dpmd_ctx* c, const int* order_host) {
+    const int S = c->S, n = c->n_obj();
+    hipStream_t st = c->st;
+    c->order.upload(order_host, n, st);
+    const int* active = c->one.p;
+    k_set_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, c->flag.p, active);
+    RebuildArgs r = rebuild_args(c);
+    k_gather<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
+    k_copy_back<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
+    k_clear_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, r.ca.flag, r.any, active);
+    HIP_OK(hipGetLastError());
+}
+
 void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all) {
     const int S = c->S;
     hipStream_t st = c->st;

@@ -72,8 +72,10 @@ __device__ __forceinline__ void fire_convergence_sys(const FireArgs& f, const in
     const double diff = avg_prev == 0 ? 0 : fabs(avg / avg_prev - 1);
     if (pred == DPMD_FIRE_PRED_REFERENCE) {
         if (diff < a || avg < b) f.dt[s] = 0.0;
-    } else {
+    } else if (pred == DPMD_FIRE_PRED_MAX_FORCE) {
         if (sqrt(maxf2) < a) f.dt[s] = 0.0;
+    } else {   // DPMD_FIRE_PRED_FORCE_OR_PE
+        if (sqrt(maxf2) < a || avg < b) f.dt[s] = 0.0;
     }
     f.peprev[s] = avg;
 }

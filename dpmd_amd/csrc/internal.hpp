@@ -242,6 +242,9 @@ void tune_get(dpmd_ctx* c, dpmd_tune_config* t);
 void tune_apply(dpmd_ctx* c, const dpmd_tune_config& t);
 void tune_snapshot(dpmd_ctx* c, double* dst);                   // canonical x, y, vx, vy (4N)
 void tune_restore(dpmd_ctx* c, const double* src, int mode);    // and recompute forces
+// Puts every particle back at the current position static_index0[k] had (canonical k), so
+// later rebuilds break in-cell ties as they would have without the trials in between.
+void tune_restore_order(dpmd_ctx* c, const std::vector<int>& static_index0);
 // Runs `steps` steps of FIRE (no stopping: max |F| < 0) or velocity Verlet from the current
 // state through the device-driven path; returns particle-steps done (steps x running N).
 int64_t tune_run(dpmd_ctx* c, int integrator, const double* dt, int mode, int steps);
@@ -249,6 +252,7 @@ int64_t tune_run(dpmd_ctx* c, int integrator, const double* dt, int mode, int st
 // cells.hip
 void build_chunks(dpmd_ctx* c);
 void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all);
+void permute_particles(dpmd_ctx* c, const int* order_host);
 void clear_flags(dpmd_ctx* c, const int* active);
 void naive_neighbors(dpmd_ctx* c);
 void check_overflow(dpmd_ctx* c);

@@ -317,6 +317,14 @@ int dpmd_autotune(dpmd_ctx* c, int integrator, const double* dt, int mode, int s
         dpmd::DVec<double> snap;
         snap.resize(4 * (int64_t)N);
         tune_snapshot(c, snap.p);
+        const std::vector<int> si0 = dl(c, c->static_index);
+        // ends every path: the starting particle order and state under configuration `cfg`
+        auto finish = [&](const dpmd_tune_config& cfg) {
+            tune_restore(c, snap.p, mode);
+            tune_restore_order(c, si0);
+            tune_apply(c, cfg);
+            tune_restore(c, snap.p, mode);
+        };
         const int warm = std::min(steps, 32);
         auto trial = [&](const dpmd_tune_config& cfg) -> double {
             tune_apply(c, cfg);
@@ -369,12 +377,10 @@ int dpmd_autotune(dpmd_ctx* c, int integrator, const double* dt, int mode, int s
             const dpmd_tune_config base4 = best;
             for (int v : batches) { dpmd_tune_config t = base4; t.max_batch = v; consider(t); }
         } catch (...) {
-            tune_apply(c, best);
-            tune_restore(c, snap.p, mode);
+            finish(best);
             throw;
         }
-        tune_apply(c, best);
-        tune_restore(c, snap.p, mode);
+        finish(best);
         HIP_OK(hipStreamSynchronize(c->st));
         if (cache_path) {
             auto ents = read_cache(cache_path);

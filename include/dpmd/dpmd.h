@@ -52,8 +52,9 @@ enum dpmd_force_mode { DPMD_FORCE_PBC = 0, DPMD_FORCE_WALL = 1, DPMD_FORCE_PER_S
 enum dpmd_order { DPMD_ORDER_CURRENT = 0, DPMD_ORDER_CANONICAL = 1 };
 
 /* FIRE stopping predicate: REFERENCE = fire_convergence_kernel (fire.cu:59-78);
- * MAX_FORCE = every particle's |F_i| below a tolerance (per system). */
-enum dpmd_fire_predicate { DPMD_FIRE_PRED_REFERENCE = 0, DPMD_FIRE_PRED_MAX_FORCE = 1 };
+ * MAX_FORCE = every particle's |F_i| below a tolerance (per system); FORCE_OR_PE = MAX_FORCE
+ * or the reference's unjammed test <pe> < avg_pe_target (per system). */
+enum dpmd_fire_predicate { DPMD_FIRE_PRED_REFERENCE = 0, DPMD_FIRE_PRED_MAX_FORCE = 1, DPMD_FIRE_PRED_FORCE_OR_PE = 2 };
 
 /* Fields (reference: public DeviceField members, base_particle.hpp:31-103,
  * base_point_particle.hpp:86-89, base_poly_particle.hpp:104-120, rigid_bumpy.cuh:31-45).

@@ -480,12 +480,13 @@ struct Disk {
     }
     // fire.cuh:98-109 with fire.cu:59-78.  pred 0: the reference's <pe> test (note the
     // double division of pe_prev, kept as written); pred 1: max|F_i| < tol (added for the
-    // max|F| stopping rule; pe_prev is still updated the same way).
+    // max|F| stopping rule; pe_prev is still updated the same way); pred 2: max|F_i| < tol
+    // or <pe> < pe_target.
     int fire_converged(double diff_target, double pe_target, int pred, double ftol) {
         FireState& F = fire;
         std::vector<double> pt(S), mf(S);
         pe_total(pt.data());
-        if (pred == 1) max_force2(mf.data());
+        if (pred != 0) max_force2(mf.data());
         bool all = true;
         for (int s = 0; s < S; ++s) {
             const int dof = off[s + 1] - off[s];
@@ -494,8 +495,10 @@ struct Disk {
             const double diff = avg_prev == 0 ? 0 : std::abs(avg / avg_prev - 1);
             if (pred == 0) {
                 if (diff < diff_target || avg < pe_target) F.dt[s] = 0.0;
-            } else {
+            } else if (pred == 1) {
                 if (std::sqrt(mf[s]) < ftol) F.dt[s] = 0.0;
+            } else {
+                if (std::sqrt(mf[s]) < ftol || avg < pe_target) F.dt[s] = 0.0;
             }
             F.pe_prev[s] = avg;
             if (F.dt[s] != 0.0) all = false;
@@ -922,7 +925,8 @@ struct Rigid {
             const double avg = pt[s] / dof, avg_prev = F.pe_prev[s] / dof;
             const double diff = avg_prev == 0 ? 0 : std::abs(avg / avg_prev - 1);
             if (pred == 0) { if (diff < diff_target || avg < pe_target) F.dt[s] = 0.0; }
-            else { if (std::sqrt(mf[s]) < ftol) F.dt[s] = 0.0; }
+            else if (pred == 1) { if (std::sqrt(mf[s]) < ftol) F.dt[s] = 0.0; }
+            else { if (std::sqrt(mf[s]) < ftol || avg < pe_target) F.dt[s] = 0.0; }
             F.pe_prev[s] = avg;
             if (F.dt[s] != 0.0) all = false;
         }
@@ -1225,7 +1229,7 @@ static int fire_minimize_counted(Disk* d, const double* dt, int max_steps, int p
     while (step < max_steps && !c) {
         for (int s = 0; s < S; ++s) if (!frozen[s]) nsteps[s]++;
         d->fire_step(mode);
-        c = pred == 0 ? d->fire_converged(a, b, 0, 0.0) : d->fire_converged(0.0, 0.0, 1, a);
+        c = pred == 0 ? d->fire_converged(a, b, 0, 0.0) : d->fire_converged(0.0, b, pred, a);
         for (int s = 0; s < S; ++s) {
             if (prev[s]) frozen[s] = 1;
             prev[s] = d->fire.dt[s] == 0.0;

@@ -167,6 +167,13 @@ def test_autotune_fire_restores_state_and_caches(tmp_path):
     assert now["skin"] == pytest.approx(c["skin"], rel=1e-15)
     hit = T.cache_load(cache, res.signature)
     assert hit is not None and hit[0] == c
+    # the tuned context is in the state of a fresh one with the configuration applied: same
+    # particle order and neighbor lists, so both run the same arithmetic from here
+    R = make(b)
+    T.apply(R, c)
+    assert np.array_equal(P.get("STATIC_INDEX"), R.get("STATIC_INDEX"))
+    for a1, a2 in zip(P.neighbors_csr(), R.neighbors_csr()):
+        assert np.array_equal(a1, a2)
     # a fresh context with the same workload: cache hit, no trials, same configuration applied
     Q = make(b)
     res2 = T.autotune(Q, T.FIRE, 1e-2, mode=L.FORCE_PER_SYSTEM, steps=64, cache=cache)

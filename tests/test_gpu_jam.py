@@ -100,3 +100,18 @@ def test_frozen_systems_skip_work_exactly():
     assert len(set(ns.tolist())) > 1, "systems should stop at different steps"
     for f, n in FIELDS:
         assert np.array_equal(P.get(n), o.get(f)), f
+
+
+def test_force_or_pe_predicate_matches_oracle():
+    """Jamming with stops at max|F| < 1e-9 or <pe> < 1e-16: library and oracle bit for bit."""
+    orc.lib().or_set_threads(1)
+    b = small_batch()
+    o = orc.OracleDisk(b)
+    st, nc, ns, work = o.jam_compress(1e-2, DPHI, PT, CMAX, FMAX, pred=2, a=FTOL, b=1e-16, mode=2)
+    P = load(b)
+    gst, gnc, gwork = jam_compress(P, 1e-2, DPHI, PT, CMAX, FMAX, L.FIRE_PRED_FORCE_OR_PE, FTOL, 1e-16,
+                                   mode=L.FORCE_PER_SYSTEM)
+    assert np.array_equal(gst, st) and np.array_equal(gnc, nc)
+    assert np.array_equal(P.get("FIRE_STEPS"), ns) and gwork == work
+    for f, n in FIELDS:
+        assert np.array_equal(P.get(n), o.get(f)), f

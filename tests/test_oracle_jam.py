@@ -106,3 +106,24 @@ def test_counted_steps_stop_one_after_stop():
     assert all(v is not None for v in stop)
     total = max(stop)
     assert list(ns) == [min(v + 1, total) for v in stop]
+
+
+def test_force_or_pe_predicate_stops_at_the_earlier_test():
+    """FORCE_OR_PE (pred 2) on one system stops at min(step of max|F| < a alone, step of
+    <pe> < b alone); the pe-only run is the reference predicate with a diff target of 0,
+    which never fires.  Two systems: one below jamming (pe test first), one above (force
+    test first)."""
+    for phi, expect_pe_first in ((0.78, True), (0.88, False)):
+        b = W.disk_batch([60], [phi], [0], seed=9)
+        runs = {}
+        for key, kw in (("f", dict(pred=1, ftol=1e-11)), ("pe", dict(pred=0, diff=0.0, tgt=1e-16)),
+                        ("both", dict(pred=2, ftol=1e-11, tgt=1e-16))):
+            o = orc.OracleDisk(b)
+            steps, conv = o.fire_minimize(1e-2, 60000, mode=0, **kw)
+            runs[key] = (steps, conv, o.get("x"))
+        assert runs["both"][1]
+        first = min(runs["f"][0] if runs["f"][1] else 10**9, runs["pe"][0] if runs["pe"][1] else 10**9)
+        assert runs["both"][0] == first
+        src = "pe" if expect_pe_first else "f"
+        assert runs[src][0] == first
+        assert np.array_equal(runs["both"][2], runs[src][2])
