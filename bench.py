@@ -109,8 +109,9 @@ def cpu_baseline_jam(b, systems: int, steps: int):
 def run_jam512(args):
     """512 uneven systems (N_s in [64, 4096], phi_s in [0.80, 0.86], even systems periodic, odd
     walled) jammed by dphi = 1e-3 compressions + FIRE (max|F| < 1e-10 per round) until
-    P_s >= 1e-7.  Batched: one context, fused FIRE rounds in hipGraphs, frozen systems skipped.
-    Plain: one context per system (the same driver on a batch of one), timed on a sample."""
+    P_s >= 1e-7.  Batched: one context; each FIRE round is one persistent launch with one
+    workgroup per system (stopped systems cost nothing).  Plain: one context per system with
+    per-step launches (graph-captured batches), timed on a sample."""
     import numpy as np
 
     from dpmd_amd import _lib as L
@@ -157,6 +158,9 @@ def run_jam512(args):
     work_p, t_p, worst, worst_abs = 0, 0.0, 0.0, 0.0
     for s in sample:
         Q = load(W.system_of(b, s))
+        # the plain method: one system per launch sequence (per-step launches, no persistent
+        # per-system kernel)
+        L.check(L.lib().dpmd_set_option(Q.ctx, L.OPT_PERSISTENT_FIRE, 0), Q.ctx)
         if tune:   # the same per-system configuration, so both paths do the same arithmetic
             from dpmd_amd import autotune as T
             T.apply(Q, tune["config"])

@@ -428,8 +428,12 @@ hipGraphExec_t capture_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, 
     return ex;
 }
 
-// Runs up to max_steps steps; returns the number taken and (FIRE) whether all systems stopped.
-int lean_run(dpmd_ctx* c, int max_steps, bool fire, int pred, double a, double b, bool use_graph, int* conv) {
+}  // namespace
+
+// Runs up to max_steps steps as graph-captured batches of per-step launches; returns the
+// number taken and (FIRE) whether all systems stopped.
+int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, double a, double b, bool use_graph,
+                           int* conv) {
     std::vector<hipGraphExec_t> graphs(8, nullptr);   // batch sizes 1, 2, 4, ..., 128
     int total = 0, m = 8, since_halt = 0;
     bool done = false;
@@ -495,6 +499,15 @@ int lean_run(dpmd_ctx* c, int max_steps, bool fire, int pred, double a, double b
     for (auto g : graphs) if (g) HIP_OK(hipGraphExecDestroy(g));
     if (conv) *conv = (fire && read_run(c).running <= 0) ? 1 : 0;
     return total;
+}
+
+namespace {
+
+// Runs up to max_steps device-driven steps; returns the number taken and (FIRE) whether all
+// systems stopped.  FIRE runs of disk batches go to the persistent per-system engine.
+int lean_run(dpmd_ctx* c, int max_steps, bool fire, int pred, double a, double b, bool use_graph, int* conv) {
+    if (fire && fire_persist_ok(c)) return fire_persistent_run(c, max_steps, pred, a, b, use_graph, conv);
+    return lean_run_batches(c, max_steps, fire, pred, a, b, use_graph, conv);
 }
 
 
@@ -906,7 +919,7 @@ int dpmd_ctx_destroy(dpmd_ctx* c) {
     (void)hipStreamSynchronize(c->st);
     if (c->g_fire) (void)hipGraphExecDestroy(c->g_fire);
     if (c->g_vv) (void)hipGraphExecDestroy(c->g_vv);
-    // All DVec members free in their destructors' absence: release explicitly.
+    // Free the large buffers first; every DVec member also frees itself when c is deleted.
     dpmd::DVec<double>* dv[] = {&c->Lx, &c->Ly, &c->iLx, &c->iLy, &c->e, &c->cutoff, &c->thresh2, &c->pe_tot,
                                 &c->ke_tot, &c->pw_tot, &c->press, &c->temp, &c->phi, &c->maxf2, &c->st_xx,
                                 &c->st_xy, &c->st_yx, &c->st_yy, &c->x, &c->y, &c->vx, &c->vy, &c->fx, &c->fy,
@@ -924,7 +937,8 @@ int dpmd_ctx_destroy(dpmd_ctx* c) {
                              &c->cell_count, &c->cell_start, &c->cell_wptr, &c->cell_sys, &c->order,
                              &c->order_inv, &c->tmp_i, &c->nn, &c->nbr, &c->overflow,
                              &c->scan_part, &c->chunk_sys, &c->chunk_beg, &c->sys_chunk, &c->f_ngood,
-                             &c->f_nbad, &c->f_stopped, &c->f_frozen, &c->f_nsteps, &c->s_ngood, &c->s_nbad, &c->run, &c->one, &c->sv_static};
+                             &c->f_nbad, &c->f_stopped, &c->f_frozen, &c->f_nsteps, &c->s_ngood, &c->s_nbad, &c->run, &c->one, &c->sv_static,
+                             &c->persist_order, &c->persist_todo, &c->persist_stop};
     for (auto* v : iv) v->release();
     c->flag.release();
     c->sys_counter.release();
@@ -1466,6 +1480,18 @@ int dpmd_set_option(dpmd_ctx* c, int key, int value) {
                      "DPMD_OPT_MAX_BATCH must be a power of two from 1 to 128");
                 c->max_batch = value;
                 break;
+            case DPMD_OPT_PERSISTENT_FIRE:
+                need(value >= 0 && value <= 2, "DPMD_OPT_PERSISTENT_FIRE must be 0, 1 or 2");
+                c->persist_fire = value;
+                break;
+            case DPMD_OPT_PERSIST_PHASE:
+                need(value >= 1, "DPMD_OPT_PERSIST_PHASE must be positive");
+                c->persist_phase = value;
+                break;
+            case DPMD_OPT_PERSIST_BLOCK:
+                need(value == 256 || value == 512 || value == 1024, "DPMD_OPT_PERSIST_BLOCK must be 256, 512 or 1024");
+                c->persist_block = value;
+                break;
             default: throw ArgError("unknown option key");
         }
     });
@@ -1478,6 +1504,9 @@ int dpmd_get_option(dpmd_ctx* c, int key, int* value) {
             case DPMD_OPT_COOP_IN_GRAPH: *value = c->coop_in_graph ? 1 : 0; break;
             case DPMD_OPT_FUSED_VV: *value = c->fuse_vv ? 1 : 0; break;
             case DPMD_OPT_BLOCK: *value = c->block; break;
+            case DPMD_OPT_PERSISTENT_FIRE: *value = c->persist_fire; break;
+            case DPMD_OPT_PERSIST_PHASE: *value = c->persist_phase; break;
+            case DPMD_OPT_PERSIST_BLOCK: *value = c->persist_block; break;
             case DPMD_OPT_MAX_BATCH: *value = c->max_batch; break;
             default: throw ArgError("unknown option key");
         }

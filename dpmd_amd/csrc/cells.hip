@@ -18,6 +18,7 @@
 //  * the neighbor list is a fixed-capacity column-major ELL array nbr[k*n+i] (coalesced
 //    for one-thread-per-object force kernels), with the reference's stencil order.
 #include "internal.hpp"
+#include "cells_dev.hpp"
 #include <hip/hip_cooperative_groups.h>
 
 namespace cg = cooperative_groups;
@@ -27,16 +28,6 @@ namespace {
 
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = DPMD_BLOCK * SCAN_ITEMS;
-
-struct RebuildArgs {
-    ObjArgs o; CellArgs ca; PermArgs pa; NbrArgs nb;
-    int* order; int* order_inv; int* static_index;
-    double* lx; double* ly;           // disks: displacement reference, reset per object
-    // rigid bodies: per-particle reference = first vertex of each particle
-    int N; const int* psid; const int* poff; double* plx; double* ply;
-    int* tile_sum; int nt; int S;
-    int* any;                          // the run word "some rebuild flag is set"
-};
 
 __device__ __forceinline__ int block_excl_scan_int(int v, int* lds4, int& total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -56,27 +47,6 @@ __device__ __forceinline__ int block_excl_scan_int(int v, int* lds4, int& total)
 }
 
 // ---- per-element phase bodies -------------------------------------------------------
-
-__device__ __forceinline__ void ph_zero(const RebuildArgs& r, int c) {
-    if (r.ca.flag[r.ca.cell_sys[c]]) r.ca.cell_count[c] = 0;
-}
-
-// assign_cell_ids_kernel (base_particle_kernels.cu:205-233) + count_cells_kernel (:235-244)
-__device__ __forceinline__ void ph_hash(const RebuildArgs& r, int i) {
-    const ObjArgs& o = r.o; const CellArgs& ca = r.ca;
-    const int s = o.sid[i];
-    if (!ca.flag[s]) return;
-    const double u = wrap01(o.x[i] * ca.iLx[s]);
-    const double v = wrap01(o.y[i] * ca.iLy[s]);
-    const int nx = ca.cdx[s], ny = ca.cdy[s];
-    int cx = (int)floor(u * nx);
-    int cy = (int)floor(v * ny);
-    cx = cx < nx ? (cx > 0 ? cx : 0) : nx - 1;   // guard: u*nx may round up to nx
-    cy = cy < ny ? (cy > 0 ? cy : 0) : ny - 1;
-    const int cell = ca.cstart[s] + cx + cy * nx;
-    ca.cell_id[i] = cell;
-    atomicAdd(&ca.cell_count[cell], 1);
-}
 
 // Block-level: sum of one tile of cell counts.
 __device__ __forceinline__ void ph_tile_sum(const RebuildArgs& r, int t, int* lds) {
@@ -127,83 +97,6 @@ __device__ __forceinline__ void ph_tile_down(const RebuildArgs& r, int t, int* l
     }
 }
 
-__device__ __forceinline__ void ph_scatter(const RebuildArgs& r, int i) {
-    if (!r.ca.flag[r.o.sid[i]]) return;
-    const int slot = atomicAdd(&r.ca.cell_wptr[r.ca.cell_id[i]], 1);
-    r.order[slot] = i;
-}
-
-// In-cell order: ascending current index (a stable sort by cell id).
-__device__ __forceinline__ void ph_sort(const RebuildArgs& r, int c) {
-    if (!r.ca.flag[r.ca.cell_sys[c]]) return;
-    const int b = r.ca.cell_start[c], e = r.ca.cell_start[c + 1];
-    int* ord = r.order;
-    for (int k = b + 1; k < e; ++k) {
-        const int key = ord[k];
-        int m = k - 1;
-        while (m >= b && ord[m] > key) { ord[m + 1] = ord[m]; --m; }
-        ord[m + 1] = key;
-    }
-}
-
-// reorder_particles_impl (disk.cu:1102-1125): gather into scratch by the new order.
-__device__ __forceinline__ void ph_gather(const RebuildArgs& r, int p) {
-    if (!r.ca.flag[r.o.sid[p]]) return;
-    const int n = r.o.n;
-    const int o = r.order[p];
-    for (int k = 0; k < r.pa.nd; ++k) r.pa.tmp_d[(size_t)k * n + p] = r.pa.d[k][o];
-    for (int k = 0; k < r.pa.ni; ++k) r.pa.tmp_i[(size_t)k * n + p] = r.pa.i[k][o];
-    r.order_inv[o] = p;
-}
-
-// Copy back, update static_index (update_static_index_kernel, base_particle_kernels.cu:
-// 172-180) and reset the displacement reference (reset_displacements_impl, disk.cu:1097).
-__device__ __forceinline__ void ph_copy_back(const RebuildArgs& r, int p) {
-    if (!r.ca.flag[r.o.sid[p]]) return;
-    const int n = r.o.n;
-    for (int k = 0; k < r.pa.nd; ++k) r.pa.d[k][p] = r.pa.tmp_d[(size_t)k * n + p];
-    for (int k = 0; k < r.pa.ni; ++k) r.pa.i[k][p] = r.pa.tmp_i[(size_t)k * n + p];
-    r.static_index[p] = r.order_inv[r.static_index[p]];
-    if (r.lx) { r.lx[p] = r.pa.tmp_d[p]; r.ly[p] = r.pa.tmp_d[(size_t)n + p]; }
-}
-
-// count_cell_neighbors_kernel + fill_neighbors_cell_kernel (base_point_particle_kernels.cu:
-// 37-157) in one pass, writing a fixed-capacity ELL list.
-__device__ __forceinline__ void ph_neighbors(const RebuildArgs& r, int i) {
-    const ObjArgs& o = r.o; const CellArgs& ca = r.ca; const NbrArgs& nb = r.nb;
-    const int s = o.sid[i];
-    if (!ca.flag[s]) return;
-    const int nx = ca.cdx[s], ny = ca.cdy[s], cs = ca.cstart[s];
-    const int loc = ca.cell_id[i] - cs;
-    const int cxi = loc % nx, cyi = loc / nx;
-    const double Lx = ca.Lx[s], Ly = ca.Ly[s], iLx = ca.iLx[s], iLy = ca.iLy[s], cut0 = ca.cutoff[s];
-    const double xi = o.x[i], yi = o.y[i], ri = o.rad[i];
-    const int pi = o.pid ? o.pid[i] : -1;
-    int cnt = 0;
-    for (int dy = -1; dy <= 1; ++dy) {
-        int yy = cyi + dy; if (yy < 0) yy += ny; else if (yy >= ny) yy -= ny;
-        for (int dx = -1; dx <= 1; ++dx) {
-            int xx = cxi + dx; if (xx < 0) xx += nx; else if (xx >= nx) xx -= nx;
-            const int nc = cs + yy * nx + xx;
-            const int b = ca.cell_start[nc], e = ca.cell_start[nc + 1];
-            for (int j = b; j < e; ++j) {
-                if (j == i) continue;
-                if (o.pid && o.pid[j] == pi) continue;
-                const double ddx = min_image(xi - o.x[j], Lx, iLx);
-                const double ddy = min_image(yi - o.y[j], Ly, iLy);
-                const double r2 = ddx * ddx + ddy * ddy;
-                const double cut = ri + o.rad[j] + cut0;
-                if (r2 < cut * cut) {
-                    if (cnt < nb.K) nb.nbr[(size_t)cnt * o.n + i] = j;
-                    ++cnt;
-                }
-            }
-        }
-    }
-    if (cnt > nb.K) atomicMax(nb.overflow, cnt);
-    nb.nn[i] = cnt < nb.K ? cnt : nb.K;
-}
-
 // reset_displacements_kernel for rigid bodies (rigid_bumpy.cu:1070-1085): the skin
 // reference point is the first vertex of each particle.
 __device__ __forceinline__ void ph_rigid_reset(const RebuildArgs& r, int i) {
@@ -247,42 +140,9 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_neighbors(RebuildArgs r, const i
         const int s = o.sid[i];
         rebuilt = ca.flag[s] != 0;
         if (rebuilt) {
-            const int nx = ca.cdx[s], ny = ca.cdy[s], cs = ca.cstart[s];
-            const int loc = ca.cell_id[i] - cs;
-            const int cxi = loc % nx, cyi = loc / nx;
-            const double Lx = ca.Lx[s], Ly = ca.Ly[s], iLx = ca.iLx[s], iLy = ca.iLy[s], cut0 = ca.cutoff[s];
-            const double xi = o.x[i], yi = o.y[i], ri = o.rad[i];
-            const int pi = o.pid ? o.pid[i] : -1;
-            // Each stencil row (cells cx-1, cx, cx+1) is one contiguous object range unless it
-            // wraps in x; scanning ranges in stencil order visits the same objects in the same
-            // order as the cell-by-cell loop.
-            auto scan = [&](int b, int e) {
-                for (int j = b; j < e; ++j) {
-                    if (j == i) continue;
-                    if (o.pid && o.pid[j] == pi) continue;
-                    const double ddx = min_image(xi - o.x[j], Lx, iLx);
-                    const double ddy = min_image(yi - o.y[j], Ly, iLy);
-                    const double r2 = ddx * ddx + ddy * ddy;
-                    const double cut = ri + o.rad[j] + cut0;
-                    if (r2 < cut * cut) {
-                        if (cnt < K) nbr_l[cnt * DPMD_BLOCK + threadIdx.x] = j;
-                        ++cnt;
-                    }
-                }
-            };
-            for (int dy = -1; dy <= 1; ++dy) {
-                int yy = cyi + dy; if (yy < 0) yy += ny; else if (yy >= ny) yy -= ny;
-                const int row = cs + yy * nx;
-                if (cxi == 0) {               // cells nx-1 | 0, 1
-                    scan(ca.cell_start[row + nx - 1], ca.cell_start[row + nx]);
-                    scan(ca.cell_start[row], ca.cell_start[row + 2]);
-                } else if (cxi == nx - 1) {   // cells nx-2, nx-1 | 0
-                    scan(ca.cell_start[row + nx - 2], ca.cell_start[row + nx]);
-                    scan(ca.cell_start[row], ca.cell_start[row + 1]);
-                } else {                      // cells cx-1 .. cx+1
-                    scan(ca.cell_start[row + cxi - 1], ca.cell_start[row + cxi + 2]);
-                }
-            }
+            cnt = neighbor_scan(r, i, s, [&](int k, int j) {
+                if (k < K) nbr_l[k * DPMD_BLOCK + threadIdx.x] = j;
+            });
             if (cnt > K) atomicMax(nb.overflow, cnt);
             nb.nn[i] = cnt < K ? cnt : K;
         }
@@ -449,7 +309,7 @@ PermArgs perm_args(dpmd_ctx* c) {
     return pa;
 }
 
-RebuildArgs rebuild_args(dpmd_ctx* c) {
+RebuildArgs make_rebuild_args(dpmd_ctx* c) {
     RebuildArgs r;
     r.o = obj_args(c); r.ca = cell_args(c); r.pa = perm_args(c); r.nb = nbr_args(c);
     r.order = c->order.p; r.order_inv = c->order_inv.p; r.static_index = c->static_index.p;
@@ -496,6 +356,8 @@ int coop_grid(dpmd_ctx* c) {
 
 }  // namespace
 
+RebuildArgs rebuild_args(dpmd_ctx* c) { return make_rebuild_args(c); }
+
 void build_chunks(dpmd_ctx* c) {
     std::vector<int> cs, cb, sc(c->S + 1, 0);
     for (int s = 0; s < c->S; ++s) {
@@ -521,7 +383,7 @@ void permute_particles(dpmd_ctx* c, const int* order_host) {
     c->order.upload(order_host, n, st);
     const int* active = c->one.p;
     k_set_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, c->flag.p, active);
-    RebuildArgs r = rebuild_args(c);
+    RebuildArgs r = make_rebuild_args(c);
     k_gather<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     k_copy_back<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     k_clear_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, r.ca.flag, r.any, active);
@@ -532,7 +394,7 @@ void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all) {
     const int S = c->S;
     hipStream_t st = c->st;
     if (force_all) k_set_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, c->flag.p, active);
-    RebuildArgs r = rebuild_args(c);
+    RebuildArgs r = make_rebuild_args(c);
     const bool coop = c->use_coop && !(c->capturing && !c->coop_in_graph);
     if (coop) {
         void* args[] = {(void*)&r, (void*)&active};

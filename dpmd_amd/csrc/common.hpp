@@ -7,6 +7,8 @@
 #include <stdexcept>
 
 #define DPMD_BLOCK 256
+// largest system the persistent per-system FIRE kernel takes (64 chunks of 256)
+constexpr int kPersistMaxN = 16384;
 
 namespace dpmd {
 
@@ -22,12 +24,16 @@ struct OverflowError : Error { using Error::Error; };   // maps to DPMD_ERR_OVER
             throw ::dpmd::Error(std::string(#expr) + ": " + hipGetErrorString(_e));           \
     } while (0)
 
-// Device vector owned by a context.  Grows only; never shrinks while alive.
+// Owning device vector (not copyable).  Grows only; never shrinks while alive.
 template <class T>
 struct DVec {
     T* p = nullptr;
     int64_t n = 0;
     int64_t cap = 0;
+    DVec() = default;
+    DVec(const DVec&) = delete;
+    DVec& operator=(const DVec&) = delete;
+    ~DVec() { release(); }
     void resize(int64_t m) {
         if (m > cap) {
             if (p) HIP_OK(hipFree(p));

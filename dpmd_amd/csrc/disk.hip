@@ -8,91 +8,10 @@
 // reference's sequence of primitives.
 #include "internal.hpp"
 #include "fire.hpp"
+#include "disk_dev.hpp"
 
 namespace dpmd {
 namespace {
-
-struct DiskArgs {
-    double* x; double* y; double* vx; double* vy; double* fx; double* fy; double* pe;
-    const double* rad; const double* mass;
-    const int* sid; const int* off;
-    const double* Lx; const double* Ly; const double* iLx; const double* iLy; const double* e;
-    const double* thresh2; const int* boundary;
-    double* lx; double* ly; unsigned* flag; int* any;
-    const int* nn; const int* nbr;
-    int n; int cell;
-};
-
-DiskArgs disk_args(dpmd_ctx* c) {
-    DiskArgs a;
-    a.x = c->x.p; a.y = c->y.p; a.vx = c->vx.p; a.vy = c->vy.p; a.fx = c->fx.p; a.fy = c->fy.p; a.pe = c->pe.p;
-    a.rad = c->rad.p; a.mass = c->mass.p; a.sid = c->sid.p; a.off = c->off.p;
-    a.Lx = c->Lx.p; a.Ly = c->Ly.p; a.iLx = c->iLx.p; a.iLy = c->iLy.p; a.e = c->e.p;
-    a.thresh2 = c->thresh2.p; a.boundary = c->boundary.p;
-    a.lx = c->lx.p; a.ly = c->ly.p; a.flag = c->flag.p; a.any = c->run.p + RUN_ANY;
-    a.nn = c->nn.p; a.nbr = c->nbr.p; a.n = c->N; a.cell = c->method == DPMD_NEIGHBOR_CELL;
-    return a;
-}
-
-// compute_pair_forces_kernel (disk.cu:103-160) and compute_wall_forces_kernel
-// (disk.cu:163-246).  In walled boxes the pair displacement is xi - xj (repulsive,
-// consistent with the 0.5 k delta^2 energy); the reference writes xj - xi there (:218).
-template <bool WALL>
-__device__ __forceinline__ void disk_force_at(const DiskArgs& a, int i, int s, double& fxo, double& fyo, double& peo) {
-    const double ei = a.e[s];
-    const double xi = a.x[i], yi = a.y[i], ri = a.rad[i];
-    double fxi = 0.0, fyi = 0.0, pei = 0.0;
-    double Lx = a.Lx[s], Ly = a.Ly[s];
-    if (WALL) {
-        if (xi < ri) { const double d = ri - xi; fxi += ei * d; pei += (0.5 * ei * d * d); }
-        if (xi > Lx - ri) { const double d = ri - (Lx - xi); fxi -= ei * d; pei += (0.5 * ei * d * d); }
-        if (yi < ri) { const double d = ri - yi; fyi += ei * d; pei += (0.5 * ei * d * d); }
-        if (yi > Ly - ri) { const double d = ri - (Ly - yi); fyi -= ei * d; pei += (0.5 * ei * d * d); }
-    }
-    const double iLx = a.iLx[s], iLy = a.iLy[s];
-    const int cnt = a.nn[i];
-    const int* __restrict__ col = a.nbr + i;
-    // Neighbors in groups of four: every id and coordinate of a group is requested before
-    // any is used (four independent gather chains in flight per lane); the contributions
-    // are then added in list order, so the sums are those of the one-at-a-time loop.
-    constexpr int G = 4;
-    for (int k0 = 0; k0 < cnt; k0 += G) {
-        int jj[G];
-        double xj[G], yj[G], rj[G];
-#pragma unroll
-        for (int u = 0; u < G; ++u) jj[u] = (k0 + u < cnt) ? col[(size_t)(k0 + u) * a.n] : i;
-#pragma unroll
-        for (int u = 0; u < G; ++u) { xj[u] = a.x[jj[u]]; yj[u] = a.y[jj[u]]; rj[u] = a.rad[jj[u]]; }
-#pragma unroll
-        for (int u = 0; u < G; ++u) {
-            if (k0 + u >= cnt) break;
-            double dx = xi - xj[u], dy = yi - yj[u];
-            if (!WALL) { dx = min_image(dx, Lx, iLx); dy = min_image(dy, Ly, iLy); }
-            const double r2 = dx * dx + dy * dy;
-            const double radsum = ri + rj[u];
-            if (r2 >= radsum * radsum) continue;
-            const double r = sqrt(r2);
-            const double inv = 1.0 / r;
-            const double nx = dx * inv, ny = dy * inv;
-            const double delta = radsum - r;
-            const double fmag = ei * delta;
-            fxi += fmag * nx;
-            fyi += fmag * ny;
-            pei += (0.5 * ei * delta * delta) * 0.5;
-        }
-    }
-    fxo = fxi; fyo = fyi; peo = pei;
-}
-
-template <int MODE>
-__device__ __forceinline__ void disk_force_mode(const DiskArgs& a, int i, int s, double& fx, double& fy, double& pe) {
-    if (MODE == DPMD_FORCE_PBC) disk_force_at<false>(a, i, s, fx, fy, pe);
-    else if (MODE == DPMD_FORCE_WALL) disk_force_at<true>(a, i, s, fx, fy, pe);
-    else {
-        if (a.boundary[s]) disk_force_at<true>(a, i, s, fx, fy, pe);
-        else disk_force_at<false>(a, i, s, fx, fy, pe);
-    }
-}
 
 template <int MODE>
 __global__ void __launch_bounds__(DPMD_BLOCK) k_force(DiskArgs a) {
@@ -146,17 +65,6 @@ __global__ void k_scale_positions(DiskArgs a, const double* __restrict__ scale) 
     a.x[i] *= sc; a.y[i] *= sc;
 }
 
-// mix_velocities_and_forces_kernel (disk.cu:307-335): per-particle |v_i|/|F_i| mixing.
-__device__ __forceinline__ void disk_mix_one(double& vx, double& vy, double fx, double fy, double a) {
-    const double fn = sqrt(fx * fx + fy * fy);
-    const double vn = sqrt(vx * vx + vy * vy);
-    double ratio = 0.0;
-    if (fn > 1e-16) ratio = vn / fn * a;
-    else { vx = 0.0; vy = 0.0; ratio = 0.0; }
-    vx = vx * (1 - a) + fx * ratio;
-    vy = vy * (1 - a) + fy * ratio;
-}
-
 __global__ void k_mix(DiskArgs a, const double* __restrict__ alpha) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -208,7 +116,6 @@ __global__ void k_stress(DiskArgs a, double* sxx, double* sxy, double* syx, doub
 // k_fire_drift commits s_* into the live scalars, so the live dt/alpha/N_good/N_bad are
 // always what the reference holds after the same number of steps.
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ bool fire_drift_one(const DiskArgs& a, const FireArgs& f, int i, int s);
 
 template <int B>
 __global__ void __launch_bounds__(B) k_fire_drift(DiskArgs a, FireArgs f, Gate g) {
@@ -219,42 +126,6 @@ __global__ void __launch_bounds__(B) k_fire_drift(DiskArgs a, FireArgs f, Gate g
     bool flag = false;
     if (valid && !fire_frozen(f, s)) flag = fire_drift_one(a, f, i, s);
     if (a.cell) raise_flags(a.flag, a.any, s, flag);
-}
-
-__device__ __forceinline__ bool fire_drift_one(const DiskArgs& a, const FireArgs& f, int i, int s) {
-    const double dt = f.s_dt[s], alpha = f.s_alpha[s], dtrev = f.s_dtrev[s], vscale = f.s_vscale[s];
-    const double m = a.mass[i];
-    const double fx = a.fx[i], fy = a.fy[i];
-    double x = a.x[i], y = a.y[i], vx = a.vx[i], vy = a.vy[i];
-    // update_positions(dt_reverse, 1.0)
-    const double s1 = dtrev * 1.0;
-    x += vx * s1; y += vy * s1;
-    bool flag = false;
-    if (a.cell) {
-        const double dx = x - a.lx[i], dy = y - a.ly[i];
-        flag = (dx * dx + dy * dy) > a.thresh2[s];
-    }
-    // scale_velocities(velocity_scale)
-    vx *= vscale; vy *= vscale;
-    // update_velocities(dt, 0.5)
-    const double smi = dt * 0.5 / m;
-    vx = vx + fx * smi; vy = vy + fy * smi;
-    // mix_velocities_and_forces(alpha)
-    disk_mix_one(vx, vy, fx, fy, alpha);
-    // scale_velocities(velocity_scale)
-    vx *= vscale; vy *= vscale;
-    // update_positions(dt, 0.5)
-    const double s2 = dt * 0.5;
-    x += vx * s2; y += vy * s2;
-    if (a.cell) {
-        const double dx = x - a.lx[i], dy = y - a.ly[i];
-        flag = flag || ((dx * dx + dy * dy) > a.thresh2[s]);
-    }
-    a.x[i] = x; a.y[i] = y; a.vx[i] = vx; a.vy[i] = vy;
-    if (i == a.off[s]) {
-        f.dt[s] = dt; f.alpha[s] = alpha; f.ngood[s] = f.s_ngood[s]; f.nbad[s] = f.s_nbad[s];
-    }
-    return flag;
 }
 
 template <int MODE>

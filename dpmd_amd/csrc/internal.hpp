@@ -187,6 +187,12 @@ struct dpmd_ctx {
     // steps captured into one hipGraph batch (a power of two, 1 .. 128)
     int block = DPMD_BLOCK;
     int max_batch = 128;
+    // persistent per-system FIRE (disk batches of systems up to kPersistMaxN particles):
+    // the switch, its workgroup size, and the per-run work lists
+    int persist_fire = 1;        // 0 off, 1 with hand-off to per-step launches, 2 persistent only
+    int persist_block = 512;
+    int persist_phase = 2000;    // steps per persistent phase
+    dpmd::DVec<int> persist_order, persist_todo, persist_stop;
     // kernel timing (force kernels; outside graph capture only)
     int prof = 0;   // 0 off, 1 force kernels, 2 fused velocity-Verlet kernels
     bool capturing = false;
@@ -236,6 +242,13 @@ std::vector<T> dl(dpmd_ctx* c, const dpmd::DVec<T>& v) {
     }
     return h;
 }
+
+// fire_persist.hip: device-driven FIRE with one workgroup per system (same results as the
+// lean batches of lean_run, for disk batches whose systems all have <= kPersistMaxN particles)
+bool fire_persist_ok(dpmd_ctx* c);
+int fire_persistent_run(dpmd_ctx* c, int max_steps, int pred, double a, double b, bool use_graph, int* conv);
+// abi.hip: device-driven runs as graph-captured batches of per-step launches
+int lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, double a, double b, bool use_graph, int* conv);
 
 // abi.hip: the autotuner's engine (tune.hip holds the search and the cache)
 void tune_get(dpmd_ctx* c, dpmd_tune_config* t);

@@ -278,10 +278,19 @@ int dpmd_stream_copy(double* dst, const double* src, int64_t n, int repeats, dou
  *   DPMD_OPT_BLOCK: workgroup size of the disk step kernels (velocity-Verlet drift, force and
  *     fused launches, FIRE drift): 64, 128, 256 (default) or 512.  Same results.
  *   DPMD_OPT_MAX_BATCH: the largest number of steps captured into one hipGraph batch of a
- *     device-driven run, a power of two from 1 to 128 (default 128).  Same results. */
+ *     device-driven run, a power of two from 1 to 128 (default 128).  Same results.
+ *   DPMD_OPT_PERSISTENT_FIRE: engine of device-driven FIRE runs of disk batches whose systems
+ *     all have at most 16384 particles.  1 (default): persistent phases, one workgroup per
+ *     system for DPMD_OPT_PERSIST_PHASE steps with no per-step launches, handing the systems
+ *     still running to per-step launches once those are the cheaper engine by a cost model
+ *     (few or large systems left); 2: persistent phases only; 0: graph-captured batches of
+ *     per-step launches only.  Same results.
+ *   DPMD_OPT_PERSIST_BLOCK: workgroup size of the persistent FIRE kernel: 256, 512 (default)
+ *     or 1024.  Same results.
+ *   DPMD_OPT_PERSIST_PHASE: steps per persistent phase (default 2000).  Same results. */
 enum dpmd_option {
     DPMD_OPT_COOP_REBUILD = 0, DPMD_OPT_COOP_IN_GRAPH = 1, DPMD_OPT_FUSED_VV = 2, DPMD_OPT_BLOCK = 3,
-    DPMD_OPT_MAX_BATCH = 4
+    DPMD_OPT_MAX_BATCH = 4, DPMD_OPT_PERSISTENT_FIRE = 5, DPMD_OPT_PERSIST_BLOCK = 6, DPMD_OPT_PERSIST_PHASE = 7
 };
 int dpmd_set_option(dpmd_ctx* ctx, int key, int value);
 int dpmd_get_option(dpmd_ctx* ctx, int key, int* value);
