@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_vv_fused launch on the default workload (N = 2^20, phi = 0.84), from the
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes in profiles/k_vv_fused_disk_N1M_fire_nve.md, each
 # scaled by what the counter reads for a copy kernel of known size (FETCH x 2.0, WRITE x 1.0).
-FUSED_TRAFFIC_N1M = (51687.0 * (131072.0 / 65545.5) + 57344.0) * 1024.0
+FUSED_TRAFFIC_N1M = (51309.2 * (131072.0 / 65545.5) + 32768.0) * 1024.0
 
 
 def log(msg: str) -> None:
@@ -42,9 +42,11 @@ def parse():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=40)
-    ap.add_argument("--workload", choices=("disk1m", "jam512", "adjoint"), default="disk1m",
+    ap.add_argument("--workload", choices=("disk1m", "jam512", "adjoint", "single1024", "rigid"), default="disk1m",
                     help="disk1m: one system of 2^20 disks, FIRE + NVE (default); jam512: the "
-                         "512-system uneven batch jammed by compression, batched vs one context per system")
+                         "512-system uneven batch jammed by compression, batched vs one context per system; "
+                         "adjoint: gradients through FIRE minima and NVE; single1024: the plain N=1024 "
+                         "system, FIRE to 1e-12 then 1e4 NVE steps; rigid: 256 x 256 rigid bumpy polygons")
     ap.add_argument("--plain-sample", type=int, default=16,
                     help="jam512: systems timed on the plain (one context per system) path")
     ap.add_argument("--no-graph", action="store_true",
@@ -203,6 +205,172 @@ def run_jam512(args):
     print(json.dumps(out))
 
 
+def run_single1024(args):
+    """N = 1024 bidisperse disks, phi = 0.85, PBC: FIRE until max|F_i| < 1e-12, then 1e4
+    velocity-Verlet steps (dt 1e-2) from Gaussian velocities at T = 1e-4 with zero net
+    momentum.  Timed on the default engine (one persistent workgroup for FIRE) and with
+    per-step launches; the oracle runs the same protocol on the host (bit-identical)."""
+    import numpy as np
+
+    from dpmd_amd import _lib as L
+    from dpmd_amd import workloads as W
+    from dpmd_amd.system import Disk, VelocityVerlet, run_fire
+
+    b = W.single_disk_system(1024, 0.85, seed=1)
+
+    def go(engine):
+        P = Disk()
+        W.load_into(P, b)
+        P.init_neighbors()
+        L.check(L.lib().dpmd_set_option(P.ctx, L.OPT_PERSISTENT_FIRE, engine), P.ctx)
+        P.synchronize()
+        t0 = time.perf_counter()
+        steps, conv = run_fire(P, 1e-2, 1000000, L.FIRE_PRED_MAX_FORCE, 1e-12, use_graph=not args.no_graph)
+        P.synchronize()
+        t1 = time.perf_counter()
+        P.compute_pe_total(); P.compute_pressure()
+        pe, pr = P.get("PE_TOTAL")[0], P.get("PRESSURE")[0]
+        P.rand_normal_2d("VEL_X", "VEL_Y", 1, 1)
+        P.set_average_velocity(0.0, 0.0)
+        P.set_temperature(1e-4)
+        vv = VelocityVerlet(P, 1e-2)
+        vv.init()
+        vv.run(10000, use_graph=not args.no_graph)
+        P.synchronize()
+        t2 = time.perf_counter()
+        P.compute_ke_total(); P.compute_pe_total()
+        return dict(fire_steps=steps, converged=bool(conv), fire_seconds=t1 - t0, nve_seconds=t2 - t1,
+                    pe=pe, pressure=pr, e_final=P.get("KE_TOTAL")[0] + P.get("PE_TOTAL")[0])
+
+    if args.warmup > 0:
+        go(1)
+    res = go(1)
+    plain = go(0)
+    total_steps = res["fire_steps"] + 10000
+    t = res["fire_seconds"] + res["nve_seconds"]
+    log(f"single1024: FIRE {res['fire_steps']} steps {res['fire_seconds']:.3f} s, NVE {res['nve_seconds']:.3f} s; "
+        f"per-step launches FIRE {plain['fire_seconds']:.3f} s")
+    out = {
+        "metric": "particle-steps/sec (FIRE to max|F|<1e-12 + 1e4 NVE steps, N=1024)",
+        "value": 1024 * total_steps / t, "unit": "particle-steps/s", "n_gpus": 1, "steps": total_steps,
+        "warmup": args.warmup, "ms_per_step": t * 1e3 / total_steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "plain single system: N=1024 bidisperse disks (0.5/0.7), phi=0.85, PBC, FIRE to "
+                               "max|F_i|<1e-12, then 1e4 NVE steps dt=1e-2 at T=1e-4",
+                   **{k: v for k, v in res.items()},
+                   "per_step_launches": {"fire_seconds": plain["fire_seconds"], "nve_seconds": plain["nve_seconds"],
+                                         "fire_steps": plain["fire_steps"]},
+                   "engines_bit_identical": bool(plain["pe"] == res["pe"] and plain["pressure"] == res["pressure"]
+                                                 and plain["fire_steps"] == res["fire_steps"])},
+        "roofline": None,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as orc
+        orc.lib().or_set_threads(1)
+        d = orc.OracleDisk(b)
+        t0 = time.perf_counter()
+        rsteps, _ = d.fire_minimize(1e-2, 1000000, pred=1, ftol=1e-12)
+        tf = time.perf_counter() - t0
+        pe_o, pr_o = d.pe_total()[0], d.pressure()[0]
+        out["config"]["oracle_fire_steps"] = int(rsteps)
+        out["config"]["oracle_pe_rel_diff"] = abs(pe_o - res["pe"]) / abs(pe_o)
+        out["config"]["oracle_pressure_rel_diff"] = abs(pr_o - res["pressure"]) / abs(pr_o)
+        out["cpu_baseline"] = {"value": 1024 * rsteps / tf, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle, the same FIRE minimisation ({rsteps} steps, {tf:.1f} s), 1 thread"}
+    print(json.dumps(out))
+
+
+def run_rigid(args):
+    """256 wall-box systems x 256 rigid bumpy polygons (n_v uniform in [12, 26], circle radii
+    0.5 / 0.7, bump radius half the vertex spacing), phi = 0.70: FIRE on centre positions and
+    angles (max|F| < 1e-9, cap 20000 steps), then 1e4 velocity-Verlet steps (dt 1e-2) at
+    T = 1e-4.  The roofline is taken for the vertex-force kernel over the last NVE steps."""
+    import numpy as np
+
+    from dpmd_amd import _lib as L
+    from dpmd_amd import workloads as W
+    from dpmd_amd.system import RigidBumpy, VelocityVerlet, run_fire
+
+    lib = L.lib()
+    b = W.rigid_batch(S=256, n_per=256, phi=0.70, seed=1)
+
+    def load():
+        P = RigidBumpy()
+        W.load_rigid_into(P, b)
+        P.init_neighbors()
+        P.synchronize()
+        return P
+
+    if args.warmup > 0:
+        Q = load()
+        run_fire(Q, 1e-2, args.warmup, L.FIRE_PRED_MAX_FORCE, 1e-9, mode=L.FORCE_WALL, use_graph=not args.no_graph)
+        del Q
+    P = load()
+    log(f"rigid: S={b.S} N={b.N} vertices={b.Nv}")
+    n_nve, n_prof = 10000, 500
+    t0 = time.perf_counter()
+    steps, conv = run_fire(P, 1e-2, 20000, L.FIRE_PRED_MAX_FORCE, 1e-9, mode=L.FORCE_WALL, use_graph=not args.no_graph)
+    P.synchronize()
+    t_fire = time.perf_counter() - t0
+    fire_work = int(P.get("FIRE_STEPS").astype(np.int64).sum()) * 256
+    P.rand_normal_2d("VEL_X", "VEL_Y", 1, 1)
+    P.set_average_velocity(0.0, 0.0)
+    P.set_temperature(1e-4)
+    vv = VelocityVerlet(P, 1e-2, wall=True)
+    vv.init()
+    t1 = time.perf_counter()
+    vv.run(n_nve - n_prof, use_graph=not args.no_graph)
+    lib.dpmd_profile(P.ctx, 1)                        # HIP events around every vertex-force launch
+    vv.run(n_prof, use_graph=False)
+    P.synchronize()
+    t_nve = time.perf_counter() - t1
+    tot_ms, cnt = C.c_double(), C.c_int()
+    L.check(lib.dpmd_profile_read(P.ctx, C.byref(tot_ms), C.byref(cnt)), P.ctx)
+    lib.dpmd_profile(P.ctx, 0)
+    cap, mx, nnb = P.neighbor_info()
+    k = nnb / b.Nv
+    # k_vertex_force per vertex: system id (4), x, y, r (24), count (4), k ids (4k) read;
+    # fx, fy, pe (24) written -> 56 + 4k bytes (neighbor gathers excluded)
+    bytes_per_launch = b.Nv * (56 + 4 * k)
+    avg_ms = tot_ms.value / max(1, cnt.value)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    rate_nve = b.N * n_nve / t_nve
+    log(f"rigid: FIRE {steps} steps {t_fire:.2f} s (converged {bool(conv)}), NVE {t_nve:.2f} s; "
+        f"k_vertex_force {avg_ms * 1e3:.1f} us")
+    out = {
+        "metric": "particle-steps/sec (rigid bumpy polygons, 256 x 256, FIRE + NVE)",
+        "value": (fire_work + b.N * n_nve) / (t_fire + t_nve), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": steps + n_nve, "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "256 wall-box systems x 256 rigid bumpy polygons (n_v 12..26, R 0.5/0.7), phi=0.70; "
+                               "FIRE (max|F|<1e-9, cap 20000) then 1e4 NVE steps dt=1e-2 at T=1e-4",
+                   "n_particles": b.N, "n_vertices": b.Nv, "fire_steps": steps, "fire_converged": bool(conv),
+                   "fire_seconds": t_fire, "fire_particle_steps": fire_work, "nve_seconds": t_nve,
+                   "nve_particle_steps_per_s": rate_nve, "nve_vertex_steps_per_s": b.Nv * n_nve / t_nve,
+                   "mean_vertex_neighbors": k},
+        "roofline": {"bound": "hbm", "kernel": "k_vertex_force (vertex contact forces, walls)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "avg_launch_us": avg_ms * 1e3, "launches": cnt.value,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as orc
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        orc.lib().or_set_threads(threads)
+        sub = W.rigid_batch(S=16, n_per=256, phi=0.70, seed=1)
+        d = orc.OracleRigid(sub)
+        t0 = time.perf_counter()
+        d.vv_init(1e-2, 1)
+        d.vv_run(100, 1)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": sub.N * 100 / tc, "unit": "particle-steps/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"oracle, 16 of the systems (4096 bodies), 100 NVE steps, {tc:.1f} s"}
+    print(json.dumps(out))
+
+
 def run_adjoint(args):
     """128 periodic systems x 1024 bidisperse disks at phi = 0.86.
     (a) dP_s/dr through FIRE minima (max|F| < 1e-12): implicit differentiation on each
@@ -340,6 +508,12 @@ def main():
     if args.workload == "adjoint":
         run_adjoint(args)
         return
+    if args.workload == "single1024":
+        run_single1024(args)
+        return
+    if args.workload == "rigid":
+        run_rigid(args)
+        return
     import numpy as np
 
     from dpmd_amd import _lib as L
@@ -393,9 +567,9 @@ def main():
     cap, mx, nnb = P.neighbor_info()
     k = nnb / N
     # k_vv_fused per particle: reads x,y,rad (24), sid (4), count (4), k ids (4k), vx,vy,mass (24),
-    # last-rebuild x,y (16); writes fx,fy,pe,vx,vy (40) and the new x,y (16)
-    # -> 128 + 4k bytes (neighbor coordinate gathers excluded).
-    bytes_per_launch = N * (128 + 4 * k)
+    # last-rebuild x,y (16); writes vx,vy (16) and the new x,y (16)
+    # -> 104 + 4k bytes (neighbor coordinate gathers excluded; forces stay in registers).
+    bytes_per_launch = N * (104 + 4 * k)
     avg_ms = tot_ms.value / max(1, cnt.value)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
 

@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(B) k_fire_drift(DiskArgs a, FireArgs f, Gate g
 template <int MODE>
 __global__ void __launch_bounds__(DPMD_BLOCK)
 k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, Gate g) {
-    __shared__ double lds[4];
+    __shared__ double lds[12];
     if (!gate_force(g)) return;
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     const int s = r.chunk_sys[c];
@@ -154,11 +154,16 @@ k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, Gate g) {
         vpe = pe;
         vf2 = fx * fx + fy * fy;
     }
-    vp = block_sum256(vp, lds);
-    vpe = block_sum256(vpe, lds);
-    vf2 = block_max256(vf2, lds);
+    // block_sum256 / block_max256 of the three values with one barrier: wavefront
+    // butterflies, then (w0 + w1) + (w2 + w3)
+    vp = wave_sum(vp); vpe = wave_sum(vpe); vf2 = wave_max(vf2);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { lds[w] = vp; lds[4 + w] = vpe; lds[8 + w] = vf2; }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        r.part[c] = vp; r.part[r.nch + c] = vpe; r.part[2 * r.nch + c] = vf2;
+        r.part[c] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+        r.part[r.nch + c] = (lds[4] + lds[5]) + (lds[6] + lds[7]);
+        r.part[2 * r.nch + c] = fmax(fmax(lds[8], lds[9]), fmax(lds[10], lds[11]));
     }
 }
 
@@ -173,11 +178,28 @@ k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_updat
     const int s = blockIdx.x;
     if (fire_frozen(f, s)) return;
     const int cb = r.sys_chunk[s], ce = r.sys_chunk[s + 1];
+    // thread 0 fetches the system's FIRE scalars now, overlapping the fold below
+    FireSys q{};
+    int was_stopped = 0;
+    if (threadIdx.x == 0) { q = fire_load(f, s); was_stopped = f.stopped[s]; }
     double ap = 0.0, ape = 0.0, af = 0.0;
-    for (int k = cb + threadIdx.x; k < ce; k += DPMD_BLOCK) {
-        ap = ap + r.part[k];
-        ape = ape + r.part[r.nch + k];
-        af = fmax(af, r.part[2 * r.nch + k]);
+    // each thread's chunks in sequence, sixteen loads issued ahead of their additions
+    constexpr int U = 16;
+    for (int k0 = cb + threadIdx.x; k0 < ce; k0 += U * DPMD_BLOCK) {
+        double q0[U], q1[U], q2[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * DPMD_BLOCK;
+            const int kk = k < ce ? k : k0;
+            q0[u] = r.part[kk]; q1[u] = r.part[r.nch + kk]; q2[u] = r.part[2 * r.nch + kk];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (k0 + u * DPMD_BLOCK >= ce) break;
+            ap = ap + q0[u];
+            ape = ape + q1[u];
+            af = fmax(af, q2[u]);
+        }
     }
     ap = block_sum256(ap, lds);
     ape = block_sum256(ape, lds);
@@ -185,11 +207,22 @@ k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_updat
     if (threadIdx.x == 0) {
         // A system that had stopped before this step has now had its velocities zeroed
         // (vscale = 0 after a stop), so every later step leaves it unchanged: freeze it.
-        const int was_stopped = f.stopped[s];
+        // The scalars were loaded together and are stored once.
         r.pw_tot[s] = ap; r.pe_tot[s] = ape; r.maxf2[s] = af;
-        if (do_conv) fire_convergence_sys(f, off, s, ape, af, pred, pa, pb);
-        if (do_conv) fire_count_stopped(f, s, g.k);
-        if (do_update) fire_update_sys(f, s, ap);
+        if (do_conv) {
+            fire_convergence_core(q, off[s + 1] - off[s], ape, af, pred, pa, pb);
+            f.dt[s] = q.dt;
+            f.peprev[s] = q.peprev;
+            if (q.dt == 0.0 && !was_stopped) {   // fire_count_stopped
+                f.stopped[s] = 1;
+                if (atomicSub(&f.run[RUN_RUNNING], 1) == 1) f.run[RUN_CONV_AT] = g.k;
+            }
+        }
+        if (do_update) {
+            fire_update_core(q, ap);
+            f.s_dt[s] = q.s_dt; f.s_alpha[s] = q.s_alpha; f.s_ngood[s] = q.s_ngood; f.s_nbad[s] = q.s_nbad;
+            f.s_dtrev[s] = q.s_dtrev; f.s_vscale[s] = q.s_vscale;
+        }
         if (f.frozen && was_stopped) f.frozen[s] = 1;
         if (do_conv) f.nsteps[s] += 1;
     }
@@ -297,9 +330,12 @@ k_vv_fused(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, const d
     const int s = valid ? a.sid[i] : -1;
     bool flag = false;
     if (valid) {
+        // The forces are used here only: every fused batch ends with k_vv_force, and a batch
+        // that a rebuild halts has its halted step's forces computed again, so fx, fy and pe
+        // are not stored.
         double fx, fy, pe;
         disk_force_mode<MODE>(a, i, s, fx, fy, pe);
-        a.fx[i] = fx; a.fy[i] = fy; a.pe[i] = pe;
+        (void)pe;
         const double smi = dth[s] * 1.0 / a.mass[i];
         double vx = a.vx[i] + fx * smi;          // end of this step
         double vy = a.vy[i] + fy * smi;

@@ -60,21 +60,34 @@ __device__ __forceinline__ void vertex_force_at(const RigidArgs& a, int i, int s
     }
     const double iLx = a.iLx[s], iLy = a.iLy[s];
     const int cnt = a.nn[i];
-    for (int k = 0; k < cnt; ++k) {
-        const int j = a.nbr[(size_t)k * a.nv + i];
-        double dx = xi - a.vpx[j], dy = yi - a.vpy[j];
-        if (!WALL) { dx = min_image(dx, Lx, iLx); dy = min_image(dy, Ly, iLy); }
-        const double r2 = dx * dx + dy * dy;
-        const double radsum = ri + a.vrad[j];
-        if (r2 >= radsum * radsum) continue;
-        const double r = sqrt(r2);
-        const double inv = 1.0 / r;
-        const double nx = dx * inv, ny = dy * inv;
-        const double delta = radsum - r;
-        const double fmag = ei * delta;
-        fxi += fmag * nx;
-        fyi += fmag * ny;
-        pei += (0.5 * ei * delta * delta) * 0.5;
+    const int* __restrict__ col = a.nbr + i;
+    // neighbors in groups of four, every load of a group issued before any is used; the
+    // contributions are added in list order
+    constexpr int G = 4;
+    for (int k0 = 0; k0 < cnt; k0 += G) {
+        int jj[G];
+        double xj[G], yj[G], rj[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) jj[u] = (k0 + u < cnt) ? col[(size_t)(k0 + u) * a.nv] : i;
+#pragma unroll
+        for (int u = 0; u < G; ++u) { xj[u] = a.vpx[jj[u]]; yj[u] = a.vpy[jj[u]]; rj[u] = a.vrad[jj[u]]; }
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (k0 + u >= cnt) break;
+            double dx = xi - xj[u], dy = yi - yj[u];
+            if (!WALL) { dx = min_image(dx, Lx, iLx); dy = min_image(dy, Ly, iLy); }
+            const double r2 = dx * dx + dy * dy;
+            const double radsum = ri + rj[u];
+            if (r2 >= radsum * radsum) continue;
+            const double r = sqrt(r2);
+            const double inv = 1.0 / r;
+            const double nx = dx * inv, ny = dy * inv;
+            const double delta = radsum - r;
+            const double fmag = ei * delta;
+            fxi += fmag * nx;
+            fyi += fmag * ny;
+            pei += (0.5 * ei * delta * delta) * 0.5;
+        }
     }
     a.vfx[i] = fxi; a.vfy[i] = fyi; a.vpe[i] = pei;
 }
@@ -97,14 +110,29 @@ __device__ __forceinline__ void particle_force_one(const RigidArgs& a, int i, do
                                                    double& peo) {
     const double px = a.x[i], py = a.y[i];
     double fxi = 0.0, fyi = 0.0, t = 0.0, p = 0.0;
-    for (int j = a.poff[i]; j < a.poff[i + 1]; ++j) {
-        const int v = a.vstatic[j];
-        const double fvx = a.vfx[v], fvy = a.vfy[v];
-        fxi += fvx;
-        fyi += fvy;
-        p += a.vpe[v];
-        const double dx = a.vpx[v] - px, dy = a.vpy[v] - py;
-        t += fvy * dx - fvx * dy;
+    const int jb = a.poff[i], je = a.poff[i + 1];
+    // vertices in groups of four: the loads of a group are issued before the sums, which
+    // run in canonical vertex order
+    constexpr int G = 4;
+    for (int j0 = jb; j0 < je; j0 += G) {
+        int v[G];
+        double fvx[G], fvy[G], pv[G], vx[G], vy[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) v[u] = a.vstatic[j0 + u < je ? j0 + u : j0];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            fvx[u] = a.vfx[v[u]]; fvy[u] = a.vfy[v[u]]; pv[u] = a.vpe[v[u]];
+            vx[u] = a.vpx[v[u]]; vy[u] = a.vpy[v[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (j0 + u >= je) break;
+            fxi += fvx[u];
+            fyi += fvy[u];
+            p += pv[u];
+            const double dx = vx[u] - px, dy = vy[u] - py;
+            t += fvy[u] * dx - fvx[u] * dy;
+        }
     }
     fxo = fxi; fyo = fyi; to = t; peo = p;
 }
@@ -295,7 +323,7 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_fire_drift(RigidArgs a, FireArgs
 
 __global__ void __launch_bounds__(DPMD_BLOCK)
 k_fire_particle(RigidArgs a, FireArgs f, RedArgs r, int kick, Gate g) {
-    __shared__ double lds[4];
+    __shared__ double lds[12];
     if (!gate_open(g)) return;
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     const int s = r.chunk_sys[c];
@@ -317,10 +345,16 @@ k_fire_particle(RigidArgs a, FireArgs f, RedArgs r, int kick, Gate g) {
         vpe = p;
         vf2 = fx * fx + fy * fy;
     }
-    vp = block_sum256(vp, lds);
-    vpe = block_sum256(vpe, lds);
-    vf2 = block_max256(vf2, lds);
-    if (threadIdx.x == 0) { r.part[c] = vp; r.part[r.nch + c] = vpe; r.part[2 * r.nch + c] = vf2; }
+    // block_sum256 / block_max256 of the three values with one barrier
+    vp = wave_sum(vp); vpe = wave_sum(vpe); vf2 = wave_max(vf2);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { lds[w] = vp; lds[4 + w] = vpe; lds[8 + w] = vf2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        r.part[c] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+        r.part[r.nch + c] = (lds[4] + lds[5]) + (lds[6] + lds[7]);
+        r.part[2 * r.nch + c] = fmax(fmax(lds[8], lds[9]), fmax(lds[10], lds[11]));
+    }
 }
 
 // ---- fused velocity Verlet (velocity_verlet.cuh:34-41) -----------------------------------
