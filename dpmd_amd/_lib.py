@@ -107,6 +107,10 @@ PROTOS = {
     "dpmd_vv_init": (i32, [vp, vp, i32]),
     "dpmd_vv_step": (i32, [vp, i32]),
     "dpmd_vv_run": (i32, [vp, i32, i32]),
+    "dpmd_damped_vv_init": (i32, [vp, vp, vp, i32]),
+    "dpmd_compute_damping_forces": (i32, [vp, vp]),
+    "dpmd_nvt_rescale_run": (i32, [vp, i32, i32, vp, i32, vp, i32, i32, i32]),
+    "dpmd_jam_binary_search": (i32, [vp, vp, i32, i32, f64, f64, f64, f64, i32, i32, vp, vp, pi32]),
     "dpmd_fire_init": (i32, [vp, vp, i32]),
     "dpmd_fire_step": (i32, [vp]),
     "dpmd_fire_converged": (i32, [vp, f64, f64, pi32]),

@@ -531,6 +531,42 @@ void vv_run_impl(dpmd_ctx* c, int n, bool use_graph) {
     check_overflow(c);
 }
 
+void set_temperature_impl(dpmd_ctx* c, const double* T, bool remove_avg) {   // base_particle.hpp:449-462
+    per_system_temperature(c);
+    temperature_scale(c, T, c->scr_c.p);
+    if (remove_avg) set_average_velocity_impl(c, nullptr, nullptr);
+    scale_velocities_impl(c, c->scr_c.p);
+}
+
+// Rescale-thermostat NVT with affine compression.  Global step k = step0 + i: one
+// velocity-Verlet step, then increment_packing_fraction(inc) if compress_every > 0 and
+// k % compress_every == 0, then set_temperature(T) if temperature_every > 0 and
+// k % temperature_every == 0.  The steps between two such events run as one device-driven
+// batch run.  (The compression touches only positions, box and lists and the thermostat only
+// velocities, so the two commute exactly.)
+void nvt_rescale_impl(dpmd_ctx* c, int n, int step0, const double* inc, int cf, const double* T, int tf, bool rm,
+                      bool use_graph) {
+    need_state(c->vv_ready, "nvt_rescale_run: call vv_init first");
+    need(n >= 0 && step0 >= 0 && cf >= 0 && tf >= 0, "nvt_rescale_run: negative step counts");
+    need(cf == 0 || inc != nullptr, "nvt_rescale_run: compress_every > 0 needs phi_increment");
+    need(tf == 0 || T != nullptr, "nvt_rescale_run: temperature_every > 0 needs temperature_target");
+    auto compress_at = [&](int k) { return cf > 0 && k % cf == 0; };
+    auto thermo_at = [&](int k) { return tf > 0 && k % tf == 0; };
+    int i = 0;
+    while (i < n) {
+        int j = i;
+        while (j < n && !compress_at(step0 + j) && !thermo_at(step0 + j)) ++j;
+        const int len = (j < n ? j + 1 : n) - i;
+        vv_run_impl(c, len, use_graph);
+        i += len;
+        if (j < n) {
+            const int k = step0 + j;
+            if (compress_at(k)) increment_packing_fraction_impl(c, inc);
+            if (thermo_at(k)) set_temperature_impl(c, T, rm);
+        }
+    }
+}
+
 void snapshot(dpmd_ctx* c, double* dst) {   // canonical x, y, vx, vy
     const int N = c->N;
     adj_to_canonical(c, c->x.p, dst);
@@ -563,6 +599,7 @@ struct RecordTo {
 void vv_checkpointed_impl(dpmd_ctx* c, int n, int every, bool use_graph) {
     need_disk_cells(c);
     need_state(c->vv_ready, "vv_run_checkpointed: call vv_init first");
+    need(!c->vv_damped, "vv_run_checkpointed: the reverse mode covers undamped velocity Verlet");
     need(n >= 0 && every > 0, "vv_run_checkpointed: n_steps >= 0 and every > 0 required");
     const int N = c->N;
     const int J = (n + every - 1) / every;
@@ -586,6 +623,7 @@ void vv_checkpointed_impl(dpmd_ctx* c, int n, int every, bool use_graph) {
 void vv_backprop_impl(dpmd_ctx* c, const double* gx, const double* gy, const double* gvx, const double* gvy, double* ox,
                       double* oy, double* ovx, double* ovy, bool use_graph) {
     need_disk_cells(c);
+    need(!c->vv_damped, "vv_backprop: the reverse mode covers undamped velocity Verlet");
     need_state(c->ck_count > 0, "vv_backprop: run vv_run_checkpointed first");
     const int N = c->N, S = c->S, n = c->ck_steps, every = c->ck_every, J = c->ck_count - 1;
     const int mode = c->vv_mode;
@@ -759,7 +797,143 @@ double newton_impl(dpmd_ctx* c, int mode, int max_newton, double ftol, double cg
     return f;
 }
 
-void vv_init_impl(dpmd_ctx* c, const double* dt, int mode) {
+// save_state / restore_state (base_particle.hpp:708-736; disk.cu:348-432,1158-1225)
+void save_state_impl(dpmd_ctx* c, const int* flag, int tv) {
+    const int N = c->N, S = c->S;
+    if (!c->saved) {
+        dpmd::DVec<double>* dv[] = {&c->sv_x, &c->sv_y, &c->sv_rad, &c->sv_mass, &c->sv_angle};
+        for (auto* v : dv) { v->resize(N); v->zero(c->st); }
+        c->sv_static.resize(N); c->sv_static.zero(c->st);
+        c->sv_Lx.resize(S); c->sv_Ly.resize(S);
+        c->saved = true;
+    }
+    save_restore(c, flag, tv, false);
+}
+void restore_state_impl(dpmd_ctx* c, const int* flag, int tv) {
+    need_state(c->saved, "restore_state: nothing has been saved");
+    need_state(!rigid(c), "restore_state: rigid bodies restore is not supported");
+    save_restore(c, flag, tv, true);
+    box_inv(c);
+    check_neighbors_impl(c, true);
+}
+
+// minimize_fire[_wall]_copy_dt (minimizers.cuh:34-58,85-121): FIRE until every system stopped
+// or max_steps, stopping test converged(avg_pe_target, avg_pe_diff_target) as the reference
+// passes it (the arguments land swapped in converged(diff, target), fire.cuh:98); returns the
+// final dt per system and zeroes the velocities.
+std::vector<double> minimize_copy_dt(dpmd_ctx* c, const std::vector<double>& dt, int max_steps, double pe_target,
+                                     double pe_diff_target, int mode, bool use_graph) {
+    const int S = c->S;
+    c->scr_a.upload(dt.data(), S, c->st);
+    fire_init_impl(c, c->scr_a.p, mode);
+    if (max_steps > 0) {
+        FrozenMask mask(c);
+        (void)lean_run(c, max_steps, true, DPMD_FIRE_PRED_REFERENCE, pe_target, pe_diff_target, use_graph, nullptr);
+    }
+    check_overflow(c);
+    std::vector<double> out = dl(c, c->f_dt);
+    zero_scale(c);
+    scale_velocities_impl(c, c->scr_c.p);
+    return out;
+}
+
+// jam_binary_search_pbc / _wall (jammers.cuh:60-256; kernels jammers.cu:7-92).  The per-system
+// scalar kernels (remove_badly_initialized_systems, reinit_dt, jamming_update, scale_box) act
+// on S values and run on the host here, in the same arithmetic; every particle operation runs
+// on the device.  As in the reference, the box is changed and the lists rebuilt before the
+// positions are scaled (the next minimisation's skin test catches the displacement).
+// action: -1 unjammed (state saved), 1 jammed (reverted), 0 done; phi: the bisection's phi.
+void jam_bisect_impl(dpmd_ctx* c, const double* dt, int max_comp, int max_min, double pe_t, double pe_diff_t,
+                     double dphi, double phi_tol, int mode, bool use_graph, int* action_out, double* phi_out,
+                     int* steps_out) {
+    need(!rigid(c), "jam_binary_search: disk systems (rigid-body restore_state is not supported)");
+    need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
+    need(dt != nullptr && max_comp >= 0 && max_min >= 0, "jam_binary_search: bad arguments");
+    const int S = c->S;
+    std::vector<double> dt_copy(S);
+    HIP_OK(hipMemcpyAsync(dt_copy.data(), dt, S * 8, hipMemcpyDeviceToHost, c->st));
+    HIP_OK(hipStreamSynchronize(c->st));
+    std::vector<int> action(S, -1);
+    dpmd::DVec<int> d_action;
+    d_action.resize(S);
+    auto push_action = [&] { d_action.upload(action.data(), S, c->st); };
+    if (mode != DPMD_FORCE_WALL) {   // jam_binary_search_pbc zeroes the velocities first (jammers.cuh:85)
+        zero_scale(c);
+        scale_velocities_impl(c, c->scr_c.p);
+    }
+    std::vector<double> dtf = minimize_copy_dt(c, dt_copy, max_min, pe_t, pe_diff_t, mode, use_graph);
+    for (int s = 0; s < S; ++s) if (dtf[s] != 0.0) action[s] = 0;   // remove_badly_initialized_systems_kernel
+    push_action();
+    save_state_impl(c, d_action.p, -1);
+    sys_reduce(c, RED_PHI, c->phi.p);
+    std::vector<double> phi = dl(c, c->phi), phi_low = phi, phi_high(S, -1.0);
+    std::vector<double> dtw(S), sf(S), area(S);
+    int step = 0;
+    while (step < max_comp) {
+        for (int s = 0; s < S; ++s) dtw[s] = action[s] != 0 ? dt_copy[s] : 0.0;   // reinit_dt_kernel
+        (void)minimize_copy_dt(c, dtw, max_min, pe_t, pe_diff_t, mode, use_graph);
+        sys_reduce(c, RED_PE, c->pe_tot.p);
+        std::vector<double> pe = dl(c, c->pe_tot);
+        for (int s = 0; s < S; ++s) {   // jamming_update_kernel
+            if (action[s] == 0) continue;
+            action[s] = -2;
+            const int dof = c->h_off[s + 1] - c->h_off[s];
+            const double avg_pe = pe[s] / dof;
+            if (avg_pe > pe_t) {
+                phi_high[s] = phi[s];
+                phi[s] = (phi_high[s] + phi_low[s]) / 2.0;
+                action[s] = 1;
+            } else {
+                action[s] = -1;
+                phi_low[s] = phi[s];
+                if (phi_high[s] > 0) phi[s] = (phi_high[s] + phi_low[s]) / 2.0;
+                else phi[s] += dphi;
+            }
+            if (std::abs(phi_high[s] / phi_low[s] - 1) < phi_tol && phi_high[s] > 0) action[s] = 0;
+        }
+        push_action();
+        save_state_impl(c, d_action.p, -1);
+        restore_state_impl(c, d_action.p, 1);
+        sys_reduce_arr(c, c->area.p, c->scr_a.p);   // compute_particle_area_total
+        std::vector<double> A(S);
+        HIP_OK(hipMemcpyAsync(A.data(), c->scr_a.p, S * 8, hipMemcpyDeviceToHost, c->st));
+        std::vector<double> Lx = dl(c, c->Lx), Ly = dl(c, c->Ly);
+        for (int s = 0; s < S; ++s) {   // scale_box_kernel
+            if (action[s] == 0 || phi[s] <= 0.0) { sf[s] = 1.0; continue; }
+            const double L = sqrt(A[s] / phi[s]);
+            sf[s] = L / Lx[s];
+            Lx[s] = L; Ly[s] = L;
+        }
+        c->Lx.upload(Lx.data(), S, c->st);
+        c->Ly.upload(Ly.data(), S, c->st);
+        box_inv(c);
+        if (c->method == DPMD_NEIGHBOR_CELL) {
+            setup_cells(c, CELLS_BOX_CHANGED);
+            alloc_obj_buffers(c);
+        }
+        check_neighbors_impl(c, true);
+        sys_reduce(c, RED_PHI, c->phi.p);
+        c->scr_b.upload(sf.data(), S, c->st);
+        scale_positions_impl(c, c->scr_b.p);
+        bool done = true;
+        for (int s = 0; s < S; ++s) done = done && action[s] == 0;
+        if (done) break;
+        ++step;
+    }
+    compute_forces_impl(c, mode);
+    sys_reduce(c, RED_PE, c->pe_tot.p);
+    sys_reduce(c, RED_PHI, c->phi.p);
+    HIP_OK(hipStreamSynchronize(c->st));
+    for (int s = 0; s < S; ++s) {
+        if (action_out) action_out[s] = action[s];
+        if (phi_out) phi_out[s] = phi[s];
+    }
+    if (steps_out) *steps_out = step;
+}
+
+// damping == nullptr: velocity Verlet (velocity_verlet.cuh:16-32); otherwise damped velocity
+// Verlet (damped_velocity_verlet.cuh:16-56), whose init computes the forces without the drag.
+void vv_init_impl(dpmd_ctx* c, const double* dt, int mode, const double* damping = nullptr) {
     const int S = c->S;
     std::vector<double> h(S), hh(S);
     HIP_OK(hipMemcpyAsync(h.data(), dt, S * 8, hipMemcpyDeviceToHost, c->st));
@@ -767,6 +941,11 @@ void vv_init_impl(dpmd_ctx* c, const double* dt, int mode) {
     for (int s = 0; s < S; ++s) hh[s] = h[s] * 0.5;   // dt_half = 0.5 * dt (velocity_verlet.cuh:28-29)
     c->vv_dt.upload(h.data(), S, c->st);
     c->vv_dth.upload(hh.data(), S, c->st);
+    c->vv_damped = damping != nullptr;
+    if (damping) {
+        c->vv_damp.resize(S);
+        HIP_OK(hipMemcpyAsync(c->vv_damp.p, damping, S * 8, hipMemcpyDeviceToDevice, c->st));
+    }
     c->vv_mode = mode;
     compute_forces_impl(c, mode);
     c->vv_ready = true;
@@ -1161,37 +1340,16 @@ int dpmd_set_average_velocity(dpmd_ctx* c, const double* ax, const double* ay) {
     return guard(c, [&] { set_average_velocity_impl(c, ax, ay); });
 }
 int dpmd_set_temperature(dpmd_ctx* c, const double* T, int rm) {
-    return guard(c, [&] {
-        per_system_temperature(c);
-        temperature_scale(c, T, c->scr_c.p);
-        if (rm) set_average_velocity_impl(c, nullptr, nullptr);
-        scale_velocities_impl(c, c->scr_c.p);
-    });
+    return guard(c, [&] { set_temperature_impl(c, T, rm != 0); });
 }
 int dpmd_increment_packing_fraction(dpmd_ctx* c, const double* inc) {
     return guard(c, [&] { increment_packing_fraction_impl(c, inc); });
 }
 int dpmd_save_state(dpmd_ctx* c, const int* flag, int tv) {
-    return guard(c, [&] {
-        const int N = c->N, S = c->S;
-        if (!c->saved) {
-            dpmd::DVec<double>* dv[] = {&c->sv_x, &c->sv_y, &c->sv_rad, &c->sv_mass, &c->sv_angle};
-            for (auto* v : dv) { v->resize(N); v->zero(c->st); }
-            c->sv_static.resize(N); c->sv_static.zero(c->st);
-            c->sv_Lx.resize(S); c->sv_Ly.resize(S);
-            c->saved = true;
-        }
-        save_restore(c, flag, tv, false);
-    });
+    return guard(c, [&] { save_state_impl(c, flag, tv); });
 }
 int dpmd_restore_state(dpmd_ctx* c, const int* flag, int tv) {
-    return guard(c, [&] {
-        need_state(c->saved, "restore_state: nothing has been saved");
-        need_state(!rigid(c), "restore_state: rigid bodies restore is not supported");
-        save_restore(c, flag, tv, true);
-        box_inv(c);
-        check_neighbors_impl(c, true);
-    });
+    return guard(c, [&] { restore_state_impl(c, flag, tv); });
 }
 
 int dpmd_rand_uniform_2d(dpmd_ctx* c, int fx, int fy, uint64_t seed, uint64_t call, double lox, double hix,
@@ -1220,6 +1378,30 @@ int dpmd_rand_normal_2d(dpmd_ctx* c, int fx, int fy, uint64_t seed, uint64_t cal
 
 int dpmd_vv_init(dpmd_ctx* c, const double* dt, int mode) {
     return guard(c, [&] { vv_init_impl(c, dt, mode); });
+}
+
+int dpmd_nvt_rescale_run(dpmd_ctx* c, int n_steps, int step0, const double* phi_increment, int compress_every,
+                         const double* temperature_target, int temperature_every, int remove_average_velocity,
+                         int use_graph) {
+    return guard(c, [&] {
+        nvt_rescale_impl(c, n_steps, step0, phi_increment, compress_every, temperature_target, temperature_every,
+                         remove_average_velocity != 0, use_graph != 0);
+        check_overflow(c);
+    });
+}
+
+int dpmd_damped_vv_init(dpmd_ctx* c, const double* dt, const double* damping, int mode) {
+    return guard(c, [&] {
+        need(dt != nullptr && damping != nullptr, "damped_vv_init: dt and damping are required");
+        vv_init_impl(c, dt, mode, damping);
+    });
+}
+
+int dpmd_compute_damping_forces(dpmd_ctx* c, const double* scale) {
+    return guard(c, [&] {
+        need(scale != nullptr, "compute_damping_forces: scale is required");
+        if (rigid(c)) rigid_damping(c, scale); else disk_damping(c, scale);
+    });
 }
 
 int dpmd_vv_step(dpmd_ctx* c, int n) {
@@ -1441,6 +1623,16 @@ int dpmd_jam_compress(dpmd_ctx* c, const double* dt, double dphi, double p_targe
             if (n_comp) n_comp[s] = ncomp[s];
         }
         if (work) *work = total;
+    });
+}
+
+int dpmd_jam_binary_search(dpmd_ctx* c, const double* dt, int max_compression_steps, int max_minimization_steps,
+                           double avg_pe_target, double avg_pe_diff_target, double phi_increment,
+                           double phi_tolerance, int mode, int use_graph, int* action, double* phi,
+                           int* compression_steps) {
+    return guard(c, [&] {
+        jam_bisect_impl(c, dt, max_compression_steps, max_minimization_steps, avg_pe_target, avg_pe_diff_target,
+                        phi_increment, phi_tolerance, mode, use_graph != 0, action, phi, compression_steps);
     });
 }
 

@@ -51,6 +51,15 @@ __global__ void k_update_velocities(DiskArgs a, const double* __restrict__ scale
     a.vy[i] = a.vy[i] + a.fy[i] * smi;
 }
 
+// compute_damping_forces_kernel (disk.cu:249-263): F_i += -v_i * scale[s]
+__global__ void k_damping(DiskArgs a, const double* __restrict__ scale) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double sc = scale[a.sid[i]];
+    a.fx[i] += -a.vx[i] * sc;
+    a.fy[i] += -a.vy[i] * sc;
+}
+
 __global__ void k_scale_velocities(DiskArgs a, const double* __restrict__ scale) {   // disk.cu:278-289
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -275,13 +284,18 @@ k_vv_drift(DiskArgs a, const double* __restrict__ dt, const double* __restrict__
 
 template <int MODE, int B>
 __global__ void __launch_bounds__(B)
-k_vv_force(DiskArgs a, const double* __restrict__ dth, Gate g) {
+k_vv_force(DiskArgs a, const double* __restrict__ dth, const double* __restrict__ damp, Gate g) {
     if (!gate_force(g)) return;
     const int i = xcd_remap(blockIdx.x, gridDim.x) * B + threadIdx.x;
     if (i >= a.n) return;
     const int s = a.sid[i];
     double fx, fy, pe;
     disk_force_mode<MODE>(a, i, s, fx, fy, pe);
+    if (damp) {   // compute_damping_forces_kernel (disk.cu:249-263) at the half-step velocity
+        const double d = damp[s];
+        fx += -a.vx[i] * d;
+        fy += -a.vy[i] * d;
+    }
     a.fx[i] = fx; a.fy[i] = fy; a.pe[i] = pe;
     const double smi = dth[s] * 1.0 / a.mass[i];
     a.vx[i] = a.vx[i] + fx * smi;
@@ -323,7 +337,7 @@ k_vv_drift_to(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, cons
 template <int MODE, int B>
 __global__ void __launch_bounds__(B)
 k_vv_fused(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, const double* __restrict__ dt,
-           const double* __restrict__ dth, Gate g) {
+           const double* __restrict__ dth, const double* __restrict__ damp, Gate g) {
     if (!gate_force(g)) return;
     const int i = xcd_remap(blockIdx.x, gridDim.x) * B + threadIdx.x;
     const bool valid = i < a.n;
@@ -336,9 +350,15 @@ k_vv_fused(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, const d
         double fx, fy, pe;
         disk_force_mode<MODE>(a, i, s, fx, fy, pe);
         (void)pe;
+        double vx = a.vx[i], vy = a.vy[i];
+        if (damp) {   // damped velocity Verlet: the drag at the half-step velocity joins the force
+            const double d = damp[s];
+            fx += -vx * d;
+            fy += -vy * d;
+        }
         const double smi = dth[s] * 1.0 / a.mass[i];
-        double vx = a.vx[i] + fx * smi;          // end of this step
-        double vy = a.vy[i] + fy * smi;
+        vx = vx + fx * smi;                      // end of this step
+        vy = vy + fy * smi;
         vx = vx + fx * smi;                      // next step: update_velocities(dt_half)
         vy = vy + fy * smi;
         const double sc = dt[s] * 1.0;           // update_positions(dt)
@@ -386,6 +406,10 @@ void disk_update_positions(dpmd_ctx* c, const double* scale, double scale2) {
 }
 void disk_update_velocities(dpmd_ctx* c, const double* scale, double scale2) {
     k_update_velocities<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), scale, scale2);
+    HIP_OK(hipGetLastError());
+}
+void disk_damping(dpmd_ctx* c, const double* scale) {
+    k_damping<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(disk_args(c), scale);
     HIP_OK(hipGetLastError());
 }
 void disk_scale_velocities(dpmd_ctx* c, const double* scale) {
@@ -458,7 +482,7 @@ void disk_vv_fused(dpmd_ctx* c, Gate g, const double* xs, const double* ys, doub
         by_block(c->block, [&](auto b) {
             constexpr int B = decltype(b)::value;
             k_vv_fused<decltype(m)::value, B><<<blocks_for(c->N, B), B, 0, c->st>>>(a, xo, yo, c->vv_dt.p,
-                                                                                     c->vv_dth.p, g);
+                                                                                     c->vv_dth.p, vv_damping(c), g);
         });
     });
     HIP_OK(hipGetLastError());
@@ -470,7 +494,8 @@ void disk_vv_force(dpmd_ctx* c, Gate g) {
     by_mode(c->vv_mode, [&](auto m) {
         by_block(c->block, [&](auto b) {
             constexpr int B = decltype(b)::value;
-            k_vv_force<decltype(m)::value, B><<<blocks_for(c->N, B), B, 0, c->st>>>(a, c->vv_dth.p, g);
+            k_vv_force<decltype(m)::value, B><<<blocks_for(c->N, B), B, 0, c->st>>>(a, c->vv_dth.p, vv_damping(c),
+                                                                                     g);
         });
     });
     HIP_OK(hipGetLastError());

@@ -148,6 +148,9 @@ struct dpmd_ctx {
     // ---- VV
     dpmd::DVec<double> vv_dt, vv_dth;
     int vv_mode = 0; bool vv_ready = false;
+    // damped velocity Verlet: per-system drag coefficients, applied when vv_damped is set
+    dpmd::DVec<double> vv_damp;
+    bool vv_damped = false;
     // ---- run control: [0] steps_done [1] max_steps [2] n_running [3] active
     dpmd::DVec<int> run, one;
     // ---- saved state (save_state/restore_state)
@@ -229,6 +232,8 @@ int guard(dpmd_ctx* c, F&& f) {
         return DPMD_ERR_HIP;
     }
 }
+// Drag coefficients of the velocity-Verlet force kernels: null for plain velocity Verlet.
+inline const double* vv_damping(const dpmd_ctx* c) { return c->vv_damped ? c->vv_damp.p : nullptr; }
 inline void need(bool ok, const std::string& msg) { if (!ok) throw ArgError(msg); }
 inline void need_state(bool ok, const std::string& msg) { if (!ok) throw StateError(msg); }
 
@@ -306,6 +311,7 @@ void disk_vv_drift(dpmd_ctx* c, Gate g);
 void disk_vv_drift_to(dpmd_ctx* c, Gate g, double* xo, double* yo, int any_word);
 void disk_vv_fused(dpmd_ctx* c, Gate g, const double* xs, const double* ys, double* xo, double* yo, int any_word);
 void disk_vv_force(dpmd_ctx* c, Gate g);
+void disk_damping(dpmd_ctx* c, const double* scale);
 void disk_ke(dpmd_ctx* c);
 void fire_recompute_update(dpmd_ctx* c);
 void rigid_compute_forces(dpmd_ctx* c, int mode);
@@ -320,6 +326,7 @@ void rigid_fire_drift(dpmd_ctx* c, Gate g);
 void rigid_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, double a, double b);
 void rigid_vv_drift(dpmd_ctx* c, Gate g);
 void rigid_vv_force(dpmd_ctx* c, Gate g);
+void rigid_damping(dpmd_ctx* c, const double* scale);
 
 // reduce.hip
 enum RedKind { RED_PE = 0, RED_KE, RED_POWER, RED_VX, RED_VY, RED_PHI, RED_SXX, RED_SXY, RED_SYX, RED_SYY, RED_MAXF2,

@@ -201,6 +201,16 @@ __global__ void k_update_velocities(RigidArgs a, const double* __restrict__ scal
     a.vx[i] = vx; a.vy[i] = vy; a.w[i] = w;
 }
 
+// compute_damping_forces_kernel (rigid_bumpy.cu:310-327): F += -v s, torque += -omega s
+__global__ void k_damping(RigidArgs a, const double* __restrict__ scale) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double sc = scale[a.sid[i]];
+    a.fx[i] += -a.vx[i] * sc;
+    a.fy[i] += -a.vy[i] * sc;
+    a.tq[i] += -a.w[i] * sc;
+}
+
 __global__ void k_scale_velocities(RigidArgs a, const double* __restrict__ scale) {   // :540-553
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -374,14 +384,21 @@ k_vv_drift(RigidArgs a, const double* __restrict__ dt, const double* __restrict_
     if (a.cell) raise_flags(a.flag, a.any, s, flag);
 }
 
-__global__ void __launch_bounds__(DPMD_BLOCK) k_vv_particle(RigidArgs a, const double* __restrict__ dth, Gate g) {
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_vv_particle(RigidArgs a, const double* __restrict__ dth, const double* __restrict__ damp, Gate g) {
     if (!gate_open(g)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     double fx, fy, t, p;
     particle_force_one(a, i, fx, fy, t, p);
-    a.fx[i] = fx; a.fy[i] = fy; a.tq[i] = t; a.pe[i] = p;
     double vx = a.vx[i], vy = a.vy[i], w = a.w[i];
+    if (damp) {   // compute_damping_forces_kernel (rigid_bumpy.cu:310-327) at the half-step velocity
+        const double sc = damp[a.sid[i]];
+        fx += -vx * sc;
+        fy += -vy * sc;
+        t += -w * sc;
+    }
+    a.fx[i] = fx; a.fy[i] = fy; a.tq[i] = t; a.pe[i] = p;
     kick_one(a, i, dth[a.sid[i]] * 1.0, fx, fy, t, vx, vy, w);
     a.vx[i] = vx; a.vy[i] = vy; a.w[i] = w;
 }
@@ -411,6 +428,10 @@ void rigid_update_positions(dpmd_ctx* c, const double* scale, double scale2) {
 }
 void rigid_update_velocities(dpmd_ctx* c, const double* scale, double scale2) {
     k_update_velocities<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), scale, scale2);
+    HIP_OK(hipGetLastError());
+}
+void rigid_damping(dpmd_ctx* c, const double* scale) {
+    k_damping<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), scale);
     HIP_OK(hipGetLastError());
 }
 void rigid_scale_velocities(dpmd_ctx* c, const double* scale) {
@@ -450,7 +471,7 @@ void rigid_vv_drift(dpmd_ctx* c, Gate g) {
 }
 void rigid_vv_force(dpmd_ctx* c, Gate g) {
     launch_vertex_force(c, c->vv_mode, g);
-    k_vv_particle<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vv_dth.p, g);
+    k_vv_particle<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vv_dth.p, vv_damping(c), g);
     HIP_OK(hipGetLastError());
 }
 

@@ -166,6 +166,9 @@ class _Particles:
     def update_velocities(self, scale, scale2: float = 1.0):
         self._c(lib().dpmd_update_velocities(self.ctx, self._ps(scale).ptr, scale2))
 
+    def compute_damping_forces(self, scale):
+        self._c(lib().dpmd_compute_damping_forces(self.ctx, self._ps(scale).ptr))
+
     def scale_velocities(self, scale): self._c(lib().dpmd_scale_velocities(self.ctx, self._ps(scale).ptr))
     def scale_positions(self, scale): self._c(lib().dpmd_scale_positions(self.ctx, self._ps(scale).ptr))
 
@@ -228,6 +231,23 @@ class VelocityVerlet:
         self.dt = _per_system(dt, P.n_systems())
 
     def init(self): self.P._c(lib().dpmd_vv_init(self.P.ctx, self.dt.ptr, self.mode))
+    def step(self, n: int = 1): self.P._c(lib().dpmd_vv_step(self.P.ctx, n))
+    def run(self, n: int, use_graph: bool = True): self.P._c(lib().dpmd_vv_run(self.P.ctx, n, int(use_graph)))
+
+
+class DampedVelocityVerlet:
+    """DampedVelocityVerlet / DampedVelocityVerletWall (include/integrators/damped_velocity_verlet.cuh:62-87):
+    velocity Verlet with compute_damping_forces(damping) after the forces of every step."""
+
+    def __init__(self, P: _Particles, dt, damping, wall: bool = False, mode: Optional[int] = None):
+        self.P = P
+        self.mode = mode if mode is not None else (L.FORCE_WALL if wall else L.FORCE_PBC)
+        self.dt = _per_system(dt, P.n_systems())
+        self.damping = _per_system(damping, P.n_systems())
+
+    def init(self):
+        self.P._c(lib().dpmd_damped_vv_init(self.P.ctx, self.dt.ptr, self.damping.ptr, self.mode))
+
     def step(self, n: int = 1): self.P._c(lib().dpmd_vv_step(self.P.ctx, n))
     def run(self, n: int, use_graph: bool = True): self.P._c(lib().dpmd_vv_run(self.P.ctx, n, int(use_graph)))
 
@@ -312,3 +332,50 @@ def jam_compress(P: _Particles, dt, dphi: float, p_target: float, max_compressio
                                  mode, int(use_graph), st.ctypes.data_as(C.POINTER(C.c_int)),
                                  nc.ctypes.data_as(C.POINTER(C.c_int)), C.byref(work)))
     return st, nc, work.value
+
+
+def jam_binary_search(P: _Particles, dt, max_compression_steps: int, max_minimization_steps: int,
+                      avg_pe_target: float, avg_pe_diff_target: float, phi_increment: float, phi_tolerance: float,
+                      wall: bool = False, mode: Optional[int] = None, use_graph: bool = True):
+    """jam_binary_search_pbc / _wall (include/routines/jammers.cuh:60-256) on the device
+    (dpmd_jam_binary_search).  Returns (action[S], phi[S], compression_rounds)."""
+    S = P.n_systems()
+    d = _per_system(dt, S)
+    m = mode if mode is not None else (L.FORCE_WALL if wall else L.FORCE_PBC)
+    act = np.zeros(S, np.int32)
+    phi = np.zeros(S)
+    rounds = C.c_int()
+    P._c(lib().dpmd_jam_binary_search(P.ctx, d.ptr, max_compression_steps, max_minimization_steps, avg_pe_target,
+                                      avg_pe_diff_target, phi_increment, phi_tolerance, m, int(use_graph),
+                                      act.ctypes.data, phi.ctypes.data, C.byref(rounds)))
+    return act, phi, rounds.value
+
+
+def nvt_rescale_run(P: _Particles, n_steps: int, phi_increment=None, compress_every: int = 0,
+                    temperature_target=None, temperature_every: int = 0, step0: int = 0,
+                    remove_average_velocity: bool = True, use_graph: bool = True):
+    """Rescale-thermostat NVT with affine compression (dpmd_nvt_rescale_run) over the
+    integrator set up by VelocityVerlet.init() or DampedVelocityVerlet.init(): for global step
+    k = step0 + i, one step, then increment_packing_fraction if k % compress_every == 0, then
+    set_temperature if k % temperature_every == 0 (tests/nvt_rescale_disk_pbc_compress.cu:41-68)."""
+    S = P.n_systems()
+    inc = _per_system(phi_increment, S) if phi_increment is not None else None
+    T = _per_system(temperature_target, S) if temperature_target is not None else None
+    P._c(lib().dpmd_nvt_rescale_run(P.ctx, n_steps, step0, inc.ptr if inc else None, compress_every,
+                                    T.ptr if T else None, temperature_every, int(remove_average_velocity),
+                                    int(use_graph)))
+
+
+def nvt_rescale_compress(P: _Particles, dt, n_steps: int, phi_increment: float, temperature_target: float,
+                         use_graph: bool = True):
+    """The reference's two-phase compression driver (tests/nvt_rescale_disk_pbc_compress.cu:14-68):
+    n_steps of velocity Verlet with phi_increment spread over compressions every
+    min(n_steps, 100) steps and set_temperature every min(n_steps, 10) steps, then n_steps more
+    with the thermostat only."""
+    cf = min(n_steps, 100)
+    tf = min(n_steps, 10)
+    phi_step = phi_increment / (n_steps // cf)
+    vv = VelocityVerlet(P, dt)
+    vv.init()
+    nvt_rescale_run(P, n_steps, phi_step, cf, temperature_target, tf, 0, True, use_graph)
+    nvt_rescale_run(P, n_steps, None, 0, temperature_target, tf, n_steps, True, use_graph)

@@ -153,6 +153,9 @@ int dpmd_increment_packing_fraction(dpmd_ctx* ctx, const double* increment);
 /* save_state / restore_state (base_particle.hpp:708-736) */
 int dpmd_save_state(dpmd_ctx* ctx, const int* flag, int true_val);
 int dpmd_restore_state(dpmd_ctx* ctx, const int* flag, int true_val);
+/* compute_damping_forces (base_particle.hpp:386; disk.cu:249-263, rigid_bumpy.cu:310-327):
+ * F_i += -v_i * scale[s] (rigid bodies also torque_i += -omega_i * scale[s]). */
+int dpmd_compute_damping_forces(dpmd_ctx* ctx, const double* scale);
 
 /* ---- stateless Philox4x32-10 (device_fields.cuh:219-298) ------------------------------ */
 int dpmd_rand_uniform_2d(dpmd_ctx* ctx, int field_x, int field_y, uint64_t seed, uint64_t call,
@@ -165,6 +168,26 @@ int dpmd_rand_normal_2d(dpmd_ctx* ctx, int field_x, int field_y, uint64_t seed, 
 /* VelocityVerlet[Wall](P, dt).init() / .step() (velocity_verlet.cuh:17-47). */
 int dpmd_vv_init(dpmd_ctx* ctx, const double* dt, int mode);
 int dpmd_vv_step(dpmd_ctx* ctx, int n_steps);
+/* DampedVelocityVerlet[Wall](P, dt, damping).init() (damped_velocity_verlet.cuh:16-56): dt and
+ * damping are device arrays of length S.  init computes the forces without the drag; each
+ * later step is the velocity-Verlet step with compute_damping_forces(damping) applied after
+ * the forces, at the half-step velocity, before the closing half kick.  Steps are taken with
+ * dpmd_vv_step / dpmd_vv_run; dpmd_vv_init switches back to plain velocity Verlet. */
+int dpmd_damped_vv_init(dpmd_ctx* ctx, const double* dt, const double* damping, int mode);
+/* Rescale-thermostat NVT with affine compression: the step loop of the reference's
+ * compression drivers (tests/nvt_rescale_disk_pbc_compress.cu:41-68: vv.step(), then
+ * increment_packing_fraction every compression_frequency steps and set_temperature every
+ * temperature_frequency steps; tests/damped_dynamics_compress_to_packing_fraction.cu:41-52,
+ * the same with set_temperature every step) as one routine over the integrator set up by
+ * dpmd_vv_init or dpmd_damped_vv_init.  For global step k = step0 + i, i in [0, n_steps):
+ * one step; then increment_packing_fraction(phi_increment) if compress_every > 0 and
+ * k % compress_every == 0; then set_temperature(temperature_target, remove_average_velocity)
+ * if temperature_every > 0 and k % temperature_every == 0.  phi_increment and
+ * temperature_target are device arrays of length S (NULL when the event is off).  Steps
+ * between events run device-driven (graph batches when use_graph). */
+int dpmd_nvt_rescale_run(dpmd_ctx* ctx, int n_steps, int step0, const double* phi_increment, int compress_every,
+                         const double* temperature_target, int temperature_every, int remove_average_velocity,
+                         int use_graph);
 /* FIRE[Wall](P, dt).init() / .step() / .converged() / .get_dt() (fire.cuh:44-109). */
 int dpmd_fire_init(dpmd_ctx* ctx, const double* dt, int mode);
 int dpmd_fire_step(dpmd_ctx* ctx);
@@ -200,6 +223,22 @@ int dpmd_fire_minimize(dpmd_ctx* ctx, const double* dt, int max_steps, int predi
 int dpmd_jam_compress(dpmd_ctx* ctx, const double* dt, double dphi, double p_target, int max_compressions,
                       int fire_max_steps, int predicate, double a, double b, int mode, int use_graph,
                       int* jam_state, int* n_compressions, int64_t* work);
+/* jam_binary_search_pbc / jam_binary_search_wall (jammers.cuh:60-256, jammers.cu:7-92) for
+ * disk batches; mode DPMD_FORCE_PBC / _WALL picks the variant (DPMD_FORCE_PER_SYSTEM uses the
+ * per-system boundary flag with the PBC variant's steps).  Per system: minimise with
+ * minimize_fire_copy_dt (the reference's stopping test and argument order), drop systems
+ * whose first minimisation did not stop, then bisect phi between the last unjammed state
+ * (<pe> <= avg_pe_target, saved) and the first jammed one (reverted to the saved state),
+ * growing phi by phi_increment until a jammed state is seen, until |phi_high / phi_low - 1| <
+ * phi_tolerance or max_compression_steps rounds.  The box is resized and the lists rebuilt
+ * before the positions are scaled, as in the reference.  dt: device [S].  action (host [S],
+ * may be NULL): 0 done or dropped, -1 last state unjammed, 1 last state jammed (reverted).
+ * phi (host [S], may be NULL): the bisection's packing fraction.  compression_steps (host,
+ * may be NULL): rounds run.  Ends with the forces, pe totals and packing fractions computed. */
+int dpmd_jam_binary_search(dpmd_ctx* ctx, const double* dt, int max_compression_steps, int max_minimization_steps,
+                           double avg_pe_target, double avg_pe_diff_target, double phi_increment,
+                           double phi_tolerance, int mode, int use_graph, int* action, double* phi,
+                           int* compression_steps);
 /* ---- adjoints (disks) --------------------------------------------------------------------
  * New; the reference has no derivatives of its dynamics.  Its Hessian observable
  * (compute_hessian, src/particles/disk.cu:565-661) fixes the pair block used here:

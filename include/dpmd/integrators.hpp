@@ -1,7 +1,9 @@
-// md::integrators -- VelocityVerlet[Wall] and FIRE[Wall] over the C ABI.
+// md::integrators -- VelocityVerlet[Wall], DampedVelocityVerlet[Wall] and FIRE[Wall] over the
+// C ABI.
 //
 // Reference interface: include/integrators/base_integrator.hpp:8-32,
-// include/integrators/velocity_verlet.cuh:10-78, include/integrators/fire.cuh:44-178.  Each
+// include/integrators/velocity_verlet.cuh:10-78, damped_velocity_verlet.cuh:10-87,
+// include/integrators/fire.cuh:44-178.  Each
 // step() is one fused device step (drift, device-side rebuild check, forces); the particle
 // object is held by reference and must outlive the integrator, as in the reference.
 #pragma once
@@ -45,6 +47,43 @@ class VelocityVerletWall final : public BaseVelocityVerlet<ParticleT, DPMD_FORCE
 template <class ParticleT>
 class VelocityVerletPerSystem final : public BaseVelocityVerlet<ParticleT, DPMD_FORCE_PER_SYSTEM> {
     using BaseVelocityVerlet<ParticleT, DPMD_FORCE_PER_SYSTEM>::BaseVelocityVerlet;
+};
+
+// DampedVelocityVerlet[Wall] (damped_velocity_verlet.cuh:10-87): velocity Verlet with
+// compute_damping_forces(damping) after the forces of every step.
+template <class ParticleT, int Mode>
+class BaseDampedVelocityVerlet {
+public:
+    BaseDampedVelocityVerlet(ParticleT& p, double dt, double damping)
+        : p_(p), dt_(p.per_system(dt)), damping_(p.per_system(damping)) {}
+    BaseDampedVelocityVerlet(ParticleT& p, const df::DeviceField1D<double>& dt, const df::DeviceField1D<double>& damping)
+        : p_(p), dt_(dt), damping_(damping) {
+        if (dt.size() != p.n_systems())
+            throw std::invalid_argument("BaseDampedVelocityVerlet: dt_init must have the same size as the number of systems");
+        if (damping.size() != p.n_systems())
+            throw std::invalid_argument(
+                "BaseDampedVelocityVerlet: damping_init must have the same size as the number of systems");
+    }
+    BaseDampedVelocityVerlet(const BaseDampedVelocityVerlet&) = delete;
+    BaseDampedVelocityVerlet& operator=(const BaseDampedVelocityVerlet&) = delete;
+
+    void init() { ok(dpmd_damped_vv_init(p_.ctx(), dt_.ptr(), damping_.ptr(), Mode)); }
+    void step() { ok(dpmd_vv_step(p_.ctx(), 1)); }
+    void run(int n, bool use_graph = true) { ok(dpmd_vv_run(p_.ctx(), n, use_graph ? 1 : 0)); }
+
+protected:
+    void ok(int st) { df::detail::check(st, p_.ctx()); }
+    ParticleT& p_;
+    df::DeviceField1D<double> dt_, damping_;
+};
+
+template <class ParticleT>
+class DampedVelocityVerlet final : public BaseDampedVelocityVerlet<ParticleT, DPMD_FORCE_PBC> {
+    using BaseDampedVelocityVerlet<ParticleT, DPMD_FORCE_PBC>::BaseDampedVelocityVerlet;
+};
+template <class ParticleT>
+class DampedVelocityVerletWall final : public BaseDampedVelocityVerlet<ParticleT, DPMD_FORCE_WALL> {
+    using BaseDampedVelocityVerlet<ParticleT, DPMD_FORCE_WALL>::BaseDampedVelocityVerlet;
 };
 
 template <class ParticleT, int Mode>

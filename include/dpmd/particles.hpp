@@ -124,6 +124,7 @@ public:
     void update_positions(F1 scale, double scale2 = 1.0) { ok(dpmd_update_positions(ctx_, scale.ptr(), scale2)); }
     void update_velocities(F1 scale, double scale2 = 1.0) { ok(dpmd_update_velocities(ctx_, scale.ptr(), scale2)); }
     void mix_velocities_and_forces(F1 alpha) { ok(dpmd_mix_velocities_and_forces(ctx_, alpha.ptr())); }
+    void compute_damping_forces(F1 scale) { ok(dpmd_compute_damping_forces(ctx_, scale.ptr())); }
     void scale_velocities(F1 scale) { ok(dpmd_scale_velocities(ctx_, scale.ptr())); }
     void scale_velocities(double s) { scale_velocities(per_system(s)); }
     void scale_positions(F1 scale) { ok(dpmd_scale_positions(ctx_, scale.ptr())); }

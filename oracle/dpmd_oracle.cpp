@@ -135,7 +135,7 @@ struct Disk {
     std::vector<std::vector<int>> nbr;
     int max_count = 0;
     FireState fire;
-    std::vector<double> vv_dt, vv_dt_half;
+    std::vector<double> vv_dt, vv_dt_half, vv_damp;
     // saved state (base_particle.hpp:708-736; disk.cu:348-432)
     std::vector<double> sx, sy, srad, smass, sLx, sLy;
     std::vector<int> sstatic;
@@ -325,6 +325,14 @@ struct Disk {
             vy[i] = vy[i] + fy[i] * smi;
         }
     }
+    // compute_damping_forces_kernel (disk.cu:249-263)
+    void damping(const double* scale) {
+        for (int i = 0; i < N; ++i) {
+            const double sc = scale[sid[i]];
+            fx[i] += -vx[i] * sc;
+            fy[i] += -vy[i] * sc;
+        }
+    }
     void scale_velocities(const double* scale) {   // disk.cu:278-289
         for (int i = 0; i < N; ++i) { vx[i] *= scale[sid[i]]; vy[i] *= scale[sid[i]]; }
     }
@@ -507,10 +515,13 @@ struct Disk {
     }
 
     // ---------------- velocity Verlet (include/integrators/velocity_verlet.cuh:34-41) ---
-    void vv_init(const double* dt, int mode) {
+    // velocity_verlet.cuh:16-32; with damping: damped_velocity_verlet.cuh:16-56 (init
+    // computes the forces without the drag)
+    void vv_init(const double* dt, int mode, const double* damp = nullptr) {
         vv_dt.assign(dt, dt + S);
         vv_dt_half.resize(S);
         for (int s = 0; s < S; ++s) vv_dt_half[s] = dt[s] * 0.5;
+        if (damp) vv_damp.assign(damp, damp + S); else vv_damp.clear();
         compute_forces(mode);
     }
     void vv_step(int mode) {
@@ -518,6 +529,7 @@ struct Disk {
         update_positions(vv_dt.data(), 1.0);
         check_neighbors(false);
         compute_forces(mode);
+        if (!vv_damp.empty()) damping(vv_damp.data());   // damped_velocity_verlet.cuh:42-50
         update_velocities(vv_dt_half.data(), 1.0);
     }
 
@@ -632,7 +644,7 @@ struct Rigid {
     std::vector<std::vector<int>> nbr;
     int max_count = 0;
     FireState fire;
-    std::vector<double> vv_dt, vv_dt_half;
+    std::vector<double> vv_dt, vv_dt_half, vv_damp;
 
     void sync_box() {
         ix.resize(S); iy.resize(S);
@@ -790,6 +802,15 @@ struct Rigid {
         }
     }
     // update_velocities_kernel (rigid_bumpy.cu:130-161)
+    // compute_damping_forces_kernel (rigid_bumpy.cu:310-327)
+    void damping(const double* scale) {
+        for (int i = 0; i < N; ++i) {
+            const double sc = scale[sid[i]];
+            fx[i] += -vx[i] * sc;
+            fy[i] += -vy[i] * sc;
+            tq[i] += -w[i] * sc;
+        }
+    }
     void update_velocities(const double* scale, double scale2) {
         for (int i = 0; i < N; ++i) {
             const double sc = scale[sid[i]] * scale2;
@@ -932,10 +953,13 @@ struct Rigid {
         }
         return all ? 1 : 0;
     }
-    void vv_init(const double* dt, int mode) {
+    // velocity_verlet.cuh:16-32; with damping: damped_velocity_verlet.cuh:16-56 (init
+    // computes the forces without the drag)
+    void vv_init(const double* dt, int mode, const double* damp = nullptr) {
         vv_dt.assign(dt, dt + S);
         vv_dt_half.resize(S);
         for (int s = 0; s < S; ++s) vv_dt_half[s] = dt[s] * 0.5;
+        if (damp) vv_damp.assign(damp, damp + S); else vv_damp.clear();
         compute_forces(mode);
     }
     void vv_step(int mode) {
@@ -943,6 +967,7 @@ struct Rigid {
         update_positions(vv_dt.data(), 1.0);
         check_neighbors(false);
         compute_forces(mode);
+        if (!vv_damp.empty()) damping(vv_damp.data());   // damped_velocity_verlet.cuh:42-50
         update_velocities(vv_dt_half.data(), 1.0);
     }
 };
@@ -1037,6 +1062,10 @@ int or_rigid_fire_minimize(void* h, const double* dt, int max_steps, double d, d
     return step;
 }
 void or_rigid_vv_init(void* h, const double* dt, int mode) { ((Rigid*)h)->vv_init(dt, mode); }
+void or_rigid_damped_vv_init(void* h, const double* dt, const double* damp, int mode) {
+    ((Rigid*)h)->vv_init(dt, mode, damp);
+}
+void or_rigid_damping(void* h, const double* scale) { ((Rigid*)h)->damping(scale); }
 void or_rigid_vv_run(void* h, int n, int mode) { Rigid* r = (Rigid*)h; for (int k = 0; k < n; ++k) r->vv_step(mode); }
 }  // extern "C"
 
@@ -1282,7 +1311,102 @@ long long or_disk_jam_compress(void* h, const double* dt0, double dphi, double p
     return work;
 }
 
+// jam_binary_search_pbc / _wall (include/routines/jammers.cuh:60-256, src/routines/jammers.cu:7-92)
+// over the primitives above.  minimize_fire_copy_dt (minimizers.cuh:34-58) passes
+// (avg_pe_target, avg_pe_diff_target) into converged(diff, target) (fire.cuh:98), so the
+// diff test uses avg_pe_target and the <pe> test avg_pe_diff_target, as written there.
+// Returns the number of compression rounds; action/phi as the library's
+// dpmd_jam_binary_search.
+int or_disk_jam_bisect(void* h, const double* dt, int max_comp, int max_min, double pe_t, double pe_diff_t,
+                       double dphi, double phi_tol, int mode, int* action_out, double* phi_out) {
+    Disk* d = (Disk*)h;
+    const int S = d->S;
+    std::vector<double> dt_copy(dt, dt + S), dtw(S), zero(S, 0.0);
+    std::vector<int> action(S, -1);
+    auto minimize_copy_dt = [&](const double* dts) {
+        d->fire_init(dts, mode);
+        int step = 0, conv = 0;
+        while (step < max_min && !conv) {
+            d->fire_step(mode);
+            conv = d->fire_converged(pe_t, pe_diff_t, 0, 0.0);
+            ++step;
+        }
+        std::vector<double> out = d->fire.dt;
+        d->scale_velocities(zero.data());
+        return out;
+    };
+    if (mode != 1) d->scale_velocities(zero.data());   // jammers.cuh:85 zero_velocities
+    std::vector<double> dtf = minimize_copy_dt(dt_copy.data());
+    for (int s = 0; s < S; ++s) if (dtf[s] != 0.0) action[s] = 0;   // jammers.cu:7-17
+    d->save_state(action.data(), -1);
+    std::vector<double> phi(S);
+    d->packing_fraction(phi.data());
+    std::vector<double> phi_low = phi, phi_high(S, -1.0), pe(S), A(S), sf(S);
+    int step = 0;
+    while (step < max_comp) {
+        for (int s = 0; s < S; ++s) dtw[s] = action[s] != 0 ? dt_copy[s] : 0.0;   // jammers.cu:19-28
+        minimize_copy_dt(dtw.data());
+        d->pe_total(pe.data());
+        for (int s = 0; s < S; ++s) {   // jammers.cu:30-69
+            if (action[s] == 0) continue;
+            action[s] = -2;
+            const int dof = d->off[s + 1] - d->off[s];
+            const double avg_pe = pe[s] / dof;
+            if (avg_pe > pe_t) {
+                phi_high[s] = phi[s];
+                phi[s] = (phi_high[s] + phi_low[s]) / 2.0;
+                action[s] = 1;
+            } else {
+                action[s] = -1;
+                phi_low[s] = phi[s];
+                if (phi_high[s] > 0) phi[s] = (phi_high[s] + phi_low[s]) / 2.0;
+                else phi[s] += dphi;
+            }
+            if (std::abs(phi_high[s] / phi_low[s] - 1) < phi_tol && phi_high[s] > 0) action[s] = 0;
+        }
+        d->save_state(action.data(), -1);
+        d->restore_state(action.data(), 1);
+        d->sys_reduce([&](int i) { return d->area[i]; }, A.data());
+        for (int s = 0; s < S; ++s) {   // jammers.cu:71-90
+            if (action[s] == 0 || phi[s] <= 0.0) { sf[s] = 1.0; continue; }
+            const double L = std::sqrt(A[s] / phi[s]);
+            sf[s] = L / d->Lx[s];
+            d->Lx[s] = L; d->Ly[s] = L;
+        }
+        d->sync_box();
+        d->regrid_if_needed();
+        d->check_neighbors(true);
+        d->scale_positions(sf.data());
+        bool done = true;
+        for (int s = 0; s < S; ++s) done = done && action[s] == 0;
+        if (done) break;
+        ++step;
+    }
+    d->compute_forces(mode);
+    for (int s = 0; s < S; ++s) { action_out[s] = action[s]; phi_out[s] = phi[s]; }
+    return step;
+}
+
+// Rescale-thermostat NVT with affine compression (the library's dpmd_nvt_rescale_run; the
+// loop of tests/nvt_rescale_disk_pbc_compress.cu:41-68): per global step k = step0 + i one
+// velocity-Verlet step, then increment_packing_fraction if k % cf == 0 (cf > 0), then
+// set_temperature if k % tf == 0 (tf > 0).
+void or_disk_nvt_rescale_run(void* h, int n, int step0, const double* inc, int cf, const double* T, int tf, int rm,
+                             int mode) {
+    Disk* d = (Disk*)h;
+    for (int i = 0; i < n; ++i) {
+        const int k = step0 + i;
+        d->vv_step(mode);
+        if (cf > 0 && k % cf == 0) d->increment_packing_fraction(inc);
+        if (tf > 0 && k % tf == 0) d->set_temperature(T, rm);
+    }
+}
+
 void or_disk_vv_init(void* h, const double* dt, int mode) { ((Disk*)h)->vv_init(dt, mode); }
+void or_disk_damped_vv_init(void* h, const double* dt, const double* damp, int mode) {
+    ((Disk*)h)->vv_init(dt, mode, damp);
+}
+void or_disk_damping(void* h, const double* scale) { ((Disk*)h)->damping(scale); }
 void or_disk_vv_run(void* h, int n, int mode) { Disk* d = (Disk*)h; for (int k = 0; k < n; ++k) d->vv_step(mode); }
 
 // Brute-force O(N^2) min-image neighbor set of particle i (for pinning the cell list).

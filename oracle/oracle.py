@@ -53,6 +53,12 @@ _PROTOS = {
     "or_disk_save_state": (None, [vp, vp, i32]),
     "or_disk_restore_state": (None, [vp, vp, i32]),
     "or_disk_fire_init": (None, [vp, vp, i32]),
+    "or_disk_damped_vv_init": (None, [vp, vp, vp, i32]),
+    "or_disk_jam_bisect": (i32, [vp, vp, i32, i32, f64, f64, f64, f64, i32, vp, vp]),
+    "or_disk_nvt_rescale_run": (None, [vp, i32, i32, vp, i32, vp, i32, i32, i32]),
+    "or_disk_damping": (None, [vp, vp]),
+    "or_rigid_damped_vv_init": (None, [vp, vp, vp, i32]),
+    "or_rigid_damping": (None, [vp, vp]),
     "or_disk_fire_step": (None, [vp, i32]),
     "or_disk_fire_converged": (i32, [vp, f64, f64, i32, f64]),
     "or_disk_fire_get": (None, [vp, vp, vp, vp, vp]),
@@ -290,6 +296,26 @@ class OracleDisk:
     def vv_init(self, dt, mode=0):
         a = self._ps(dt); lib().or_disk_vv_init(self.h, _p(a), mode)
 
+    def damped_vv_init(self, dt, damping, mode=0):
+        a, d = self._ps(dt), self._ps(damping); lib().or_disk_damped_vv_init(self.h, _p(a), _p(d), mode)
+
+    def jam_bisect(self, dt, max_comp, max_min, pe_target, pe_diff_target, dphi, phi_tol, mode=0):
+        """jam_binary_search_pbc / _wall; returns (action[S], phi[S], rounds)."""
+        d = self._ps(dt)
+        act = np.zeros(self.S, np.int32)
+        phi = np.zeros(self.S)
+        r = lib().or_disk_jam_bisect(self.h, _p(d), max_comp, max_min, pe_target, pe_diff_target, dphi, phi_tol,
+                                     mode, _p(act), _p(phi))
+        return act, phi, int(r)
+
+    def nvt_rescale_run(self, n, inc=None, cf=0, T=None, tf=0, step0=0, rm=1, mode=0):
+        a = self._ps(inc if inc is not None else 0.0)
+        t = self._ps(T if T is not None else 0.0)
+        lib().or_disk_nvt_rescale_run(self.h, n, step0, _p(a), cf, _p(t), tf, rm, mode)
+
+    def compute_damping_forces(self, scale):
+        a = self._ps(scale); lib().or_disk_damping(self.h, _p(a))
+
     def vv_run(self, n, mode=0): lib().or_disk_vv_run(self.h, n, mode)
 
 
@@ -445,5 +471,11 @@ class OracleRigid:
 
     def vv_init(self, dt, mode=0):
         a = self._ps(dt); lib().or_rigid_vv_init(self.h, _p(a), mode)
+
+    def damped_vv_init(self, dt, damping, mode=0):
+        a, d = self._ps(dt), self._ps(damping); lib().or_rigid_damped_vv_init(self.h, _p(a), _p(d), mode)
+
+    def compute_damping_forces(self, scale):
+        a = self._ps(scale); lib().or_rigid_damping(self.h, _p(a))
 
     def vv_run(self, n, mode=0): lib().or_rigid_vv_run(self.h, n, mode)

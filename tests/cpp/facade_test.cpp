@@ -1,6 +1,6 @@
 // Drop-in check of the C++ facade: a driver written against the reference's API
-// (md::disk::Disk, df::DeviceField1D/2D, md::integrators::*, md::routines::minimize_fire),
-// in the call order of the reference's tests (tests/test_disk.cu:120-207,
+// (md::disk::Disk, df::DeviceField1D/2D, md::integrators::*, md::routines::minimize_fire,
+// nvt_rescale_run, jam_binary_search_pbc), in the call order of the reference's tests (tests/test_disk.cu:120-207,
 // tests/fire_disk_pbc.cu).  It writes the final canonical state to a binary file so the
 // test harness can compare it with the same sequence issued through the Python binding.
 //
@@ -68,6 +68,15 @@ int main(int argc, char** argv) {
     vv.init();
     for (int k = 0; k < 50; ++k) vv.step();
     vv.run(50);
+    md::integrators::DampedVelocityVerlet<md::disk::Disk> dvv(P, 1e-2, 0.2);
+    dvv.init();
+    for (int k = 0; k < 20; ++k) dvv.step();
+    dvv.run(20);
+    df::DeviceField1D<double> inc = P.per_system(2e-3), T = P.per_system(2e-4);
+    md::routines::nvt_rescale_run(P, 60, &inc, 25, &T, 10);
+    df::DeviceField1D<double> jdt(S);
+    jdt.fill(1e-2);
+    const int rounds = md::routines::jam_binary_search_pbc(P, jdt, 3, 500, 1e-10, 1e-10, 5e-3, 1e-4);
     P.compute_pe_total();
     P.compute_ke_total();
     P.compute_pressure();
@@ -89,7 +98,7 @@ int main(int argc, char** argv) {
     };
     put(x, true); put(y, true); put(vx, true); put(vy, true);
     put(pe0, false); put(pe, false); put(ke, false); put(pr, false);
-    std::cout << "facade_test: fire_steps=" << steps << " n_neighbors=" << P.n_neighbors()
+    std::cout << "facade_test: fire_steps=" << steps << " jam_rounds=" << rounds << " n_neighbors=" << P.n_neighbors()
               << " n_cells=" << P.n_cells() << " pe=" << pe[0] << "," << pe[1] << std::endl;
     return 0;
 }

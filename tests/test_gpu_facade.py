@@ -11,7 +11,8 @@ import pytest
 import oracle as orc
 from dpmd_amd import _lib as L
 from dpmd_amd import workloads as W
-from dpmd_amd.system import FIRE, Disk, VelocityVerlet, minimize_fire
+from dpmd_amd.system import (FIRE, DampedVelocityVerlet, Disk, VelocityVerlet, jam_binary_search, minimize_fire,
+                             nvt_rescale_run)
 
 pytestmark = pytest.mark.gpu
 ROOT = Path(__file__).resolve().parent.parent
@@ -50,11 +51,17 @@ def python_sequence():
     vv.init()
     vv.step(50)
     vv.run(50)
+    dvv = DampedVelocityVerlet(P, 1e-2, 0.2)
+    dvv.init()
+    dvv.step(20)
+    dvv.run(20)
+    nvt_rescale_run(P, 60, 2e-3, 25, 2e-4, 10)
+    _, _, rounds = jam_binary_search(P, 1e-2, 3, 500, 1e-10, 1e-10, 5e-3, 1e-4)
     P.compute_pe_total(); P.compute_ke_total(); P.compute_pressure()
     c = L.ORDER_CANONICAL
     state = np.concatenate([P.get("POS_X", c), P.get("POS_Y", c), P.get("VEL_X", c), P.get("VEL_Y", c),
                             pe0, P.get("PE_TOTAL"), P.get("KE_TOTAL"), P.get("PRESSURE")])
-    return state, steps
+    return state, steps, rounds
 
 
 def test_cpp_facade_matches_python_binding(tmp_path):
@@ -65,7 +72,7 @@ def test_cpp_facade_matches_python_binding(tmp_path):
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     cpp = np.fromfile(out, dtype=np.float64)
-    py, steps = python_sequence()
-    assert f"fire_steps={steps}" in r.stdout
+    py, steps, rounds = python_sequence()
+    assert f"fire_steps={steps} jam_rounds={rounds}" in r.stdout
     assert cpp.shape == py.shape
     assert np.array_equal(cpp, py)
