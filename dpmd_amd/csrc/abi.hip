@@ -805,13 +805,19 @@ void save_state_impl(dpmd_ctx* c, const int* flag, int tv) {
         for (auto* v : dv) { v->resize(N); v->zero(c->st); }
         c->sv_static.resize(N); c->sv_static.zero(c->st);
         c->sv_Lx.resize(S); c->sv_Ly.resize(S);
+        if (rigid(c)) {
+            const int Nv = c->Nv;
+            dpmd::DVec<double>* vv[] = {&c->sv_vpx, &c->sv_vpy, &c->sv_vrad, &c->sv_vmass};
+            for (auto* v : vv) { v->resize(Nv); v->zero(c->st); }
+            c->sv_vpid.resize(Nv); c->sv_vpid.zero(c->st);
+            c->sv_vstatic.resize(Nv); c->sv_vstatic.zero(c->st);
+        }
         c->saved = true;
     }
     save_restore(c, flag, tv, false);
 }
 void restore_state_impl(dpmd_ctx* c, const int* flag, int tv) {
     need_state(c->saved, "restore_state: nothing has been saved");
-    need_state(!rigid(c), "restore_state: rigid bodies restore is not supported");
     save_restore(c, flag, tv, true);
     box_inv(c);
     check_neighbors_impl(c, true);
@@ -846,7 +852,6 @@ std::vector<double> minimize_copy_dt(dpmd_ctx* c, const std::vector<double>& dt,
 void jam_bisect_impl(dpmd_ctx* c, const double* dt, int max_comp, int max_min, double pe_t, double pe_diff_t,
                      double dphi, double phi_tol, int mode, bool use_graph, int* action_out, double* phi_out,
                      int* steps_out) {
-    need(!rigid(c), "jam_binary_search: disk systems (rigid-body restore_state is not supported)");
     need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
     need(dt != nullptr && max_comp >= 0 && max_min >= 0, "jam_binary_search: bad arguments");
     const int S = c->S;
@@ -1143,6 +1148,7 @@ int dpmd_allocate_systems(dpmd_ctx* c, int S) {
         c->voff.resize(S + 1); c->voff.zero(c->st);
         c->flag.resize(S); c->flag.zero(c->st);
         c->synced = false; c->nbr_ready = false; c->fire_ready = false; c->vv_ready = false;
+        c->saved = false;   // a saved state no longer matches the arrays
     });
 }
 
@@ -1163,6 +1169,7 @@ int dpmd_allocate_particles(dpmd_ctx* c, int N) {
             c->static_index.upload(iota.data(), N, c->st);
         }
         c->synced = false; c->nbr_ready = false; c->fire_ready = false; c->vv_ready = false;
+        c->saved = false;   // a saved state no longer matches the arrays
     });
 }
 
@@ -1179,6 +1186,7 @@ int dpmd_allocate_vertices(dpmd_ctx* c, int Nv) {
         std::iota(iota.begin(), iota.end(), 0);
         c->static_index.upload(iota.data(), Nv, c->st);
         c->synced = false; c->nbr_ready = false;
+        c->saved = false;
     });
 }
 

@@ -156,6 +156,8 @@ struct dpmd_ctx {
     // ---- saved state (save_state/restore_state)
     dpmd::DVec<double> sv_x, sv_y, sv_rad, sv_mass, sv_Lx, sv_Ly, sv_angle, sv_vx, sv_vy;
     dpmd::DVec<int> sv_static;
+    dpmd::DVec<double> sv_vpx, sv_vpy, sv_vrad, sv_vmass;   // rigid bodies: vertex state
+    dpmd::DVec<int> sv_vpid, sv_vstatic;
     bool saved = false;
     // scratch per-system
     dpmd::DVec<double> scr_a, scr_b, scr_c;

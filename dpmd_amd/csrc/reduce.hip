@@ -153,6 +153,21 @@ __global__ void k_save_restore_p(int N, const int* sid, const int* flag, int tv,
     }
 }
 
+// save_/restore_vertex_state_kernel (rigid_bumpy.cu:655-674,711-730) over vertices in current
+// order, with the vertex masses kept alongside (they are reordered with the vertices).
+__global__ void k_save_restore_v(int Nv, const int* vsid, const int* flag, int tv, int restore, double* vx, double* vy,
+                                 double* vr, double* vm, int* vpid, int* stat, double* sx, double* sy, double* sr,
+                                 double* sm, int* spid, int* sstat) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= Nv) return;
+    if (flag[vsid[v]] != tv) return;
+    if (restore) {
+        vx[v] = sx[v]; vy[v] = sy[v]; vr[v] = sr[v]; vm[v] = sm[v]; vpid[v] = spid[v]; stat[v] = sstat[v];
+    } else {
+        sx[v] = vx[v]; sy[v] = vy[v]; sr[v] = vr[v]; sm[v] = vm[v]; spid[v] = vpid[v]; sstat[v] = stat[v];
+    }
+}
+
 __global__ void k_save_restore_s(int S, const int* flag, int tv, int restore, double* Lx, double* Ly, double* sLx,
                                  double* sLy) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -259,6 +274,10 @@ void save_restore(dpmd_ctx* c, const int* flag, int tv, bool restore) {
         c->N, c->sid.p, flag, tv, restore ? 1 : 0, c->x.p, c->y.p, rigid ? nullptr : c->rad.p,
         rigid ? nullptr : c->mass.p, rigid ? nullptr : c->static_index.p, rigid ? c->angle.p : nullptr,
         c->sv_x.p, c->sv_y.p, c->sv_rad.p, c->sv_mass.p, c->sv_static.p, c->sv_angle.p);
+    if (rigid)
+        k_save_restore_v<<<blocks_for(c->Nv), DPMD_BLOCK, 0, c->st>>>(
+            c->Nv, c->vsid.p, flag, tv, restore ? 1 : 0, c->vpx.p, c->vpy.p, c->vrad.p, c->vmass.p, c->vpid.p,
+            c->static_index.p, c->sv_vpx.p, c->sv_vpy.p, c->sv_vrad.p, c->sv_vmass.p, c->sv_vpid.p, c->sv_vstatic.p);
     k_save_restore_s<<<blocks_for(c->S), DPMD_BLOCK, 0, c->st>>>(c->S, flag, tv, restore ? 1 : 0, c->Lx.p, c->Ly.p,
                                                                 c->sv_Lx.p, c->sv_Ly.p);
     HIP_OK(hipGetLastError());

@@ -953,6 +953,53 @@ struct Rigid {
         }
         return all ? 1 : 0;
     }
+    // compute_fractional_packing_fraction_kernel (base_particle_kernels.cu:102-113)
+    void packing_fraction(double* out) {
+        sys_reduce([&](int i) { return area[i] / (Lx[sid[i]] * Ly[sid[i]]); }, out);
+    }
+    // increment_packing_fraction (base_particle.hpp:505-529) with the library's grid guard
+    void increment_packing_fraction(const double* inc) {
+        std::vector<double> phi(S), sc(S);
+        packing_fraction(phi.data());
+        for (int s = 0; s < S; ++s) {
+            const double nphi = phi[s] + inc[s];
+            sc[s] = std::sqrt(phi[s] / nphi);
+            Lx[s] = Lx[s] * sc[s]; Ly[s] = Ly[s] * sc[s];
+        }
+        scale_positions(sc.data());
+        sync_box();
+        regrid_if_needed(true);
+        check_neighbors(true);
+    }
+    // save_state_impl / restore_state_impl (rigid_bumpy.cu:632-741,1763-1854): COM and angle
+    // per particle; vertex positions, radii, particle ids and static_index in current order
+    // (vertex masses kept with them); box.  restore_state (base_particle.hpp:717-727) then
+    // syncs the box and forces a rebuild.
+    std::vector<double> sx, sy, sth, svpx, svpy, svrad, svmass, sLx, sLy;
+    std::vector<int> svpid, sstatic;
+    void save_state(const int* flg, int tv) {
+        if ((int)sx.size() != N) {
+            sx = x; sy = y; sth = th; svpx = vpx; svpy = vpy; svrad = vrad; svmass = vmass; svpid = vpid;
+            sstatic = static_index; sLx = Lx; sLy = Ly;
+        }
+        for (int i = 0; i < N; ++i) if (flg[sid[i]] == tv) { sx[i] = x[i]; sy[i] = y[i]; sth[i] = th[i]; }
+        for (int v = 0; v < Nv; ++v) if (flg[vsid[v]] == tv) {
+            svpx[v] = vpx[v]; svpy[v] = vpy[v]; svrad[v] = vrad[v]; svmass[v] = vmass[v]; svpid[v] = vpid[v];
+            sstatic[v] = static_index[v];
+        }
+        for (int s = 0; s < S; ++s) if (flg[s] == tv) { sLx[s] = Lx[s]; sLy[s] = Ly[s]; }
+    }
+    void restore_state(const int* flg, int tv) {
+        for (int i = 0; i < N; ++i) if (flg[sid[i]] == tv) { x[i] = sx[i]; y[i] = sy[i]; th[i] = sth[i]; }
+        for (int v = 0; v < Nv; ++v) if (flg[vsid[v]] == tv) {
+            vpx[v] = svpx[v]; vpy[v] = svpy[v]; vrad[v] = svrad[v]; vmass[v] = svmass[v]; vpid[v] = svpid[v];
+            static_index[v] = sstatic[v];
+        }
+        for (int s = 0; s < S; ++s) if (flg[s] == tv) { Lx[s] = sLx[s]; Ly[s] = sLy[s]; }
+        sync_box();
+        check_neighbors(true);
+    }
+
     // velocity_verlet.cuh:16-32; with damping: damped_velocity_verlet.cuh:16-56 (init
     // computes the forces without the drag)
     void vv_init(const double* dt, int mode, const double* damp = nullptr) {
@@ -1317,9 +1364,10 @@ long long or_disk_jam_compress(void* h, const double* dt0, double dphi, double p
 // diff test uses avg_pe_target and the <pe> test avg_pe_diff_target, as written there.
 // Returns the number of compression rounds; action/phi as the library's
 // dpmd_jam_binary_search.
-int or_disk_jam_bisect(void* h, const double* dt, int max_comp, int max_min, double pe_t, double pe_diff_t,
-                       double dphi, double phi_tol, int mode, int* action_out, double* phi_out) {
-    Disk* d = (Disk*)h;
+}  // extern "C"
+template <class P>
+static int jam_bisect(P* d, const double* dt, int max_comp, int max_min, double pe_t, double pe_diff_t, double dphi,
+                      double phi_tol, int mode, int* action_out, double* phi_out) {
     const int S = d->S;
     std::vector<double> dt_copy(dt, dt + S), dtw(S), zero(S, 0.0);
     std::vector<int> action(S, -1);
@@ -1374,7 +1422,7 @@ int or_disk_jam_bisect(void* h, const double* dt, int max_comp, int max_min, dou
             d->Lx[s] = L; d->Ly[s] = L;
         }
         d->sync_box();
-        d->regrid_if_needed();
+        d->regrid_if_needed(true);
         d->check_neighbors(true);
         d->scale_positions(sf.data());
         bool done = true;
@@ -1386,6 +1434,19 @@ int or_disk_jam_bisect(void* h, const double* dt, int max_comp, int max_min, dou
     for (int s = 0; s < S; ++s) { action_out[s] = action[s]; phi_out[s] = phi[s]; }
     return step;
 }
+extern "C" {
+int or_disk_jam_bisect(void* h, const double* dt, int max_comp, int max_min, double pe_t, double pe_diff_t,
+                       double dphi, double phi_tol, int mode, int* action_out, double* phi_out) {
+    return jam_bisect((Disk*)h, dt, max_comp, max_min, pe_t, pe_diff_t, dphi, phi_tol, mode, action_out, phi_out);
+}
+int or_rigid_jam_bisect(void* h, const double* dt, int max_comp, int max_min, double pe_t, double pe_diff_t,
+                        double dphi, double phi_tol, int mode, int* action_out, double* phi_out) {
+    return jam_bisect((Rigid*)h, dt, max_comp, max_min, pe_t, pe_diff_t, dphi, phi_tol, mode, action_out, phi_out);
+}
+void or_rigid_packing_fraction(void* h, double* o) { ((Rigid*)h)->packing_fraction(o); }
+void or_rigid_increment_packing_fraction(void* h, const double* inc) { ((Rigid*)h)->increment_packing_fraction(inc); }
+void or_rigid_save_state(void* h, const int* flg, int tv) { ((Rigid*)h)->save_state(flg, tv); }
+void or_rigid_restore_state(void* h, const int* flg, int tv) { ((Rigid*)h)->restore_state(flg, tv); }
 
 // Rescale-thermostat NVT with affine compression (the library's dpmd_nvt_rescale_run; the
 // loop of tests/nvt_rescale_disk_pbc_compress.cu:41-68): per global step k = step0 + i one

@@ -55,6 +55,11 @@ _PROTOS = {
     "or_disk_fire_init": (None, [vp, vp, i32]),
     "or_disk_damped_vv_init": (None, [vp, vp, vp, i32]),
     "or_disk_jam_bisect": (i32, [vp, vp, i32, i32, f64, f64, f64, f64, i32, vp, vp]),
+    "or_rigid_jam_bisect": (i32, [vp, vp, i32, i32, f64, f64, f64, f64, i32, vp, vp]),
+    "or_rigid_packing_fraction": (None, [vp, vp]),
+    "or_rigid_increment_packing_fraction": (None, [vp, vp]),
+    "or_rigid_save_state": (None, [vp, vp, i32]),
+    "or_rigid_restore_state": (None, [vp, vp, i32]),
     "or_disk_nvt_rescale_run": (None, [vp, i32, i32, vp, i32, vp, i32, i32, i32]),
     "or_disk_damping": (None, [vp, vp]),
     "or_rigid_damped_vv_init": (None, [vp, vp, vp, i32]),
@@ -474,6 +479,26 @@ class OracleRigid:
 
     def damped_vv_init(self, dt, damping, mode=0):
         a, d = self._ps(dt), self._ps(damping); lib().or_rigid_damped_vv_init(self.h, _p(a), _p(d), mode)
+
+    def packing_fraction(self): return self._per_sys(lib().or_rigid_packing_fraction)
+
+    def increment_packing_fraction(self, inc):
+        a = self._ps(inc); lib().or_rigid_increment_packing_fraction(self.h, _p(a))
+
+    def save_state(self, flag=1, tv=1):
+        f = self._ps(flag, np.int32); lib().or_rigid_save_state(self.h, _p(f), tv)
+
+    def restore_state(self, flag=1, tv=1):
+        f = self._ps(flag, np.int32); lib().or_rigid_restore_state(self.h, _p(f), tv)
+
+    def jam_bisect(self, dt, max_comp, max_min, pe_target, pe_diff_target, dphi, phi_tol, mode=0):
+        """jam_binary_search_pbc / _wall on rigid bodies; returns (action[S], phi[S], rounds)."""
+        d = self._ps(dt)
+        act = np.zeros(self.S, np.int32)
+        phi = np.zeros(self.S)
+        r = lib().or_rigid_jam_bisect(self.h, _p(d), max_comp, max_min, pe_target, pe_diff_target, dphi, phi_tol,
+                                      mode, _p(act), _p(phi))
+        return act, phi, int(r)
 
     def compute_damping_forces(self, scale):
         a = self._ps(scale); lib().or_rigid_damping(self.h, _p(a))

@@ -166,3 +166,68 @@ def test_jam_binary_search_one_disk_walls():
     assert np.all(act == 0) and rounds == rrounds
     assert np.array_equal(phi, rphi)
     assert np.all(np.abs(phi / (math.pi / 4) - 1) < 2 * tol)
+
+
+RNAMES = {"x": "POS_X", "y": "POS_Y", "angle": "ANGLE", "vx": "VEL_X", "vy": "VEL_Y", "w": "ANGULAR_VEL",
+          "fx": "FORCE_X", "fy": "FORCE_Y", "torque": "TORQUE", "pe": "PE", "vpx": "VERTEX_POS_X",
+          "vpy": "VERTEX_POS_Y"}
+
+
+def make_rigid(b):
+    P = RigidBumpy()
+    W.load_rigid_into(P, b)
+    P.init_neighbors()
+    return P, orc.OracleRigid(b)
+
+
+def assert_same_rigid(P, ref, fields, what):
+    for f in fields:
+        g, r = P.get(RNAMES[f]), ref.get(f)
+        if not np.array_equal(g, r):
+            raise AssertionError(f"{what}: {f} differs, max |diff| = {np.max(np.abs(g - r)):.3e}")
+
+
+def test_rigid_save_restore_matches_oracle():
+    b = W.rigid_batch(S=2, n_per=24, seed=4, boundary=0, phi=0.6)
+    P, ref = make_rigid(b)
+    P.save_state([1, 0], 1)
+    ref.save_state([1, 0], 1)
+    rng = np.random.default_rng(2)
+    for f, r, s in (("VEL_X", "vx", 0.05), ("VEL_Y", "vy", 0.05), ("ANGULAR_VEL", "w", 0.1)):
+        v = rng.normal(size=b.N) * s
+        P.set(f, v); ref.set(r, v)
+    vv = VelocityVerlet(P, 1e-2)
+    vv.init()
+    vv.run(300)
+    ref.vv_init(1e-2); ref.vv_run(300)
+    P.restore_state([1, 0], 1)
+    ref.restore_state([1, 0], 1)
+    assert np.array_equal(P.get("STATIC_INDEX"), ref.static_index())
+    assert_same_rigid(P, ref, ("x", "y", "angle", "vpx", "vpy"), "rigid restore_state")
+    P.compute_forces()
+    ref.compute_forces(0)
+    assert_same_rigid(P, ref, ("fx", "fy", "torque", "pe"), "forces after restore")
+
+
+def test_rigid_jam_binary_search_matches_oracle():
+    b = W.rigid_batch(S=2, n_per=32, seed=6, boundary=0, phi=0.62)
+    P, ref = make_rigid(b)
+    args = (1e-2, 5, 3000, 1e-10, 1e-10, 1e-2, 1e-4)
+    act, phi, rounds = jam_binary_search(P, *args, mode=L.FORCE_PBC)
+    ract, rphi, rrounds = ref.jam_bisect(*args, mode=0)
+    assert rounds == rrounds and np.array_equal(act, ract) and np.array_equal(phi, rphi)
+    assert_same_rigid(P, ref, ("x", "y", "angle", "vpx", "vpy", "fx", "fy", "torque"), "rigid jam")
+
+
+def test_rigid_jam_binary_search_one_body_walls():
+    """Closed form: a regular bumpy 12-gon jams in a square wall box of side
+    2 R cos(pi / 12) + 2 r_b, phi_J = area / L_J^2."""
+    from test_oracle_routines import one_rigid_body_walls
+    n_v, R = 12, 0.5
+    b = one_rigid_body_walls(n_v, R)
+    P, ref = make_rigid(b)
+    act, phi, rounds = jam_binary_search(P, 1e-2, 300, 40000, 1e-16, 1e-16, 0.02, 1e-6, wall=True)
+    ract, rphi, rrounds = ref.jam_bisect(1e-2, 300, 40000, 1e-16, 1e-16, 0.02, 1e-6, mode=1)
+    assert rounds == rrounds and np.array_equal(phi, rphi)
+    LJ = 2 * R * math.cos(math.pi / n_v) + 2 * R * math.sin(math.pi / n_v)
+    assert act[0] == 0 and abs(phi[0] / (b.area[0] / LJ ** 2) - 1) < 1e-5

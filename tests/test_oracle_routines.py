@@ -183,3 +183,71 @@ def test_jam_bisect_drops_unstopped_systems():
     # 5 FIRE steps cannot relax the phi = 0.9 system: dropped with its phi unchanged
     assert act[0] == 0
     np.testing.assert_allclose(o.packing_fraction()[0], phi[0], rtol=1e-12)
+
+
+def one_rigid_body_walls(n_v=12, R=0.5, phi0=0.35, theta=0.3, pos=(0.45, 0.55)):
+    """One regular bumpy n_v-gon (R0 recipe) in a walled square box."""
+    rb = R * math.sin(math.pi / n_v)
+    area = 0.5 * n_v * R * R * math.sin(2 * math.pi / n_v) + 0.5 * n_v * rb * rb * (math.pi + 2 * math.pi / n_v)
+    L = math.sqrt(area / phi0)
+    x, y = pos[0] * L, pos[1] * L
+    ang = theta + 2 * math.pi * np.arange(n_v) / n_v
+    return W.RigidBatch(off=np.array([0, 1], np.int32), nvpp=np.array([n_v], np.int32), x=np.array([x]),
+                        y=np.array([y]), angle=np.array([theta]), mass=np.array([1.0]), inertia=np.array([R * R]),
+                        area=np.array([area]), vx=x + R * np.cos(ang), vy=y + R * np.sin(ang),
+                        vrad=np.full(n_v, rb), vmass=np.full(n_v, 1.0 / n_v), Lx=np.array([L]), Ly=np.array([L]),
+                        e=np.array([1.0]), cutoff=np.array([0.05]), thresh2=np.array([0.025 ** 2]),
+                        boundary=np.array([1], np.int32), cdx=np.array([3], np.int32), cdy=np.array([3], np.int32))
+
+
+def test_rigid_save_restore_round_trip():
+    b = W.rigid_batch(S=2, n_per=24, seed=4, boundary=0, phi=0.6)
+    o = orc.OracleRigid(b)
+    o.save_state([1, 0], 1)                 # system 0 only
+    snap = {f: o.get(f).copy() for f in ("x", "y", "angle", "vpx", "vpy")}
+    si0 = o.static_index().copy()
+    rng = np.random.default_rng(2)
+    o.set("vx", rng.normal(size=b.N) * 0.05); o.set("vy", rng.normal(size=b.N) * 0.05)
+    o.set("w", rng.normal(size=b.N) * 0.1)
+    o.vv_init(1e-2); o.vv_run(300)          # rebuilds reorder the vertices
+    moved = {f: o.get(f).copy() for f in ("x", "y", "angle")}
+    o.restore_state([1, 0], 1)
+    n0 = b.off[1]
+    for f in ("x", "y", "angle"):
+        assert np.array_equal(o.get(f)[:n0], snap[f][:n0]), f             # restored
+        assert np.array_equal(o.get(f)[n0:], moved[f][n0:]), f            # untouched
+    # canonical vertex positions of system 0 are back (vertex order may differ after the rebuild)
+    si = o.static_index()
+    nv0 = int(np.sum(b.nvpp[:n0]))
+    for f in ("vpx", "vpy"):
+        assert np.array_equal(o.get(f)[si[:nv0]], snap[f][si0[:nv0]]), f
+
+
+def test_rigid_increment_packing_fraction_keeps_shapes():
+    b = W.rigid_batch(S=2, n_per=32, seed=5, boundary=1, phi=0.5)
+    o = orc.OracleRigid(b)
+    phi0 = o.packing_fraction()
+    np.testing.assert_allclose(phi0, 0.5, rtol=1e-12)
+    o.increment_packing_fraction([0.01, 0.02])
+    np.testing.assert_allclose(o.packing_fraction(), phi0 + [0.01, 0.02], rtol=0, atol=1e-12)
+    # rigid shapes: every vertex keeps its distance R from its own centre
+    si = o.static_index()
+    vx, vy = o.get("vpx")[si], o.get("vpy")[si]
+    pid = np.repeat(np.arange(b.N), b.nvpp)
+    d = np.hypot(vx - o.get("x")[pid], vy - o.get("y")[pid])
+    R = np.sqrt(b.inertia / b.mass)[pid]
+    np.testing.assert_allclose(d, R, rtol=1e-12)
+
+
+def test_rigid_jam_bisect_one_body_walls():
+    """A regular bumpy 12-gon (a multiple of 4 vertices) fits a square wall box down to the side
+    2 R cos(pi / n) + 2 r_b (edge-aligned in both axes), so phi_J = area / L_J^2."""
+    n_v, R = 12, 0.5
+    b = one_rigid_body_walls(n_v, R)
+    o = orc.OracleRigid(b)
+    tol = 1e-6
+    act, phi, rounds = o.jam_bisect(1e-2, 300, 40000, 1e-16, 1e-16, 0.02, tol, mode=1)
+    LJ = 2 * R * math.cos(math.pi / n_v) + 2 * R * math.sin(math.pi / n_v)
+    phi_j = b.area[0] / LJ ** 2
+    assert act[0] == 0
+    assert abs(phi[0] / phi_j - 1) < 1e-5
