@@ -26,14 +26,14 @@ STATUS = {0: "OK", 1: "ARGUMENT", 2: "STATE", 3: "HIP", 4: "OVERFLOW"}
 F = dict(
     POS_X=0, POS_Y=1, VEL_X=2, VEL_Y=3, FORCE_X=4, FORCE_Y=5, PE=6, KE=7, RAD=8, MASS=9, AREA=10,
     LAST_POS_X=11, LAST_POS_Y=12, ANGLE=13, ANGULAR_VEL=14, TORQUE=15, MOMENT_INERTIA=16,
-    STRESS_XX=17, STRESS_XY=18, STRESS_YX=19, STRESS_YY=20,
+    STRESS_XX=17, STRESS_XY=18, STRESS_YX=19, STRESS_YY=20, OVERLAP=21, CONTACT_COUNT=22,
     BOX_SIZE_X=32, BOX_SIZE_Y=33, BOX_INV_X=34, BOX_INV_Y=35, E_INTERACTION=36, NEIGHBOR_CUTOFF=37,
     THRESH2=38, PE_TOTAL=39, KE_TOTAL=40, FPOWER_TOTAL=41, PRESSURE=42, TEMPERATURE=43,
     PACKING_FRACTION=44, MAX_FORCE2=45, STRESS_TOTAL_XX=46, STRESS_TOTAL_XY=47, STRESS_TOTAL_YX=48,
     STRESS_TOTAL_YY=49,
     SYSTEM_OFFSET=64, SYSTEM_SIZE=65, CELL_DIM_X=66, CELL_DIM_Y=67, CELL_SYSTEM_START=68, BOUNDARY=69,
-    N_DOF=70, REBUILD_FLAG=71, FIRE_STEPS=72,
-    SYSTEM_ID=80, STATIC_INDEX=81, CELL_ID=82, NEIGHBOR_COUNT=83,
+    N_DOF=70, REBUILD_FLAG=71, FIRE_STEPS=72, N_CONTACTS_TOTAL=73,
+    SYSTEM_ID=80, STATIC_INDEX=81, CELL_ID=82, NEIGHBOR_COUNT=83, CONTACTS=84,
     VERTEX_POS_X=96, VERTEX_POS_Y=97, VERTEX_FORCE_X=98, VERTEX_FORCE_Y=99, VERTEX_PE=100,
     VERTEX_RAD=101, VERTEX_MASS=102,
     VERTEX_PARTICLE_ID=112, PARTICLE_OFFSET=113, N_VERTICES_PER_PARTICLE=114, VERTEX_SYSTEM_ID=115,
@@ -109,6 +109,10 @@ PROTOS = {
     "dpmd_vv_run": (i32, [vp, i32, i32]),
     "dpmd_damped_vv_init": (i32, [vp, vp, vp, i32]),
     "dpmd_compute_damping_forces": (i32, [vp, vp]),
+    "dpmd_compute_contacts": (i32, [vp]),
+    "dpmd_compute_overlaps": (i32, [vp]),
+    "dpmd_compute_n_contacts_total": (i32, [vp]),
+    "dpmd_compute_pair_observables": (i32, [vp, vp, vp, vp, vp, vp, vp, i64]),
     "dpmd_nvt_rescale_run": (i32, [vp, i32, i32, vp, i32, vp, i32, i32, i32]),
     "dpmd_jam_binary_search": (i32, [vp, vp, i32, i32, f64, f64, f64, f64, i32, i32, vp, vp, pi32]),
     "dpmd_fire_init": (i32, [vp, vp, i32]),

@@ -41,6 +41,10 @@ FieldRef field_ref(dpmd_ctx* c, int f) {
         case DPMD_F_STRESS_XY: return ref(c->sxy, 0);
         case DPMD_F_STRESS_YX: return ref(c->syx, 0);
         case DPMD_F_STRESS_YY: return ref(c->syy, 0);
+        case DPMD_F_OVERLAP: return ref(c->overlap, 0);
+        case DPMD_F_CONTACT_COUNT: return ref(c->ccount, 0);
+        case DPMD_F_N_CONTACTS_TOTAL: return ref(c->ncontacts, 2);
+        case DPMD_F_CONTACTS: return ref(c->contacts, 0);
         case DPMD_F_BOX_SIZE_X: return ref(c->Lx, 2);
         case DPMD_F_BOX_SIZE_Y: return ref(c->Ly, 2);
         case DPMD_F_BOX_INV_X: return ref(c->iLx, 2);
@@ -1141,7 +1145,8 @@ int dpmd_allocate_systems(dpmd_ctx* c, int S) {
                                     &c->s_alpha, &c->s_dtrev, &c->s_vscale, &c->vv_dt, &c->vv_dth};
         for (auto* v : dv) { v->resize(S); v->zero(c->st); }
         dpmd::DVec<int>* iv[] = {&c->size, &c->cdx, &c->cdy, &c->boundary, &c->ndof, &c->f_ngood, &c->f_nbad,
-                                 &c->f_stopped, &c->f_frozen, &c->f_nsteps, &c->s_ngood, &c->s_nbad};
+                                 &c->f_stopped, &c->f_frozen, &c->f_nsteps, &c->s_ngood, &c->s_nbad,
+                                 &c->ncontacts};
         for (auto* v : iv) { v->resize(S); v->zero(c->st); }
         c->off.resize(S + 1); c->off.zero(c->st);
         c->cstart.resize(S + 1); c->cstart.zero(c->st);
@@ -1158,9 +1163,11 @@ int dpmd_allocate_particles(dpmd_ctx* c, int N) {
         c->N = N;
         dpmd::DVec<double>* dv[] = {&c->x, &c->y, &c->vx, &c->vy, &c->fx, &c->fy, &c->pe, &c->ke, &c->rad,
                                     &c->mass, &c->area, &c->lx, &c->ly, &c->angle, &c->avel, &c->torque,
-                                    &c->inertia, &c->sxx, &c->sxy, &c->syx, &c->syy, &c->x2, &c->y2};
+                                    &c->inertia, &c->sxx, &c->sxy, &c->syx, &c->syy, &c->x2, &c->y2,
+                                    &c->overlap, &c->ccount};
         for (auto* v : dv) { v->resize(N); v->zero(c->st); }
         c->sid.resize(N); c->sid.zero(c->st);
+        c->contacts.resize(N); c->contacts.zero(c->st);
         c->nvpp.resize(N); c->nvpp.zero(c->st);
         c->poff.resize(N + 1); c->poff.zero(c->st);
         if (!rigid(c)) {
@@ -1395,6 +1402,38 @@ int dpmd_nvt_rescale_run(dpmd_ctx* c, int n_steps, int step0, const double* phi_
         nvt_rescale_impl(c, n_steps, step0, phi_increment, compress_every, temperature_target, temperature_every,
                          remove_average_velocity != 0, use_graph != 0);
         check_overflow(c);
+    });
+}
+
+int dpmd_compute_contacts(dpmd_ctx* c) {
+    return guard(c, [&] {
+        need(!rigid(c), "compute_contacts: implemented for disks");
+        need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
+        disk_contacts(c);
+    });
+}
+int dpmd_compute_n_contacts_total(dpmd_ctx* c) {
+    return guard(c, [&] {
+        need(!rigid(c), "compute_n_contacts_total: implemented for disks");
+        need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
+        disk_contacts(c);
+        contacts_total(c);
+    });
+}
+int dpmd_compute_overlaps(dpmd_ctx* c) { return dpmd_compute_contacts(c); }
+
+int dpmd_compute_pair_observables(dpmd_ctx* c, int* pair_i, int* pair_j, double* dist, double* fx, double* fy,
+                                  double* const* hessian, int64_t n) {
+    return guard(c, [&] {
+        need(!rigid(c), "pair observables: implemented for disks");
+        need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
+        need((pair_i == nullptr) == (pair_j == nullptr), "pair observables: pair_i and pair_j go together");
+        need((fx == nullptr) == (fy == nullptr), "pair observables: fx and fy go together");
+        if (hessian)
+            for (int m = 0; m < 8; ++m) need(hessian[m] != nullptr, "pair observables: 8 Hessian arrays or none");
+        const int64_t tot = pair_setup(c);
+        need(n == tot, "pair observables: n must equal the number of neighbor entries (" + std::to_string(tot) + ")");
+        disk_pair_obs(c, pair_i, pair_j, dist, fx, fy, hessian);
     });
 }
 

@@ -151,6 +151,9 @@ struct dpmd_ctx {
     // damped velocity Verlet: per-system drag coefficients, applied when vv_damped is set
     dpmd::DVec<double> vv_damp;
     bool vv_damped = false;
+    // ---- contact observables (compute_contacts / _overlaps / pair observables)
+    dpmd::DVec<int> contacts, ncontacts, pair_start, canon;
+    dpmd::DVec<double> overlap, ccount;
     // ---- run control: [0] steps_done [1] max_steps [2] n_running [3] active
     dpmd::DVec<int> run, one;
     // ---- saved state (save_state/restore_state)
@@ -314,6 +317,12 @@ void disk_vv_drift_to(dpmd_ctx* c, Gate g, double* xo, double* yo, int any_word)
 void disk_vv_fused(dpmd_ctx* c, Gate g, const double* xs, const double* ys, double* xo, double* yo, int any_word);
 void disk_vv_force(dpmd_ctx* c, Gate g);
 void disk_damping(dpmd_ctx* c, const double* scale);
+
+// observe.hip (disks)
+void disk_contacts(dpmd_ctx* c);
+void contacts_total(dpmd_ctx* c);
+int64_t pair_setup(dpmd_ctx* c);
+void disk_pair_obs(dpmd_ctx* c, int* pi, int* pj, double* dist, double* fx, double* fy, double* const* h);
 void disk_ke(dpmd_ctx* c);
 void fire_recompute_update(dpmd_ctx* c);
 void rigid_compute_forces(dpmd_ctx* c, int mode);

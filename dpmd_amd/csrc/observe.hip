@@ -1,0 +1,177 @@
+// Disk contact observables over the Verlet lists: contact counts and overlaps per particle,
+// contacts per system, and the per-neighbor-entry pair distance, pair force and Hessian
+// blocks.
+//
+// Reference: src/particles/disk.cu:434-700 (compute_contacts_kernel, compute_pair_dist_kernel,
+// compute_particle_pair_forces_kernel, compute_hessian_kernel, compute_overlaps_kernel) and
+// :962-1074 (their host sides); base_particle.hpp:649-702.  As there, the pair geometry uses
+// the minimum image in every box (walled boxes included) and the walls contribute nothing.
+// Outputs indexed by neighbor entry follow the CSR order of dpmd_get_neighbors_csr (particles
+// in current order, each particle's neighbors in list order); particle ids in pair outputs are
+// canonical.  (The reference's pair-distance and Hessian host code hands static_index where
+// its kernels expect the inverse map, which yields canonical ids only while the two coincide;
+// here every pair output carries canonical ids.)
+#include "internal.hpp"
+
+namespace dpmd {
+namespace {
+
+struct ObsArgs {
+    const double* x; const double* y; const double* rad; const int* sid;
+    const double* Lx; const double* Ly; const double* iLx; const double* iLy; const double* e;
+    const int* nn; const int* nbr; int n;
+};
+
+ObsArgs obs_args(dpmd_ctx* c) {
+    ObsArgs a;
+    a.x = c->x.p; a.y = c->y.p; a.rad = c->rad.p; a.sid = c->sid.p;
+    a.Lx = c->Lx.p; a.Ly = c->Ly.p; a.iLx = c->iLx.p; a.iLy = c->iLy.p; a.e = c->e.p;
+    a.nn = c->nn.p; a.nbr = c->nbr.p; a.n = c->N;
+    return a;
+}
+
+// compute_contacts_kernel + compute_overlaps_kernel (disk.cu:434-471,663-700)
+__global__ void k_contacts(ObsArgs a, int* contacts, double* overlap, double* count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int s = a.sid[i];
+    const double Lx = a.Lx[s], Ly = a.Ly[s], iLx = a.iLx[s], iLy = a.iLy[s];
+    const double xi = a.x[i], yi = a.y[i], ri = a.rad[i];
+    const int cnt = a.nn[i];
+    int nc = 0;
+    double ov = 0.0, cc = 0.0;
+    for (int k = 0; k < cnt; ++k) {
+        const int j = a.nbr[(size_t)k * a.n + i];
+        const double dx = min_image(xi - a.x[j], Lx, iLx), dy = min_image(yi - a.y[j], Ly, iLy);
+        const double r2 = dx * dx + dy * dy;
+        const double radsum = ri + a.rad[j];
+        if (r2 >= radsum * radsum) continue;
+        ++nc;
+        ov += radsum - sqrt(r2);
+        cc += 1.0;
+    }
+    contacts[i] = nc;
+    overlap[i] = ov;
+    count[i] = cc;
+}
+
+// Per-system integer sum (compute_n_contacts_total, base_particle.hpp:653-659): one block
+// per system; integer addition makes the order irrelevant.
+__global__ void k_int_sys_sum(const int* off, const int* v, int* out) {
+    __shared__ int lds[DPMD_BLOCK];
+    const int s = blockIdx.x;
+    int acc = 0;
+    for (int i = off[s] + threadIdx.x; i < off[s + 1]; i += blockDim.x) acc += v[i];
+    lds[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s] = lds[0];
+}
+
+__global__ void k_invert(int n, const int* perm, int* inv) {   // inv[perm[c]] = c
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) inv[perm[c]] = c;
+}
+
+struct PairOut {
+    const int* start; const int* canon;
+    int* pi; int* pj; double* dist; double* fx; double* fy; double* h[8];
+};
+
+// compute_pair_dist_kernel, compute_particle_pair_forces_kernel and compute_hessian_kernel
+// (disk.cu:473-661) in one pass.  The Hessian terms are the reference's expressions with its
+// unit derivative factors substituted (each substitution is exact): for contact k of i with
+// j, d = (dx, dy) = x_i - x_j (minimum image), c = e, t = -e delta,
+//   H_ia,ib += c d_a d_b / r^2 + t (delta_ab / r - d_a d_b / (r^2 r)),
+//   H_ia,jb += c (-d_a d_b) / r^2 + t (-delta_ab / r - (-d_a d_b) / (r^2 r)).
+__global__ void k_pair_obs(ObsArgs a, PairOut o) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int s = a.sid[i];
+    const double e = a.e[s];
+    const double Lx = a.Lx[s], Ly = a.Ly[s], iLx = a.iLx[s], iLy = a.iLy[s];
+    const double xi = a.x[i], yi = a.y[i], ri = a.rad[i];
+    const int cnt = a.nn[i];
+    const int k0 = o.start[i];
+    const int ci = o.canon[i];
+    for (int q = 0; q < cnt; ++q) {
+        const int k = k0 + q;
+        const int j = a.nbr[(size_t)q * a.n + i];
+        const double dx = min_image(xi - a.x[j], Lx, iLx), dy = min_image(yi - a.y[j], Ly, iLy);
+        const double r2 = dx * dx + dy * dy;
+        if (o.pi) { o.pi[k] = ci; o.pj[k] = o.canon[j]; }
+        if (o.dist) o.dist[k] = sqrt(r2);
+        const double radsum = ri + a.rad[j];
+        double fxk = 0.0, fyk = 0.0;
+        double h[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (r2 < radsum * radsum) {
+            const double r = sqrt(r2);
+            const double inv = 1.0 / r;
+            const double nx = dx * inv, ny = dy * inv;
+            const double delta = radsum - r;
+            const double fmag = e * delta;
+            // the reference adds each term into zero-filled storage: 0.0 + v
+            fxk = 0.0 + fmag * nx;
+            fyk = 0.0 + fmag * ny;
+            if (o.h[0]) {
+                const double t = -e * delta, cc = e;
+                const double dd[2] = {dx, dy};
+                for (int p = 0; p < 2; ++p)
+                    for (int b = 0; b < 2; ++b) {
+                        const double dib = p == b ? 1.0 : 0.0;    // d(q_ib)/d(x_ia)
+                        const double djb = p == b ? -1.0 : 0.0;   // d(q_jb)/d(x_ia)
+                        const double qq = dd[p] * dd[b];
+                        const double qj = dd[p] * -dd[b];
+                        h[(p * 2 + b) * 2 + 0] = 0.0 + (0.0 + (cc * qq / r2 + t * (dib / r - qq / (r2 * r))));
+                        h[(p * 2 + b) * 2 + 1] = 0.0 + (0.0 + (cc * qj / r2 + t * (djb / r - qj / (r2 * r))));
+                    }
+            }
+        }
+        if (o.fx) { o.fx[k] = fxk; o.fy[k] = fyk; }
+        if (o.h[0])
+            for (int m = 0; m < 8; ++m) o.h[m][k] = h[m];
+    }
+}
+
+}  // namespace
+
+void disk_contacts(dpmd_ctx* c) {
+    k_contacts<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(obs_args(c), c->contacts.p, c->overlap.p, c->ccount.p);
+    HIP_OK(hipGetLastError());
+}
+
+void contacts_total(dpmd_ctx* c) {
+    k_int_sys_sum<<<c->S, DPMD_BLOCK, 0, c->st>>>(c->off.p, c->contacts.p, c->ncontacts.p);
+    HIP_OK(hipGetLastError());
+}
+
+// CSR starts of the current lists (host scan of the counts; the observables are off the
+// step path) and the current -> canonical map.
+int64_t pair_setup(dpmd_ctx* c) {
+    const int N = c->N;
+    std::vector<int> cnt = dl(c, c->nn);
+    std::vector<int> st(N + 1, 0);
+    int64_t tot = 0;
+    for (int i = 0; i < N; ++i) { st[i] = (int)tot; tot += cnt[i]; }
+    need(tot <= 0x7fffffff, "pair observables: more than 2^31 neighbor entries");
+    st[N] = (int)tot;
+    c->pair_start.upload(st.data(), N + 1, c->st);
+    c->canon.resize(N);
+    k_invert<<<blocks_for(N), DPMD_BLOCK, 0, c->st>>>(N, c->static_index.p, c->canon.p);
+    HIP_OK(hipGetLastError());
+    return tot;
+}
+
+void disk_pair_obs(dpmd_ctx* c, int* pi, int* pj, double* dist, double* fx, double* fy, double* const* h) {
+    PairOut o;
+    o.start = c->pair_start.p; o.canon = c->canon.p;
+    o.pi = pi; o.pj = pj; o.dist = dist; o.fx = fx; o.fy = fy;
+    for (int m = 0; m < 8; ++m) o.h[m] = h ? h[m] : nullptr;
+    k_pair_obs<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(obs_args(c), o);
+    HIP_OK(hipGetLastError());
+}
+
+}  // namespace dpmd

@@ -166,6 +166,33 @@ class _Particles:
     def update_velocities(self, scale, scale2: float = 1.0):
         self._c(lib().dpmd_update_velocities(self.ctx, self._ps(scale).ptr, scale2))
 
+    # ---- contact observables (base_particle.hpp:649-702; disk.cu:434-700) -------------------
+    def compute_contacts(self): self._c(lib().dpmd_compute_contacts(self.ctx))
+    def compute_overlaps(self): self._c(lib().dpmd_compute_overlaps(self.ctx))
+    def compute_n_contacts_total(self): self._c(lib().dpmd_compute_n_contacts_total(self.ctx))
+
+    def pair_observables(self, ids: bool = True, dist: bool = True, forces: bool = True, hessian: bool = True):
+        """compute_pair_dist / compute_pair_forces / compute_hessian in one pass.  Returns a dict
+        of host arrays indexed by neighbor entry (CSR order of neighbors_csr()): pair_i, pair_j
+        (canonical ids), dist, fx, fy and the Hessian columns h_ix_ix, h_ix_jx, h_ix_iy, h_ix_jy,
+        h_iy_ix, h_iy_jx, h_iy_iy, h_iy_jy."""
+        n = self.n_neighbors()
+        bufs = {}
+        def dev(name, dtype=np.float64):
+            bufs[name] = DeviceArray(np.zeros(max(1, n), dtype), dtype)
+            return bufs[name].ptr
+        pi = dev("pair_i", np.int32) if ids else None
+        pj = dev("pair_j", np.int32) if ids else None
+        d = dev("dist") if dist else None
+        fx = dev("fx") if forces else None
+        fy = dev("fy") if forces else None
+        h = None
+        if hessian:
+            names = ("h_ix_ix", "h_ix_jx", "h_ix_iy", "h_ix_jy", "h_iy_ix", "h_iy_jx", "h_iy_iy", "h_iy_jy")
+            h = (C.c_void_p * 8)(*[dev(k) for k in names])
+        self._c(lib().dpmd_compute_pair_observables(self.ctx, pi, pj, d, fx, fy, h, n))
+        return {k: v.to_host()[:n] for k, v in bufs.items()}
+
     def compute_damping_forces(self, scale):
         self._c(lib().dpmd_compute_damping_forces(self.ctx, self._ps(scale).ptr))
 

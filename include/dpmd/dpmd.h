@@ -65,6 +65,7 @@ enum dpmd_field {
     DPMD_F_PE, DPMD_F_KE, DPMD_F_RAD, DPMD_F_MASS, DPMD_F_AREA, DPMD_F_LAST_POS_X, DPMD_F_LAST_POS_Y,
     DPMD_F_ANGLE, DPMD_F_ANGULAR_VEL, DPMD_F_TORQUE, DPMD_F_MOMENT_INERTIA,
     DPMD_F_STRESS_XX, DPMD_F_STRESS_XY, DPMD_F_STRESS_YX, DPMD_F_STRESS_YY,
+    DPMD_F_OVERLAP, DPMD_F_CONTACT_COUNT,   /* overlaps (N,2): total overlap, contact count */
     /* double, S */
     DPMD_F_BOX_SIZE_X = 32, DPMD_F_BOX_SIZE_Y, DPMD_F_BOX_INV_X, DPMD_F_BOX_INV_Y,
     DPMD_F_E_INTERACTION, DPMD_F_NEIGHBOR_CUTOFF, DPMD_F_THRESH2,
@@ -76,8 +77,9 @@ enum dpmd_field {
     DPMD_F_CELL_SYSTEM_START /* S+1 */, DPMD_F_BOUNDARY, DPMD_F_N_DOF, DPMD_F_REBUILD_FLAG,
     DPMD_F_FIRE_STEPS /* steps that did work per system in the last dpmd_fire_minimize /
                          dpmd_jam_compress: a system stops costing work one step after it stops */,
+    DPMD_F_N_CONTACTS_TOTAL,
     /* int, P (or V for rigid bodies: the neighbor / cell objects are vertices) */
-    DPMD_F_SYSTEM_ID = 80, DPMD_F_STATIC_INDEX, DPMD_F_CELL_ID, DPMD_F_NEIGHBOR_COUNT,
+    DPMD_F_SYSTEM_ID = 80, DPMD_F_STATIC_INDEX, DPMD_F_CELL_ID, DPMD_F_NEIGHBOR_COUNT, DPMD_F_CONTACTS,
     /* rigid bumpy (base_poly_particle.hpp:104-120) */
     DPMD_F_VERTEX_POS_X = 96, DPMD_F_VERTEX_POS_Y, DPMD_F_VERTEX_FORCE_X, DPMD_F_VERTEX_FORCE_Y,
     DPMD_F_VERTEX_PE, DPMD_F_VERTEX_RAD, DPMD_F_VERTEX_MASS,              /* double, V */
@@ -153,6 +155,27 @@ int dpmd_increment_packing_fraction(dpmd_ctx* ctx, const double* increment);
 /* save_state / restore_state (base_particle.hpp:708-736) */
 int dpmd_save_state(dpmd_ctx* ctx, const int* flag, int true_val);
 int dpmd_restore_state(dpmd_ctx* ctx, const int* flag, int true_val);
+/* ---- contact observables (disks) ---------------------------------------------------- */
+/* compute_contacts + compute_overlaps (base_particle.hpp:649-651,698-699; disk.cu:434-471,
+ * 663-700): per particle the number of contacts (DPMD_F_CONTACTS), the summed overlap
+ * (DPMD_F_OVERLAP) and the contact count as a double (DPMD_F_CONTACT_COUNT), over the Verlet
+ * lists with the minimum image in every box.  compute_overlaps is the same call. */
+int dpmd_compute_contacts(dpmd_ctx* ctx);
+int dpmd_compute_overlaps(dpmd_ctx* ctx);
+/* compute_n_contacts_total (base_particle.hpp:653-659): contacts, then their per-system sum
+ * (DPMD_F_N_CONTACTS_TOTAL). */
+int dpmd_compute_n_contacts_total(dpmd_ctx* ctx);
+/* compute_pair_dist / compute_pair_forces / compute_hessian (base_particle.hpp:662-702;
+ * disk.cu:473-661,976-1060) in one pass.  Outputs are DEVICE arrays of n entries, n = the
+ * number of neighbor entries (dpmd_neighbor_info's total), in the CSR order of
+ * dpmd_get_neighbors_csr; any group may be NULL.  pair_i, pair_j: canonical particle ids of
+ * entry k; dist: |x_i - x_j| (minimum image); fx, fy: the contact force on i from j (0 when
+ * not in contact); hessian: NULL or a HOST array of 8 device pointers, the reference's
+ * hessian_xx/xy/yx/yy (N_neighbors, 2) columns in the order H_ix_ix, H_ix_jx, H_ix_iy, H_ix_jy,
+ * H_iy_ix, H_iy_jx, H_iy_iy, H_iy_jy (0 when not in contact): entry k holds contact k's share
+ * of the diagonal block of i and the off-diagonal block (i, j). */
+int dpmd_compute_pair_observables(dpmd_ctx* ctx, int* pair_i, int* pair_j, double* dist, double* fx, double* fy,
+                                  double* const* hessian, int64_t n);
 /* compute_damping_forces (base_particle.hpp:386; disk.cu:249-263, rigid_bumpy.cu:310-327):
  * F_i += -v_i * scale[s] (rigid bodies also torque_i += -omega_i * scale[s]). */
 int dpmd_compute_damping_forces(dpmd_ctx* ctx, const double* scale);

@@ -192,6 +192,50 @@ public:
     static constexpr int kKind = DPMD_DISK;
     explicit Disk(void* hip_stream = nullptr) : ParticleBase<Disk>(kKind, hip_stream) {}
     F1 rad{ctx_, DPMD_F_RAD};
+
+    // ---- contact observables (base_particle.hpp:39-44,85,101-103,649-702; disk.cu:434-700) --
+    I1 contacts{ctx_, DPMD_F_CONTACTS};
+    F2 overlaps{ctx_, DPMD_F_OVERLAP, DPMD_F_CONTACT_COUNT};
+    I1 n_contacts_total{ctx_, DPMD_F_N_CONTACTS_TOTAL};
+    // per neighbor entry (CSR order of neighbor_lists), sized on each compute call
+    df::DeviceField2D<int> pair_ids;
+    F1 pair_dist;
+    F2 pair_forces;
+    F2 hessian_xx, hessian_xy, hessian_yx, hessian_yy;
+
+    void compute_contacts() { ok(dpmd_compute_contacts(ctx_)); }
+    void compute_overlaps() { ok(dpmd_compute_overlaps(ctx_)); }
+    void compute_n_contacts_total() { ok(dpmd_compute_n_contacts_total(ctx_)); }
+    void compute_pair_dist() {
+        const int n = size_pairs();
+        pair_dist.resize(n);
+        ok(dpmd_compute_pair_observables(ctx_, pair_ids.xptr(), pair_ids.yptr(), pair_dist.ptr(), nullptr, nullptr,
+                                         nullptr, n));
+    }
+    void compute_pair_forces() {
+        const int n = size_pairs();
+        pair_forces.resize(n);
+        ok(dpmd_compute_pair_observables(ctx_, pair_ids.xptr(), pair_ids.yptr(), nullptr, pair_forces.xptr(),
+                                         pair_forces.yptr(), nullptr, n));
+    }
+    void compute_hessian() {
+        const int n = size_pairs();
+        F2* h[] = {&hessian_xx, &hessian_xy, &hessian_yx, &hessian_yy};
+        double* cols[8];
+        for (int m = 0; m < 4; ++m) {
+            h[m]->resize(n);
+            cols[2 * m] = h[m]->xptr();
+            cols[2 * m + 1] = h[m]->yptr();
+        }
+        ok(dpmd_compute_pair_observables(ctx_, pair_ids.xptr(), pair_ids.yptr(), nullptr, nullptr, nullptr, cols, n));
+    }
+
+private:
+    int size_pairs() {
+        const int n = n_neighbors();
+        pair_ids.resize(n);
+        return n;
+    }
 };
 }  // namespace disk
 

@@ -1225,6 +1225,78 @@ void or_disk_get_neighbors(void* h, int* start, int* ids) {
     for (int i = 0; i < d->N; ++i) { start[i] = w; for (int j : d->nbr[i]) ids[w++] = j; }
     start[d->N] = w;
 }
+// compute_contacts_kernel / compute_overlaps_kernel (disk.cu:434-471,663-700): contacts[i],
+// summed overlap and contact count over the Verlet list, minimum image in every box.
+void or_disk_contacts(void* h, int* contacts, double* overlap, double* count) {
+    Disk* d = (Disk*)h;
+    for (int i = 0; i < d->N; ++i) {
+        const int s = d->sid[i];
+        int nc = 0;
+        double ov = 0.0, cc = 0.0;
+        for (int j : d->nbr[i]) {
+            double dx, dy;
+            const double r2 = disp_pbc_L(d->x[i], d->y[i], d->x[j], d->y[j], d->Lx[s], d->Ly[s], d->ix[s], d->iy[s], dx, dy);
+            const double radsum = d->rad[i] + d->rad[j];
+            if (r2 >= radsum * radsum) continue;
+            ++nc; ov += radsum - std::sqrt(r2); cc += 1.0;
+        }
+        contacts[i] = nc; overlap[i] = ov; count[i] = cc;
+    }
+}
+// compute_pair_dist / compute_particle_pair_forces / compute_hessian kernels (disk.cu:473-661)
+// per CSR entry of the current lists, canonical ids.  The Hessian loop is the reference's,
+// derivative factors and all, accumulated into zeroed storage.
+void or_disk_pair_obs(void* h, int* pi, int* pj, double* dist, double* fx, double* fy, double* hs) {
+    Disk* d = (Disk*)h;
+    std::vector<int> canon(d->N);
+    for (int c = 0; c < d->N; ++c) canon[d->static_index[c]] = c;
+    long k = 0, tot = 0;
+    for (auto& L : d->nbr) tot += (long)L.size();
+    for (int i = 0; i < d->N; ++i) {
+        const int s = d->sid[i];
+        const double e = d->e[s];
+        for (int j : d->nbr[i]) {
+            double dx, dy;
+            const double r2 = disp_pbc_L(d->x[i], d->y[i], d->x[j], d->y[j], d->Lx[s], d->Ly[s], d->ix[s], d->iy[s], dx, dy);
+            pi[k] = canon[i]; pj[k] = canon[j];
+            dist[k] = std::sqrt(r2);
+            fx[k] = 0.0; fy[k] = 0.0;
+            for (int m = 0; m < 8; ++m) hs[m * tot + k] = 0.0;
+            const double radsum = d->rad[i] + d->rad[j];
+            if (r2 < radsum * radsum) {
+                const double r = std::sqrt(r2);
+                const double inv = 1.0 / r;
+                const double nx = dx * inv, ny = dy * inv;
+                const double delta = radsum - r;
+                const double fmag = e * delta;
+                fx[k] += fmag * nx;
+                fy[k] += fmag * ny;
+                const double t_ij = -e * delta, c_ij = e;
+                double hib[4] = {0.0, 0.0, 0.0, 0.0}, hjb[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int a = 0; a < 2; ++a)
+                    for (int b = 0; b < 2; ++b) {
+                        const double d_ia_dx = 1 * (a == 0), d_ia_dy = 1 * (a == 1);
+                        const double d_jb_dx = -1 * (b == 0), d_jb_dy = -1 * (b == 1);
+                        const double q_ia = dx * d_ia_dx + dy * d_ia_dy;
+                        const double q_jb = dx * d_jb_dx + dy * d_jb_dy;
+                        const double d_ia_q_jb = d_ia_dx * d_jb_dx + dx * 0.0 + d_ia_dy * d_jb_dy + dy * 0.0;
+                        hjb[a * 2 + b] += c_ij * (q_ia * q_jb) / (r2) + t_ij * (d_ia_q_jb / r - (q_ia * q_jb) / (r2 * r));
+                        const double d_ib_dx = 1 * (b == 0), d_ib_dy = 1 * (b == 1);
+                        const double q_ib = dx * d_ib_dx + dy * d_ib_dy;
+                        const double d_ia_q_ib = d_ia_dx * d_ib_dx + dx * 0.0 + d_ia_dy * d_ib_dy + dy * 0.0;
+                        hib[a * 2 + b] += c_ij * (q_ia * q_ib) / (r2) + t_ij * (d_ia_q_ib / r - (q_ia * q_ib) / (r2 * r));
+                    }
+                // columns: ix_ix, ix_jx, ix_iy, ix_jy, iy_ix, iy_jx, iy_iy, iy_jy
+                for (int ab = 0; ab < 4; ++ab) {
+                    hs[(2 * ab) * tot + k] += hib[ab];
+                    hs[(2 * ab + 1) * tot + k] += hjb[ab];
+                }
+            }
+            ++k;
+        }
+    }
+}
+
 void or_disk_flags(void* h, unsigned* out) { Disk* d = (Disk*)h; for (int s = 0; s < d->S; ++s) out[s] = d->flag[s]; }
 void or_disk_check_neighbors(void* h, int force) { ((Disk*)h)->check_neighbors(force != 0); }
 void or_disk_compute_forces(void* h, int mode) { ((Disk*)h)->compute_forces(mode); }

@@ -55,6 +55,8 @@ _PROTOS = {
     "or_disk_fire_init": (None, [vp, vp, i32]),
     "or_disk_damped_vv_init": (None, [vp, vp, vp, i32]),
     "or_disk_jam_bisect": (i32, [vp, vp, i32, i32, f64, f64, f64, f64, i32, vp, vp]),
+    "or_disk_contacts": (None, [vp, vp, vp, vp]),
+    "or_disk_pair_obs": (None, [vp, vp, vp, vp, vp, vp, vp]),
     "or_rigid_jam_bisect": (i32, [vp, vp, i32, i32, f64, f64, f64, f64, i32, vp, vp]),
     "or_rigid_packing_fraction": (None, [vp, vp]),
     "or_rigid_increment_packing_fraction": (None, [vp, vp]),
@@ -303,6 +305,26 @@ class OracleDisk:
 
     def damped_vv_init(self, dt, damping, mode=0):
         a, d = self._ps(dt), self._ps(damping); lib().or_disk_damped_vv_init(self.h, _p(a), _p(d), mode)
+
+    def contacts(self):
+        """(contacts[N], overlap[N], contact_count[N]) in current order."""
+        c = np.zeros(self.N, np.int32); ov = np.zeros(self.N); cc = np.zeros(self.N)
+        lib().or_disk_contacts(self.h, _p(c), _p(ov), _p(cc))
+        return c, ov, cc
+
+    def pair_observables(self):
+        """Per CSR entry of the current lists: pair ids (canonical), dist, fx, fy and the 8
+        Hessian columns (same names as dpmd_amd.system's pair_observables)."""
+        n = lib().or_disk_n_neighbors(self.h)
+        pi = np.zeros(max(1, n), np.int32); pj = np.zeros(max(1, n), np.int32)
+        d, fx, fy = np.zeros(max(1, n)), np.zeros(max(1, n)), np.zeros(max(1, n))
+        hs = np.zeros(8 * max(1, n))
+        lib().or_disk_pair_obs(self.h, _p(pi), _p(pj), _p(d), _p(fx), _p(fy), _p(hs))
+        out = dict(pair_i=pi[:n], pair_j=pj[:n], dist=d[:n], fx=fx[:n], fy=fy[:n])
+        names = ("h_ix_ix", "h_ix_jx", "h_ix_iy", "h_ix_jy", "h_iy_ix", "h_iy_jx", "h_iy_iy", "h_iy_jy")
+        for m, k in enumerate(names):
+            out[k] = hs[m * n:(m + 1) * n].copy()
+        return out
 
     def jam_bisect(self, dt, max_comp, max_min, pe_target, pe_diff_target, dphi, phi_tol, mode=0):
         """jam_binary_search_pbc / _wall; returns (action[S], phi[S], rounds)."""

@@ -77,6 +77,16 @@ int main(int argc, char** argv) {
     df::DeviceField1D<double> jdt(S);
     jdt.fill(1e-2);
     const int rounds = md::routines::jam_binary_search_pbc(P, jdt, 3, 500, 1e-10, 1e-10, 5e-3, 1e-4);
+    P.compute_n_contacts_total();
+    P.compute_pair_dist();
+    P.compute_pair_forces();
+    P.compute_hessian();
+    std::vector<int> nct;
+    P.n_contacts_total.to_host(nct);
+    std::vector<double> hxx;
+    P.hessian_xx.x.to_host(hxx);
+    double trace = 0.0;
+    for (double v : hxx) trace += v;
     P.compute_pe_total();
     P.compute_ke_total();
     P.compute_pressure();
@@ -98,7 +108,8 @@ int main(int argc, char** argv) {
     };
     put(x, true); put(y, true); put(vx, true); put(vy, true);
     put(pe0, false); put(pe, false); put(ke, false); put(pr, false);
-    std::cout << "facade_test: fire_steps=" << steps << " jam_rounds=" << rounds << " n_neighbors=" << P.n_neighbors()
+    std::cout << "facade_test: fire_steps=" << steps << " jam_rounds=" << rounds << " contacts=" << nct[0] << ","
+              << nct[1] << " hessian_xx_sum=" << trace << " n_neighbors=" << P.n_neighbors()
               << " n_cells=" << P.n_cells() << " pe=" << pe[0] << "," << pe[1] << std::endl;
     return 0;
 }
