@@ -434,11 +434,64 @@ hipGraphExec_t capture_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, 
 
 }  // namespace
 
+namespace {
+
+// Kernel arguments a captured step batch bakes in: the run's parameters, the execution
+// options and the device buffers its launches touch.  A cached batch is replayed only while
+// all of them are unchanged.
+std::vector<int64_t> batch_key(dpmd_ctx* c, bool fire, int pred, double a, double b) {
+    auto P = [](const void* p) { return (int64_t)(intptr_t)p; };
+    int64_t ab[2];
+    std::memcpy(&ab[0], &a, 8);
+    std::memcpy(&ab[1], &b, 8);
+    return {fire ? 1 : 0, pred, ab[0], ab[1], c->block, c->fuse_vv ? 1 : 0, c->vv_mode, c->fire_mode,
+            c->mask_frozen ? 1 : 0, c->N, c->Nv, c->S, c->K, c->method, c->nchunks, P(c->rec_seg), P(vv_damping(c)),
+            P(c->x.p), P(c->y.p), P(c->x2.p), P(c->y2.p), P(c->vx.p), P(c->vy.p), P(c->fx.p), P(c->fy.p), P(c->pe.p),
+            P(c->rad.p), P(c->mass.p), P(c->lx.p), P(c->ly.p), P(c->nn.p), P(c->nbr.p), P(c->run.p), P(c->flag.p),
+            P(c->vv_dt.p), P(c->vv_dth.p), P(c->f_dt.p), P(c->f_frozen.p), P(c->part.p), P(c->chunk_sys.p),
+            P(c->static_index.p), P(c->angle.p), P(c->avel.p), P(c->torque.p), P(c->vpx.p), P(c->vpy.p),
+            P(c->vfx.p), P(c->vfy.p), P(c->vpe.p), P(c->sid.p), P(c->off.p), P(c->Lx.p), P(c->thresh2.p)};
+}
+
+void drop_cached_graphs(dpmd_ctx* c) {
+    for (auto& g : c->gcache) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    c->gcache_key.clear();
+}
+
+// Keeps captured step batches across the device-driven runs of one routine call (segments of
+// a checkpointed run, the replays of a reverse sweep, the chunks of a thermostatted run).
+struct GraphScope {
+    dpmd_ctx* c;
+    bool outer;
+    explicit GraphScope(dpmd_ctx* ctx) : c(ctx), outer(!ctx->gcache_on) { c->gcache_on = true; }
+    ~GraphScope() {
+        if (outer) {
+            drop_cached_graphs(c);
+            c->gcache_on = false;
+        }
+    }
+};
+
+}  // namespace
+
 // Runs up to max_steps steps as graph-captured batches of per-step launches; returns the
 // number taken and (FIRE) whether all systems stopped.
 int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, double a, double b, bool use_graph,
                            int* conv) {
-    std::vector<hipGraphExec_t> graphs(8, nullptr);   // batch sizes 1, 2, 4, ..., 128
+    hipGraphExec_t local[8] = {};   // batch sizes 1, 2, 4, ..., 128
+    hipGraphExec_t* graphs = local;
+    const bool cached = use_graph && c->gcache_on;
+    if (cached) {
+        std::vector<int64_t> key = batch_key(c, fire, pred, a, b);
+        if (key != c->gcache_key) {
+            drop_cached_graphs(c);
+            c->gcache_key = std::move(key);
+        }
+        graphs = c->gcache;
+    }
     int total = 0, m = 8, since_halt = 0;
     bool done = false;
     try {
@@ -497,10 +550,12 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
             }
         }
     } catch (...) {
-        for (auto g : graphs) if (g) (void)hipGraphExecDestroy(g);
+        if (cached) drop_cached_graphs(c);
+        else for (auto g : local) if (g) (void)hipGraphExecDestroy(g);
         throw;
     }
-    for (auto g : graphs) if (g) HIP_OK(hipGraphExecDestroy(g));
+    if (!cached)
+        for (auto g : local) if (g) HIP_OK(hipGraphExecDestroy(g));
     if (conv) *conv = (fire && read_run(c).running <= 0) ? 1 : 0;
     return total;
 }
@@ -554,6 +609,7 @@ void nvt_rescale_impl(dpmd_ctx* c, int n, int step0, const double* inc, int cf, 
     need(n >= 0 && step0 >= 0 && cf >= 0 && tf >= 0, "nvt_rescale_run: negative step counts");
     need(cf == 0 || inc != nullptr, "nvt_rescale_run: compress_every > 0 needs phi_increment");
     need(tf == 0 || T != nullptr, "nvt_rescale_run: temperature_every > 0 needs temperature_target");
+    GraphScope graphs(c);
     auto compress_at = [&](int k) { return cf > 0 && k % cf == 0; };
     auto thermo_at = [&](int k) { return tf > 0 && k % tf == 0; };
     int i = 0;
@@ -607,6 +663,7 @@ void vv_checkpointed_impl(dpmd_ctx* c, int n, int every, bool use_graph) {
     need(n >= 0 && every > 0, "vv_run_checkpointed: n_steps >= 0 and every > 0 required");
     const int N = c->N;
     const int J = (n + every - 1) / every;
+    GraphScope graphs(c);
     c->ck_state.resize((int64_t)(J + 1) * 4 * N);
     snapshot(c, c->ck_state.p);
     for (int j = 0; j < J; ++j) {
@@ -629,13 +686,15 @@ void vv_backprop_impl(dpmd_ctx* c, const double* gx, const double* gy, const dou
     need_disk_cells(c);
     need(!c->vv_damped, "vv_backprop: the reverse mode covers undamped velocity Verlet");
     need_state(c->ck_count > 0, "vv_backprop: run vv_run_checkpointed first");
+    GraphScope graphs(c);
     const int N = c->N, S = c->S, n = c->ck_steps, every = c->ck_every, J = c->ck_count - 1;
     const int mode = c->vv_mode;
     hipStream_t st = c->st;
     c->adj_t0.resize(2 * (int64_t)N); c->adj_t1.resize(2 * (int64_t)N); c->adj_t2.resize(2 * (int64_t)N);
     c->adj_t3.resize(2 * (int64_t)N); c->adj_t4.resize(N); c->adj_t5.resize(N); c->adj_bx.resize(N);
+    c->adj_t6.resize(2 * (int64_t)N);
     c->adj_u0.resize(S);
-    double *xb = c->adj_t0.p, *vb = c->adj_t1.p, *w_c = c->adj_t2.p, *z = c->adj_t3.p;
+    double *xb = c->adj_t0.p, *vb = c->adj_t1.p, *w_c = c->adj_t2.p, *w_n = c->adj_t6.p;
     double *rad_c = c->adj_t4.p, *m_c = c->adj_t5.p;
     auto in = [&](const double* src, double* dst) {
         if (src) HIP_OK(hipMemcpyAsync(dst, src, (size_t)N * 8, hipMemcpyDeviceToDevice, st));
@@ -705,16 +764,18 @@ void vv_backprop_impl(dpmd_ctx* c, const double* gx, const double* gy, const dou
                 c->cutoff.upload(cut0.data(), S, st);
                 setup_cells(c, CELLS_BOX_CHANGED);
             }
+            // Each launch is one reverse step (Hessian product, xbar update) plus the next
+            // step's vbar and w, which need only per-particle data; w ping-pongs between
+            // two buffers because the product reads the neighbors' w.
             if (first) {
                 adj_rev_w(c, m_c, vb, w_c);
-                adj_hvp_segment(c, mode, hi - k0, rad_c, w_c, z);
-                adj_rev_x(c, 0.5, z, xb);
+                adj_rev_step(c, mode, hi - k0, rad_c, m_c, w_c, 0.5, xb, vb, w_n, n > 0);
+                std::swap(w_c, w_n);
                 first = false;
             }
             for (int k = hi - 1; k >= a; --k) {
-                adj_rev_vw(c, m_c, xb, vb, w_c);
-                adj_hvp_segment(c, mode, k - k0, rad_c, w_c, z);
-                adj_rev_x(c, k > 0 ? 1.0 : 0.5, z, xb);
+                adj_rev_step(c, mode, k - k0, rad_c, m_c, w_c, k > 0 ? 1.0 : 0.5, xb, vb, w_n, k > 0);
+                std::swap(w_c, w_n);
             }
             hi = a;
         }

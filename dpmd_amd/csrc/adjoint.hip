@@ -411,6 +411,47 @@ __global__ void k_rev_x(int n, const int* sid, const double* dt, double coef, co
     xby[i] -= h * zy[i];
 }
 
+// One whole reverse velocity-Verlet step in one launch: z = H(x_slot) w (k_hvp's sum, no mask,
+// no shift), xbar -= coef dt_s z (k_rev_x), and, when next is set, the following step's
+// vbar += dt_s xbar, w' = vbar / m (k_rev_vw) into the other w buffer.  Each expression is the
+// one of the three separate kernels, so the results are the same bits.
+template <int MODE>
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_rev_step(AdjGeom g, const double* __restrict__ wx, const double* __restrict__ wy, const double* __restrict__ dt,
+           double coef, const double* __restrict__ m, double* xbx, double* xby, double* vbx, double* vby, double* wox,
+           double* woy, int next) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    const int s = g.sid[i];
+    const bool wall = walled<MODE>(g, s);
+    const double e = g.e[s];
+    const double wxi = wx[i], wyi = wy[i];
+    double ax = 0.0, ay = 0.0;
+    if (wall) {
+        const double xi = g.x[i], yi = g.y[i], ri = g.rad[i];
+        if (xi < ri) ax += e * wxi;
+        if (xi > g.Lx[s] - ri) ax += e * wxi;
+        if (yi < ri) ay += e * wyi;
+        if (yi > g.Ly[s] - ri) ay += e * wyi;
+    }
+    for_contacts(g, i, s, wall, [&](int j, double, double, double nx, double ny, double d, double delta) {
+        double kx, ky;
+        pair_block(e, nx, ny, d, delta, wxi - wx[j], wyi - wy[j], kx, ky);
+        ax += kx;
+        ay += ky;
+    });
+    const double zx = ax + 0.0 * wxi, zy = ay + 0.0 * wyi;
+    const double h = coef * dt[s];
+    const double xbx_i = xbx[i] - h * zx, xby_i = xby[i] - h * zy;
+    xbx[i] = xbx_i; xby[i] = xby_i;
+    if (next) {
+        const double hh = dt[s];
+        const double vx = vbx[i] + hh * xbx_i, vy = vby[i] + hh * xby_i;
+        vbx[i] = vx; vby[i] = vy;
+        wox[i] = vx / m[i]; woy[i] = vy / m[i];
+    }
+}
+
 // Canonical-order copy of the current contact list: nn_c[c] = nn[si[c]],
 // nbr_c[k n + c] = canonical index of nbr[k n + si[c]] (inv: current -> canonical).
 __global__ void k_inv_perm(int n, const int* si, int* inv) {
@@ -514,6 +555,22 @@ void adj_hvp_segment(dpmd_ctx* c, int mode, int slot, const double* rad_c, const
     g.nbr = c->ck_nbr.p;
     by_mode(mode, [&](auto m) {
         k_hvp<decltype(m)::value><<<blocks_for(N), DPMD_BLOCK, 0, c->st>>>(g, w, w + N, z, z + N, 0.0);
+    });
+    HIP_OK(hipGetLastError());
+}
+
+void adj_rev_step(dpmd_ctx* c, int mode, int slot, const double* rad_c, const double* m, const double* w, double coef,
+                  double* xb, double* vb, double* w_next, bool next) {
+    const int N = c->N;
+    AdjGeom g = geom_current(c);
+    g.x = c->ck_seg.p + (size_t)slot * 2 * N;
+    g.y = g.x + N;
+    g.rad = rad_c;
+    g.nn = c->ck_nn.p;
+    g.nbr = c->ck_nbr.p;
+    by_mode(mode, [&](auto md) {
+        k_rev_step<decltype(md)::value><<<blocks_for(N), DPMD_BLOCK, 0, c->st>>>(
+            g, w, w + N, c->vv_dt.p, coef, m, xb, xb + N, vb, vb + N, w_next, w_next + N, next ? 1 : 0);
     });
     HIP_OK(hipGetLastError());
 }

@@ -166,6 +166,11 @@ struct dpmd_ctx {
     dpmd::DVec<double> scr_a, scr_b, scr_c;
     // graphs
     hipGraphExec_t g_fire = nullptr, g_vv = nullptr;
+    // Step batches kept across the device-driven runs of one routine call (GraphScope in
+    // abi.hip); key = everything the captured launches take from the context.
+    bool gcache_on = false;
+    std::vector<int64_t> gcache_key;
+    hipGraphExec_t gcache[8] = {};
     int g_fire_key[4] = {0, 0, 0, 0};
     double g_fire_keyd[2] = {0, 0};
     int g_vv_key = -1;
@@ -175,7 +180,7 @@ struct dpmd_ctx {
     int coop_blocks = 0;
     // adjoints: scratch, CG vectors (current order), NVE checkpoints (canonical order)
     dpmd::DVec<double> adj_bx, adj_by, adj_sa, adj_sb, adj_sc, adj_sd;
-    dpmd::DVec<double> adj_t0, adj_t1, adj_t2, adj_t3, adj_t4, adj_t5;
+    dpmd::DVec<double> adj_t0, adj_t1, adj_t2, adj_t3, adj_t4, adj_t5, adj_t6;
     dpmd::DVec<double> adj_u0, adj_u1, adj_u2, adj_u3, adj_u4;   // per-system scratch
     dpmd::DVec<double> cg_rx, cg_ry, cg_px, cg_py, cg_qx, cg_qy;
     dpmd::DVec<double> cg_rr, cg_rrn, cg_pq, cg_bb, cg_beta;
@@ -298,6 +303,8 @@ void adj_rev_vw(dpmd_ctx* c, const double* m, double* xb, double* vb, double* w)
 void adj_rev_w(dpmd_ctx* c, const double* m, const double* vb, double* w);
 void adj_rev_x(dpmd_ctx* c, double coef, const double* z, double* xb);
 void adj_hvp_segment(dpmd_ctx* c, int mode, int slot, const double* rad_c, const double* w, double* z);
+void adj_rev_step(dpmd_ctx* c, int mode, int slot, const double* rad_c, const double* m, const double* w, double coef,
+                  double* xb, double* vb, double* w_next, bool next);
 void adj_add(dpmd_ctx* c, int64_t n, const double* a, const double* b, double* out);
 void adj_axpy2(dpmd_ctx* c, double a, const double* xx, const double* xy, double* yx, double* yy);
 void adj_norm2(dpmd_ctx* c, const double* ax, const double* ay, double* out);

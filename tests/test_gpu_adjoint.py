@@ -165,10 +165,17 @@ def test_vv_backprop_matches_host_and_fd():
             VelocityVerlet(P, dt, mode=MODE).run(n)
         return canon(P, "POS_X"), canon(P, "POS_Y")
 
+    xP, yP = run(vx0, vy0)
     xT, yT = run(vx0, vy0, ck=True)
+    # checkpointed segments (step batches replayed across segments) = one plain run, bit for bit
+    assert np.array_equal(xT, xP) and np.array_equal(yT, yP)
     gx, gy = 2 * (xT - x0) / cnt[sid], 2 * (yT - y0) / cnt[sid]
     gx0, gy0, gvx, gvy = A.vv_backprop(P, gx, gy, None, None)
     assert np.array_equal(canon(P, "POS_X"), xT)          # the context is left at the final state
+    # graph-captured replays (cached across segments) = direct launches, bit for bit
+    direct = A.vv_backprop(P, gx, gy, None, None, use_graph=False)
+    for u, v in zip((gx0, gy0, gvx, gvy), direct):
+        assert np.array_equal(u, v)
     for s in range(b.S):
         sl = slice(int(b.off[s]), int(b.off[s + 1]))
         args = (x0[sl], y0[sl], vx0[sl], vy0[sl], b.rad[sl], b.mass[sl], b.Lx[s], b.Ly[s], b.e[s],
