@@ -363,13 +363,13 @@ RunCtl read_run(dpmd_ctx* c) {
     return RunCtl{run[RUN_RUNNING], run[RUN_HALT], run[RUN_HALT_AT], run[RUN_CONV_AT]};
 }
 
-// Fused velocity-Verlet batches (disks, even batch sizes): drift of step 0 into the second
-// position buffer B, then m - 1 launches that each finish step q (forces at x_q, closing half
-// kick) and start step q + 1 (opening half kick, drift into the other buffer), then the
-// forces and closing half kick of step m - 1.  x_q lives in B for even q and in the primary
-// buffer A for odd q, so an even batch ends with the positions back in A.  The rebuild flags
-// of x_q go to run word W(q) (RUN_ANY for even q, RUN_ANY2 for odd q) and gate the launch
-// that computes forces at x_q.
+// Fused velocity-Verlet batches (disks, even batch sizes): drift of step 0 from the
+// context's positions A into the interleaved buffer P0, then m - 1 launches that each finish
+// step q (forces at x_q, closing half kick) and start step q + 1 (opening half kick, drift
+// into the other buffer), then the forces and closing half kick of step m - 1, which also
+// write x_{m-1} back to A.  x_q lives in P(q mod 2).  The rebuild flags of x_q go to run word
+// W(q) (RUN_ANY for even q, RUN_ANY2 for odd q) and gate the launch that computes forces at
+// x_q; a halted batch copies x_q back to A before the host finishes the step.
 bool fused_vv_batch(dpmd_ctx* c, int m, bool fire) {
     return !fire && !rigid(c) && c->fuse_vv && !c->rec_seg && m >= 2 && m % 2 == 0;
 }
@@ -377,8 +377,8 @@ int any_word(int q) { return q % 2 == 0 ? RUN_ANY : RUN_ANY2; }
 
 void lean_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b) {
     if (fused_vv_batch(c, m, fire)) {
-        double *Ax = c->x.p, *Ay = c->y.p, *Bx = c->x2.p, *By = c->y2.p;
-        disk_vv_drift_to(c, Gate{nullptr, c->run.p, 0}, Bx, By, any_word(0));
+        double* P[2] = {c->xy0.p, c->xy1.p};
+        disk_vv_drift_to(c, Gate{nullptr, c->run.p, 0}, P[0], any_word(0));
         const bool prof = c->prof == 2 && !c->capturing;
         if (prof) {   // one pair around the batch's fused launches; kept only if no rebuild halts it
             HIP_OK(hipEventCreate(&c->pend_b)); HIP_OK(hipEventCreate(&c->pend_e));
@@ -388,13 +388,12 @@ void lean_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b) {
         for (int q = 0; q + 1 < m; ++q) {
             Gate g{nullptr, c->run.p, q};
             g.any = any_word(q);
-            const bool inB = q % 2 == 0;
-            disk_vv_fused(c, g, inB ? Bx : Ax, inB ? By : Ay, inB ? Ax : Bx, inB ? Ay : By, any_word(q + 1));
+            disk_vv_fused(c, g, P[q % 2], P[(q + 1) % 2], any_word(q + 1));
         }
         if (prof) HIP_OK(hipEventRecord(c->pend_e, c->st));
         Gate gf{nullptr, c->run.p, m - 1};
         gf.any = any_word(m - 1);
-        disk_vv_force(c, gf);
+        disk_vv_force_packed(c, gf, P[(m - 1) % 2]);
         return;
     }
     for (int k = 0; k < m; ++k) {
@@ -446,7 +445,7 @@ std::vector<int64_t> batch_key(dpmd_ctx* c, bool fire, int pred, double a, doubl
     std::memcpy(&ab[1], &b, 8);
     return {fire ? 1 : 0, pred, ab[0], ab[1], c->block, c->fuse_vv ? 1 : 0, c->vv_mode, c->fire_mode,
             c->mask_frozen ? 1 : 0, c->N, c->Nv, c->S, c->K, c->method, c->nchunks, P(c->rec_seg), P(vv_damping(c)),
-            P(c->x.p), P(c->y.p), P(c->x2.p), P(c->y2.p), P(c->vx.p), P(c->vy.p), P(c->fx.p), P(c->fy.p), P(c->pe.p),
+            P(c->x.p), P(c->y.p), P(c->xy0.p), P(c->xy1.p), P(c->vx.p), P(c->vy.p), P(c->fx.p), P(c->fy.p), P(c->pe.p),
             P(c->rad.p), P(c->mass.p), P(c->lx.p), P(c->ly.p), P(c->nn.p), P(c->nbr.p), P(c->run.p), P(c->flag.p),
             P(c->vv_dt.p), P(c->vv_dth.p), P(c->f_dt.p), P(c->f_frozen.p), P(c->part.p), P(c->chunk_sys.p),
             P(c->static_index.p), P(c->angle.p), P(c->avel.p), P(c->torque.p), P(c->vpx.p), P(c->vpy.p),
@@ -520,10 +519,8 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
             if (r.halt) {
                 total += r.halt_at + 1;
                 since_halt += r.halt_at;
-                if (fused && r.halt_at % 2 == 0) {   // the halted step's positions are in buffer B
-                    HIP_OK(hipMemcpyAsync(c->x.p, c->x2.p, (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->st));
-                    HIP_OK(hipMemcpyAsync(c->y.p, c->y2.p, (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->st));
-                }
+                if (fused)   // the halted step's positions are in the interleaved buffer P(halt_at mod 2)
+                    disk_unpack_positions(c, r.halt_at % 2 == 0 ? c->xy0.p : c->xy1.p);
                 // finish the halted step: rebuild, forces, bookkeeping
                 rebuild_neighbors(c, active_one(c), false);
                 if (fire) {
@@ -1173,7 +1170,7 @@ int dpmd_ctx_destroy(dpmd_ctx* c) {
                                 &c->ke_tot, &c->pw_tot, &c->press, &c->temp, &c->phi, &c->maxf2, &c->st_xx,
                                 &c->st_xy, &c->st_yx, &c->st_yy, &c->x, &c->y, &c->vx, &c->vy, &c->fx, &c->fy,
                                 &c->pe, &c->ke, &c->rad, &c->mass, &c->area, &c->lx, &c->ly, &c->angle, &c->avel,
-                                &c->torque, &c->inertia, &c->sxx, &c->sxy, &c->syx, &c->syy, &c->x2, &c->y2,
+                                &c->torque, &c->inertia, &c->sxx, &c->sxy, &c->syx, &c->syy, &c->xy0, &c->xy1,
                                 &c->vpx, &c->vpy,
                                 &c->vfx, &c->vfy, &c->vpe, &c->vrad, &c->vmass, &c->tmp_d, &c->part, &c->f_dt,
                                 &c->f_dtmax, &c->f_dtmin, &c->f_alpha, &c->f_peprev, &c->s_dt, &c->s_alpha,
@@ -1224,9 +1221,11 @@ int dpmd_allocate_particles(dpmd_ctx* c, int N) {
         c->N = N;
         dpmd::DVec<double>* dv[] = {&c->x, &c->y, &c->vx, &c->vy, &c->fx, &c->fy, &c->pe, &c->ke, &c->rad,
                                     &c->mass, &c->area, &c->lx, &c->ly, &c->angle, &c->avel, &c->torque,
-                                    &c->inertia, &c->sxx, &c->sxy, &c->syx, &c->syy, &c->x2, &c->y2,
+                                    &c->inertia, &c->sxx, &c->sxy, &c->syx, &c->syy,
                                     &c->overlap, &c->ccount};
         for (auto* v : dv) { v->resize(N); v->zero(c->st); }
+        c->xy0.resize(2 * (int64_t)N); c->xy0.zero(c->st);
+        c->xy1.resize(2 * (int64_t)N); c->xy1.zero(c->st);
         c->sid.resize(N); c->sid.zero(c->st);
         c->contacts.resize(N); c->contacts.zero(c->st);
         c->nvpp.resize(N); c->nvpp.zero(c->st);

@@ -306,8 +306,8 @@ k_vv_force(DiskArgs a, const double* __restrict__ dth, const double* __restrict_
 // positions to (xo, yo): the start of a fused run.
 template <int B>
 __global__ void __launch_bounds__(B)
-k_vv_drift_to(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, const double* __restrict__ dt,
-              const double* __restrict__ dth, Gate g) {
+k_vv_drift_to(DiskArgs a, double2* __restrict__ xo, const double* __restrict__ dt, const double* __restrict__ dth,
+              Gate g) {
     if (!gate_open(g)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < a.n;
@@ -320,7 +320,7 @@ k_vv_drift_to(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, cons
         const double sc = dt[s] * 1.0;
         double x = a.x[i], y = a.y[i];
         x += vx * sc; y += vy * sc;
-        xo[i] = x; yo[i] = y; a.vx[i] = vx; a.vy[i] = vy;
+        xo[i] = make_double2(x, y); a.vx[i] = vx; a.vy[i] = vy;
         if (a.cell) {
             const double dx = x - a.lx[i], dy = y - a.ly[i];
             flag = (dx * dx + dy * dy) > a.thresh2[s];
@@ -329,14 +329,15 @@ k_vv_drift_to(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, cons
     if (a.cell) raise_flags(a.flag, a.any, s, flag);
 }
 
-// Velocity Verlet across a step boundary in one launch: forces at the positions (a.x, a.y)
-// and the step's closing half kick, then the next step's opening half kick, its drift into
-// (xo, yo) and the skin test of the new positions.  Every force must see its neighbors' old
-// positions, hence the second buffer.  Each expression is the one of k_vv_force / k_vv_drift,
-// in the same order, so a fused run equals the two-launch run bit for bit.
+// Velocity Verlet across a step boundary in one launch: forces at the positions xs and the
+// step's closing half kick, then the next step's opening half kick, its drift into xo and
+// the skin test of the new positions.  Every force must see its neighbors' old positions,
+// hence two buffers; both hold interleaved (x, y) pairs, so each neighbor costs one 16-byte
+// gather.  Each expression is the one of k_vv_force / k_vv_drift, in the same order, so a
+// fused run equals the two-launch run bit for bit.
 template <int MODE, int B>
 __global__ void __launch_bounds__(B)
-k_vv_fused(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, const double* __restrict__ dt,
+k_vv_fused(DiskArgs a, const double2* __restrict__ xs, double2* __restrict__ xo, const double* __restrict__ dt,
            const double* __restrict__ dth, const double* __restrict__ damp, Gate g) {
     if (!gate_force(g)) return;
     const int i = xcd_remap(blockIdx.x, gridDim.x) * B + threadIdx.x;
@@ -348,7 +349,7 @@ k_vv_fused(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, const d
         // that a rebuild halts has its halted step's forces computed again, so fx, fy and pe
         // are not stored.
         double fx, fy, pe;
-        disk_force_mode<MODE>(a, i, s, fx, fy, pe);
+        disk_force_mode_pos<MODE>(a, PackedPos{xs}, i, s, fx, fy, pe);
         (void)pe;
         double vx = a.vx[i], vy = a.vy[i];
         if (damp) {   // damped velocity Verlet: the drag at the half-step velocity joins the force
@@ -362,15 +363,48 @@ k_vv_fused(DiskArgs a, double* __restrict__ xo, double* __restrict__ yo, const d
         vx = vx + fx * smi;                      // next step: update_velocities(dt_half)
         vy = vy + fy * smi;
         const double sc = dt[s] * 1.0;           // update_positions(dt)
-        double x = a.x[i], y = a.y[i];
+        const double2 p = xs[i];
+        double x = p.x, y = p.y;
         x += vx * sc; y += vy * sc;
-        xo[i] = x; yo[i] = y; a.vx[i] = vx; a.vy[i] = vy;
+        xo[i] = make_double2(x, y); a.vx[i] = vx; a.vy[i] = vy;
         if (a.cell) {
             const double dx = x - a.lx[i], dy = y - a.ly[i];
             flag = (dx * dx + dy * dy) > a.thresh2[s];
         }
     }
     if (a.cell) raise_flags(a.flag, a.any, s, flag);
+}
+
+// The last step of a fused batch: k_vv_force at the interleaved positions xs, which it also
+// writes back to the context's (x, y).
+template <int MODE, int B>
+__global__ void __launch_bounds__(B)
+k_vv_force_pk(DiskArgs a, const double2* __restrict__ xs, const double* __restrict__ dth,
+              const double* __restrict__ damp, Gate g) {
+    if (!gate_force(g)) return;
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * B + threadIdx.x;
+    if (i >= a.n) return;
+    const int s = a.sid[i];
+    double fx, fy, pe;
+    disk_force_mode_pos<MODE>(a, PackedPos{xs}, i, s, fx, fy, pe);
+    if (damp) {
+        const double d = damp[s];
+        fx += -a.vx[i] * d;
+        fy += -a.vy[i] * d;
+    }
+    const double2 p = xs[i];
+    a.x[i] = p.x; a.y[i] = p.y;
+    a.fx[i] = fx; a.fy[i] = fy; a.pe[i] = pe;
+    const double smi = dth[s] * 1.0 / a.mass[i];
+    a.vx[i] = a.vx[i] + fx * smi;
+    a.vy[i] = a.vy[i] + fy * smi;
+}
+
+__global__ void k_unpack(int n, const double2* __restrict__ xs, double* __restrict__ x, double* __restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 p = xs[i];
+    x[i] = p.x; y[i] = p.y;
 }
 
 // Workgroup size of the per-particle step kernels (the tunable block shape).
@@ -463,28 +497,44 @@ void disk_vv_drift(dpmd_ctx* c, Gate g) {
     HIP_OK(hipGetLastError());
 }
 
-void disk_vv_drift_to(dpmd_ctx* c, Gate g, double* xo, double* yo, int any_word) {
+void disk_vv_drift_to(dpmd_ctx* c, Gate g, double* xo, int any_word) {
     DiskArgs a = disk_args(c);
     a.any = c->run.p + any_word;
     by_block(c->block, [&](auto b) {
         constexpr int B = decltype(b)::value;
-        k_vv_drift_to<B><<<blocks_for(c->N, B), B, 0, c->st>>>(a, xo, yo, c->vv_dt.p, c->vv_dth.p, g);
+        k_vv_drift_to<B><<<blocks_for(c->N, B), B, 0, c->st>>>(a, (double2*)xo, c->vv_dt.p, c->vv_dth.p, g);
     });
     HIP_OK(hipGetLastError());
 }
 
-void disk_vv_fused(dpmd_ctx* c, Gate g, const double* xs, const double* ys, double* xo, double* yo, int any_word) {
+void disk_vv_fused(dpmd_ctx* c, Gate g, const double* xs, double* xo, int any_word) {
     DiskArgs a = disk_args(c);
-    a.x = const_cast<double*>(xs);
-    a.y = const_cast<double*>(ys);
     a.any = c->run.p + any_word;
     by_mode(c->vv_mode, [&](auto m) {
         by_block(c->block, [&](auto b) {
             constexpr int B = decltype(b)::value;
-            k_vv_fused<decltype(m)::value, B><<<blocks_for(c->N, B), B, 0, c->st>>>(a, xo, yo, c->vv_dt.p,
-                                                                                     c->vv_dth.p, vv_damping(c), g);
+            k_vv_fused<decltype(m)::value, B><<<blocks_for(c->N, B), B, 0, c->st>>>(
+                a, (const double2*)xs, (double2*)xo, c->vv_dt.p, c->vv_dth.p, vv_damping(c), g);
         });
     });
+    HIP_OK(hipGetLastError());
+}
+
+void disk_vv_force_packed(dpmd_ctx* c, Gate g, const double* xs) {
+    DiskArgs a = disk_args(c);
+    ProfScope prof(c);
+    by_mode(c->vv_mode, [&](auto m) {
+        by_block(c->block, [&](auto b) {
+            constexpr int B = decltype(b)::value;
+            k_vv_force_pk<decltype(m)::value, B><<<blocks_for(c->N, B), B, 0, c->st>>>(a, (const double2*)xs,
+                                                                                        c->vv_dth.p, vv_damping(c), g);
+        });
+    });
+    HIP_OK(hipGetLastError());
+}
+
+void disk_unpack_positions(dpmd_ctx* c, const double* xs) {
+    k_unpack<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(c->N, (const double2*)xs, c->x.p, c->y.p);
     HIP_OK(hipGetLastError());
 }
 

@@ -33,13 +33,27 @@ inline DiskArgs disk_args(dpmd_ctx* c) {
     return a;
 }
 
+// Position sources of the contact sum: the context's SoA arrays, or an interleaved (x, y)
+// buffer (the ping-pong buffers of fused velocity-Verlet batches), where a neighbor's
+// coordinates are one 16-byte gather instead of two 8-byte ones.
+struct SoAPos {
+    const double* x; const double* y;
+    __device__ __forceinline__ double2 at(int j) const { return make_double2(x[j], y[j]); }
+};
+struct PackedPos {
+    const double2* p;
+    __device__ __forceinline__ double2 at(int j) const { return p[j]; }
+};
+
 // compute_pair_forces_kernel (disk.cu:103-160) and compute_wall_forces_kernel
 // (disk.cu:163-246).  In walled boxes the pair displacement is xi - xj (repulsive,
 // consistent with the 0.5 k delta^2 energy); the reference writes xj - xi there (:218).
-template <bool WALL>
-__device__ __forceinline__ void disk_force_at(const DiskArgs& a, int i, int s, double& fxo, double& fyo, double& peo) {
+template <bool WALL, class Pos>
+__device__ __forceinline__ void disk_force_pos(const DiskArgs& a, const Pos& pos, int i, int s, double& fxo,
+                                               double& fyo, double& peo) {
     const double ei = a.e[s];
-    const double xi = a.x[i], yi = a.y[i], ri = a.rad[i];
+    const double2 pi = pos.at(i);
+    const double xi = pi.x, yi = pi.y, ri = a.rad[i];
     double fxi = 0.0, fyi = 0.0, pei = 0.0;
     double Lx = a.Lx[s], Ly = a.Ly[s];
     if (WALL) {
@@ -51,17 +65,18 @@ __device__ __forceinline__ void disk_force_at(const DiskArgs& a, int i, int s, d
     const double iLx = a.iLx[s], iLy = a.iLy[s];
     const int cnt = a.nn[i];
     const int* __restrict__ col = a.nbr + i;
-    // Neighbors in groups of four: every id and coordinate of a group is requested before
-    // any is used (four independent gather chains in flight per lane); the contributions
-    // are then added in list order, so the sums are those of the one-at-a-time loop.
+    // The first 8 neighbor ids are requested at once (one round trip instead of one per
+    // group); then neighbors go in groups of G: every coordinate of a group is requested
+    // before any is used.  Contributions are added in list order, so the sums are those of
+    // the one-at-a-time loop.
     constexpr int G = 4;
-    for (int k0 = 0; k0 < cnt; k0 += G) {
-        int jj[G];
+    auto group = [&](int k0, const int (&jj)[G]) {
         double xj[G], yj[G], rj[G];
 #pragma unroll
-        for (int u = 0; u < G; ++u) jj[u] = (k0 + u < cnt) ? col[(size_t)(k0 + u) * a.n] : i;
-#pragma unroll
-        for (int u = 0; u < G; ++u) { xj[u] = a.x[jj[u]]; yj[u] = a.y[jj[u]]; rj[u] = a.rad[jj[u]]; }
+        for (int u = 0; u < G; ++u) {
+            const double2 q = pos.at(jj[u]);
+            xj[u] = q.x; yj[u] = q.y; rj[u] = a.rad[jj[u]];
+        }
 #pragma unroll
         for (int u = 0; u < G; ++u) {
             if (k0 + u >= cnt) break;
@@ -79,8 +94,35 @@ __device__ __forceinline__ void disk_force_at(const DiskArgs& a, int i, int s, d
             fyi += fmag * ny;
             pei += (0.5 * ei * delta * delta) * 0.5;
         }
+    };
+    int pre[2 * G];
+#pragma unroll
+    for (int u = 0; u < 2 * G; ++u) pre[u] = (u < cnt) ? col[(size_t)u * a.n] : i;
+    if (cnt > 0) { const int jj[G] = {pre[0], pre[1], pre[2], pre[3]}; group(0, jj); }
+    if (cnt > G) { const int jj[G] = {pre[4], pre[5], pre[6], pre[7]}; group(G, jj); }
+    for (int k0 = 2 * G; k0 < cnt; k0 += G) {
+        int jj[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) jj[u] = (k0 + u < cnt) ? col[(size_t)(k0 + u) * a.n] : i;
+        group(k0, jj);
     }
     fxo = fxi; fyo = fyi; peo = pei;
+}
+
+template <bool WALL>
+__device__ __forceinline__ void disk_force_at(const DiskArgs& a, int i, int s, double& fxo, double& fyo, double& peo) {
+    disk_force_pos<WALL>(a, SoAPos{a.x, a.y}, i, s, fxo, fyo, peo);
+}
+
+template <int MODE, class Pos>
+__device__ __forceinline__ void disk_force_mode_pos(const DiskArgs& a, const Pos& pos, int i, int s, double& fx,
+                                                    double& fy, double& pe) {
+    if (MODE == DPMD_FORCE_PBC) disk_force_pos<false>(a, pos, i, s, fx, fy, pe);
+    else if (MODE == DPMD_FORCE_WALL) disk_force_pos<true>(a, pos, i, s, fx, fy, pe);
+    else {
+        if (a.boundary[s]) disk_force_pos<true>(a, pos, i, s, fx, fy, pe);
+        else disk_force_pos<false>(a, pos, i, s, fx, fy, pe);
+    }
 }
 
 template <int MODE>

@@ -194,7 +194,7 @@ struct dpmd_ctx {
     int ck_steps = 0, ck_every = 0, ck_count = 0;
     double* rec_seg = nullptr;       // when set, lean VV batches record positions per step
     // fused velocity Verlet (disks): second position buffer and the switch
-    dpmd::DVec<double> x2, y2;
+    dpmd::DVec<double> xy0, xy1;   // interleaved (x, y) ping-pong positions of fused VV batches (2N each)
     bool fuse_vv = true;
     // workgroup size of the disk step kernels (64, 128, 256, 512) and the largest number of
     // steps captured into one hipGraph batch (a power of two, 1 .. 128)
@@ -320,8 +320,11 @@ void disk_stress(dpmd_ctx* c);
 void disk_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, double a, double b);
 void disk_fire_drift(dpmd_ctx* c, Gate g);
 void disk_vv_drift(dpmd_ctx* c, Gate g);
-void disk_vv_drift_to(dpmd_ctx* c, Gate g, double* xo, double* yo, int any_word);
-void disk_vv_fused(dpmd_ctx* c, Gate g, const double* xs, const double* ys, double* xo, double* yo, int any_word);
+// fused velocity-Verlet batches: positions ping-pong between interleaved (x, y) buffers
+void disk_vv_drift_to(dpmd_ctx* c, Gate g, double* xo, int any_word);
+void disk_vv_fused(dpmd_ctx* c, Gate g, const double* xs, double* xo, int any_word);
+void disk_vv_force_packed(dpmd_ctx* c, Gate g, const double* xs);
+void disk_unpack_positions(dpmd_ctx* c, const double* xs);
 void disk_vv_force(dpmd_ctx* c, Gate g);
 void disk_damping(dpmd_ctx* c, const double* scale);
 
