@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_vv_fused launch on the default workload (N = 2^20, phi = 0.84), from the
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes in profiles/k_vv_fused_disk_N1M_fire_nve.md, each
 # scaled by what the counter reads for a copy kernel of known size (FETCH x 2.0, WRITE x 1.0).
-FUSED_TRAFFIC_N1M = (51309.2 * (131072.0 / 65545.5) + 32768.0) * 1024.0
+FUSED_TRAFFIC_N1M = (51352.8 * (131072.0 / 65545.5) + 32768.0) * 1024.0
 
 
 def log(msg: str) -> None:
