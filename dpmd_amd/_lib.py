@@ -113,6 +113,8 @@ PROTOS = {
     "dpmd_compute_overlaps": (i32, [vp]),
     "dpmd_compute_n_contacts_total": (i32, [vp]),
     "dpmd_compute_pair_observables": (i32, [vp, vp, vp, vp, vp, vp, vp, i64]),
+    "dpmd_particle_neighbors": (i32, [vp, vp, vp, i64, pi64]),
+    "dpmd_compute_rigid_pair_observables": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i64]),
     "dpmd_nvt_rescale_run": (i32, [vp, i32, i32, vp, i32, vp, i32, i32, i32]),
     "dpmd_jam_binary_search": (i32, [vp, vp, i32, i32, f64, f64, f64, f64, i32, i32, vp, vp, pi32]),
     "dpmd_fire_init": (i32, [vp, vp, i32]),

@@ -41,8 +41,8 @@ FieldRef field_ref(dpmd_ctx* c, int f) {
         case DPMD_F_STRESS_XY: return ref(c->sxy, 0);
         case DPMD_F_STRESS_YX: return ref(c->syx, 0);
         case DPMD_F_STRESS_YY: return ref(c->syy, 0);
-        case DPMD_F_OVERLAP: return ref(c->overlap, 0);
-        case DPMD_F_CONTACT_COUNT: return ref(c->ccount, 0);
+        case DPMD_F_OVERLAP: return ref(c->overlap, c->kind == DPMD_RIGID_BUMPY ? 1 : 0);
+        case DPMD_F_CONTACT_COUNT: return ref(c->ccount, c->kind == DPMD_RIGID_BUMPY ? 1 : 0);
         case DPMD_F_N_CONTACTS_TOTAL: return ref(c->ncontacts, 2);
         case DPMD_F_CONTACTS: return ref(c->contacts, 0);
         case DPMD_F_BOX_SIZE_X: return ref(c->Lx, 2);
@@ -1467,25 +1467,69 @@ int dpmd_nvt_rescale_run(dpmd_ctx* c, int n_steps, int step0, const double* phi_
 
 int dpmd_compute_contacts(dpmd_ctx* c) {
     return guard(c, [&] {
-        need(!rigid(c), "compute_contacts: implemented for disks");
         need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
-        disk_contacts(c);
+        if (rigid(c)) {
+            rigid_particle_neighbors(c);
+            rigid_pair_obs(c, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+        } else {
+            disk_contacts(c);
+        }
     });
 }
 int dpmd_compute_n_contacts_total(dpmd_ctx* c) {
     return guard(c, [&] {
-        need(!rigid(c), "compute_n_contacts_total: implemented for disks");
         need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
-        disk_contacts(c);
+        if (rigid(c)) {
+            rigid_particle_neighbors(c);
+            rigid_pair_obs(c, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+        } else {
+            disk_contacts(c);
+        }
         contacts_total(c);
     });
 }
-int dpmd_compute_overlaps(dpmd_ctx* c) { return dpmd_compute_contacts(c); }
+int dpmd_compute_overlaps(dpmd_ctx* c) {
+    return guard(c, [&] {
+        need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
+        if (rigid(c)) rigid_overlaps(c); else disk_contacts(c);
+    });
+}
+
+int dpmd_particle_neighbors(dpmd_ctx* c, int* start, int* ids, int64_t ids_len, int64_t* total) {
+    return guard(c, [&] {
+        need(rigid(c), "particle_neighbors: rigid bumpy contexts (disk lists are particle lists)");
+        need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
+        const int64_t tot = rigid_particle_neighbors(c);
+        if (total) *total = tot;
+        if (start) {
+            HIP_OK(hipMemcpyAsync(start, c->pn_start.p, (size_t)(c->N + 1) * 4, hipMemcpyDeviceToHost, c->st));
+        }
+        if (ids) {
+            need(ids_len >= tot, "particle_neighbors: ids buffer too small");
+            if (tot) HIP_OK(hipMemcpyAsync(ids, c->pn_ids.p, (size_t)tot * 4, hipMemcpyDeviceToHost, c->st));
+        }
+        HIP_OK(hipStreamSynchronize(c->st));
+    });
+}
+
+int dpmd_compute_rigid_pair_observables(dpmd_ctx* c, int* pair_i, int* pair_j, double* dist, double* fx, double* fy,
+                                        int* vertex_contacts_i, int* vertex_contacts_j, double* friction, int64_t n) {
+    return guard(c, [&] {
+        need(rigid(c), "rigid pair observables: rigid bumpy contexts");
+        need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
+        need((pair_i == nullptr) == (pair_j == nullptr), "rigid pair observables: pair_i and pair_j go together");
+        need((fx == nullptr) == (fy == nullptr), "rigid pair observables: fx and fy go together");
+        const int64_t tot = rigid_particle_neighbors(c);
+        need(n == tot, "rigid pair observables: n must equal the number of particle neighbor entries (" +
+                           std::to_string(tot) + ")");
+        rigid_pair_obs(c, pair_i, pair_j, dist, fx, fy, vertex_contacts_i, vertex_contacts_j, friction);
+    });
+}
 
 int dpmd_compute_pair_observables(dpmd_ctx* c, int* pair_i, int* pair_j, double* dist, double* fx, double* fy,
                                   double* const* hessian, int64_t n) {
     return guard(c, [&] {
-        need(!rigid(c), "pair observables: implemented for disks");
+        need(!rigid(c), "pair observables: disks (rigid bodies: dpmd_compute_rigid_pair_observables)");
         need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
         need((pair_i == nullptr) == (pair_j == nullptr), "pair observables: pair_i and pair_j go together");
         need((fx == nullptr) == (fy == nullptr), "pair observables: fx and fy go together");

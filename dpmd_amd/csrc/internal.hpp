@@ -153,7 +153,10 @@ struct dpmd_ctx {
     bool vv_damped = false;
     // ---- contact observables (compute_contacts / _overlaps / pair observables)
     dpmd::DVec<int> contacts, ncontacts, pair_start, canon;
-    dpmd::DVec<double> overlap, ccount;
+    dpmd::DVec<double> overlap, ccount;   // per particle (disks) or per vertex (rigid bodies)
+    // rigid bodies: particle-level neighbor lists (build_particle_neighbors)
+    dpmd::DVec<int> pn_start, pn_count, pn_ids, pn_vc, pn_flag;
+    int64_t pn_total = 0;
     // ---- run control: [0] steps_done [1] max_steps [2] n_running [3] active
     dpmd::DVec<int> run, one;
     // ---- saved state (save_state/restore_state)
@@ -332,6 +335,10 @@ void disk_damping(dpmd_ctx* c, const double* scale);
 void disk_contacts(dpmd_ctx* c);
 void contacts_total(dpmd_ctx* c);
 int64_t pair_setup(dpmd_ctx* c);
+int64_t rigid_particle_neighbors(dpmd_ctx* c);
+void rigid_pair_obs(dpmd_ctx* c, int* pi, int* pj, double* dist, double* fx, double* fy, int* vci, int* vcj,
+                    double* friction);
+void rigid_overlaps(dpmd_ctx* c);
 void disk_pair_obs(dpmd_ctx* c, int* pi, int* pj, double* dist, double* fx, double* fy, double* const* h);
 void disk_ke(dpmd_ctx* c);
 void fire_recompute_update(dpmd_ctx* c);

@@ -136,7 +136,223 @@ __global__ void k_pair_obs(ObsArgs a, PairOut o) {
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Rigid bumpy polygons.  Particles keep canonical order; vertices are in current order and
+// particle i's vertices are static_index[particle_offset[i] .. particle_offset[i+1]).
+// ---------------------------------------------------------------------------------------
+struct RigidObs {
+    const double* x; const double* y;                 // centres (canonical)
+    const double* vx; const double* vy; const double* vr; const int* vpid;   // vertices (current)
+    const int* vstatic; const int* poff; const int* sid;
+    const double* Lx; const double* Ly; const double* iLx; const double* iLy; const double* e;
+    const int* nn; const int* nbr; int n; int nv;
+};
+
+RigidObs rigid_obs(dpmd_ctx* c) {
+    RigidObs a;
+    a.x = c->x.p; a.y = c->y.p;
+    a.vx = c->vpx.p; a.vy = c->vpy.p; a.vr = c->vrad.p; a.vpid = c->vpid.p;
+    a.vstatic = c->static_index.p; a.poff = c->poff.p; a.sid = c->sid.p;
+    a.Lx = c->Lx.p; a.Ly = c->Ly.p; a.iLx = c->iLx.p; a.iLy = c->iLy.p; a.e = c->e.p;
+    a.nn = c->nn.p; a.nbr = c->nbr.p; a.n = c->N; a.nv = c->Nv;
+    return a;
+}
+
+// build_particle_neighbors (base_poly_particle.hpp:299-375): the particles whose vertices
+// appear in the vertex lists of particle i's vertices, ascending and without repeats (what
+// sorting and deduplicating the (i, j) keys yields).  FILL = false counts, FILL = true
+// writes at start[i].  More than kMaxParticleNeighbors sets *overflow.
+constexpr int kMaxParticleNeighbors = 64;
+template <bool FILL>
+__global__ void k_rigid_pn(RigidObs a, int* count, const int* start, int* ids, int* overflow) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    int list[kMaxParticleNeighbors];
+    int m = 0;
+    bool full = false;
+    for (int c = a.poff[i]; c < a.poff[i + 1]; ++c) {
+        const int u = a.vstatic[c];
+        const int cnt = a.nn[u];
+        for (int k = 0; k < cnt; ++k) {
+            const int p = a.vpid[a.nbr[(size_t)k * a.nv + u]];
+            int pos = 0;
+            while (pos < m && list[pos] < p) ++pos;
+            if (pos < m && list[pos] == p) continue;
+            if (m == kMaxParticleNeighbors) { full = true; continue; }
+            for (int q = m; q > pos; --q) list[q] = list[q - 1];
+            list[pos] = p;
+            ++m;
+        }
+    }
+    if (full) atomicOr(overflow, 1);
+    if (FILL) {
+        const int b = start[i];
+        for (int q = 0; q < m; ++q) ids[b + q] = list[q];
+    } else {
+        count[i] = m;
+    }
+}
+
+struct RigidPairOut {
+    const int* start; const int* ids;
+    int* pi; int* pj; double* dist; double* fx; double* fy; int* vci; int* vcj; double* friction;
+    int* contacts;
+};
+
+// count_particle_contacts_kernel, compute_pair_dist_kernel, compute_friction_coeff_kernel and
+// the pair sums of compute_particle_pair_forces_kernel (rigid_bumpy.cu:759-928,1261-1327),
+// one thread per particle, its pair entries in order.  Per entry (i, j): the number of i's
+// vertices touching some vertex of j, the centre distance (minimum image), the summed
+// contact force on i's vertices from j's vertices (i's vertices in canonical order, each
+// vertex's list in order) and the friction ratio |F_t| / |F_n| about the centre line.
+__global__ void k_rigid_pairs(RigidObs a, RigidPairOut o) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int s = a.sid[i];
+    const double e = a.e[s];
+    const double Lx = a.Lx[s], Ly = a.Ly[s], iLx = a.iLx[s], iLy = a.iLy[s];
+    int ncontact = 0;
+    for (int pk = o.start[i]; pk < o.start[i + 1]; ++pk) {
+        const int j = o.ids[pk];
+        int nvi = 0;
+        double fx = 0.0, fy = 0.0;
+        for (int c = a.poff[i]; c < a.poff[i + 1]; ++c) {
+            const int u = a.vstatic[c];
+            const double xu = a.vx[u], yu = a.vy[u], ru = a.vr[u];
+            const int cnt = a.nn[u];
+            bool found = false;
+            for (int k = 0; k < cnt; ++k) {
+                const int w = a.nbr[(size_t)k * a.nv + u];
+                if (a.vpid[w] != j) continue;
+                const double dx = min_image(xu - a.vx[w], Lx, iLx), dy = min_image(yu - a.vy[w], Ly, iLy);
+                const double r2 = dx * dx + dy * dy;
+                const double radsum = ru + a.vr[w];
+                if (r2 >= radsum * radsum) continue;
+                found = true;
+                const double r = sqrt(r2);
+                const double inv = 1.0 / r;
+                const double nx = dx * inv, ny = dy * inv;
+                const double delta = radsum - r;
+                const double fmag = e * delta;
+                fx += fmag * nx;
+                fy += fmag * ny;
+            }
+            if (found) ++nvi;
+        }
+        if (nvi > 0) ++ncontact;
+        const double cx = min_image(a.x[i] - a.x[j], Lx, iLx), cy = min_image(a.y[i] - a.y[j], Ly, iLy);
+        const double rp = sqrt(cx * cx + cy * cy);
+        if (o.pi) { o.pi[pk] = i; o.pj[pk] = j; }
+        if (o.dist) o.dist[pk] = rp;
+        if (o.fx) { o.fx[pk] = fx; o.fy[pk] = fy; }
+        if (o.vci) o.vci[pk] = nvi;
+        if (o.friction) {
+            const double nx = cx / rp, ny = cy / rp;
+            const double fn = fx * nx + fy * ny;
+            const double fabs_n = fabs(fn);
+            double mu = 0.0;
+            if (fabs_n != 0.0) {
+                const double tx = fx - fn * nx, ty = fy - fn * ny;
+                mu = sqrt(tx * tx + ty * ty) / fabs_n;
+            }
+            o.friction[pk] = mu;
+        }
+    }
+    if (o.contacts) o.contacts[i] = ncontact;
+}
+
+// vertex contacts of the reverse entry: entry (i, j) gets the count entry (j, i) holds
+__global__ void k_rigid_reverse(int n, const int* start, const int* ids, const int* vci, int* vcj) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int pk = start[i]; pk < start[i + 1]; ++pk) {
+        const int j = ids[pk];
+        int lo = start[j], hi = start[j + 1];
+        while (lo < hi) {   // ids of a particle are ascending
+            const int mid = (lo + hi) >> 1;
+            if (ids[mid] < i) lo = mid + 1; else hi = mid;
+        }
+        vcj[pk] = (lo < start[j + 1] && ids[lo] == i) ? vci[lo] : 0;
+    }
+}
+
+// compute_overlaps_kernel (rigid_bumpy.cu:929-966): per vertex, over its list
+__global__ void k_rigid_overlaps(RigidObs a, const int* vsid, double* overlap, double* count) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= a.nv) return;
+    const int s = vsid[u];
+    const double Lx = a.Lx[s], Ly = a.Ly[s], iLx = a.iLx[s], iLy = a.iLy[s];
+    const double xu = a.vx[u], yu = a.vy[u], ru = a.vr[u];
+    double ov = 0.0, cc = 0.0;
+    for (int k = 0; k < a.nn[u]; ++k) {
+        const int w = a.nbr[(size_t)k * a.nv + u];
+        const double dx = min_image(xu - a.vx[w], Lx, iLx), dy = min_image(yu - a.vy[w], Ly, iLy);
+        const double r2 = dx * dx + dy * dy;
+        const double radsum = ru + a.vr[w];
+        if (r2 >= radsum * radsum) continue;
+        ov += radsum - sqrt(r2);
+        cc += 1.0;
+    }
+    overlap[u] = ov;
+    count[u] = cc;
+}
+
 }  // namespace
+
+// Particle-level neighbor lists of rigid bodies into c->pn_start / c->pn_ids; returns the
+// number of entries.
+int64_t rigid_particle_neighbors(dpmd_ctx* c) {
+    const int N = c->N;
+    RigidObs a = rigid_obs(c);
+    c->pn_start.resize(N + 1);
+    c->pn_count.resize(N);
+    c->pn_flag.resize(1);
+    c->pn_flag.zero(c->st);
+    k_rigid_pn<false><<<blocks_for(N), DPMD_BLOCK, 0, c->st>>>(a, c->pn_count.p, nullptr, nullptr, c->pn_flag.p);
+    HIP_OK(hipGetLastError());
+    std::vector<int> cnt = dl(c, c->pn_count);
+    int ov = 0;
+    HIP_OK(hipMemcpyAsync(&ov, c->pn_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIP_OK(hipStreamSynchronize(c->st));
+    need(ov == 0, "build_particle_neighbors: a particle has more than " + std::to_string(kMaxParticleNeighbors) +
+                      " particle neighbors");
+    std::vector<int> st(N + 1, 0);
+    int64_t tot = 0;
+    for (int i = 0; i < N; ++i) { st[i] = (int)tot; tot += cnt[i]; }
+    need(tot <= 0x7fffffff, "build_particle_neighbors: more than 2^31 entries");
+    st[N] = (int)tot;
+    c->pn_start.upload(st.data(), N + 1, c->st);
+    c->pn_ids.resize(std::max<int64_t>(1, tot));
+    k_rigid_pn<true><<<blocks_for(N), DPMD_BLOCK, 0, c->st>>>(a, nullptr, c->pn_start.p, c->pn_ids.p, c->pn_flag.p);
+    HIP_OK(hipGetLastError());
+    c->pn_total = tot;
+    return tot;
+}
+
+void rigid_pair_obs(dpmd_ctx* c, int* pi, int* pj, double* dist, double* fx, double* fy, int* vci, int* vcj,
+                    double* friction) {
+    const int N = c->N;
+    c->pn_vc.resize(std::max<int64_t>(1, c->pn_total));
+    RigidPairOut o;
+    o.start = c->pn_start.p; o.ids = c->pn_ids.p;
+    o.pi = pi; o.pj = pj; o.dist = dist; o.fx = fx; o.fy = fy; o.friction = friction;
+    o.vci = vci ? vci : c->pn_vc.p;
+    o.vcj = vcj;
+    o.contacts = c->contacts.p;
+    k_rigid_pairs<<<blocks_for(N), DPMD_BLOCK, 0, c->st>>>(rigid_obs(c), o);
+    HIP_OK(hipGetLastError());
+    if (vcj) {
+        k_rigid_reverse<<<blocks_for(N), DPMD_BLOCK, 0, c->st>>>(N, c->pn_start.p, c->pn_ids.p, o.vci, vcj);
+        HIP_OK(hipGetLastError());
+    }
+}
+
+void rigid_overlaps(dpmd_ctx* c) {
+    c->overlap.resize(c->Nv);
+    c->ccount.resize(c->Nv);
+    k_rigid_overlaps<<<blocks_for(c->Nv), DPMD_BLOCK, 0, c->st>>>(rigid_obs(c), c->vsid.p, c->overlap.p, c->ccount.p);
+    HIP_OK(hipGetLastError());
+}
 
 void disk_contacts(dpmd_ctx* c) {
     k_contacts<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(obs_args(c), c->contacts.p, c->overlap.p, c->ccount.p);

@@ -248,6 +248,28 @@ class Disk(_Particles):
 class RigidBumpy(_Particles):
     kind = L.RIGID_BUMPY
 
+    def particle_neighbors(self):
+        """build_particle_neighbors (base_poly_particle.hpp:299-375): (start[N+1], ids)."""
+        tot = C.c_int64()
+        self._c(lib().dpmd_particle_neighbors(self.ctx, None, None, 0, C.byref(tot)))
+        st = np.zeros(self.n_particles() + 1, np.int32)
+        ids = np.zeros(max(1, tot.value), np.int32)
+        self._c(lib().dpmd_particle_neighbors(self.ctx, st.ctypes.data, ids.ctypes.data, ids.size, C.byref(tot)))
+        return st, ids[:tot.value]
+
+    def pair_observables(self):
+        """Rigid contacts / pair distances / friction / pair forces per particle-neighbor entry
+        (dpmd_compute_rigid_pair_observables).  Returns a dict of host arrays."""
+        tot = C.c_int64()
+        self._c(lib().dpmd_particle_neighbors(self.ctx, None, None, 0, C.byref(tot)))
+        n = tot.value
+        names = (("pair_i", np.int32), ("pair_j", np.int32), ("dist", np.float64), ("fx", np.float64),
+                 ("fy", np.float64), ("vertex_contacts_i", np.int32), ("vertex_contacts_j", np.int32),
+                 ("friction", np.float64))
+        bufs = {k: DeviceArray(np.zeros(max(1, n), t), t) for k, t in names}
+        self._c(lib().dpmd_compute_rigid_pair_observables(self.ctx, *[bufs[k].ptr for k, _ in names], n))
+        return {k: v.to_host()[:n] for k, v in bufs.items()}
+
 
 class VelocityVerlet:
     """VelocityVerlet / VelocityVerletWall (include/integrators/velocity_verlet.cuh:53-78)."""

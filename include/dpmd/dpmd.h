@@ -65,7 +65,7 @@ enum dpmd_field {
     DPMD_F_PE, DPMD_F_KE, DPMD_F_RAD, DPMD_F_MASS, DPMD_F_AREA, DPMD_F_LAST_POS_X, DPMD_F_LAST_POS_Y,
     DPMD_F_ANGLE, DPMD_F_ANGULAR_VEL, DPMD_F_TORQUE, DPMD_F_MOMENT_INERTIA,
     DPMD_F_STRESS_XX, DPMD_F_STRESS_XY, DPMD_F_STRESS_YX, DPMD_F_STRESS_YY,
-    DPMD_F_OVERLAP, DPMD_F_CONTACT_COUNT,   /* overlaps (N,2): total overlap, contact count */
+    DPMD_F_OVERLAP, DPMD_F_CONTACT_COUNT,   /* overlaps: total overlap, contact count (V for rigid) */
     /* double, S */
     DPMD_F_BOX_SIZE_X = 32, DPMD_F_BOX_SIZE_Y, DPMD_F_BOX_INV_X, DPMD_F_BOX_INV_Y,
     DPMD_F_E_INTERACTION, DPMD_F_NEIGHBOR_CUTOFF, DPMD_F_THRESH2,
@@ -155,12 +155,16 @@ int dpmd_increment_packing_fraction(dpmd_ctx* ctx, const double* increment);
 /* save_state / restore_state (base_particle.hpp:708-736) */
 int dpmd_save_state(dpmd_ctx* ctx, const int* flag, int true_val);
 int dpmd_restore_state(dpmd_ctx* ctx, const int* flag, int true_val);
-/* ---- contact observables (disks) ---------------------------------------------------- */
-/* compute_contacts + compute_overlaps (base_particle.hpp:649-651,698-699; disk.cu:434-471,
- * 663-700): per particle the number of contacts (DPMD_F_CONTACTS), the summed overlap
- * (DPMD_F_OVERLAP) and the contact count as a double (DPMD_F_CONTACT_COUNT), over the Verlet
- * lists with the minimum image in every box.  compute_overlaps is the same call. */
+/* ---- contact observables ------------------------------------------------------------ */
+/* compute_contacts (base_particle.hpp:649-651): disks (disk.cu:434-471): per particle the
+ * number of contacts over the Verlet list (DPMD_F_CONTACTS), with the summed overlap and the
+ * contact count as a double as compute_overlaps gives them; rigid bodies
+ * (rigid_bumpy.cu:759-825,1598-1618): per particle the number of particle neighbors touched
+ * by at least one vertex.  Minimum image in every box. */
 int dpmd_compute_contacts(dpmd_ctx* ctx);
+/* compute_overlaps (base_particle.hpp:698-699): DPMD_F_OVERLAP / DPMD_F_CONTACT_COUNT per
+ * particle for disks (disk.cu:663-700) and per vertex, current order, for rigid bodies
+ * (rigid_bumpy.cu:929-966). */
 int dpmd_compute_overlaps(dpmd_ctx* ctx);
 /* compute_n_contacts_total (base_particle.hpp:653-659): contacts, then their per-system sum
  * (DPMD_F_N_CONTACTS_TOTAL). */
@@ -176,6 +180,21 @@ int dpmd_compute_n_contacts_total(dpmd_ctx* ctx);
  * of the diagonal block of i and the off-diagonal block (i, j). */
 int dpmd_compute_pair_observables(dpmd_ctx* ctx, int* pair_i, int* pair_j, double* dist, double* fx, double* fy,
                                   double* const* hessian, int64_t n);
+/* build_particle_neighbors (base_poly_particle.hpp:299-375), rigid bodies: for each particle
+ * (canonical order) the particles whose vertices are in its vertices' lists, ascending, no
+ * repeats.  start: host [N+1]; ids: host [ids_len >= total]; either may be NULL;
+ * total: number of entries. */
+int dpmd_particle_neighbors(dpmd_ctx* ctx, int* start, int* ids, int64_t ids_len, int64_t* total);
+/* compute_contacts / compute_pair_dist / compute_friction_coeff / compute_pair_forces for rigid
+ * bodies (rigid_bumpy.cu:759-928,1261-1327,1598-1676) per particle-neighbor entry (the order of
+ * dpmd_particle_neighbors).  DEVICE arrays of n entries, any group NULL: pair_i, pair_j
+ * (particle ids); dist: centre distance (minimum image); fx, fy: summed contact force on i's
+ * vertices from j's; vertex_contacts_i / _j: the reference's pair_vertex_contacts (i's
+ * vertices touching j, j's vertices touching i); friction: |F_t| / |F_n| of that force about
+ * the centre line (0 when F_n = 0).  Also fills DPMD_F_CONTACTS. */
+int dpmd_compute_rigid_pair_observables(dpmd_ctx* ctx, int* pair_i, int* pair_j, double* dist, double* fx,
+                                        double* fy, int* vertex_contacts_i, int* vertex_contacts_j,
+                                        double* friction, int64_t n);
 /* compute_damping_forces (base_particle.hpp:386; disk.cu:249-263, rigid_bumpy.cu:310-327):
  * F_i += -v_i * scale[s] (rigid bodies also torque_i += -omega_i * scale[s]). */
 int dpmd_compute_damping_forces(dpmd_ctx* ctx, const double* scale);

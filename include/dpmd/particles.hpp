@@ -13,6 +13,7 @@
 // base_poly_particle.hpp:96).
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
@@ -260,6 +261,61 @@ public:
     I1 n_vertices_per_particle{ctx_, DPMD_F_N_VERTICES_PER_PARTICLE};
     I1 vertex_system_id{ctx_, DPMD_F_VERTEX_SYSTEM_ID};
     I1 vertex_system_offset{ctx_, DPMD_F_VERTEX_SYSTEM_OFFSET};
+
+    // ---- contact observables (base_poly_particle.hpp:299-375; rigid_bumpy.cu:759-966,
+    // 1261-1327,1598-1690).  Per particle-neighbor entry fields are sized on each call.
+    I1 contacts{ctx_, DPMD_F_CONTACTS};
+    F2 overlaps{ctx_, DPMD_F_OVERLAP, DPMD_F_CONTACT_COUNT};   // per vertex (current order)
+    I1 n_contacts_total{ctx_, DPMD_F_N_CONTACTS_TOTAL};
+    std::vector<int> particle_neighbor_start, particle_neighbor_ids;   // host copies
+    df::DeviceField2D<int> pair_ids, pair_vertex_contacts;
+    F1 pair_dist, friction_coeff;
+    F2 pair_forces;
+
+    void build_particle_neighbors() {
+        int64_t tot = 0;
+        ok(dpmd_particle_neighbors(ctx_, nullptr, nullptr, 0, &tot));
+        particle_neighbor_start.assign(n_particles() + 1, 0);
+        particle_neighbor_ids.assign(std::max<int64_t>(1, tot), 0);
+        ok(dpmd_particle_neighbors(ctx_, particle_neighbor_start.data(), particle_neighbor_ids.data(),
+                                   (int64_t)particle_neighbor_ids.size(), &tot));
+        particle_neighbor_ids.resize(tot);
+    }
+    void compute_contacts() {
+        build_particle_neighbors();
+        const int n = size_pairs();
+        pair_vertex_contacts.resize(n);
+        ok(dpmd_compute_rigid_pair_observables(ctx_, pair_ids.xptr(), pair_ids.yptr(), nullptr, nullptr, nullptr,
+                                               pair_vertex_contacts.xptr(), pair_vertex_contacts.yptr(), nullptr, n));
+    }
+    void compute_n_contacts_total() { ok(dpmd_compute_n_contacts_total(ctx_)); }
+    void compute_overlaps() { ok(dpmd_compute_overlaps(ctx_)); }
+    void compute_pair_dist() {
+        const int n = size_pairs();
+        pair_dist.resize(n);
+        ok(dpmd_compute_rigid_pair_observables(ctx_, pair_ids.xptr(), pair_ids.yptr(), pair_dist.ptr(), nullptr,
+                                               nullptr, nullptr, nullptr, nullptr, n));
+    }
+    void compute_pair_forces() {
+        const int n = size_pairs();
+        pair_forces.resize(n);
+        ok(dpmd_compute_rigid_pair_observables(ctx_, pair_ids.xptr(), pair_ids.yptr(), nullptr, pair_forces.xptr(),
+                                               pair_forces.yptr(), nullptr, nullptr, nullptr, n));
+    }
+    void compute_friction_coeff() {
+        const int n = size_pairs();
+        friction_coeff.resize(n);
+        ok(dpmd_compute_rigid_pair_observables(ctx_, pair_ids.xptr(), pair_ids.yptr(), nullptr, nullptr, nullptr,
+                                               nullptr, nullptr, friction_coeff.ptr(), n));
+    }
+
+private:
+    int size_pairs() {
+        int64_t tot = 0;
+        ok(dpmd_particle_neighbors(ctx_, nullptr, nullptr, 0, &tot));
+        pair_ids.resize((int)tot);
+        return (int)tot;
+    }
 };
 }  // namespace rigid_bumpy
 

@@ -1515,6 +1515,104 @@ int or_rigid_jam_bisect(void* h, const double* dt, int max_comp, int max_min, do
                         double dphi, double phi_tol, int mode, int* action_out, double* phi_out) {
     return jam_bisect((Rigid*)h, dt, max_comp, max_min, pe_t, pe_diff_t, dphi, phi_tol, mode, action_out, phi_out);
 }
+// build_particle_neighbors (base_poly_particle.hpp:299-375): per particle, the sorted distinct
+// particle ids of its vertices' list entries.  Returns the total; start [N+1], ids may be null.
+int or_rigid_particle_neighbors(void* h, int* start, int* ids) {
+    Rigid* r = (Rigid*)h;
+    int tot = 0;
+    for (int i = 0; i < r->N; ++i) {
+        std::vector<int> L;
+        for (int c = r->poff[i]; c < r->poff[i + 1]; ++c)
+            for (int w : r->nbr[r->static_index[c]]) L.push_back(r->vpid[w]);
+        std::sort(L.begin(), L.end());
+        L.erase(std::unique(L.begin(), L.end()), L.end());
+        if (start) start[i] = tot;
+        if (ids) for (size_t q = 0; q < L.size(); ++q) ids[tot + q] = L[q];
+        tot += (int)L.size();
+    }
+    if (start) start[r->N] = tot;
+    return tot;
+}
+// count_particle_contacts / compute_pair_dist / compute_friction_coeff / pair forces
+// (rigid_bumpy.cu:759-928,1261-1327) per particle-neighbor entry: vertices of i in canonical
+// order, each vertex's list in order (the reference adds the pair forces atomically).
+void or_rigid_pair_obs(void* h, int* pi, int* pj, double* dist, double* fx, double* fy, int* vci, int* vcj,
+                       double* friction, int* contacts) {
+    Rigid* r = (Rigid*)h;
+    std::vector<int> st(r->N + 1);
+    const int tot = or_rigid_particle_neighbors(h, st.data(), nullptr);
+    std::vector<int> ids(std::max(1, tot));
+    or_rigid_particle_neighbors(h, st.data(), ids.data());
+    for (int i = 0; i < r->N; ++i) {
+        const int s = r->sid[i];
+        const double e = r->e[s];
+        int ncontact = 0;
+        for (int pk = st[i]; pk < st[i + 1]; ++pk) {
+            const int j = ids[pk];
+            int nvi = 0;
+            double f_x = 0.0, f_y = 0.0;
+            for (int c = r->poff[i]; c < r->poff[i + 1]; ++c) {
+                const int u = r->static_index[c];
+                bool found = false;
+                for (int w : r->nbr[u]) {
+                    if (r->vpid[w] != j) continue;
+                    double dx, dy;
+                    const double r2 = disp_pbc_L(r->vpx[u], r->vpy[u], r->vpx[w], r->vpy[w], r->Lx[s], r->Ly[s], r->ix[s],
+                                                 r->iy[s], dx, dy);
+                    const double radsum = r->vrad[u] + r->vrad[w];
+                    if (r2 >= radsum * radsum) continue;
+                    found = true;
+                    const double rr = std::sqrt(r2);
+                    const double inv = 1.0 / rr;
+                    const double nx = dx * inv, ny = dy * inv;
+                    const double delta = radsum - rr;
+                    const double fmag = e * delta;
+                    f_x += fmag * nx;
+                    f_y += fmag * ny;
+                }
+                if (found) ++nvi;
+            }
+            if (nvi > 0) ++ncontact;
+            double cx, cy;
+            const double rp = std::sqrt(disp_pbc_L(r->x[i], r->y[i], r->x[j], r->y[j], r->Lx[s], r->Ly[s], r->ix[s],
+                                                   r->iy[s], cx, cy));
+            pi[pk] = i; pj[pk] = j; dist[pk] = rp; fx[pk] = f_x; fy[pk] = f_y; vci[pk] = nvi;
+            const double nx = cx / rp, ny = cy / rp;
+            const double fn = f_x * nx + f_y * ny;
+            double mu = 0.0;
+            if (std::fabs(fn) != 0.0) {
+                const double tx = f_x - fn * nx, ty = f_y - fn * ny;
+                mu = std::sqrt(tx * tx + ty * ty) / std::fabs(fn);
+            }
+            friction[pk] = mu;
+        }
+        contacts[i] = ncontact;
+    }
+    for (int i = 0; i < r->N; ++i)
+        for (int pk = st[i]; pk < st[i + 1]; ++pk) {
+            const int j = ids[pk];
+            vcj[pk] = 0;
+            for (int pl = st[j]; pl < st[j + 1]; ++pl) if (ids[pl] == i) vcj[pk] = vci[pl];
+        }
+}
+// compute_overlaps_kernel (rigid_bumpy.cu:929-966), per vertex in current order
+void or_rigid_overlaps(void* h, double* overlap, double* count) {
+    Rigid* r = (Rigid*)h;
+    for (int u = 0; u < r->Nv; ++u) {
+        const int s = r->vsid[u];
+        double ov = 0.0, cc = 0.0;
+        for (int w : r->nbr[u]) {
+            double dx, dy;
+            const double r2 = disp_pbc_L(r->vpx[u], r->vpy[u], r->vpx[w], r->vpy[w], r->Lx[s], r->Ly[s], r->ix[s], r->iy[s],
+                                         dx, dy);
+            const double radsum = r->vrad[u] + r->vrad[w];
+            if (r2 >= radsum * radsum) continue;
+            ov += radsum - std::sqrt(r2);
+            cc += 1.0;
+        }
+        overlap[u] = ov; count[u] = cc;
+    }
+}
 void or_rigid_packing_fraction(void* h, double* o) { ((Rigid*)h)->packing_fraction(o); }
 void or_rigid_increment_packing_fraction(void* h, const double* inc) { ((Rigid*)h)->increment_packing_fraction(inc); }
 void or_rigid_save_state(void* h, const int* flg, int tv) { ((Rigid*)h)->save_state(flg, tv); }

@@ -58,6 +58,9 @@ _PROTOS = {
     "or_disk_contacts": (None, [vp, vp, vp, vp]),
     "or_disk_pair_obs": (None, [vp, vp, vp, vp, vp, vp, vp]),
     "or_rigid_jam_bisect": (i32, [vp, vp, i32, i32, f64, f64, f64, f64, i32, vp, vp]),
+    "or_rigid_particle_neighbors": (i32, [vp, vp, vp]),
+    "or_rigid_pair_obs": (None, [vp] * 10),
+    "or_rigid_overlaps": (None, [vp, vp, vp]),
     "or_rigid_packing_fraction": (None, [vp, vp]),
     "or_rigid_increment_packing_fraction": (None, [vp, vp]),
     "or_rigid_save_state": (None, [vp, vp, i32]),
@@ -503,6 +506,35 @@ class OracleRigid:
         a, d = self._ps(dt), self._ps(damping); lib().or_rigid_damped_vv_init(self.h, _p(a), _p(d), mode)
 
     def packing_fraction(self): return self._per_sys(lib().or_rigid_packing_fraction)
+
+    def particle_neighbors(self):
+        """build_particle_neighbors: (start[N+1], ids) of particle-level lists."""
+        st = np.zeros(self.N + 1, np.int32)
+        tot = lib().or_rigid_particle_neighbors(self.h, _p(st), None)
+        ids = np.zeros(max(1, tot), np.int32)
+        lib().or_rigid_particle_neighbors(self.h, _p(st), _p(ids))
+        return st, ids[:tot]
+
+    def pair_observables(self):
+        """Per particle-neighbor entry: pair_i, pair_j, dist, fx, fy, vertex_contacts_i/_j,
+        friction; plus contacts[N]."""
+        st, ids = self.particle_neighbors()
+        n = max(1, len(ids))
+        out = dict(pair_i=np.zeros(n, np.int32), pair_j=np.zeros(n, np.int32), dist=np.zeros(n), fx=np.zeros(n),
+                   fy=np.zeros(n), vertex_contacts_i=np.zeros(n, np.int32), vertex_contacts_j=np.zeros(n, np.int32),
+                   friction=np.zeros(n))
+        contacts = np.zeros(self.N, np.int32)
+        lib().or_rigid_pair_obs(self.h, *[_p(out[k]) for k in ("pair_i", "pair_j", "dist", "fx", "fy",
+                                                                 "vertex_contacts_i", "vertex_contacts_j",
+                                                                 "friction")], _p(contacts))
+        out = {k: v[:len(ids)] for k, v in out.items()}
+        out["contacts"] = contacts
+        return out
+
+    def overlaps(self):
+        ov, cc = np.zeros(self.Nv), np.zeros(self.Nv)
+        lib().or_rigid_overlaps(self.h, _p(ov), _p(cc))
+        return ov, cc
 
     def increment_packing_fraction(self, inc):
         a = self._ps(inc); lib().or_rigid_increment_packing_fraction(self.h, _p(a))

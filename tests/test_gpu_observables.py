@@ -52,3 +52,31 @@ def test_hessian_times_vector_matches_adjoint_hvp():
     yx, yy = hvp(P, w[:, 0], w[:, 1], mode=L.FORCE_PBC)
     y = H @ w.reshape(-1)
     np.testing.assert_allclose(np.stack([yx, yy], 1).reshape(-1), y, rtol=0, atol=1e-12 * np.abs(y).max())
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_rigid_contact_observables_match_oracle(boundary):
+    from dpmd_amd.system import RigidBumpy
+    rb = W.rigid_batch(S=3, n_per=32, seed=8, boundary=boundary, phi=0.76)
+    P = RigidBumpy()
+    W.load_rigid_into(P, rb)
+    P.init_neighbors()
+    ref = orc.OracleRigid(rb)
+    mode = L.FORCE_WALL if boundary else L.FORCE_PBC
+    run_fire(P, 1e-2, 300, L.FIRE_PRED_MAX_FORCE, 1e-9, mode=mode)
+    ref.fire_minimize(1e-2, 300, pred=1, ftol=1e-9, mode=mode)
+    gs, gi = P.particle_neighbors()
+    rs, ri = ref.particle_neighbors()
+    assert np.array_equal(gs, rs) and np.array_equal(gi, ri)
+    g = P.pair_observables()
+    r = ref.pair_observables()
+    for k in g:
+        assert np.array_equal(g[k], r[k]), k
+    P.compute_contacts()
+    assert np.array_equal(P.get("CONTACTS"), r["contacts"])
+    P.compute_n_contacts_total()
+    assert np.array_equal(P.get("N_CONTACTS_TOTAL"),
+                          [int(r["contacts"][rb.off[s]:rb.off[s + 1]].sum()) for s in range(rb.S)])
+    P.compute_overlaps()
+    ov, cc = ref.overlaps()
+    assert np.array_equal(P.get("OVERLAP"), ov) and np.array_equal(P.get("CONTACT_COUNT"), cc)

@@ -116,3 +116,122 @@ def test_hessian_matches_finite_differences(packed):
     # eigenvalues from the -k delta / d tangential terms are allowed)
     ev = np.linalg.eigvalsh(H)
     assert ev.min() > -0.05 * ev.max()
+
+
+# ---- rigid bumpy polygons -------------------------------------------------------------
+@pytest.fixture(scope="module")
+def rigid_packed():
+    rb = W.rigid_batch(S=2, n_per=24, seed=8, boundary=0, phi=0.78)
+    o = orc.OracleRigid(rb)
+    o.fire_minimize(1e-2, 300, pred=1, ftol=1e-9)
+    o.check_neighbors(True)      # lists exact at the current positions (Verlet lists are supersets)
+    o.compute_forces(0)
+    return rb, o
+
+
+def rigid_brute(rb, o):
+    """Vertex positions (canonical), particle ids, pair sets from an O(Nv^2) scan."""
+    si = o.static_index()
+    vx, vy = o.get("vpx")[si], o.get("vpy")[si]
+    vr = rb.vrad
+    pid = np.repeat(np.arange(rb.N), rb.nvpp)
+    sid_p = np.repeat(np.arange(rb.S), np.diff(rb.off))
+    vsid = sid_p[pid]
+    near, touch = set(), {}
+    for s in range(rb.S):
+        idx = np.nonzero(vsid == s)[0]
+        L = rb.Lx[s]
+        dx = min_image(vx[idx, None] - vx[None, idx], L)
+        dy = min_image(vy[idx, None] - vy[None, idx], L)
+        d2 = dx * dx + dy * dy
+        rs = vr[idx, None] + vr[None, idx]
+        same = pid[idx, None] == pid[None, idx]
+        nb = (d2 < (rs + rb.cutoff[s]) ** 2) & ~same
+        ov = (d2 < rs * rs) & ~same
+        for a, c in zip(*np.nonzero(nb)):
+            near.add((pid[idx[a]], pid[idx[c]]))
+        for a, c in zip(*np.nonzero(ov)):
+            touch.setdefault((pid[idx[a]], pid[idx[c]]), set()).add(idx[a])
+    return near, touch
+
+
+def test_rigid_particle_neighbors_contacts_brute_force(rigid_packed):
+    rb, o = rigid_packed
+    st, ids = o.particle_neighbors()
+    pairs = {(i, int(j)) for i in range(rb.N) for j in ids[st[i]:st[i + 1]]}
+    near, touch = rigid_brute(rb, o)
+    assert pairs == near
+    for i in range(rb.N):
+        assert np.all(np.diff(ids[st[i]:st[i + 1]]) > 0)          # ascending, no repeats
+    po = o.pair_observables()
+    for k, (i, j) in enumerate(zip(po["pair_i"], po["pair_j"])):
+        assert po["vertex_contacts_i"][k] == len(touch.get((i, j), ()))
+        assert po["vertex_contacts_j"][k] == len(touch.get((j, i), ()))
+    want = np.zeros(rb.N, int)
+    for (i, j) in touch:
+        want[i] += 1
+    assert np.array_equal(po["contacts"], want)
+    assert want.sum() > rb.N                                       # a packing with contacts
+
+
+def test_rigid_pair_forces_sum_to_particle_forces(rigid_packed):
+    rb, o = rigid_packed
+    po = o.pair_observables()
+    fx = np.zeros(rb.N); fy = np.zeros(rb.N)
+    np.add.at(fx, po["pair_i"], po["fx"]); np.add.at(fy, po["pair_i"], po["fy"])
+    scale = np.abs(o.get("fx")).max()
+    np.testing.assert_allclose(fx, o.get("fx"), rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(fy, o.get("fy"), rtol=0, atol=1e-12 * scale)
+    key = {(a, c): k for k, (a, c) in enumerate(zip(po["pair_i"], po["pair_j"]))}
+    for k, (a, c) in enumerate(zip(po["pair_i"], po["pair_j"])):
+        kk = key[(c, a)]
+        assert abs(po["fx"][k] + po["fx"][kk]) <= 1e-12 * scale and abs(po["fy"][k] + po["fy"][kk]) <= 1e-12 * scale
+    # centre distances
+    sid_p = np.repeat(np.arange(rb.S), np.diff(rb.off))
+    x, y = o.get("x"), o.get("y")
+    L = rb.Lx[sid_p[po["pair_i"]]]
+    d = np.hypot(min_image(x[po["pair_i"]] - x[po["pair_j"]], L), min_image(y[po["pair_i"]] - y[po["pair_j"]], L))
+    np.testing.assert_allclose(po["dist"], d, rtol=1e-14)
+
+
+def test_rigid_overlaps_brute_force(rigid_packed):
+    rb, o = rigid_packed
+    ov, cc = o.overlaps()
+    si = o.static_index()
+    vx, vy = o.get("vpx"), o.get("vpy")
+    pid = np.repeat(np.arange(rb.N), rb.nvpp)
+    pid_cur = np.empty_like(pid); pid_cur[si] = pid
+    vr = np.empty(rb.Nv); vr[si] = rb.vrad
+    sid_p = np.repeat(np.arange(rb.S), np.diff(rb.off))
+    for s in range(rb.S):
+        idx = np.nonzero(sid_p[pid_cur] == s)[0]
+        L = rb.Lx[s]
+        d = np.hypot(min_image(vx[idx, None] - vx[None, idx], L), min_image(vy[idx, None] - vy[None, idx], L))
+        rs = vr[idx, None] + vr[None, idx]
+        t = (d < rs) & (pid_cur[idx, None] != pid_cur[None, idx])
+        np.testing.assert_allclose(ov[idx], np.where(t, rs - d, 0.0).sum(1), rtol=1e-12, atol=1e-15)
+        assert np.array_equal(cc[idx], t.sum(1).astype(float))
+
+
+def test_rigid_friction_closed_form():
+    """One vertex contact whose normal makes angle theta with the x axis: the friction ratio is
+    |tan| of the angle between the contact force and the centre line."""
+    r, delta, theta = 0.3, 0.05, 0.4
+    a, b = (2 * r - delta) * np.cos(theta), (2 * r - delta) * np.sin(theta)
+    x0, y0 = 5.0, 5.0
+    v1x, v1y = x0 + 0.5 + a, y0 + b
+    x1, y1 = v1x + 0.5, v1y
+    rb = W.RigidBatch(
+        off=np.array([0, 2], np.int32), nvpp=np.array([2, 2], np.int32),
+        x=np.array([x0, x1]), y=np.array([y0, y1]), angle=np.zeros(2), mass=np.ones(2),
+        inertia=np.array([0.25, 0.25]), area=np.ones(2),
+        vx=np.array([x0 + 0.5, x0 - 0.5, v1x, x1 + 0.5]), vy=np.array([y0, y0, v1y, y1]),
+        vrad=np.full(4, r), vmass=np.full(4, 0.5), Lx=np.array([20.0]), Ly=np.array([20.0]), e=np.array([1.0]),
+        cutoff=np.array([0.2]), thresh2=np.array([0.01]), boundary=np.array([0], np.int32),
+        cdx=np.array([0], np.int32), cdy=np.array([0], np.int32))
+    o = orc.OracleRigid(rb)
+    po = o.pair_observables()
+    assert list(po["pair_i"]) == [0, 1] and list(po["vertex_contacts_i"]) == [1, 1]
+    want = abs(np.tan(theta - np.arctan2(y1 - y0, x1 - x0)))
+    np.testing.assert_allclose(po["friction"], [want, want], rtol=1e-12)
+    np.testing.assert_allclose(np.hypot(po["fx"][0], po["fy"][0]), delta, rtol=1e-12)
