@@ -321,8 +321,14 @@ def run_rigid(args):
     vv.init()
     t1 = time.perf_counter()
     vv.run(n_nve - n_prof, use_graph=not args.no_graph)
+    # The timed segment brackets every vertex-force launch with its own HIP event pair, so it
+    # launches directly; its batches halt for rebuilds (the host finishes those steps) instead
+    # of running the device rebuild chain each step, whose many short launches would leave
+    # the stream waiting on the host between an event and its kernel.
+    L.check(lib.dpmd_set_option(P.ctx, L.OPT_DEVICE_REBUILD, 1), P.ctx)
     lib.dpmd_profile(P.ctx, 1)                        # HIP events around every vertex-force launch
     vv.run(n_prof, use_graph=False)
+    L.check(lib.dpmd_set_option(P.ctx, L.OPT_DEVICE_REBUILD, 0), P.ctx)
     P.synchronize()
     t_nve = time.perf_counter() - t1
     tot_ms, cnt = C.c_double(), C.c_int()

@@ -18,7 +18,7 @@ FORCE_PBC, FORCE_WALL, FORCE_PER_SYSTEM = 0, 1, 2
 ORDER_CURRENT, ORDER_CANONICAL = 0, 1
 FIRE_PRED_REFERENCE, FIRE_PRED_MAX_FORCE, FIRE_PRED_FORCE_OR_PE = 0, 1, 2
 OPT_COOP_REBUILD, OPT_COOP_IN_GRAPH, OPT_FUSED_VV, OPT_BLOCK, OPT_MAX_BATCH = 0, 1, 2, 3, 4
-OPT_PERSISTENT_FIRE, OPT_PERSIST_BLOCK, OPT_PERSIST_PHASE = 5, 6, 7
+OPT_PERSISTENT_FIRE, OPT_PERSIST_BLOCK, OPT_PERSIST_PHASE, OPT_DEVICE_REBUILD = 5, 6, 7, 8
 TUNE_FIRE, TUNE_NVE = 0, 1
 
 STATUS = {0: "OK", 1: "ARGUMENT", 2: "STATE", 3: "HIP", 4: "OVERFLOW"}
@@ -114,6 +114,7 @@ PROTOS = {
     "dpmd_compute_n_contacts_total": (i32, [vp]),
     "dpmd_compute_pair_observables": (i32, [vp, vp, vp, vp, vp, vp, vp, i64]),
     "dpmd_particle_neighbors": (i32, [vp, vp, vp, i64, pi64]),
+    "dpmd_compute_rigid_hessian": (i32, [vp, vp, i64]),
     "dpmd_compute_rigid_pair_observables": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i64]),
     "dpmd_nvt_rescale_run": (i32, [vp, i32, i32, vp, i32, vp, i32, i32, i32]),
     "dpmd_jam_binary_search": (i32, [vp, vp, i32, i32, f64, f64, f64, f64, i32, i32, vp, vp, pi32]),

@@ -346,7 +346,7 @@ void vv_step_body(dpmd_ctx* c) {
 // are captured once per size into hipGraphs when use_graph is set.
 // ---------------------------------------------------------------------------------------
 struct RunCtl {
-    int running, halt, halt_at, conv_at;
+    int running, halt, halt_at, conv_at, rebuilds;
 };
 
 // Turns on frozen-system skipping for the duration of one fused FIRE run.
@@ -360,7 +360,7 @@ RunCtl read_run(dpmd_ctx* c) {
     int run[RUN_WORDS];
     HIP_OK(hipMemcpyAsync(run, c->run.p, sizeof(run), hipMemcpyDeviceToHost, c->st));
     HIP_OK(hipStreamSynchronize(c->st));
-    return RunCtl{run[RUN_RUNNING], run[RUN_HALT], run[RUN_HALT_AT], run[RUN_CONV_AT]};
+    return RunCtl{run[RUN_RUNNING], run[RUN_HALT], run[RUN_HALT_AT], run[RUN_CONV_AT], run[RUN_REBUILDS]};
 }
 
 // Fused velocity-Verlet batches (disks, even batch sizes): drift of step 0 from the
@@ -375,8 +375,11 @@ bool fused_vv_batch(dpmd_ctx* c, int m, bool fire) {
 }
 int any_word(int q) { return q % 2 == 0 ? RUN_ANY : RUN_ANY2; }
 
-void lean_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b) {
-    if (fused_vv_batch(c, m, fire)) {
+// dev: every step rebuilds the flagged systems on the device between its drift and its
+// forces (device-gated rebuild chain), so the batch never halts; used when rebuilds are
+// frequent (many systems, each rebuilding now and then).
+void lean_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b, bool dev) {
+    if (!dev && fused_vv_batch(c, m, fire)) {
         double* P[2] = {c->xy0.p, c->xy1.p};
         disk_vv_drift_to(c, Gate{nullptr, c->run.p, 0}, P[0], any_word(0));
         const bool prof = c->prof == 2 && !c->capturing;
@@ -400,22 +403,24 @@ void lean_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b) {
         const Gate g = Gate{nullptr, c->run.p, k};
         if (fire) {
             if (rigid(c)) rigid_fire_drift(c, g); else disk_fire_drift(c, g);
+            if (dev) rebuild_on_device(c);
             if (rigid(c)) rigid_fire_force(c, g, 1, 1, pred, a, b);
             else disk_fire_force(c, g, 1, 1, pred, a, b);
         } else {
             if (rigid(c)) rigid_vv_drift(c, g); else disk_vv_drift(c, g);
             if (c->rec_seg) adj_record_positions(c, g);
+            if (dev) rebuild_on_device(c);
             if (rigid(c)) rigid_vv_force(c, g); else disk_vv_force(c, g);
         }
     }
 }
 
-hipGraphExec_t capture_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b) {
+hipGraphExec_t capture_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b, bool dev) {
     hipGraph_t g = nullptr;
     HIP_OK(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
     c->capturing = true;
     try {
-        lean_batch(c, m, fire, pred, a, b);
+        lean_batch(c, m, fire, pred, a, b, dev);
     } catch (...) {
         c->capturing = false;
         (void)hipStreamEndCapture(c->st, &g);
@@ -480,7 +485,7 @@ struct GraphScope {
 // number taken and (FIRE) whether all systems stopped.
 int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, double a, double b, bool use_graph,
                            int* conv) {
-    hipGraphExec_t local[8] = {};   // batch sizes 1, 2, 4, ..., 128
+    hipGraphExec_t local[16] = {};   // batch sizes 1, 2, 4, ..., 128; halting / device-rebuild
     hipGraphExec_t* graphs = local;
     const bool cached = use_graph && c->gcache_on;
     if (cached) {
@@ -493,21 +498,39 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
     }
     int total = 0, m = 8, since_halt = 0;
     bool done = false;
+    // Rebuilds per step, smoothed over batches.  A halted batch costs a host round trip and
+    // restarts with small batches; above one rebuild per 5 steps (back below one per 10) the
+    // batches rebuild on the device instead.
+    double rate = 0.0;
+    bool dev = false;
+    int rebuilds = read_run(c).rebuilds;
     try {
         while (!done && total < max_steps) {
             int lg = 0;
+            const bool cell = c->method == DPMD_NEIGHBOR_CELL;
+            dev = cell && (c->dev_rebuild == 2 || (c->dev_rebuild == 0 && rate > (dev ? 0.1 : 0.2)));
+            if (dev) m = std::max(m, 8);
             while ((2 << lg) <= std::min(std::min(m, c->max_batch), max_steps - total) && lg < 7) ++lg;
             const int bm = 1 << lg;
             if (c->rec_seg)   // recorded positions go to slots total + 1 ... of the segment buffer
                 HIP_OK(hipMemcpyAsync(c->run.p + RUN_REC, &total, sizeof(int), hipMemcpyHostToDevice, c->st));
-            const bool fused = fused_vv_batch(c, bm, fire);
+            const bool fused = !dev && fused_vv_batch(c, bm, fire);
+            const int slot = lg + (dev ? 8 : 0);
             if (use_graph) {
-                if (!graphs[lg]) graphs[lg] = capture_batch(c, bm, fire, pred, a, b);
-                HIP_OK(hipGraphLaunch(graphs[lg], c->st));
+                if (!graphs[slot]) graphs[slot] = capture_batch(c, bm, fire, pred, a, b, dev);
+                HIP_OK(hipGraphLaunch(graphs[slot], c->st));
             } else {
-                lean_batch(c, bm, fire, pred, a, b);
+                lean_batch(c, bm, fire, pred, a, b, dev);
             }
             RunCtl r = read_run(c);
+            {
+                const int dn = r.rebuilds - rebuilds;
+                rebuilds = r.rebuilds;
+                int steps_b = bm, events = dn;
+                if (r.halt) { steps_b = r.halt_at + 1; events += 1; }
+                else if (fire && r.running <= 0) steps_b = r.conv_at + 1;
+                rate = 0.5 * rate + 0.5 * (double)events / (double)std::max(1, steps_b);
+            }
             if (c->pend_b) {   // profiled fused batch: keep its event pair if it ran to the end
                 if (!r.halt) {
                     c->prof_ev.push_back(c->pend_b); c->prof_ev.push_back(c->pend_e); c->prof_n.push_back(c->pend_n);
@@ -1512,6 +1535,19 @@ int dpmd_particle_neighbors(dpmd_ctx* c, int* start, int* ids, int64_t ids_len, 
     });
 }
 
+int dpmd_compute_rigid_hessian(dpmd_ctx* c, double* const* hessian, int64_t n) {
+    return guard(c, [&] {
+        need(rigid(c), "rigid hessian: rigid bumpy contexts");
+        need_state(c->nbr_ready, "neighbors are not initialised (call init_neighbors)");
+        need(hessian != nullptr, "rigid hessian: 18 column pointers are required");
+        for (int m = 0; m < 18; ++m) need(hessian[m] != nullptr, "rigid hessian: 18 column pointers are required");
+        const int64_t tot = rigid_particle_neighbors(c);
+        need(n == tot, "rigid hessian: n must equal the number of particle neighbor entries (" + std::to_string(tot) +
+                           ")");
+        if (tot > 0) rigid_hessian(c, hessian);
+    });
+}
+
 int dpmd_compute_rigid_pair_observables(dpmd_ctx* c, int* pair_i, int* pair_j, double* dist, double* fx, double* fy,
                                         int* vertex_contacts_i, int* vertex_contacts_j, double* friction, int64_t n) {
     return guard(c, [&] {
@@ -1835,6 +1871,10 @@ int dpmd_set_option(dpmd_ctx* c, int key, int value) {
                 need(value == 256 || value == 512 || value == 1024, "DPMD_OPT_PERSIST_BLOCK must be 256, 512 or 1024");
                 c->persist_block = value;
                 break;
+            case DPMD_OPT_DEVICE_REBUILD:
+                need(value >= 0 && value <= 2, "DPMD_OPT_DEVICE_REBUILD must be 0, 1 or 2");
+                c->dev_rebuild = value;
+                break;
             default: throw ArgError("unknown option key");
         }
     });
@@ -1851,6 +1891,7 @@ int dpmd_get_option(dpmd_ctx* c, int key, int* value) {
             case DPMD_OPT_PERSIST_PHASE: *value = c->persist_phase; break;
             case DPMD_OPT_PERSIST_BLOCK: *value = c->persist_block; break;
             case DPMD_OPT_MAX_BATCH: *value = c->max_batch; break;
+            case DPMD_OPT_DEVICE_REBUILD: *value = c->dev_rebuild; break;
             default: throw ArgError("unknown option key");
         }
     });

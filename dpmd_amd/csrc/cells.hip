@@ -188,6 +188,14 @@ __global__ void k_tile_down(RebuildArgs r, const int* active) {
     ph_tile_down(r, blockIdx.x, lds);
 }
 
+// Gate of a device-side rebuild: copy the "some flag is set" word (the rebuild clears it)
+// and count the rebuilds run.  One thread.
+__global__ void k_rebuild_gate(int* run) {
+    const int a = run[RUN_ANY] != 0 ? 1 : 0;
+    run[RUN_GATE] = a;
+    if (a) run[RUN_REBUILDS] += 1;
+}
+
 __global__ void k_set_flags(int S, unsigned* flag, const int* active) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S || !*active) return;
@@ -388,6 +396,12 @@ void permute_particles(dpmd_ctx* c, const int* order_host) {
     k_copy_back<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     k_clear_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, r.ca.flag, r.any, active);
     HIP_OK(hipGetLastError());
+}
+
+void rebuild_on_device(dpmd_ctx* c) {
+    k_rebuild_gate<<<1, 1, 0, c->st>>>(c->run.p);
+    HIP_OK(hipGetLastError());
+    rebuild_neighbors(c, c->run.p + RUN_GATE, false);
 }
 
 void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all) {

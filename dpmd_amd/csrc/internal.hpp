@@ -46,8 +46,11 @@ struct NbrArgs {
 //   [9] second "some rebuild flag is set" word: fused velocity-Verlet launches raise the
 //       flags of the positions they write into the word of the next step's parity, so a
 //       launch never reads a word that its own workgroups may be setting
+//  [10] gate of a device-side rebuild inside a step batch (a copy of [7], so clearing [7]
+//       cannot race with the rebuild's own gate reads)
+//  [11] number of device-side rebuilds run (batches that rebuild on the device)
 enum { RUN_MAX = 1, RUN_RUNNING = 2, RUN_ACTIVE = 3, RUN_HALT = 4, RUN_HALT_AT = 5, RUN_CONV_AT = 6, RUN_ANY = 7,
-       RUN_REC = 8, RUN_ANY2 = 9, RUN_WORDS = 10 };
+       RUN_REC = 8, RUN_ANY2 = 9, RUN_GATE = 10, RUN_REBUILDS = 11, RUN_WORDS = 12 };
 
 // Launch gate.  Plain form (run == nullptr): the kernel works iff *active != 0.  Batch form
 // (lean fused runs): a step works while nothing halted and some system runs; the force
@@ -173,7 +176,10 @@ struct dpmd_ctx {
     // abi.hip); key = everything the captured launches take from the context.
     bool gcache_on = false;
     std::vector<int64_t> gcache_key;
-    hipGraphExec_t gcache[8] = {};
+    hipGraphExec_t gcache[16] = {};   // [0, 8): batches that halt on rebuilds, [8, 16): device-side rebuilds
+    // step batches rebuild on the device instead of halting: 0 by the measured rebuild rate,
+    // 1 never, 2 always (DPMD_OPT_DEVICE_REBUILD)
+    int dev_rebuild = 0;
     int g_fire_key[4] = {0, 0, 0, 0};
     double g_fire_keyd[2] = {0, 0};
     int g_vv_key = -1;
@@ -283,6 +289,8 @@ int64_t tune_run(dpmd_ctx* c, int integrator, const double* dt, int mode, int st
 // cells.hip
 void build_chunks(dpmd_ctx* c);
 void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all);
+// inside a step batch: rebuild the flagged systems iff some drift raised a flag this step
+void rebuild_on_device(dpmd_ctx* c);
 void permute_particles(dpmd_ctx* c, const int* order_host);
 void clear_flags(dpmd_ctx* c, const int* active);
 void naive_neighbors(dpmd_ctx* c);
@@ -339,6 +347,7 @@ int64_t rigid_particle_neighbors(dpmd_ctx* c);
 void rigid_pair_obs(dpmd_ctx* c, int* pi, int* pj, double* dist, double* fx, double* fy, int* vci, int* vcj,
                     double* friction);
 void rigid_overlaps(dpmd_ctx* c);
+void rigid_hessian(dpmd_ctx* c, double* const* h18);
 void disk_pair_obs(dpmd_ctx* c, int* pi, int* pj, double* dist, double* fx, double* fy, double* const* h);
 void disk_ke(dpmd_ctx* c);
 void fire_recompute_update(dpmd_ctx* c);

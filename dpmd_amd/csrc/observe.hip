@@ -261,6 +261,68 @@ __global__ void k_rigid_pairs(RigidObs a, RigidPairOut o) {
     if (o.contacts) o.contacts[i] = ncontact;
 }
 
+// compute_hessian_kernel (rigid_bumpy.cu:1129-1259): per particle-neighbor entry (i, j), the
+// contributions of i's vertex contacts with j to the (x, y, theta) blocks H_i,i and H_i,j,
+// expression for expression (unit derivative factors written as the reference writes them).
+// Column 2m holds (i, i) and 2m + 1 holds (i, j) for m = 3a + b over (x, y, theta).  Walls
+// contribute nothing, and vertex offsets from the centres take no minimum image, as there.
+__global__ void k_rigid_hessian(RigidObs a, const int* start, const int* ids, double* const* hs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int s = a.sid[i];
+    const double e = a.e[s];
+    const double Lx = a.Lx[s], Ly = a.Ly[s], iLx = a.iLx[s], iLy = a.iLy[s];
+    const double xi = a.x[i], yi = a.y[i];
+    for (int k = start[i]; k < start[i + 1]; ++k) {
+        const int j = ids[k];
+        const double xj = a.x[j], yj = a.y[j];
+        double hib[9], hjb[9];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) { hib[m] = 0.0; hjb[m] = 0.0; }
+        for (int c = a.poff[i]; c < a.poff[i + 1]; ++c) {
+            const int mu = a.vstatic[c];
+            const double xm = a.vx[mu], ym = a.vy[mu];
+            const double rxm = xm - xi, rym = ym - yi;
+            const int cnt = a.nn[mu];
+            for (int q = 0; q < cnt; ++q) {
+                const int nu = a.nbr[(size_t)q * a.nv + mu];
+                if (a.vpid[nu] != j) continue;
+                const double xn = a.vx[nu], yn = a.vy[nu];
+                const double dx = min_image(xm - xn, Lx, iLx), dy = min_image(ym - yn, Ly, iLy);
+                const double r2 = dx * dx + dy * dy;
+                const double radsum = a.vr[mu] + a.vr[nu];
+                if (r2 >= radsum * radsum) continue;
+                const double rr = sqrt(r2);
+                const double delta = radsum - rr;
+                const double fmag = e * delta;
+                const double t = -fmag, cc = e;
+                const double rxn = xn - xj, ryn = yn - yj;
+#pragma unroll
+                for (int p = 0; p < 3; ++p)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) {
+                        const double ia_x = (p == 0) - rym * (p == 2), ia_y = (p == 1) + rxm * (p == 2);
+                        const double jb_x = -1.0 * ((b == 0) - ryn * (b == 2));
+                        const double jb_y = -1.0 * ((b == 1) + rxn * (b == 2));
+                        const double qa = dx * ia_x + dy * ia_y, qjb = dx * jb_x + dy * jb_y;
+                        const double dq_j = (ia_x * jb_x + ia_y * jb_y) / rr;
+                        hjb[p * 3 + b] += cc * qa * qjb / r2 + t * (dq_j - qa * qjb / (r2 * rr));
+                        const double ib_x = (b == 0) - rym * (b == 2), ib_y = (b == 1) + rxm * (b == 2);
+                        const double iab_x = -rxm * (p == 2) * (b == 2), iab_y = -rym * (p == 2) * (b == 2);
+                        const double qib = dx * ib_x + dy * ib_y;
+                        const double dq_i = (ia_x * ib_x + dx * iab_x + ia_y * ib_y + dy * iab_y) / rr;
+                        hib[p * 3 + b] += cc * qa * qib / r2 + t * (dq_i - qa * qib / (r2 * rr));
+                    }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 9; ++m) {
+            hs[2 * m][k] = 0.0 + hib[m];
+            hs[2 * m + 1][k] = 0.0 + hjb[m];
+        }
+    }
+}
+
 // vertex contacts of the reverse entry: entry (i, j) gets the count entry (j, i) holds
 __global__ void k_rigid_reverse(int n, const int* start, const int* ids, const int* vci, int* vcj) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -345,6 +407,15 @@ void rigid_pair_obs(dpmd_ctx* c, int* pi, int* pj, double* dist, double* fx, dou
         k_rigid_reverse<<<blocks_for(N), DPMD_BLOCK, 0, c->st>>>(N, c->pn_start.p, c->pn_ids.p, o.vci, vcj);
         HIP_OK(hipGetLastError());
     }
+}
+
+void rigid_hessian(dpmd_ctx* c, double* const* h18) {
+    dpmd::DVec<double*> cols;
+    std::vector<double*> hp(h18, h18 + 18);
+    cols.upload(hp.data(), 18, c->st);
+    k_rigid_hessian<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_obs(c), c->pn_start.p, c->pn_ids.p, cols.p);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->st));   // cols is freed on return
 }
 
 void rigid_overlaps(dpmd_ctx* c) {

@@ -257,6 +257,22 @@ class RigidBumpy(_Particles):
         self._c(lib().dpmd_particle_neighbors(self.ctx, st.ctypes.data, ids.ctypes.data, ids.size, C.byref(tot)))
         return st, ids[:tot.value]
 
+    def hessian(self):
+        """compute_hessian per particle-neighbor entry (dpmd_compute_rigid_hessian): dict with
+        'h_i{a}_i{b}' and 'h_i{a}_j{b}' for a, b in x, y, t (theta), plus pair_i / pair_j."""
+        st, ids = self.particle_neighbors()
+        n = len(ids)
+        cols = [DeviceArray(np.zeros(max(1, n))) for _ in range(18)]
+        arr = (C.c_void_p * 18)(*[d.ptr for d in cols])
+        self._c(lib().dpmd_compute_rigid_hessian(self.ctx, arr, n))
+        out = {"pair_i": np.repeat(np.arange(self.n_particles(), dtype=np.int32), np.diff(st)), "pair_j": ids}
+        q = "xyt"
+        for m in range(9):
+            a, b = q[m // 3], q[m % 3]
+            out[f"h_i{a}_i{b}"] = cols[2 * m].to_host()[:n]
+            out[f"h_i{a}_j{b}"] = cols[2 * m + 1].to_host()[:n]
+        return out
+
     def pair_observables(self):
         """Rigid contacts / pair distances / friction / pair forces per particle-neighbor entry
         (dpmd_compute_rigid_pair_observables).  Returns a dict of host arrays."""

@@ -192,6 +192,13 @@ int dpmd_particle_neighbors(dpmd_ctx* ctx, int* start, int* ids, int64_t ids_len
  * vertices from j's; vertex_contacts_i / _j: the reference's pair_vertex_contacts (i's
  * vertices touching j, j's vertices touching i); friction: |F_t| / |F_n| of that force about
  * the centre line (0 when F_n = 0).  Also fills DPMD_F_CONTACTS. */
+/* compute_hessian for rigid bodies (rigid_bumpy.cu:1129-1259,1691-1730) per particle-neighbor
+ * entry k = (i, j) (the order of dpmd_particle_neighbors): hessian is a HOST array of 18
+ * DEVICE pointers of n entries each; column 2m holds entry k's contribution to the diagonal
+ * block H_(i a),(i b) and column 2m + 1 the off-diagonal H_(i a),(j b), m = 3a + b with a, b
+ * over (x, y, theta): the reference's hessian_xx, _xy, _xt, _yx, _yy, _yt, _tx, _ty, _tt
+ * (N_pairs, 2) fields in that order.  Walls contribute nothing, as in the reference. */
+int dpmd_compute_rigid_hessian(dpmd_ctx* ctx, double* const* hessian, int64_t n);
 int dpmd_compute_rigid_pair_observables(dpmd_ctx* ctx, int* pair_i, int* pair_j, double* dist, double* fx,
                                         double* fy, int* vertex_contacts_i, int* vertex_contacts_j,
                                         double* friction, int64_t n);
@@ -368,10 +375,16 @@ int dpmd_stream_copy(double* dst, const double* src, int64_t n, int repeats, dou
  *     per-step launches only.  Same results.
  *   DPMD_OPT_PERSIST_BLOCK: workgroup size of the persistent FIRE kernel: 256, 512 (default)
  *     or 1024.  Same results.
- *   DPMD_OPT_PERSIST_PHASE: steps per persistent phase (default 2000).  Same results. */
+ *   DPMD_OPT_PERSIST_PHASE: steps per persistent phase (default 2000).  Same results.
+ *   DPMD_OPT_DEVICE_REBUILD: how step batches of device-driven runs meet a rebuild.  1: the
+ *     batch halts and the host finishes the step (cheapest when rebuilds are rare); 2: every
+ *     step runs the rebuild chain on the device, gated by the rebuild flags, and batches never
+ *     halt; 0 (default): 1 while rebuilds are rarer than one per 5 steps, 2 above (back below
+ *     one per 10).  Same results. */
 enum dpmd_option {
     DPMD_OPT_COOP_REBUILD = 0, DPMD_OPT_COOP_IN_GRAPH = 1, DPMD_OPT_FUSED_VV = 2, DPMD_OPT_BLOCK = 3,
-    DPMD_OPT_MAX_BATCH = 4, DPMD_OPT_PERSISTENT_FIRE = 5, DPMD_OPT_PERSIST_BLOCK = 6, DPMD_OPT_PERSIST_PHASE = 7
+    DPMD_OPT_MAX_BATCH = 4, DPMD_OPT_PERSISTENT_FIRE = 5, DPMD_OPT_PERSIST_BLOCK = 6, DPMD_OPT_PERSIST_PHASE = 7,
+    DPMD_OPT_DEVICE_REBUILD = 8
 };
 int dpmd_set_option(dpmd_ctx* ctx, int key, int value);
 int dpmd_get_option(dpmd_ctx* ctx, int key, int* value);

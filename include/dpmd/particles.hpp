@@ -302,6 +302,20 @@ public:
         ok(dpmd_compute_rigid_pair_observables(ctx_, pair_ids.xptr(), pair_ids.yptr(), nullptr, pair_forces.xptr(),
                                                pair_forces.yptr(), nullptr, nullptr, nullptr, n));
     }
+    // hessian_ab (N_pairs, 2): .x = entry's share of H_(i a),(i b), .y = H_(i a),(j b)
+    F2 hessian_xx, hessian_xy, hessian_xt, hessian_yx, hessian_yy, hessian_yt, hessian_tx, hessian_ty, hessian_tt;
+    void compute_hessian() {
+        const int n = size_pairs();
+        F2* h[] = {&hessian_xx, &hessian_xy, &hessian_xt, &hessian_yx, &hessian_yy,
+                   &hessian_yt, &hessian_tx, &hessian_ty, &hessian_tt};
+        double* cols[18];
+        for (int m = 0; m < 9; ++m) {
+            h[m]->resize(n);
+            cols[2 * m] = h[m]->xptr();
+            cols[2 * m + 1] = h[m]->yptr();
+        }
+        ok(dpmd_compute_rigid_hessian(ctx_, cols, n));
+    }
     void compute_friction_coeff() {
         const int n = size_pairs();
         friction_coeff.resize(n);

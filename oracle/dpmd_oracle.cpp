@@ -1076,6 +1076,7 @@ static std::vector<double>* rfield(Rigid* r, int f) {
 void or_rigid_get(void* h, int f, double* out) { auto* v = rfield((Rigid*)h, f); std::memcpy(out, v->data(), v->size() * 8); }
 void or_rigid_set(void* h, int f, const double* in) { auto* v = rfield((Rigid*)h, f); std::memcpy(v->data(), in, v->size() * 8); }
 void or_rigid_get_static_index(void* h, int* out) { Rigid* r = (Rigid*)h; std::memcpy(out, r->static_index.data(), r->Nv * 4); }
+void or_rigid_flags(void* h, unsigned* out) { Rigid* r = (Rigid*)h; for (int s = 0; s < r->S; ++s) out[s] = r->flag[s]; }
 void or_rigid_get_cells(void* h, int* nx, int* ny) { Rigid* r = (Rigid*)h; for (int s = 0; s < r->S; ++s) { nx[s] = r->nx[s]; ny[s] = r->ny[s]; } }
 int or_rigid_n_neighbors(void* h) { Rigid* r = (Rigid*)h; long t = 0; for (auto& L : r->nbr) t += L.size(); return (int)t; }
 int or_rigid_max_count(void* h) { return ((Rigid*)h)->max_count; }
@@ -1595,6 +1596,64 @@ void or_rigid_pair_obs(void* h, int* pi, int* pj, double* dist, double* fx, doub
             for (int pl = st[j]; pl < st[j + 1]; ++pl) if (ids[pl] == i) vcj[pk] = vci[pl];
         }
 }
+// compute_hessian_kernel (rigid_bumpy.cu:1129-1259): per particle-neighbor entry (i, j), the
+// contributions of i's vertex contacts with j to the (x, y, theta) blocks H_i,i and H_i,j.
+// hs[(2 m + side) * tot + k], m = 3 a + b over (x, y, theta), side 0 = (i, i), 1 = (i, j).
+// Walls contribute nothing, and vertex offsets from the centres use no minimum image, as in
+// the reference.
+void or_rigid_hessian(void* h, double* hs) {
+    Rigid* r = (Rigid*)h;
+    std::vector<int> st(r->N + 1);
+    const int tot = or_rigid_particle_neighbors(h, st.data(), nullptr);
+    std::vector<int> ids(std::max(1, tot));
+    or_rigid_particle_neighbors(h, st.data(), ids.data());
+    for (int i = 0; i < r->N; ++i) {
+        const int s = r->sid[i];
+        const double e = r->e[s];
+        const double xi = r->x[i], yi = r->y[i];
+        for (int k = st[i]; k < st[i + 1]; ++k) {
+            const int j = ids[k];
+            const double xj = r->x[j], yj = r->y[j];
+            double hib[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, hjb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int c = r->poff[i]; c < r->poff[i + 1]; ++c) {
+                const int mu = r->static_index[c];
+                const double rxm = r->vpx[mu] - xi, rym = r->vpy[mu] - yi;
+                for (int nu : r->nbr[mu]) {
+                    if (r->vpid[nu] != j) continue;
+                    double dx, dy;
+                    const double r2 = disp_pbc_L(r->vpx[mu], r->vpy[mu], r->vpx[nu], r->vpy[nu], r->Lx[s], r->Ly[s],
+                                                 r->ix[s], r->iy[s], dx, dy);
+                    const double radsum = r->vrad[mu] + r->vrad[nu];
+                    if (r2 >= radsum * radsum) continue;
+                    const double rr = std::sqrt(r2);
+                    const double delta = radsum - rr;
+                    const double fmag = e * delta;
+                    const double t = -fmag, cc = e;
+                    const double rxn = r->vpx[nu] - xj, ryn = r->vpy[nu] - yj;
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) {
+                            const double ia_x = (a == 0) - rym * (a == 2), ia_y = (a == 1) + rxm * (a == 2);
+                            const double jb_x = -1.0 * ((b == 0) - ryn * (b == 2));
+                            const double jb_y = -1.0 * ((b == 1) + rxn * (b == 2));
+                            const double qa = dx * ia_x + dy * ia_y, qjb = dx * jb_x + dy * jb_y;
+                            const double dq_j = (ia_x * jb_x + ia_y * jb_y) / rr;
+                            hjb[a * 3 + b] += cc * qa * qjb / r2 + t * (dq_j - qa * qjb / (r2 * rr));
+                            const double ib_x = (b == 0) - rym * (b == 2), ib_y = (b == 1) + rxm * (b == 2);
+                            const double iab_x = -rxm * (a == 2) * (b == 2), iab_y = -rym * (a == 2) * (b == 2);
+                            const double qib = dx * ib_x + dy * ib_y;
+                            const double dq_i = (ia_x * ib_x + dx * iab_x + ia_y * ib_y + dy * iab_y) / rr;
+                            hib[a * 3 + b] += cc * qa * qib / r2 + t * (dq_i - qa * qib / (r2 * rr));
+                        }
+                }
+            }
+            for (int m = 0; m < 9; ++m) {
+                hs[(2 * m) * (size_t)tot + k] = 0.0 + hib[m];
+                hs[(2 * m + 1) * (size_t)tot + k] = 0.0 + hjb[m];
+            }
+        }
+    }
+}
+
 // compute_overlaps_kernel (rigid_bumpy.cu:929-966), per vertex in current order
 void or_rigid_overlaps(void* h, double* overlap, double* count) {
     Rigid* r = (Rigid*)h;

@@ -61,6 +61,7 @@ _PROTOS = {
     "or_rigid_particle_neighbors": (i32, [vp, vp, vp]),
     "or_rigid_pair_obs": (None, [vp] * 10),
     "or_rigid_overlaps": (None, [vp, vp, vp]),
+    "or_rigid_hessian": (None, [vp, vp]),
     "or_rigid_packing_fraction": (None, [vp, vp]),
     "or_rigid_increment_packing_fraction": (None, [vp, vp]),
     "or_rigid_save_state": (None, [vp, vp, i32]),
@@ -529,6 +530,21 @@ class OracleRigid:
                                                                  "friction")], _p(contacts))
         out = {k: v[:len(ids)] for k, v in out.items()}
         out["contacts"] = contacts
+        return out
+
+    def hessian(self):
+        """compute_hessian per particle-neighbor entry: dict 'h_ia_ib' / 'h_ia_jb' for a, b in
+        x, y, t (theta), plus pair_i / pair_j."""
+        st, ids = self.particle_neighbors()
+        n = len(ids)
+        hs = np.zeros(18 * max(1, n))
+        lib().or_rigid_hessian(self.h, _p(hs))
+        out = {"pair_i": np.repeat(np.arange(self.N, dtype=np.int32), np.diff(st)), "pair_j": ids}
+        q = "xyt"
+        for m in range(9):
+            a, b = q[m // 3], q[m % 3]
+            out[f"h_i{a}_i{b}"] = hs[(2 * m) * n:(2 * m + 1) * n].copy()
+            out[f"h_i{a}_j{b}"] = hs[(2 * m + 1) * n:(2 * m + 2) * n].copy()
         return out
 
     def overlaps(self):

@@ -80,3 +80,18 @@ def test_rigid_contact_observables_match_oracle(boundary):
     P.compute_overlaps()
     ov, cc = ref.overlaps()
     assert np.array_equal(P.get("OVERLAP"), ov) and np.array_equal(P.get("CONTACT_COUNT"), cc)
+
+
+def test_rigid_hessian_matches_oracle():
+    from dpmd_amd.system import RigidBumpy
+    rb = W.rigid_batch(S=2, n_per=24, seed=4, boundary=0, phi=0.82)
+    P = RigidBumpy()
+    W.load_rigid_into(P, rb)
+    P.init_neighbors()
+    ref = orc.OracleRigid(rb)
+    run_fire(P, 1e-2, 40, L.FIRE_PRED_MAX_FORCE, 1e-9, mode=L.FORCE_PBC)
+    ref.fire_minimize(1e-2, 40, pred=1, ftol=1e-9, mode=0)
+    g, r = P.hessian(), ref.hessian()
+    assert set(g) == set(r)
+    for k in r:
+        assert np.array_equal(g[k], r[k]), k

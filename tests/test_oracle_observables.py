@@ -235,3 +235,54 @@ def test_rigid_friction_closed_form():
     want = abs(np.tan(theta - np.arctan2(y1 - y0, x1 - x0)))
     np.testing.assert_allclose(po["friction"], [want, want], rtol=1e-12)
     np.testing.assert_allclose(np.hypot(po["fx"][0], po["fy"][0]), delta, rtol=1e-12)
+
+
+def assemble_rigid(po, N):
+    H = np.zeros((3 * N, 3 * N))
+    q = "xyt"
+    for a in range(3):
+        for bb in range(3):
+            rows = 3 * po["pair_i"] + a
+            np.add.at(H, (rows, 3 * po["pair_i"] + bb), po[f"h_i{q[a]}_i{q[bb]}"])
+            np.add.at(H, (rows, 3 * po["pair_j"] + bb), po[f"h_i{q[a]}_j{q[bb]}"])
+    return H
+
+
+def test_rigid_hessian_matches_finite_differences():
+    """H over (x, y, theta) per body against central differences of the centre forces and
+    torques (rigid moves of one body's vertices); periodic box, no walls."""
+    rb = W.rigid_batch(S=1, n_per=16, seed=4, boundary=0, phi=0.82)
+    o = orc.OracleRigid(rb)
+    o.fire_minimize(1e-2, 40, pred=1, ftol=1e-9, mode=0)    # a few steps: contacts, not a minimum
+    o.check_neighbors(True)
+    o.compute_forces(0)
+    N = rb.N
+    H = assemble_rigid(o.hessian(), N)
+    si = o.static_index()
+    x0, y0, vx0, vy0 = o.get("x").copy(), o.get("y").copy(), o.get("vpx").copy(), o.get("vpy").copy()
+    off = np.concatenate([[0], np.cumsum(rb.nvpp)])
+    h = 1e-6
+    J = np.zeros((3 * N, 3 * N))
+    for c in range(N):
+        vs = si[off[c]:off[c + 1]]
+        for a in range(3):
+            cols = []
+            for sgn in (1.0, -1.0):
+                x, y, vx, vy = x0.copy(), y0.copy(), vx0.copy(), vy0.copy()
+                if a == 0:
+                    x[c] += sgn * h; vx[vs] += sgn * h
+                elif a == 1:
+                    y[c] += sgn * h; vy[vs] += sgn * h
+                else:
+                    cs, sn = np.cos(sgn * h), np.sin(sgn * h)
+                    dx, dy = vx0[vs] - x0[c], vy0[vs] - y0[c]
+                    vx[vs] = x0[c] + cs * dx - sn * dy
+                    vy[vs] = y0[c] + sn * dx + cs * dy
+                o.set("x", x); o.set("y", y); o.set("vpx", vx); o.set("vpy", vy)
+                o.compute_forces(0)
+                cols.append(np.stack([o.get("fx"), o.get("fy"), o.get("torque")], 1).reshape(-1))
+            J[:, 3 * c + a] = -(cols[0] - cols[1]) / (2 * h)
+    o.set("x", x0); o.set("y", y0); o.set("vpx", vx0); o.set("vpy", vy0)
+    o.compute_forces(0)
+    assert np.abs(H).max() > 0.1
+    np.testing.assert_allclose(H, J, rtol=0, atol=1e-6 * np.abs(H).max())
