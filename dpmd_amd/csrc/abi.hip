@@ -178,6 +178,7 @@ void sync_impl(dpmd_ctx* c) {
         auto nvpp = dl(c, c->nvpp);
         std::vector<int> poff(N + 1, 0), vpid(c->Nv), vsid(c->Nv);
         for (int i = 0; i < N; ++i) { need(nvpp[i] > 0, "every rigid particle needs at least one vertex"); poff[i + 1] = poff[i] + nvpp[i]; }
+        c->nv_max = *std::max_element(nvpp.begin(), nvpp.end());
         need(poff[N] == c->Nv, "sum of n_vertices_per_particle must equal n_vertices");
         c->h_voff.assign(S + 1, 0);
         for (int s = 0; s < S; ++s) c->h_voff[s + 1] = poff[c->h_off[s + 1]];

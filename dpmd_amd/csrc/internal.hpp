@@ -223,6 +223,7 @@ struct dpmd_ctx {
     hipEvent_t pend_b = nullptr, pend_e = nullptr;   // pair of the batch in flight (kind 2)
     int pend_n = 0;
 
+    int nv_max = 0;   // rigid bodies: most vertices of one particle (set by sync)
     int n_obj() const { return kind == DPMD_RIGID_BUMPY ? Nv : N; }
 };
 

@@ -403,6 +403,116 @@ k_vv_particle(RigidArgs a, const double* __restrict__ dth, const double* __restr
     a.vx[i] = vx; a.vy[i] = vy; a.w[i] = w;
 }
 
+// ---- velocity Verlet with one thread per vertex ------------------------------------------
+// A workgroup takes G consecutive particles (G = DPMD_BLOCK / most vertices per particle, so
+// their vertices fit one thread each).  Per-particle work (kicks, the centre and angle drift,
+// the rotation's sin and cos, the final sums) runs on G threads; per-vertex work (rotating a
+// vertex, loading its force and forming its torque term) on one thread per vertex, with the
+// particle's values passed through LDS.  Sums still run over the vertices in canonical order
+// on one thread, so the results are those of the one-thread-per-particle kernels.
+constexpr int kRigidGroupMax = DPMD_BLOCK;
+
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_vv_drift_v(RigidArgs a, const double* __restrict__ dt, const double* __restrict__ dth, int G, Gate g) {
+    if (!gate_open(g)) return;
+    __shared__ double l_x0[kRigidGroupMax], l_y0[kRigidGroupMax], l_nx[kRigidGroupMax], l_ny[kRigidGroupMax];
+    __shared__ double l_cs[kRigidGroupMax], l_sn[kRigidGroupMax];
+    __shared__ int l_off[kRigidGroupMax + 1];
+    const int p0 = blockIdx.x * G;
+    const int np = min(G, a.n - p0);
+    const int t = threadIdx.x;
+    if (t < np) {
+        const int i = p0 + t;
+        const int s = a.sid[i];
+        double vx = a.vx[i], vy = a.vy[i], w = a.w[i];
+        kick_one(a, i, dth[s] * 1.0, a.fx[i], a.fy[i], a.tq[i], vx, vy, w);
+        const double sc = dt[s] * 1.0;
+        const double x0 = a.x[i], y0 = a.y[i], a0 = a.th[i];
+        const double nx = x0 + vx * sc, ny = y0 + vy * sc, na = a0 + w * sc;
+        double sn, cs;
+        own_sincos(na - a0, &sn, &cs);
+        a.x[i] = nx; a.y[i] = ny; a.th[i] = na; a.vx[i] = vx; a.vy[i] = vy; a.w[i] = w;
+        l_x0[t] = x0; l_y0[t] = y0; l_nx[t] = nx; l_ny[t] = ny; l_cs[t] = cs; l_sn[t] = sn;
+    }
+    if (t <= np) l_off[t] = a.poff[p0 + t];
+    __syncthreads();
+    const int c = l_off[0] + t;
+    const bool valid = c < l_off[np];
+    int q = 0, s = -1;
+    bool flag = false;
+    if (valid) {
+        while (l_off[q + 1] <= c) ++q;
+        const int i = p0 + q;
+        s = a.sid[i];
+        const int v = a.vstatic[c];
+        const double dx = a.vpx[v] - l_x0[q], dy = a.vpy[v] - l_y0[q];
+        const double cs = l_cs[q], sn = l_sn[q];
+        const double rx = cs * dx - sn * dy, ry = sn * dx + cs * dy;
+        const double vxn = l_nx[q] + rx, vyn = l_ny[q] + ry;
+        a.vpx[v] = vxn;
+        a.vpy[v] = vyn;
+        if (a.cell && c == l_off[q]) {   // the skin test uses the particle's first vertex
+            const double ddx = vxn - a.lx[i], ddy = vyn - a.ly[i];
+            flag = (ddx * ddx + ddy * ddy) > a.thresh2[s];
+        }
+    }
+    if (a.cell) raise_flags(a.flag, a.any, s, flag);
+}
+
+// Centre force, torque and energy of G particles (particle_force_one's sums), then, with
+// kick, the closing half kick (k_vv_particle).
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_particle_v(RigidArgs a, const double* __restrict__ dth, const double* __restrict__ damp, int G, int kick, Gate g) {
+    if (!gate_open(g)) return;
+    __shared__ double l_fx[DPMD_BLOCK], l_fy[DPMD_BLOCK], l_pe[DPMD_BLOCK], l_tq[DPMD_BLOCK];
+    __shared__ int l_off[kRigidGroupMax + 1];
+    const int p0 = blockIdx.x * G;
+    const int np = min(G, a.n - p0);
+    const int t = threadIdx.x;
+    if (t <= np) l_off[t] = a.poff[p0 + t];
+    __syncthreads();
+    const int c = l_off[0] + t;
+    if (c < l_off[np]) {
+        int q = 0;
+        while (l_off[q + 1] <= c) ++q;
+        const int i = p0 + q;
+        const int v = a.vstatic[c];
+        const double fvx = a.vfx[v], fvy = a.vfy[v];
+        const double dx = a.vpx[v] - a.x[i], dy = a.vpy[v] - a.y[i];
+        l_fx[t] = fvx; l_fy[t] = fvy; l_pe[t] = a.vpe[v];
+        l_tq[t] = fvy * dx - fvx * dy;
+    }
+    __syncthreads();
+    if (t >= np) return;
+    const int i = p0 + t;
+    double fx = 0.0, fy = 0.0, tq = 0.0, pe = 0.0;
+    for (int k = l_off[t] - l_off[0]; k < l_off[t + 1] - l_off[0]; ++k) {
+        fx += l_fx[k];
+        fy += l_fy[k];
+        pe += l_pe[k];
+        tq += l_tq[k];
+    }
+    double vx = a.vx[i], vy = a.vy[i], w = a.w[i];
+    if (damp) {   // compute_damping_forces_kernel (rigid_bumpy.cu:310-327) at the half-step velocity
+        const double sc = damp[a.sid[i]];
+        fx += -vx * sc;
+        fy += -vy * sc;
+        tq += -w * sc;
+    }
+    a.fx[i] = fx; a.fy[i] = fy; a.tq[i] = tq; a.pe[i] = pe;
+    if (kick) {
+        kick_one(a, i, dth[a.sid[i]] * 1.0, fx, fy, tq, vx, vy, w);
+        a.vx[i] = vx; a.vy[i] = vy; a.w[i] = w;
+    }
+}
+
+// Particles per workgroup of the vertex-parallel kernels (0: use the per-particle kernels);
+// at most DPMD_BLOCK - 1, so the G + 1 particle offsets are written by distinct threads.
+int rigid_group(dpmd_ctx* c) {
+    if (c->nv_max <= 0 || c->nv_max > DPMD_BLOCK) return 0;
+    return std::min(DPMD_BLOCK - 1, DPMD_BLOCK / c->nv_max);
+}
+
 void launch_vertex_force(dpmd_ctx* c, int mode, Gate g) {
     RigidArgs a = rigid_args(c);
     const int G = blocks_for(c->Nv);
@@ -418,8 +528,12 @@ void launch_vertex_force(dpmd_ctx* c, int mode, Gate g) {
 }  // namespace
 
 void rigid_compute_forces(dpmd_ctx* c, int mode) {
-    launch_vertex_force(c, mode, Gate{c->one.p, nullptr, 0});
-    k_particle_force<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c));
+    const Gate g{c->one.p, nullptr, 0};
+    launch_vertex_force(c, mode, g);
+    if (const int G = rigid_group(c))
+        k_particle_v<<<(c->N + G - 1) / G, DPMD_BLOCK, 0, c->st>>>(rigid_args(c), nullptr, nullptr, G, 0, g);
+    else
+        k_particle_force<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c));
     HIP_OK(hipGetLastError());
 }
 void rigid_update_positions(dpmd_ctx* c, const double* scale, double scale2) {
@@ -466,12 +580,18 @@ void rigid_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, doub
     fire_finalize(c, g, do_conv, pred, pa, pb);
 }
 void rigid_vv_drift(dpmd_ctx* c, Gate g) {
-    k_vv_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vv_dt.p, c->vv_dth.p, g);
+    if (const int G = rigid_group(c))
+        k_vv_drift_v<<<(c->N + G - 1) / G, DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vv_dt.p, c->vv_dth.p, G, g);
+    else
+        k_vv_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vv_dt.p, c->vv_dth.p, g);
     HIP_OK(hipGetLastError());
 }
 void rigid_vv_force(dpmd_ctx* c, Gate g) {
     launch_vertex_force(c, c->vv_mode, g);
-    k_vv_particle<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vv_dth.p, vv_damping(c), g);
+    if (const int G = rigid_group(c))
+        k_particle_v<<<(c->N + G - 1) / G, DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vv_dth.p, vv_damping(c), G, 1, g);
+    else
+        k_vv_particle<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vv_dth.p, vv_damping(c), g);
     HIP_OK(hipGetLastError());
 }
 
