@@ -68,6 +68,7 @@ class _Particles:
         check(lib().dpmd_ctx_create(C.byref(p), self.kind, stream))
         self.ctx = p.value
         self._keep = []
+        self.neighbor_method = L.NEIGHBOR_NAIVE
 
     def __del__(self):
         try:
@@ -92,6 +93,7 @@ class _Particles:
 
     def set_neighbor_method(self, m: int):
         self._c(lib().dpmd_set_neighbor_method(self.ctx, m))
+        self.neighbor_method = m
 
     def field_len(self, name: str) -> int:
         n = C.c_int64()
@@ -204,6 +206,7 @@ class _Particles:
 
     def zero_velocities(self): self._c(lib().dpmd_zero_velocities(self.ctx))
 
+    def compute_ke(self): self._c(lib().dpmd_compute_ke(self.ctx))
     def compute_ke_total(self): self._c(lib().dpmd_compute_ke_total(self.ctx))
     def compute_pe_total(self): self._c(lib().dpmd_compute_pe_total(self.ctx))
     def compute_fpower_total(self): self._c(lib().dpmd_compute_fpower_total(self.ctx))

@@ -4,17 +4,21 @@
 // tests/fire_disk_pbc.cu).  It writes the final canonical state to a binary file so the
 // test harness can compare it with the same sequence issued through the Python binding.
 //
-// usage: facade_test <out.bin>
+// With a second argument it then runs 60 more velocity-Verlet steps under io::OutputManager
+// (the reference's include/utils/output_manager.hpp), writing its output directory there.
+//
+// usage: facade_test <out.bin> [<output dir>]
 #include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <iostream>
 #include <vector>
 
+#include "dpmd/output_manager.hpp"
 #include "dpmd/routines.hpp"
 
 int main(int argc, char** argv) {
-    if (argc != 2) { std::cerr << "usage: facade_test <out.bin>\n"; return 2; }
+    if (argc != 2 && argc != 3) { std::cerr << "usage: facade_test <out.bin> [<output dir>]\n"; return 2; }
     const int S = 2, n_per = 128, N = S * n_per;
     const double phi = 0.80;
     std::vector<double> rad(N), mass(N, 1.0), area(N);
@@ -111,5 +115,24 @@ int main(int argc, char** argv) {
     std::cout << "facade_test: fire_steps=" << steps << " jam_rounds=" << rounds << " contacts=" << nct[0] << ","
               << nct[1] << " hessian_xx_sum=" << trace << " n_neighbors=" << P.n_neighbors()
               << " n_cells=" << P.n_cells() << " pe=" << pe[0] << "," << pe[1] << std::endl;
+
+    if (argc == 3) {   // the reference's OutputManager call pattern around an integrator loop
+        io::OutputManager<md::disk::Disk> om(P, argv[2], 2);
+        om.set_trajectory_fields({"pos", "vel", "pe_total", "ke_total", "temperature", "contacts", "n_contacts_total"});
+        om.set_trajectory_interval(7);
+        om.set_restart_interval(25);
+        om.set_extra_init_fields({"packing_fraction"});
+        om.set_extra_final_fields({"pe", "pressure", "stress_tensor_total_x"});
+        om.set_queue_limit(3);
+        om.initialize();
+        md::integrators::VelocityVerlet<md::disk::Disk> vv2(P, 1e-2);
+        vv2.init();
+        for (int k = 1; k <= 60; ++k) {
+            vv2.step();
+            om.step(k);
+        }
+        om.finalize();
+        std::cout << "facade_test: output written to " << argv[2] << std::endl;
+    }
     return 0;
 }

@@ -61,7 +61,7 @@ def python_sequence():
     c = L.ORDER_CANONICAL
     state = np.concatenate([P.get("POS_X", c), P.get("POS_Y", c), P.get("VEL_X", c), P.get("VEL_Y", c),
                             pe0, P.get("PE_TOTAL"), P.get("KE_TOTAL"), P.get("PRESSURE")])
-    return state, steps, rounds
+    return state, steps, rounds, P
 
 
 def test_cpp_facade_matches_python_binding(tmp_path):
@@ -72,7 +72,83 @@ def test_cpp_facade_matches_python_binding(tmp_path):
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     cpp = np.fromfile(out, dtype=np.float64)
-    py, steps, rounds = python_sequence()
+    py, steps, rounds, _ = python_sequence()
     assert f"fire_steps={steps} jam_rounds={rounds}" in r.stdout
     assert cpp.shape == py.shape
     assert np.array_equal(cpp, py)
+
+
+def test_cpp_output_manager_matches_python_output_manager(tmp_path):
+    """io::OutputManager (C++) and dpmd_amd.output.OutputManager write the same files, bit for
+    bit, for the same run; the frames are the fields read directly after those steps."""
+    from dpmd_amd import output
+    exe = ROOT / "tests" / "cpp" / "facade_test.bin"
+    r = subprocess.run([str(exe), str(tmp_path / "state.bin"), str(tmp_path / "cpp")], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    _, _, _, P = python_sequence()
+    om = output.OutputManager(P, tmp_path / "py", max_workers=2)
+    traj = ["pos", "vel", "pe_total", "ke_total", "temperature", "contacts", "n_contacts_total"]
+    om.set_trajectory_fields(traj)
+    om.set_trajectory_interval(7)
+    om.set_restart_interval(25)
+    om.set_extra_init_fields(["packing_fraction"])
+    om.set_extra_final_fields(["pe", "pressure", "stress_tensor_total_x"])
+    om.set_queue_limit(3)
+    om.initialize()
+    vv = VelocityVerlet(P, 1e-2)
+    vv.init()
+    direct = {}
+    for k in range(1, 61):
+        vv.step(1)
+        if k % 7 == 0:
+            direct[k] = P.get("POS_X", L.ORDER_CANONICAL).copy()
+        om.step(k)
+    om.finalize()
+    a, b = output.load(tmp_path / "cpp"), output.load(tmp_path / "py")
+    assert np.array_equal(a["timestep"], np.arange(7, 61, 7))
+    assert np.array_equal(a["timestep"], b["timestep"])
+    for g in ("static", "init", "restart", "final"):
+        assert set(a[g]) == set(b[g]), g
+        for n in a[g]:
+            assert a[g][n].dtype == b[g][n].dtype and np.array_equal(a[g][n], b[g][n]), (g, n)
+    assert set(a["trajectory"]) == set(traj) == set(b["trajectory"])
+    for n in traj:
+        assert np.array_equal(a["trajectory"][n], b["trajectory"][n]), n
+    assert a["trajectory"]["pos"].shape == (8, P.n_particles(), 2)
+    for t, k in enumerate(a["timestep"]):
+        assert np.array_equal(a["trajectory"]["pos"][t, :, 0], direct[int(k)])
+    assert int(a["restart"]["step"][0]) == 60 and int(a["final"]["step"][0]) == 60
+    assert np.array_equal(a["final"]["pos"][:, 0], P.get("POS_X", L.ORDER_CANONICAL))
+    assert np.array_equal(a["static"]["rad"], P.get("RAD", L.ORDER_CANONICAL))
+    assert "cell_dim" in a["static"] and a["static"]["box_size"].shape == (2, 2)
+
+
+def test_python_output_manager_rigid(tmp_path):
+    """Rigid bodies: the default groups are the reference's (vertex fields, angles), per-particle
+    fields keep the load order, per-vertex fields come back in canonical vertex order."""
+    from dpmd_amd import output
+    from dpmd_amd.system import RigidBumpy
+    rb = W.rigid_batch(S=2, n_per=16, seed=5, boundary=0, phi=0.70)
+    P = RigidBumpy()
+    W.load_rigid_into(P, rb)
+    P.init_neighbors()
+    om = output.OutputManager(P, tmp_path / "rb", max_workers=2)
+    om.set_trajectory_fields(["pos", "angle", "vertex_pos", "pe_total"])
+    om.set_trajectory_interval(10)
+    om.initialize()
+    vv = VelocityVerlet(P, 1e-2)
+    vv.init()
+    for k in range(1, 31):
+        vv.step(1)
+        om.step(k)
+    om.finalize()
+    d = output.load(tmp_path / "rb")
+    assert np.array_equal(d["timestep"], [10, 20, 30])
+    assert d["trajectory"]["vertex_pos"].shape == (3, rb.Nv, 2)
+    assert np.array_equal(d["trajectory"]["angle"][-1], P.get("ANGLE"))
+    assert np.array_equal(d["final"]["vertex_pos"][:, 1], P.get("VERTEX_POS_Y", L.ORDER_CANONICAL))
+    assert {"vertex_pos", "pos", "vel", "force", "torque", "angular_vel", "angle"} <= set(d["final"])
+    assert {"vertex_mass", "vertex_rad", "n_vertices_per_particle", "particle_offset", "moment_inertia"} <= \
+        set(d["static"])
