@@ -69,6 +69,7 @@ PROTOS = {
     "dpmd_allocate_particles": (i32, [vp, i32]),
     "dpmd_allocate_vertices": (i32, [vp, i32]),
     "dpmd_set_neighbor_method": (i32, [vp, i32]),
+    "dpmd_ctx_info": (i32, [vp, pi32, pi32]),
     "dpmd_sizes": (i32, [vp, pi32, pi32, pi32, pi32]),
     "dpmd_field": (i32, [vp, i32, C.POINTER(vp), pi64]),
     "dpmd_set_field": (i32, [vp, i32, vp, i64]),

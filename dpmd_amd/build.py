@@ -54,5 +54,51 @@ def build(verbose: bool = True) -> Path:
     return LIB
 
 
+# ---- optional HDF5 I/O library (libdpmd_h5.so) -----------------------------------------
+H5LIB = HERE / "libdpmd_h5.so"
+H5DEPS = HERE / "h5lib"     # private links to the HDF5 runtime, so nothing else is taken from its prefix
+
+
+def find_hdf5():
+    """An HDF5 C installation (header and shared library): $DPMD_HDF5_ROOT, /usr, /opt/conda."""
+    roots = [os.environ.get("DPMD_HDF5_ROOT", ""), "/usr", "/usr/lib/x86_64-linux-gnu/hdf5/serial", "/opt/conda"]
+    for r in filter(None, roots):
+        inc, lib = Path(r) / "include", Path(r) / "lib"
+        for so in sorted(lib.glob("libhdf5.so.*")) + sorted((lib / "x86_64-linux-gnu").glob("libhdf5*.so.*")):
+            if (inc / "hdf5.h").exists() and so.name.count(".") == 2 and "_" not in so.name:
+                return inc, so
+    return None
+
+
+def build_h5(verbose: bool = True):
+    """libdpmd_h5.so, linked to libdpmd_amd.so and the HDF5 runtime; skipped (and any stale
+    copy removed) when no HDF5 installation is found."""
+    found = find_hdf5()
+    if found is None:
+        H5LIB.unlink(missing_ok=True)
+        if verbose:
+            print("HDF5 not found: libdpmd_h5.so not built", file=sys.stderr)
+        return None
+    inc, so = found
+    H5DEPS.mkdir(exist_ok=True)
+    for dep in (so, *sorted(so.parent.glob("libz.so.1"))):
+        link = H5DEPS / dep.name
+        if link.is_symlink() or link.exists():
+            link.unlink()
+        link.symlink_to(dep)
+    tmp = H5LIB.with_suffix(".so.tmp")
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", f"-I{HERE.parent / 'include'}", f"-I{inc}",
+           str(HERE / "h5" / "dpmd_h5.cpp"), "-o", str(tmp), f"-L{HERE}", "-ldpmd_amd", str(H5DEPS / so.name),
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/h5lib", "-Wl,-z,defs"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"libdpmd_h5 build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, H5LIB)
+    if verbose:
+        print(f"built {H5LIB} against {so}", file=sys.stderr)
+    return H5LIB
+
+
 if __name__ == "__main__":
     build()
+    build_h5()

@@ -1298,6 +1298,13 @@ int dpmd_set_neighbor_capacity(dpmd_ctx* c, int k) {
     });
 }
 
+int dpmd_ctx_info(const dpmd_ctx* c, int* kind, int* method) {
+    if (!c) return DPMD_ERR_ARGUMENT;
+    if (kind) *kind = c->kind;
+    if (method) *method = c->method;
+    return DPMD_OK;
+}
+
 int dpmd_sizes(const dpmd_ctx* c, int* S, int* N, int* Nv, int* C) {
     if (!c) return DPMD_ERR_ARGUMENT;
     if (S) *S = c->S;

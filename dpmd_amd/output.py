@@ -1,24 +1,30 @@
-"""Trajectory and snapshot output, and its reader.
+"""Trajectory and snapshot output, and its readers.
 
 `OutputManager` mirrors the reference's io::OutputManager<P> (include/utils/output_manager.hpp:
-124-1085) for the Python binding, and include/dpmd/output_manager.hpp does the same for C++
-callers; both write the same directory layout, which `load` reads back:
+124-1085) for the Python binding; include/dpmd/output_manager.hpp does the same for C++
+callers.  Both write one of two containers:
 
-    <path>/meta/{static,init,restart,final}/<field>.npy   one array per field (+ step.npy)
-    <path>/trajectory/<field>.bin                          frames appended, raw little-endian
-    <path>/trajectory/timestep.bin                         int64 step of every frame
-    <path>/trajectory/index.json                           dtype and frame shape per field
+* "h5" (dpmd_amd.h5, needs libdpmd_h5): the reference's meta.h5 / trajectory.h5, which
+  BaseParticle::load_from_hdf5 and dpmd_amd.h5.load_from_hdf5 read back;
+* "npy" (no dependencies): the same groups as a directory of NumPy files,
 
-The reference writes the same groups to HDF5 (meta.h5 / trajectory.h5); HDF5 is not available
-here.  Per-particle fields are written in canonical (load) order, the reindexing the reference
-applies when the particles are cell-sorted (output_manager.hpp:532-574).  Per-system totals are
-computed before they are read, as the reference's registry preprocess hooks do.
+      <path>/meta/static/run.json                            class name, neighbor method, sizes
+      <path>/meta/{static,init,restart,final}/<field>.npy   one array per field (+ step.npy)
+      <path>/trajectory/<field>.bin, <field>.json            frames appended; dtype and frame shape
+      <path>/trajectory/timestep.bin                         int64 step of every frame
+      <path>/trajectory/ragged/<field>/step_<k>.npy          frames whose shape changed
+
+`load` reads either.  Per-particle fields are written in canonical (load) order, the
+reindexing the reference applies when the particles are cell-sorted (output_manager.hpp:
+532-574).  Per-system totals are computed before they are read, as the reference's registry
+preprocess hooks do.
 """
 from __future__ import annotations
 
 import json
 import os
 import queue
+import shutil
 import threading
 from pathlib import Path
 from typing import Dict, List, Optional, Sequence
@@ -129,10 +135,140 @@ def _write_npy(path: Path, a: np.ndarray) -> None:
     os.replace(tmp, path)
 
 
-class OutputManager:
-    """io::OutputManager<P> (output_manager.hpp:124-445) over the Python binding."""
+def run_info(P) -> dict:
+    """What the reference records beside the static fields (output_manager.hpp:248-256)."""
+    S, N, Nv = P.n_systems(), P.n_particles(), (P.n_vertices() if P.kind == L.RIGID_BUMPY else 0)
+    return {"class_name": "RigidBumpy" if P.kind == L.RIGID_BUMPY else "Disk",
+            "neighbor_method": "Cell" if P.neighbor_method == L.NEIGHBOR_CELL else "Naive",
+            "n_particles": N, "n_systems": S, "n_vertices": Nv}
 
-    def __init__(self, particles, path, max_workers: int = 1, append_mode: bool = False):
+
+class NpyBackend:
+    """Directory of NumPy files.  Calls come from one writer at a time."""
+
+    def open(self, path: Path, append: bool, meta: bool, traj: bool) -> bool:
+        self.path = path
+        existed = (path / "meta" / "static" / "run.json").exists()
+        if meta:
+            for g in ("static", "init", "final"):
+                (path / "meta" / g).mkdir(parents=True, exist_ok=True)
+        self.shapes: Dict[str, tuple] = {}
+        if traj:
+            t = path / "trajectory"
+            if not append and t.exists():
+                shutil.rmtree(t)
+            t.mkdir(parents=True, exist_ok=True)
+            for h in t.glob("*.json"):
+                meta_ = json.loads(h.read_text())
+                self.shapes[h.stem] = (meta_["dtype"], tuple(meta_["shape"]))
+        return existed
+
+    def write_info(self, info: dict):
+        with open(self.path / "meta" / "static" / "run.json", "w") as f:
+            json.dump(info, f)
+
+    def write_group(self, g: str, fields):
+        for n, a in fields:
+            _write_npy(self.path / "meta" / g / f"{n}.npy", a)
+
+    def write_restart(self, step: int, fields):
+        tmp, dst = self.path / "meta" / "restart_tmp", self.path / "meta" / "restart"
+        if tmp.exists():
+            shutil.rmtree(tmp)
+        tmp.mkdir(parents=True)
+        for n, a in fields:
+            _write_npy(tmp / f"{n}.npy", a)
+        _write_npy(tmp / "step.npy", np.array([step], np.int64))
+        if dst.exists():
+            shutil.rmtree(dst)
+        tmp.rename(dst)
+
+    def write_final_step(self, step: int):
+        _write_npy(self.path / "meta" / "final" / "step.npy", np.array([step], np.int64))
+
+    def write_frame(self, step: int, fields):
+        d = self.path / "trajectory"
+        for n, a in fields:
+            a = np.ascontiguousarray(a)
+            if n not in self.shapes:
+                self.shapes[n] = (a.dtype.str, a.shape)
+                with open(d / f"{n}.json", "w") as f:
+                    f.write(json.dumps({"dtype": a.dtype.str, "shape": list(a.shape)}) + "\n")
+            if self.shapes[n] != (a.dtype.str, a.shape):
+                (d / "ragged" / n).mkdir(parents=True, exist_ok=True)
+                _write_npy(d / "ragged" / n / f"step_{step}.npy", a)
+                continue
+            with open(d / f"{n}.bin", "ab") as f:
+                f.write(a.tobytes())
+        with open(d / "timestep.bin", "ab") as f:
+            f.write(np.int64(step).tobytes())
+
+    def close(self):
+        pass
+
+
+class H5Backend:
+    """The reference's meta.h5 / trajectory.h5 through libdpmd_h5 (output_manager.hpp:169-311,
+    :704-932, :1000-1037)."""
+
+    def open(self, path: Path, append: bool, meta: bool, traj: bool) -> bool:
+        from . import h5
+        self.h5 = h5
+        path.mkdir(parents=True, exist_ok=True)
+        existed = (path / "meta.h5").exists()
+        self.meta = h5.H5File(path / "meta.h5", h5.READ_WRITE) if meta else None
+        self.traj, self.rows = None, 0
+        if traj:
+            if not append and (path / "trajectory.h5").exists():
+                (path / "trajectory.h5").unlink()
+            self.traj = h5.H5File(path / "trajectory.h5", h5.READ_WRITE)
+            if self.traj.exists("/timestep"):
+                self.rows = self.traj.read("/timestep").shape[0]
+        return existed
+
+    def write_info(self, info: dict):
+        for k in ("class_name", "neighbor_method", "n_particles", "n_systems", "n_vertices"):
+            self.meta.write(f"/static/{k}", info[k])
+        self.meta.flush()
+
+    def write_group(self, g: str, fields):
+        for n, a in fields:
+            self.meta.write(f"/{g}/{n}", a)
+        self.meta.flush()
+
+    def write_restart(self, step: int, fields):
+        self.meta.delete("/restart_tmp")
+        self.meta.write("/restart_tmp/step", int(step))
+        for n, a in fields:
+            self.meta.write(f"/restart_tmp/{n}", a)
+        self.meta.move("/restart_tmp", "/restart")
+        self.meta.flush()
+
+    def write_final_step(self, step: int):
+        pass   # the reference's /final holds fields only
+
+    def write_frame(self, step: int, fields):
+        self.traj.write_frame("/timestep", np.array([step], np.int32), self.rows)
+        for n, a in fields:
+            if a.shape[0] == 0 or not self.traj.write_frame(f"/{n}", a, self.rows):
+                self.traj.write(f"/ragged/{n}/step_{step}", a)
+        self.rows += 1
+
+    def close(self):
+        for f in (self.meta, self.traj):
+            if f is not None:
+                f.close()
+        self.meta = self.traj = None
+
+
+class OutputManager:
+    """io::OutputManager<P> (output_manager.hpp:124-445) over the Python binding; format
+    "npy" or "h5"."""
+
+    def __init__(self, particles, path, max_workers: int = 1, append_mode: bool = False, format: str = "npy"):
+        if format not in ("npy", "h5"):
+            raise ValueError(f"OutputManager: unknown format '{format}'")
+        self.backend = NpyBackend() if format == "npy" else H5Backend()
         self.P, self.path = particles, Path(path)
         self.max_workers, self.append_mode = max(1, max_workers), append_mode
         self.traj_names: List[str] = []
@@ -150,7 +286,6 @@ class OutputManager:
         self._turn = threading.Condition()   # tasks are written in the order they were captured
         self._seq = self._done = 0
         self._error: Optional[BaseException] = None
-        self._index: Dict[str, dict] = {}
 
     # ---- policy (output_manager.hpp:134-157) ----
     def set_trajectory_fields(self, n): self.traj_names = list(n)
@@ -169,22 +304,17 @@ class OutputManager:
 
     # ---- run ----
     def initialize(self):
-        (self.path / "trajectory").mkdir(parents=True, exist_ok=True)
-        for g in ("static", "init", "restart", "final"):
-            (self.path / "meta" / g).mkdir(parents=True, exist_ok=True)
-        if not self.append_mode:
-            for f in (self.path / "trajectory").iterdir():
-                f.unlink()
+        existed = self.backend.open(self.path, self.append_mode, self.enable_meta, self.enable_traj)
+        if self.enable_meta:
+            self.backend.write_info(run_info(self.P))
+            if not (self.append_mode and existed):
+                self.backend.write_group("static", self._capture(_union(static_names(self.P), self.extra_static)))
+                self.backend.write_group("init", self._capture(_union(state_names(self.P), self.extra_init)))
         if self.max_workers > 1:
             self._q = queue.Queue(maxsize=self.queue_limit or self.max_workers)
             self._workers = [threading.Thread(target=self._worker, daemon=True) for _ in range(self.max_workers)]
             for t in self._workers:
                 t.start()
-        if self.enable_meta:
-            for g, names in (("static", _union(static_names(self.P), self.extra_static)),
-                             ("init", _union(state_names(self.P), self.extra_init))):
-                for n in names:
-                    _write_npy(self.path / "meta" / g / f"{n}.npy", capture(self.P, n))
 
     def step(self, step_num: int):
         self.current_step = step_num
@@ -192,26 +322,27 @@ class OutputManager:
             vals = "  ".join(f"{n}={np.ravel(capture(self.P, n))[:4]}" for n in self.console_names)
             print(f"step {step_num}  {vals}", flush=True)
         if self.enable_traj and self.traj_names and self.traj.due(step_num):
-            self._enqueue(("trajectory", step_num, [(n, capture(self.P, n)) for n in self.traj_names]))
+            self._enqueue(("trajectory", step_num, self._capture(self.traj_names)))
         if self.enable_meta and self.restart.due(step_num):
-            self._enqueue(("restart", step_num, [(n, capture(self.P, n)) for n in self._restart_names()]))
+            self._enqueue(("restart", step_num, self._capture(self._restart_names())))
 
     def finalize(self):
         if self.enable_meta:
-            self._enqueue(("restart", self.current_step, [(n, capture(self.P, n)) for n in self._restart_names()]))
-            names = _union(state_names(self.P), self.extra_final)
-            self._enqueue(("final", self.current_step, [(n, capture(self.P, n)) for n in names]))
+            self._enqueue(("restart", self.current_step, self._capture(self._restart_names())))
+            self._enqueue(("final", self.current_step, self._capture(_union(state_names(self.P), self.extra_final))))
         for _ in self._workers:
             self._q.put(None)
         for t in self._workers:
             t.join()
         self._workers = []
+        self.backend.close()
         if self._error is not None:
             raise self._error
-        with open(self.path / "trajectory" / "index.json", "w") as f:
-            json.dump({"fields": self._index}, f)
 
     # ---- internals ----
+    def _capture(self, names):
+        return [(n, capture(self.P, n)) for n in names]
+
     def _restart_names(self):
         return _union(static_names(self.P), state_names(self.P), self.extra_static, self.extra_restart)
 
@@ -246,36 +377,39 @@ class OutputManager:
 
     def _write(self, kind, step, fields):
         if kind == "trajectory":
-            d = self.path / "trajectory"
-            for n, a in fields:
-                with open(d / f"{n}.bin", "ab") as f:
-                    f.write(np.ascontiguousarray(a).tobytes())
-                self._index[n] = {"dtype": a.dtype.str, "shape": list(a.shape)}
-            with open(d / "timestep.bin", "ab") as f:
-                f.write(np.int64(step).tobytes())
+            self.backend.write_frame(step, fields)
+        elif kind == "restart":
+            self.backend.write_restart(step, fields)
         else:
-            g = self.path / "meta" / kind
-            for n, a in fields:
-                _write_npy(g / f"{n}.npy", a)
-            _write_npy(g / "step.npy", np.array([step], np.int64))
+            self.backend.write_group("final", fields)
+            self.backend.write_final_step(step)
 
 
 def load(path) -> dict:
-    """Read a run written by either OutputManager: {"static": {...}, "init": {...},
-    "restart": {...}, "final": {...}, "trajectory": {field: (T, ...) array}, "timestep": (T,)}."""
+    """Read a run written by either OutputManager, in either container: {"static": {...},
+    "init": {...}, "restart": {...}, "final": {...}, "trajectory": {field: (T, ...) array},
+    "timestep": (T,), "ragged": {field: {"step_<k>": array}}}.  Scalars of the static group
+    (class_name, neighbor_method, n_particles, n_systems, n_vertices) are Python values."""
     path = Path(path)
+    if (path / "meta.h5").exists() or (path / "trajectory.h5").exists():
+        from . import h5
+        return h5.read_run(path)
     out: dict = {}
     for g in ("static", "init", "restart", "final"):
         d = path / "meta" / g
         out[g] = {f.stem: np.load(f, allow_pickle=False) for f in sorted(d.glob("*.npy"))} if d.is_dir() else {}
+    info = path / "meta" / "static" / "run.json"
+    if info.exists():
+        out["static"].update(json.loads(info.read_text()))
     t = path / "trajectory"
-    out["trajectory"], out["timestep"] = {}, np.zeros(0, np.int64)
+    out["trajectory"], out["ragged"], out["timestep"] = {}, {}, np.zeros(0, np.int64)
     if (t / "timestep.bin").exists():
         out["timestep"] = np.fromfile(t / "timestep.bin", dtype="<i8")
-    if (t / "index.json").exists():
-        with open(t / "index.json") as f:
-            idx = json.load(f)["fields"]
-        for n, meta in idx.items():
-            a = np.fromfile(t / f"{n}.bin", dtype=np.dtype(meta["dtype"]))
-            out["trajectory"][n] = a.reshape([-1] + list(meta["shape"]))
+    for h in sorted(t.glob("*.json")):
+        meta = json.loads(h.read_text())
+        a = np.fromfile(t / f"{h.stem}.bin", dtype=np.dtype(meta["dtype"]))
+        out["trajectory"][h.stem] = a.reshape([-1] + list(meta["shape"]))
+    if (t / "ragged").is_dir():
+        for d in sorted((t / "ragged").iterdir()):
+            out["ragged"][d.name] = {f.stem: np.load(f, allow_pickle=False) for f in sorted(d.glob("*.npy"))}
     return out

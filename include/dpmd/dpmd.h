@@ -101,6 +101,9 @@ int dpmd_allocate_particles(dpmd_ctx* ctx, int n_particles);
 int dpmd_allocate_vertices(dpmd_ctx* ctx, int n_vertices);
 /* set_neighbor_method (base_particle.hpp:260) */
 int dpmd_set_neighbor_method(dpmd_ctx* ctx, int method);
+/* get_class_name / get_neighbor_method (base_particle.hpp:108-117, :743): the context's
+ * dpmd_kind and dpmd_neighbor_method */
+int dpmd_ctx_info(const dpmd_ctx* ctx, int* kind, int* neighbor_method);
 /* n_particles / n_systems / n_vertices / n_cells (base_particle.hpp:291-312) */
 int dpmd_sizes(const dpmd_ctx* ctx, int* n_systems, int* n_particles, int* n_vertices, int* n_cells);
 

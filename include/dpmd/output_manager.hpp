@@ -4,28 +4,34 @@
 // fields are captured to the host at chosen steps, put back in canonical particle order when
 // the particles are cell-sorted, and written by background workers with a bounded queue; the
 // run's initial state, static fields, periodic restart snapshots and final state go to their
-// own groups.  The reference writes HDF5 (meta.h5 / trajectory.h5); HDF5 is not available to
-// this build, so the same groups are written as a directory of NumPy files:
+// own groups.  Field names are the reference's (pos, vel, force, pe, ke, rad, mass, area,
+// angle, angular_vel, torque, box_size, packing_fraction, pe_total, ke_total, temperature,
+// pressure, n_contacts_total, ...); per-system totals are computed before they are captured,
+// as the reference's registry preprocess hooks do.
 //
-//   <path>/meta/{static,init,restart,final}/<field>.npy     one array per field
-//   <path>/trajectory/<field>.bin                            frames appended, raw little-endian
+// The container is a policy.  io::H5Backend (dpmd/output_manager_h5.hpp, libdpmd_h5) writes
+// the reference's meta.h5 / trajectory.h5.  io::NpyBackend (here, no dependencies) writes the
+// same groups as a directory of NumPy files:
+//
+//   <path>/meta/static/run.json                              class name, neighbor method, sizes
+//   <path>/meta/{static,init,restart,final}/<field>.npy     one array per field (+ step.npy)
+//   <path>/trajectory/<field>.bin, <field>.json              frames appended; dtype and frame shape
 //   <path>/trajectory/timestep.bin                           int64 step of every frame
-//   <path>/trajectory/index.json                             dtype and frame shape per field
+//   <path>/trajectory/ragged/<field>/step_<k>.npy            frames whose shape changed
 //
-// dpmd_amd/output.py reads them back.  Field names are the reference's (pos, vel, force, pe,
-// ke, rad, mass, area, angle, angular_vel, torque, box_size, packing_fraction, pe_total,
-// ke_total, temperature, pressure, n_contacts_total, ...); per-system totals are computed
-// before they are captured, as the reference's registry preprocess hooks do.
+// dpmd_amd/output.py reads both containers back.
 #pragma once
 
 #include <algorithm>
 #include <condition_variable>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <deque>
 #include <filesystem>
 #include <fstream>
 #include <functional>
+#include <iterator>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -102,16 +108,127 @@ inline std::string json_escape(const std::string& s) {
     return o;
 }
 
+// A frame header file: {"dtype": "<f8", "shape": [rows, cols]} as written below.
+inline bool read_frame_header(const std::string& path, std::string& dtype, int64_t& rows, int64_t& cols) {
+    std::ifstream in(path);
+    if (!in) return false;
+    std::string t((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    const size_t d = t.find("\"dtype\": \""), sh = t.find("\"shape\": [");
+    if (d == std::string::npos || sh == std::string::npos) return false;
+    dtype = t.substr(d + 10, 3);
+    rows = std::strtoll(t.c_str() + sh + 10, nullptr, 10);
+    const size_t comma = t.find(',', sh), close = t.find(']', sh);
+    cols = (comma != std::string::npos && comma < close) ? std::strtoll(t.c_str() + comma + 1, nullptr, 10) : 1;
+    return true;
+}
+
 }  // namespace detail
 
-template <class ParticleT>
+using Fields = std::vector<std::pair<std::string, HostField>>;
+
+// What the reference records beside the static fields (output_manager.hpp:248-256).
+struct RunInfo {
+    std::string class_name, neighbor_method;
+    int n_particles = 0, n_systems = 0, n_vertices = 0;
+};
+
+// Directory of NumPy files.  Calls come from one writer at a time.
+class NpyBackend {
+public:
+    // Returns whether the run's metadata already existed (append mode then keeps it).
+    bool open(const std::string& path, bool append, bool meta, bool traj) {
+        namespace fs = std::filesystem;
+        path_ = path;
+        const bool existed = fs::exists(path_ + "/meta/static/run.json");
+        if (meta) for (const char* g : {"static", "init", "final"}) fs::create_directories(path_ + "/meta/" + g);
+        if (traj) {
+            if (!append) fs::remove_all(path_ + "/trajectory");
+            fs::create_directories(path_ + "/trajectory");
+            for (auto& e : fs::directory_iterator(path_ + "/trajectory")) {
+                if (e.path().extension() != ".json") continue;
+                Shape s;
+                if (detail::read_frame_header(e.path().string(), s.dtype, s.rows, s.cols))
+                    shapes_[e.path().stem().string()] = s;
+            }
+        }
+        return existed;
+    }
+    void write_info(const RunInfo& r) {
+        std::ofstream o(path_ + "/meta/static/run.json");
+        o << "{\"class_name\": \"" << detail::json_escape(r.class_name) << "\", \"neighbor_method\": \""
+          << detail::json_escape(r.neighbor_method) << "\", \"n_particles\": " << r.n_particles
+          << ", \"n_systems\": " << r.n_systems << ", \"n_vertices\": " << r.n_vertices << "}\n";
+    }
+    void write_group(const std::string& g, const Fields& f) {
+        for (auto& [n, a] : f) detail::write_npy(path_ + "/meta/" + g + "/" + n + ".npy", a);
+    }
+    // Written beside and then swapped in, as output_manager.hpp:1000-1011 does with /restart_tmp.
+    void write_restart(int step, const Fields& f) {
+        namespace fs = std::filesystem;
+        const std::string tmp = path_ + "/meta/restart_tmp", dst = path_ + "/meta/restart";
+        fs::remove_all(tmp);
+        fs::create_directories(tmp);
+        for (auto& [n, a] : f) detail::write_npy(tmp + "/" + n + ".npy", a);
+        detail::write_npy(tmp + "/step.npy", step_field(step));
+        fs::remove_all(dst);
+        fs::rename(tmp, dst);
+    }
+    void write_final_step(int step) { detail::write_npy(path_ + "/meta/final/step.npy", step_field(step)); }
+    void write_frame(int step, const Fields& f) {
+        namespace fs = std::filesystem;
+        const std::string d = path_ + "/trajectory/";
+        for (auto& [n, a] : f) {
+            auto it = shapes_.find(n);
+            if (it == shapes_.end()) {
+                Shape s{a.dtype, a.rows, a.cols};
+                std::ofstream h(d + n + ".json");
+                h << "{\"dtype\": \"" << a.dtype << "\", \"shape\": [" << a.rows;
+                if (a.cols > 1) h << ", " << a.cols;
+                h << "]}\n";
+                it = shapes_.emplace(n, s).first;
+            }
+            if (it->second.dtype != a.dtype || it->second.rows != a.rows || it->second.cols != a.cols) {
+                fs::create_directories(d + "ragged/" + n);
+                detail::write_npy(d + "ragged/" + n + "/step_" + std::to_string(step) + ".npy", a);
+                continue;
+            }
+            std::ofstream o(d + n + ".bin", std::ios::binary | std::ios::app);
+            o.write(a.bytes.data(), (std::streamsize)a.bytes.size());
+        }
+        std::ofstream ts(d + "timestep.bin", std::ios::binary | std::ios::app);
+        const int64_t s = step;
+        ts.write((const char*)&s, 8);
+    }
+    void close() {}
+
+private:
+    struct Shape {
+        std::string dtype;
+        int64_t rows = 0, cols = 1;
+    };
+    static HostField step_field(int step) {
+        HostField st{"<i8", 1, 1, std::vector<char>(8)};
+        const int64_t s = step;
+        std::copy_n((const char*)&s, 8, st.bytes.data());
+        return st;
+    }
+    std::string path_;
+    std::map<std::string, Shape> shapes_;
+};
+
+template <class ParticleT, class Backend = NpyBackend>
 class OutputManager {
 public:
     OutputManager(ParticleT& particles, const std::string& path, int max_workers = 1, bool append_mode = false)
         : p_(particles), path_(path), max_workers_(std::max(1, max_workers)), append_mode_(append_mode) {}
     ~OutputManager() {
-        try { stop_workers(); } catch (...) {}
+        try {
+            stop_workers();
+            backend_.close();
+        } catch (...) {
+        }
     }
+    Backend& backend() { return backend_; }
     OutputManager(const OutputManager&) = delete;
     OutputManager& operator=(const OutputManager&) = delete;
 
@@ -135,17 +252,15 @@ public:
 
     // ---- run (output_manager.hpp:169-375) -------------------------------------------------
     void initialize() {
-        namespace fs = std::filesystem;
-        fs::create_directories(path_ + "/trajectory");
-        for (const char* g : {"static", "init", "restart", "final"}) fs::create_directories(path_ + "/meta/" + g);
-        if (!append_mode_) {
-            for (auto& e : fs::directory_iterator(path_ + "/trajectory")) fs::remove(e.path());
+        const bool existed = backend_.open(path_, append_mode_, enable_meta_, enable_traj_);
+        if (enable_meta_) {
+            backend_.write_info(run_info());
+            if (!(append_mode_ && existed)) {
+                backend_.write_group("static", capture_all(with(static_names(), extra_static_)));
+                backend_.write_group("init", capture_all(with(state_names(), extra_init_)));
+            }
         }
         start_workers();
-        if (enable_meta_) {
-            write_group("static", with(static_names(), extra_static_));
-            write_group("init", with(state_names(), extra_init_));
-        }
     }
 
     void step(int step_num) {
@@ -164,16 +279,16 @@ public:
             for (const auto& n : traj_names_) t.fields.emplace_back(n, capture(n));
             enqueue(std::move(t));
         }
-        if (enable_meta_ && restart_.due(step_num)) enqueue_group(Kind::Restart, with(all_restart_names(), {}));
+        if (enable_meta_ && restart_.due(step_num)) enqueue_group(Kind::Restart, all_restart_names());
     }
 
     void finalize() {
         if (enable_meta_) {
-            enqueue_group(Kind::Restart, with(all_restart_names(), {}));
+            enqueue_group(Kind::Restart, all_restart_names());
             enqueue_group(Kind::Final, with(state_names(), extra_final_));
         }
         stop_workers();
-        write_index();
+        backend_.close();
     }
 
 private:
@@ -310,8 +425,17 @@ private:
         return os.str();
     }
 
-    void write_group(const std::string& g, const std::vector<std::string>& names) {
-        for (const auto& n : names) detail::write_npy(path_ + "/meta/" + g + "/" + n + ".npy", capture(n));
+    Fields capture_all(const std::vector<std::string>& names) {
+        Fields f;
+        for (const auto& n : names) f.emplace_back(n, capture(n));
+        return f;
+    }
+    RunInfo run_info() {
+        RunInfo r;
+        r.class_name = kRigid ? "RigidBumpy" : "Disk";
+        r.neighbor_method = p_.get_neighbor_method() == md::NeighborMethod::Cell ? "Cell" : "Naive";
+        df::detail::check(dpmd_sizes(p_.ctx(), &r.n_systems, &r.n_particles, &r.n_vertices, nullptr), p_.ctx());
+        return r;
     }
     void enqueue_group(Kind k, const std::vector<std::string>& names) {
         Task t{k, current_step_, {}};
@@ -383,35 +507,13 @@ private:
     }
     void write(Task& t) {
         if (t.kind == Kind::Trajectory) {
-            for (auto& [n, f] : t.fields) {
-                std::ofstream o(path_ + "/trajectory/" + n + ".bin", std::ios::binary | std::ios::app);
-                o.write(f.bytes.data(), (std::streamsize)f.bytes.size());
-                index_[n] = HostField{f.dtype, f.rows, f.cols, {}};
-            }
-            std::ofstream ts(path_ + "/trajectory/timestep.bin", std::ios::binary | std::ios::app);
-            const int64_t s = t.step;
-            ts.write((const char*)&s, 8);
+            backend_.write_frame(t.step, t.fields);
+        } else if (t.kind == Kind::Restart) {
+            backend_.write_restart(t.step, t.fields);
         } else {
-            const char* g = t.kind == Kind::Restart ? "restart" : "final";
-            for (auto& [n, f] : t.fields) detail::write_npy(path_ + "/meta/" + g + "/" + n + ".npy", f);
-            HostField st{"<i8", 1, 1, std::vector<char>(8)};
-            const int64_t s = t.step;
-            std::copy_n((const char*)&s, 8, st.bytes.data());
-            detail::write_npy(path_ + "/meta/" + g + "/step.npy", st);
+            backend_.write_group("final", t.fields);
+            backend_.write_final_step(t.step);
         }
-    }
-    void write_index() {
-        std::ofstream o(path_ + "/trajectory/index.json");
-        o << "{\"fields\": {";
-        bool first = true;
-        for (auto& [n, f] : index_) {
-            o << (first ? "" : ", ") << "\"" << detail::json_escape(n) << "\": {\"dtype\": \"" << f.dtype
-              << "\", \"shape\": [" << f.rows;
-            if (f.cols > 1) o << ", " << f.cols;
-            o << "]}";
-            first = false;
-        }
-        o << "}}\n";
     }
     void set_policy(LogPolicy& p, LogMode m, int interval) {
         p.mode = interval > 0 ? m : LogMode::Disabled;
@@ -436,7 +538,7 @@ private:
     uint64_t next_seq_ = 0, written_ = 0;
     bool stop_ = false;
     std::exception_ptr error_;
-    std::map<std::string, HostField> index_;
+    Backend backend_;
 };
 
 }  // namespace io

@@ -5,20 +5,26 @@
 // test harness can compare it with the same sequence issued through the Python binding.
 //
 // With a second argument it then runs 60 more velocity-Verlet steps under io::OutputManager
-// (the reference's include/utils/output_manager.hpp), writing its output directory there.
+// (the reference's include/utils/output_manager.hpp), writing its output directory there; with
+// a third (built with DPMD_FACADE_H5 and linked to libdpmd_h5) a second manager writes the same
+// run as meta.h5 / trajectory.h5 into that directory.
 //
-// usage: facade_test <out.bin> [<output dir>]
+// usage: facade_test <out.bin> [<output dir> [<h5 output dir>]]
 #include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <vector>
 
 #include "dpmd/output_manager.hpp"
+#ifdef DPMD_FACADE_H5
+#include "dpmd/output_manager_h5.hpp"
+#endif
 #include "dpmd/routines.hpp"
 
 int main(int argc, char** argv) {
-    if (argc != 2 && argc != 3) { std::cerr << "usage: facade_test <out.bin> [<output dir>]\n"; return 2; }
+    if (argc < 2 || argc > 4) { std::cerr << "usage: facade_test <out.bin> [<output dir> [<h5 output dir>]]\n"; return 2; }
     const int S = 2, n_per = 128, N = S * n_per;
     const double phi = 0.80;
     std::vector<double> rad(N), mass(N, 1.0), area(N);
@@ -116,23 +122,41 @@ int main(int argc, char** argv) {
               << nct[1] << " hessian_xx_sum=" << trace << " n_neighbors=" << P.n_neighbors()
               << " n_cells=" << P.n_cells() << " pe=" << pe[0] << "," << pe[1] << std::endl;
 
-    if (argc == 3) {   // the reference's OutputManager call pattern around an integrator loop
+    if (argc >= 3) {   // the reference's OutputManager call pattern around an integrator loop
+        auto configure = [](auto& om) {
+            om.set_trajectory_fields({"pos", "vel", "pe_total", "ke_total", "temperature", "contacts",
+                                      "n_contacts_total"});
+            om.set_trajectory_interval(7);
+            om.set_restart_interval(25);
+            om.set_extra_init_fields({"packing_fraction"});
+            om.set_extra_final_fields({"pe", "pressure", "stress_tensor_total_x"});
+            om.set_queue_limit(3);
+            om.initialize();
+        };
         io::OutputManager<md::disk::Disk> om(P, argv[2], 2);
-        om.set_trajectory_fields({"pos", "vel", "pe_total", "ke_total", "temperature", "contacts", "n_contacts_total"});
-        om.set_trajectory_interval(7);
-        om.set_restart_interval(25);
-        om.set_extra_init_fields({"packing_fraction"});
-        om.set_extra_final_fields({"pe", "pressure", "stress_tensor_total_x"});
-        om.set_queue_limit(3);
-        om.initialize();
+        configure(om);
+#ifdef DPMD_FACADE_H5
+        std::unique_ptr<io::OutputManager<md::disk::Disk, io::H5Backend>> oh;
+        if (argc == 4) {
+            oh = std::make_unique<io::OutputManager<md::disk::Disk, io::H5Backend>>(P, argv[3], 2);
+            configure(*oh);
+        }
+#endif
         md::integrators::VelocityVerlet<md::disk::Disk> vv2(P, 1e-2);
         vv2.init();
         for (int k = 1; k <= 60; ++k) {
             vv2.step();
             om.step(k);
+#ifdef DPMD_FACADE_H5
+            if (oh) oh->step(k);
+#endif
         }
         om.finalize();
-        std::cout << "facade_test: output written to " << argv[2] << std::endl;
+#ifdef DPMD_FACADE_H5
+        if (oh) oh->finalize();
+#endif
+        std::cout << "facade_test: output written to " << argv[2] << (argc == 4 ? " and " : "")
+                  << (argc == 4 ? argv[3] : "") << std::endl;
     }
     return 0;
 }

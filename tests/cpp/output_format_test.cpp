@@ -1,9 +1,12 @@
 // Host-only check of include/dpmd/output_manager.hpp.  Links nothing from the library:
 //   output_format_test <dir>                 writes .npy files with io::detail::write_npy and
 //                                            prints the steps each log policy logs;
-//   output_format_test <dir> run <workers>   runs io::OutputManager for 30 steps over a
+//   output_format_test <dir> run <workers> [h5]
+//                                            runs io::OutputManager for 30 steps over a
 //                                            stand-in of the C ABI whose fields are closed-form
-//                                            functions of the field id and the step.
+//                                            functions of the field id and the step, with the
+//                                            NumPy container or (built with DPMD_WITH_H5 and
+//                                            linked to libdpmd_h5) the HDF5 one.
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -11,6 +14,9 @@
 #include <string>
 
 #include "dpmd/output_manager.hpp"
+#ifdef DPMD_WITH_H5
+#include "dpmd/output_manager_h5.hpp"
+#endif
 
 // ---- stand-in C ABI: S = 2 systems, N = 12 particles, cell order = a fixed permutation -----
 namespace {
@@ -27,6 +33,13 @@ int64_t length(int f) {
 
 extern "C" {
 const char* dpmd_last_error(const dpmd_ctx*) { return "stand-in error"; }
+int dpmd_sizes(const dpmd_ctx*, int* S, int* N, int* Nv, int* C) {
+    if (S) *S = kS;
+    if (N) *N = kN;
+    if (Nv) *Nv = 0;
+    if (C) *C = 0;
+    return DPMD_OK;
+}
 int dpmd_field(dpmd_ctx*, int f, void** p, int64_t* n) {
     if (p) *p = nullptr;
     if (n) *n = length(f);
@@ -58,9 +71,10 @@ struct StandInDisk {
     md::NeighborMethod get_neighbor_method() const { return md::NeighborMethod::Cell; }
 };
 
+template <class Backend>
 static int run(const std::string& d, int workers) {
     StandInDisk P;
-    io::OutputManager<StandInDisk> om(P, d, workers);
+    io::OutputManager<StandInDisk, Backend> om(P, d, workers);
     om.set_trajectory_fields({"pos", "pe_total", "system_offset"});
     om.set_trajectory_interval(5);
     om.set_restart_log_policy(io::LogMode::Logarithmic, 8);
@@ -75,7 +89,7 @@ static int run(const std::string& d, int workers) {
     om.finalize();
     for (const auto& c : g_calls) std::cout << c << "\n";
     try {
-        io::OutputManager<StandInDisk> bad(P, d + "_bad", 1);
+        io::OutputManager<StandInDisk, Backend> bad(P, d + "_bad", 1);
         bad.set_trajectory_fields({"no_such_field"});
         bad.set_trajectory_interval(1);
         bad.enable_meta(false);
@@ -89,7 +103,11 @@ static int run(const std::string& d, int workers) {
 }
 
 int main(int argc, char** argv) {
-    if (argc == 4 && std::string(argv[2]) == "run") return run(argv[1], std::atoi(argv[3]));
+    if (argc == 4 && std::string(argv[2]) == "run") return run<io::NpyBackend>(argv[1], std::atoi(argv[3]));
+#ifdef DPMD_WITH_H5
+    if (argc == 5 && std::string(argv[2]) == "run" && std::string(argv[4]) == "h5")
+        return run<io::H5Backend>(argv[1], std::atoi(argv[3]));
+#endif
     if (argc != 2) { std::cerr << "usage: output_format_test <dir> [run <workers>]\n"; return 2; }
     const std::string d = argv[1];
     io::HostField a{"<f8", 5, 2, std::vector<char>(5 * 2 * 8)};

@@ -13,12 +13,13 @@ from dpmd_amd import _lib as L
 ROOT = Path(__file__).resolve().parent.parent
 
 
-def declared_symbols():
-    names = set()
-    for h in (ROOT / "include").rglob("*.h"):
-        txt = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
-        names |= set(re.findall(r"\b(dpmd_[a-z0-9_]+)\s*\(", txt))
-    return names
+def declared_symbols(header="dpmd.h"):
+    txt = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "dpmd" / header).read_text(), flags=re.S)
+    return set(re.findall(r"\b(dpmd_[a-z0-9_]+)\s*\(", txt))
+
+
+def test_every_header_is_checked():
+    assert {h.name for h in (ROOT / "include").rglob("*.h")} == {"dpmd.h", "dpmd_h5.h"}
 
 
 def test_header_declares_entry_points():
@@ -55,3 +56,15 @@ def test_version_and_error_path_without_gpu():
     # bad kind is an argument error, reported before any device call
     assert lib.dpmd_ctx_create(C.byref(p), 7, None) == 1
     assert b"kind" in lib.dpmd_last_error(None)
+
+
+@pytest.mark.skipif(not (ROOT / "dpmd_amd" / "libdpmd_h5.so").exists(), reason="libdpmd_h5 not built (no HDF5)")
+def test_h5_library_exports_and_binding():
+    from dpmd_amd import h5
+    names = declared_symbols("dpmd_h5.h")
+    assert "dpmd_h5_load" in names and len(names) >= 15
+    lib = h5.lib()
+    assert all(hasattr(lib, n) for n in names)
+    out = subprocess.run(["nm", "-D", "--defined-only", str(h5.LIB_PATH)], capture_output=True, text=True).stdout
+    assert names <= set(re.findall(r"\bT (dpmd_h5_\w+)", out))
+    assert names <= set(h5._PROTOS)

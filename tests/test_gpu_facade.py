@@ -79,24 +79,29 @@ def test_cpp_facade_matches_python_binding(tmp_path):
 
 
 def test_cpp_output_manager_matches_python_output_manager(tmp_path):
-    """io::OutputManager (C++) and dpmd_amd.output.OutputManager write the same files, bit for
-    bit, for the same run; the frames are the fields read directly after those steps."""
-    from dpmd_amd import output
+    """io::OutputManager (C++) and dpmd_amd.output.OutputManager write the same arrays, bit for
+    bit, for the same run, in both containers; the frames are the fields read directly after
+    those steps."""
+    from dpmd_amd import h5, output
+    with_h5 = h5.available()
     exe = ROOT / "tests" / "cpp" / "facade_test.bin"
-    r = subprocess.run([str(exe), str(tmp_path / "state.bin"), str(tmp_path / "cpp")], capture_output=True,
-                       text=True, timeout=120)
+    args = [str(exe), str(tmp_path / "state.bin"), str(tmp_path / "cpp")] + ([str(tmp_path / "cpp_h5")] if with_h5 else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     _, _, _, P = python_sequence()
-    om = output.OutputManager(P, tmp_path / "py", max_workers=2)
     traj = ["pos", "vel", "pe_total", "ke_total", "temperature", "contacts", "n_contacts_total"]
-    om.set_trajectory_fields(traj)
-    om.set_trajectory_interval(7)
-    om.set_restart_interval(25)
-    om.set_extra_init_fields(["packing_fraction"])
-    om.set_extra_final_fields(["pe", "pressure", "stress_tensor_total_x"])
-    om.set_queue_limit(3)
-    om.initialize()
+    managers = []
+    for fmt, d in (("npy", "py"), ("h5", "py_h5"))[: 2 if with_h5 else 1]:
+        om = output.OutputManager(P, tmp_path / d, max_workers=2, format=fmt)
+        om.set_trajectory_fields(traj)
+        om.set_trajectory_interval(7)
+        om.set_restart_interval(25)
+        om.set_extra_init_fields(["packing_fraction"])
+        om.set_extra_final_fields(["pe", "pressure", "stress_tensor_total_x"])
+        om.set_queue_limit(3)
+        om.initialize()
+        managers.append(om)
     vv = VelocityVerlet(P, 1e-2)
     vv.init()
     direct = {}
@@ -104,18 +109,25 @@ def test_cpp_output_manager_matches_python_output_manager(tmp_path):
         vv.step(1)
         if k % 7 == 0:
             direct[k] = P.get("POS_X", L.ORDER_CANONICAL).copy()
-        om.step(k)
-    om.finalize()
-    a, b = output.load(tmp_path / "cpp"), output.load(tmp_path / "py")
+        for om in managers:
+            om.step(k)
+    for om in managers:
+        om.finalize()
+    runs = [output.load(tmp_path / d) for d in (("cpp", "py", "cpp_h5", "py_h5") if with_h5 else ("cpp", "py"))]
+    a = runs[0]
     assert np.array_equal(a["timestep"], np.arange(7, 61, 7))
-    assert np.array_equal(a["timestep"], b["timestep"])
-    for g in ("static", "init", "restart", "final"):
-        assert set(a[g]) == set(b[g]), g
-        for n in a[g]:
-            assert a[g][n].dtype == b[g][n].dtype and np.array_equal(a[g][n], b[g][n]), (g, n)
-    assert set(a["trajectory"]) == set(traj) == set(b["trajectory"])
-    for n in traj:
-        assert np.array_equal(a["trajectory"][n], b["trajectory"][n]), n
+    for b in runs[1:]:
+        assert np.array_equal(a["timestep"], b["timestep"])
+        for g in ("static", "init", "restart", "final", "trajectory"):
+            names = set(a[g]) - {"step"}
+            assert names == set(b[g]) - {"step"}, g
+            for n in names:
+                x, y = a[g][n], b[g][n]
+                if isinstance(x, np.ndarray):
+                    assert x.dtype == y.dtype and np.array_equal(x, y), (g, n)
+                else:
+                    assert x == y, (g, n)
+    assert set(a["trajectory"]) == set(traj)
     assert a["trajectory"]["pos"].shape == (8, P.n_particles(), 2)
     for t, k in enumerate(a["timestep"]):
         assert np.array_equal(a["trajectory"]["pos"][t, :, 0], direct[int(k)])
