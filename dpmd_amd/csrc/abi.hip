@@ -1883,6 +1883,10 @@ int dpmd_set_option(dpmd_ctx* c, int key, int value) {
                 need(value >= 0 && value <= 2, "DPMD_OPT_DEVICE_REBUILD must be 0, 1 or 2");
                 c->dev_rebuild = value;
                 break;
+            case DPMD_OPT_PERSIST_LDS:
+                need(value == 0 || value == 1, "DPMD_OPT_PERSIST_LDS must be 0 or 1");
+                c->persist_lds = value;
+                break;
             default: throw ArgError("unknown option key");
         }
     });
@@ -1900,6 +1904,7 @@ int dpmd_get_option(dpmd_ctx* c, int key, int* value) {
             case DPMD_OPT_PERSIST_BLOCK: *value = c->persist_block; break;
             case DPMD_OPT_MAX_BATCH: *value = c->max_batch; break;
             case DPMD_OPT_DEVICE_REBUILD: *value = c->dev_rebuild; break;
+            case DPMD_OPT_PERSIST_LDS: *value = c->persist_lds; break;
             default: throw ArgError("unknown option key");
         }
     });

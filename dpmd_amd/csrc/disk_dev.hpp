@@ -39,10 +39,18 @@ inline DiskArgs disk_args(dpmd_ctx* c) {
 struct SoAPos {
     const double* x; const double* y;
     __device__ __forceinline__ double2 at(int j) const { return make_double2(x[j], y[j]); }
+    __device__ __forceinline__ double rad(const DiskArgs& a, int j) const { return a.rad[j]; }
 };
 struct PackedPos {
     const double2* p;
     __device__ __forceinline__ double2 at(int j) const { return p[j]; }
+    __device__ __forceinline__ double rad(const DiskArgs& a, int j) const { return a.rad[j]; }
+};
+// One system's positions and radii held in LDS (index j - beg).
+struct LdsPos {
+    const double* x; const double* y; const double* r; int beg;
+    __device__ __forceinline__ double2 at(int j) const { return make_double2(x[j - beg], y[j - beg]); }
+    __device__ __forceinline__ double rad(const DiskArgs&, int j) const { return r[j - beg]; }
 };
 
 // compute_pair_forces_kernel (disk.cu:103-160) and compute_wall_forces_kernel
@@ -53,7 +61,7 @@ __device__ __forceinline__ void disk_force_pos(const DiskArgs& a, const Pos& pos
                                                double& fyo, double& peo) {
     const double ei = a.e[s];
     const double2 pi = pos.at(i);
-    const double xi = pi.x, yi = pi.y, ri = a.rad[i];
+    const double xi = pi.x, yi = pi.y, ri = pos.rad(a, i);
     double fxi = 0.0, fyi = 0.0, pei = 0.0;
     double Lx = a.Lx[s], Ly = a.Ly[s];
     if (WALL) {
@@ -75,7 +83,7 @@ __device__ __forceinline__ void disk_force_pos(const DiskArgs& a, const Pos& pos
 #pragma unroll
         for (int u = 0; u < G; ++u) {
             const double2 q = pos.at(jj[u]);
-            xj[u] = q.x; yj[u] = q.y; rj[u] = a.rad[jj[u]];
+            xj[u] = q.x; yj[u] = q.y; rj[u] = pos.rad(a, jj[u]);
         }
 #pragma unroll
         for (int u = 0; u < G; ++u) {

@@ -23,6 +23,11 @@
 namespace dpmd {
 namespace {
 
+// The LDS-resident engine: systems of up to kLdsMaxN particles, one 512-thread workgroup
+// each; 11 per-particle arrays of doubles take 132 KB of the CU's 160 KB.
+constexpr int kLdsMaxN = 1536;
+constexpr int kLdsBlock = 512;
+
 struct PersistArgs {
     const int* order;   // workgroup b runs system order[b]
     const int* todo;    // per system: 0 nothing, 1 run until it stops or max_steps, 2 one step
@@ -93,12 +98,45 @@ __device__ void rebuild_system(const RebuildArgs& r, int s, int beg, int end, in
     if (t == 0) r.ca.flag[s] = 0u;
 }
 
-template <int MODE, int W>
+// A system's per-particle state held in LDS by the LDS-resident engine (NL > 0): positions,
+// velocities, forces, energies, radii, masses and the displacement reference, 88 bytes per
+// particle.  The neighbor lists stay in global memory (read-only between rebuilds).
+template <int NL>
+struct LdsState {
+    double x[NL], y[NL], vx[NL], vy[NL], fx[NL], fy[NL], pe[NL], r[NL], m[NL], lx[NL], ly[NL];
+};
+
+template <int W, int NL>
+__device__ void lds_load(LdsState<NL>& L, const DiskArgs& a, int beg, int end) {
+    for (int i = beg + (int)threadIdx.x; i < end; i += W) {
+        const int k = i - beg;
+        L.x[k] = a.x[i]; L.y[k] = a.y[i]; L.vx[k] = a.vx[i]; L.vy[k] = a.vy[i];
+        L.fx[k] = a.fx[i]; L.fy[k] = a.fy[i]; L.pe[k] = a.pe[i]; L.r[k] = a.rad[i]; L.m[k] = a.mass[i];
+        L.lx[k] = a.cell ? a.lx[i] : 0.0; L.ly[k] = a.cell ? a.ly[i] : 0.0;
+    }
+}
+
+// What the step changes: positions, velocities, forces and energies.
+template <int W, int NL>
+__device__ void lds_store(const LdsState<NL>& L, const DiskArgs& a, int beg, int end) {
+    for (int i = beg + (int)threadIdx.x; i < end; i += W) {
+        const int k = i - beg;
+        a.x[i] = L.x[k]; a.y[i] = L.y[k]; a.vx[i] = L.vx[k]; a.vy[i] = L.vy[k];
+        a.fx[i] = L.fx[k]; a.fy[i] = L.fy[k]; a.pe[i] = L.pe[k];
+    }
+}
+
+// NL = 0: the system's state lives in global memory (L2-resident for these sizes).
+// NL > 0: it is loaded into LDS once per launch, every phase but the rebuild works there,
+// and it goes back to global memory before a rebuild (which permutes it), is reloaded after,
+// and is stored at the end.  The arithmetic and the order of every sum are the same in both.
+template <int MODE, int W, int NL>
 __global__ void __launch_bounds__(W) k_fire_persist(DiskArgs a, FireArgs f, RedArgs rd, RebuildArgs r, PersistArgs p) {
     constexpr int NW = W / 64;
     // one partial per wavefront of particles (a 256-particle chunk is 4 of them), so the
     // force pass needs a single barrier before the chunk sums are formed
-    __shared__ double red[3][kPersistMaxN / 64];
+    __shared__ double red[3][(NL ? NL : kPersistMaxN) / 64];
+    __shared__ LdsState<NL ? NL : 1> L;
     __shared__ double fold[3][4];
     __shared__ int ilds[NW];
     __shared__ double sh_dt, sh_alpha, sh_dtrev, sh_vscale, sh_live_dt;
@@ -120,6 +158,7 @@ __global__ void __launch_bounds__(W) k_fire_persist(DiskArgs a, FireArgs f, RedA
         nsteps = f.nsteps[s];
         sh_frozen = frozen;
     }
+    if constexpr (NL > 0) lds_load<W, NL>(L, a, beg, end);
     __syncthreads();
     for (int k = 0; k < steps; ++k) {
         if (sh_frozen) break;
@@ -133,7 +172,16 @@ __global__ void __launch_bounds__(W) k_fire_persist(DiskArgs a, FireArgs f, RedA
         }
         __syncthreads();
         bool flag = false;
-        {
+        if constexpr (NL > 0) {
+            const double dt = sh_dt, alpha = sh_alpha, dtrev = sh_dtrev, vscale = sh_vscale;
+            const double th2 = a.thresh2[s];
+            for (int q0 = t; q0 < end - beg; q0 += W) {
+                double x = L.x[q0], y = L.y[q0], vx = L.vx[q0], vy = L.vy[q0];
+                flag = fire_drift_vals(x, y, vx, vy, L.fx[q0], L.fy[q0], L.m[q0], a.cell, L.lx[q0], L.ly[q0], th2, dt,
+                                       alpha, dtrev, vscale) || flag;
+                L.x[q0] = x; L.y[q0] = y; L.vx[q0] = vx; L.vy[q0] = vy;
+            }
+        } else {
             // DU particles per thread per pass, every load issued before the first store
             constexpr int DU = 4;
             const double dt = sh_dt, alpha = sh_alpha, dtrev = sh_dtrev, vscale = sh_vscale;
@@ -162,13 +210,35 @@ __global__ void __launch_bounds__(W) k_fire_persist(DiskArgs a, FireArgs f, RedA
         }
         if (flag) sh_flag = 1;
         __syncthreads();
-        if (sh_flag) rebuild_system<W>(r, s, beg, end, ilds);
+        if (sh_flag) {
+            if constexpr (NL > 0) {
+                lds_store<W, NL>(L, a, beg, end);
+                __syncthreads();
+            }
+            rebuild_system<W>(r, s, beg, end, ilds);
+            if constexpr (NL > 0) {
+                lds_load<W, NL>(L, a, beg, end);
+                __syncthreads();
+            }
+        }
         // forces, kick and wavefront partials (k_fire_force)
         const double dt = sh_live_dt;
         for (int b0 = beg; b0 < end; b0 += W) {
             const int i = b0 + t;
             double vp = 0.0, vpe = 0.0, vf2 = 0.0;
-            if (i < end) {
+            if (i < end && NL > 0) {
+                const int q0 = i - beg;
+                double fx, fy, pe;
+                disk_force_mode_pos<MODE>(a, LdsPos{L.x, L.y, L.r, beg}, i, s, fx, fy, pe);
+                L.fx[q0] = fx; L.fy[q0] = fy; L.pe[q0] = pe;
+                double vx = L.vx[q0], vy = L.vy[q0];
+                const double smi = dt * 0.5 / L.m[q0];
+                vx = vx + fx * smi; vy = vy + fy * smi;
+                L.vx[q0] = vx; L.vy[q0] = vy;
+                vp = fx * vx + fy * vy;
+                vpe = pe;
+                vf2 = fx * fx + fy * fy;
+            } else if (i < end) {
                 double fx, fy, pe;
                 disk_force_mode<MODE>(a, i, s, fx, fy, pe);
                 a.fx[i] = fx; a.fy[i] = fy; a.pe[i] = pe;
@@ -219,6 +289,7 @@ __global__ void __launch_bounds__(W) k_fire_persist(DiskArgs a, FireArgs f, RedA
         __syncthreads();
         if (sh_stop) break;
     }
+    if constexpr (NL > 0) lds_store<W, NL>(L, a, beg, end);
     if (t == 0) {
         fire_store(f, s, q);
         f.stopped[s] = stopped;
@@ -236,18 +307,24 @@ void persist_by_mode(int mode, F&& f) {
     }
 }
 
-void launch(dpmd_ctx* c, const PersistArgs& p, int nblocks) {
+// nmax: the largest system of the launch; systems of up to kLdsMaxN particles run in LDS.
+void launch(dpmd_ctx* c, const PersistArgs& p, int nblocks, int nmax) {
     if (nblocks <= 0) return;
     DiskArgs a = disk_args(c);
     FireArgs f = fire_args(c);
     RedArgs rd = red_args(c);
     RebuildArgs r = rebuild_args(c);
+    const bool lds = c->persist_lds && nmax <= kLdsMaxN;
     persist_by_mode(c->fire_mode, [&](auto m) {
         constexpr int M = decltype(m)::value;
+        if (lds) {
+            k_fire_persist<M, kLdsBlock, kLdsMaxN><<<nblocks, kLdsBlock, 0, c->st>>>(a, f, rd, r, p);
+            return;
+        }
         switch (c->persist_block) {
-            case 256: k_fire_persist<M, 256><<<nblocks, 256, 0, c->st>>>(a, f, rd, r, p); break;
-            case 1024: k_fire_persist<M, 1024><<<nblocks, 1024, 0, c->st>>>(a, f, rd, r, p); break;
-            default: k_fire_persist<M, 512><<<nblocks, 512, 0, c->st>>>(a, f, rd, r, p); break;
+            case 256: k_fire_persist<M, 256, 0><<<nblocks, 256, 0, c->st>>>(a, f, rd, r, p); break;
+            case 1024: k_fire_persist<M, 1024, 0><<<nblocks, 1024, 0, c->st>>>(a, f, rd, r, p); break;
+            default: k_fire_persist<M, 512, 0><<<nblocks, 512, 0, c->st>>>(a, f, rd, r, p); break;
         }
     });
     HIP_OK(hipGetLastError());
@@ -318,7 +395,7 @@ int fire_persistent_run(dpmd_ctx* c, int max_steps, int pred, double a, double b
         c->persist_todo.upload(todo.data(), S, c->st);
         c->persist_stop.upload(stop.data(), S, c->st);
         PersistArgs p{c->persist_order.p, c->persist_todo.p, c->persist_stop.p, phase, pred, a, b};
-        launch(c, p, (int)running.size());
+        launch(c, p, (int)running.size(), nmax);
         HIP_OK(hipMemcpyAsync(stop.data(), c->persist_stop.p, S * 4, hipMemcpyDeviceToHost, c->st));
         HIP_OK(hipStreamSynchronize(c->st));
         std::vector<int> still;
@@ -332,12 +409,17 @@ int fire_persistent_run(dpmd_ctx* c, int max_steps, int pred, double a, double b
         const int total = still.empty() ? last + 1 : (end == max_steps ? max_steps : INT_MAX);
         std::vector<int> extra;
         std::fill(todo.begin(), todo.end(), 0);
+        int emax = 0;
         for (int s : running)
-            if (stop[s] >= 0 && done + stop[s] + 1 < total) { todo[s] = 2; extra.push_back(s); }
+            if (stop[s] >= 0 && done + stop[s] + 1 < total) {
+                todo[s] = 2;
+                extra.push_back(s);
+                emax = std::max(emax, size(s));
+            }
         if (!extra.empty()) {
             c->persist_order.upload(extra.data(), (int64_t)extra.size(), c->st);
             c->persist_todo.upload(todo.data(), S, c->st);
-            launch(c, p, (int)extra.size());
+            launch(c, p, (int)extra.size(), emax);
         }
         if (total != INT_MAX) {
             int nrun = (int)still.size();   // run words as the lean run leaves them

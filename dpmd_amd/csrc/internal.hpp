@@ -213,6 +213,7 @@ struct dpmd_ctx {
     // the switch, its workgroup size, and the per-run work lists
     int persist_fire = 1;        // 0 off, 1 with hand-off to per-step launches, 2 persistent only
     int persist_block = 512;
+    int persist_lds = 1;         // systems of up to 1536 particles run with their state in LDS
     int persist_phase = 2000;    // steps per persistent phase
     dpmd::DVec<int> persist_order, persist_todo, persist_stop;
     // kernel timing (force kernels; outside graph capture only)

@@ -383,11 +383,14 @@ int dpmd_stream_copy(double* dst, const double* src, int64_t n, int repeats, dou
  *     batch halts and the host finishes the step (cheapest when rebuilds are rare); 2: every
  *     step runs the rebuild chain on the device, gated by the rebuild flags, and batches never
  *     halt; 0 (default): 1 while rebuilds are rarer than one per 5 steps, 2 above (back below
- *     one per 10).  Same results. */
+ *     one per 10).  Same results.
+ *   DPMD_OPT_PERSIST_LDS: 1 (default) runs persistent FIRE launches whose systems all have at
+ *     most 1536 particles with each system's particle state held in LDS (one 512-thread
+ *     workgroup per system); 0 keeps it in global memory.  Same results. */
 enum dpmd_option {
     DPMD_OPT_COOP_REBUILD = 0, DPMD_OPT_COOP_IN_GRAPH = 1, DPMD_OPT_FUSED_VV = 2, DPMD_OPT_BLOCK = 3,
     DPMD_OPT_MAX_BATCH = 4, DPMD_OPT_PERSISTENT_FIRE = 5, DPMD_OPT_PERSIST_BLOCK = 6, DPMD_OPT_PERSIST_PHASE = 7,
-    DPMD_OPT_DEVICE_REBUILD = 8
+    DPMD_OPT_DEVICE_REBUILD = 8, DPMD_OPT_PERSIST_LDS = 9
 };
 int dpmd_set_option(dpmd_ctx* ctx, int key, int value);
 int dpmd_get_option(dpmd_ctx* ctx, int key, int* value);

@@ -26,15 +26,18 @@ def batch_mixed():
                         [0, 1, 0, 1, 0, 1, 0, 1], seed=11)
 
 
-def run(b, persist, block=512, max_steps=20000, ftol=1e-12, phase=2000):
-    """persist: 0 per-step launches, 1 persistent phases with hand-off, 2 persistent only."""
+def run(b, persist, block=512, max_steps=20000, ftol=1e-12, phase=2000, lds=1):
+    """persist: 0 per-step launches, 1 persistent phases with hand-off, 2 persistent only;
+    lds: 1 lets launches whose systems have at most 1536 particles hold them in LDS."""
     P = Disk()
     W.load_into(P, b)
     P.init_neighbors()
     L.check(L.lib().dpmd_set_option(P.ctx, L.OPT_PERSISTENT_FIRE, int(persist)), P.ctx)
     L.check(L.lib().dpmd_set_option(P.ctx, L.OPT_PERSIST_BLOCK, block), P.ctx)
     L.check(L.lib().dpmd_set_option(P.ctx, L.OPT_PERSIST_PHASE, phase), P.ctx)
-    print(f"persist={persist} block={block} phase={phase} S={b.S} N={b.N} max_steps={max_steps}", flush=True)
+    L.check(L.lib().dpmd_set_option(P.ctx, L.OPT_PERSIST_LDS, lds), P.ctx)
+    print(f"persist={persist} block={block} lds={lds} phase={phase} S={b.S} N={b.N} max_steps={max_steps}",
+          flush=True)
     steps, conv = run_fire(P, 1e-2, max_steps, L.FIRE_PRED_MAX_FORCE, ftol, mode=L.FORCE_PER_SYSTEM)
     S = P.n_systems()
     st = [np.empty(S), np.empty(S), np.empty(S, np.int32), np.empty(S, np.int32)]
@@ -48,11 +51,11 @@ def lean_converged():
     return run(batch_mixed(), 0, max_steps=100000, ftol=1e-8)
 
 
-@pytest.mark.parametrize("block", [256, 512, 1024])
-def test_persistent_equals_graph_batches_to_convergence(lean_converged, block):
+@pytest.mark.parametrize("block,lds", [(256, 0), (512, 0), (1024, 0), (512, 1)])
+def test_persistent_equals_graph_batches_to_convergence(lean_converged, block, lds):
     b = batch_mixed()
     Q, qsteps, qconv, qst = lean_converged
-    P, steps, conv, st = run(b, 2, block, max_steps=100000, ftol=1e-8, phase=7000)
+    P, steps, conv, st = run(b, 2, block, max_steps=100000, ftol=1e-8, phase=7000, lds=lds)
     assert (steps, conv) == (qsteps, qconv) and conv
     for a, c in zip(st, qst):
         assert np.array_equal(a, c)
@@ -63,10 +66,11 @@ def test_persistent_equals_graph_batches_to_convergence(lean_converged, block):
     assert ns.max() == steps and ns.min() < steps
 
 
-def test_persistent_equals_graph_batches_at_step_cap():
+@pytest.mark.parametrize("lds", [0, 1])
+def test_persistent_equals_graph_batches_at_step_cap(lds):
     b = batch_mixed()
     Q, qsteps, qconv, qst = run(b, 0, max_steps=700, ftol=1e-12)
-    P, steps, conv, st = run(b, 2, max_steps=700, ftol=1e-12, phase=300)
+    P, steps, conv, st = run(b, 2, max_steps=700, ftol=1e-12, phase=300, lds=lds)
     assert (steps, conv) == (qsteps, qconv) == (700, 0)
     for a, c in zip(st, qst):
         assert np.array_equal(a, c)
@@ -74,9 +78,10 @@ def test_persistent_equals_graph_batches_at_step_cap():
         assert np.array_equal(P.get(f), Q.get(f)), f
 
 
-def test_persistent_matches_oracle():
+@pytest.mark.parametrize("lds", [0, 1])
+def test_persistent_matches_oracle(lds):
     b = batch_mixed()
-    P, steps, conv, _ = run(b, 2, max_steps=3000, ftol=1e-11, phase=1000)
+    P, steps, conv, _ = run(b, 2, max_steps=3000, ftol=1e-11, phase=1000, lds=lds)
     ref = orc.OracleDisk(b)
     rsteps, rconv = ref.fire_minimize(1e-2, 3000, pred=1, ftol=1e-11, mode=2)
     assert P.get("FIRE_STEPS").min() < steps   # some systems stopped within a phase
@@ -116,6 +121,28 @@ def test_persistent_option_validation():
     for bad in (-1, 3):
         assert L.lib().dpmd_set_option(P.ctx, L.OPT_PERSISTENT_FIRE, bad) == 1
     assert L.lib().dpmd_set_option(P.ctx, L.OPT_PERSIST_PHASE, 0) == 1
+    L.check(L.lib().dpmd_get_option(P.ctx, L.OPT_PERSIST_LDS, C.byref(v)), P.ctx)
+    assert v.value == 1
+    assert L.lib().dpmd_set_option(P.ctx, L.OPT_PERSIST_LDS, 2) == 1
+
+
+def test_lds_engine_single_system_from_random_start():
+    """One 1024-disk system from uniform random positions (the plain-method workload): many
+    rebuilds inside the LDS-resident run, each storing the state, rebuilding and reloading.
+    Equal to the global-memory engine and to the oracle."""
+    b = W.single_disk_system(1024, 0.85, seed=1)
+    Q, qsteps, qconv, qst = run(b, 2, max_steps=4000, ftol=1e-12, phase=4000, lds=0)
+    P, steps, conv, st = run(b, 2, max_steps=4000, ftol=1e-12, phase=4000, lds=1)
+    assert (steps, conv) == (qsteps, qconv)
+    for x, y in zip(st, qst):
+        assert np.array_equal(x, y)
+    for f in FIELDS:
+        assert np.array_equal(P.get(f), Q.get(f)), f
+    ref = orc.OracleDisk(b)
+    rsteps, rconv = ref.fire_minimize(1e-2, 4000, pred=1, ftol=1e-12, mode=L.FORCE_PER_SYSTEM)
+    assert steps == rsteps and bool(conv) == bool(rconv)
+    for f, n in (("x", "POS_X"), ("vx", "VEL_X"), ("fy", "FORCE_Y"), ("pe", "PE")):
+        assert np.array_equal(P.get(n), ref.get(f)), f
 
 
 def test_hand_off_to_per_step_launches_mid_run():
