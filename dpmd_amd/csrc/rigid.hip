@@ -111,9 +111,10 @@ __device__ __forceinline__ void particle_force_one(const RigidArgs& a, int i, do
     const double px = a.x[i], py = a.y[i];
     double fxi = 0.0, fyi = 0.0, t = 0.0, p = 0.0;
     const int jb = a.poff[i], je = a.poff[i + 1];
-    // vertices in groups of four: the loads of a group are issued before the sums, which
-    // run in canonical vertex order
-    constexpr int G = 4;
+    // vertices in groups of eight: the loads of a group are issued before the sums, which
+    // run in canonical vertex order (a launch has one thread per particle, so few waves;
+    // deeper groups mean fewer dependent round trips per particle)
+    constexpr int G = 8;
     for (int j0 = jb; j0 < je; j0 += G) {
         int v[G];
         double fvx[G], fvy[G], pv[G], vx[G], vy[G];
@@ -506,6 +507,86 @@ k_particle_v(RigidArgs a, const double* __restrict__ dth, const double* __restri
     }
 }
 
+// ---- FIRE with one thread per vertex -------------------------------------------------------
+// k_fire_drift's two rigid-body drifts (update_positions(dt_reverse) and, after the kick and
+// mixing, update_positions(dt / 2)) for G particles: the centre, angle, kick and mixing on G
+// threads, then each vertex thread applies both rotations in turn, exactly as the two
+// drift_one calls do through memory.  Frozen systems are left untouched.
+__global__ void __launch_bounds__(DPMD_BLOCK) k_fire_drift_v(RigidArgs a, FireArgs f, int G, Gate g) {
+    if (!gate_open(g)) return;
+    __shared__ double l_x0[kRigidGroupMax], l_y0[kRigidGroupMax], l_x1[kRigidGroupMax], l_y1[kRigidGroupMax];
+    __shared__ double l_c1[kRigidGroupMax], l_s1[kRigidGroupMax], l_x2[kRigidGroupMax], l_y2[kRigidGroupMax];
+    __shared__ double l_c2[kRigidGroupMax], l_s2[kRigidGroupMax];
+    __shared__ int l_off[kRigidGroupMax + 1], l_live[kRigidGroupMax];
+    const int p0 = blockIdx.x * G;
+    const int np = min(G, a.n - p0);
+    const int t = threadIdx.x;
+    if (t < np) {
+        const int i = p0 + t;
+        const int s = a.sid[i];
+        const bool live = !fire_frozen(f, s);
+        if (live) {
+            const double dt = f.s_dt[s], alpha = f.s_alpha[s], dtrev = f.s_dtrev[s], vscale = f.s_vscale[s];
+            double vx = a.vx[i], vy = a.vy[i], w = a.w[i];
+            const double fx = a.fx[i], fy = a.fy[i], tq = a.tq[i];
+            const double x0 = a.x[i], y0 = a.y[i], a0 = a.th[i];
+            const double sc1 = dtrev * 1.0;                                        // update_positions(dt_rev, 1)
+            const double x1 = x0 + vx * sc1, y1 = y0 + vy * sc1, a1 = a0 + w * sc1;
+            double s1, c1;
+            own_sincos(a1 - a0, &s1, &c1);
+            vx *= vscale; vy *= vscale; w *= vscale;                                 // scale_velocities
+            kick_one(a, i, dt * 0.5, fx, fy, tq, vx, vy, w);                         // update_velocities(dt, 0.5)
+            mix_one(vx, vy, w, fx, fy, tq, alpha);                                   // mix
+            vx *= vscale; vy *= vscale; w *= vscale;                                 // scale_velocities
+            const double sc2 = dt * 0.5;                                             // update_positions(dt, 0.5)
+            const double x2 = x1 + vx * sc2, y2 = y1 + vy * sc2, a2 = a1 + w * sc2;
+            double s2, c2;
+            own_sincos(a2 - a1, &s2, &c2);
+            a.x[i] = x2; a.y[i] = y2; a.th[i] = a2; a.vx[i] = vx; a.vy[i] = vy; a.w[i] = w;
+            if (i == a.off[s]) {
+                f.dt[s] = dt; f.alpha[s] = alpha; f.ngood[s] = f.s_ngood[s]; f.nbad[s] = f.s_nbad[s];
+            }
+            l_x0[t] = x0; l_y0[t] = y0; l_x1[t] = x1; l_y1[t] = y1; l_c1[t] = c1; l_s1[t] = s1;
+            l_x2[t] = x2; l_y2[t] = y2; l_c2[t] = c2; l_s2[t] = s2;
+        }
+        l_live[t] = live ? 1 : 0;
+    }
+    if (t <= np) l_off[t] = a.poff[p0 + t];
+    __syncthreads();
+    const int c = l_off[0] + t;
+    const bool valid = c < l_off[np];
+    int q = 0, s = -1;
+    bool flag = false;
+    if (valid) {
+        while (l_off[q + 1] <= c) ++q;
+        const int i = p0 + q;
+        s = a.sid[i];
+        if (l_live[q]) {
+            const int v = a.vstatic[c];
+            const bool first = a.cell && c == l_off[q];   // the skin test uses the first vertex
+            double dx = a.vpx[v] - l_x0[q], dy = a.vpy[v] - l_y0[q];
+            double rx = l_c1[q] * dx - l_s1[q] * dy, ry = l_s1[q] * dx + l_c1[q] * dy;
+            const double v1x = l_x1[q] + rx, v1y = l_y1[q] + ry;
+            bool f1 = false, f2 = false;
+            if (first) {
+                const double ddx = v1x - a.lx[i], ddy = v1y - a.ly[i];
+                f1 = (ddx * ddx + ddy * ddy) > a.thresh2[s];
+            }
+            dx = v1x - l_x1[q]; dy = v1y - l_y1[q];
+            rx = l_c2[q] * dx - l_s2[q] * dy; ry = l_s2[q] * dx + l_c2[q] * dy;
+            const double v2x = l_x2[q] + rx, v2y = l_y2[q] + ry;
+            if (first) {
+                const double ddx = v2x - a.lx[i], ddy = v2y - a.ly[i];
+                f2 = (ddx * ddx + ddy * ddy) > a.thresh2[s];
+            }
+            a.vpx[v] = v2x;
+            a.vpy[v] = v2y;
+            flag = f2 || f1;
+        }
+    }
+    if (a.cell) raise_flags(a.flag, a.any, s, flag);
+}
+
 // Particles per workgroup of the vertex-parallel kernels (0: use the per-particle kernels);
 // at most DPMD_BLOCK - 1, so the G + 1 particle offsets are written by distinct threads.
 int rigid_group(dpmd_ctx* c) {
@@ -570,11 +651,17 @@ void rigid_stress(dpmd_ctx* c) {
     HIP_OK(hipGetLastError());
 }
 void rigid_fire_drift(dpmd_ctx* c, Gate g) {
-    k_fire_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), fire_args(c), g);
+    if (const int G = rigid_group(c))
+        k_fire_drift_v<<<(c->N + G - 1) / G, DPMD_BLOCK, 0, c->st>>>(rigid_args(c), fire_args(c), G, g);
+    else
+        k_fire_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), fire_args(c), g);
     HIP_OK(hipGetLastError());
 }
 void rigid_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, double pa, double pb) {
     launch_vertex_force(c, c->fire_mode, g);
+    // one thread per particle: the chunk partials need the particles of a 256-chunk in one
+    // workgroup, and a vertex-parallel variant plus a separate partials pass measured slower
+    // (24.9 + 4.2 us against 21.0 us on 256 x 256 bodies)
     k_fire_particle<<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(rigid_args(c), fire_args(c), red_args(c), kick, g);
     HIP_OK(hipGetLastError());
     fire_finalize(c, g, do_conv, pred, pa, pb);
