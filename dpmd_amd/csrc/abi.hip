@@ -500,8 +500,10 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
     int total = 0, m = 8, since_halt = 0;
     bool done = false;
     // Rebuilds per step, smoothed over batches.  A halted batch costs a host round trip and
-    // restarts with small batches; above one rebuild per 5 steps (back below one per 10) the
-    // batches rebuild on the device instead.
+    // restarts with small batches; above one rebuild per 2 steps (back below 0.35 per step)
+    // the batches rebuild on the device instead.  A halt costs about 95 us and the gated
+    // device chain about 21 us per step, but the smoothed rate lags FIRE's early burst of
+    // rebuilds, so the switch sits above the 0.2 break-even.
     double rate = 0.0;
     bool dev = false;
     int rebuilds = read_run(c).rebuilds;
@@ -509,7 +511,7 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
         while (!done && total < max_steps) {
             int lg = 0;
             const bool cell = c->method == DPMD_NEIGHBOR_CELL;
-            dev = cell && (c->dev_rebuild == 2 || (c->dev_rebuild == 0 && rate > (dev ? 0.1 : 0.2)));
+            dev = cell && (c->dev_rebuild == 2 || (c->dev_rebuild == 0 && rate > (dev ? 0.35 : 0.5)));
             if (dev) m = std::max(m, 8);
             while ((2 << lg) <= std::min(std::min(m, c->max_batch), max_steps - total) && lg < 7) ++lg;
             const int bm = 1 << lg;

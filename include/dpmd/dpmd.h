@@ -382,8 +382,8 @@ int dpmd_stream_copy(double* dst, const double* src, int64_t n, int repeats, dou
  *   DPMD_OPT_DEVICE_REBUILD: how step batches of device-driven runs meet a rebuild.  1: the
  *     batch halts and the host finishes the step (cheapest when rebuilds are rare); 2: every
  *     step runs the rebuild chain on the device, gated by the rebuild flags, and batches never
- *     halt; 0 (default): 1 while rebuilds are rarer than one per 5 steps, 2 above (back below
- *     one per 10).  Same results.
+ *     halt; 0 (default): 1 while rebuilds are rarer than one per 2 steps, 2 above (back below
+ *     0.35 per step).  Same results.
  *   DPMD_OPT_PERSIST_LDS: 1 (default) runs persistent FIRE launches whose systems all have at
  *     most 1536 particles with each system's particle state held in LDS (one 512-thread
  *     workgroup per system); 0 keeps it in global memory.  Same results. */
