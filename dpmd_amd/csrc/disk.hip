@@ -118,10 +118,10 @@ __global__ void k_stress(DiskArgs a, double* sxx, double* sxy, double* syx, doub
 //   k_fire_drift   fire_update result -> reverse drift, vscale, half kick, mixing, vscale,
 //                  half drift, skin check (all per particle, in the reference's order);
 //   rebuild        (device-side conditional, cells.hip);
-//   k_fire_force   forces, half kick, per-system F.v / pe / max|F|^2 partials; the last
-//                  workgroup of each system folds them (fixed tree), applies the stopping
-//                  test (fire.cu:59-78) and evaluates fire_update (fire.cu:7-57) for the
-//                  next step into the s_* ("next-step") scalars.
+//   k_fire_force   forces, half kick, per-chunk F.v / pe / max|F|^2 partials;
+//   k_fire_finalize one workgroup per system folds the partials (fixed tree), applies the
+//                  stopping test (fire.cu:59-78) and evaluates fire_update (fire.cu:7-57)
+//                  for the next step into the s_* ("next-step") scalars.
 // k_fire_drift commits s_* into the live scalars, so the live dt/alpha/N_good/N_bad are
 // always what the reference holds after the same number of steps.
 // ---------------------------------------------------------------------------------------
