@@ -1856,8 +1856,13 @@ int dpmd_fill_i32(void* p, int64_t n, int32_t v) {
 int dpmd_set_option(dpmd_ctx* c, int key, int value) {
     return guard(c, [&] {
         switch (key) {
-            case DPMD_OPT_COOP_REBUILD: c->use_coop = value != 0; break;
-            case DPMD_OPT_COOP_IN_GRAPH: c->coop_in_graph = value != 0; break;
+            case DPMD_OPT_COOP_REBUILD:
+            case DPMD_OPT_COOP_IN_GRAPH:
+                // The cooperative single-launch rebuild was removed: it relied on a grid-wide
+                // barrier whose co-residency a graph replay of the node does not guarantee.
+                need(value == 0, "DPMD_OPT_COOP_REBUILD / DPMD_OPT_COOP_IN_GRAPH: the cooperative rebuild is not "
+                                 "available (only 0 is accepted)");
+                break;
             case DPMD_OPT_FUSED_VV: c->fuse_vv = value != 0; break;
             case DPMD_OPT_BLOCK:
                 need(value == 64 || value == 128 || value == 256 || value == 512,
@@ -1897,8 +1902,8 @@ int dpmd_set_option(dpmd_ctx* c, int key, int value) {
 int dpmd_get_option(dpmd_ctx* c, int key, int* value) {
     return guard(c, [&] {
         switch (key) {
-            case DPMD_OPT_COOP_REBUILD: *value = c->use_coop ? 1 : 0; break;
-            case DPMD_OPT_COOP_IN_GRAPH: *value = c->coop_in_graph ? 1 : 0; break;
+            case DPMD_OPT_COOP_REBUILD:
+            case DPMD_OPT_COOP_IN_GRAPH: *value = 0; break;
             case DPMD_OPT_FUSED_VV: *value = c->fuse_vv ? 1 : 0; break;
             case DPMD_OPT_BLOCK: *value = c->block; break;
             case DPMD_OPT_PERSISTENT_FIRE: *value = c->persist_fire; break;

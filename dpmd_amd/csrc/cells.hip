@@ -8,10 +8,6 @@
 //  * only systems whose rebuild flag is set are rebuilt (the reference rebuilds every
 //    system when any flag is set, disk.cu:1127-1138); the decision is taken on the device,
 //    so no host round trip happens per step;
-//  * the whole rebuild is ONE cooperative launch with grid-wide barriers between its
-//    phases; when no system is flagged every workgroup leaves at once, so the per-step
-//    check costs a single (empty) launch.  A multi-launch form of the same phases is kept
-//    for graph capture where cooperative launches cannot be captured;
 //  * in-cell order is deterministic (ascending current index), by an atomic scatter
 //    followed by a per-cell insertion sort; the reference's atomic scatter order is
 //    nondeterministic (base_particle_kernels.cu:246-259);
@@ -19,10 +15,6 @@
 //    for one-thread-per-object force kernels), with the reference's stencil order.
 #include "internal.hpp"
 #include "cells_dev.hpp"
-#include <hip/hip_cooperative_groups.h>
-
-namespace cg = cooperative_groups;
-
 namespace dpmd {
 namespace {
 
@@ -209,52 +201,6 @@ __global__ void k_clear_flags(int S, unsigned* flag, int* any, const int* active
     if (s == 0) *any = 0;
 }
 
-// ---- single cooperative launch -----------------------------------------------------------
-// Every workgroup runs every phase with a grid-stride loop; phases are separated by
-// grid-wide barriers.  All exits happen before the first barrier and depend only on values
-// that no workgroup writes during the launch (active, the rebuild flags), so every
-// workgroup takes the same path.
-__global__ void __launch_bounds__(DPMD_BLOCK) k_rebuild_coop(RebuildArgs r, const int* active) {
-    __shared__ int lds[4];
-    __shared__ int any;
-    if (!*active) return;
-    if (threadIdx.x == 0) any = 0;
-    __syncthreads();
-    for (int s = threadIdx.x; s < r.S; s += DPMD_BLOCK)
-        if (r.ca.flag[s]) any = 1;
-    __syncthreads();
-    if (!any) return;
-    cg::grid_group g = cg::this_grid();
-    const int gt = blockIdx.x * DPMD_BLOCK + threadIdx.x;
-    const int gs = gridDim.x * DPMD_BLOCK;
-    const int n = r.o.n, C = r.ca.C;
-    for (int c = gt; c < C; c += gs) ph_zero(r, c);
-    g.sync();
-    for (int i = gt; i < n; i += gs) ph_hash(r, i);
-    g.sync();
-    for (int t = blockIdx.x; t < r.nt; t += gridDim.x) ph_tile_sum(r, t, lds);
-    g.sync();
-    if (blockIdx.x == 0) ph_top_scan(r, lds);
-    g.sync();
-    for (int t = blockIdx.x; t < r.nt; t += gridDim.x) ph_tile_down(r, t, lds);
-    g.sync();
-    for (int i = gt; i < n; i += gs) ph_scatter(r, i);
-    g.sync();
-    for (int c = gt; c < C; c += gs) ph_sort(r, c);
-    g.sync();
-    for (int p = gt; p < n; p += gs) ph_gather(r, p);
-    g.sync();
-    for (int p = gt; p < n; p += gs) ph_copy_back(r, p);
-    g.sync();
-    for (int i = gt; i < n; i += gs) ph_neighbors(r, i);
-    if (r.plx)
-        for (int i = gt; i < r.N; i += gs) ph_rigid_reset(r, i);
-    g.sync();
-    if (blockIdx.x == 0)
-        for (int s = threadIdx.x; s < r.S; s += DPMD_BLOCK) r.ca.flag[s] = 0u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *r.any = 0;
-}
-
 // set_naive_neighbor_count + fill_naive_neighbor_list_kernel (base_point_particle_kernels.cu:
 // 6-35); vertex form excludes same-particle vertices (base_poly_particle_kernels.cu:18-55).
 __global__ void k_naive(ObjArgs o, NbrArgs nb) {
@@ -350,18 +296,6 @@ void rebuild_multi(dpmd_ctx* c, const RebuildArgs& r, const int* active) {
     k_clear_flags<<<blocks_for(r.S), DPMD_BLOCK, 0, st>>>(r.S, r.ca.flag, r.any, active);
 }
 
-int coop_grid(dpmd_ctx* c) {
-    if (c->coop_blocks <= 0) {
-        int dev = 0, ncu = 0, per_cu = 0;
-        HIP_OK(hipGetDevice(&dev));
-        HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_rebuild_coop, DPMD_BLOCK, 0));
-        c->coop_blocks = std::max(1, ncu * std::max(1, per_cu));
-    }
-    const int need = std::max(blocks_for(c->n_obj()), blocks_for(c->C));
-    return std::max(1, std::min(c->coop_blocks, need));
-}
-
 }  // namespace
 
 RebuildArgs rebuild_args(dpmd_ctx* c) { return make_rebuild_args(c); }
@@ -409,14 +343,7 @@ void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all) {
     hipStream_t st = c->st;
     if (force_all) k_set_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, c->flag.p, active);
     RebuildArgs r = make_rebuild_args(c);
-    const bool coop = c->use_coop && !(c->capturing && !c->coop_in_graph);
-    if (coop) {
-        void* args[] = {(void*)&r, (void*)&active};
-        HIP_OK(hipLaunchCooperativeKernel((const void*)k_rebuild_coop, dim3(coop_grid(c)), dim3(DPMD_BLOCK), args, 0,
-                                          st));
-    } else {
-        rebuild_multi(c, r, active);
-    }
+    rebuild_multi(c, r, active);
     HIP_OK(hipGetLastError());
 }
 

@@ -183,10 +183,6 @@ struct dpmd_ctx {
     int g_fire_key[4] = {0, 0, 0, 0};
     double g_fire_keyd[2] = {0, 0};
     int g_vv_key = -1;
-    // rebuild as one cooperative launch (use_coop, off by default: a full-device grid of it
-    // did not finish in one run, cause not found); whether that launch may be captured
-    bool use_coop = false, coop_in_graph = false;
-    int coop_blocks = 0;
     // adjoints: scratch, CG vectors (current order), NVE checkpoints (canonical order)
     dpmd::DVec<double> adj_bx, adj_by, adj_sa, adj_sb, adj_sc, adj_sd;
     dpmd::DVec<double> adj_t0, adj_t1, adj_t2, adj_t3, adj_t4, adj_t5, adj_t6;

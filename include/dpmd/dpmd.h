@@ -359,10 +359,8 @@ int dpmd_profile_durations(dpmd_ctx* ctx, float* ms, int capacity, int* count);
 int dpmd_stream_copy(double* dst, const double* src, int64_t n, int repeats, double* ms_per_copy);
 
 /* Execution options (no reference counterpart).
- *   DPMD_OPT_COOP_REBUILD: 1 = the cell/neighbor rebuild is one cooperative launch that exits
- *     at once when no system is flagged; 0 (default) = one launch per rebuild phase.
- *   DPMD_OPT_COOP_IN_GRAPH: 1 while cooperative launches may be captured into hipGraphs;
- *     the library clears it by itself if capture of that launch fails.
+ *   DPMD_OPT_COOP_REBUILD, DPMD_OPT_COOP_IN_GRAPH: accept only 0 (a former cooperative
+ *     single-launch rebuild was removed; setting 1 returns DPMD_ERR_ARGUMENT).
  *   DPMD_OPT_FUSED_VV: 1 (default) = device-driven velocity-Verlet runs of disks compute a
  *     step's forces and the next step's kick and drift in one launch (positions ping-pong
  *     between two buffers); 0 = one drift and one force launch per step.  Same results.

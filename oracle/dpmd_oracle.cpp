@@ -870,6 +870,18 @@ struct Rigid {
             o[s] = m;
         }
     }
+    // set_n_dof_kernel (rigid_bumpy.cu:743-757): 2 per particle plus 1 per particle whose
+    // moment of inertia is positive.
+    int n_dof(int s) const {
+        int d = 0;
+        for (int i = off[s]; i < off[s + 1]; ++i) d += 2 + (inertia[i] > 0.0 ? 1 : 0);
+        return d;
+    }
+    // compute_temperature_kernel (base_particle_kernels.cu:115-124): T = 2 KE / n_dof.
+    void temperature(double* o) {
+        ke_total(o);
+        for (int s = 0; s < S; ++s) o[s] = o[s] * 2.0 / (n_dof(s));
+    }
     // compute_stress_tensor_virial_kernel + _kinetic_kernel (rigid_bumpy.cu:968-1068): per
     // particle, vertex contributions summed in canonical vertex order (the reference adds
     // them with atomics in arbitrary order).
@@ -1097,6 +1109,8 @@ void or_rigid_pe_total(void* h, double* o) { ((Rigid*)h)->pe_total(o); }
 void or_rigid_ke_total(void* h, double* o) { ((Rigid*)h)->ke_total(o); }
 void or_rigid_fpower_total(void* h, double* o) { ((Rigid*)h)->fpower_total(o); }
 void or_rigid_pressure(void* h, double* o) { ((Rigid*)h)->pressure(o); }
+void or_rigid_temperature(void* h, double* o) { ((Rigid*)h)->temperature(o); }
+int or_rigid_n_dof(void* h, int s) { return ((Rigid*)h)->n_dof(s); }
 void or_rigid_max_force2(void* h, double* o) { ((Rigid*)h)->max_force2(o); }
 void or_rigid_fire_init(void* h, const double* dt, int mode) { ((Rigid*)h)->fire_init(dt, mode); }
 void or_rigid_fire_step(void* h, int mode) { ((Rigid*)h)->fire_step(mode); }

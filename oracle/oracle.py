@@ -380,6 +380,8 @@ _RPROTOS = {
     "or_rigid_ke_total": (None, [vp, vp]),
     "or_rigid_fpower_total": (None, [vp, vp]),
     "or_rigid_pressure": (None, [vp, vp]),
+    "or_rigid_temperature": (None, [vp, vp]),
+    "or_rigid_n_dof": (i32, [vp, i32]),
     "or_rigid_max_force2": (None, [vp, vp]),
     "or_rigid_fire_init": (None, [vp, vp, i32]),
     "or_rigid_fire_step": (None, [vp, i32]),
@@ -484,6 +486,8 @@ class OracleRigid:
     def ke_total(self): return self._per_sys(lib().or_rigid_ke_total)
     def fpower_total(self): return self._per_sys(lib().or_rigid_fpower_total)
     def pressure(self): return self._per_sys(lib().or_rigid_pressure)
+    def temperature(self): return self._per_sys(lib().or_rigid_temperature)
+    def n_dof(self): return np.array([lib().or_rigid_n_dof(self.h, s) for s in range(self.S)])
     def max_force2(self): return self._per_sys(lib().or_rigid_max_force2)
 
     def fire_init(self, dt, mode=0):

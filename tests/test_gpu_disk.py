@@ -49,6 +49,31 @@ def test_philox_device_matches_host():
     assert np.array_equal(P.get("POS_Y"), oy)
 
 
+def test_device_normals_box_muller():
+    """k_stateless_normal_2d (device_fields.cuh:209-216, 280-298): z0 = sqrt(-2 log u0) cos(2 pi u1),
+    z1 = ... sin(2 pi u1) on the Philox (u0, u1) of each index.  Against numpy's libm on the same
+    uniforms: the device log/cos/sin are other implementations, so the bar is a few ulp (relative
+    2e-15 of |z|, absolute 1e-15 near zeros); moments of 2^20 draws within 5 standard errors."""
+    n, seed, call = 1 << 20, 987654321, 7
+    b = W.single_disk_system(n, 0.85, seed=3)
+    P = Disk()
+    W.load_into(P, b)
+    P.rand_normal_2d("VEL_X", "VEL_Y", seed, call, 0.5, 2.0, -1.0, 0.25)
+    i = np.arange(n, dtype=np.uint64)
+    c0, c1, c2, c3 = W.philox4x32_10(i, np.uint64(call), 0, 0, np.uint64(seed & 0xFFFFFFFF), np.uint64(seed >> 32))
+    u0, u1 = W._u01d(c0, c1), W._u01d(c2, c3)
+    r = np.sqrt(-2.0 * np.log(u0))
+    th = 6.28318530717958647692 * u1
+    zx, zy = r * np.cos(th), r * np.sin(th)
+    gx, gy = (P.get("VEL_X") - 0.5) / 2.0, (P.get("VEL_Y") + 1.0) / 0.25
+    np.testing.assert_allclose(P.get("VEL_X"), 0.5 + 2.0 * zx, rtol=2e-15, atol=1e-15)
+    np.testing.assert_allclose(P.get("VEL_Y"), -1.0 + 0.25 * zy, rtol=2e-15, atol=1e-15)
+    se = 5.0 / np.sqrt(n)
+    for z in (gx, gy):
+        assert abs(z.mean()) < se and abs(z.var() - 1.0) < 5.0 * np.sqrt(2.0 / n)
+    assert abs(np.mean(gx * gy)) < se
+
+
 @pytest.mark.parametrize("which", ["single", "batch"])
 def test_neighbor_lists_match_oracle(which):
     b = W.single_disk_system(1024, 0.85, seed=1) if which == "single" else batch_small()
@@ -133,10 +158,15 @@ def test_fire_fused_run_bit_identical(graph):
     assert_same(P, ref, ("x", "y", "fx", "fy", "pe"), "fused FIRE")
 
 
-@pytest.mark.parametrize("coop", [0, 1])
+COOP_SKIP = ("the cooperative single-launch rebuild was removed: one full-device run of it went silent, and a "
+             "graph replay of its node runs it as an ordinary dispatch without the co-residency its grid-wide "
+             "barriers need; the option now only accepts 0")
+
+
+@pytest.mark.parametrize("coop", [0, pytest.param(1, marks=pytest.mark.skip(reason=COOP_SKIP))])
 @pytest.mark.parametrize("graph", [False, True])
 def test_rebuild_forms_agree(coop, graph):
-    """The cooperative single-launch rebuild and the multi-launch rebuild give the oracle's bits."""
+    """The device rebuild chain gives the oracle's bits, with and without graphs."""
     import ctypes as C
     b = batch_small()
     P = Disk()
@@ -152,6 +182,14 @@ def test_rebuild_forms_agree(coop, graph):
     v = C.c_int()
     L.check(L.lib().dpmd_get_option(P.ctx, L.OPT_COOP_IN_GRAPH, C.byref(v)), P.ctx)
     print(f"coop={coop} graph={graph} coop_in_graph={v.value}")
+
+
+def test_cooperative_rebuild_option_is_rejected():
+    P = Disk()
+    W.load_into(P, batch_small())
+    assert L.lib().dpmd_set_option(P.ctx, L.OPT_COOP_REBUILD, 1) == 1        # DPMD_ERR_ARGUMENT
+    assert b"cooperative" in L.lib().dpmd_last_error(P.ctx)
+    assert L.lib().dpmd_set_option(P.ctx, L.OPT_COOP_REBUILD, 0) == 0
 
 
 def test_vv_bit_identical_and_graph():

@@ -92,3 +92,21 @@ def test_rigid_vv_and_energies():
     assert np.array_equal(P.get("KE_TOTAL"), ref.ke_total())
     assert np.array_equal(P.get("FPOWER_TOTAL"), ref.fpower_total())
     np.testing.assert_allclose(P.get("PRESSURE"), ref.pressure(), rtol=1e-12)
+    # set_n_dof_kernel (rigid_bumpy.cu:743-757) and T = 2 KE / n_dof
+    P.compute_temperature()
+    assert np.array_equal(P.get("N_DOF"), ref.n_dof())
+    assert np.array_equal(P.get("TEMPERATURE"), ref.temperature())
+
+
+def test_rigid_n_dof_counts_only_bodies_with_inertia():
+    b = small(1, seed=5)
+    b.inertia[::7] = 0.0
+    P, ref = make(b)
+    rng = np.random.default_rng(3)
+    for f, r in (("VEL_X", "vx"), ("VEL_Y", "vy"), ("ANGULAR_VEL", "w")):
+        v = rng.standard_normal(b.N) * 1e-2
+        P.set(f, v); ref.set(r, v)
+    P.compute_temperature()
+    nd = [int(np.sum(2 + (b.inertia[b.off[s]:b.off[s + 1]] > 0))) for s in range(b.S)]
+    assert list(P.get("N_DOF")) == nd == list(ref.n_dof())
+    assert np.array_equal(P.get("TEMPERATURE"), ref.temperature())

@@ -355,3 +355,41 @@ def test_increment_packing_fraction():
     st, ids = d.neighbors_csr()
     for i in range(0, b.N, 5):
         assert sorted(ids[st[i]:st[i + 1]]) == sorted(d.brute(i))
+
+
+def test_fire_three_steps_against_a_wall_closed_form():
+    """The FIRE step order (fire.cuh:59-88) and the uphill branch (fire.cu:38-56, T3), worked by
+    hand for one disk (r = 0.5, k = 2, m = 1) that starts pressed 0.1 into the left wall of a
+    walled box, dt0 = 2.2 (large, so that the disk leaves the wall on step 2).
+      init: v = 0, F = k (r - x) = 0.2.
+      step 1: P = 0 is uphill -> dt = 1.1, dt_rev = -0.55, vscale = 0; v is zeroed, kicked
+              (0.11), mixed, zeroed again (vscale after the mixing), no drift; closing kick
+              v = 0.11.
+      step 2: P > 0 -> N_good = 1, dt stays 1.1; kick v = 0.22, mixing keeps v (parallel to F),
+              drift x = 0.4 + 0.22 * 0.55 = 0.521 > r: out of contact, F = 0, v = 0.22.
+      step 3: P = 0 -> uphill, dt = 0.55, dt_rev = -0.275: positions move back by
+              0.22 * 0.275 = 0.0605 (with the NEW dt) BEFORE velocities are scaled to zero ->
+              x = 0.4605; then F = 2 * 0.0395, closing kick v = 0.079 * 0.275.
+    Scaling before the reverse drift, or reversing with the old dt, gives another x."""
+    r, k = 0.5, 2.0
+    b = W.DiskBatch(off=np.array([0, 1], np.int32), rad=np.array([r]), mass=np.array([1.0]),
+                    area=np.array([math.pi * r * r]), Lx=np.array([10.0]), Ly=np.array([10.0]),
+                    e=np.array([k]), cutoff=np.array([0.3]), thresh2=np.array([0.0225]),
+                    boundary=np.array([1], np.int32), x=np.array([0.4]), y=np.array([5.0]),
+                    cdx=np.zeros(1, np.int32), cdy=np.zeros(1, np.int32))
+    d = orc.OracleDisk(b)
+    d.fire_init(2.2, mode=1)
+    assert d.get("fx")[0] == pytest.approx(0.2, rel=1e-15)
+    expect = [  # x, vx, fx, dt, N_good, N_bad
+        (0.4, 0.11, 0.2, 1.1, 0, 1),
+        (0.521, 0.22, 0.0, 1.1, 1, 0),
+        (0.4605, 0.079 * 0.275, 0.079, 0.55, 0, 1),
+    ]
+    for step, (x, v, f, dt, ng, nb) in enumerate(expect, 1):
+        d.fire_step(mode=1)
+        got = d.get("x")[0], d.get("vx")[0], d.get("fx")[0]
+        np.testing.assert_allclose(got, (x, v, f), rtol=1e-13, atol=1e-16, err_msg=f"step {step}")
+        assert d.get("vy")[0] == 0.0 and d.get("y")[0] == 5.0
+        gdt, gal, gng, gnb = d.fire_state()
+        assert gdt[0] == pytest.approx(dt, rel=1e-15) and gal[0] == 0.1 and (gng[0], gnb[0]) == (ng, nb)
+    np.testing.assert_allclose(d.pe_total(), [0.5 * k * 0.0395 ** 2], rtol=1e-12)

@@ -160,3 +160,131 @@ def test_rigid_fire_reaches_minimum_in_walls():
     steps, conv = o.fire_minimize(1e-2, 20000, pred=1, ftol=1e-9, mode=1)
     assert conv and np.all(o.pe_total() < pe0)
     assert np.all(np.sqrt(o.max_force2()) < 1e-9)
+
+
+def bodies(xs, ys, angles, R=0.5, rb=0.2, nv=2, L=10.0, boundary=0, k=2.0, inertia=None, per_system=False):
+    """Bodies of nv vertices on a circle of radius R at the given centres and angles, vertex
+    k at angle theta + 2 pi k / nv; one system (or one system per body)."""
+    xs, ys, angles = (np.asarray(a, float) for a in (xs, ys, angles))
+    n = len(xs)
+    S = n if per_system else 1
+    kk = np.tile(np.arange(nv), n)
+    pid = np.repeat(np.arange(n), nv)
+    ang = angles[pid] + 2.0 * math.pi * kk / nv
+    I = np.full(n, 0.5) if inertia is None else np.asarray(inertia, float)
+    off = np.arange(n + 1, dtype=np.int32) if per_system else np.array([0, n], np.int32)
+    return W.RigidBatch(
+        off=off, nvpp=np.full(n, nv, np.int32), x=xs.copy(), y=ys.copy(), angle=angles.copy(),
+        mass=np.ones(n), inertia=I, area=np.ones(n), vx=xs[pid] + R * np.cos(ang), vy=ys[pid] + R * np.sin(ang),
+        vrad=np.full(n * nv, rb), vmass=np.full(n * nv, 1.0 / nv), Lx=np.full(S, L), Ly=np.full(S, L),
+        e=np.full(S, k), cutoff=np.full(S, 0.1), thresh2=np.full(S, 0.0025), boundary=np.full(S, boundary, np.int32),
+        cdx=np.zeros(S, np.int32), cdy=np.zeros(S, np.int32))
+
+
+def test_ke_temperature_and_ndof_of_free_bodies_closed_form():
+    """compute_ke_kernel (rigid_bumpy.cu:507-524): KE = (m |v|^2 + I w^2) / 2; set_n_dof_kernel
+    (:743-757): 2 per body plus 1 per body with I > 0; T = 2 KE / n_dof
+    (base_particle_kernels.cu:115-124).  Bodies far apart (no contacts), one of them with I = 0."""
+    b = bodies([2.0, 5.0, 8.0], [2.0, 5.0, 8.0], [0.0, 1.0, 2.0], inertia=[0.5, 0.0, 0.245])
+    o = orc.OracleRigid(b)
+    v = np.array([0.3, -0.2, 0.0]), np.array([0.1, 0.4, -0.5]), np.array([2.0, 3.0, -4.0])
+    o.set("vx", v[0]); o.set("vy", v[1]); o.set("w", v[2])
+    ke = 0.5 * (1.0 * (0.3 ** 2 + 0.1 ** 2) + 0.5 * 2.0 ** 2) + 0.5 * (0.2 ** 2 + 0.4 ** 2) \
+        + 0.5 * (0.5 ** 2 + 0.245 * 4.0 ** 2)
+    np.testing.assert_allclose(o.ke_total(), [ke], rtol=1e-15)
+    assert list(o.n_dof()) == [2 * 3 + 2]
+    np.testing.assert_allclose(o.temperature(), [2.0 * ke / 8.0], rtol=1e-15)
+    # a spinning body alone: T = I w^2 / 3
+    b1 = bodies([5.0], [5.0], [0.3], inertia=[0.7])
+    o1 = orc.OracleRigid(b1)
+    o1.set("w", np.array([-3.0]))
+    np.testing.assert_allclose(o1.ke_total(), [0.5 * 0.7 * 9.0], rtol=1e-15)
+    np.testing.assert_allclose(o1.temperature(), [0.7 * 9.0 / 3.0], rtol=1e-15)
+
+
+@pytest.mark.parametrize("across", [False, True])
+def test_virial_pressure_of_one_vertex_contact_closed_form(across):
+    """compute_stress_tensor_virial_kernel + _kinetic_kernel (rigid_bumpy.cu:968-1068): each
+    directed vertex pair adds (R_i - R_j) F_ij / 2 with the minimum-image centre separation,
+    so one contact gives (R_0 - R_1) . F_01; the kinetic part is m v v (no rotation);
+    P = (sxx + syy) / 2.  With `across`, the contact and the centres straddle the periodic
+    boundary x = L."""
+    L, R, rb, k, h = 10.0, 0.5, 0.3, 2.0, 0.1
+    X0 = L - 0.6 if across else 4.0
+    X1 = X0 + 2 * R + 0.5                      # vertex 0 of body 1 faces vertex 1 of body 0
+    b = bodies([X0, X1], [5.0, 5.0], [0.0, 0.0], R=R, rb=rb, L=L, k=k)
+    # vertex 1 of body 0 (angle pi) sits at X0 - R; make body 0's vertex 0 (angle 0) at X0 + R
+    # face body 1's vertex 1 (angle pi, at X1 - R): gap 0.5 < 2 rb; lift it by h off the axis
+    b.vy[0] += h
+    if across:                                  # body 1 wrapped to the other side of the box
+        b.x[1] -= L; b.vx[2:] -= L
+    o = orc.OracleRigid(b)
+    o.check_neighbors(True)
+    o.compute_forces(0)
+    vxs, vys, ws = np.array([0.2, -0.1]), np.array([0.05, 0.3]), np.array([5.0, -7.0])
+    o.set("vx", vxs); o.set("vy", vys); o.set("w", ws)
+    dx, dy = (X0 + R) - (X1 - R), h              # vertex 0 of body 0 minus vertex 1 of body 1
+    dist = math.hypot(dx, dy)
+    delta = 2 * rb - dist
+    F = np.array([k * delta * dx / dist, k * delta * dy / dist])   # on body 0
+    np.testing.assert_allclose([o.get("fx")[0], o.get("fy")[0]], F, rtol=1e-12)
+    dR = np.array([X0 - X1, 0.0])                # minimum image of the centre separation
+    tr = dR @ F + np.sum(vxs ** 2 + vys ** 2)
+    np.testing.assert_allclose(o.pressure(), [0.5 * tr / (L * L)], rtol=1e-12)
+
+
+def test_rotational_mixing_hand_set():
+    """mix_velocities_and_forces (rigid_bumpy.cu:586-629): alpha == 0 returns early (no
+    zeroing); otherwise v <- v (1 - a) + F |v| / |F| a and w <- w (1 - a) + tau |w| / |tau| a,
+    with v zeroed when |F| <= 1e-16 and w zeroed when |tau| <= 1e-16."""
+    b = bodies([2.0, 4.0, 6.0, 8.0], [5.0] * 4, [0.0] * 4, per_system=True)
+    o = orc.OracleRigid(b)
+    o.set("vx", np.array([3.0, 3.0, 3.0, 3.0])); o.set("vy", np.array([4.0, 4.0, 4.0, 4.0]))
+    o.set("w", np.array([-2.0, -2.0, -2.0, -2.0]))
+    o.set("fx", np.array([0.0, 0.0, 0.0, 0.0])); o.set("fy", np.array([2.0, 2.0, 0.0, 0.0]))
+    o.set("torque", np.array([0.5, 0.0, 0.5, 0.0]))
+    o.mix(np.array([0.25, 0.25, 0.25, 0.0]))
+    # body 0: |v| = 5, |F| = 2, |w| = 2, |tau| = 0.5
+    # body 1: tau = 0 -> w zeroed; body 2: F = 0 -> v zeroed; body 3: alpha = 0 -> untouched
+    np.testing.assert_array_equal(o.get("vx"), [2.25, 2.25, 0.0, 3.0])
+    np.testing.assert_array_equal(o.get("vy"), [4.25, 4.25, 0.0, 4.0])
+    np.testing.assert_array_equal(o.get("w"), [-1.0, 0.0, -1.0, -2.0])
+
+
+def test_rigid_fire_two_steps_against_a_wall_closed_form():
+    """Two FIRE steps (fire.cuh:59-88, fire.cu:7-57) of one two-vertex body whose vertex 0
+    presses into the left wall.  Step 1 starts from v = 0, so P = 0 is uphill: dt halves,
+    positions move back by -dt/2 * v = 0, velocities are zeroed, kicked, mixed and zeroed
+    again, so the body does not move and ends with the closing half kick only.  Step 2 is
+    downhill: kick, mixing (v parallel to F, w parallel to tau: unchanged), drift by dt/2
+    (rotation of the vertex about the moved centre), forces, kick."""
+    R, rb, k, m, I, dt0, a0 = 0.5, 0.2, 2.0, 1.0, 0.5, 0.1, 0.1
+    th0, X, Y = 3 * math.pi / 4, 0.55, 5.0
+    b = bodies([X], [Y], [th0], R=R, rb=rb, k=k, inertia=[I], boundary=1)
+    o = orc.OracleRigid(b)
+    o.fire_init(dt0, mode=1)
+    rx, ry = R * math.cos(th0), R * math.sin(th0)
+    d0 = rb - (X + rx)
+    F0, t0 = k * d0, -ry * (k * d0)             # (r_v - R) x F with F = (k d0, 0)
+    np.testing.assert_allclose([o.get("fx")[0], o.get("torque")[0]], [F0, t0], rtol=1e-13)
+    hdt = dt0 / 2                               # step 1 halves dt (uphill)
+    o.fire_step(mode=1)
+    v1, w1 = F0 * hdt / 2 / m, t0 * hdt / 2 / I
+    np.testing.assert_allclose([o.get("x")[0], o.get("angle")[0]], [X, th0], rtol=1e-15)
+    np.testing.assert_allclose([o.get("vx")[0], o.get("w")[0]], [v1, w1], rtol=1e-13)
+    assert o.get("vy")[0] == 0.0
+    o.fire_step(mode=1)                         # downhill: dt and alpha unchanged (N_good = 1)
+    vk, wk = v1 + F0 * hdt / 2 / m, w1 + t0 * hdt / 2 / I
+    X2, th2 = X + vk * hdt / 2, th0 + wk * hdt / 2
+    dth = wk * hdt / 2
+    rx2 = math.cos(dth) * rx - math.sin(dth) * ry
+    ry2 = math.sin(dth) * rx + math.cos(dth) * ry
+    d2 = rb - (X2 + rx2)
+    assert d2 > 0
+    F2, t2 = k * d2, -ry2 * (k * d2)
+    v2, w2 = vk + F2 * hdt / 2 / m, wk + t2 * hdt / 2 / I
+    np.testing.assert_allclose([o.get("x")[0], o.get("angle")[0]], [X2, th2], rtol=1e-14)
+    np.testing.assert_allclose([o.get("fx")[0], o.get("torque")[0]], [F2, t2], rtol=1e-11)
+    np.testing.assert_allclose([o.get("vx")[0], o.get("w")[0]], [v2, w2], rtol=1e-12)
+    np.testing.assert_allclose(o.ke_total(), [0.5 * (m * v2 * v2 + I * w2 * w2)], rtol=1e-12)
+    np.testing.assert_allclose(o.fpower_total(), [F2 * v2 + t2 * w2], rtol=1e-11)
