@@ -145,6 +145,7 @@ void setup_cells(dpmd_ctx* c, int mode) {
     c->cell_start.resize(c->C + 1); c->cell_start.zero(c->st);
     c->cell_wptr.resize(std::max(1, c->C));
     c->scan_part.resize(std::max(1, (c->C + 2047) / 2048));
+    build_tiles(c, cdx, cdy);
 }
 
 void alloc_obj_buffers(dpmd_ctx* c) {
@@ -327,14 +328,14 @@ void fire_init_impl(dpmd_ctx* c, const double* dt, int mode) {
 // device-conditional rebuild chain, forces (+ FIRE bookkeeping).
 void fire_step_body(dpmd_ctx* c, int do_conv, int pred, double a, double b) {
     if (rigid(c)) rigid_fire_drift(c, plain(c)); else disk_fire_drift(c, plain(c));
-    if (c->method == DPMD_NEIGHBOR_CELL) rebuild_neighbors(c, active_one(c), false);
+    if (c->method == DPMD_NEIGHBOR_CELL) rebuild_neighbors(c, active_one(c), false, true);
     if (rigid(c)) rigid_fire_force(c, plain(c), 1, do_conv, pred, a, b);
     else disk_fire_force(c, plain(c), 1, do_conv, pred, a, b);
 }
 
 void vv_step_body(dpmd_ctx* c) {
     if (rigid(c)) rigid_vv_drift(c, plain(c)); else disk_vv_drift(c, plain(c));
-    if (c->method == DPMD_NEIGHBOR_CELL) rebuild_neighbors(c, active_one(c), false);
+    if (c->method == DPMD_NEIGHBOR_CELL) rebuild_neighbors(c, active_one(c), false, true);
     if (rigid(c)) rigid_vv_force(c, plain(c)); else disk_vv_force(c, plain(c));
 }
 
@@ -455,7 +456,13 @@ std::vector<int64_t> batch_key(dpmd_ctx* c, bool fire, int pred, double a, doubl
             P(c->rad.p), P(c->mass.p), P(c->lx.p), P(c->ly.p), P(c->nn.p), P(c->nbr.p), P(c->run.p), P(c->flag.p),
             P(c->vv_dt.p), P(c->vv_dth.p), P(c->f_dt.p), P(c->f_frozen.p), P(c->part.p), P(c->chunk_sys.p),
             P(c->static_index.p), P(c->angle.p), P(c->avel.p), P(c->torque.p), P(c->vpx.p), P(c->vpy.p),
-            P(c->vfx.p), P(c->vfy.p), P(c->vpe.p), P(c->sid.p), P(c->off.p), P(c->Lx.p), P(c->thresh2.p)};
+            P(c->vfx.p), P(c->vfy.p), P(c->vpe.p), P(c->sid.p), P(c->off.p), P(c->Lx.p), P(c->thresh2.p),
+            // the device-side rebuild chain's grid, buffers and engine
+            c->C, c->ntiles, c->rebuild_engine, P(c->cell_id.p), P(c->cell_count.p), P(c->cell_start.p),
+            P(c->cell_wptr.p), P(c->cell_sys.p), P(c->cstart.p), P(c->cdx.p), P(c->cdy.p), P(c->scan_part.p),
+            P(c->order.p), P(c->order_inv.p), P(c->tmp_d.p), P(c->tmp_i.p), P(c->tiles.p), P(c->overflow.p),
+            P(c->Ly.p), P(c->iLx.p), P(c->iLy.p), P(c->cutoff.p), P(c->vsid.p), P(c->voff.p), P(c->vpid.p),
+            P(c->poff.p), P(c->vrad.p), P(c->vmass.p), P(c->inertia.p), P(c->area.p)};
 }
 
 void drop_cached_graphs(dpmd_ctx* c) {
@@ -548,7 +555,7 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
                 if (fused)   // the halted step's positions are in the interleaved buffer P(halt_at mod 2)
                     disk_unpack_positions(c, r.halt_at % 2 == 0 ? c->xy0.p : c->xy1.p);
                 // finish the halted step: rebuild, forces, bookkeeping
-                rebuild_neighbors(c, active_one(c), false);
+                rebuild_neighbors(c, active_one(c), false, true);
                 if (fire) {
                     if (rigid(c)) rigid_fire_force(c, plain(c), 1, 1, pred, a, b);
                     else disk_fire_force(c, plain(c), 1, 1, pred, a, b);
@@ -1894,6 +1901,10 @@ int dpmd_set_option(dpmd_ctx* c, int key, int value) {
                 need(value == 0 || value == 1, "DPMD_OPT_PERSIST_LDS must be 0 or 1");
                 c->persist_lds = value;
                 break;
+            case DPMD_OPT_REBUILD_ENGINE:
+                need(value >= 0 && value <= 2, "DPMD_OPT_REBUILD_ENGINE must be 0, 1 or 2");
+                c->rebuild_engine = value;
+                break;
             default: throw ArgError("unknown option key");
         }
     });
@@ -1912,6 +1923,7 @@ int dpmd_get_option(dpmd_ctx* c, int key, int* value) {
             case DPMD_OPT_MAX_BATCH: *value = c->max_batch; break;
             case DPMD_OPT_DEVICE_REBUILD: *value = c->dev_rebuild; break;
             case DPMD_OPT_PERSIST_LDS: *value = c->persist_lds; break;
+            case DPMD_OPT_REBUILD_ENGINE: *value = c->rebuild_engine; break;
             default: throw ArgError("unknown option key");
         }
     });

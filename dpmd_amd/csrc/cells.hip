@@ -65,7 +65,7 @@ __device__ __forceinline__ void ph_top_scan(const RebuildArgs& r, int* lds) {
     }
 }
 
-// Block-level: cell_start for one tile; write pointers for flagged systems' cells.  Counts
+// Block-level: cell_start for one tile.  Counts
 // of systems that are not rebuilt are unchanged since their last rebuild, so their starts
 // are recomputed to the same values.
 __device__ __forceinline__ void ph_tile_down(const RebuildArgs& r, int t, int* lds) {
@@ -82,7 +82,6 @@ __device__ __forceinline__ void ph_tile_down(const RebuildArgs& r, int t, int* l
         const int c = base + k;
         if (c < C) {
             r.ca.cell_start[c] = run;
-            if (r.ca.flag[r.ca.cell_sys[c]]) r.ca.cell_wptr[c] = run;
             if (c == C - 1) r.ca.cell_start[C] = run + v[k];
         }
         run += v[k];
@@ -97,7 +96,7 @@ __device__ __forceinline__ void ph_rigid_reset(const RebuildArgs& r, int i) {
     r.plx[i] = r.o.x[fv]; r.ply[i] = r.o.y[fv];
 }
 
-// ---- multi-launch form ----------------------------------------------------------------
+// ---- device-wide chain ---------------------------------------------------------------------
 #define DPMD_EACH(name, count, body)                                                          \
     __global__ void name(RebuildArgs r, const int* active) {                                  \
         const int i = blockIdx.x * blockDim.x + threadIdx.x;                                  \
@@ -105,63 +104,352 @@ __device__ __forceinline__ void ph_rigid_reset(const RebuildArgs& r, int i) {
         body(r, i);                                                                           \
     }
 DPMD_EACH(k_zero, r.ca.C, ph_zero)
-DPMD_EACH(k_hash, r.o.n, ph_hash)
-DPMD_EACH(k_scatter, r.o.n, ph_scatter)
 DPMD_EACH(k_sort, r.ca.C, ph_sort)
 DPMD_EACH(k_gather, r.o.n, ph_gather)
 DPMD_EACH(k_copy_back, r.o.n, ph_copy_back)
 DPMD_EACH(k_rigid_reset, r.N, ph_rigid_reset)
 #undef DPMD_EACH
 
-// ---- neighbor build, output staged in LDS -----------------------------------------------
-// Each object appends its neighbors to its own column of an LDS table nbr_l[k][thread]; the
-// workgroup then writes rows 0 .. (its largest count - 1) of the column-major ELL table as
-// whole 1 KB rows.  Written directly, each lane stores its k-th neighbor at a different time
-// of its divergent stencil loop, and rows become scattered partial cache lines (about 4x the
-// payload in HBM writes).  Objects of systems that are not rebuilt keep their lists.
-__global__ void __launch_bounds__(DPMD_BLOCK) k_neighbors(RebuildArgs r, const int* active) {
-    extern __shared__ int nbr_l[];   // K rows x DPMD_BLOCK
-    __shared__ int wmax[DPMD_BLOCK / 64];
-    if (!*active) return;
-    const ObjArgs& o = r.o; const CellArgs& ca = r.ca; const NbrArgs& nb = r.nb;
-    const int i = blockIdx.x * DPMD_BLOCK + threadIdx.x;
-    const int K = nb.K;
-    int cnt = 0;
-    bool rebuilt = false;
-    if (i < o.n) {
-        const int s = o.sid[i];
-        rebuilt = ca.flag[s] != 0;
-        if (rebuilt) {
-            cnt = neighbor_scan(r, i, s, [&](int k, int j) {
-                if (k < K) nbr_l[k * DPMD_BLOCK + threadIdx.x] = j;
-            });
-            if (cnt > K) atomicMax(nb.overflow, cnt);
-            nb.nn[i] = cnt < K ? cnt : K;
-        }
-    }
-    // largest stored count of the workgroup: the rows to write.  (No global maximum is kept
-    // here: one same-address atomic per wave serialises the whole launch.)
-    int m = cnt;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
-    __syncthreads();
-    int rows = 0;
-#pragma unroll
-    for (int w = 0; w < DPMD_BLOCK / 64; ++w) rows = max(rows, wmax[w]);
-    rows = min(rows, K);
-    const int mine = min(cnt, K);
-    if (rebuilt)
-        for (int k = 0; k < rows; ++k)
-            nb.nbr[(size_t)k * o.n + i] = k < mine ? nbr_l[k * DPMD_BLOCK + threadIdx.x] : -1;
+// Cell id and the object's arrival rank in its cell (the value its count atomic returned);
+// the rank goes to order_inv, which the gather overwrites later.
+__global__ void k_hash_rank(RebuildArgs r, const int* active) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r.o.n || !*active) return;
+    const int s = r.o.sid[i];
+    if (!r.ca.flag[s]) return;
+    const int cell = r.ca.cstart[s] + hash_local(r.o, r.ca, i, s);
+    r.ca.cell_id[i] = cell;
+    r.order_inv[i] = atomicAdd(&r.ca.cell_count[cell], 1);
 }
 
-// Capacities above kLdsCapacity: write the ELL table directly.
-constexpr int kLdsCapacity = 64;
-__global__ void __launch_bounds__(DPMD_BLOCK) k_neighbors_direct(RebuildArgs r, const int* active) {
-    if (!*active) return;
+// Scatter by start + rank: no second atomic.  The rank order is arbitrary; k_sort then puts
+// each cell in ascending current index.
+__global__ void k_scatter_rank(RebuildArgs r, const int* active) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < r.o.n) ph_neighbors(r, i);
+    if (i >= r.o.n || !*active) return;
+    if (!r.ca.flag[r.o.sid[i]]) return;
+    r.order[r.ca.cell_start[r.ca.cell_id[i]] + r.order_inv[i]] = i;
+}
+
+// ---- tiled neighbor build (device-wide chain) ----------------------------------------------
+// One workgroup per row tile: the cells [cx0, cx1) of one cell row of one system.  The tile's
+// objects and the objects of the cells around it (rows cy - 1, cy, cy + 1, cells cx0 - 1 ..
+// cx1, periodic) are contiguous object ranges in cell order (at most three per row), read
+// from the gathered arrays with coalesced loads into LDS; every stencil candidate is then an
+// LDS read.  Objects are visited in the reference's stencil order (rows dy = -1, 0, 1, cells
+// dx = -1, 0, 1 within a row, objects of a cell in cell order), as neighbor_scan does.  The
+// workgroup also copies its own objects' gathered state back (the gather wrote it to the
+// scratch arrays), updates static_index over the same index range and resets the
+// displacement reference, so no separate copy-back pass runs.  A tile whose surroundings
+// hold more than kTileCap objects reads its candidates from global memory instead.
+constexpr int kTileThreads = 256;
+constexpr int kTileCap = 1024;
+constexpr int kTileMaxCells = 256;          // cells of one row segment
+constexpr int kHalo = kTileMaxCells + 2;    // cells of one halo row
+constexpr int kTileStageMaxK = 64;          // capacities up to this stage the lists in LDS
+
+template <bool RIGID, bool STAGE>
+__global__ void __launch_bounds__(kTileThreads) k_neighbors_tiled(RebuildArgs r, const int* active) {
+    extern __shared__ int nbr_l[];                     // STAGE: K rows x kTileThreads
+    __shared__ double hx[kTileCap], hy[kTileCap], hr[kTileCap];
+    __shared__ int hj[kTileCap];
+    __shared__ int hp[RIGID ? kTileCap : 1];
+    __shared__ int hbs[3][kHalo], hbe[3][kHalo];        // LDS slots [hbs, hbe) of each halo cell
+    __shared__ int rg0[9], rln[9], rbase[10];           // runs: global start, length, LDS base
+    __shared__ int wmax[kTileThreads / 64];
+    if (!*active) return;
+    const int* tl = r.tiles + 4 * blockIdx.x;
+    const int s = tl[0], cy = tl[1], cx0 = tl[2], cx1 = tl[3];
+    if (!r.ca.flag[s]) return;
+    const int t = threadIdx.x;
+    const int n = r.o.n, K = r.nb.K;
+    const int nx = r.ca.cdx[s], ny = r.ca.cdy[s], cs = r.ca.cstart[s];
+    const int* cst = r.ca.cell_start;
+    const double* gx = r.pa.tmp_d;
+    const double* gy = r.pa.tmp_d + (size_t)n;
+    const double* gr = r.pa.tmp_d + (size_t)r.ird * n;
+    const int* gcell = r.pa.tmp_i;
+    const int* gpid = RIGID ? r.pa.tmp_i + (size_t)n : nullptr;
+    const int w = cx1 - cx0 + 2;                        // halo cells per row
+    // runs of each halo row: [nx - 1] if cx0 == 0; cells max(cx0 - 1, 0) .. min(cx1, nx - 1);
+    // [0] if cx1 == nx
+    if (t < 3) {
+        int yy = cy + t - 1; if (yy < 0) yy += ny; else if (yy >= ny) yy -= ny;
+        const int row = cs + yy * nx;
+        const int a = cx0 > 0 ? cx0 - 1 : 0, bb = cx1 < nx ? cx1 : nx - 1;
+        int q = 3 * t;
+        rg0[q] = cst[row + nx - 1]; rln[q] = cx0 == 0 ? cst[row + nx] - cst[row + nx - 1] : 0; ++q;
+        rg0[q] = cst[row + a]; rln[q] = cst[row + bb + 1] - cst[row + a]; ++q;
+        rg0[q] = cst[row]; rln[q] = cx1 == nx ? cst[row + 1] - cst[row] : 0;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int q = 0; q < 9; ++q) { rbase[q] = acc; acc += rln[q]; }
+        rbase[9] = acc;
+    }
+    __syncthreads();
+    const bool staged = rbase[9] <= kTileCap;
+    // halo cells: LDS slots (staged) or global object ranges
+    for (int h = t; h < 3 * w; h += kTileThreads) {
+        const int q = h / w, u = h % w;                 // row q, halo position u <-> cell cx0 - 1 + u
+        int yy = cy + q - 1; if (yy < 0) yy += ny; else if (yy >= ny) yy -= ny;
+        int x = cx0 - 1 + u;
+        int run = 3 * q + 1;
+        if (x < 0) { x += nx; run = 3 * q; } else if (x >= nx) { x -= nx; run = 3 * q + 2; }
+        const int cell = cs + yy * nx + x;
+        const int g0 = cst[cell], g1 = cst[cell + 1];
+        if (staged) { hbs[q][u] = rbase[run] + (g0 - rg0[run]); hbe[q][u] = rbase[run] + (g1 - rg0[run]); }
+        else { hbs[q][u] = g0; hbe[q][u] = g1; }
+    }
+    if (staged) {
+        for (int q = 0; q < 9; ++q) {
+            const int g0 = rg0[q], len = rln[q], base = rbase[q];
+            for (int k = t; k < len; k += kTileThreads) {
+                const int j = g0 + k;
+                hx[base + k] = gx[j]; hy[base + k] = gy[j]; hr[base + k] = gr[j]; hj[base + k] = j;
+                if (RIGID) hp[base + k] = gpid[j];
+            }
+        }
+    }
+    __syncthreads();
+    const double Lx = r.ca.Lx[s], Ly = r.ca.Ly[s], iLx = r.ca.iLx[s], iLy = r.ca.iLy[s], cut0 = r.ca.cutoff[s];
+    const int o0 = cst[cs + cy * nx + cx0], o1 = cst[cs + cy * nx + cx1];
+    for (int base = o0; base < o1; base += kTileThreads) {
+        const int i = base + t;
+        const bool mine = i < o1;
+        int cnt = 0;
+        if (mine) {
+            const double xi = gx[i], yi = gy[i], ri = gr[i];
+            const int pi = RIGID ? gpid[i] : -1;
+            const int ui = gcell[i] - cs - cy * nx - cx0 + 1;   // halo position of the own cell
+            int* col = r.nb.nbr + i;
+            for (int q = 0; q < 3; ++q) {
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int lo = hbs[q][ui + dx], hi = hbe[q][ui + dx];
+                    for (int m = lo; m < hi; ++m) {
+                        int j, pj;
+                        double xj, yj, rj;
+                        if (staged) { j = hj[m]; xj = hx[m]; yj = hy[m]; rj = hr[m]; pj = RIGID ? hp[m] : -2; }
+                        else { j = m; xj = gx[m]; yj = gy[m]; rj = gr[m]; pj = RIGID ? gpid[m] : -2; }
+                        if (j == i) continue;
+                        if (RIGID && pj == pi) continue;
+                        const double ddx = min_image(xi - xj, Lx, iLx);
+                        const double ddy = min_image(yi - yj, Ly, iLy);
+                        const double r2 = ddx * ddx + ddy * ddy;
+                        const double cut = ri + rj + cut0;
+                        if (r2 < cut * cut) {
+                            if (cnt < K) {
+                                if (STAGE) nbr_l[cnt * kTileThreads + t] = j;
+                                else col[(size_t)cnt * n] = j;
+                            }
+                            ++cnt;
+                        }
+                    }
+                }
+            }
+            if (cnt > K) atomicMax(r.nb.overflow, cnt);
+            r.nb.nn[i] = cnt < K ? cnt : K;
+            // copy back the gathered state of object i; static_index over the same range
+            for (int k = 0; k < r.pa.nd; ++k) r.pa.d[k][i] = r.pa.tmp_d[(size_t)k * n + i];
+            for (int k = 0; k < r.pa.ni; ++k) r.pa.i[k][i] = r.pa.tmp_i[(size_t)k * n + i];
+            r.static_index[i] = r.order_inv[r.static_index[i]];
+            if (r.lx) { r.lx[i] = xi; r.ly[i] = yi; }
+        }
+        if (STAGE) {
+            int m = cnt;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+            if ((t & 63) == 0) wmax[t >> 6] = m;
+            __syncthreads();
+            int rows = 0;
+#pragma unroll
+            for (int q = 0; q < kTileThreads / 64; ++q) rows = max(rows, wmax[q]);
+            rows = min(rows, K);
+            const int have = min(cnt, K);
+            if (mine)
+                for (int k = 0; k < rows; ++k)
+                    r.nb.nbr[(size_t)k * n + i] = k < have ? nbr_l[k * kTileThreads + t] : -1;
+            __syncthreads();
+        }
+    }
+}
+
+// ---- per-system workgroup rebuild ------------------------------------------------------------
+// One workgroup per system (grid = S); systems whose flag is clear leave at once, so a step
+// with nothing to rebuild costs one near-empty launch and a rebuild costs one workgroup's
+// work on the flagged systems only.  The workgroup runs its system's whole rebuild through
+// LDS: cell counts with LDS atomics, the scan of the system's cells, an LDS scatter followed
+// by a per-cell insertion sort (ascending current index: the same order as the device-wide
+// chain), the permutation of every live array staged through LDS, static_index and the
+// displacement reference, then the neighbor lists with the system's positions, radii and cell
+// starts in LDS (read from global memory when they do not fit).
+constexpr int kSysLdsBytes = 156 * 1024;
+constexpr int kSysThreads = 1024;
+
+template <int W>
+__device__ __forceinline__ int wg_scan_excl(int v, int* lds, int& total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += u;
+    }
+    if (lane == 63) lds[wv] = inc;
+    __syncthreads();
+    int pre = 0;
+    total = 0;
+#pragma unroll
+    for (int k = 0; k < W / 64; ++k) {
+        if (k < wv) pre += lds[k];
+        total += lds[k];
+    }
+    __syncthreads();
+    return pre + inc - v;
+}
+
+template <bool RIGID>
+__global__ void __launch_bounds__(kSysThreads) k_rebuild_sys(RebuildArgs r, const int* active) {
+    constexpr int W = kSysThreads;
+    __shared__ __align__(16) unsigned char lds_raw[kSysLdsBytes];
+    if (!*active) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *r.any = 0;
+    const int s = blockIdx.x;
+    if (!r.ca.flag[s]) return;
+    const int t = threadIdx.x;
+    const int b = r.o.off[s], e = r.o.off[s + 1], n = e - b;
+    const int cs = r.ca.cstart[s], nc = r.ca.cstart[s + 1] - cs;
+    int* scratch = (int*)lds_raw;                      // 16 words
+    int* ord = scratch + 16;                           // [n] new local position -> old local index
+    int* cid = ord + n;                                // [n] old object's local cell, then the inverse
+    unsigned char* rest = lds_raw + 64 + 8 * (size_t)n;
+    int* cnt = (int*)rest;                             // [nc]
+    double* stage = (double*)rest;                     // [n], after the sort
+    for (int c = t; c < nc; c += W) cnt[c] = 0;
+    __syncthreads();
+    for (int q = t; q < n; q += W) {
+        const int c = hash_local(r.o, r.ca, b + q, s);
+        cid[q] = c;
+        atomicAdd(&cnt[c], 1);
+    }
+    __syncthreads();
+    {
+        const int per = (nc + W - 1) / W;
+        const int c0 = min(t * per, nc), c1 = min(c0 + per, nc);
+        int sum = 0;
+        for (int c = c0; c < c1; ++c) sum += cnt[c];
+        int total;
+        int run = wg_scan_excl<W>(sum, scratch, total);
+        for (int c = c0; c < c1; ++c) {
+            const int k = cnt[c];
+            r.ca.cell_start[cs + c] = b + run;
+            r.ca.cell_count[cs + c] = k;
+            cnt[c] = run;
+            run += k;
+        }
+        if (s == r.S - 1 && t == 0) r.ca.cell_start[cs + nc] = e;
+    }
+    __syncthreads();
+    for (int q = t; q < n; q += W) ord[atomicAdd(&cnt[cid[q]], 1)] = q;
+    __syncthreads();
+    for (int c = t; c < nc; c += W) {                  // cnt[c] is now the end of cell c
+        const int lo = c == 0 ? 0 : cnt[c - 1], hi = cnt[c];
+        for (int k = lo + 1; k < hi; ++k) {
+            const int key = ord[k];
+            int m = k - 1;
+            while (m >= lo && ord[m] > key) { ord[m + 1] = ord[m]; --m; }
+            ord[m + 1] = key;
+        }
+    }
+    __syncthreads();
+    int* cell_id = r.pa.i[0];
+    for (int p = t; p < n; p += W) cell_id[b + p] = cs + cid[ord[p]];
+    __syncthreads();
+    for (int p = t; p < n; p += W) cid[ord[p]] = p;   // inverse permutation
+    __syncthreads();
+    for (int q = t; q < n; q += W) r.static_index[b + q] = b + cid[r.static_index[b + q] - b];
+    for (int k = 0; k < r.pa.nd; ++k) {
+        double* d = r.pa.d[k];
+        for (int p = t; p < n; p += W) stage[p] = d[b + ord[p]];
+        __syncthreads();
+        double* l = (k == 0) ? r.lx : (k == 1 ? r.ly : nullptr);
+        for (int p = t; p < n; p += W) {
+            d[b + p] = stage[p];
+            if (l) l[b + p] = stage[p];
+        }
+        __syncthreads();
+    }
+    for (int k = 1; k < r.pa.ni; ++k) {
+        int* d = r.pa.i[k];
+        int* st = (int*)stage;
+        for (int p = t; p < n; p += W) st[p] = d[b + ord[p]];
+        __syncthreads();
+        for (int p = t; p < n; p += W) d[b + p] = st[p];
+        __syncthreads();
+    }
+    // neighbor lists
+    double* xs = (double*)lds_raw;
+    double* ys = xs + n;
+    double* rs = ys + n;
+    int* st0 = (int*)(rs + n);
+    int* ps = st0 + nc + 1;
+    const bool in_lds = (size_t)n * (RIGID ? 28 : 24) + 4 * (size_t)(nc + 1) <= (size_t)kSysLdsBytes;
+    if (in_lds) {
+        for (int p = t; p < n; p += W) {
+            xs[p] = r.o.x[b + p]; ys[p] = r.o.y[b + p]; rs[p] = r.o.rad[b + p];
+            if (RIGID) ps[p] = r.o.pid[b + p];
+        }
+        for (int c = t; c <= nc; c += W) st0[c] = c == nc ? n : r.ca.cell_start[cs + c] - b;
+    }
+    __syncthreads();
+    if (in_lds) {
+        const int nx = r.ca.cdx[s], ny = r.ca.cdy[s];
+        const double Lx = r.ca.Lx[s], Ly = r.ca.Ly[s], iLx = r.ca.iLx[s], iLy = r.ca.iLy[s], cut0 = r.ca.cutoff[s];
+        const int K = r.nb.K, N = r.o.n;
+        for (int p = t; p < n; p += W) {
+            const int i = b + p;
+            const int loc = cell_id[i] - cs;
+            const int cxi = loc % nx, cyi = loc / nx;
+            const double xi = xs[p], yi = ys[p], ri = rs[p];
+            const int pi = RIGID ? ps[p] : -1;
+            int* col = r.nb.nbr + i;
+            int cntn = 0;
+            for (int dy = -1; dy <= 1; ++dy) {
+                int yy = cyi + dy; if (yy < 0) yy += ny; else if (yy >= ny) yy -= ny;
+                const int row = yy * nx;
+                int c0[2], c1[2], nr = 0;
+                if (cxi == 0) { c0[0] = row + nx - 1; c1[0] = row + nx; c0[1] = row; c1[1] = row + 2; nr = 2; }
+                else if (cxi == nx - 1) { c0[0] = row + nx - 2; c1[0] = row + nx; c0[1] = row; c1[1] = row + 1; nr = 2; }
+                else { c0[0] = row + cxi - 1; c1[0] = row + cxi + 2; nr = 1; }
+                for (int g = 0; g < nr; ++g) {
+                    const int lo = st0[c0[g]], hi = st0[c1[g]];
+                    for (int q = lo; q < hi; ++q) {
+                        if (q == p) continue;
+                        if (RIGID && ps[q] == pi) continue;
+                        const double ddx = min_image(xi - xs[q], Lx, iLx);
+                        const double ddy = min_image(yi - ys[q], Ly, iLy);
+                        const double r2 = ddx * ddx + ddy * ddy;
+                        const double cut = ri + rs[q] + cut0;
+                        if (r2 < cut * cut) {
+                            if (cntn < K) col[(size_t)cntn * N] = b + q;
+                            ++cntn;
+                        }
+                    }
+                }
+            }
+            if (cntn > K) atomicMax(r.nb.overflow, cntn);
+            r.nb.nn[i] = cntn < K ? cntn : K;
+        }
+    } else {
+        for (int i = b + t; i < e; i += W) ph_neighbors(r, i);
+    }
+    if (RIGID) {
+        __syncthreads();
+        for (int i = r.psys_off[s] + t; i < r.psys_off[s + 1]; i += W) ph_rigid_reset(r, i);
+    }
+    __syncthreads();
+    if (t == 0) r.ca.flag[s] = 0u;
 }
 
 __global__ void k_tile_sum(RebuildArgs r, const int* active) {
@@ -246,59 +534,117 @@ NbrArgs nbr_args(dpmd_ctx* c) {
     return nb;
 }
 
-PermArgs perm_args(dpmd_ctx* c) {
+// Arrays a rebuild permutes.  The live ones come first; inside a step (between the drift and
+// the force evaluation) the forces and energies are about to be overwritten, so only the
+// live arrays move: disks x, y, vx, vy, rad, mass, area (then fx, fy, pe); vertices vpx, vpy,
+// vmass, vrad (then vfx, vfy, vpe).  pa.i: cell_id, then the owning particle of a vertex.
+PermArgs perm_args(dpmd_ctx* c, bool in_step) {
     PermArgs pa{};
     if (c->kind == DPMD_RIGID_BUMPY) {
-        double* d[] = {c->vpx.p, c->vpy.p, c->vfx.p, c->vfy.p, c->vmass.p, c->vrad.p, c->vpe.p};
-        pa.nd = 7;
-        for (int k = 0; k < pa.nd; ++k) pa.d[k] = d[k];
-        pa.i[0] = c->vpid.p; pa.i[1] = c->cell_id.p; pa.ni = 2;
+        double* d[] = {c->vpx.p, c->vpy.p, c->vmass.p, c->vrad.p, c->vfx.p, c->vfy.p, c->vpe.p};
+        pa.nd = in_step ? 4 : 7;
+        for (int k = 0; k < 7; ++k) pa.d[k] = d[k];
+        pa.i[0] = c->cell_id.p; pa.i[1] = c->vpid.p; pa.ni = 2;
     } else {
-        double* d[] = {c->x.p, c->y.p, c->vx.p, c->vy.p, c->fx.p, c->fy.p, c->rad.p, c->mass.p, c->pe.p, c->area.p};
-        pa.nd = 10;
-        for (int k = 0; k < pa.nd; ++k) pa.d[k] = d[k];
+        double* d[] = {c->x.p, c->y.p, c->vx.p, c->vy.p, c->rad.p, c->mass.p, c->area.p, c->fx.p, c->fy.p, c->pe.p};
+        pa.nd = in_step ? 7 : 10;
+        for (int k = 0; k < 10; ++k) pa.d[k] = d[k];
         pa.i[0] = c->cell_id.p; pa.ni = 1;
     }
     pa.tmp_d = c->tmp_d.p; pa.tmp_i = c->tmp_i.p;
     return pa;
 }
 
-RebuildArgs make_rebuild_args(dpmd_ctx* c) {
+RebuildArgs make_rebuild_args(dpmd_ctx* c, bool in_step) {
     RebuildArgs r;
-    r.o = obj_args(c); r.ca = cell_args(c); r.pa = perm_args(c); r.nb = nbr_args(c);
+    r.o = obj_args(c); r.ca = cell_args(c); r.pa = perm_args(c, in_step); r.nb = nbr_args(c);
     r.order = c->order.p; r.order_inv = c->order_inv.p; r.static_index = c->static_index.p;
     const bool rigid = c->kind == DPMD_RIGID_BUMPY;
     r.lx = rigid ? nullptr : c->lx.p; r.ly = rigid ? nullptr : c->ly.p;
     r.N = c->N; r.psid = c->sid.p; r.poff = c->poff.p;
     r.plx = rigid ? c->lx.p : nullptr; r.ply = rigid ? c->ly.p : nullptr;
+    r.psys_off = c->off.p;
     r.tile_sum = c->scan_part.p; r.nt = (c->C + SCAN_TILE - 1) / SCAN_TILE; r.S = c->S;
     r.any = c->run.p + RUN_ANY;
+    r.ird = rigid ? 3 : 4;
+    r.tiles = c->tiles.p; r.ntiles = c->ntiles;
     return r;
 }
 
+// The device-wide chain: cell ids and ranks, the scan of every cell, the rank scatter, the
+// per-cell sort, the gather of the live arrays, then the tiled neighbor build (which also
+// copies the gathered state back).
 void rebuild_multi(dpmd_ctx* c, const RebuildArgs& r, const int* active) {
     hipStream_t st = c->st;
     const int n = r.o.n, C = r.ca.C;
     k_zero<<<blocks_for(C), DPMD_BLOCK, 0, st>>>(r, active);
-    k_hash<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
+    k_hash_rank<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     k_tile_sum<<<r.nt, DPMD_BLOCK, 0, st>>>(r, active);
     k_top_scan<<<1, DPMD_BLOCK, 0, st>>>(r, active);
     k_tile_down<<<r.nt, DPMD_BLOCK, 0, st>>>(r, active);
-    k_scatter<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
+    k_scatter_rank<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     k_sort<<<blocks_for(C), DPMD_BLOCK, 0, st>>>(r, active);
     k_gather<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
-    k_copy_back<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
-    if (c->K <= kLdsCapacity)
-        k_neighbors<<<blocks_for(n), DPMD_BLOCK, (size_t)c->K * DPMD_BLOCK * sizeof(int), st>>>(r, active);
-    else
-        k_neighbors_direct<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
+    const bool stage = c->K <= kTileStageMaxK;
+    const size_t dyn = stage ? (size_t)c->K * kTileThreads * sizeof(int) : 0;
+    need(r.ntiles > 0, "rebuild: no cell tiles (cells not set up)");
+    if (c->kind == DPMD_RIGID_BUMPY) {
+        if (stage) k_neighbors_tiled<true, true><<<r.ntiles, kTileThreads, dyn, st>>>(r, active);
+        else k_neighbors_tiled<true, false><<<r.ntiles, kTileThreads, 0, st>>>(r, active);
+    } else {
+        if (stage) k_neighbors_tiled<false, true><<<r.ntiles, kTileThreads, dyn, st>>>(r, active);
+        else k_neighbors_tiled<false, false><<<r.ntiles, kTileThreads, 0, st>>>(r, active);
+    }
     if (r.plx) k_rigid_reset<<<blocks_for(r.N), DPMD_BLOCK, 0, st>>>(r, active);
     k_clear_flags<<<blocks_for(r.S), DPMD_BLOCK, 0, st>>>(r.S, r.ca.flag, r.any, active);
 }
 
+// Whether every system fits the per-system workgroup rebuild's LDS: the cell counts and
+// the permutation stage after the two index arrays.
+bool sys_rebuild_fits(dpmd_ctx* c) {
+    const int64_t need = 64 + 8 * (int64_t)c->max_sys_obj + std::max<int64_t>(4 * (int64_t)c->max_sys_cells,
+                                                                                   8 * (int64_t)c->max_sys_obj);
+    return c->max_sys_obj > 0 && need <= kSysLdsBytes;
+}
+
+bool use_sys_rebuild(dpmd_ctx* c) {
+    if (c->rebuild_engine == 2) return false;
+    const bool fits = sys_rebuild_fits(c);
+    if (c->rebuild_engine == 1)
+        need(fits, "DPMD_OPT_REBUILD_ENGINE = 1: a system is too large for the per-system workgroup rebuild");
+    return fits;
+}
+
 }  // namespace
 
-RebuildArgs rebuild_args(dpmd_ctx* c) { return make_rebuild_args(c); }
+RebuildArgs rebuild_args(dpmd_ctx* c) { return make_rebuild_args(c, false); }
+
+void build_tiles(dpmd_ctx* c, const std::vector<int>& cdx, const std::vector<int>& cdy) {
+    const std::vector<int>& off = c->kind == DPMD_RIGID_BUMPY ? c->h_voff : c->h_off;
+    std::vector<int> tl;
+    int mo = 0, mc = 0;
+    for (int s = 0; s < c->S; ++s) {
+        const int n = off[s + 1] - off[s], nx = cdx[s], ny = cdy[s];
+        mo = std::max(mo, n);
+        mc = std::max(mc, nx * ny);
+        if (nx <= 0 || ny <= 0) continue;
+        // row segments of about 200 objects (one per thread of a tile workgroup)
+        const double rho = (double)n / ((double)nx * ny);
+        int tw = rho > 0.0 ? (int)(200.0 / rho) : nx;
+        tw = std::max(1, std::min(tw, std::min(nx, kTileMaxCells)));
+        const int nseg = (nx + tw - 1) / tw;
+        for (int cy = 0; cy < ny; ++cy)
+            for (int k = 0; k < nseg; ++k) {
+                tl.push_back(s); tl.push_back(cy);
+                tl.push_back((int)((int64_t)k * nx / nseg)); tl.push_back((int)((int64_t)(k + 1) * nx / nseg));
+            }
+    }
+    c->ntiles = (int)(tl.size() / 4);
+    if (tl.empty()) tl.assign(4, 0);
+    c->tiles.upload(tl.data(), (int64_t)tl.size(), c->st);
+    c->max_sys_obj = mo;
+    c->max_sys_cells = mc;
+}
 
 void build_chunks(dpmd_ctx* c) {
     std::vector<int> cs, cb, sc(c->S + 1, 0);
@@ -325,7 +671,7 @@ void permute_particles(dpmd_ctx* c, const int* order_host) {
     c->order.upload(order_host, n, st);
     const int* active = c->one.p;
     k_set_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, c->flag.p, active);
-    RebuildArgs r = make_rebuild_args(c);
+    RebuildArgs r = make_rebuild_args(c, false);
     k_gather<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     k_copy_back<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     k_clear_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, r.ca.flag, r.any, active);
@@ -335,15 +681,20 @@ void permute_particles(dpmd_ctx* c, const int* order_host) {
 void rebuild_on_device(dpmd_ctx* c) {
     k_rebuild_gate<<<1, 1, 0, c->st>>>(c->run.p);
     HIP_OK(hipGetLastError());
-    rebuild_neighbors(c, c->run.p + RUN_GATE, false);
+    rebuild_neighbors(c, c->run.p + RUN_GATE, false, true);
 }
 
-void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all) {
+void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all, bool in_step) {
     const int S = c->S;
     hipStream_t st = c->st;
     if (force_all) k_set_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, c->flag.p, active);
-    RebuildArgs r = make_rebuild_args(c);
-    rebuild_multi(c, r, active);
+    RebuildArgs r = make_rebuild_args(c, in_step);
+    if (use_sys_rebuild(c)) {
+        if (c->kind == DPMD_RIGID_BUMPY) k_rebuild_sys<true><<<S, kSysThreads, 0, st>>>(r, active);
+        else k_rebuild_sys<false><<<S, kSysThreads, 0, st>>>(r, active);
+    } else {
+        rebuild_multi(c, r, active);
+    }
     HIP_OK(hipGetLastError());
 }
 

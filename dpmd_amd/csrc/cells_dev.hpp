@@ -15,19 +15,24 @@ struct RebuildArgs {
     double* lx; double* ly;           // disks: displacement reference, reset per object
     // rigid bodies: per-particle reference = first vertex of each particle
     int N; const int* psid; const int* poff; double* plx; double* ply;
+    const int* psys_off;              // per-system particle offsets (S+1)
     int* tile_sum; int nt; int S;
     int* any;                          // the run word "some rebuild flag is set"
+    // Positions within the permuted arrays: pa.d[0], pa.d[1] are x, y; pa.d[ird] is the
+    // radius; pa.i[0] is cell_id and pa.i[1] (rigid bodies) the owning particle.
+    int ird;
+    // Row tiles of the cell grids for the tiled neighbor build: 4 ints per tile (system,
+    // cell row, first cell, one past the last cell of the row segment).
+    const int* tiles; int ntiles;
 };
 
 __device__ __forceinline__ void ph_zero(const RebuildArgs& r, int c) {
     if (r.ca.flag[r.ca.cell_sys[c]]) r.ca.cell_count[c] = 0;
 }
 
-// assign_cell_ids_kernel (base_particle_kernels.cu:205-233) + count_cells_kernel (:235-244)
-__device__ __forceinline__ void ph_hash(const RebuildArgs& r, int i) {
-    const ObjArgs& o = r.o; const CellArgs& ca = r.ca;
-    const int s = o.sid[i];
-    if (!ca.flag[s]) return;
+// assign_cell_ids_kernel (base_particle_kernels.cu:205-233): the cell of object i within its
+// system's grid (cx + cy * nx).
+__device__ __forceinline__ int hash_local(const ObjArgs& o, const CellArgs& ca, int i, int s) {
     const double u = wrap01(o.x[i] * ca.iLx[s]);
     const double v = wrap01(o.y[i] * ca.iLy[s]);
     const int nx = ca.cdx[s], ny = ca.cdy[s];
@@ -35,7 +40,15 @@ __device__ __forceinline__ void ph_hash(const RebuildArgs& r, int i) {
     int cy = (int)floor(v * ny);
     cx = cx < nx ? (cx > 0 ? cx : 0) : nx - 1;   // guard: u*nx may round up to nx
     cy = cy < ny ? (cy > 0 ? cy : 0) : ny - 1;
-    const int cell = ca.cstart[s] + cx + cy * nx;
+    return cx + cy * nx;
+}
+
+// assign_cell_ids_kernel + count_cells_kernel (base_particle_kernels.cu:205-244)
+__device__ __forceinline__ void ph_hash(const RebuildArgs& r, int i) {
+    const ObjArgs& o = r.o; const CellArgs& ca = r.ca;
+    const int s = o.sid[i];
+    if (!ca.flag[s]) return;
+    const int cell = ca.cstart[s] + hash_local(o, ca, i, s);
     ca.cell_id[i] = cell;
     atomicAdd(&ca.cell_count[cell], 1);
 }

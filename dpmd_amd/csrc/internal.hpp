@@ -130,6 +130,12 @@ struct dpmd_ctx {
     dpmd::DVec<int> tmp_i; dpmd::DVec<double> tmp_d;
     dpmd::DVec<int> nn, nbr, overflow;
     dpmd::DVec<int> scan_part;
+    dpmd::DVec<int> tiles;      // 4 ints per cell-row tile (system, row, first cell, end cell)
+    int ntiles = 0;
+    int max_sys_obj = 0, max_sys_cells = 0;   // largest system (objects, cells)
+    // rebuild engine: 0 per-system workgroups when every system fits, else the device-wide
+    // chain; 1 per-system workgroups; 2 device-wide chain (DPMD_OPT_REBUILD_ENGINE)
+    int rebuild_engine = 0;
 
     // ---- system-aligned 256-wide chunks over particles (deterministic reductions)
     int nchunks = 0;
@@ -286,7 +292,11 @@ int64_t tune_run(dpmd_ctx* c, int integrator, const double* dt, int mode, int st
 
 // cells.hip
 void build_chunks(dpmd_ctx* c);
-void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all);
+// in_step: the rebuild runs between a step's drift and its force evaluation, so forces and
+// energies are dead and are not permuted
+void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all, bool in_step = false);
+// cell-row tiles of the tiled neighbor build and the per-system extents (after setup_cells)
+void build_tiles(dpmd_ctx* c, const std::vector<int>& cdx, const std::vector<int>& cdy);
 // inside a step batch: rebuild the flagged systems iff some drift raised a flag this step
 void rebuild_on_device(dpmd_ctx* c);
 void permute_particles(dpmd_ctx* c, const int* order_host);

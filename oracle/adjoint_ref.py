@@ -232,3 +232,105 @@ def vv_backprop(xs, gx, gy, gvx, gvy, r, m, Lx, Ly, k, wall, dt):
         vb = vhb.copy()
         xb = xb - h * (Ho @ (vhb / mm))
     return xb[0::2], xb[1::2], vb[0::2], vb[1::2]
+
+
+# ---- sparse forms (the same functions over k-d-tree contact lists, for large systems) ----
+def contacts_sparse(x, y, r, Lx, Ly, wall):
+    """contacts() with candidate pairs from a k-d tree (periodic unless walled): the same
+    contacts, in the same (i < j, lexicographic) order."""
+    from scipy.spatial import cKDTree
+    if wall:
+        tree = cKDTree(np.stack([x, y], 1))
+    else:
+        u = x - Lx * np.floor(x / Lx); u[u >= Lx] = 0.0
+        v = y - Ly * np.floor(y / Ly); v[v >= Ly] = 0.0
+        tree = cKDTree(np.stack([u, v], 1), boxsize=[Lx, Ly])
+    P = tree.query_pairs(2.0 * float(np.max(r)) * (1 + 1e-12), output_type="ndarray")
+    if len(P) == 0:
+        e = np.zeros(0, int)
+        return e, e, np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0)
+    I, J = np.minimum(P[:, 0], P[:, 1]), np.maximum(P[:, 0], P[:, 1])
+    o = np.lexsort((J, I))
+    I, J = I[o], J[o]
+    qx = x[I] - x[J]
+    qy = y[I] - y[J]
+    if not wall:
+        qx = qx - Lx * np.rint(qx * (1.0 / Lx))
+        qy = qy - Ly * np.rint(qy * (1.0 / Ly))
+    d2 = qx * qx + qy * qy
+    rs = r[I] + r[J]
+    on = d2 < rs * rs
+    I, J, qx, qy, d2, rs = I[on], J[on], qx[on], qy[on], d2[on], rs[on]
+    d = np.sqrt(d2)
+    return I, J, qx, qy, d, rs - d
+
+
+def forces_sparse(x, y, r, Lx, Ly, k, wall):
+    n = len(x)
+    I, J, qx, qy, d, de = contacts_sparse(x, y, r, Lx, Ly, wall)
+    fx, fy = np.zeros(n), np.zeros(n)
+    np.add.at(fx, I, k * de * qx / d)
+    np.add.at(fy, I, k * de * qy / d)
+    np.add.at(fx, J, -k * de * qx / d)
+    np.add.at(fy, J, -k * de * qy / d)
+    if wall:
+        olx, ohx, oly, ohy = wall_gaps(x, y, r, Lx, Ly)
+        fx += k * olx - k * ohx
+        fy += k * oly - k * ohy
+    return fx, fy
+
+
+def hvp_sparse(x, y, r, Lx, Ly, k, wall, w):
+    """H w for an interleaved vector w (x0, y0, x1, y1, ...), block by block as hessian()."""
+    n = len(x)
+    I, J, qx, qy, d, de = contacts_sparse(x, y, r, Lx, Ly, wall)
+    ux, uy = qx / d, qy / d
+    c, t = k, -(k * de / d)
+    wx, wy = w[0::2], w[1::2]
+    dwx, dwy = wx[I] - wx[J], wy[I] - wy[J]
+    # K = c n n^T + t (I - n n^T)
+    un = ux * dwx + uy * dwy
+    kx = c * ux * un + t * (dwx - ux * un)
+    ky = c * uy * un + t * (dwy - uy * un)
+    out = np.zeros(2 * n)
+    ox, oy = np.zeros(n), np.zeros(n)
+    np.add.at(ox, I, kx); np.add.at(oy, I, ky)
+    np.add.at(ox, J, -kx); np.add.at(oy, J, -ky)
+    if wall:
+        olx, ohx, oly, ohy = wall_gaps(x, y, r, Lx, Ly)
+        ox += k * ((olx > 0).astype(float) + (ohx > 0)) * wx
+        oy += k * ((oly > 0).astype(float) + (ohy > 0)) * wy
+    out[0::2], out[1::2] = ox, oy
+    return out
+
+
+def vv_trajectory_sparse(x, y, vx, vy, r, m, Lx, Ly, k, wall, dt, n):
+    """vv_trajectory() with forces_sparse."""
+    xs = [np.stack([x, y]).copy()]
+    fx, fy = forces_sparse(x, y, r, Lx, Ly, k, wall)
+    x, y, vx, vy = x.copy(), y.copy(), vx.copy(), vy.copy()
+    h = dt * 0.5
+    for _ in range(n):
+        vx = vx + fx * (h / m); vy = vy + fy * (h / m)
+        x = x + vx * dt; y = y + vy * dt
+        fx, fy = forces_sparse(x, y, r, Lx, Ly, k, wall)
+        vx = vx + fx * (h / m); vy = vy + fy * (h / m)
+        xs.append(np.stack([x, y]).copy())
+    return xs, (x, y, vx, vy)
+
+
+def vv_backprop_sparse(xs, gx, gy, gvx, gvy, r, m, Lx, Ly, k, wall, dt):
+    """vv_backprop() with hvp_sparse in place of the dense Hessians."""
+    n = len(xs) - 1
+    xb = np.empty(2 * len(r)); xb[0::2], xb[1::2] = gx, gy
+    vb = np.empty(2 * len(r)); vb[0::2], vb[1::2] = gvx, gvy
+    mm = np.repeat(m, 2)
+    h = dt * 0.5
+    for s in range(n - 1, -1, -1):
+        vhb = vb.copy()
+        xpb = xb - h * hvp_sparse(xs[s + 1][0], xs[s + 1][1], r, Lx, Ly, k, wall, vb / mm)
+        xb = xpb.copy()
+        vhb = vhb + dt * xpb
+        vb = vhb.copy()
+        xb = xb - h * hvp_sparse(xs[s][0], xs[s][1], r, Lx, Ly, k, wall, vhb / mm)
+    return xb[0::2], xb[1::2], vb[0::2], vb[1::2]

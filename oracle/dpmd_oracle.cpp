@@ -188,7 +188,10 @@ struct Disk {
             cell_id[b + p] = cstart[s] + cid[order[p]];
             lastx[b + p] = x[b + p]; lasty[b + p] = y[b + p];
         }
-        // neighbor fill -- base_point_particle_kernels.cu:98-157
+        // neighbor fill -- base_point_particle_kernels.cu:98-157 (each object's list is
+        // independent: parallel over objects)
+        int mc = max_count;
+#pragma omp parallel for schedule(static) reduction(max : mc)
         for (int p = 0; p < n; ++p) {
             const int i = b + p;
             std::vector<int>& L = nbr[i];
@@ -210,8 +213,9 @@ struct Disk {
                     }
                 }
             }
-            max_count = std::max(max_count, (int)L.size());
+            mc = std::max(mc, (int)L.size());
         }
+        max_count = mc;
         flag[s] = 0u;
     }
 
@@ -678,7 +682,8 @@ struct Rigid {
     }
     // vertex pipeline: base_poly_particle.hpp:239-260, base_poly_particle_kernels.cu:78-195,
     // reorder rigid_bumpy.cu:1471-1492, reset_displacements_kernel :1070-1085
-    void rebuild_system(int s) {
+    int rebuild_system(int s) {   // returns the system's largest list
+        int mcount = 0;
         const int b = voff[s], en = voff[s + 1], n = en - b;
         const int C = nx[s] * ny[s];
         std::vector<int> cid(n), cnt(C + 1, 0), order(n), inv(n);
@@ -687,10 +692,10 @@ struct Rigid {
         std::vector<int> cs(cnt.begin(), cnt.end()), wp(cnt.begin(), cnt.end() - 1);
         for (int t = 0; t < n; ++t) order[wp[cid[t]]++] = t;
         for (int p = 0; p < n; ++p) inv[order[p]] = p;
-        auto gather = [&](auto& a) {
-            auto tmp = a;
-            for (int p = 0; p < n; ++p) tmp[b + p] = a[b + order[p]];
-            a = tmp;
+        auto gather = [&](auto& a) {   // this system's slice only (systems rebuild in parallel)
+            std::vector<typename std::decay_t<decltype(a)>::value_type> tmp(n);
+            for (int p = 0; p < n; ++p) tmp[p] = a[b + order[p]];
+            for (int p = 0; p < n; ++p) a[b + p] = tmp[p];
         };
         gather(vpx); gather(vpy); gather(vfx); gather(vfy); gather(vmass); gather(vrad); gather(vpe); gather(vpid);
         for (int c = b; c < en; ++c) static_index[c] = b + inv[static_index[c] - b];
@@ -717,15 +722,24 @@ struct Rigid {
                     }
                 }
             }
-            max_count = std::max(max_count, (int)L.size());
+            mcount = std::max(mcount, (int)L.size());
         }
         for (int i = off[s]; i < off[s + 1]; ++i) {
             const int fv = static_index[poff[i]];
             lastx[i] = vpx[fv]; lasty[i] = vpy[fv];
         }
         flag[s] = 0u;
+        return mcount;
     }
-    void check_neighbors(bool force) { for (int s = 0; s < S; ++s) if (force || flag[s]) rebuild_system(s); }
+    void check_neighbors(bool force) {
+        // systems are independent; max_count is combined after the parallel loop
+        std::vector<int> mc(S, 0);
+        const int m0 = max_count;
+#pragma omp parallel for schedule(dynamic)
+        for (int s = 0; s < S; ++s) if (force || flag[s]) mc[s] = rebuild_system(s);
+        max_count = m0;
+        for (int s = 0; s < S; ++s) max_count = std::max(max_count, mc[s]);
+    }
 
     // rigid_bumpy.cu:164-221 (PBC) / :224-307 (walls; pair term repulsive xi - xj)
     void vertex_force(int i, bool wall) {
@@ -775,7 +789,9 @@ struct Rigid {
         fx[i] = fxi; fy[i] = fyi; tq[i] = t; pe[i] = p;
     }
     void compute_forces(int mode) {
+#pragma omp parallel for schedule(static)
         for (int v = 0; v < Nv; ++v) vertex_force(v, mode == 1 || (mode == 2 && boundary[vsid[v]] != 0));
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < N; ++i) particle_force(i);
     }
     // update_positions_kernel_naive/_cell (rigid_bumpy.cu:20-127)

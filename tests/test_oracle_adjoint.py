@@ -195,3 +195,34 @@ def test_vv_backprop_matches_fd():
         e = np.zeros(b.N); e[i] = h
         assert (msd(vx0 + e, vy0) - msd(vx0 - e, vy0)) / (2 * h) == pytest.approx(gvx[i], rel=1e-6, abs=1e-12)
         assert (msd(vx0, vy0 + e) - msd(vx0, vy0 - e)) / (2 * h) == pytest.approx(gvy[i], rel=1e-6, abs=1e-12)
+
+
+@pytest.mark.parametrize("wall", [False, True])
+def test_sparse_forms_equal_dense(wall):
+    """The k-d-tree forms used at full size: the same contact set and order, forces, Hessian
+    products, trajectory and reverse mode as the dense restatement (pinned above)."""
+    b = one_system(90, 0.88, 4, wall)
+    x, y, r, m = b.x, b.y, b.rad, b.mass
+    L_ = b.Lx[0]
+    D = R.contacts(x, y, r, L_, L_, wall)
+    Sp = R.contacts_sparse(x, y, r, L_, L_, wall)
+    for a, c in zip(D, Sp):
+        assert np.array_equal(a, c)
+    fd, fs = R.forces(x, y, r, L_, L_, 1.0, wall), R.forces_sparse(x, y, r, L_, L_, 1.0, wall)
+    assert np.array_equal(fd[0], fs[0]) and np.array_equal(fd[1], fs[1])
+    rng = np.random.default_rng(3)
+    w = rng.standard_normal(2 * b.N)
+    H = R.hessian(x, y, r, L_, L_, 1.0, wall)
+    np.testing.assert_allclose(R.hvp_sparse(x, y, r, L_, L_, 1.0, wall, w), H @ w, rtol=0,
+                               atol=1e-13 * np.max(np.abs(H @ w)))
+    vx, vy = rng.standard_normal(b.N) * 0.05, rng.standard_normal(b.N) * 0.05
+    xs_d, fin_d = R.vv_trajectory(x, y, vx, vy, r, m, L_, L_, 1.0, wall, 1e-2, 30)
+    xs_s, fin_s = R.vv_trajectory_sparse(x, y, vx, vy, r, m, L_, L_, 1.0, wall, 1e-2, 30)
+    for a, c in zip(fin_d, fin_s):
+        assert np.array_equal(a, c)
+    gx, gy = rng.standard_normal(b.N), rng.standard_normal(b.N)
+    z = np.zeros(b.N)
+    bd = R.vv_backprop(xs_d, gx, gy, z, z, r, m, L_, L_, 1.0, wall, 1e-2)
+    bs = R.vv_backprop_sparse(xs_s, gx, gy, z, z, r, m, L_, L_, 1.0, wall, 1e-2)
+    for a, c in zip(bd, bs):
+        np.testing.assert_allclose(c, a, rtol=0, atol=1e-12 * np.max(np.abs(a)))

@@ -288,3 +288,18 @@ def test_rigid_fire_two_steps_against_a_wall_closed_form():
     np.testing.assert_allclose([o.get("vx")[0], o.get("w")[0]], [v2, w2], rtol=1e-12)
     np.testing.assert_allclose(o.ke_total(), [0.5 * (m * v2 * v2 + I * w2 * w2)], rtol=1e-12)
     np.testing.assert_allclose(o.fpower_total(), [F2 * v2 + t2 * w2], rtol=1e-11)
+
+
+def test_oracle_results_do_not_depend_on_thread_count():
+    """Systems are rebuilt in parallel and forces are evaluated in parallel: the same bits
+    with one thread and with several (each system's rebuild touches only its own slices)."""
+    b = W.rigid_batch(S=6, n_per=24, seed=3, boundary=1, phi=0.70)
+    out = []
+    for th in (1, 4):
+        orc.lib().or_set_threads(th)
+        o = orc.OracleRigid(b)
+        o.fire_minimize(1e-2, 60, pred=1, ftol=0.0, mode=1)
+        out.append([o.get(f) for f in ("x", "angle", "vpx", "vfx", "fx")] + [o.static_index()])
+    orc.lib().or_set_threads(1)
+    for a, c in zip(*out):
+        assert np.array_equal(a, c)
