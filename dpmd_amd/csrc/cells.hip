@@ -148,145 +148,6 @@ constexpr int kTileMaxCells = 256;          // cells of one row segment
 constexpr int kHalo = kTileMaxCells + 2;    // cells of one halo row
 constexpr int kTileStageMaxK = 64;          // capacities up to this stage the lists in LDS
 
-template <bool RIGID, bool STAGE>
-__global__ void __launch_bounds__(kTileThreads) k_neighbors_tiled(RebuildArgs r, const int* active) {
-    extern __shared__ int nbr_l[];                     // STAGE: K rows x kTileThreads
-    __shared__ double hx[kTileCap], hy[kTileCap], hr[kTileCap];
-    __shared__ int hj[kTileCap];
-    __shared__ int hp[RIGID ? kTileCap : 1];
-    __shared__ int hbs[3][kHalo], hbe[3][kHalo];        // LDS slots [hbs, hbe) of each halo cell
-    __shared__ int rg0[9], rln[9], rbase[10];           // runs: global start, length, LDS base
-    __shared__ int wmax[kTileThreads / 64];
-    if (!*active) return;
-    const int* tl = r.tiles + 4 * blockIdx.x;
-    const int s = tl[0], cy = tl[1], cx0 = tl[2], cx1 = tl[3];
-    if (!r.ca.flag[s]) return;
-    const int t = threadIdx.x;
-    const int n = r.o.n, K = r.nb.K;
-    const int nx = r.ca.cdx[s], ny = r.ca.cdy[s], cs = r.ca.cstart[s];
-    const int* cst = r.ca.cell_start;
-    const double* gx = r.pa.tmp_d;
-    const double* gy = r.pa.tmp_d + (size_t)n;
-    const double* gr = r.pa.tmp_d + (size_t)r.ird * n;
-    const int* gcell = r.pa.tmp_i;
-    const int* gpid = RIGID ? r.pa.tmp_i + (size_t)n : nullptr;
-    const int w = cx1 - cx0 + 2;                        // halo cells per row
-    // runs of each halo row: [nx - 1] if cx0 == 0; cells max(cx0 - 1, 0) .. min(cx1, nx - 1);
-    // [0] if cx1 == nx
-    if (t < 3) {
-        int yy = cy + t - 1; if (yy < 0) yy += ny; else if (yy >= ny) yy -= ny;
-        const int row = cs + yy * nx;
-        const int a = cx0 > 0 ? cx0 - 1 : 0, bb = cx1 < nx ? cx1 : nx - 1;
-        int q = 3 * t;
-        rg0[q] = cst[row + nx - 1]; rln[q] = cx0 == 0 ? cst[row + nx] - cst[row + nx - 1] : 0; ++q;
-        rg0[q] = cst[row + a]; rln[q] = cst[row + bb + 1] - cst[row + a]; ++q;
-        rg0[q] = cst[row]; rln[q] = cx1 == nx ? cst[row + 1] - cst[row] : 0;
-    }
-    __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int q = 0; q < 9; ++q) { rbase[q] = acc; acc += rln[q]; }
-        rbase[9] = acc;
-    }
-    __syncthreads();
-    const bool staged = rbase[9] <= kTileCap;
-    // halo cells: LDS slots (staged) or global object ranges
-    for (int h = t; h < 3 * w; h += kTileThreads) {
-        const int q = h / w, u = h % w;                 // row q, halo position u <-> cell cx0 - 1 + u
-        int yy = cy + q - 1; if (yy < 0) yy += ny; else if (yy >= ny) yy -= ny;
-        int x = cx0 - 1 + u;
-        int run = 3 * q + 1;
-        if (x < 0) { x += nx; run = 3 * q; } else if (x >= nx) { x -= nx; run = 3 * q + 2; }
-        const int cell = cs + yy * nx + x;
-        const int g0 = cst[cell], g1 = cst[cell + 1];
-        if (staged) { hbs[q][u] = rbase[run] + (g0 - rg0[run]); hbe[q][u] = rbase[run] + (g1 - rg0[run]); }
-        else { hbs[q][u] = g0; hbe[q][u] = g1; }
-    }
-    if (staged) {
-        for (int q = 0; q < 9; ++q) {
-            const int g0 = rg0[q], len = rln[q], base = rbase[q];
-            for (int k = t; k < len; k += kTileThreads) {
-                const int j = g0 + k;
-                hx[base + k] = gx[j]; hy[base + k] = gy[j]; hr[base + k] = gr[j]; hj[base + k] = j;
-                if (RIGID) hp[base + k] = gpid[j];
-            }
-        }
-    }
-    __syncthreads();
-    const double Lx = r.ca.Lx[s], Ly = r.ca.Ly[s], iLx = r.ca.iLx[s], iLy = r.ca.iLy[s], cut0 = r.ca.cutoff[s];
-    const int o0 = cst[cs + cy * nx + cx0], o1 = cst[cs + cy * nx + cx1];
-    for (int base = o0; base < o1; base += kTileThreads) {
-        const int i = base + t;
-        const bool mine = i < o1;
-        int cnt = 0;
-        if (mine) {
-            const double xi = gx[i], yi = gy[i], ri = gr[i];
-            const int pi = RIGID ? gpid[i] : -1;
-            const int ui = gcell[i] - cs - cy * nx - cx0 + 1;   // halo position of the own cell
-            int* col = r.nb.nbr + i;
-            for (int q = 0; q < 3; ++q) {
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int lo = hbs[q][ui + dx], hi = hbe[q][ui + dx];
-                    for (int m = lo; m < hi; ++m) {
-                        int j, pj;
-                        double xj, yj, rj;
-                        if (staged) { j = hj[m]; xj = hx[m]; yj = hy[m]; rj = hr[m]; pj = RIGID ? hp[m] : -2; }
-                        else { j = m; xj = gx[m]; yj = gy[m]; rj = gr[m]; pj = RIGID ? gpid[m] : -2; }
-                        if (j == i) continue;
-                        if (RIGID && pj == pi) continue;
-                        const double ddx = min_image(xi - xj, Lx, iLx);
-                        const double ddy = min_image(yi - yj, Ly, iLy);
-                        const double r2 = ddx * ddx + ddy * ddy;
-                        const double cut = ri + rj + cut0;
-                        if (r2 < cut * cut) {
-                            if (cnt < K) {
-                                if (STAGE) nbr_l[cnt * kTileThreads + t] = j;
-                                else col[(size_t)cnt * n] = j;
-                            }
-                            ++cnt;
-                        }
-                    }
-                }
-            }
-            if (cnt > K) atomicMax(r.nb.overflow, cnt);
-            r.nb.nn[i] = cnt < K ? cnt : K;
-            // copy back the gathered state of object i; static_index over the same range
-            for (int k = 0; k < r.pa.nd; ++k) r.pa.d[k][i] = r.pa.tmp_d[(size_t)k * n + i];
-            for (int k = 0; k < r.pa.ni; ++k) r.pa.i[k][i] = r.pa.tmp_i[(size_t)k * n + i];
-            r.static_index[i] = r.order_inv[r.static_index[i]];
-            if (r.lx) { r.lx[i] = xi; r.ly[i] = yi; }
-        }
-        if (STAGE) {
-            int m = cnt;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
-            if ((t & 63) == 0) wmax[t >> 6] = m;
-            __syncthreads();
-            int rows = 0;
-#pragma unroll
-            for (int q = 0; q < kTileThreads / 64; ++q) rows = max(rows, wmax[q]);
-            rows = min(rows, K);
-            const int have = min(cnt, K);
-            if (mine)
-                for (int k = 0; k < rows; ++k)
-                    r.nb.nbr[(size_t)k * n + i] = k < have ? nbr_l[k * kTileThreads + t] : -1;
-            __syncthreads();
-        }
-    }
-}
-
-// ---- per-system workgroup rebuild ------------------------------------------------------------
-// One workgroup per system (grid = S); systems whose flag is clear leave at once, so a step
-// with nothing to rebuild costs one near-empty launch and a rebuild costs one workgroup's
-// work on the flagged systems only.  The workgroup runs its system's whole rebuild through
-// LDS: cell counts with LDS atomics, the scan of the system's cells, an LDS scatter followed
-// by a per-cell insertion sort (ascending current index: the same order as the device-wide
-// chain), the permutation of every live array staged through LDS, static_index and the
-// displacement reference, then the neighbor lists with the system's positions, radii and cell
-// starts in LDS (read from global memory when they do not fit).
-constexpr int kSysLdsBytes = 156 * 1024;
-constexpr int kSysThreads = 1024;
-
 template <int W>
 __device__ __forceinline__ int wg_scan_excl(int v, int* lds, int& total) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -309,12 +170,205 @@ __device__ __forceinline__ int wg_scan_excl(int v, int* lds, int& total) {
     return pre + inc - v;
 }
 
+struct TileLds {
+    double hx[kTileCap], hy[kTileCap], hr[kTileCap];
+    int hj[kTileCap];
+    int hp[kTileCap];
+    int hbs[3][kHalo], hbe[3][kHalo];                  // LDS slots [hbs, hbe) of each halo cell
+    int rg0[9], rln[9], rbase[10];                     // runs: global start, length, LDS base
+    int wmax[kTileThreads / 64];
+};
+
+// The neighbor lists of one tile.  INPLACE: the objects are already in their new order in the
+// live arrays (the per-system engine permuted them); otherwise they are in the gather's
+// scratch arrays, and the workgroup copies its objects back.
+template <bool RIGID, bool STAGE, bool INPLACE>
+__device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L, int* nbr_l) {
+    const int* tl = r.tiles + 4 * tile;
+    const int s = tl[0], cy = tl[1], cx0 = tl[2], cx1 = tl[3];
+    const int t = threadIdx.x;
+    const int n = r.o.n, K = r.nb.K;
+    const int nx = r.ca.cdx[s], ny = r.ca.cdy[s], cs = r.ca.cstart[s];
+    const int* cst = r.ca.cell_start;
+    const double* gx = INPLACE ? r.o.x : r.pa.tmp_d;
+    const double* gy = INPLACE ? r.o.y : r.pa.tmp_d + (size_t)n;
+    const double* gr = INPLACE ? r.o.rad : r.pa.tmp_d + (size_t)r.ird * n;
+    const int* gcell = INPLACE ? r.ca.cell_id : r.pa.tmp_i;
+    const int* gpid = RIGID ? (INPLACE ? r.o.pid : r.pa.tmp_i + (size_t)n) : nullptr;
+    const int w = cx1 - cx0 + 2;                        // halo cells per row
+    // runs of each halo row: [nx - 1] if cx0 == 0; cells max(cx0 - 1, 0) .. min(cx1, nx - 1);
+    // [0] if cx1 == nx
+    if (t < 3) {
+        int yy = cy + t - 1; if (yy < 0) yy += ny; else if (yy >= ny) yy -= ny;
+        const int row = cs + yy * nx;
+        const int a = cx0 > 0 ? cx0 - 1 : 0, bb = cx1 < nx ? cx1 : nx - 1;
+        int q = 3 * t;
+        L.rg0[q] = cst[row + nx - 1]; L.rln[q] = cx0 == 0 ? cst[row + nx] - cst[row + nx - 1] : 0; ++q;
+        L.rg0[q] = cst[row + a]; L.rln[q] = cst[row + bb + 1] - cst[row + a]; ++q;
+        L.rg0[q] = cst[row]; L.rln[q] = cx1 == nx ? cst[row + 1] - cst[row] : 0;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int q = 0; q < 9; ++q) { L.rbase[q] = acc; acc += L.rln[q]; }
+        L.rbase[9] = acc;
+    }
+    __syncthreads();
+    const bool staged = L.rbase[9] <= kTileCap;
+    // halo cells: LDS slots (staged) or global object ranges
+    for (int h = t; h < 3 * w; h += kTileThreads) {
+        const int q = h / w, u = h % w;                 // row q, halo position u <-> cell cx0 - 1 + u
+        int yy = cy + q - 1; if (yy < 0) yy += ny; else if (yy >= ny) yy -= ny;
+        int x = cx0 - 1 + u;
+        int run = 3 * q + 1;
+        if (x < 0) { x += nx; run = 3 * q; } else if (x >= nx) { x -= nx; run = 3 * q + 2; }
+        const int cell = cs + yy * nx + x;
+        const int g0 = cst[cell], g1 = cst[cell + 1];
+        if (staged) { L.hbs[q][u] = L.rbase[run] + (g0 - L.rg0[run]); L.hbe[q][u] = L.rbase[run] + (g1 - L.rg0[run]); }
+        else { L.hbs[q][u] = g0; L.hbe[q][u] = g1; }
+    }
+    if (staged) {
+        for (int q = 0; q < 9; ++q) {
+            const int g0 = L.rg0[q], len = L.rln[q], base = L.rbase[q];
+            for (int k = t; k < len; k += kTileThreads) {
+                const int j = g0 + k;
+                L.hx[base + k] = gx[j]; L.hy[base + k] = gy[j]; L.hr[base + k] = gr[j]; L.hj[base + k] = j;
+                if (RIGID) L.hp[base + k] = gpid[j];
+            }
+        }
+    }
+    __syncthreads();
+    const double Lx = r.ca.Lx[s], Ly = r.ca.Ly[s], iLx = r.ca.iLx[s], iLy = r.ca.iLy[s], cut0 = r.ca.cutoff[s];
+    const int o0 = cst[cs + cy * nx + cx0], o1 = cst[cs + cy * nx + cx1];
+    for (int base = o0; base < o1; base += kTileThreads) {
+        const int i = base + t;
+        const bool mine = i < o1;
+        int cnt = 0;
+        if (mine) {
+            const double xi = gx[i], yi = gy[i], ri = gr[i];
+            const int pi = RIGID ? gpid[i] : -1;
+            const int ui = gcell[i] - cs - cy * nx - cx0 + 1;   // halo position of the own cell
+            int* col = r.nb.nbr + i;
+            for (int q = 0; q < 3; ++q) {
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int lo = L.hbs[q][ui + dx], hi = L.hbe[q][ui + dx];
+                    for (int m = lo; m < hi; ++m) {
+                        int j, pj;
+                        double xj, yj, rj;
+                        if (staged) { j = L.hj[m]; xj = L.hx[m]; yj = L.hy[m]; rj = L.hr[m]; pj = RIGID ? L.hp[m] : -2; }
+                        else { j = m; xj = gx[m]; yj = gy[m]; rj = gr[m]; pj = RIGID ? gpid[m] : -2; }
+                        if (j == i) continue;
+                        if (RIGID && pj == pi) continue;
+                        const double ddx = min_image(xi - xj, Lx, iLx);
+                        const double ddy = min_image(yi - yj, Ly, iLy);
+                        const double r2 = ddx * ddx + ddy * ddy;
+                        const double cut = ri + rj + cut0;
+                        if (r2 < cut * cut) {
+                            if (cnt < K) {
+                                if (STAGE) nbr_l[cnt * kTileThreads + t] = j;
+                                else col[(size_t)cnt * n] = j;
+                            }
+                            ++cnt;
+                        }
+                    }
+                }
+            }
+            if (cnt > K) atomicMax(r.nb.overflow, cnt);
+            r.nb.nn[i] = cnt < K ? cnt : K;
+            if (!INPLACE) {
+                // copy back the gathered state of object i; static_index over the same range
+                for (int k = 0; k < r.pa.nd; ++k) r.pa.d[k][i] = r.pa.tmp_d[(size_t)k * n + i];
+                for (int k = 0; k < r.pa.ni; ++k) r.pa.i[k][i] = r.pa.tmp_i[(size_t)k * n + i];
+                r.static_index[i] = r.order_inv[r.static_index[i]];
+                if (r.lx) { r.lx[i] = xi; r.ly[i] = yi; }
+            }
+        }
+        if (STAGE) {
+            int m = cnt;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+            if ((t & 63) == 0) L.wmax[t >> 6] = m;
+            __syncthreads();
+            int rows = 0;
+#pragma unroll
+            for (int q = 0; q < kTileThreads / 64; ++q) rows = max(rows, L.wmax[q]);
+            rows = min(rows, K);
+            const int have = min(cnt, K);
+            if (mine)
+                for (int k = 0; k < rows; ++k)
+                    r.nb.nbr[(size_t)k * n + i] = k < have ? nbr_l[k * kTileThreads + t] : -1;
+            __syncthreads();
+        }
+    }
+}
+
+// Device-wide chain: one workgroup per tile; tiles of systems not rebuilt leave at once.
+template <bool RIGID, bool STAGE>
+__global__ void __launch_bounds__(kTileThreads) k_neighbors_tiled(RebuildArgs r, const int* active) {
+    extern __shared__ int nbr_l[];                     // STAGE: K rows x kTileThreads
+    __shared__ TileLds L;
+    if (!*active) return;
+    if (!r.ca.flag[r.tiles[4 * blockIdx.x]]) return;
+    tile_neighbors<RIGID, STAGE, false>(r, blockIdx.x, L, nbr_l);
+}
+
+// Per-system engine: the tiles of the systems the sort kernel rebuilt (its list), spread over
+// a fixed grid; each workgroup forms the list's tile prefix in LDS and walks the virtual tile
+// index v = blockIdx.x, blockIdx.x + gridDim.x, ...
+constexpr int kListMax = 2048;   // systems of a context that the list mode takes
+constexpr int kListGrid = 1024;  // workgroups of the listed neighbor build
+
+template <bool RIGID, bool STAGE>
+__global__ void __launch_bounds__(kTileThreads) k_neighbors_listed(RebuildArgs r, const int* active) {
+    extern __shared__ int nbr_l[];
+    __shared__ TileLds L;
+    __shared__ int pre[kListMax + 1], sys[kListMax];
+    __shared__ int scan_w[kTileThreads / 64];
+    if (!*active) return;
+    const int F = *r.list_count;
+    if (F == 0) return;
+    const int t = threadIdx.x;
+    // prefix of the listed systems' tile counts (sequential per thread, then a block scan)
+    const int per = (F + kTileThreads - 1) / kTileThreads;
+    const int f0 = min(t * per, F), f1 = min(f0 + per, F);
+    int sum = 0;
+    for (int f = f0; f < f1; ++f) {
+        const int sy = r.list[f];
+        sys[f] = sy;
+        pre[f] = r.sys_tile[sy + 1] - r.sys_tile[sy];
+        sum += pre[f];
+    }
+    int total;
+    int run = wg_scan_excl<kTileThreads>(sum, scan_w, total);
+    for (int f = f0; f < f1; ++f) { const int c = pre[f]; pre[f] = run; run += c; }
+    if (t == 0) pre[F] = total;
+    __syncthreads();
+    for (int v = blockIdx.x; v < total; v += gridDim.x) {
+        int lo = 0, hi = F;                            // largest f with pre[f] <= v
+        while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (pre[mid] <= v) lo = mid; else hi = mid; }
+        const int tile = r.sys_tile[sys[lo]] + (v - pre[lo]);
+        tile_neighbors<RIGID, STAGE, true>(r, tile, L, nbr_l);
+        __syncthreads();
+    }
+}
+
+// ---- per-system workgroup rebuild ------------------------------------------------------------
+// One workgroup per system (grid = S); systems whose flag is clear leave at once, so a step
+// with nothing to rebuild costs near-empty launches and a rebuild costs work on the flagged
+// systems only.  The workgroup sorts its system through LDS: cell counts with LDS atomics,
+// the scan of the system's cells, an LDS scatter followed by a per-cell insertion sort
+// (ascending current index: the same order as the device-wide chain), the permutation of
+// every live array staged through LDS, static_index and the displacement reference; it then
+// lists its system for k_neighbors_listed, which builds the lists of the listed systems' cell
+// tiles on many workgroups (one workgroup per system spent most of its time there).
+constexpr int kSysLdsBytes = 156 * 1024;
+constexpr int kSysThreads = 1024;
+
 template <bool RIGID>
 __global__ void __launch_bounds__(kSysThreads) k_rebuild_sys(RebuildArgs r, const int* active) {
     constexpr int W = kSysThreads;
     __shared__ __align__(16) unsigned char lds_raw[kSysLdsBytes];
     if (!*active) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *r.any = 0;
     const int s = blockIdx.x;
     if (!r.ca.flag[s]) return;
     const int t = threadIdx.x;
@@ -388,68 +442,20 @@ __global__ void __launch_bounds__(kSysThreads) k_rebuild_sys(RebuildArgs r, cons
         for (int p = t; p < n; p += W) d[b + p] = st[p];
         __syncthreads();
     }
-    // neighbor lists
-    double* xs = (double*)lds_raw;
-    double* ys = xs + n;
-    double* rs = ys + n;
-    int* st0 = (int*)(rs + n);
-    int* ps = st0 + nc + 1;
-    const bool in_lds = (size_t)n * (RIGID ? 28 : 24) + 4 * (size_t)(nc + 1) <= (size_t)kSysLdsBytes;
-    if (in_lds) {
-        for (int p = t; p < n; p += W) {
-            xs[p] = r.o.x[b + p]; ys[p] = r.o.y[b + p]; rs[p] = r.o.rad[b + p];
-            if (RIGID) ps[p] = r.o.pid[b + p];
-        }
-        for (int c = t; c <= nc; c += W) st0[c] = c == nc ? n : r.ca.cell_start[cs + c] - b;
-    }
-    __syncthreads();
-    if (in_lds) {
-        const int nx = r.ca.cdx[s], ny = r.ca.cdy[s];
-        const double Lx = r.ca.Lx[s], Ly = r.ca.Ly[s], iLx = r.ca.iLx[s], iLy = r.ca.iLy[s], cut0 = r.ca.cutoff[s];
-        const int K = r.nb.K, N = r.o.n;
-        for (int p = t; p < n; p += W) {
-            const int i = b + p;
-            const int loc = cell_id[i] - cs;
-            const int cxi = loc % nx, cyi = loc / nx;
-            const double xi = xs[p], yi = ys[p], ri = rs[p];
-            const int pi = RIGID ? ps[p] : -1;
-            int* col = r.nb.nbr + i;
-            int cntn = 0;
-            for (int dy = -1; dy <= 1; ++dy) {
-                int yy = cyi + dy; if (yy < 0) yy += ny; else if (yy >= ny) yy -= ny;
-                const int row = yy * nx;
-                int c0[2], c1[2], nr = 0;
-                if (cxi == 0) { c0[0] = row + nx - 1; c1[0] = row + nx; c0[1] = row; c1[1] = row + 2; nr = 2; }
-                else if (cxi == nx - 1) { c0[0] = row + nx - 2; c1[0] = row + nx; c0[1] = row; c1[1] = row + 1; nr = 2; }
-                else { c0[0] = row + cxi - 1; c1[0] = row + cxi + 2; nr = 1; }
-                for (int g = 0; g < nr; ++g) {
-                    const int lo = st0[c0[g]], hi = st0[c1[g]];
-                    for (int q = lo; q < hi; ++q) {
-                        if (q == p) continue;
-                        if (RIGID && ps[q] == pi) continue;
-                        const double ddx = min_image(xi - xs[q], Lx, iLx);
-                        const double ddy = min_image(yi - ys[q], Ly, iLy);
-                        const double r2 = ddx * ddx + ddy * ddy;
-                        const double cut = ri + rs[q] + cut0;
-                        if (r2 < cut * cut) {
-                            if (cntn < K) col[(size_t)cntn * N] = b + q;
-                            ++cntn;
-                        }
-                    }
-                }
-            }
-            if (cntn > K) atomicMax(r.nb.overflow, cntn);
-            r.nb.nn[i] = cntn < K ? cntn : K;
-        }
-    } else {
-        for (int i = b + t; i < e; i += W) ph_neighbors(r, i);
-    }
     if (RIGID) {
         __syncthreads();
         for (int i = r.psys_off[s] + t; i < r.psys_off[s + 1]; i += W) ph_rigid_reset(r, i);
     }
-    __syncthreads();
-    if (t == 0) r.ca.flag[s] = 0u;
+    // the neighbor kernel takes this system's tiles from the list
+    if (t == 0) r.list[atomicAdd(r.list_count, 1)] = s;
+}
+
+// End of a per-system rebuild: clear the flags, the any-flag word and the list.
+__global__ void k_clear_listed(RebuildArgs r, const int* active) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= r.S || !*active) return;
+    r.ca.flag[s] = 0u;
+    if (s == 0) { *r.any = 0; *r.list_count = 0; }
 }
 
 __global__ void k_tile_sum(RebuildArgs r, const int* active) {
@@ -567,7 +573,8 @@ RebuildArgs make_rebuild_args(dpmd_ctx* c, bool in_step) {
     r.tile_sum = c->scan_part.p; r.nt = (c->C + SCAN_TILE - 1) / SCAN_TILE; r.S = c->S;
     r.any = c->run.p + RUN_ANY;
     r.ird = rigid ? 3 : 4;
-    r.tiles = c->tiles.p; r.ntiles = c->ntiles;
+    r.tiles = c->tiles.p; r.ntiles = c->ntiles; r.sys_tile = c->sys_tile.p;
+    r.list = c->sys_list.p; r.list_count = c->sys_list_n.p;
     return r;
 }
 
@@ -609,7 +616,7 @@ bool sys_rebuild_fits(dpmd_ctx* c) {
 
 bool use_sys_rebuild(dpmd_ctx* c) {
     if (c->rebuild_engine == 2) return false;
-    const bool fits = sys_rebuild_fits(c);
+    const bool fits = sys_rebuild_fits(c) && c->S <= kListMax;
     if (c->rebuild_engine == 1)
         need(fits, "DPMD_OPT_REBUILD_ENGINE = 1: a system is too large for the per-system workgroup rebuild");
     return fits;
@@ -640,8 +647,14 @@ void build_tiles(dpmd_ctx* c, const std::vector<int>& cdx, const std::vector<int
             }
     }
     c->ntiles = (int)(tl.size() / 4);
+    std::vector<int> st(c->S + 1, 0);                   // tiles of system s: [st[s], st[s + 1])
+    for (int k = 0; k < c->ntiles; ++k) st[tl[4 * k] + 1]++;
+    for (int s = 0; s < c->S; ++s) st[s + 1] += st[s];
     if (tl.empty()) tl.assign(4, 0);
     c->tiles.upload(tl.data(), (int64_t)tl.size(), c->st);
+    c->sys_tile.upload(st.data(), c->S + 1, c->st);
+    c->sys_list.resize(std::max(1, c->S));
+    if (c->sys_list_n.n == 0) { c->sys_list_n.resize(1); c->sys_list_n.zero(c->st); }
     c->max_sys_obj = mo;
     c->max_sys_cells = mc;
 }
@@ -690,8 +703,19 @@ void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all, bool in_s
     if (force_all) k_set_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, c->flag.p, active);
     RebuildArgs r = make_rebuild_args(c, in_step);
     if (use_sys_rebuild(c)) {
-        if (c->kind == DPMD_RIGID_BUMPY) k_rebuild_sys<true><<<S, kSysThreads, 0, st>>>(r, active);
-        else k_rebuild_sys<false><<<S, kSysThreads, 0, st>>>(r, active);
+        const bool stage = c->K <= kTileStageMaxK;
+        const size_t dyn = stage ? (size_t)c->K * kTileThreads * sizeof(int) : 0;
+        const int grid = std::max(1, std::min(r.ntiles, kListGrid));
+        if (c->kind == DPMD_RIGID_BUMPY) {
+            k_rebuild_sys<true><<<S, kSysThreads, 0, st>>>(r, active);
+            if (stage) k_neighbors_listed<true, true><<<grid, kTileThreads, dyn, st>>>(r, active);
+            else k_neighbors_listed<true, false><<<grid, kTileThreads, 0, st>>>(r, active);
+        } else {
+            k_rebuild_sys<false><<<S, kSysThreads, 0, st>>>(r, active);
+            if (stage) k_neighbors_listed<false, true><<<grid, kTileThreads, dyn, st>>>(r, active);
+            else k_neighbors_listed<false, false><<<grid, kTileThreads, 0, st>>>(r, active);
+        }
+        k_clear_listed<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(r, active);
     } else {
         rebuild_multi(c, r, active);
     }

@@ -24,6 +24,8 @@ struct RebuildArgs {
     // Row tiles of the cell grids for the tiled neighbor build: 4 ints per tile (system,
     // cell row, first cell, one past the last cell of the row segment).
     const int* tiles; int ntiles;
+    const int* sys_tile;               // tiles of system s: [sys_tile[s], sys_tile[s + 1])
+    int* list; int* list_count;        // systems rebuilt by the per-system engine
 };
 
 __device__ __forceinline__ void ph_zero(const RebuildArgs& r, int c) {

@@ -132,6 +132,8 @@ struct dpmd_ctx {
     dpmd::DVec<int> scan_part;
     dpmd::DVec<int> tiles;      // 4 ints per cell-row tile (system, row, first cell, end cell)
     int ntiles = 0;
+    dpmd::DVec<int> sys_tile;   // S+1 tile offsets per system
+    dpmd::DVec<int> sys_list, sys_list_n;   // systems rebuilt by the per-system engine, and their count
     int max_sys_obj = 0, max_sys_cells = 0;   // largest system (objects, cells)
     // rebuild engine: 0 per-system workgroups when every system fits, else the device-wide
     // chain; 1 per-system workgroups; 2 device-wide chain (DPMD_OPT_REBUILD_ENGINE)
