@@ -128,6 +128,7 @@ PROTOS = {
     "dpmd_fire_state": (i32, [vp, vp, vp, vp, vp]),
     "dpmd_fire_minimize": (i32, [vp, vp, i32, i32, f64, f64, i32, i32, i32, pi32, pi32]),
     "dpmd_adjoint_hvp": (i32, [vp, i32, vp, vp, vp, vp]),
+    "dpmd_adjoint_hvp_noalloc": (i32, [vp, i32, vp, vp, vp, vp]),
     "dpmd_adjoint_vjp": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "dpmd_adjoint_solve": (i32, [vp, i32, vp, vp, vp, vp, f64, i32, pi32, C.POINTER(C.c_double)]),
     "dpmd_adjoint_pressure": (i32, [vp, i32, f64, i32, vp, vp, vp, pi32, C.POINTER(C.c_double)]),

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 
 using namespace dpmd;
@@ -1173,6 +1174,36 @@ const char* dpmd_version(void) { return "dpmd_amd 0.1 (gfx950, fp64)"; }
 
 const char* dpmd_last_error(const dpmd_ctx* c) { return c ? c->err.c_str() : global_error().c_str(); }
 
+// ---- scratch of the entry point that allocates nothing ------------------------------------
+// dpmd_adjoint_hvp_noalloc keeps the current-order input and output vectors (4 x N doubles)
+// in this file-scope device array instead of context buffers, so it calls no allocator.
+// Each GPU the library runs on holds its own copy (32 MiB) until the process ends; every
+// call on a GPU shares it, so calls that overlap on one GPU race.
+constexpr int64_t kNoAllocMaxN = 1 << 20;
+static __device__ __align__(256) double scratch[4 * kNoAllocMaxN];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};   // device address of `scratch` per device
+
+// The device address of `scratch` on the current device (the array's name is a host-side
+// stand-in).  Resolved once per device: at context creation, else at the first call.
+double* scratch_ptr() {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    need(dev >= 0 && dev < kMaxDevices, "scratch: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        need(p != nullptr, "scratch: hipGetSymbolAddress returned a null address");
+        g_scratch_dev[dev] = (double*)p;
+    }
+    return g_scratch_dev[dev];
+}
+}  // namespace
+
 int dpmd_ctx_create(dpmd_ctx** out, int kind, void* stream) {
     return guard(nullptr, [&] {
         need(out != nullptr, "dpmd_ctx_create: out is NULL");
@@ -1188,6 +1219,7 @@ int dpmd_ctx_create(dpmd_ctx** out, int kind, void* stream) {
             c->one.upload(&one, 1, c->st);
             c->run.resize(RUN_WORDS); c->run.zero(c->st);
             c->overflow.resize(1); c->overflow.zero(c->st);
+            (void)scratch_ptr();   // set-up of the allocation-free entry point on this device
         } catch (...) { delete c; throw; }
         c->err.clear();
         *out = c;
@@ -1627,6 +1659,24 @@ int dpmd_vv_run(dpmd_ctx* c, int n, int use_graph) {
         c->run.upload(run, RUN_WORDS, c->st);
         (void)lean_run(c, n, false, 0, 0.0, 0.0, use_graph != 0, nullptr);
         check_overflow(c);
+    });
+}
+
+int dpmd_adjoint_hvp_noalloc(dpmd_ctx* c, int mode, const double* wx, const double* wy, double* yx, double* yy) {
+    return guard(c, [&] {
+        need_disk_cells(c);
+        need(wx && wy && yx && yy, "adjoint_hvp_noalloc: null vector");
+        const int64_t N = c->N;
+        need(N <= kNoAllocMaxN, "adjoint_hvp_noalloc: " + std::to_string(N) + " particles exceed the scratch maximum of " +
+                                    std::to_string(kNoAllocMaxN) + " (use dpmd_adjoint_hvp)");
+        double* sp = scratch_ptr();
+        double* tx = sp;                     // w in current order
+        double* ty = sp + kNoAllocMaxN;
+        double* ux = sp + 2 * kNoAllocMaxN;  // H w in current order
+        double* uy = sp + 3 * kNoAllocMaxN;
+        adj_to_current(c, wx, tx); adj_to_current(c, wy, ty);
+        adj_hvp_current(c, mode, tx, ty, ux, uy);
+        adj_to_canonical(c, ux, yx); adj_to_canonical(c, uy, yy);
     });
 }
 

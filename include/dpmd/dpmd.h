@@ -302,6 +302,14 @@ int dpmd_jam_binary_search(dpmd_ctx* ctx, const double* dt, int max_compression_
  * device arrays of length S.  mode: dpmd_force_mode. */
 /* y = H w. */
 int dpmd_adjoint_hvp(dpmd_ctx* ctx, int mode, const double* wx, const double* wy, double* yx, double* yy);
+/* y = H w, the same bits as dpmd_adjoint_hvp, calling no allocator (no hipMalloc / hipFree,
+ * no stream synchronisation: it only enqueues on the context's stream, so it can be called
+ * under HIP graph capture and in tight solver loops).  No counterpart in the reference: for
+ * callers that cannot allocate.  Its current-order work vectors live in a library-owned
+ * device array of 4 x 2^20 doubles (32 MiB per GPU, held until the process ends); systems of
+ * more than 2^20 particles in total are rejected with DPMD_ERR_ARGUMENT.  Calls that overlap
+ * on one GPU share that array and race. */
+int dpmd_adjoint_hvp_noalloc(dpmd_ctx* ctx, int mode, const double* wx, const double* wy, double* yx, double* yy);
 /* Vector-Jacobian product of (U_s, P_s, F) at the current state: given cotangents gU[S],
  * gP[S], gF[N] (each may be NULL = 0), returns d/dx, d/dr and d/dL of
  * sum_s gU_s U_s + gP_s P_s + sum_i gF_i . F_i (any output may be NULL). */
