@@ -1,14 +1,19 @@
-"""Benchmark: particle-steps/s of the disk hot path on one MI355X.
+"""Benchmark of the batched 2-D granular-MD hot path on one MI355X.
 
-Workload (one GPU's share at its real shape): a single system of N = 2**20 bidisperse disks
-(radii 0.5 / 0.7, 50:50), phi = 0.84, periodic box, FP64, neighbor skin 0.3; FIRE steps then
-velocity-Verlet (NVE) steps at dt = 1e-2 from Gaussian velocities at T = 1e-4.  The default
-schedule is the workload's 2000 FIRE + 5000 NVE steps; --steps K keeps that 2:5 split.
-
-value = N * (FIRE steps + NVE steps) / wall time of the timed region (inputs resident in
-HBM).  The dominant kernel's roofline figure comes from HIP events around every fused
-velocity-Verlet launch of the last NVE steps; cpu_baseline times the oracle (the C++ FP64 restatement)
-on a bounded sample on this host.
+Default line (one GPU's share at its real shape):
+  * value: a single system of N = 2**20 bidisperse disks (radii 0.5 / 0.7, 50:50), phi = 0.84,
+    periodic box, FP64, neighbor skin 0.3; FIRE steps then velocity-Verlet (NVE) steps at
+    dt = 1e-2 from Gaussian velocities at T = 1e-4.  The default schedule is the workload's
+    2000 FIRE + 5000 NVE steps; --steps K keeps that 2:5 split.  value = N * (FIRE steps + NVE
+    steps) / wall time of the timed region (inputs resident in HBM).
+  * roofline: HIP events around the fused velocity-Verlet launches of an untimed tail of at
+    least 256 launches after the timed region (whatever --steps is); fire_force_roofline:
+    events around the FIRE force launches of 300 untimed FIRE steps.
+  * batched / rigid / adjoint: bounded runs of the 512-system uneven jamming batch (batched
+    against one context per system), the 256 x 256 rigid bumpy batch (with the vertex-force
+    roofline) and the 128 x 1024 differentiation workload (adjoint / forward time).
+  * cpu_baseline: the oracle (the C++ FP64 restatement) on a bounded sample on this host.
+--workload selects the full-length runs of the other workloads instead.
 """
 from __future__ import annotations
 
@@ -24,10 +29,16 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/sec (batched FIRE/NVE) + %HBM roofline; adjoint/forward time"
 # HBM bytes per k_vv_fused launch on the default workload (N = 2^20, phi = 0.84), from the
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes in profiles/k_vv_fused_disk_N1M_fire_nve.md, each
-# scaled by what the counter reads for a copy kernel of known size (FETCH x 2.0, WRITE x 1.0).
+# scaled by what the counter reads for a copy kernel of known size (FETCH x 2.0, WRITE x 1.0),
+# at the profiled run's mean neighbor count.  A run reports it scaled to its own count.
 FUSED_TRAFFIC_N1M = (51352.8 * (131072.0 / 65545.5) + 32768.0) * 1024.0
+FUSED_TRAFFIC_K = 5.50
+FUSED_TRAFFIC_SOURCE = ("profile-derived: profiles/k_vv_fused_disk_N1M_fire_nve.md (FETCH_SIZE x 2.0 + "
+                        "WRITE_SIZE x 1.0), neighbor-id bytes scaled to this run's mean count")
+TAIL_LAUNCHES = 256   # fused launches timed after the timed region for the roofline
 
 
 def log(msg: str) -> None:
@@ -56,6 +67,8 @@ def parse():
                     help="jam512: skip the autotuner (untimed) and run the library defaults")
     ap.add_argument("--tune-cache", default=None,
                     help="jam512: JSON autotune cache to read (and extend); default: tune afresh")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="default workload: skip the bounded batched / rigid / adjoint legs of the line")
     return ap.parse_args()
 
 
@@ -502,6 +515,213 @@ def run_adjoint(args):
     print(json.dumps(out))
 
 
+# ---- bounded legs of the default run: the metric's batched, rigid and adjoint parts ---------
+JAM_B = dict(dt=1e-2, dphi=1e-3, p_target=1e-7, max_comp=6, fire_max=20000, ftol=1e-10, pe_target=1e-16)
+
+
+def leg_batched(args):
+    """The 512-system uneven batch (N_s 64..4096, phi_s 0.80..0.86, half PBC half walls)
+    through the pressure-target jamming driver, bounded: at most JAM_B['max_comp']
+    compressions of 1e-3 and JAM_B['fire_max'] FIRE steps per round; a round stops at
+    max|F| < 1e-10 or <pe> < 1e-16 (the reference's unjammed test).  Batched (one context,
+    persistent per-system FIRE) against the plain path (one context per system, per-step
+    launches) on 32 systems spread over the size range, with the same autotuned configuration;
+    energies of jammed systems compared at 1e-10 relative."""
+    import numpy as np
+
+    from dpmd_amd import _lib as L
+    from dpmd_amd import workloads as W
+    from dpmd_amd.system import Disk, jam_compress
+
+    b = W.uneven_batch(S=512, seed=1)
+    sizes = np.diff(b.off)
+    kw = dict(predicate=L.FIRE_PRED_FORCE_OR_PE, a=JAM_B["ftol"], b=JAM_B["pe_target"], mode=L.FORCE_PER_SYSTEM,
+              use_graph=not args.no_graph)
+    jargs = (JAM_B["dt"], JAM_B["dphi"], JAM_B["p_target"], JAM_B["max_comp"], JAM_B["fire_max"])
+
+    def load(bb):
+        P = Disk()
+        W.load_into(P, bb)
+        P.init_neighbors()
+        P.synchronize()
+        return P
+
+    jam_compress(load(W.system_of(b, 0)), JAM_B["dt"], JAM_B["dphi"], JAM_B["p_target"], 1, 200, **kw)  # warm-up
+    P = load(b)
+    from dpmd_amd import autotune as T
+    t0 = time.perf_counter()
+    res = T.autotune(P, T.FIRE, JAM_B["dt"], mode=L.FORCE_PER_SYSTEM, steps=256, cache=args.tune_cache)
+    t_tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    st, nc, work = jam_compress(P, *jargs, **kw)
+    P.synchronize()
+    t_b = time.perf_counter() - t0
+    steps = P.get("FIRE_STEPS")
+    P.compute_pe_total(); P.compute_pressure()
+    pe_b, pr_b = P.get("PE_TOTAL"), P.get("PRESSURE")
+    rate_b = work / t_b
+    log(f"batched: {t_b:.2f} s, {rate_b:.3e} particle-steps/s; jammed {int((st == 1).sum())}, "
+        f"dropped {int((st == -1).sum())}, max steps/system {int(steps.max())}")
+    order = np.argsort(sizes, kind="stable")
+    sample = sorted(order[np.linspace(0, b.S - 1, 32).astype(int)].tolist())
+    work_p, t_p, worst, worst_abs, same_decisions = 0, 0.0, 0.0, 0.0, True
+    for s_ in sample:
+        Q = load(W.system_of(b, s_))
+        L.check(L.lib().dpmd_set_option(Q.ctx, L.OPT_PERSISTENT_FIRE, 0), Q.ctx)
+        T.apply(Q, res.config)
+        t1 = time.perf_counter()
+        qst, qnc, qwork = jam_compress(Q, *jargs, **kw)
+        Q.synchronize()
+        t_p += time.perf_counter() - t1
+        work_p += qwork
+        Q.compute_pe_total(); Q.compute_pressure()
+        same_decisions = same_decisions and qst[0] == st[s_] and qnc[0] == nc[s_]
+        for got, ref in ((Q.get("PE_TOTAL")[0], pe_b[s_]), (Q.get("PRESSURE")[0], pr_b[s_])):
+            if st[s_] == 1:
+                worst = max(worst, abs(got - ref) / max(abs(ref), 1e-300))
+            else:
+                worst_abs = max(worst_abs, abs(got - ref))
+    rate_p = work_p / t_p
+    log(f"plain: 32 systems {t_p:.2f} s, {rate_p:.3e}/s; batched/plain {rate_b / rate_p:.1f}x")
+    return {"workload": "512 uneven disk systems (N_s 64..4096, phi 0.80..0.86, half PBC half walls), compressions "
+                        f"of 1e-3 (at most {JAM_B['max_comp']}) + FIRE (max|F|<1e-10 or <pe><1e-16, cap "
+                        f"{JAM_B['fire_max']} steps per round) until P >= 1e-7",
+            "n_particles": int(b.N), "value": rate_b, "unit": "particle-steps/s", "seconds": t_b,
+            "work_particle_steps": int(work), "jammed": int((st == 1).sum()), "dropped": int((st == -1).sum()),
+            "compressing_at_end": int((st == 0).sum()), "fire_steps_max": int(steps.max()),
+            "plain_sample_systems": len(sample), "plain_value": rate_p, "plain_seconds": t_p,
+            "batched_over_plain": rate_b / rate_p, "same_decisions": bool(same_decisions),
+            "max_rel_energy_pressure_diff_jammed": worst, "max_abs_diff_unjammed": worst_abs,
+            "autotune": {"config": res.config, "trials": len(res.trials), "seconds": t_tune}}
+
+
+def leg_rigid(args):
+    """256 wall boxes x 256 rigid bumpy polygons (phi 0.70), bounded: 2000 FIRE steps then 2000
+    velocity-Verlet steps at T = 1e-4 (graph-captured batches); then an untimed tail of 300 NVE
+    steps with HIP events around every vertex-force launch (the roofline)."""
+    import numpy as np
+
+    from dpmd_amd import _lib as L
+    from dpmd_amd import workloads as W
+    from dpmd_amd.system import RigidBumpy, VelocityVerlet, run_fire
+
+    lib = L.lib()
+    b = W.rigid_batch(S=256, n_per=256, phi=0.70, seed=1)
+    P = RigidBumpy()
+    W.load_rigid_into(P, b)
+    P.init_neighbors()
+    P.synchronize()
+    run_fire(P, 1e-2, 50, L.FIRE_PRED_MAX_FORCE, 1e-9, mode=L.FORCE_WALL, use_graph=not args.no_graph)   # warm-up
+    P = RigidBumpy()
+    W.load_rigid_into(P, b)
+    P.init_neighbors()
+    P.synchronize()
+    n_fire, n_nve = 2000, 2000
+    t0 = time.perf_counter()
+    steps, conv = run_fire(P, 1e-2, n_fire, L.FIRE_PRED_MAX_FORCE, 1e-9, mode=L.FORCE_WALL, use_graph=not args.no_graph)
+    P.synchronize()
+    t_fire = time.perf_counter() - t0
+    fire_work = int(P.get("FIRE_STEPS").astype(np.int64).sum()) * 256
+    P.rand_normal_2d("VEL_X", "VEL_Y", 1, 1)
+    P.set_average_velocity(0.0, 0.0)
+    P.set_temperature(1e-4)
+    vv = VelocityVerlet(P, 1e-2, wall=True)
+    vv.init()
+    P.synchronize()
+    t1 = time.perf_counter()
+    vv.run(n_nve, use_graph=not args.no_graph)
+    P.synchronize()
+    t_nve = time.perf_counter() - t1
+    L.check(lib.dpmd_set_option(P.ctx, L.OPT_DEVICE_REBUILD, 1), P.ctx)
+    lib.dpmd_profile(P.ctx, 1)
+    vv.run(300, use_graph=False)
+    P.synchronize()
+    L.check(lib.dpmd_set_option(P.ctx, L.OPT_DEVICE_REBUILD, 0), P.ctx)
+    tot_ms, cnt = C.c_double(), C.c_int()
+    L.check(lib.dpmd_profile_read(P.ctx, C.byref(tot_ms), C.byref(cnt)), P.ctx)
+    lib.dpmd_profile(P.ctx, 0)
+    cap, mx, nnb = P.neighbor_info()
+    k = nnb / b.Nv
+    bytes_per_launch = b.Nv * (56 + 4 * k)
+    avg_ms = tot_ms.value / max(1, cnt.value)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    rate = (fire_work + b.N * n_nve) / (t_fire + t_nve)
+    log(f"rigid: FIRE {steps} steps {t_fire:.3f} s, NVE {n_nve} steps {t_nve:.3f} s; k_vertex_force "
+        f"{avg_ms * 1e3:.1f} us ({achieved / HBM_PEAK_GBS:.2f} of peak)")
+    return {"workload": "256 wall boxes x 256 rigid bumpy polygons (n_v 12..26, R 0.5/0.7), phi 0.70: 2000 FIRE "
+                        "steps then 2000 NVE steps dt=1e-2 at T=1e-4",
+            "n_particles": b.N, "n_vertices": b.Nv, "value": rate, "unit": "particle-steps/s",
+            "fire_seconds": t_fire, "fire_particle_steps": fire_work, "nve_seconds": t_nve,
+            "nve_particle_steps_per_s": b.N * n_nve / t_nve, "nve_us_per_step": t_nve * 1e6 / n_nve,
+            "roofline": {"bound": "hbm", "kernel": "k_vertex_force (vertex contacts, walls)", "achieved": achieved,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "avg_launch_us": avg_ms * 1e3, "launches": cnt.value,
+                         "algorithmic_bytes_per_launch": bytes_per_launch, "mean_vertex_neighbors": k}}
+
+
+def leg_adjoint(args):
+    """128 periodic systems x 1024 disks at phi 0.86.  (a) FIRE to max|F| < 1e-12 (forward)
+    then d(pressure)/d(radii) by implicit differentiation (MINRES to 1e-10); (b) 1000
+    velocity-Verlet steps at T = 1e-4 (forward) and the reverse mode with checkpoints every 50
+    steps.  Reports adjoint / forward time for both."""
+    from dpmd_amd import _lib as L
+    from dpmd_amd import adjoint as A
+    from dpmd_amd import workloads as W
+    from dpmd_amd.system import Disk, VelocityVerlet, run_fire
+
+    S, n = 128, 1024
+    b = W.disk_batch([n] * S, [0.86] * S, [0] * S, seed=1)
+    P = Disk()
+    W.load_into(P, b)
+    P.init_neighbors()
+    Co = L.ORDER_CANONICAL
+    t0 = time.perf_counter()
+    steps, conv = run_fire(P, 1e-2, 300000, L.FIRE_PRED_MAX_FORCE, 1e-12)
+    P.zero_velocities()
+    P.synchronize()
+    t_fa = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dr, dLx, dLy, it, res = A.pressure_grad(P, L.FORCE_PBC, tol=1e-10, max_iter=50000)
+    P.synchronize()
+    t_aa = time.perf_counter() - t0
+    P.rand_normal_2d("VEL_X", "VEL_Y", 7, 1)
+    P.set_average_velocity(0.0, 0.0)
+    P.set_temperature(1e-4)
+    xs, ys = P.get("POS_X", Co), P.get("POS_Y", Co)
+    vx0, vy0 = P.get("VEL_X", Co), P.get("VEL_Y", Co)
+
+    def reset():
+        A.set_canonical(P, "POS_X", xs); A.set_canonical(P, "POS_Y", ys)
+        A.set_canonical(P, "VEL_X", vx0); A.set_canonical(P, "VEL_Y", vy0)
+        P.check_neighbors(True)
+        VelocityVerlet(P, 1e-2).init()
+        P.synchronize()
+
+    reset(); VelocityVerlet(P, 1e-2).run(1000)          # warm the graphs
+    reset()
+    t0 = time.perf_counter()
+    VelocityVerlet(P, 1e-2).run(1000)
+    P.synchronize()
+    t_fb = time.perf_counter() - t0
+    reset()
+    A.vv_run_checkpointed(P, 1000, 50)
+    P.synchronize()
+    ax, ay = P.get("POS_X", Co), P.get("POS_Y", Co)
+    t0 = time.perf_counter()
+    A.vv_backprop(P, 2 * (ax - xs) / n, 2 * (ay - ys) / n, None, None)
+    P.synchronize()
+    t_ab = time.perf_counter() - t0
+    log(f"adjoint: (a) FIRE {steps} steps {t_fa:.2f} s, implicit grad {t_aa:.3f} s ({it} MINRES it); "
+        f"(b) forward {t_fb:.3f} s, backprop {t_ab:.3f} s ({t_ab / t_fb:.2f}x)")
+    return {"workload": "128 PBC systems x 1024 disks, phi 0.86: (a) dP/dr through FIRE minima (max|F|<1e-12), "
+                        "implicit differentiation with MINRES to 1e-10; (b) dMSD/d(x0, v0) through 1000 NVE steps, "
+                        "checkpoints every 50",
+            "a_fire_steps": steps, "a_fire_converged": bool(conv), "a_forward_seconds": t_fa,
+            "a_adjoint_seconds": t_aa, "a_adjoint_over_forward": t_aa / t_fa, "a_minres_iterations": it,
+            "a_minres_residual": res, "b_forward_seconds": t_fb, "b_backprop_seconds": t_ab,
+            "b_adjoint_over_forward": t_ab / t_fb, "unit": "x"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -545,9 +765,10 @@ def main():
     P.synchronize()
 
 
-    # NVE: the bulk runs as lean graph batches; the last n_prof steps run the same batches
-    # launched directly, with HIP events around every fused launch (the roofline measurement).
-    n_prof = min(500, max(1, n_nve // 5))
+    # Timed region: FIRE then NVE, graph-captured batches.  Afterwards, untimed: a tail of NVE
+    # steps launched directly with HIP events around the fused launches of every batch that
+    # no rebuild halted (at least TAIL_LAUNCHES launches, whatever --steps is), then FIRE steps
+    # with events around every FIRE force launch.
     log(f"timed region: {n_fire} FIRE + {n_nve} NVE steps, N={N}")
     t0 = time.perf_counter()
     steps_fire, _ = run_fire(P, 1e-2, n_fire, L.FIRE_PRED_MAX_FORCE, 1e-12, use_graph=not args.no_graph)
@@ -559,19 +780,54 @@ def main():
     P.set_temperature(1e-4)
     vv = VelocityVerlet(P, 1e-2)
     vv.init()
-    vv.run(n_nve - n_prof, use_graph=not args.no_graph)
-    lib.dpmd_profile(P.ctx, 2)                       # HIP events around every fused launch
-    vv.run(n_prof, use_graph=False)
+    vv.run(n_nve, use_graph=not args.no_graph)
     P.synchronize()
     t = time.perf_counter() - t0
     log(f"NVE done: {n_nve} steps, {t - t_fire:.3f} s")
 
-    # one HIP event pair around the fused launches of every batch that no rebuild halted
+    # The tails run on the workload's own state: when the timed region was shorter than the
+    # workload's 2000 FIRE steps, the minimisation is completed first (untimed) and the system
+    # re-thermalised, so the figures do not depend on --steps.
+    settle = max(0, 2000 - steps_fire)
+    if settle:
+        run_fire(P, 1e-2, settle, L.FIRE_PRED_MAX_FORCE, 1e-12, use_graph=not args.no_graph)
+        P.rand_normal_2d("VEL_X", "VEL_Y", 1, 1)
+        P.set_average_velocity(0.0, 0.0)
+        P.set_temperature(1e-4)
+        vv.init()
+        vv.run(500, use_graph=not args.no_graph)
     tot_ms, cnt = C.c_double(), C.c_int()
-    L.check(lib.dpmd_profile_read(P.ctx, C.byref(tot_ms), C.byref(cnt)), P.ctx)
+    lib.dpmd_profile(P.ctx, 2)
+    tail = 0
+    while cnt.value < TAIL_LAUNCHES and tail < 20 * TAIL_LAUNCHES:
+        vv.run(128, use_graph=False)
+        tail += 128
+        P.synchronize()
+        t_ms, c_ = C.c_double(), C.c_int()
+        L.check(lib.dpmd_profile_read(P.ctx, C.byref(t_ms), C.byref(c_)), P.ctx)
+        tot_ms.value += t_ms.value
+        cnt.value += c_.value
     lib.dpmd_profile(P.ctx, 0)
     cap, mx, nnb = P.neighbor_info()
     k = nnb / N
+    # FIRE force kernel (forces, kick, power / energy / max|F| chunk partials) in an untimed tail
+    lib.dpmd_profile(P.ctx, 1)
+    run_fire(P, 1e-2, 300, L.FIRE_PRED_MAX_FORCE, 0.0, use_graph=False)
+    P.synchronize()
+    durs = (C.c_float * 4096)()
+    nd = C.c_int()
+    L.check(lib.dpmd_profile_durations(P.ctx, durs, 4096, C.byref(nd)), P.ctx)
+    lib.dpmd_profile(P.ctx, 0)
+    d = np.array(durs[:min(nd.value, 4096)], dtype=np.float64)
+    if len(d):   # launches of a batch that a rebuild halted exit at once (a few us)
+        d = d[d > 0.5 * np.percentile(d, 90)]
+    kf = P.neighbor_info()[2] / N
+    # k_fire_force per particle: x, y, r, sid, count, k ids, vx, vy, m (60 + 4k) read; fx, fy, pe,
+    # vx, vy (40) written -> 100 + 4k bytes (neighbor gathers and per-chunk partials excluded)
+    ff_bytes = N * (100 + 4 * kf)
+    ff_avg = float(d.mean()) if len(d) else float("nan")
+    ff_cnt = C.c_int(len(d))
+    ff_gbs = ff_bytes / (ff_avg * 1e-3) / 1e9
     # k_vv_fused per particle: reads x,y,rad (24), sid (4), count (4), k ids (4k), vx,vy,mass (24),
     # last-rebuild x,y (16); writes vx,vy (16) and the new x,y (16)
     # -> 104 + 4k bytes (neighbor coordinate gathers excluded; forces stay in registers).
@@ -593,7 +849,7 @@ def main():
     total_steps = steps_fire + n_nve
     value = N * total_steps / t
     out = {
-        "metric": "particle-steps/sec (FIRE + NVE, single large system)",
+        "metric": METRIC,
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -608,17 +864,28 @@ def main():
         "config": {"workload": "single large disk system: N=2^20 bidisperse (0.5/0.7, 50:50), phi=0.84, "
                                "PBC, FP64, skin 0.3; FIRE then NVE dt=1e-2 (T=1e-4)",
                    "n_particles": N, "fire_steps": steps_fire, "nve_steps": n_nve,
-                   "fire_seconds": t_fire, "nve_seconds": t - t_fire, "nve_profiled_steps": n_prof,
+                   "fire_seconds": t_fire, "nve_seconds": t - t_fire, "roofline_tail_nve_steps": tail,
+                   "roofline_settle_fire_steps": settle,
                    "mean_neighbors": k, "neighbor_capacity": cap, "max_neighbors": mx},
         "roofline": {"bound": "hbm",
                      "kernel": "k_vv_fused (NVE: forces, closing half kick, next opening half kick, drift, skin test)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": FUSED_TRAFFIC_N1M if N == 1 << 20 else None,
+                     "traffic": (FUSED_TRAFFIC_N1M + N * 4 * (k - FUSED_TRAFFIC_K)) if N == 1 << 20 else None,
+                     "traffic_source": FUSED_TRAFFIC_SOURCE if N == 1 << 20 else None,
                      "avg_launch_us": avg_ms * 1e3, "launches": cnt.value,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "copy_kernel_gbs": copy_gbs},
+        "fire_force_roofline": {"bound": "hbm", "kernel": "k_fire_force (forces, FIRE kick, chunk partials)",
+                                "achieved": ff_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": ff_gbs / HBM_PEAK_GBS, "avg_launch_us": ff_avg * 1e3,
+                                "launches": ff_cnt.value, "algorithmic_bytes_per_launch": ff_bytes},
     }
+    del P
+    if not args.no_legs:
+        out["batched"] = leg_batched(args)
+        out["rigid"] = leg_rigid(args)
+        out["adjoint"] = leg_adjoint(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, args.cpu_steps)
     print(json.dumps(out))
