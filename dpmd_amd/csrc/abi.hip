@@ -1833,7 +1833,7 @@ int dpmd_jam_compress(dpmd_ctx* c, const double* dt, double dphi, double p_targe
         HIP_OK(hipMemcpyAsync(dt0.data(), dt, S * 8, hipMemcpyDeviceToHost, c->st));
         HIP_OK(hipStreamSynchronize(c->st));
         // state: 0 compressing, 1 jammed, 2 budget used up, 3 not run, -1 dropped
-        std::vector<int> state(S), ncomp(S, 0), nsteps(S, 0), stopped(S), ns(S);
+        std::vector<int> state(S), ncomp(S, 0), nsteps(S, 0), stopped(S), ns(S), retries(S, 0);
         for (int s = 0; s < S; ++s) state[s] = dt0[s] == 0.0 ? 3 : 0;
         int64_t total = 0;
         for (;;) {
@@ -1856,18 +1856,25 @@ int dpmd_jam_compress(dpmd_ctx* c, const double* dt, double dphi, double p_targe
             HIP_OK(hipMemcpyAsync(ns.data(), c->f_nsteps.p, S * 4, hipMemcpyDeviceToHost, c->st));
             HIP_OK(hipMemcpyAsync(P.data(), c->press.p, S * 8, hipMemcpyDeviceToHost, c->st));
             HIP_OK(hipStreamSynchronize(c->st));
-            bool grow = false;
+            bool grow = false, retry = false;
             for (int s = 0; s < S; ++s) {
                 inc[s] = 0.0;
                 if (dts[s] == 0.0) continue;
                 nsteps[s] += ns[s];
                 total += (int64_t)(c->h_off[s + 1] - c->h_off[s]) * ns[s];
-                if (!stopped[s]) state[s] = -1;
+                if (!stopped[s]) {
+                    // a round that hit the step cap is run again from where it stopped (FIRE
+                    // restarted, no compression), up to jam_retries times, then the system is dropped
+                    if (retries[s] < c->jam_retries) { retries[s] += 1; retry = true; }
+                    else state[s] = -1;
+                }
                 else if (P[s] >= p_target) state[s] = 1;
                 else if (ncomp[s] >= max_comp) state[s] = 2;
                 else { inc[s] = dphi; ncomp[s] += 1; grow = true; }
             }
-            if (!grow) break;
+            if (!grow && !retry) break;
+            // every round boundary rebuilds every system (the compression does), so a retried
+            // system starts its next minimisation from the same lists alone or in a batch
             c->scr_b.upload(inc.data(), S, c->st);
             increment_packing_fraction_impl(c, c->scr_b.p);
         }
@@ -1956,6 +1963,10 @@ int dpmd_set_option(dpmd_ctx* c, int key, int value) {
                 need(value >= 0 && value <= 2, "DPMD_OPT_REBUILD_ENGINE must be 0, 1 or 2");
                 c->rebuild_engine = value;
                 break;
+            case DPMD_OPT_JAM_RETRIES:
+                need(value >= 0, "DPMD_OPT_JAM_RETRIES must be non-negative");
+                c->jam_retries = value;
+                break;
             default: throw ArgError("unknown option key");
         }
     });
@@ -1975,6 +1986,7 @@ int dpmd_get_option(dpmd_ctx* c, int key, int* value) {
             case DPMD_OPT_DEVICE_REBUILD: *value = c->dev_rebuild; break;
             case DPMD_OPT_PERSIST_LDS: *value = c->persist_lds; break;
             case DPMD_OPT_REBUILD_ENGINE: *value = c->rebuild_engine; break;
+            case DPMD_OPT_JAM_RETRIES: *value = c->jam_retries; break;
             default: throw ArgError("unknown option key");
         }
     });

@@ -138,6 +138,9 @@ struct dpmd_ctx {
     // rebuild engine: 0 per-system workgroups when every system fits, else the device-wide
     // chain; 1 per-system workgroups; 2 device-wide chain (DPMD_OPT_REBUILD_ENGINE)
     int rebuild_engine = 0;
+    // dpmd_jam_compress: times a round that hit its FIRE step cap is run again before the
+    // system is dropped (DPMD_OPT_JAM_RETRIES)
+    int jam_retries = 0;
 
     // ---- system-aligned 256-wide chunks over particles (deterministic reductions)
     int nchunks = 0;

@@ -384,8 +384,11 @@ def run_fire(P: _Particles, dt, max_steps: int, predicate: int, a: float, b: flo
 
 def jam_compress(P: _Particles, dt, dphi: float, p_target: float, max_compressions: int, fire_max_steps: int,
                  predicate: int = L.FIRE_PRED_MAX_FORCE, a: float = 1e-10, b: float = 0.0,
-                 mode: int = L.FORCE_PER_SYSTEM, use_graph: bool = True):
+                 mode: int = L.FORCE_PER_SYSTEM, use_graph: bool = True, retries: int = 0):
     """Pressure-target jamming by compression (dpmd_jam_compress).
+
+    retries: times a round that hit fire_max_steps is run again (FIRE restarted, no
+    compression) before the system is dropped (DPMD_OPT_JAM_RETRIES).
 
     Returns (jam_state[S], n_compressions[S], work) where jam_state is 1 jammed, 0 never run
     or below p_target after max_compressions, -1 dropped (a minimisation did not stop), and
@@ -396,6 +399,7 @@ def jam_compress(P: _Particles, dt, dphi: float, p_target: float, max_compressio
     st = np.zeros(S, np.int32)
     nc = np.zeros(S, np.int32)
     work = C.c_int64()
+    P._c(lib().dpmd_set_option(P.ctx, L.OPT_JAM_RETRIES, int(retries)))
     P._c(lib().dpmd_jam_compress(P.ctx, d.ptr, dphi, p_target, max_compressions, fire_max_steps, predicate, a, b,
                                  mode, int(use_graph), st.ctypes.data_as(C.POINTER(C.c_int)),
                                  nc.ctypes.data_as(C.POINTER(C.c_int)), C.byref(work)))

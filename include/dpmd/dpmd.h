@@ -396,11 +396,14 @@ int dpmd_stream_copy(double* dst, const double* src, int64_t n, int repeats, dou
  *   DPMD_OPT_REBUILD_ENGINE: how cells and lists are rebuilt.  1: one workgroup per system,
  *     whose flagged systems run their whole rebuild in LDS (systems of up to about 8000
  *     objects); 2: the device-wide chain of launches over all objects with the tiled
- *     neighbor build; 0 (default): 1 when every system fits, else 2.  Same results. */
+ *     neighbor build; 0 (default): 1 when every system fits, else 2.  Same results.
+ *   DPMD_OPT_JAM_RETRIES: dpmd_jam_compress runs a round whose minimisation hit the FIRE step
+ *     cap again (FIRE restarted from where it stopped, no compression) up to this many times
+ *     per system before dropping the system; default 0 (drop at once). */
 enum dpmd_option {
     DPMD_OPT_COOP_REBUILD = 0, DPMD_OPT_COOP_IN_GRAPH = 1, DPMD_OPT_FUSED_VV = 2, DPMD_OPT_BLOCK = 3,
     DPMD_OPT_MAX_BATCH = 4, DPMD_OPT_PERSISTENT_FIRE = 5, DPMD_OPT_PERSIST_BLOCK = 6, DPMD_OPT_PERSIST_PHASE = 7,
-    DPMD_OPT_DEVICE_REBUILD = 8, DPMD_OPT_PERSIST_LDS = 9, DPMD_OPT_REBUILD_ENGINE = 10
+    DPMD_OPT_DEVICE_REBUILD = 8, DPMD_OPT_PERSIST_LDS = 9, DPMD_OPT_REBUILD_ENGINE = 10, DPMD_OPT_JAM_RETRIES = 11
 };
 int dpmd_set_option(dpmd_ctx* ctx, int key, int value);
 int dpmd_get_option(dpmd_ctx* ctx, int key, int* value);

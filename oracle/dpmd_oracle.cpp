@@ -1425,11 +1425,15 @@ static int fire_minimize_counted(Disk* d, const double* dt, int max_steps, int p
 // jammed if P_s >= p_target, else phi_s += dphi (increment_packing_fraction,
 // base_particle.hpp:505-529), until max_comp compressions.  jam_state: 1 jammed, 0 never
 // run or budget used up, -1 minimisation did not stop.
+// retries: a round whose minimisation hit fire_max is run again from where it stopped (FIRE
+// restarted, no compression; every system is rebuilt at the round boundary) up to `retries`
+// times per system before the system is dropped.
 long long or_disk_jam_compress(void* h, const double* dt0, double dphi, double p_target, int max_comp, int fire_max,
-                               int pred, double a, double b, int mode, int* jam_state, int* n_comp, int* nsteps_out) {
+                               int pred, double a, double b, int mode, int* jam_state, int* n_comp, int* nsteps_out,
+                               int retries) {
     Disk* d = (Disk*)h;
     const int S = d->S;
-    std::vector<int> state(S), ncomp(S, 0), nsteps(S, 0), ns, stopped;
+    std::vector<int> state(S), ncomp(S, 0), nsteps(S, 0), ns, stopped, nretry(S, 0);
     std::vector<double> dts(S), inc(S), P(S);
     for (int s = 0; s < S; ++s) state[s] = dt0[s] == 0.0 ? 3 : 0;
     long long work = 0;
@@ -1439,18 +1443,21 @@ long long or_disk_jam_compress(void* h, const double* dt0, double dphi, double p
         if (!any) break;
         fire_minimize_counted(d, dts.data(), fire_max, pred, a, b, mode, ns, stopped);
         d->pressure(P.data());
-        bool grow = false;
+        bool grow = false, retry = false;
         for (int s = 0; s < S; ++s) {
             inc[s] = 0.0;
             if (dts[s] == 0.0) continue;
             nsteps[s] += ns[s];
             work += (long long)(d->off[s + 1] - d->off[s]) * ns[s];
-            if (!stopped[s]) state[s] = -1;
+            if (!stopped[s]) {
+                if (nretry[s] < retries) { nretry[s] += 1; retry = true; }
+                else state[s] = -1;
+            }
             else if (P[s] >= p_target) state[s] = 1;
             else if (ncomp[s] >= max_comp) state[s] = 2;
             else { inc[s] = dphi; ncomp[s] += 1; grow = true; }
         }
-        if (!grow) break;
+        if (!grow && !retry) break;
         d->increment_packing_fraction(inc.data());
     }
     for (int s = 0; s < S; ++s) {

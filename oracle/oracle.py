@@ -75,7 +75,7 @@ _PROTOS = {
     "or_disk_fire_get": (None, [vp, vp, vp, vp, vp]),
     "or_fire_update_scalar": (None, [vp, vp, vp, vp, f64, f64, f64, vp, vp]),
     "or_disk_fire_minimize": (i32, [vp, vp, i32, f64, f64, i32, f64, i32, vp]),
-    "or_disk_jam_compress": (C.c_longlong, [vp, vp, f64, f64, i32, i32, i32, f64, f64, i32, vp, vp, vp]),
+    "or_disk_jam_compress": (C.c_longlong, [vp, vp, f64, f64, i32, i32, i32, f64, f64, i32, vp, vp, vp, i32]),
     "or_disk_vv_init": (None, [vp, vp, i32]),
     "or_disk_vv_run": (None, [vp, i32, i32]),
     "or_disk_brute_neighbors": (i32, [vp, i32, vp]),
@@ -296,12 +296,12 @@ class OracleDisk:
         steps = lib().or_disk_fire_minimize(self.h, _p(a), max_steps, diff, tgt, pred, ftol, mode, _p(conv))
         return steps, bool(conv[0])
 
-    def jam_compress(self, dt, dphi, p_target, max_comp, fire_max, pred=1, a=1e-10, b=0.0, mode=0):
+    def jam_compress(self, dt, dphi, p_target, max_comp, fire_max, pred=1, a=1e-10, b=0.0, mode=0, retries=0):
         """Returns (jam_state[S], n_compressions[S], fire_steps[S], work)."""
         d = self._ps(dt)
         st, nc, ns = (np.zeros(self.S, np.int32) for _ in range(3))
         work = lib().or_disk_jam_compress(self.h, _p(d), dphi, p_target, max_comp, fire_max, pred, a, b, mode,
-                                          _p(st), _p(nc), _p(ns))
+                                          _p(st), _p(nc), _p(ns), int(retries))
         return st, nc, ns, int(work)
 
     def vv_init(self, dt, mode=0):

@@ -115,3 +115,41 @@ def test_force_or_pe_predicate_matches_oracle():
     assert np.array_equal(P.get("FIRE_STEPS"), ns) and gwork == work
     for f, n in FIELDS:
         assert np.array_equal(P.get(n), o.get(f)), f
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_jam_with_round_retries_matches_oracle(use_graph):
+    """Rounds that hit a short step cap run again (DPMD_OPT_JAM_RETRIES = 2): decisions,
+    compressions, step counts and the final state bit for bit with the oracle's driver."""
+    orc.lib().or_set_threads(1)
+    cap = 1500
+    b = small_batch()
+    o = orc.OracleDisk(b)
+    st, nc, ns, work = o.jam_compress(1e-2, DPHI, PT, CMAX, cap, pred=1, a=FTOL, mode=2, retries=2)
+    P = load(b)
+    print(f"jam retries S={b.S} graph={use_graph}", flush=True)
+    gst, gnc, gwork = jam_compress(P, 1e-2, DPHI, PT, CMAX, cap, L.FIRE_PRED_MAX_FORCE, FTOL,
+                                   mode=L.FORCE_PER_SYSTEM, use_graph=use_graph, retries=2)
+    assert np.array_equal(gst, st) and np.array_equal(gnc, nc)
+    assert np.array_equal(P.get("FIRE_STEPS"), ns) and gwork == work
+    for f, n in FIELDS:
+        assert np.array_equal(P.get(n), o.get(f)), f
+
+
+def test_jam_retries_batched_equals_plain_per_system():
+    """With retries, a system's rounds (and the rebuild at each round boundary) do not depend
+    on the other systems: positions bit-identical alone and in the batch."""
+    cap = 1500
+    b = small_batch()
+    P = load(b)
+    jst, jnc, _ = jam_compress(P, 1e-2, DPHI, PT, CMAX, cap, L.FIRE_PRED_MAX_FORCE, FTOL, mode=L.FORCE_PER_SYSTEM,
+                               retries=2)
+    batched = final_state(P)
+    for s in range(b.S):
+        Q = load(W.system_of(b, s))
+        qst, qnc, _ = jam_compress(Q, 1e-2, DPHI, PT, CMAX, cap, L.FIRE_PRED_MAX_FORCE, FTOL,
+                                   mode=L.FORCE_PER_SYSTEM, retries=2)
+        assert qst[0] == jst[s] and qnc[0] == jnc[s]
+        one = final_state(Q)
+        for f in ("x", "y"):
+            assert np.array_equal(one[f], batched[f][b.off[s]:b.off[s + 1]]), (s, f)

@@ -127,3 +127,55 @@ def test_force_or_pe_predicate_stops_at_the_earlier_test():
         src = "pe" if expect_pe_first else "f"
         assert runs[src][0] == first
         assert np.array_equal(runs["both"][2], runs[src][2])
+
+
+def test_retried_rounds_equal_loop_over_primitives():
+    """Round retries (a round that hit the step cap runs again, FIRE restarted, no
+    compression, every system rebuilt at the round boundary; dropped after `retries`
+    retries): the driver against the same rule written out over the primitives, with a
+    short cap so that rounds do hit it, bit for bit."""
+    cap, retries = 1500, 2
+    b = small_batch()
+    o = orc.OracleDisk(b)
+    st, nc, ns, work = o.jam_compress(1e-2, DPHI, PT, CMAX, cap, pred=1, a=FTOL, mode=2, retries=retries)
+    q = orc.OracleDisk(small_batch())
+    S = q.S
+    state = np.zeros(S, int)
+    ncomp = np.zeros(S, int)
+    nret = np.zeros(S, int)
+    n_retried = 0
+    while True:
+        active = state == 0
+        if not active.any():
+            break
+        dts = np.where(active, 1e-2, 0.0)
+        q.fire_minimize(dts, cap, pred=1, ftol=FTOL, mode=2)
+        fdt = q.fire_state()[0]
+        P = q.pressure()
+        inc = np.zeros(S)
+        retry = False
+        for s in np.flatnonzero(active):
+            if fdt[s] != 0.0:
+                if nret[s] < retries:
+                    nret[s] += 1; retry = True; n_retried += 1
+                else:
+                    state[s] = -1
+            elif P[s] >= PT:
+                state[s] = 1
+            elif ncomp[s] >= CMAX:
+                state[s] = 2
+            else:
+                inc[s] = DPHI
+                ncomp[s] += 1
+        if not inc.any() and not retry:
+            break
+        q.increment_packing_fraction(inc)
+    assert n_retried > 0, "the cap should make some round run again"
+    assert np.array_equal(np.where(state == 2, 0, state), st)
+    assert np.array_equal(ncomp, nc)
+    for f in ("x", "y", "fx", "fy"):
+        assert np.array_equal(q.get(f), o.get(f)), f
+    # a retry gives a capped round more steps: no system is dropped with fewer retries used
+    o0 = orc.OracleDisk(small_batch())
+    st0, *_ = o0.jam_compress(1e-2, DPHI, PT, CMAX, cap, pred=1, a=FTOL, mode=2, retries=0)
+    assert np.sum(st == -1) <= np.sum(st0 == -1)
