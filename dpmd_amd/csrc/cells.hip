@@ -143,7 +143,7 @@ __global__ void k_scatter_rank(RebuildArgs r, const int* active) {
 // displacement reference, so no separate copy-back pass runs.  A tile whose surroundings
 // hold more than kTileCap objects reads its candidates from global memory instead.
 constexpr int kTileThreads = 256;
-constexpr int kTileCap = 1024;
+constexpr int kTileCap = 768;      // staged candidates (slot numbers fit 16 bits)
 constexpr int kTileMaxCells = 256;          // cells of one row segment
 constexpr int kHalo = kTileMaxCells + 2;    // cells of one halo row
 constexpr int kTileStageMaxK = 64;          // capacities up to this stage the lists in LDS
@@ -170,20 +170,53 @@ __device__ __forceinline__ int wg_scan_excl(int v, int* lds, int& total) {
     return pre + inc - v;
 }
 
+// Copy back the gathered state of object i, update static_index at the same index and reset
+// the displacement reference.  Every load is issued before the first store (the stores could
+// alias the loads as far as the compiler knows, which would serialise the arrays).
+__device__ __forceinline__ void copy_back_one(const RebuildArgs& r, int i, int n, double xi, double yi) {
+    constexpr int MD = 10, MI = 2;
+    double v[MD];
+    int w[MI];
+#pragma unroll
+    for (int k = 0; k < MD; ++k) v[k] = k < r.pa.nd ? r.pa.tmp_d[(size_t)k * n + i] : 0.0;
+#pragma unroll
+    for (int k = 0; k < MI; ++k) w[k] = k < r.pa.ni ? r.pa.tmp_i[(size_t)k * n + i] : 0;
+    const int si = r.order_inv[r.static_index[i]];
+#pragma unroll
+    for (int k = 0; k < MD; ++k) if (k < r.pa.nd) r.pa.d[k][i] = v[k];
+#pragma unroll
+    for (int k = 0; k < MI; ++k) if (k < r.pa.ni) r.pa.i[k][i] = w[k];
+    r.static_index[i] = si;
+    if (r.lx) { r.lx[i] = xi; r.ly[i] = yi; }
+}
+
+// A staged candidate: position, radius, object index and owning particle in one 32-byte
+// record (two 16-byte LDS reads per candidate).
+struct __align__(16) TileCand {
+    double x, y, r;
+    int j, p;
+};
+
 struct TileLds {
-    double hx[kTileCap], hy[kTileCap], hr[kTileCap];
-    int hj[kTileCap];
-    int hp[kTileCap];
-    int hbs[3][kHalo], hbe[3][kHalo];                  // LDS slots [hbs, hbe) of each halo cell
-    int rg0[9], rln[9], rbase[10];                     // runs: global start, length, LDS base
-    int wmax[kTileThreads / 64];
+    TileCand hc[kTileCap];                             // staged candidates, in halo-cell order
+    int hg0[3 * kHalo], hbs[3 * kHalo], hcnt[3 * kHalo];   // per halo cell: first object, LDS slot, count
+    int scan_w[kTileThreads / 64];
 };
 
 // The neighbor lists of one tile.  INPLACE: the objects are already in their new order in the
 // live arrays (the per-system engine permuted them); otherwise they are in the gather's
 // scratch arrays, and the workgroup copies its objects back.
+//
+// Halo cells are numbered h = q * w + u (stencil row q = 0, 1, 2 for dy = -1, 0, 1; u = 0 ..
+// w - 1 for cells cx0 - 1 .. cx1, periodic); each thread owns a contiguous run of them, so the
+// slots follow from one workgroup scan of the cells' counts and the copies of neighboring
+// threads touch neighboring objects.  Two global round trips (cell starts, then the objects)
+// precede the candidate tests, which read only LDS.  Each object's list is staged in its own LDS
+// column as 16-bit slot numbers, then written out k-th entry by k-th entry, so the k-th ids of
+// a wave's objects are one coalesced store (entries past an object's count are not written:
+// every reader stops at the count).
 template <bool RIGID, bool STAGE, bool INPLACE>
-__device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L, int* nbr_l) {
+__device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L, unsigned short* nbr_l) {
     const int* tl = r.tiles + 4 * tile;
     const int s = tl[0], cy = tl[1], cx0 = tl[2], cx1 = tl[3];
     const int t = threadIdx.x;
@@ -196,52 +229,43 @@ __device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L, int* 
     const int* gcell = INPLACE ? r.ca.cell_id : r.pa.tmp_i;
     const int* gpid = RIGID ? (INPLACE ? r.o.pid : r.pa.tmp_i + (size_t)n) : nullptr;
     const int w = cx1 - cx0 + 2;                        // halo cells per row
-    // runs of each halo row: [nx - 1] if cx0 == 0; cells max(cx0 - 1, 0) .. min(cx1, nx - 1);
-    // [0] if cx1 == nx
-    if (t < 3) {
-        int yy = cy + t - 1; if (yy < 0) yy += ny; else if (yy >= ny) yy -= ny;
-        const int row = cs + yy * nx;
-        const int a = cx0 > 0 ? cx0 - 1 : 0, bb = cx1 < nx ? cx1 : nx - 1;
-        int q = 3 * t;
-        L.rg0[q] = cst[row + nx - 1]; L.rln[q] = cx0 == 0 ? cst[row + nx] - cst[row + nx - 1] : 0; ++q;
-        L.rg0[q] = cst[row + a]; L.rln[q] = cst[row + bb + 1] - cst[row + a]; ++q;
-        L.rg0[q] = cst[row]; L.rln[q] = cx1 == nx ? cst[row + 1] - cst[row] : 0;
-    }
-    __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int q = 0; q < 9; ++q) { L.rbase[q] = acc; acc += L.rln[q]; }
-        L.rbase[9] = acc;
-    }
-    __syncthreads();
-    const bool staged = L.rbase[9] <= kTileCap;
-    // halo cells: LDS slots (staged) or global object ranges
-    for (int h = t; h < 3 * w; h += kTileThreads) {
-        const int q = h / w, u = h % w;                 // row q, halo position u <-> cell cx0 - 1 + u
+    const int H = 3 * w;
+    const int per = (H + kTileThreads - 1) / kTileThreads;
+    const int h0 = min(t * per, H), h1 = min(h0 + per, H);
+    int sum = 0;
+    for (int h = h0; h < h1; ++h) {
+        const int q = h / w, u = h - q * w;
         int yy = cy + q - 1; if (yy < 0) yy += ny; else if (yy >= ny) yy -= ny;
-        int x = cx0 - 1 + u;
-        int run = 3 * q + 1;
-        if (x < 0) { x += nx; run = 3 * q; } else if (x >= nx) { x -= nx; run = 3 * q + 2; }
+        int x = cx0 - 1 + u; if (x < 0) x += nx; else if (x >= nx) x -= nx;
         const int cell = cs + yy * nx + x;
-        const int g0 = cst[cell], g1 = cst[cell + 1];
-        if (staged) { L.hbs[q][u] = L.rbase[run] + (g0 - L.rg0[run]); L.hbe[q][u] = L.rbase[run] + (g1 - L.rg0[run]); }
-        else { L.hbs[q][u] = g0; L.hbe[q][u] = g1; }
+        const int g0 = cst[cell], c = cst[cell + 1] - g0;
+        L.hg0[h] = g0; L.hcnt[h] = c;
+        sum += c;
     }
-    if (staged) {
-        for (int q = 0; q < 9; ++q) {
-            const int g0 = L.rg0[q], len = L.rln[q], base = L.rbase[q];
-            for (int k = t; k < len; k += kTileThreads) {
+    int total;
+    int base = wg_scan_excl<kTileThreads>(sum, L.scan_w, total);
+    const bool staged = total <= kTileCap;
+    for (int h = h0; h < h1; ++h) {
+        const int g0 = L.hg0[h], c = L.hcnt[h];
+        if (staged) {
+            L.hbs[h] = base;
+            for (int k = 0; k < c; ++k) {
                 const int j = g0 + k;
-                L.hx[base + k] = gx[j]; L.hy[base + k] = gy[j]; L.hr[base + k] = gr[j]; L.hj[base + k] = j;
-                if (RIGID) L.hp[base + k] = gpid[j];
+                TileCand q;
+                q.x = gx[j]; q.y = gy[j]; q.r = gr[j]; q.j = j; q.p = RIGID ? gpid[j] : -2;
+                L.hc[base + k] = q;
             }
+        } else {
+            L.hbs[h] = g0;                              // candidates read from global memory
         }
+        base += c;
     }
     __syncthreads();
     const double Lx = r.ca.Lx[s], Ly = r.ca.Ly[s], iLx = r.ca.iLx[s], iLy = r.ca.iLy[s], cut0 = r.ca.cutoff[s];
     const int o0 = cst[cs + cy * nx + cx0], o1 = cst[cs + cy * nx + cx1];
-    for (int base = o0; base < o1; base += kTileThreads) {
-        const int i = base + t;
+    const bool stage_out = STAGE && staged;
+    for (int ob = o0; ob < o1; ob += kTileThreads) {
+        const int i = ob + t;
         const bool mine = i < o1;
         int cnt = 0;
         if (mine) {
@@ -249,55 +273,53 @@ __device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L, int* 
             const int pi = RIGID ? gpid[i] : -1;
             const int ui = gcell[i] - cs - cy * nx - cx0 + 1;   // halo position of the own cell
             int* col = r.nb.nbr + i;
-            for (int q = 0; q < 3; ++q) {
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int lo = L.hbs[q][ui + dx], hi = L.hbe[q][ui + dx];
-                    for (int m = lo; m < hi; ++m) {
-                        int j, pj;
-                        double xj, yj, rj;
-                        if (staged) { j = L.hj[m]; xj = L.hx[m]; yj = L.hy[m]; rj = L.hr[m]; pj = RIGID ? L.hp[m] : -2; }
-                        else { j = m; xj = gx[m]; yj = gy[m]; rj = gr[m]; pj = RIGID ? gpid[m] : -2; }
-                        if (j == i) continue;
-                        if (RIGID && pj == pi) continue;
-                        const double ddx = min_image(xi - xj, Lx, iLx);
-                        const double ddy = min_image(yi - yj, Ly, iLy);
-                        const double r2 = ddx * ddx + ddy * ddy;
-                        const double cut = ri + rj + cut0;
-                        if (r2 < cut * cut) {
-                            if (cnt < K) {
-                                if (STAGE) nbr_l[cnt * kTileThreads + t] = j;
-                                else col[(size_t)cnt * n] = j;
-                            }
-                            ++cnt;
-                        }
+            auto test = [&](int m, int j, double xj, double yj, double rj, int pj) {
+                if (j == i) return;
+                if (RIGID && pj == pi) return;
+                const double ddx = min_image(xi - xj, Lx, iLx);
+                const double ddy = min_image(yi - yj, Ly, iLy);
+                const double r2 = ddx * ddx + ddy * ddy;
+                const double cut = ri + rj + cut0;
+                if (r2 < cut * cut) {
+                    if (cnt < K) {
+                        if (stage_out) nbr_l[cnt * kTileThreads + t] = (unsigned short)m;
+                        else col[(size_t)cnt * n] = j;
+                    }
+                    ++cnt;
+                }
+            };
+            if (staged) {
+                // the three cells of a stencil row are consecutive halo cells, so their slots form
+                // one range in visiting order; candidates are read four at a time, every read of a
+                // group issued before the first test
+                constexpr int G = 4;
+                for (int q = 0; q < 3; ++q) {
+                    const int h = q * w + ui;
+                    const int lo = L.hbs[h - 1], hi = L.hbs[h + 1] + L.hcnt[h + 1];
+                    for (int m0 = lo; m0 < hi; m0 += G) {
+                        TileCand cg[G];
+#pragma unroll
+                        for (int u = 0; u < G; ++u) cg[u] = L.hc[min(m0 + u, hi - 1)];
+#pragma unroll
+                        for (int u = 0; u < G; ++u)
+                            if (m0 + u < hi) test(m0 + u, cg[u].j, cg[u].x, cg[u].y, cg[u].r, cg[u].p);
                     }
                 }
+            } else {
+                for (int q = 0; q < 3; ++q)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int h = q * w + ui + dx;
+                        const int lo = L.hbs[h], hi = lo + L.hcnt[h];
+                        for (int m = lo; m < hi; ++m) test(m, m, gx[m], gy[m], gr[m], RIGID ? gpid[m] : -2);
+                    }
             }
             if (cnt > K) atomicMax(r.nb.overflow, cnt);
             r.nb.nn[i] = cnt < K ? cnt : K;
-            if (!INPLACE) {
-                // copy back the gathered state of object i; static_index over the same range
-                for (int k = 0; k < r.pa.nd; ++k) r.pa.d[k][i] = r.pa.tmp_d[(size_t)k * n + i];
-                for (int k = 0; k < r.pa.ni; ++k) r.pa.i[k][i] = r.pa.tmp_i[(size_t)k * n + i];
-                r.static_index[i] = r.order_inv[r.static_index[i]];
-                if (r.lx) { r.lx[i] = xi; r.ly[i] = yi; }
-            }
+            if (!INPLACE) copy_back_one(r, i, n, xi, yi);
         }
-        if (STAGE) {
-            int m = cnt;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
-            if ((t & 63) == 0) L.wmax[t >> 6] = m;
-            __syncthreads();
-            int rows = 0;
-#pragma unroll
-            for (int q = 0; q < kTileThreads / 64; ++q) rows = max(rows, L.wmax[q]);
-            rows = min(rows, K);
+        if (stage_out && mine) {   // the k-th ids of a wave's objects go out together
             const int have = min(cnt, K);
-            if (mine)
-                for (int k = 0; k < rows; ++k)
-                    r.nb.nbr[(size_t)k * n + i] = k < have ? nbr_l[k * kTileThreads + t] : -1;
-            __syncthreads();
+            for (int k = 0; k < have; ++k) r.nb.nbr[(size_t)k * n + i] = L.hc[nbr_l[k * kTileThreads + t]].j;
         }
     }
 }
@@ -305,7 +327,7 @@ __device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L, int* 
 // Device-wide chain: one workgroup per tile; tiles of systems not rebuilt leave at once.
 template <bool RIGID, bool STAGE>
 __global__ void __launch_bounds__(kTileThreads) k_neighbors_tiled(RebuildArgs r, const int* active) {
-    extern __shared__ int nbr_l[];                     // STAGE: K rows x kTileThreads
+    extern __shared__ unsigned short nbr_l[];          // STAGE: K rows x kTileThreads slot numbers
     __shared__ TileLds L;
     if (!*active) return;
     if (!r.ca.flag[r.tiles[4 * blockIdx.x]]) return;
@@ -320,7 +342,7 @@ constexpr int kListGrid = 1024;  // workgroups of the listed neighbor build
 
 template <bool RIGID, bool STAGE>
 __global__ void __launch_bounds__(kTileThreads) k_neighbors_listed(RebuildArgs r, const int* active) {
-    extern __shared__ int nbr_l[];
+    extern __shared__ unsigned short nbr_l[];
     __shared__ TileLds L;
     __shared__ int pre[kListMax + 1], sys[kListMax];
     __shared__ int scan_w[kTileThreads / 64];
@@ -593,7 +615,7 @@ void rebuild_multi(dpmd_ctx* c, const RebuildArgs& r, const int* active) {
     k_sort<<<blocks_for(C), DPMD_BLOCK, 0, st>>>(r, active);
     k_gather<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     const bool stage = c->K <= kTileStageMaxK;
-    const size_t dyn = stage ? (size_t)c->K * kTileThreads * sizeof(int) : 0;
+    const size_t dyn = stage ? (size_t)c->K * kTileThreads * sizeof(unsigned short) : 0;
     need(r.ntiles > 0, "rebuild: no cell tiles (cells not set up)");
     if (c->kind == DPMD_RIGID_BUMPY) {
         if (stage) k_neighbors_tiled<true, true><<<r.ntiles, kTileThreads, dyn, st>>>(r, active);
@@ -704,7 +726,7 @@ void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all, bool in_s
     RebuildArgs r = make_rebuild_args(c, in_step);
     if (use_sys_rebuild(c)) {
         const bool stage = c->K <= kTileStageMaxK;
-        const size_t dyn = stage ? (size_t)c->K * kTileThreads * sizeof(int) : 0;
+        const size_t dyn = stage ? (size_t)c->K * kTileThreads * sizeof(unsigned short) : 0;
         const int grid = std::max(1, std::min(r.ntiles, kListGrid));
         if (c->kind == DPMD_RIGID_BUMPY) {
             k_rebuild_sys<true><<<S, kSysThreads, 0, st>>>(r, active);

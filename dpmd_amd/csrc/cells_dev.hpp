@@ -74,13 +74,24 @@ __device__ __forceinline__ void ph_sort(const RebuildArgs& r, int c) {
     }
 }
 
-// reorder_particles_impl (disk.cu:1102-1125): gather into scratch by the new order.
+// reorder_particles_impl (disk.cu:1102-1125): gather into scratch by the new order.  Every
+// load is issued before the first store (the compiler cannot rule out aliasing, which would
+// serialise the arrays).
 __device__ __forceinline__ void ph_gather(const RebuildArgs& r, int p) {
     if (!r.ca.flag[r.o.sid[p]]) return;
     const int n = r.o.n;
     const int o = r.order[p];
-    for (int k = 0; k < r.pa.nd; ++k) r.pa.tmp_d[(size_t)k * n + p] = r.pa.d[k][o];
-    for (int k = 0; k < r.pa.ni; ++k) r.pa.tmp_i[(size_t)k * n + p] = r.pa.i[k][o];
+    constexpr int MD = 10, MI = 2;
+    double v[MD];
+    int w[MI];
+#pragma unroll
+    for (int k = 0; k < MD; ++k) v[k] = k < r.pa.nd ? r.pa.d[k][o] : 0.0;
+#pragma unroll
+    for (int k = 0; k < MI; ++k) w[k] = k < r.pa.ni ? r.pa.i[k][o] : 0;
+#pragma unroll
+    for (int k = 0; k < MD; ++k) if (k < r.pa.nd) r.pa.tmp_d[(size_t)k * n + p] = v[k];
+#pragma unroll
+    for (int k = 0; k < MI; ++k) if (k < r.pa.ni) r.pa.tmp_i[(size_t)k * n + p] = w[k];
     r.order_inv[o] = p;
 }
 
@@ -89,10 +100,20 @@ __device__ __forceinline__ void ph_gather(const RebuildArgs& r, int p) {
 __device__ __forceinline__ void ph_copy_back(const RebuildArgs& r, int p) {
     if (!r.ca.flag[r.o.sid[p]]) return;
     const int n = r.o.n;
-    for (int k = 0; k < r.pa.nd; ++k) r.pa.d[k][p] = r.pa.tmp_d[(size_t)k * n + p];
-    for (int k = 0; k < r.pa.ni; ++k) r.pa.i[k][p] = r.pa.tmp_i[(size_t)k * n + p];
-    r.static_index[p] = r.order_inv[r.static_index[p]];
-    if (r.lx) { r.lx[p] = r.pa.tmp_d[p]; r.ly[p] = r.pa.tmp_d[(size_t)n + p]; }
+    constexpr int MD = 10, MI = 2;
+    double v[MD];
+    int w[MI];
+#pragma unroll
+    for (int k = 0; k < MD; ++k) v[k] = k < r.pa.nd ? r.pa.tmp_d[(size_t)k * n + p] : 0.0;
+#pragma unroll
+    for (int k = 0; k < MI; ++k) w[k] = k < r.pa.ni ? r.pa.tmp_i[(size_t)k * n + p] : 0;
+    const int si = r.order_inv[r.static_index[p]];
+#pragma unroll
+    for (int k = 0; k < MD; ++k) if (k < r.pa.nd) r.pa.d[k][p] = v[k];
+#pragma unroll
+    for (int k = 0; k < MI; ++k) if (k < r.pa.ni) r.pa.i[k][p] = w[k];
+    r.static_index[p] = si;
+    if (r.lx) { r.lx[p] = v[0]; r.ly[p] = v[1]; }
 }
 
 // count_cell_neighbors_kernel + fill_neighbors_cell_kernel (base_point_particle_kernels.cu:

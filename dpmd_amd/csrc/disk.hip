@@ -137,10 +137,19 @@ __global__ void __launch_bounds__(B) k_fire_drift(DiskArgs a, FireArgs f, Gate g
     if (a.cell) raise_flags(a.flag, a.any, s, flag);
 }
 
+// Fold arguments of k_fire_force: fold = 1 lets the last chunk workgroup of each system fold
+// the system's partials (fire_fold_system), so no k_fire_finalize launch follows.
+struct FoldArgs {
+    int fold, do_conv, do_update, pred;
+    double pa, pb;
+    const int* off;
+};
+
 template <int MODE>
 __global__ void __launch_bounds__(DPMD_BLOCK)
-k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, Gate g) {
+k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, Gate g, FoldArgs fo) {
     __shared__ double lds[12];
+    __shared__ int last;
     if (!gate_force(g)) return;
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     const int s = r.chunk_sys[c];
@@ -174,18 +183,29 @@ k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, Gate g) {
         r.part[r.nch + c] = (lds[4] + lds[5]) + (lds[6] + lds[7]);
         r.part[2 * r.nch + c] = fmax(fmax(lds[8], lds[9]), fmax(lds[10], lds[11]));
     }
+    if (!fo.fold) return;
+    // the last chunk of the system to finish folds (release: the partial before the count;
+    // acquire: the fold reads the others' partials with agent-scope loads)
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned nch_s = (unsigned)(r.sys_chunk[s + 1] - r.sys_chunk[s]);
+        const unsigned prev = atomicAdd(&r.counter[s], 1u);
+        last = prev == nch_s - 1 ? 1 : 0;
+        if (last) { r.counter[s] = 0u; __threadfence(); }
+    }
+    __syncthreads();
+    if (last) fire_fold_system<true>(fo.off, f, r, s, fo.do_conv, fo.do_update, fo.pred, fo.pa, fo.pb, g, lds);
 }
 
-// One workgroup per system folds the chunk partials in a fixed order (strided sequential
-// accumulators, then the block tree), applies the stopping test and evaluates the next
-// step's FIRE update.  A separate launch orders it after every chunk without fences.
-__global__ void __launch_bounds__(DPMD_BLOCK)
-k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_update, int pred, double pa, double pb,
-                Gate g) {
-    __shared__ double lds[4];
-    if (!gate_open(g)) return;
-    const int s = blockIdx.x;
-    if (fire_frozen(f, s)) return;
+// The fold of one system's chunk partials in a fixed order (strided sequential accumulators,
+// then the block tree), the stopping test and the next step's FIRE update.  Run by a whole
+// 256-thread workgroup: k_fire_finalize (one workgroup per system) or the last chunk
+// workgroup of a system in k_fire_force.  coherent: read the partials with agent-scope loads
+// (they were written by other workgroups of the same launch).
+template <bool COHERENT>
+__device__ __forceinline__ void fire_fold_system(const int* off, const FireArgs& f, const RedArgs& r, int s,
+                                                 int do_conv, int do_update, int pred, double pa, double pb,
+                                                 const Gate& g, double* lds) {
     const int cb = r.sys_chunk[s], ce = r.sys_chunk[s + 1];
     // thread 0 fetches the system's FIRE scalars now, overlapping the fold below
     FireSys q{};
@@ -200,7 +220,12 @@ k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_updat
         for (int u = 0; u < U; ++u) {
             const int k = k0 + u * DPMD_BLOCK;
             const int kk = k < ce ? k : k0;
-            q0[u] = r.part[kk]; q1[u] = r.part[r.nch + kk]; q2[u] = r.part[2 * r.nch + kk];
+            if (COHERENT) {
+                q0[u] = load_agent(r.part + kk); q1[u] = load_agent(r.part + r.nch + kk);
+                q2[u] = load_agent(r.part + 2 * r.nch + kk);
+            } else {
+                q0[u] = r.part[kk]; q1[u] = r.part[r.nch + kk]; q2[u] = r.part[2 * r.nch + kk];
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -235,6 +260,18 @@ k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_updat
         if (f.frozen && was_stopped) f.frozen[s] = 1;
         if (do_conv) f.nsteps[s] += 1;
     }
+}
+
+// One workgroup per system folds the chunk partials (fire_fold_system).  A separate launch
+// orders it after every chunk without fences.
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_update, int pred, double pa, double pb,
+                Gate g) {
+    __shared__ double lds[4];
+    if (!gate_open(g)) return;
+    const int s = blockIdx.x;
+    if (fire_frozen(f, s)) return;
+    fire_fold_system<false>(off, f, r, s, do_conv, do_update, pred, pa, pb, g, lds);
 }
 
 // Stopping test + next-step update for the step()/converged() call pattern (fire.cu:59-78).
@@ -475,13 +512,15 @@ void disk_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, doubl
     DiskArgs a = disk_args(c);
     FireArgs f = fire_args(c);
     RedArgs r = red_args(c);
+    // the force launch folds each system's partials in its last chunk workgroup
+    FoldArgs fo{1, do_conv, 1, pred, pa, pb, c->off.p};
     {
         ProfScope prof(c);
         by_mode(c->fire_mode, [&](auto m) {
-            k_fire_force<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(a, f, r, kick, g);
+            k_fire_force<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(a, f, r, kick, g, fo);
         });
     }
-    fire_finalize(c, g, do_conv, pred, pa, pb);
+    HIP_OK(hipGetLastError());
 }
 
 void fire_finalize(dpmd_ctx* c, Gate g, int do_conv, int pred, double pa, double pb) {
