@@ -1253,7 +1253,6 @@ int dpmd_ctx_destroy(dpmd_ctx* c) {
                              &c->persist_order, &c->persist_todo, &c->persist_stop};
     for (auto* v : iv) v->release();
     c->flag.release();
-    c->sys_counter.release();
     delete c;
     return DPMD_OK;
 }

@@ -65,10 +65,10 @@ __device__ __forceinline__ void ph_top_scan(const RebuildArgs& r, int* lds) {
     }
 }
 
-// Block-level: cell_start for one tile.  Counts
+// Block-level: cell_start for one tile, from the sum of the counts before it (carry).  Counts
 // of systems that are not rebuilt are unchanged since their last rebuild, so their starts
 // are recomputed to the same values.
-__device__ __forceinline__ void ph_tile_down(const RebuildArgs& r, int t, int* lds) {
+__device__ __forceinline__ void ph_tile_down(const RebuildArgs& r, int t, int* lds, int carry) {
     const int C = r.ca.C;
     const int base = t * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS];
@@ -76,7 +76,7 @@ __device__ __forceinline__ void ph_tile_down(const RebuildArgs& r, int t, int* l
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = (base + k < C) ? r.ca.cell_count[base + k] : 0; s += v[k]; }
     int total;
-    int run = block_excl_scan_int(s, lds, total) + r.tile_sum[t];
+    int run = block_excl_scan_int(s, lds, total) + carry;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         const int c = base + k;
@@ -110,16 +110,35 @@ DPMD_EACH(k_copy_back, r.o.n, ph_copy_back)
 DPMD_EACH(k_rigid_reset, r.N, ph_rigid_reset)
 #undef DPMD_EACH
 
-// Cell id and the object's arrival rank in its cell (the value its count atomic returned);
-// the rank goes to order_inv, which the gather overwrites later.
+// Cell id and the object's rank in its cell.  The objects are still in the cell order of the
+// last rebuild, so consecutive lanes mostly share a cell: the first lane of each run of equal
+// cells in a wavefront takes the run's ranks with one atomic (its count's old value is the
+// run's first rank).  The ranks of a cell are distinct and dense, in no set order; k_sort puts
+// each cell in ascending current index.  The rank goes to order_inv, which the gather
+// overwrites later.
 __global__ void k_hash_rank(RebuildArgs r, const int* active) {
+    if (!*active) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= r.o.n || !*active) return;
-    const int s = r.o.sid[i];
-    if (!r.ca.flag[s]) return;
-    const int cell = r.ca.cstart[s] + hash_local(r.o, r.ca, i, s);
-    r.ca.cell_id[i] = cell;
-    r.order_inv[i] = atomicAdd(&r.ca.cell_count[cell], 1);
+    const int lane = threadIdx.x & 63;
+    int cell = -1;
+    if (i < r.o.n) {
+        const int s = r.o.sid[i];
+        if (r.ca.flag[s]) cell = r.ca.cstart[s] + hash_local(r.o, r.ca, i, s);
+    }
+    const int prev = __shfl_up(cell, 1, 64);
+    const bool head = lane == 0 || prev != cell;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = (2ull << lane) - 1ull;       // lanes 0 .. lane (all at lane 63)
+    const int first = 63 - __builtin_clzll(heads & upto);       // head of this lane's run
+    const unsigned long long after = heads & ~upto;
+    const int end = after ? __builtin_ffsll((long long)after) - 1 : 64;
+    int base = 0;
+    if (head && cell >= 0) base = atomicAdd(&r.ca.cell_count[cell], end - lane);
+    base = __shfl(base, first, 64);
+    if (cell >= 0) {
+        r.ca.cell_id[i] = cell;
+        r.order_inv[i] = base + (lane - first);
+    }
 }
 
 // Scatter by start + rank: no second atomic.  The rank order is arbitrary; k_sort then puts
@@ -129,6 +148,52 @@ __global__ void k_scatter_rank(RebuildArgs r, const int* active) {
     if (i >= r.o.n || !*active) return;
     if (!r.ca.flag[r.o.sid[i]]) return;
     r.order[r.ca.cell_start[r.ca.cell_id[i]] + r.order_inv[i]] = i;
+}
+
+// The per-cell sort (one thread per cell: ph_sort), then the gather of the workgroup's cells'
+// objects.  The objects of cells [c0, c0 + 256) are one range of new positions and this
+// workgroup alone sorted them, so a workgroup barrier orders the sort before the gather.  Two
+// objects per thread and round, every load issued before the first store (ph_gather's order).
+__global__ void __launch_bounds__(DPMD_BLOCK) k_sort_gather(RebuildArgs r, const int* active) {
+    if (!*active) return;
+    const int C = r.ca.C, n = r.o.n;
+    const int c0 = blockIdx.x * DPMD_BLOCK;
+    if (c0 + (int)threadIdx.x < C) ph_sort(r, c0 + threadIdx.x);
+    __syncthreads();
+    const int pb = r.ca.cell_start[c0], pe = r.ca.cell_start[min(c0 + DPMD_BLOCK, C)];
+    constexpr int MD = 10, MI = 2;
+    for (int p0 = pb + threadIdx.x; p0 < pe; p0 += 2 * DPMD_BLOCK) {
+        const int p1 = p0 + DPMD_BLOCK;
+        const bool a0 = r.ca.flag[r.o.sid[p0]] != 0u;
+        const bool a1 = p1 < pe && r.ca.flag[r.o.sid[p1]] != 0u;
+        const int o0 = a0 ? r.order[p0] : 0, o1 = a1 ? r.order[p1] : 0;
+        double v0[MD], v1[MD];
+        int w0[MI], w1[MI];
+#pragma unroll
+        for (int k = 0; k < MD; ++k) {
+            v0[k] = a0 && k < r.pa.nd ? r.pa.d[k][o0] : 0.0;
+            v1[k] = a1 && k < r.pa.nd ? r.pa.d[k][o1] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < MI; ++k) {
+            w0[k] = a0 && k < r.pa.ni ? r.pa.i[k][o0] : 0;
+            w1[k] = a1 && k < r.pa.ni ? r.pa.i[k][o1] : 0;
+        }
+        if (a0) {
+#pragma unroll
+            for (int k = 0; k < MD; ++k) if (k < r.pa.nd) r.pa.tmp_d[(size_t)k * n + p0] = v0[k];
+#pragma unroll
+            for (int k = 0; k < MI; ++k) if (k < r.pa.ni) r.pa.tmp_i[(size_t)k * n + p0] = w0[k];
+            r.order_inv[o0] = p0;
+        }
+        if (a1) {
+#pragma unroll
+            for (int k = 0; k < MD; ++k) if (k < r.pa.nd) r.pa.tmp_d[(size_t)k * n + p1] = v1[k];
+#pragma unroll
+            for (int k = 0; k < MI; ++k) if (k < r.pa.ni) r.pa.tmp_i[(size_t)k * n + p1] = w1[k];
+            r.order_inv[o1] = p1;
+        }
+    }
 }
 
 // ---- tiled neighbor build (device-wide chain) ----------------------------------------------
@@ -143,7 +208,7 @@ __global__ void k_scatter_rank(RebuildArgs r, const int* active) {
 // displacement reference, so no separate copy-back pass runs.  A tile whose surroundings
 // hold more than kTileCap objects reads its candidates from global memory instead.
 constexpr int kTileThreads = 256;
-constexpr int kTileCap = 768;      // staged candidates (slot numbers fit 16 bits)
+constexpr int kTileCap = 3 * kTileThreads;   // staged candidates: three loads per thread (slots fit 16 bits)
 constexpr int kTileMaxCells = 256;          // cells of one row segment
 constexpr int kHalo = kTileMaxCells + 2;    // cells of one halo row
 constexpr int kTileStageMaxK = 64;          // capacities up to this stage the lists in LDS
@@ -170,24 +235,36 @@ __device__ __forceinline__ int wg_scan_excl(int v, int* lds, int& total) {
     return pre + inc - v;
 }
 
-// Copy back the gathered state of object i, update static_index at the same index and reset
-// the displacement reference.  Every load is issued before the first store (the stores could
-// alias the loads as far as the compiler knows, which would serialise the arrays).
-__device__ __forceinline__ void copy_back_one(const RebuildArgs& r, int i, int n, double xi, double yi) {
+// The copy-back of one own object, in two halves: own_load issues every load (the cell id and,
+// for a gathered tile, the gathered state and static_index), own_store writes the state back,
+// updates static_index at the same index and resets the displacement reference.  The loads
+// of a workgroup's first round go out together with its candidate loads.
+struct OwnLoad {
+    double v[10];
+    int w[2];
+    int si, cell;
+};
+
+template <bool INPLACE>
+__device__ __forceinline__ void own_load(const RebuildArgs& r, int i, int n, const int* gcell, OwnLoad& q) {
     constexpr int MD = 10, MI = 2;
-    double v[MD];
-    int w[MI];
+    q.cell = gcell[i];
+    if (INPLACE) return;
 #pragma unroll
-    for (int k = 0; k < MD; ++k) v[k] = k < r.pa.nd ? r.pa.tmp_d[(size_t)k * n + i] : 0.0;
+    for (int k = 0; k < MD; ++k) q.v[k] = k < r.pa.nd ? r.pa.tmp_d[(size_t)k * n + i] : 0.0;
 #pragma unroll
-    for (int k = 0; k < MI; ++k) w[k] = k < r.pa.ni ? r.pa.tmp_i[(size_t)k * n + i] : 0;
-    const int si = r.order_inv[r.static_index[i]];
+    for (int k = 0; k < MI; ++k) q.w[k] = k < r.pa.ni ? r.pa.tmp_i[(size_t)k * n + i] : 0;
+    q.si = r.static_index[i];
+}
+
+__device__ __forceinline__ void own_store(const RebuildArgs& r, int i, const OwnLoad& q, int si_new) {
+    constexpr int MD = 10, MI = 2;
 #pragma unroll
-    for (int k = 0; k < MD; ++k) if (k < r.pa.nd) r.pa.d[k][i] = v[k];
+    for (int k = 0; k < MD; ++k) if (k < r.pa.nd) r.pa.d[k][i] = q.v[k];
 #pragma unroll
-    for (int k = 0; k < MI; ++k) if (k < r.pa.ni) r.pa.i[k][i] = w[k];
-    r.static_index[i] = si;
-    if (r.lx) { r.lx[i] = xi; r.ly[i] = yi; }
+    for (int k = 0; k < MI; ++k) if (k < r.pa.ni) r.pa.i[k][i] = q.w[k];
+    r.static_index[i] = si_new;
+    if (r.lx) { r.lx[i] = q.v[0]; r.ly[i] = q.v[1]; }
 }
 
 // A staged candidate: position, radius, object index and owning particle in one 32-byte
@@ -246,32 +323,66 @@ __device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L, unsig
     int base = wg_scan_excl<kTileThreads>(sum, L.scan_w, total);
     const bool staged = total <= kTileCap;
     for (int h = h0; h < h1; ++h) {
-        const int g0 = L.hg0[h], c = L.hcnt[h];
-        if (staged) {
-            L.hbs[h] = base;
-            for (int k = 0; k < c; ++k) {
-                const int j = g0 + k;
-                TileCand q;
-                q.x = gx[j]; q.y = gy[j]; q.r = gr[j]; q.j = j; q.p = RIGID ? gpid[j] : -2;
-                L.hc[base + k] = q;
-            }
-        } else {
-            L.hbs[h] = g0;                              // candidates read from global memory
+        L.hbs[h] = staged ? base : L.hg0[h];            // unstaged: candidates read from global memory
+        base += L.hcnt[h];
+    }
+    __syncthreads();
+    // own objects: the cells cx0 .. cx1 - 1 of row cy are the halo cells w + 1 .. 2w - 2
+    const int o0 = L.hg0[w + 1], o1 = L.hg0[2 * w - 2] + L.hcnt[2 * w - 2];
+    const int own_slot0 = L.hbs[w + 1];
+    OwnLoad q;
+    if (o0 + t < o1) own_load<INPLACE>(r, o0 + t, n, gcell, q);
+    if (staged) {
+        // The halo cells u = [0, b1), [b1, b2), [b2, w) of a row are consecutive cells of the
+        // grid (the row wraps in x at most twice), so each of these pieces is one object range
+        // whose slots are consecutive too: slot m of the piece starting at slot sl holds object
+        // g + (m - sl).  Three slots per thread, all loads issued before the first LDS store.
+        const int b1 = min(cx0 == 0 ? 1 : nx - cx0 + 1, w), b2 = min(b1 + nx, w);
+        int sl[9], gd[9];
+#pragma unroll
+        for (int p = 0; p < 9; ++p) {
+            const int rq = p / 3, k = p % 3;
+            const int ua = k == 0 ? 0 : (k == 1 ? b1 : b2), ub = k == 0 ? b1 : (k == 1 ? b2 : w);
+            sl[p] = ua < ub ? L.hbs[rq * w + ua] : INT_MAX;
+            gd[p] = ua < ub ? L.hg0[rq * w + ua] - sl[p] : 0;
         }
-        base += c;
+        TileCand cq[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int m = t + k * kTileThreads;
+            int d = gd[0];
+#pragma unroll
+            for (int p = 1; p < 9; ++p) d = sl[p] <= m ? gd[p] : d;
+            const int j = m + d;
+            if (m < total) {
+                cq[k].x = gx[j]; cq[k].y = gy[j]; cq[k].r = gr[j]; cq[k].j = j;
+                cq[k].p = RIGID ? gpid[j] : -2;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (t + k * kTileThreads < total) L.hc[t + k * kTileThreads] = cq[k];
     }
     __syncthreads();
     const double Lx = r.ca.Lx[s], Ly = r.ca.Ly[s], iLx = r.ca.iLx[s], iLy = r.ca.iLy[s], cut0 = r.ca.cutoff[s];
-    const int o0 = cst[cs + cy * nx + cx0], o1 = cst[cs + cy * nx + cx1];
     const bool stage_out = STAGE && staged;
     for (int ob = o0; ob < o1; ob += kTileThreads) {
         const int i = ob + t;
         const bool mine = i < o1;
         int cnt = 0;
         if (mine) {
-            const double xi = gx[i], yi = gy[i], ri = gr[i];
-            const int pi = RIGID ? gpid[i] : -1;
-            const int ui = gcell[i] - cs - cy * nx - cx0 + 1;   // halo position of the own cell
+            const int si_new = INPLACE ? 0 : r.order_inv[q.si];   // used at the end: latency hidden
+            double xi, yi, ri;
+            int pi = -1;
+            if (staged) {
+                const TileCand& me = L.hc[own_slot0 + (i - o0)];
+                xi = me.x; yi = me.y; ri = me.r;
+                if (RIGID) pi = me.p;
+            } else {
+                xi = gx[i]; yi = gy[i]; ri = gr[i];
+                if (RIGID) pi = gpid[i];
+            }
+            const int ui = q.cell - cs - cy * nx - cx0 + 1;   // halo position of the own cell
             int* col = r.nb.nbr + i;
             auto test = [&](int m, int j, double xj, double yj, double rj, int pj) {
                 if (j == i) return;
@@ -293,8 +404,8 @@ __device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L, unsig
                 // one range in visiting order; candidates are read four at a time, every read of a
                 // group issued before the first test
                 constexpr int G = 4;
-                for (int q = 0; q < 3; ++q) {
-                    const int h = q * w + ui;
+                for (int rq = 0; rq < 3; ++rq) {
+                    const int h = rq * w + ui;
                     const int lo = L.hbs[h - 1], hi = L.hbs[h + 1] + L.hcnt[h + 1];
                     for (int m0 = lo; m0 < hi; m0 += G) {
                         TileCand cg[G];
@@ -306,21 +417,22 @@ __device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L, unsig
                     }
                 }
             } else {
-                for (int q = 0; q < 3; ++q)
+                for (int rq = 0; rq < 3; ++rq)
                     for (int dx = -1; dx <= 1; ++dx) {
-                        const int h = q * w + ui + dx;
+                        const int h = rq * w + ui + dx;
                         const int lo = L.hbs[h], hi = lo + L.hcnt[h];
                         for (int m = lo; m < hi; ++m) test(m, m, gx[m], gy[m], gr[m], RIGID ? gpid[m] : -2);
                     }
             }
             if (cnt > K) atomicMax(r.nb.overflow, cnt);
             r.nb.nn[i] = cnt < K ? cnt : K;
-            if (!INPLACE) copy_back_one(r, i, n, xi, yi);
+            if (!INPLACE) own_store(r, i, q, si_new);
         }
         if (stage_out && mine) {   // the k-th ids of a wave's objects go out together
             const int have = min(cnt, K);
             for (int k = 0; k < have; ++k) r.nb.nbr[(size_t)k * n + i] = L.hc[nbr_l[k * kTileThreads + t]].j;
         }
+        if (i + kTileThreads < o1) own_load<INPLACE>(r, i + kTileThreads, n, gcell, q);   // next round
     }
 }
 
@@ -472,14 +584,6 @@ __global__ void __launch_bounds__(kSysThreads) k_rebuild_sys(RebuildArgs r, cons
     if (t == 0) r.list[atomicAdd(r.list_count, 1)] = s;
 }
 
-// End of a per-system rebuild: clear the flags, the any-flag word and the list.
-__global__ void k_clear_listed(RebuildArgs r, const int* active) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= r.S || !*active) return;
-    r.ca.flag[s] = 0u;
-    if (s == 0) { *r.any = 0; *r.list_count = 0; }
-}
-
 __global__ void k_tile_sum(RebuildArgs r, const int* active) {
     __shared__ int lds[4];
     if (!*active) return;
@@ -490,18 +594,22 @@ __global__ void k_top_scan(RebuildArgs r, const int* active) {
     if (!*active) return;
     ph_top_scan(r, lds);
 }
-__global__ void k_tile_down(RebuildArgs r, const int* active) {
+__global__ void k_tile_down(RebuildArgs r, const int* active) {   // after k_top_scan
     __shared__ int lds[4];
     if (!*active) return;
-    ph_tile_down(r, blockIdx.x, lds);
+    ph_tile_down(r, blockIdx.x, lds, r.tile_sum[blockIdx.x]);
 }
-
-// Gate of a device-side rebuild: copy the "some flag is set" word (the rebuild clears it)
-// and count the rebuilds run.  One thread.
-__global__ void k_rebuild_gate(int* run) {
-    const int a = run[RUN_ANY] != 0 ? 1 : 0;
-    run[RUN_GATE] = a;
-    if (a) run[RUN_REBUILDS] += 1;
+// Without k_top_scan: each workgroup adds up the tile sums before its own (at most
+// kSelfScanTiles of them).
+constexpr int kSelfScanTiles = 1024;
+__global__ void k_tile_down_self(RebuildArgs r, const int* active) {
+    __shared__ int lds[4];
+    if (!*active) return;
+    int v = 0;
+    for (int u = threadIdx.x; u < (int)blockIdx.x; u += DPMD_BLOCK) v += r.tile_sum[u];
+    int carry;
+    (void)block_excl_scan_int(v, lds, carry);
+    ph_tile_down(r, blockIdx.x, lds, carry);
 }
 
 __global__ void k_set_flags(int S, unsigned* flag, const int* active) {
@@ -510,11 +618,20 @@ __global__ void k_set_flags(int S, unsigned* flag, const int* active) {
     flag[s] = 1u;
 }
 
-__global__ void k_clear_flags(int S, unsigned* flag, int* any, const int* active) {
+// End of a rebuild: clear every flag and the any-flag word (and the per-system engine's list).
+// Unconditional, because the device-gated chain's gate is the any-flag word itself, which this
+// kernel clears; a flag is only ever set together with that word, or for a host-decided
+// rebuild, so clearing a flag that was not rebuilt cannot happen.  count: a device-gated
+// rebuild, counted in RUN_REBUILDS when it ran.
+__global__ void k_clear_flags(int S, unsigned* flag, int* run, int* list_count, int count) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S || !*active) return;
+    if (s >= S) return;
     flag[s] = 0u;
-    if (s == 0) *any = 0;
+    if (s == 0) {
+        if (count && run[RUN_ANY]) run[RUN_REBUILDS] += 1;
+        run[RUN_ANY] = 0;
+        if (list_count) *list_count = 0;
+    }
 }
 
 // set_naive_neighbor_count + fill_naive_neighbor_list_kernel (base_point_particle_kernels.cu:
@@ -601,19 +718,22 @@ RebuildArgs make_rebuild_args(dpmd_ctx* c, bool in_step) {
 }
 
 // The device-wide chain: cell ids and ranks, the scan of every cell, the rank scatter, the
-// per-cell sort, the gather of the live arrays, then the tiled neighbor build (which also
+// per-cell sort with the gather of the live arrays, then the tiled neighbor build (which also
 // copies the gathered state back).
-void rebuild_multi(dpmd_ctx* c, const RebuildArgs& r, const int* active) {
+void rebuild_multi(dpmd_ctx* c, const RebuildArgs& r, const int* active, int count) {
     hipStream_t st = c->st;
     const int n = r.o.n, C = r.ca.C;
     k_zero<<<blocks_for(C), DPMD_BLOCK, 0, st>>>(r, active);
     k_hash_rank<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     k_tile_sum<<<r.nt, DPMD_BLOCK, 0, st>>>(r, active);
-    k_top_scan<<<1, DPMD_BLOCK, 0, st>>>(r, active);
-    k_tile_down<<<r.nt, DPMD_BLOCK, 0, st>>>(r, active);
+    if (r.nt <= kSelfScanTiles) {
+        k_tile_down_self<<<r.nt, DPMD_BLOCK, 0, st>>>(r, active);
+    } else {
+        k_top_scan<<<1, DPMD_BLOCK, 0, st>>>(r, active);
+        k_tile_down<<<r.nt, DPMD_BLOCK, 0, st>>>(r, active);
+    }
     k_scatter_rank<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
-    k_sort<<<blocks_for(C), DPMD_BLOCK, 0, st>>>(r, active);
-    k_gather<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
+    k_sort_gather<<<blocks_for(C), DPMD_BLOCK, 0, st>>>(r, active);
     const bool stage = c->K <= kTileStageMaxK;
     const size_t dyn = stage ? (size_t)c->K * kTileThreads * sizeof(unsigned short) : 0;
     need(r.ntiles > 0, "rebuild: no cell tiles (cells not set up)");
@@ -625,7 +745,7 @@ void rebuild_multi(dpmd_ctx* c, const RebuildArgs& r, const int* active) {
         else k_neighbors_tiled<false, false><<<r.ntiles, kTileThreads, 0, st>>>(r, active);
     }
     if (r.plx) k_rigid_reset<<<blocks_for(r.N), DPMD_BLOCK, 0, st>>>(r, active);
-    k_clear_flags<<<blocks_for(r.S), DPMD_BLOCK, 0, st>>>(r.S, r.ca.flag, r.any, active);
+    k_clear_flags<<<blocks_for(r.S), DPMD_BLOCK, 0, st>>>(r.S, r.ca.flag, c->run.p, nullptr, count);
 }
 
 // Whether every system fits the per-system workgroup rebuild's LDS: the cell counts and
@@ -692,7 +812,6 @@ void build_chunks(dpmd_ctx* c) {
     c->chunk_sys.upload(cs.data(), cs.size(), c->st);
     c->chunk_beg.upload(cb.data(), cb.size(), c->st);
     c->sys_chunk.upload(sc.data(), sc.size(), c->st);
-    c->sys_counter.resize(c->S); c->sys_counter.zero(c->st);
     c->part.resize(4 * (int64_t)std::max(1, c->nchunks));
 }
 
@@ -709,17 +828,24 @@ void permute_particles(dpmd_ctx* c, const int* order_host) {
     RebuildArgs r = make_rebuild_args(c, false);
     k_gather<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     k_copy_back<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
-    k_clear_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, r.ca.flag, r.any, active);
+    k_clear_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, r.ca.flag, c->run.p, nullptr, 0);
     HIP_OK(hipGetLastError());
 }
 
-void rebuild_on_device(dpmd_ctx* c) {
-    k_rebuild_gate<<<1, 1, 0, c->st>>>(c->run.p);
-    HIP_OK(hipGetLastError());
-    rebuild_neighbors(c, c->run.p + RUN_GATE, false, true);
+namespace {
+void rebuild_impl(dpmd_ctx* c, const int* active, bool force_all, bool in_step, int count);
 }
+
+// The device-gated chain: its gate is the any-flag word the drift kernels raise, which the
+// chain's last kernel clears; a step without flags costs only near-empty launches.
+void rebuild_on_device(dpmd_ctx* c) { rebuild_impl(c, c->run.p + RUN_ANY, false, true, 1); }
 
 void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all, bool in_step) {
+    rebuild_impl(c, active, force_all, in_step, 0);
+}
+
+namespace {
+void rebuild_impl(dpmd_ctx* c, const int* active, bool force_all, bool in_step, int count) {
     const int S = c->S;
     hipStream_t st = c->st;
     if (force_all) k_set_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, c->flag.p, active);
@@ -737,17 +863,13 @@ void rebuild_neighbors(dpmd_ctx* c, const int* active, bool force_all, bool in_s
             if (stage) k_neighbors_listed<false, true><<<grid, kTileThreads, dyn, st>>>(r, active);
             else k_neighbors_listed<false, false><<<grid, kTileThreads, 0, st>>>(r, active);
         }
-        k_clear_listed<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(r, active);
+        k_clear_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, r.ca.flag, c->run.p, r.list_count, count);
     } else {
-        rebuild_multi(c, r, active);
+        rebuild_multi(c, r, active, count);
     }
     HIP_OK(hipGetLastError());
 }
-
-void clear_flags(dpmd_ctx* c, const int* active) {
-    k_clear_flags<<<blocks_for(c->S), DPMD_BLOCK, 0, c->st>>>(c->S, c->flag.p, c->run.p + RUN_ANY, active);
-    HIP_OK(hipGetLastError());
-}
+}  // namespace
 
 void naive_neighbors(dpmd_ctx* c) {
     k_naive<<<blocks_for(c->n_obj()), DPMD_BLOCK, 0, c->st>>>(obj_args(c), nbr_args(c));

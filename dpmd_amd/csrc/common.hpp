@@ -99,12 +99,6 @@ __device__ __forceinline__ double block_max256(double v, double* lds4) {
     return r;
 }
 
-// Coherent (agent-scope) loads for data another workgroup published before an
-// acq_rel atomic on the same counter.
-__device__ __forceinline__ double load_agent(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Workgroups are handed to the 8 XCDs round-robin (block b -> XCD b%8).  Remap so that
 // each XCD works on one contiguous range of chunks: spatially sorted particles then keep
 // their neighbor gathers inside that XCD's L2.

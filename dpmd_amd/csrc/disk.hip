@@ -137,72 +137,11 @@ __global__ void __launch_bounds__(B) k_fire_drift(DiskArgs a, FireArgs f, Gate g
     if (a.cell) raise_flags(a.flag, a.any, s, flag);
 }
 
-// Fold arguments of k_fire_force: fold = 1 lets the last chunk workgroup of each system fold
-// the system's partials (fire_fold_system), so no k_fire_finalize launch follows.
-struct FoldArgs {
-    int fold, do_conv, do_update, pred;
-    double pa, pb;
-    const int* off;
-};
-
-template <int MODE>
-__global__ void __launch_bounds__(DPMD_BLOCK)
-k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, Gate g, FoldArgs fo) {
-    __shared__ double lds[12];
-    __shared__ int last;
-    if (!gate_force(g)) return;
-    const int c = xcd_remap(blockIdx.x, gridDim.x);
-    const int s = r.chunk_sys[c];
-    if (fire_frozen(f, s)) return;   // whole workgroup: chunks never straddle systems
-    const int beg = r.chunk_beg[c];
-    const int end = min(beg + DPMD_BLOCK, a.off[s + 1]);
-    const int i = beg + threadIdx.x;
-    double vp = 0.0, vpe = 0.0, vf2 = 0.0;
-    if (i < end) {
-        double fx, fy, pe;
-        disk_force_mode<MODE>(a, i, s, fx, fy, pe);
-        a.fx[i] = fx; a.fy[i] = fy; a.pe[i] = pe;
-        double vx = a.vx[i], vy = a.vy[i];
-        if (kick) {
-            const double smi = f.dt[s] * 0.5 / a.mass[i];
-            vx = vx + fx * smi; vy = vy + fy * smi;
-            a.vx[i] = vx; a.vy[i] = vy;
-        }
-        vp = fx * vx + fy * vy;
-        vpe = pe;
-        vf2 = fx * fx + fy * fy;
-    }
-    // block_sum256 / block_max256 of the three values with one barrier: wavefront
-    // butterflies, then (w0 + w1) + (w2 + w3)
-    vp = wave_sum(vp); vpe = wave_sum(vpe); vf2 = wave_max(vf2);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) { lds[w] = vp; lds[4 + w] = vpe; lds[8 + w] = vf2; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        r.part[c] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
-        r.part[r.nch + c] = (lds[4] + lds[5]) + (lds[6] + lds[7]);
-        r.part[2 * r.nch + c] = fmax(fmax(lds[8], lds[9]), fmax(lds[10], lds[11]));
-    }
-    if (!fo.fold) return;
-    // the last chunk of the system to finish folds (release: the partial before the count;
-    // acquire: the fold reads the others' partials with agent-scope loads)
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned nch_s = (unsigned)(r.sys_chunk[s + 1] - r.sys_chunk[s]);
-        const unsigned prev = atomicAdd(&r.counter[s], 1u);
-        last = prev == nch_s - 1 ? 1 : 0;
-        if (last) { r.counter[s] = 0u; __threadfence(); }
-    }
-    __syncthreads();
-    if (last) fire_fold_system<true>(fo.off, f, r, s, fo.do_conv, fo.do_update, fo.pred, fo.pa, fo.pb, g, lds);
-}
-
 // The fold of one system's chunk partials in a fixed order (strided sequential accumulators,
-// then the block tree), the stopping test and the next step's FIRE update.  Run by a whole
-// 256-thread workgroup: k_fire_finalize (one workgroup per system) or the last chunk
-// workgroup of a system in k_fire_force.  coherent: read the partials with agent-scope loads
-// (they were written by other workgroups of the same launch).
-template <bool COHERENT>
+// then the block tree), the stopping test and the next step's FIRE update, by a whole
+// 256-thread workgroup.  A separate launch orders it after every chunk.  (Folding in the last
+// chunk workgroup of k_fire_force instead costs more at 2^20 disks: every workgroup then waits
+// for an agent-scope atomic's round trip before it retires.)
 __device__ __forceinline__ void fire_fold_system(const int* off, const FireArgs& f, const RedArgs& r, int s,
                                                  int do_conv, int do_update, int pred, double pa, double pb,
                                                  const Gate& g, double* lds) {
@@ -220,12 +159,7 @@ __device__ __forceinline__ void fire_fold_system(const int* off, const FireArgs&
         for (int u = 0; u < U; ++u) {
             const int k = k0 + u * DPMD_BLOCK;
             const int kk = k < ce ? k : k0;
-            if (COHERENT) {
-                q0[u] = load_agent(r.part + kk); q1[u] = load_agent(r.part + r.nch + kk);
-                q2[u] = load_agent(r.part + 2 * r.nch + kk);
-            } else {
-                q0[u] = r.part[kk]; q1[u] = r.part[r.nch + kk]; q2[u] = r.part[2 * r.nch + kk];
-            }
+            q0[u] = r.part[kk]; q1[u] = r.part[r.nch + kk]; q2[u] = r.part[2 * r.nch + kk];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -262,8 +196,46 @@ __device__ __forceinline__ void fire_fold_system(const int* off, const FireArgs&
     }
 }
 
-// One workgroup per system folds the chunk partials (fire_fold_system).  A separate launch
-// orders it after every chunk without fences.
+template <int MODE>
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, Gate g) {
+    __shared__ double lds[12];
+    if (!gate_force(g)) return;
+    const int c = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = r.chunk_sys[c];
+    if (fire_frozen(f, s)) return;   // whole workgroup: chunks never straddle systems
+    const int beg = r.chunk_beg[c];
+    const int end = min(beg + DPMD_BLOCK, a.off[s + 1]);
+    const int i = beg + threadIdx.x;
+    double vp = 0.0, vpe = 0.0, vf2 = 0.0;
+    if (i < end) {
+        double fx, fy, pe;
+        disk_force_mode<MODE>(a, i, s, fx, fy, pe);
+        a.fx[i] = fx; a.fy[i] = fy; a.pe[i] = pe;
+        double vx = a.vx[i], vy = a.vy[i];
+        if (kick) {
+            const double smi = f.dt[s] * 0.5 / a.mass[i];
+            vx = vx + fx * smi; vy = vy + fy * smi;
+            a.vx[i] = vx; a.vy[i] = vy;
+        }
+        vp = fx * vx + fy * vy;
+        vpe = pe;
+        vf2 = fx * fx + fy * fy;
+    }
+    // block_sum256 / block_max256 of the three values with one barrier: wavefront
+    // butterflies, then (w0 + w1) + (w2 + w3)
+    vp = wave_sum(vp); vpe = wave_sum(vpe); vf2 = wave_max(vf2);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { lds[w] = vp; lds[4 + w] = vpe; lds[8 + w] = vf2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        r.part[c] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+        r.part[r.nch + c] = (lds[4] + lds[5]) + (lds[6] + lds[7]);
+        r.part[2 * r.nch + c] = fmax(fmax(lds[8], lds[9]), fmax(lds[10], lds[11]));
+    }
+}
+
+// One workgroup per system folds the chunk partials (fire_fold_system).
 __global__ void __launch_bounds__(DPMD_BLOCK)
 k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_update, int pred, double pa, double pb,
                 Gate g) {
@@ -271,7 +243,7 @@ k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_updat
     if (!gate_open(g)) return;
     const int s = blockIdx.x;
     if (fire_frozen(f, s)) return;
-    fire_fold_system<false>(off, f, r, s, do_conv, do_update, pred, pa, pb, g, lds);
+    fire_fold_system(off, f, r, s, do_conv, do_update, pred, pa, pb, g, lds);
 }
 
 // Stopping test + next-step update for the step()/converged() call pattern (fire.cu:59-78).
@@ -512,15 +484,14 @@ void disk_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, doubl
     DiskArgs a = disk_args(c);
     FireArgs f = fire_args(c);
     RedArgs r = red_args(c);
-    // the force launch folds each system's partials in its last chunk workgroup
-    FoldArgs fo{1, do_conv, 1, pred, pa, pb, c->off.p};
     {
         ProfScope prof(c);
         by_mode(c->fire_mode, [&](auto m) {
-            k_fire_force<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(a, f, r, kick, g, fo);
+            k_fire_force<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(a, f, r, kick, g);
         });
     }
     HIP_OK(hipGetLastError());
+    fire_finalize(c, g, do_conv, pred, pa, pb);
 }
 
 void fire_finalize(dpmd_ctx* c, Gate g, int do_conv, int pred, double pa, double pb) {

@@ -16,7 +16,7 @@ struct FireArgs {
 
 struct RedArgs {
     const int* chunk_sys; const int* chunk_beg; const int* sys_chunk;
-    unsigned* counter; double* part; int nch;
+    double* part; int nch;
     double* pw_tot; double* pe_tot; double* maxf2;
 };
 
@@ -35,7 +35,7 @@ inline FireArgs fire_args(dpmd_ctx* c) {
 inline RedArgs red_args(dpmd_ctx* c) {
     RedArgs r;
     r.chunk_sys = c->chunk_sys.p; r.chunk_beg = c->chunk_beg.p; r.sys_chunk = c->sys_chunk.p;
-    r.counter = c->sys_counter.p; r.part = c->part.p; r.nch = c->nchunks;
+    r.part = c->part.p; r.nch = c->nchunks;
     r.pw_tot = c->pw_tot.p; r.pe_tot = c->pe_tot.p; r.maxf2 = c->maxf2.p;
     return r;
 }

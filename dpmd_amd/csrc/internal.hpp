@@ -50,7 +50,7 @@ struct NbrArgs {
 //       cannot race with the rebuild's own gate reads)
 //  [11] number of device-side rebuilds run (batches that rebuild on the device)
 enum { RUN_MAX = 1, RUN_RUNNING = 2, RUN_ACTIVE = 3, RUN_HALT = 4, RUN_HALT_AT = 5, RUN_CONV_AT = 6, RUN_ANY = 7,
-       RUN_REC = 8, RUN_ANY2 = 9, RUN_GATE = 10, RUN_REBUILDS = 11, RUN_WORDS = 12 };
+       RUN_REC = 8, RUN_ANY2 = 9, RUN_REBUILDS = 11, RUN_WORDS = 12 };
 
 // Launch gate.  Plain form (run == nullptr): the kernel works iff *active != 0.  Batch form
 // (lean fused runs): a step works while nothing halted and some system runs; the force
@@ -145,7 +145,6 @@ struct dpmd_ctx {
     // ---- system-aligned 256-wide chunks over particles (deterministic reductions)
     int nchunks = 0;
     dpmd::DVec<int> chunk_sys, chunk_beg, sys_chunk;   // sys_chunk: S+1
-    dpmd::DVec<unsigned> sys_counter;
     dpmd::DVec<double> part;                            // 4 * nchunks
 
     // ---- FIRE (fire.cuh:116-147): committed and next-step ("shadow") scalars
@@ -305,7 +304,6 @@ void build_tiles(dpmd_ctx* c, const std::vector<int>& cdx, const std::vector<int
 // inside a step batch: rebuild the flagged systems iff some drift raised a flag this step
 void rebuild_on_device(dpmd_ctx* c);
 void permute_particles(dpmd_ctx* c, const int* order_host);
-void clear_flags(dpmd_ctx* c, const int* active);
 void naive_neighbors(dpmd_ctx* c);
 void check_overflow(dpmd_ctx* c);
 
