@@ -402,6 +402,17 @@ void lean_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b, boo
         disk_vv_force_packed(c, gf, P[(m - 1) % 2]);
         return;
     }
+    if (!fire && rigid(c) && c->fuse_vv && !c->rec_seg && rigid_vv_fusable(c)) {
+        // rigid bodies: drift of step 0, then per step the (gated) rebuild, the vertex forces
+        // and one launch that closes the step and opens the next (the last step only closes)
+        rigid_vv_drift(c, Gate{nullptr, c->run.p, 0});
+        for (int k = 0; k < m; ++k) {
+            const Gate g = Gate{nullptr, c->run.p, k};
+            if (dev) rebuild_on_device(c);
+            if (k + 1 < m) rigid_vv_fused(c, g); else rigid_vv_force(c, g);
+        }
+        return;
+    }
     for (int k = 0; k < m; ++k) {
         const Gate g = Gate{nullptr, c->run.p, k};
         if (fire) {

@@ -208,10 +208,13 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_sort_gather(RebuildArgs r, const
 // displacement reference, so no separate copy-back pass runs.  A tile whose surroundings
 // hold more than kTileCap objects reads its candidates from global memory instead.
 constexpr int kTileThreads = 256;
-constexpr int kTileCap = 3 * kTileThreads;   // staged candidates: three loads per thread (slots fit 16 bits)
-constexpr int kTileMaxCells = 256;          // cells of one row segment
+// LDS per workgroup is about 30 KB, so five workgroups share a CU.
+constexpr int kTileCap = 640;               // staged candidates, at most three per thread (slots fit 16 bits)
+constexpr int kTileMaxCells = 96;           // cells of one row segment
 constexpr int kHalo = kTileMaxCells + 2;    // cells of one halo row
-constexpr int kTileStageMaxK = 64;          // capacities up to this stage the lists in LDS
+constexpr int kStageRows = 12;              // list entries staged in LDS per object (the rest go out directly)
+constexpr int kTileObjects = 170;           // objects per tile the tile widths aim at
+static_assert(kTileCap <= 3 * kTileThreads, "three staging loads per thread");
 
 template <int W>
 __device__ __forceinline__ int wg_scan_excl(int v, int* lds, int& total) {
@@ -277,6 +280,7 @@ struct __align__(16) TileCand {
 struct TileLds {
     TileCand hc[kTileCap];                             // staged candidates, in halo-cell order
     int hg0[3 * kHalo], hbs[3 * kHalo], hcnt[3 * kHalo];   // per halo cell: first object, LDS slot, count
+    unsigned short nbr_l[kStageRows * kTileThreads];   // the first list entries, as slot numbers
     int scan_w[kTileThreads / 64];
 };
 
@@ -286,14 +290,15 @@ struct TileLds {
 //
 // Halo cells are numbered h = q * w + u (stencil row q = 0, 1, 2 for dy = -1, 0, 1; u = 0 ..
 // w - 1 for cells cx0 - 1 .. cx1, periodic); each thread owns a contiguous run of them, so the
-// slots follow from one workgroup scan of the cells' counts and the copies of neighboring
-// threads touch neighboring objects.  Two global round trips (cell starts, then the objects)
-// precede the candidate tests, which read only LDS.  Each object's list is staged in its own LDS
-// column as 16-bit slot numbers, then written out k-th entry by k-th entry, so the k-th ids of
-// a wave's objects are one coalesced store (entries past an object's count are not written:
-// every reader stops at the count).
-template <bool RIGID, bool STAGE, bool INPLACE>
-__device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L, unsigned short* nbr_l) {
+// slots follow from one workgroup scan of the cells' counts.  Two global round trips (cell
+// starts, then the candidates and the own objects' state, all in one round) precede the
+// candidate tests, which read only LDS.  The first kStageRows entries of each object's list are
+// staged in its own LDS column as 16-bit slot numbers, then written out k-th entry by k-th
+// entry, so the k-th ids of a wave's objects are one coalesced store; later entries are
+// written as they are found (entries past an object's count are not written: every reader
+// stops at the count).
+template <bool RIGID, bool INPLACE>
+__device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L) {
     const int* tl = r.tiles + 4 * tile;
     const int s = tl[0], cy = tl[1], cx0 = tl[2], cx1 = tl[3];
     const int t = threadIdx.x;
@@ -365,7 +370,6 @@ __device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L, unsig
     }
     __syncthreads();
     const double Lx = r.ca.Lx[s], Ly = r.ca.Ly[s], iLx = r.ca.iLx[s], iLy = r.ca.iLy[s], cut0 = r.ca.cutoff[s];
-    const bool stage_out = STAGE && staged;
     for (int ob = o0; ob < o1; ob += kTileThreads) {
         const int i = ob + t;
         const bool mine = i < o1;
@@ -393,7 +397,7 @@ __device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L, unsig
                 const double cut = ri + rj + cut0;
                 if (r2 < cut * cut) {
                     if (cnt < K) {
-                        if (stage_out) nbr_l[cnt * kTileThreads + t] = (unsigned short)m;
+                        if (staged && cnt < kStageRows) L.nbr_l[cnt * kTileThreads + t] = (unsigned short)m;
                         else col[(size_t)cnt * n] = j;
                     }
                     ++cnt;
@@ -428,60 +432,34 @@ __device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L, unsig
             r.nb.nn[i] = cnt < K ? cnt : K;
             if (!INPLACE) own_store(r, i, q, si_new);
         }
-        if (stage_out && mine) {   // the k-th ids of a wave's objects go out together
-            const int have = min(cnt, K);
-            for (int k = 0; k < have; ++k) r.nb.nbr[(size_t)k * n + i] = L.hc[nbr_l[k * kTileThreads + t]].j;
+        if (staged && mine) {   // the k-th ids of a wave's objects go out together
+            const int have = min(min(cnt, K), kStageRows);
+            for (int k = 0; k < have; ++k) r.nb.nbr[(size_t)k * n + i] = L.hc[L.nbr_l[k * kTileThreads + t]].j;
         }
         if (i + kTileThreads < o1) own_load<INPLACE>(r, i + kTileThreads, n, gcell, q);   // next round
     }
 }
 
 // Device-wide chain: one workgroup per tile; tiles of systems not rebuilt leave at once.
-template <bool RIGID, bool STAGE>
+template <bool RIGID>
 __global__ void __launch_bounds__(kTileThreads) k_neighbors_tiled(RebuildArgs r, const int* active) {
-    extern __shared__ unsigned short nbr_l[];          // STAGE: K rows x kTileThreads slot numbers
     __shared__ TileLds L;
     if (!*active) return;
     if (!r.ca.flag[r.tiles[4 * blockIdx.x]]) return;
-    tile_neighbors<RIGID, STAGE, false>(r, blockIdx.x, L, nbr_l);
+    tile_neighbors<RIGID, false>(r, blockIdx.x, L);
 }
 
-// Per-system engine: the tiles of the systems the sort kernel rebuilt (its list), spread over
-// a fixed grid; each workgroup forms the list's tile prefix in LDS and walks the virtual tile
-// index v = blockIdx.x, blockIdx.x + gridDim.x, ...
-constexpr int kListMax = 2048;   // systems of a context that the list mode takes
+// Per-system engine: the tiles that the sort kernel listed (every tile of each system it
+// rebuilt), spread over a fixed grid: workgroup b takes list entries b, b + gridDim.x, ...
 constexpr int kListGrid = 1024;  // workgroups of the listed neighbor build
 
-template <bool RIGID, bool STAGE>
+template <bool RIGID>
 __global__ void __launch_bounds__(kTileThreads) k_neighbors_listed(RebuildArgs r, const int* active) {
-    extern __shared__ unsigned short nbr_l[];
     __shared__ TileLds L;
-    __shared__ int pre[kListMax + 1], sys[kListMax];
-    __shared__ int scan_w[kTileThreads / 64];
     if (!*active) return;
-    const int F = *r.list_count;
-    if (F == 0) return;
-    const int t = threadIdx.x;
-    // prefix of the listed systems' tile counts (sequential per thread, then a block scan)
-    const int per = (F + kTileThreads - 1) / kTileThreads;
-    const int f0 = min(t * per, F), f1 = min(f0 + per, F);
-    int sum = 0;
-    for (int f = f0; f < f1; ++f) {
-        const int sy = r.list[f];
-        sys[f] = sy;
-        pre[f] = r.sys_tile[sy + 1] - r.sys_tile[sy];
-        sum += pre[f];
-    }
-    int total;
-    int run = wg_scan_excl<kTileThreads>(sum, scan_w, total);
-    for (int f = f0; f < f1; ++f) { const int c = pre[f]; pre[f] = run; run += c; }
-    if (t == 0) pre[F] = total;
-    __syncthreads();
-    for (int v = blockIdx.x; v < total; v += gridDim.x) {
-        int lo = 0, hi = F;                            // largest f with pre[f] <= v
-        while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (pre[mid] <= v) lo = mid; else hi = mid; }
-        const int tile = r.sys_tile[sys[lo]] + (v - pre[lo]);
-        tile_neighbors<RIGID, STAGE, true>(r, tile, L, nbr_l);
+    const int T = *r.list_count;
+    for (int v = blockIdx.x; v < T; v += gridDim.x) {
+        tile_neighbors<RIGID, true>(r, r.list[v], L);
         __syncthreads();
     }
 }
@@ -492,11 +470,13 @@ __global__ void __launch_bounds__(kTileThreads) k_neighbors_listed(RebuildArgs r
 // systems only.  The workgroup sorts its system through LDS: cell counts with LDS atomics,
 // the scan of the system's cells, an LDS scatter followed by a per-cell insertion sort
 // (ascending current index: the same order as the device-wide chain), the permutation of
-// every live array staged through LDS, static_index and the displacement reference; it then
-// lists its system for k_neighbors_listed, which builds the lists of the listed systems' cell
-// tiles on many workgroups (one workgroup per system spent most of its time there).
+// every live array (through registers, or LDS for the largest systems), static_index and the
+// displacement reference; it then lists its system's cell tiles for k_neighbors_listed, which
+// builds their lists on many workgroups (one workgroup per system spent most of its time there).
 constexpr int kSysLdsBytes = 156 * 1024;
 constexpr int kSysThreads = 1024;
+constexpr int kSysRegArrays = 4;   // arrays permuted per round through registers
+constexpr int kSysRegPos = 5;      // positions per thread of that path (systems up to 5 * 1024 objects)
 
 template <bool RIGID>
 __global__ void __launch_bounds__(kSysThreads) k_rebuild_sys(RebuildArgs r, const int* active) {
@@ -557,31 +537,69 @@ __global__ void __launch_bounds__(kSysThreads) k_rebuild_sys(RebuildArgs r, cons
     for (int p = t; p < n; p += W) cid[ord[p]] = p;   // inverse permutation
     __syncthreads();
     for (int q = t; q < n; q += W) r.static_index[b + q] = b + cid[r.static_index[b + q] - b];
-    for (int k = 0; k < r.pa.nd; ++k) {
-        double* d = r.pa.d[k];
-        for (int p = t; p < n; p += W) stage[p] = d[b + ord[p]];
-        __syncthreads();
-        double* l = (k == 0) ? r.lx : (k == 1 ? r.ly : nullptr);
-        for (int p = t; p < n; p += W) {
-            d[b + p] = stage[p];
-            if (l) l[b + p] = stage[p];
+    if (n <= kSysRegPos * W) {
+        // kSysRegArrays arrays at a time: every thread gathers its positions' values of the
+        // group into registers, a barrier, then the stores (the permutation is in place)
+        for (int k0 = 0; k0 < r.pa.nd; k0 += kSysRegArrays) {
+            const bool last = k0 + kSysRegArrays >= r.pa.nd;
+            double v[kSysRegArrays][kSysRegPos];
+            int w[kSysRegPos];
+#pragma unroll
+            for (int u = 0; u < kSysRegPos; ++u) {
+                const int p = t + u * W;
+                const int o = p < n ? b + ord[p] : b;
+#pragma unroll
+                for (int k = 0; k < kSysRegArrays; ++k)
+                    v[k][u] = (p < n && k0 + k < r.pa.nd) ? r.pa.d[k0 + k][o] : 0.0;
+                w[u] = (last && p < n && r.pa.ni > 1) ? r.pa.i[1][o] : 0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < kSysRegPos; ++u) {
+                const int p = t + u * W;
+                if (p >= n) continue;
+#pragma unroll
+                for (int k = 0; k < kSysRegArrays; ++k) {
+                    if (k0 + k >= r.pa.nd) continue;
+                    r.pa.d[k0 + k][b + p] = v[k][u];
+                    if (k0 + k == 0 && r.lx) r.lx[b + p] = v[k][u];
+                    if (k0 + k == 1 && r.ly) r.ly[b + p] = v[k][u];
+                }
+                if (last && r.pa.ni > 1) r.pa.i[1][b + p] = w[u];
+            }
+            __syncthreads();
         }
-        __syncthreads();
-    }
-    for (int k = 1; k < r.pa.ni; ++k) {
-        int* d = r.pa.i[k];
-        int* st = (int*)stage;
-        for (int p = t; p < n; p += W) st[p] = d[b + ord[p]];
-        __syncthreads();
-        for (int p = t; p < n; p += W) d[b + p] = st[p];
-        __syncthreads();
+    } else {
+        for (int k = 0; k < r.pa.nd; ++k) {
+            double* d = r.pa.d[k];
+            for (int p = t; p < n; p += W) stage[p] = d[b + ord[p]];
+            __syncthreads();
+            double* l = (k == 0) ? r.lx : (k == 1 ? r.ly : nullptr);
+            for (int p = t; p < n; p += W) {
+                d[b + p] = stage[p];
+                if (l) l[b + p] = stage[p];
+            }
+            __syncthreads();
+        }
+        for (int k = 1; k < r.pa.ni; ++k) {
+            int* d = r.pa.i[k];
+            int* st = (int*)stage;
+            for (int p = t; p < n; p += W) st[p] = d[b + ord[p]];
+            __syncthreads();
+            for (int p = t; p < n; p += W) d[b + p] = st[p];
+            __syncthreads();
+        }
     }
     if (RIGID) {
         __syncthreads();
         for (int i = r.psys_off[s] + t; i < r.psys_off[s + 1]; i += W) ph_rigid_reset(r, i);
     }
     // the neighbor kernel takes this system's tiles from the list
-    if (t == 0) r.list[atomicAdd(r.list_count, 1)] = s;
+    __shared__ int list_base;
+    const int tb = r.sys_tile[s], nts = r.sys_tile[s + 1] - tb;
+    if (t == 0) list_base = atomicAdd(r.list_count, nts);
+    __syncthreads();
+    for (int k = t; k < nts; k += W) r.list[list_base + k] = tb + k;
 }
 
 __global__ void k_tile_sum(RebuildArgs r, const int* active) {
@@ -734,16 +752,9 @@ void rebuild_multi(dpmd_ctx* c, const RebuildArgs& r, const int* active, int cou
     }
     k_scatter_rank<<<blocks_for(n), DPMD_BLOCK, 0, st>>>(r, active);
     k_sort_gather<<<blocks_for(C), DPMD_BLOCK, 0, st>>>(r, active);
-    const bool stage = c->K <= kTileStageMaxK;
-    const size_t dyn = stage ? (size_t)c->K * kTileThreads * sizeof(unsigned short) : 0;
     need(r.ntiles > 0, "rebuild: no cell tiles (cells not set up)");
-    if (c->kind == DPMD_RIGID_BUMPY) {
-        if (stage) k_neighbors_tiled<true, true><<<r.ntiles, kTileThreads, dyn, st>>>(r, active);
-        else k_neighbors_tiled<true, false><<<r.ntiles, kTileThreads, 0, st>>>(r, active);
-    } else {
-        if (stage) k_neighbors_tiled<false, true><<<r.ntiles, kTileThreads, dyn, st>>>(r, active);
-        else k_neighbors_tiled<false, false><<<r.ntiles, kTileThreads, 0, st>>>(r, active);
-    }
+    if (c->kind == DPMD_RIGID_BUMPY) k_neighbors_tiled<true><<<r.ntiles, kTileThreads, 0, st>>>(r, active);
+    else k_neighbors_tiled<false><<<r.ntiles, kTileThreads, 0, st>>>(r, active);
     if (r.plx) k_rigid_reset<<<blocks_for(r.N), DPMD_BLOCK, 0, st>>>(r, active);
     k_clear_flags<<<blocks_for(r.S), DPMD_BLOCK, 0, st>>>(r.S, r.ca.flag, c->run.p, nullptr, count);
 }
@@ -758,7 +769,7 @@ bool sys_rebuild_fits(dpmd_ctx* c) {
 
 bool use_sys_rebuild(dpmd_ctx* c) {
     if (c->rebuild_engine == 2) return false;
-    const bool fits = sys_rebuild_fits(c) && c->S <= kListMax;
+    const bool fits = sys_rebuild_fits(c);
     if (c->rebuild_engine == 1)
         need(fits, "DPMD_OPT_REBUILD_ENGINE = 1: a system is too large for the per-system workgroup rebuild");
     return fits;
@@ -777,9 +788,9 @@ void build_tiles(dpmd_ctx* c, const std::vector<int>& cdx, const std::vector<int
         mo = std::max(mo, n);
         mc = std::max(mc, nx * ny);
         if (nx <= 0 || ny <= 0) continue;
-        // row segments of about 200 objects (one per thread of a tile workgroup)
+        // row segments of about kTileObjects objects (at most one per thread of a tile workgroup)
         const double rho = (double)n / ((double)nx * ny);
-        int tw = rho > 0.0 ? (int)(200.0 / rho) : nx;
+        int tw = rho > 0.0 ? (int)(kTileObjects / rho) : nx;
         tw = std::max(1, std::min(tw, std::min(nx, kTileMaxCells)));
         const int nseg = (nx + tw - 1) / tw;
         for (int cy = 0; cy < ny; ++cy)
@@ -795,7 +806,7 @@ void build_tiles(dpmd_ctx* c, const std::vector<int>& cdx, const std::vector<int
     if (tl.empty()) tl.assign(4, 0);
     c->tiles.upload(tl.data(), (int64_t)tl.size(), c->st);
     c->sys_tile.upload(st.data(), c->S + 1, c->st);
-    c->sys_list.resize(std::max(1, c->S));
+    c->sys_list.resize(std::max(1, c->ntiles));          // the per-system engine's tile list
     if (c->sys_list_n.n == 0) { c->sys_list_n.resize(1); c->sys_list_n.zero(c->st); }
     c->max_sys_obj = mo;
     c->max_sys_cells = mc;
@@ -851,17 +862,13 @@ void rebuild_impl(dpmd_ctx* c, const int* active, bool force_all, bool in_step, 
     if (force_all) k_set_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, c->flag.p, active);
     RebuildArgs r = make_rebuild_args(c, in_step);
     if (use_sys_rebuild(c)) {
-        const bool stage = c->K <= kTileStageMaxK;
-        const size_t dyn = stage ? (size_t)c->K * kTileThreads * sizeof(unsigned short) : 0;
         const int grid = std::max(1, std::min(r.ntiles, kListGrid));
         if (c->kind == DPMD_RIGID_BUMPY) {
             k_rebuild_sys<true><<<S, kSysThreads, 0, st>>>(r, active);
-            if (stage) k_neighbors_listed<true, true><<<grid, kTileThreads, dyn, st>>>(r, active);
-            else k_neighbors_listed<true, false><<<grid, kTileThreads, 0, st>>>(r, active);
+            k_neighbors_listed<true><<<grid, kTileThreads, 0, st>>>(r, active);
         } else {
             k_rebuild_sys<false><<<S, kSysThreads, 0, st>>>(r, active);
-            if (stage) k_neighbors_listed<false, true><<<grid, kTileThreads, dyn, st>>>(r, active);
-            else k_neighbors_listed<false, false><<<grid, kTileThreads, 0, st>>>(r, active);
+            k_neighbors_listed<false><<<grid, kTileThreads, 0, st>>>(r, active);
         }
         k_clear_flags<<<blocks_for(S), DPMD_BLOCK, 0, st>>>(S, r.ca.flag, c->run.p, r.list_count, count);
     } else {

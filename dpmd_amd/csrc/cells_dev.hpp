@@ -25,7 +25,7 @@ struct RebuildArgs {
     // cell row, first cell, one past the last cell of the row segment).
     const int* tiles; int ntiles;
     const int* sys_tile;               // tiles of system s: [sys_tile[s], sys_tile[s + 1])
-    int* list; int* list_count;        // systems rebuilt by the per-system engine
+    int* list; int* list_count;        // tiles of the systems the per-system engine rebuilt
 };
 
 __device__ __forceinline__ void ph_zero(const RebuildArgs& r, int c) {

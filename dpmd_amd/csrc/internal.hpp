@@ -133,7 +133,7 @@ struct dpmd_ctx {
     dpmd::DVec<int> tiles;      // 4 ints per cell-row tile (system, row, first cell, end cell)
     int ntiles = 0;
     dpmd::DVec<int> sys_tile;   // S+1 tile offsets per system
-    dpmd::DVec<int> sys_list, sys_list_n;   // systems rebuilt by the per-system engine, and their count
+    dpmd::DVec<int> sys_list, sys_list_n;   // tiles of the systems the per-system engine rebuilt, and their count
     int max_sys_obj = 0, max_sys_cells = 0;   // largest system (objects, cells)
     // rebuild engine: 0 per-system workgroups when every system fits, else the device-wide
     // chain; 1 per-system workgroups; 2 device-wide chain (DPMD_OPT_REBUILD_ENGINE)
@@ -374,6 +374,10 @@ void rigid_fire_drift(dpmd_ctx* c, Gate g);
 void rigid_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, double a, double b);
 void rigid_vv_drift(dpmd_ctx* c, Gate g);
 void rigid_vv_force(dpmd_ctx* c, Gate g);
+// Forces of the step at the current vertex positions, its closing half kick and the next
+// step's opening half kick and drift (one launch pair; rigid_vv_fusable must hold).
+bool rigid_vv_fusable(dpmd_ctx* c);
+void rigid_vv_fused(dpmd_ctx* c, Gate g);
 void rigid_damping(dpmd_ctx* c, const double* scale);
 
 // reduce.hip

@@ -507,6 +507,87 @@ k_particle_v(RigidArgs a, const double* __restrict__ dth, const double* __restri
     }
 }
 
+// Velocity Verlet across a step boundary for G particles: k_particle_v's sums and closing half
+// kick, then k_vv_drift_v's opening half kick, drift, vertex rotation and skin test, each in
+// the same expressions and order, so a fused run equals the two-launch run bit for bit.  Each
+// vertex position is loaded once, for its torque term and its rotation; a workgroup reads and
+// writes only its own particles' vertices, so they are updated in place.
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_vv_fused_v(RigidArgs a, const double* __restrict__ dt, const double* __restrict__ dth,
+             const double* __restrict__ damp, int G, Gate g) {
+    if (!gate_open(g)) return;
+    __shared__ double l_fx[DPMD_BLOCK], l_fy[DPMD_BLOCK], l_pe[DPMD_BLOCK], l_tq[DPMD_BLOCK];
+    __shared__ double l_x0[kRigidGroupMax], l_y0[kRigidGroupMax], l_nx[kRigidGroupMax], l_ny[kRigidGroupMax];
+    __shared__ double l_cs[kRigidGroupMax], l_sn[kRigidGroupMax];
+    __shared__ int l_off[kRigidGroupMax + 1];
+    const int p0 = blockIdx.x * G;
+    const int np = min(G, a.n - p0);
+    const int t = threadIdx.x;
+    if (t <= np) l_off[t] = a.poff[p0 + t];
+    __syncthreads();
+    const int c = l_off[0] + t;
+    const bool valid = c < l_off[np];
+    int q = 0, v = 0;
+    double vx0 = 0.0, vy0 = 0.0;
+    if (valid) {
+        while (l_off[q + 1] <= c) ++q;
+        const int i = p0 + q;
+        v = a.vstatic[c];
+        const double fvx = a.vfx[v], fvy = a.vfy[v];
+        vx0 = a.vpx[v]; vy0 = a.vpy[v];
+        const double dx = vx0 - a.x[i], dy = vy0 - a.y[i];
+        l_fx[t] = fvx; l_fy[t] = fvy; l_pe[t] = a.vpe[v];
+        l_tq[t] = fvy * dx - fvx * dy;
+    }
+    __syncthreads();
+    if (t < np) {
+        const int i = p0 + t;
+        const int s = a.sid[i];
+        double fx = 0.0, fy = 0.0, tq = 0.0, pe = 0.0;
+        for (int k = l_off[t] - l_off[0]; k < l_off[t + 1] - l_off[0]; ++k) {
+            fx += l_fx[k];
+            fy += l_fy[k];
+            pe += l_pe[k];
+            tq += l_tq[k];
+        }
+        double vx = a.vx[i], vy = a.vy[i], w = a.w[i];
+        if (damp) {
+            const double sc = damp[s];
+            fx += -vx * sc;
+            fy += -vy * sc;
+            tq += -w * sc;
+        }
+        a.fx[i] = fx; a.fy[i] = fy; a.tq[i] = tq; a.pe[i] = pe;
+        kick_one(a, i, dth[s] * 1.0, fx, fy, tq, vx, vy, w);     // end of this step
+        kick_one(a, i, dth[s] * 1.0, fx, fy, tq, vx, vy, w);     // next step: update_velocities(dt_half)
+        const double sc = dt[s] * 1.0;                           // update_positions(dt)
+        const double x0 = a.x[i], y0 = a.y[i], a0 = a.th[i];
+        const double nx = x0 + vx * sc, ny = y0 + vy * sc, na = a0 + w * sc;
+        double sn, cs;
+        own_sincos(na - a0, &sn, &cs);
+        a.x[i] = nx; a.y[i] = ny; a.th[i] = na; a.vx[i] = vx; a.vy[i] = vy; a.w[i] = w;
+        l_x0[t] = x0; l_y0[t] = y0; l_nx[t] = nx; l_ny[t] = ny; l_cs[t] = cs; l_sn[t] = sn;
+    }
+    __syncthreads();
+    int s = -1;
+    bool flag = false;
+    if (valid) {
+        const int i = p0 + q;
+        s = a.sid[i];
+        const double dx = vx0 - l_x0[q], dy = vy0 - l_y0[q];
+        const double cs = l_cs[q], sn = l_sn[q];
+        const double rx = cs * dx - sn * dy, ry = sn * dx + cs * dy;
+        const double vxn = l_nx[q] + rx, vyn = l_ny[q] + ry;
+        a.vpx[v] = vxn;
+        a.vpy[v] = vyn;
+        if (a.cell && c == l_off[q]) {   // the skin test uses the particle's first vertex
+            const double ddx = vxn - a.lx[i], ddy = vyn - a.ly[i];
+            flag = (ddx * ddx + ddy * ddy) > a.thresh2[s];
+        }
+    }
+    if (a.cell) raise_flags(a.flag, a.any, s, flag);
+}
+
 // ---- FIRE with one thread per vertex -------------------------------------------------------
 // k_fire_drift's two rigid-body drifts (update_positions(dt_reverse) and, after the kick and
 // mixing, update_positions(dt / 2)) for G particles: the centre, angle, kick and mixing on G
@@ -671,6 +752,16 @@ void rigid_vv_drift(dpmd_ctx* c, Gate g) {
         k_vv_drift_v<<<(c->N + G - 1) / G, DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vv_dt.p, c->vv_dth.p, G, g);
     else
         k_vv_drift<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vv_dt.p, c->vv_dth.p, g);
+    HIP_OK(hipGetLastError());
+}
+// Fused velocity Verlet needs the one-thread-per-vertex form.
+bool rigid_vv_fusable(dpmd_ctx* c) { return rigid_group(c) != 0; }
+void rigid_vv_fused(dpmd_ctx* c, Gate g) {
+    const int G = rigid_group(c);
+    need(G != 0, "rigid fused velocity Verlet: a particle has more vertices than a workgroup holds");
+    launch_vertex_force(c, c->vv_mode, g);
+    k_vv_fused_v<<<(c->N + G - 1) / G, DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vv_dt.p, c->vv_dth.p, vv_damping(c),
+                                                              G, g);
     HIP_OK(hipGetLastError());
 }
 void rigid_vv_force(dpmd_ctx* c, Gate g) {

@@ -7,7 +7,7 @@ import pytest
 import oracle as orc
 from dpmd_amd import _lib as L
 from dpmd_amd import workloads as W
-from dpmd_amd.system import FIRE, RigidBumpy, VelocityVerlet, run_fire
+from dpmd_amd.system import FIRE, DampedVelocityVerlet, RigidBumpy, VelocityVerlet, run_fire
 
 pytestmark = pytest.mark.gpu
 
@@ -110,3 +110,41 @@ def test_rigid_n_dof_counts_only_bodies_with_inertia():
     nd = [int(np.sum(2 + (b.inertia[b.off[s]:b.off[s + 1]] > 0))) for s in range(b.S)]
     assert list(P.get("N_DOF")) == nd == list(ref.n_dof())
     assert np.array_equal(P.get("TEMPERATURE"), ref.temperature())
+
+
+@pytest.mark.parametrize("damped", [0, 1])
+@pytest.mark.parametrize("graph", [True, False])
+def test_rigid_fused_vv_equals_two_launch_vv(graph, damped):
+    """Fused rigid velocity-Verlet batches (one launch closes a step and opens the next) against
+    one drift and one force launch per step and against the oracle, hot enough that rebuilds
+    halt batches on many steps; plain and damped."""
+    b = small(1, seed=4)
+    rng = np.random.default_rng(6)
+    vel = {f: rng.standard_normal(b.N) * sc for f, sc in (("vx", 0.08), ("vy", 0.08), ("w", 0.2))}
+    key = {"vx": "VEL_X", "vy": "VEL_Y", "w": "ANGULAR_VEL"}
+    runs = []
+    for fused in (1, 0):
+        P = RigidBumpy()
+        W.load_rigid_into(P, b)
+        L.check(L.lib().dpmd_set_option(P.ctx, L.OPT_FUSED_VV, fused), P.ctx)
+        P.init_neighbors()
+        for f, v in vel.items():
+            P.set(key[f], v)
+        vv = DampedVelocityVerlet(P, 1e-2, 0.3, wall=True) if damped else VelocityVerlet(P, 1e-2, wall=True)
+        vv.init()
+        vv.run(300, use_graph=graph)
+        runs.append(P)
+    ref = orc.OracleRigid(b)
+    for f, v in vel.items():
+        ref.set(f, v)
+    if damped:
+        ref.damped_vv_init(1e-2, 0.3, 1)
+    else:
+        ref.vv_init(1e-2, 1)
+    ref.vv_run(300, 1)
+    fields = ("x", "y", "angle", "vx", "vy", "w", "fx", "fy", "torque", "pe", "vpx", "vpy")
+    for f in fields:
+        g0 = runs[0].get(PNAMES.get(f) or VNAMES[f])
+        g1 = runs[1].get(PNAMES.get(f) or VNAMES[f])
+        assert np.array_equal(g0, g1), f
+    assert_same(runs[0], ref, fields, "rigid fused VV vs oracle")
