@@ -67,6 +67,9 @@ def parse():
                     help="jam512: skip the autotuner (untimed) and run the library defaults")
     ap.add_argument("--tune-cache", default=None,
                     help="jam512: JSON autotune cache to read (and extend); default: tune afresh")
+    ap.add_argument("--rigid-fire-cap", type=int, default=20000,
+                    help="--workload rigid: FIRE step cap (shorter for counter passes)")
+    ap.add_argument("--rigid-nve", type=int, default=10000, help="--workload rigid: NVE steps")
     ap.add_argument("--no-legs", action="store_true",
                     help="default workload: skip the bounded batched / rigid / adjoint legs of the line")
     return ap.parse_args()
@@ -322,9 +325,11 @@ def run_rigid(args):
         del Q
     P = load()
     log(f"rigid: S={b.S} N={b.N} vertices={b.Nv}")
-    n_nve, n_prof = 10000, 500
+    n_nve = max(2, args.rigid_nve)
+    n_prof = min(500, n_nve // 2)
     t0 = time.perf_counter()
-    steps, conv = run_fire(P, 1e-2, 20000, L.FIRE_PRED_MAX_FORCE, 1e-9, mode=L.FORCE_WALL, use_graph=not args.no_graph)
+    steps, conv = run_fire(P, 1e-2, args.rigid_fire_cap, L.FIRE_PRED_MAX_FORCE, 1e-9, mode=L.FORCE_WALL,
+                           use_graph=not args.no_graph)
     P.synchronize()
     t_fire = time.perf_counter() - t0
     fire_work = int(P.get("FIRE_STEPS").astype(np.int64).sum()) * 256
@@ -364,7 +369,8 @@ def run_rigid(args):
         "steps": steps + n_nve, "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "256 wall-box systems x 256 rigid bumpy polygons (n_v 12..26, R 0.5/0.7), phi=0.70; "
-                               "FIRE (max|F|<1e-9, cap 20000) then 1e4 NVE steps dt=1e-2 at T=1e-4",
+                               f"FIRE (max|F|<1e-9, cap {args.rigid_fire_cap}) then {n_nve} NVE steps dt=1e-2 "
+                               "at T=1e-4",
                    "n_particles": b.N, "n_vertices": b.Nv, "fire_steps": steps, "fire_converged": bool(conv),
                    "fire_seconds": t_fire, "fire_particle_steps": fire_work, "nve_seconds": t_nve,
                    "nve_particle_steps_per_s": rate_nve, "nve_vertex_steps_per_s": b.Nv * n_nve / t_nve,

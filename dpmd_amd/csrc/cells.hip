@@ -249,12 +249,15 @@ struct OwnLoad {
 };
 
 template <bool INPLACE>
-__device__ __forceinline__ void own_load(const RebuildArgs& r, int i, int n, const int* gcell, OwnLoad& q) {
+__device__ __forceinline__ void own_load(const RebuildArgs& r, int i, int n, const int* gcell, OwnLoad& q,
+                                         bool staged) {
     constexpr int MD = 10, MI = 2;
     q.cell = gcell[i];
     if (INPLACE) return;
+    // staged: x, y and the radius come from the object's own LDS record instead
 #pragma unroll
-    for (int k = 0; k < MD; ++k) q.v[k] = k < r.pa.nd ? r.pa.tmp_d[(size_t)k * n + i] : 0.0;
+    for (int k = 0; k < MD; ++k)
+        q.v[k] = k < r.pa.nd && !(staged && (k == 0 || k == 1 || k == r.ird)) ? r.pa.tmp_d[(size_t)k * n + i] : 0.0;
 #pragma unroll
     for (int k = 0; k < MI; ++k) q.w[k] = k < r.pa.ni ? r.pa.tmp_i[(size_t)k * n + i] : 0;
     q.si = r.static_index[i];
@@ -299,11 +302,14 @@ struct TileLds {
 // stops at the count).
 template <bool RIGID, bool INPLACE>
 __device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L) {
-    const int* tl = r.tiles + 4 * tile;
-    const int s = tl[0], cy = tl[1], cx0 = tl[2], cx1 = tl[3];
+    const int4* tl = (const int4*)(r.tiles + kTileRec * tile);
+    const int4 ta = tl[0], tb = tl[1];
+    const int s = ta.x, cy = ta.y, cx0 = ta.z, cx1 = ta.w;
+    const int nx = tb.x, ny = tb.y, cs = tb.z;
     const int t = threadIdx.x;
     const int n = r.o.n, K = r.nb.K;
-    const int nx = r.ca.cdx[s], ny = r.ca.cdy[s], cs = r.ca.cstart[s];
+    // the box and cutoff are needed only by the tests; their loads go out now
+    const double Lx = r.ca.Lx[s], Ly = r.ca.Ly[s], iLx = r.ca.iLx[s], iLy = r.ca.iLy[s], cut0 = r.ca.cutoff[s];
     const int* cst = r.ca.cell_start;
     const double* gx = INPLACE ? r.o.x : r.pa.tmp_d;
     const double* gy = INPLACE ? r.o.y : r.pa.tmp_d + (size_t)n;
@@ -336,7 +342,7 @@ __device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L) {
     const int o0 = L.hg0[w + 1], o1 = L.hg0[2 * w - 2] + L.hcnt[2 * w - 2];
     const int own_slot0 = L.hbs[w + 1];
     OwnLoad q;
-    if (o0 + t < o1) own_load<INPLACE>(r, o0 + t, n, gcell, q);
+    if (o0 + t < o1) own_load<INPLACE>(r, o0 + t, n, gcell, q, staged);
     if (staged) {
         // The halo cells u = [0, b1), [b1, b2), [b2, w) of a row are consecutive cells of the
         // grid (the row wraps in x at most twice), so each of these pieces is one object range
@@ -369,7 +375,6 @@ __device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L) {
             if (t + k * kTileThreads < total) L.hc[t + k * kTileThreads] = cq[k];
     }
     __syncthreads();
-    const double Lx = r.ca.Lx[s], Ly = r.ca.Ly[s], iLx = r.ca.iLx[s], iLy = r.ca.iLy[s], cut0 = r.ca.cutoff[s];
     for (int ob = o0; ob < o1; ob += kTileThreads) {
         const int i = ob + t;
         const bool mine = i < o1;
@@ -382,6 +387,9 @@ __device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L) {
                 const TileCand& me = L.hc[own_slot0 + (i - o0)];
                 xi = me.x; yi = me.y; ri = me.r;
                 if (RIGID) pi = me.p;
+                q.v[0] = xi; q.v[1] = yi;
+#pragma unroll
+                for (int k = 2; k < 10; ++k) if (k == r.ird) q.v[k] = ri;
             } else {
                 xi = gx[i]; yi = gy[i]; ri = gr[i];
                 if (RIGID) pi = gpid[i];
@@ -436,7 +444,7 @@ __device__ void tile_neighbors(const RebuildArgs& r, int tile, TileLds& L) {
             const int have = min(min(cnt, K), kStageRows);
             for (int k = 0; k < have; ++k) r.nb.nbr[(size_t)k * n + i] = L.hc[L.nbr_l[k * kTileThreads + t]].j;
         }
-        if (i + kTileThreads < o1) own_load<INPLACE>(r, i + kTileThreads, n, gcell, q);   // next round
+        if (i + kTileThreads < o1) own_load<INPLACE>(r, i + kTileThreads, n, gcell, q, staged);   // next round
     }
 }
 
@@ -445,8 +453,11 @@ template <bool RIGID>
 __global__ void __launch_bounds__(kTileThreads) k_neighbors_tiled(RebuildArgs r, const int* active) {
     __shared__ TileLds L;
     if (!*active) return;
-    if (!r.ca.flag[r.tiles[4 * blockIdx.x]]) return;
-    tile_neighbors<RIGID, false>(r, blockIdx.x, L);
+    // each XCD takes one contiguous range of tiles, so the rows above and below a tile are
+    // mostly staged by the same XCD, from its L2
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    if (!r.ca.flag[r.tiles[kTileRec * tile]]) return;
+    tile_neighbors<RIGID, false>(r, tile, L);
 }
 
 // Per-system engine: the tiles that the sort kernel listed (every tile of each system it
@@ -458,9 +469,11 @@ __global__ void __launch_bounds__(kTileThreads) k_neighbors_listed(RebuildArgs r
     __shared__ TileLds L;
     if (!*active) return;
     const int T = *r.list_count;
+    int tile = r.list[blockIdx.x];   // in bounds (the grid is at most the tile count); used if < T
     for (int v = blockIdx.x; v < T; v += gridDim.x) {
-        tile_neighbors<RIGID, true>(r, r.list[v], L);
+        tile_neighbors<RIGID, true>(r, tile, L);
         __syncthreads();
+        if (v + (int)gridDim.x < T) tile = r.list[v + gridDim.x];
     }
 }
 
@@ -782,8 +795,8 @@ RebuildArgs rebuild_args(dpmd_ctx* c) { return make_rebuild_args(c, false); }
 void build_tiles(dpmd_ctx* c, const std::vector<int>& cdx, const std::vector<int>& cdy) {
     const std::vector<int>& off = c->kind == DPMD_RIGID_BUMPY ? c->h_voff : c->h_off;
     std::vector<int> tl;
-    int mo = 0, mc = 0;
-    for (int s = 0; s < c->S; ++s) {
+    int mo = 0, mc = 0, cs = 0;
+    for (int s = 0; s < c->S; cs += std::max(0, cdx[s]) * std::max(0, cdy[s]), ++s) {   // cs: first cell
         const int n = off[s + 1] - off[s], nx = cdx[s], ny = cdy[s];
         mo = std::max(mo, n);
         mc = std::max(mc, nx * ny);
@@ -795,15 +808,16 @@ void build_tiles(dpmd_ctx* c, const std::vector<int>& cdx, const std::vector<int
         const int nseg = (nx + tw - 1) / tw;
         for (int cy = 0; cy < ny; ++cy)
             for (int k = 0; k < nseg; ++k) {
-                tl.push_back(s); tl.push_back(cy);
-                tl.push_back((int)((int64_t)k * nx / nseg)); tl.push_back((int)((int64_t)(k + 1) * nx / nseg));
+                const int rec[kTileRec] = {s, cy, (int)((int64_t)k * nx / nseg), (int)((int64_t)(k + 1) * nx / nseg),
+                                           nx, ny, cs, 0};
+                tl.insert(tl.end(), rec, rec + kTileRec);
             }
     }
-    c->ntiles = (int)(tl.size() / 4);
+    c->ntiles = (int)(tl.size() / kTileRec);
     std::vector<int> st(c->S + 1, 0);                   // tiles of system s: [st[s], st[s + 1])
-    for (int k = 0; k < c->ntiles; ++k) st[tl[4 * k] + 1]++;
+    for (int k = 0; k < c->ntiles; ++k) st[tl[kTileRec * k] + 1]++;
     for (int s = 0; s < c->S; ++s) st[s + 1] += st[s];
-    if (tl.empty()) tl.assign(4, 0);
+    if (tl.empty()) tl.assign(kTileRec, 0);
     c->tiles.upload(tl.data(), (int64_t)tl.size(), c->st);
     c->sys_tile.upload(st.data(), c->S + 1, c->st);
     c->sys_list.resize(std::max(1, c->ntiles));          // the per-system engine's tile list

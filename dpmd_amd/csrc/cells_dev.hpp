@@ -9,6 +9,8 @@
 
 namespace dpmd {
 
+constexpr int kTileRec = 8;
+
 struct RebuildArgs {
     ObjArgs o; CellArgs ca; PermArgs pa; NbrArgs nb;
     int* order; int* order_inv; int* static_index;
@@ -21,8 +23,9 @@ struct RebuildArgs {
     // Positions within the permuted arrays: pa.d[0], pa.d[1] are x, y; pa.d[ird] is the
     // radius; pa.i[0] is cell_id and pa.i[1] (rigid bodies) the owning particle.
     int ird;
-    // Row tiles of the cell grids for the tiled neighbor build: 4 ints per tile (system,
-    // cell row, first cell, one past the last cell of the row segment).
+    // Row tiles of the cell grids for the tiled neighbor build: kTileRec ints per tile
+    // (system, cell row, first cell, one past the last cell of the row segment, the system's
+    // cell grid nx, ny and first global cell, padding), so a tile's workgroup needs one load.
     const int* tiles; int ntiles;
     const int* sys_tile;               // tiles of system s: [sys_tile[s], sys_tile[s + 1])
     int* list; int* list_count;        // tiles of the systems the per-system engine rebuilt
