@@ -34,10 +34,19 @@ METRIC = "particle-steps/sec (batched FIRE/NVE) + %HBM roofline; adjoint/forward
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes in profiles/k_vv_fused_disk_N1M_fire_nve.md, each
 # scaled by what the counter reads for a copy kernel of known size (FETCH x 2.0, WRITE x 1.0),
 # at the profiled run's mean neighbor count.  A run reports it scaled to its own count.
-FUSED_TRAFFIC_N1M = (51352.8 * (131072.0 / 65545.5) + 32768.0) * 1024.0
-FUSED_TRAFFIC_K = 5.50
+FUSED_TRAFFIC_N1M = (51455.8 * (131072.0 / 65545.5) + 32768.0) * 1024.0
+FUSED_TRAFFIC_K = 5.46
 FUSED_TRAFFIC_SOURCE = ("profile-derived: profiles/k_vv_fused_disk_N1M_fire_nve.md (FETCH_SIZE x 2.0 + "
                         "WRITE_SIZE x 1.0), neighbor-id bytes scaled to this run's mean count")
+# The same for k_fire_force on that workload (its profiled run's mean count: 5.46), and for
+# k_vertex_force on the 256 x 256 rigid workload (profiles/k_vertex_force_rigid_256x256_fire_nve.md,
+# mean vertex neighbor count 3.77).
+FIRE_FORCE_TRAFFIC_N1M = (41430.9 * (131072.0 / 65545.5) + 41056.0) * 1024.0
+FIRE_FORCE_TRAFFIC_K = 5.46
+RIGID_VF_TRAFFIC = (43563.0 * (131072.0 / 65545.5) + 29164.1) * 1024.0
+RIGID_VF_TRAFFIC_K = 3.77
+RIGID_VF_TRAFFIC_SOURCE = ("profile-derived: profiles/k_vertex_force_rigid_256x256_fire_nve.md (FETCH_SIZE x 2.0 + "
+                           "WRITE_SIZE x 1.0), neighbor-id bytes scaled to this run's mean count")
 TAIL_LAUNCHES = 256   # fused launches timed after the timed region for the roofline
 
 
@@ -377,7 +386,8 @@ def run_rigid(args):
                    "mean_vertex_neighbors": k},
         "roofline": {"bound": "hbm", "kernel": "k_vertex_force (vertex contact forces, walls)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_launch_us": avg_ms * 1e3, "launches": cnt.value,
+                     "traffic": RIGID_VF_TRAFFIC + b.Nv * 4 * (k - RIGID_VF_TRAFFIC_K),
+                     "traffic_source": RIGID_VF_TRAFFIC_SOURCE, "avg_launch_us": avg_ms * 1e3, "launches": cnt.value,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
     if not args.no_cpu:
@@ -662,7 +672,9 @@ def leg_rigid(args):
             "fire_seconds": t_fire, "fire_particle_steps": fire_work, "nve_seconds": t_nve,
             "nve_particle_steps_per_s": b.N * n_nve / t_nve, "nve_us_per_step": t_nve * 1e6 / n_nve,
             "roofline": {"bound": "hbm", "kernel": "k_vertex_force (vertex contacts, walls)", "achieved": achieved,
-                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": RIGID_VF_TRAFFIC + b.Nv * 4 * (k - RIGID_VF_TRAFFIC_K),
+                         "traffic_source": RIGID_VF_TRAFFIC_SOURCE,
                          "avg_launch_us": avg_ms * 1e3, "launches": cnt.value,
                          "algorithmic_bytes_per_launch": bytes_per_launch, "mean_vertex_neighbors": k}}
 
@@ -886,7 +898,12 @@ def main():
                      "copy_kernel_gbs": copy_gbs},
         "fire_force_roofline": {"bound": "hbm", "kernel": "k_fire_force (forces, FIRE kick, chunk partials)",
                                 "achieved": ff_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                "frac": ff_gbs / HBM_PEAK_GBS, "avg_launch_us": ff_avg * 1e3,
+                                "frac": ff_gbs / HBM_PEAK_GBS,
+                                "traffic": (FIRE_FORCE_TRAFFIC_N1M + N * 4 * (k - FIRE_FORCE_TRAFFIC_K))
+                                if N == 1 << 20 else None,
+                                "traffic_source": FUSED_TRAFFIC_SOURCE.replace("neighbor-id", "k_fire_force rows; "
+                                                                               "neighbor-id") if N == 1 << 20 else None,
+                                "avg_launch_us": ff_avg * 1e3,
                                 "launches": ff_cnt.value, "algorithmic_bytes_per_launch": ff_bytes},
     }
     del P
