@@ -67,7 +67,7 @@ def parse():
                          "512-system uneven batch jammed by compression, batched vs one context per system; "
                          "adjoint: gradients through FIRE minima and NVE; single1024: the plain N=1024 "
                          "system, FIRE to 1e-12 then 1e4 NVE steps; rigid: 256 x 256 rigid bumpy polygons")
-    ap.add_argument("--plain-sample", type=int, default=16,
+    ap.add_argument("--plain-sample", type=int, default=32,
                     help="jam512: systems timed on the plain (one context per system) path")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the lean step batches directly instead of replaying hipGraphs "

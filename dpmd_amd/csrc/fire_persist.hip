@@ -27,6 +27,13 @@ namespace {
 // each; 11 per-particle arrays of doubles take 132 KB of the CU's 160 KB.
 constexpr int kLdsMaxN = 1536;
 constexpr int kLdsBlock = 512;
+// That state needs gfx950's LDS; device code built for a GPU with less per workgroup gets an
+// empty LDS-resident kernel, which the host never selects there (lds_engine_fits).
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(__gfx950__)
+constexpr int kLdsArchN = 1;
+#else
+constexpr int kLdsArchN = kLdsMaxN;
+#endif
 
 struct PersistArgs {
     const int* order;   // workgroup b runs system order[b]
@@ -131,7 +138,7 @@ __device__ void lds_store(const LdsState<NL>& L, const DiskArgs& a, int beg, int
 // and it goes back to global memory before a rebuild (which permutes it), is reloaded after,
 // and is stored at the end.  The arithmetic and the order of every sum are the same in both.
 template <int MODE, int W, int NL>
-__global__ void __launch_bounds__(W) k_fire_persist(DiskArgs a, FireArgs f, RedArgs rd, RebuildArgs r, PersistArgs p) {
+__device__ __forceinline__ void fire_persist_body(DiskArgs a, FireArgs f, RedArgs rd, RebuildArgs r, PersistArgs p) {
     constexpr int NW = W / 64;
     // one partial per wavefront of particles (a 256-particle chunk is 4 of them), so the
     // force pass needs a single barrier before the chunk sums are formed
@@ -298,6 +305,29 @@ __global__ void __launch_bounds__(W) k_fire_persist(DiskArgs a, FireArgs f, RedA
     }
 }
 
+template <int MODE, int W, int NL>
+__global__ void __launch_bounds__(W) k_fire_persist(DiskArgs a, FireArgs f, RedArgs rd, RebuildArgs r, PersistArgs p) {
+    if constexpr (NL <= kLdsArchN) fire_persist_body<MODE, W, NL>(a, f, rd, r, p);
+}
+
+// Whether this device gives one workgroup the LDS-resident kernel's static LDS (checked once
+// per device).
+bool lds_engine_fits() {
+    static int fits[64];   // 0 unknown, 1 yes, 2 no
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return false;
+    if (fits[dev] == 0) {
+        int maxl = 0;
+        HIP_OK(hipDeviceGetAttribute(&maxl, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        hipFuncAttributes fa{};
+        HIP_OK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(
+                                             &k_fire_persist<DPMD_FORCE_PBC, kLdsBlock, kLdsMaxN>)));
+        fits[dev] = fa.sharedSizeBytes > 1024 && fa.sharedSizeBytes <= (size_t)maxl ? 1 : 2;
+    }
+    return fits[dev] == 1;
+}
+
 template <class F>
 void persist_by_mode(int mode, F&& f) {
     switch (mode) {
@@ -314,7 +344,7 @@ void launch(dpmd_ctx* c, const PersistArgs& p, int nblocks, int nmax) {
     FireArgs f = fire_args(c);
     RedArgs rd = red_args(c);
     RebuildArgs r = rebuild_args(c);
-    const bool lds = c->persist_lds && nmax <= kLdsMaxN;
+    const bool lds = c->persist_lds && nmax <= kLdsMaxN && lds_engine_fits();
     persist_by_mode(c->fire_mode, [&](auto m) {
         constexpr int M = decltype(m)::value;
         if (lds) {
