@@ -7,8 +7,8 @@ Default line (one GPU's share at its real shape):
     2000 FIRE + 5000 NVE steps; --steps K keeps that 2:5 split.  value = N * (FIRE steps + NVE
     steps) / wall time of the timed region (inputs resident in HBM).
   * roofline: HIP events around the fused velocity-Verlet launches of an untimed tail of at
-    least 256 launches after the timed region (whatever --steps is); fire_force_roofline:
-    events around the FIRE force launches of 300 untimed FIRE steps.
+    least 256 launches after the timed region (whatever --steps is); fire_roofline: events
+    around the fused FIRE launches of 300 untimed FIRE steps.
   * batched / rigid / adjoint: bounded runs of the 512-system uneven jamming batch (batched
     against one context per system), the 256 x 256 rigid bumpy batch (with the vertex-force
     roofline) and the 128 x 1024 differentiation workload (adjoint / forward time).
@@ -38,11 +38,11 @@ FUSED_TRAFFIC_N1M = (51455.8 * (131072.0 / 65545.5) + 32768.0) * 1024.0
 FUSED_TRAFFIC_K = 5.46
 FUSED_TRAFFIC_SOURCE = ("profile-derived: profiles/k_vv_fused_disk_N1M_fire_nve.md (FETCH_SIZE x 2.0 + "
                         "WRITE_SIZE x 1.0), neighbor-id bytes scaled to this run's mean count")
-# The same for k_fire_force on that workload (its profiled run's mean count: 5.46), and for
+# The same for k_fire_fused on that workload (its profiled run's mean count: 5.46), and for
 # k_vertex_force on the 256 x 256 rigid workload (profiles/k_vertex_force_rigid_256x256_fire_nve.md,
 # mean vertex neighbor count 3.77).
-FIRE_FORCE_TRAFFIC_N1M = (41430.9 * (131072.0 / 65545.5) + 41056.0) * 1024.0
-FIRE_FORCE_TRAFFIC_K = 5.46
+FIRE_FUSED_TRAFFIC_N1M = None   # no counter pass of k_fire_fused yet
+FIRE_FUSED_TRAFFIC_K = 5.46
 RIGID_VF_TRAFFIC = (43563.0 * (131072.0 / 65545.5) + 29164.1) * 1024.0
 RIGID_VF_TRAFFIC_K = 3.77
 RIGID_VF_TRAFFIC_SOURCE = ("profile-derived: profiles/k_vertex_force_rigid_256x256_fire_nve.md (FETCH_SIZE x 2.0 + "
@@ -830,8 +830,9 @@ def main():
     lib.dpmd_profile(P.ctx, 0)
     cap, mx, nnb = P.neighbor_info()
     k = nnb / N
-    # FIRE force kernel (forces, kick, power / energy / max|F| chunk partials) in an untimed tail
-    lib.dpmd_profile(P.ctx, 1)
+    # fused FIRE kernel (step k's forces, kick and power / energy / max|F| chunk partials, step
+    # k + 1's drift) in an untimed tail
+    lib.dpmd_profile(P.ctx, 3)
     run_fire(P, 1e-2, 300, L.FIRE_PRED_MAX_FORCE, 0.0, use_graph=False)
     P.synchronize()
     durs = (C.c_float * 4096)()
@@ -842,8 +843,9 @@ def main():
     if len(d):   # launches of a batch that a rebuild halted exit at once (a few us)
         d = d[d > 0.5 * np.percentile(d, 90)]
     kf = P.neighbor_info()[2] / N
-    # k_fire_force per particle: x, y, r, sid, count, k ids, vx, vy, m (60 + 4k) read; fx, fy, pe,
-    # vx, vy (40) written -> 100 + 4k bytes (neighbor gathers and per-chunk partials excluded)
+    # k_fire_fused per particle: x, y (16), vx, vy (16), r (8), count (4), k ids (4k), m (8),
+    # last-rebuild x, y (16) read; the next step's x, y, vx, vy (32) written -> 100 + 4k bytes
+    # (neighbor gathers and per-chunk partials excluded)
     ff_bytes = N * (100 + 4 * kf)
     ff_avg = float(d.mean()) if len(d) else float("nan")
     ff_cnt = C.c_int(len(d))
@@ -896,13 +898,15 @@ def main():
                      "avg_launch_us": avg_ms * 1e3, "launches": cnt.value,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "copy_kernel_gbs": copy_gbs},
-        "fire_force_roofline": {"bound": "hbm", "kernel": "k_fire_force (forces, FIRE kick, chunk partials)",
-                                "achieved": ff_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                "frac": ff_gbs / HBM_PEAK_GBS,
-                                "traffic": (FIRE_FORCE_TRAFFIC_N1M + N * 4 * (k - FIRE_FORCE_TRAFFIC_K))
-                                if N == 1 << 20 else None,
-                                "traffic_source": FUSED_TRAFFIC_SOURCE.replace("neighbor-id", "k_fire_force rows; "
-                                                                               "neighbor-id") if N == 1 << 20 else None,
+        "fire_roofline": {"bound": "hbm",
+                          "kernel": "k_fire_fused (FIRE: forces, kick, chunk partials, next step's guessed drift)",
+                          "achieved": ff_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": ff_gbs / HBM_PEAK_GBS,
+                          "traffic": (FIRE_FUSED_TRAFFIC_N1M + N * 4 * (k - FIRE_FUSED_TRAFFIC_K))
+                          if (N == 1 << 20 and FIRE_FUSED_TRAFFIC_N1M) else None,
+                          "traffic_source": FUSED_TRAFFIC_SOURCE.replace("neighbor-id", "k_fire_fused rows; "
+                                                                         "neighbor-id")
+                          if (N == 1 << 20 and FIRE_FUSED_TRAFFIC_N1M) else None,
                                 "avg_launch_us": ff_avg * 1e3,
                                 "launches": ff_cnt.value, "algorithmic_bytes_per_launch": ff_bytes},
     }

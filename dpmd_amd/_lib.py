@@ -19,7 +19,7 @@ ORDER_CURRENT, ORDER_CANONICAL = 0, 1
 FIRE_PRED_REFERENCE, FIRE_PRED_MAX_FORCE, FIRE_PRED_FORCE_OR_PE = 0, 1, 2
 OPT_COOP_REBUILD, OPT_COOP_IN_GRAPH, OPT_FUSED_VV, OPT_BLOCK, OPT_MAX_BATCH = 0, 1, 2, 3, 4
 OPT_PERSISTENT_FIRE, OPT_PERSIST_BLOCK, OPT_PERSIST_PHASE, OPT_DEVICE_REBUILD, OPT_PERSIST_LDS = 5, 6, 7, 8, 9
-OPT_REBUILD_ENGINE, OPT_JAM_RETRIES = 10, 11
+OPT_REBUILD_ENGINE, OPT_JAM_RETRIES, OPT_FUSED_FIRE = 10, 11, 12
 TUNE_FIRE, TUNE_NVE = 0, 1
 
 STATUS = {0: "OK", 1: "ARGUMENT", 2: "STATE", 3: "HIP", 4: "OVERFLOW"}

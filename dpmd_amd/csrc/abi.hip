@@ -378,10 +378,38 @@ bool fused_vv_batch(dpmd_ctx* c, int m, bool fire) {
 }
 int any_word(int q) { return q % 2 == 0 ? RUN_ANY : RUN_ANY2; }
 
+// Fused FIRE batches (one disk system, even batch sizes): the first drift from the context's
+// arrays into the buffer pair (P0, V0); then per step q < m - 1 one launch for step q's
+// forces, kick and partials plus step q + 1's drift with the scalars F.v > 0 gives (into the
+// other pair), and one that folds the partials, commits the next step's scalars and redoes
+// the drift when the guess was wrong (or writes the final state after a stop); then the
+// forces of step m - 1, which also write the state back, and its fold.  Rebuild tests go to
+// the parity words as in fused velocity-Verlet batches (a redone drift to RUN_FIXW); a halted
+// batch unpacks the halted step's pair and flags the system before the rebuild.
+bool fused_fire_batch(dpmd_ctx* c, int m, bool fire) {
+    return fire && !rigid(c) && c->fuse_fire && c->S == 1 && m >= 2 && m % 2 == 0;
+}
+
 // dev: every step rebuilds the flagged systems on the device between its drift and its
 // forces (device-gated rebuild chain), so the batch never halts; used when rebuilds are
 // frequent (many systems, each rebuilding now and then).
 void lean_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b, bool dev) {
+    if (!dev && fused_fire_batch(c, m, fire)) {
+        double* P[2] = {c->xy0.p, c->xy1.p};
+        double* V[2] = {c->vxy0.p, c->vxy1.p};
+        disk_fire_drift_to(c, Gate{nullptr, c->run.p, 0}, P[0], V[0], any_word(0));
+        for (int q = 0; q + 1 < m; ++q) {
+            Gate g{nullptr, c->run.p, q};
+            g.any = any_word(q);
+            const int nx = (q + 1) % 2, nw = any_word(q + 1);
+            disk_fire_fused(c, g, P[q % 2], V[q % 2], P[nx], V[nx], nw);
+            disk_fire_settle(c, g, pred, a, b, P[q % 2], V[q % 2], P[nx], V[nx], nw);
+        }
+        Gate gf{nullptr, c->run.p, m - 1};
+        gf.any = any_word(m - 1);
+        disk_fire_force_packed(c, gf, P[(m - 1) % 2], V[(m - 1) % 2], pred, a, b);
+        return;
+    }
     if (!dev && fused_vv_batch(c, m, fire)) {
         double* P[2] = {c->xy0.p, c->xy1.p};
         disk_vv_drift_to(c, Gate{nullptr, c->run.p, 0}, P[0], any_word(0));
@@ -462,7 +490,9 @@ std::vector<int64_t> batch_key(dpmd_ctx* c, bool fire, int pred, double a, doubl
     int64_t ab[2];
     std::memcpy(&ab[0], &a, 8);
     std::memcpy(&ab[1], &b, 8);
-    return {fire ? 1 : 0, pred, ab[0], ab[1], c->block, c->fuse_vv ? 1 : 0, c->vv_mode, c->fire_mode,
+    return {fire ? 1 : 0, pred, ab[0], ab[1], c->block, c->fuse_vv ? 1 : 0, c->fuse_fire ? 1 : 0, c->vv_mode,
+            c->fire_mode, P(c->vxy0.p), P(c->vxy1.p), P(c->f_snap.p), P(c->i_snap.p), P(c->f_alpha.p), P(c->f_ngood.p),
+            P(c->f_nbad.p), P(c->f_dtmax.p), P(c->f_dtmin.p), P(c->f_stopped.p), P(c->s_dt.p), P(c->pw_tot.p),
             c->mask_frozen ? 1 : 0, c->N, c->Nv, c->S, c->K, c->method, c->nchunks, P(c->rec_seg), P(vv_damping(c)),
             P(c->x.p), P(c->y.p), P(c->xy0.p), P(c->xy1.p), P(c->vx.p), P(c->vy.p), P(c->fx.p), P(c->fy.p), P(c->pe.p),
             P(c->rad.p), P(c->mass.p), P(c->lx.p), P(c->ly.p), P(c->nn.p), P(c->nbr.p), P(c->run.p), P(c->flag.p),
@@ -538,6 +568,7 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
             if (c->rec_seg)   // recorded positions go to slots total + 1 ... of the segment buffer
                 HIP_OK(hipMemcpyAsync(c->run.p + RUN_REC, &total, sizeof(int), hipMemcpyHostToDevice, c->st));
             const bool fused = !dev && fused_vv_batch(c, bm, fire);
+            const bool fused_fire = !dev && fused_fire_batch(c, bm, fire);
             const int slot = lg + (dev ? 8 : 0);
             if (use_graph) {
                 if (!graphs[slot]) graphs[slot] = capture_batch(c, bm, fire, pred, a, b, dev);
@@ -567,6 +598,13 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
                 since_halt += r.halt_at;
                 if (fused)   // the halted step's positions are in the interleaved buffer P(halt_at mod 2)
                     disk_unpack_positions(c, r.halt_at % 2 == 0 ? c->xy0.p : c->xy1.p);
+                if (fused_fire) {   // and for FIRE its velocities in V(halt_at mod 2); the launches
+                    // of a fused FIRE batch raise only the run words, so the system is flagged here
+                    disk_unpack_pv(c, r.halt_at % 2 == 0 ? c->xy0.p : c->xy1.p,
+                                   r.halt_at % 2 == 0 ? c->vxy0.p : c->vxy1.p);
+                    const unsigned one = 1u;
+                    HIP_OK(hipMemcpyAsync(c->flag.p, &one, sizeof(one), hipMemcpyHostToDevice, c->st));
+                }
                 // finish the halted step: rebuild, forces, bookkeeping
                 rebuild_neighbors(c, active_one(c), false, true);
                 if (fire) {
@@ -577,8 +615,10 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
                 }
                 const int clear[2] = {0, -1};
                 HIP_OK(hipMemcpyAsync(c->run.p + RUN_HALT, clear, sizeof(clear), hipMemcpyHostToDevice, c->st));
-                const int zero = 0;   // the rebuild cleared RUN_ANY; fused batches also use RUN_ANY2
-                HIP_OK(hipMemcpyAsync(c->run.p + RUN_ANY2, &zero, sizeof(int), hipMemcpyHostToDevice, c->st));
+                const int zero[5] = {0, 0, 0, 0, 0};   // the rebuild cleared RUN_ANY; fused batches also
+                // use RUN_ANY2 and (FIRE) RUN_MIS and RUN_FIXW
+                HIP_OK(hipMemcpyAsync(c->run.p + RUN_ANY2, zero, sizeof(int), hipMemcpyHostToDevice, c->st));
+                HIP_OK(hipMemcpyAsync(c->run.p + RUN_MIS, zero, 2 * sizeof(int), hipMemcpyHostToDevice, c->st));
                 r = read_run(c);
                 if (fire && r.running <= 0) done = true;
                 m = std::max(1, std::min(128, since_halt));
@@ -1229,6 +1269,8 @@ int dpmd_ctx_create(dpmd_ctx** out, int kind, void* stream) {
             int one = 1;
             c->one.upload(&one, 1, c->st);
             c->run.resize(RUN_WORDS); c->run.zero(c->st);
+            c->f_snap.resize(4); c->f_snap.zero(c->st);
+            c->i_snap.resize(4); c->i_snap.zero(c->st);
             c->overflow.resize(1); c->overflow.zero(c->st);
             (void)scratch_ptr();   // set-up of the allocation-free entry point on this device
         } catch (...) { delete c; throw; }
@@ -1248,7 +1290,7 @@ int dpmd_ctx_destroy(dpmd_ctx* c) {
                                 &c->st_xy, &c->st_yx, &c->st_yy, &c->x, &c->y, &c->vx, &c->vy, &c->fx, &c->fy,
                                 &c->pe, &c->ke, &c->rad, &c->mass, &c->area, &c->lx, &c->ly, &c->angle, &c->avel,
                                 &c->torque, &c->inertia, &c->sxx, &c->sxy, &c->syx, &c->syy, &c->xy0, &c->xy1,
-                                &c->vpx, &c->vpy,
+                                &c->vxy0, &c->vxy1, &c->f_snap, &c->vpx, &c->vpy,
                                 &c->vfx, &c->vfy, &c->vpe, &c->vrad, &c->vmass, &c->tmp_d, &c->part, &c->f_dt,
                                 &c->f_dtmax, &c->f_dtmin, &c->f_alpha, &c->f_peprev, &c->s_dt, &c->s_alpha,
                                 &c->s_dtrev, &c->s_vscale, &c->vv_dt, &c->vv_dth, &c->sv_x, &c->sv_y, &c->sv_rad,
@@ -1260,7 +1302,7 @@ int dpmd_ctx_destroy(dpmd_ctx* c) {
                              &c->cell_count, &c->cell_start, &c->cell_wptr, &c->cell_sys, &c->order,
                              &c->order_inv, &c->tmp_i, &c->nn, &c->nbr, &c->overflow,
                              &c->scan_part, &c->chunk_sys, &c->chunk_beg, &c->sys_chunk, &c->f_ngood,
-                             &c->f_nbad, &c->f_stopped, &c->f_frozen, &c->f_nsteps, &c->s_ngood, &c->s_nbad, &c->run, &c->one, &c->sv_static,
+                             &c->f_nbad, &c->f_stopped, &c->f_frozen, &c->f_nsteps, &c->s_ngood, &c->s_nbad, &c->run, &c->one, &c->sv_static, &c->i_snap,
                              &c->persist_order, &c->persist_todo, &c->persist_stop};
     for (auto* v : iv) v->release();
     c->flag.release();
@@ -1302,6 +1344,8 @@ int dpmd_allocate_particles(dpmd_ctx* c, int N) {
         for (auto* v : dv) { v->resize(N); v->zero(c->st); }
         c->xy0.resize(2 * (int64_t)N); c->xy0.zero(c->st);
         c->xy1.resize(2 * (int64_t)N); c->xy1.zero(c->st);
+        c->vxy0.resize(2 * (int64_t)N); c->vxy0.zero(c->st);
+        c->vxy1.resize(2 * (int64_t)N); c->vxy1.zero(c->st);
         c->sid.resize(N); c->sid.zero(c->st);
         c->contacts.resize(N); c->contacts.zero(c->st);
         c->nvpp.resize(N); c->nvpp.zero(c->st);
@@ -1939,6 +1983,7 @@ int dpmd_set_option(dpmd_ctx* c, int key, int value) {
                                  "available (only 0 is accepted)");
                 break;
             case DPMD_OPT_FUSED_VV: c->fuse_vv = value != 0; break;
+            case DPMD_OPT_FUSED_FIRE: c->fuse_fire = value != 0; break;
             case DPMD_OPT_BLOCK:
                 need(value == 64 || value == 128 || value == 256 || value == 512,
                      "DPMD_OPT_BLOCK must be 64, 128, 256 or 512");
@@ -1988,6 +2033,7 @@ int dpmd_get_option(dpmd_ctx* c, int key, int* value) {
             case DPMD_OPT_COOP_REBUILD:
             case DPMD_OPT_COOP_IN_GRAPH: *value = 0; break;
             case DPMD_OPT_FUSED_VV: *value = c->fuse_vv ? 1 : 0; break;
+            case DPMD_OPT_FUSED_FIRE: *value = c->fuse_fire ? 1 : 0; break;
             case DPMD_OPT_BLOCK: *value = c->block; break;
             case DPMD_OPT_PERSISTENT_FIRE: *value = c->persist_fire; break;
             case DPMD_OPT_PERSIST_PHASE: *value = c->persist_phase; break;
@@ -2008,7 +2054,7 @@ int dpmd_profile(dpmd_ctx* c, int enable) {
         for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
         c->prof_ev.clear();
         c->prof_n.clear();
-        need(enable >= 0 && enable <= 2, "dpmd_profile: enable must be 0, 1 or 2");
+        need(enable >= 0 && enable <= 3, "dpmd_profile: enable must be 0, 1, 2 or 3");
         c->prof = enable;
     });
 }

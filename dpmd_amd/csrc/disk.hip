@@ -137,20 +137,13 @@ __global__ void __launch_bounds__(B) k_fire_drift(DiskArgs a, FireArgs f, Gate g
     if (a.cell) raise_flags(a.flag, a.any, s, flag);
 }
 
-// The fold of one system's chunk partials in a fixed order (strided sequential accumulators,
-// then the block tree), the stopping test and the next step's FIRE update, by a whole
-// 256-thread workgroup.  A separate launch orders it after every chunk.  (Folding in the last
-// chunk workgroup of k_fire_force instead costs more at 2^20 disks: every workgroup then waits
-// for an agent-scope atomic's round trip before it retires.)
-__device__ __forceinline__ void fire_fold_system(const int* off, const FireArgs& f, const RedArgs& r, int s,
-                                                 int do_conv, int do_update, int pred, double pa, double pb,
-                                                 const Gate& g, double* lds) {
+// The fold of one system's chunk partials in a fixed order: 256 strided sequential
+// accumulators (thread t adds chunks t, t + 256, ... in turn), then the block tree.  Every
+// thread of the 256-thread workgroup gets the power, energy and max |F|^2 totals.
+__device__ __forceinline__ void fold_partials(const RedArgs& r, int s, double* lds, double& ap, double& ape,
+                                              double& af) {
     const int cb = r.sys_chunk[s], ce = r.sys_chunk[s + 1];
-    // thread 0 fetches the system's FIRE scalars now, overlapping the fold below
-    FireSys q{};
-    int was_stopped = 0;
-    if (threadIdx.x == 0) { q = fire_load(f, s); was_stopped = f.stopped[s]; }
-    double ap = 0.0, ape = 0.0, af = 0.0;
+    ap = 0.0; ape = 0.0; af = 0.0;
     // each thread's chunks in sequence, sixteen loads issued ahead of their additions
     constexpr int U = 16;
     for (int k0 = cb + threadIdx.x; k0 < ce; k0 += U * DPMD_BLOCK) {
@@ -172,6 +165,21 @@ __device__ __forceinline__ void fire_fold_system(const int* off, const FireArgs&
     ap = block_sum256(ap, lds);
     ape = block_sum256(ape, lds);
     af = block_max256(af, lds);
+}
+
+// The fold of one system's partials, the stopping test and the next step's FIRE update, by a
+// whole 256-thread workgroup.  A separate launch orders it after every chunk.  (Folding in the
+// last chunk workgroup of k_fire_force instead costs more at 2^20 disks: every workgroup then
+// waits for an agent-scope atomic's round trip before it retires.)
+__device__ __forceinline__ void fire_fold_system(const int* off, const FireArgs& f, const RedArgs& r, int s,
+                                                 int do_conv, int do_update, int pred, double pa, double pb,
+                                                 const Gate& g, double* lds) {
+    // thread 0 fetches the system's FIRE scalars now, overlapping the fold below
+    FireSys q{};
+    int was_stopped = 0;
+    if (threadIdx.x == 0) { q = fire_load(f, s); was_stopped = f.stopped[s]; }
+    double ap, ape, af;
+    fold_partials(r, s, lds, ap, ape, af);
     if (threadIdx.x == 0) {
         // A system that had stopped before this step has now had its velocities zeroed
         // (vscale = 0 after a stop), so every later step leaves it unchanged: freeze it.
@@ -244,6 +252,245 @@ k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_updat
     const int s = blockIdx.x;
     if (fire_frozen(f, s)) return;
     fire_fold_system(off, f, r, s, do_conv, do_update, pred, pa, pb, g, lds);
+}
+
+// ---------------------------------------------------------------------------------------
+// FIRE across step boundaries (runs of one system).  A step's drift needs the system's power
+// F.v, a sum over every particle, so FIRE cannot close a step and open the next in one launch
+// the way velocity Verlet does.  It can guess, though: F.v > 0 (the common case) fixes the
+// next step's dt, alpha, counters, reverse drift (0) and velocity scale (1) from the current
+// scalars alone.  Two launches per step:
+//   k_fire_fused   step k's forces, kick and chunk partials, then step k + 1's drift with the
+//                  guessed scalars into the other buffer pair (positions and velocities
+//                  ping-pong between two interleaved (x, y) and (vx, vy) pairs, so step k's
+//                  inputs survive); it also records the scalars it started from (snapshot);
+//   k_fire_settle  every workgroup folds the partials in the fixed tree and evaluates the
+//                  stopping test and the update from the snapshot (the same bits in each);
+//                  workgroup 0 stores the scalars and commits the next step's; when the guess
+//                  was wrong, all workgroups redo step k's forces and kick from the kept
+//                  inputs and drift with the true scalars, and when the system stopped they
+//                  write step k's final state to the context's arrays.
+// The bookkeeping that other workgroups of k_fire_settle would read while it is written (the
+// running count, the step a stop happened at) is done by the next launch's gate, driven by
+// the run word RUN_MIS (k + 1 after a wrong guess at step k, -(k + 1) after a stop, else 0).
+// Every value goes through the expressions of k_fire_drift, k_fire_force and k_fire_finalize
+// in their order, so a fused run equals the three-launch run bit for bit.  Rebuild tests raise
+// only the run words: W(k + 1) for the guessed drift, RUN_FIXW for a redone one; a halted batch
+// flags the system on the host before the rebuild.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void raise_word(int* w, bool f) {
+    const unsigned long long m = __ballot(f);
+    if (m != 0 && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1) atomicOr(w, 1);
+}
+
+// Gate of a fused FIRE launch of step k (k_fire_fused, k_fire_force_pk).
+__device__ __forceinline__ bool gate_fire_fused(const Gate& g) {
+    if (g.run[RUN_HALT] != 0 || g.run[RUN_RUNNING] <= 0) return false;
+    const int v = g.k > 0 ? g.run[RUN_MIS] : 0;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (g.k > 0 && v == -g.k) {   // the system stopped at step k - 1, whose state is final
+        if (lead) { g.run[RUN_CONV_AT] = g.k - 1; atomicSub(&g.run[RUN_RUNNING], 1); }
+        return false;
+    }
+    const bool miss = g.k > 0 && v == g.k;   // step k's positions come from a redone drift
+    if (g.run[miss ? RUN_FIXW : g.any] != 0) {
+        if (lead) { g.run[RUN_HALT_AT] = g.k; g.run[RUN_HALT] = 1; }
+        return false;
+    }
+    if (miss && lead) g.run[g.any] = 0;   // withdraw what the wrong guess raised
+    return true;
+}
+
+// The first drift of a fused batch (k_fire_drift's body) from the context's arrays into the
+// buffer pair (xo, vo).
+template <int B>
+__global__ void __launch_bounds__(B)
+k_fire_drift_to(DiskArgs a, FireArgs f, double2* __restrict__ xo, double2* __restrict__ vo, Gate g) {
+    if (!gate_open(g)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < a.n;
+    const int s = valid ? a.sid[i] : -1;
+    bool flag = false;
+    if (valid && !fire_frozen(f, s)) {
+        const double dt = f.s_dt[s], alpha = f.s_alpha[s], dtrev = f.s_dtrev[s], vscale = f.s_vscale[s];
+        double x = a.x[i], y = a.y[i], vx = a.vx[i], vy = a.vy[i];
+        const double lx = a.cell ? a.lx[i] : 0.0, ly = a.cell ? a.ly[i] : 0.0, th2 = a.cell ? a.thresh2[s] : 0.0;
+        flag = fire_drift_vals(x, y, vx, vy, a.fx[i], a.fy[i], a.mass[i], a.cell, lx, ly, th2, dt, alpha, dtrev,
+                               vscale);
+        xo[i] = make_double2(x, y); vo[i] = make_double2(vx, vy);
+        if (i == a.off[s]) {
+            f.dt[s] = dt; f.alpha[s] = alpha; f.ngood[s] = f.s_ngood[s]; f.nbad[s] = f.s_nbad[s];
+        }
+    }
+    if (a.cell) raise_flags(a.flag, a.any, s, flag);
+}
+
+// Forces and kick of one particle at the positions xs and velocities vs (k_fire_force's body):
+// returns the kicked velocity, the force and the energy.
+template <int MODE>
+__device__ __forceinline__ void fire_force_kick(const DiskArgs& a, const double2* __restrict__ xs,
+                                                const double2* __restrict__ vs, int i, double dt, double& fx,
+                                                double& fy, double& pe, double& vx, double& vy) {
+    disk_force_mode_pos<MODE>(a, PackedPos{xs}, i, 0, fx, fy, pe);
+    const double2 v = vs[i];
+    vx = v.x; vy = v.y;
+    const double smi = dt * 0.5 / a.mass[i];
+    vx = vx + fx * smi; vy = vy + fy * smi;
+}
+
+// Partials of chunk c (the tree of k_fire_force).
+__device__ __forceinline__ void fire_chunk_partials(const RedArgs& r, int c, double vp, double vpe, double vf2,
+                                                    double* lds) {
+    vp = wave_sum(vp); vpe = wave_sum(vpe); vf2 = wave_max(vf2);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { lds[w] = vp; lds[4 + w] = vpe; lds[8 + w] = vf2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        r.part[c] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+        r.part[r.nch + c] = (lds[4] + lds[5]) + (lds[6] + lds[7]);
+        r.part[2 * r.nch + c] = fmax(fmax(lds[8], lds[9]), fmax(lds[10], lds[11]));
+    }
+}
+
+// One system: chunk c covers particles [256 c, 256 c + 256).
+template <int MODE>
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_fire_fused(DiskArgs a, FireArgs f, RedArgs r, const double2* __restrict__ xs, const double2* __restrict__ vs,
+             double2* __restrict__ xo, double2* __restrict__ vo, double* __restrict__ snap, int* __restrict__ isnap,
+             Gate g) {
+    __shared__ double lds[12];
+    if (!gate_fire_fused(g)) return;
+    const int c = xcd_remap(blockIdx.x, gridDim.x);
+    const int i = c * DPMD_BLOCK + threadIdx.x;
+    // this step's dt (kick) and the next step's scalars as F.v > 0 makes them
+    FireSys q{};
+    q.dt = f.dt[0]; q.alpha = f.alpha[0]; q.ngood = f.ngood[0]; q.nbad = f.nbad[0];
+    q.dtmax = f.dtmax[0]; q.dtmin = f.dtmin[0];
+    if (c == 0 && threadIdx.x == 0) {
+        snap[0] = q.dt; snap[1] = q.alpha; snap[2] = f.peprev[0];
+        isnap[0] = q.ngood; isnap[1] = q.nbad;
+    }
+    const double dt = q.dt;
+    fire_update_core(q, 1.0);
+    double vp = 0.0, vpe = 0.0, vf2 = 0.0;
+    bool flag = false;
+    if (i < a.n) {
+        double fx, fy, pe, vx, vy;
+        fire_force_kick<MODE>(a, xs, vs, i, dt, fx, fy, pe, vx, vy);
+        vp = fx * vx + fy * vy;
+        vpe = pe;
+        vf2 = fx * fx + fy * fy;
+        const double2 p = xs[i];
+        double x = p.x, y = p.y;
+        const double lx = a.cell ? a.lx[i] : 0.0, ly = a.cell ? a.ly[i] : 0.0, th2 = a.cell ? a.thresh2[0] : 0.0;
+        flag = fire_drift_vals(x, y, vx, vy, fx, fy, a.mass[i], a.cell, lx, ly, th2, q.s_dt, q.s_alpha, q.s_dtrev,
+                               q.s_vscale);
+        xo[i] = make_double2(x, y); vo[i] = make_double2(vx, vy);
+    }
+    if (a.cell) raise_word(a.any, flag);
+    fire_chunk_partials(r, c, vp, vpe, vf2, lds);
+}
+
+// After k_fire_fused of step k (gate: nothing halted, the system running, as k_fire_fused saw
+// it; neither word changes during this launch).  anyw: the run word the guessed drift raised.
+template <int MODE>
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_fire_settle(DiskArgs a, FireArgs f, RedArgs r, const double* __restrict__ snap, const int* __restrict__ isnap,
+              int pred, double pa, double pb, const double2* __restrict__ xs, const double2* __restrict__ vs,
+              double2* __restrict__ xo, double2* __restrict__ vo, Gate g, int anyw) {
+    __shared__ double lds[4];
+    __shared__ double sh[5];
+    __shared__ int shi[2];
+    if (g.run[RUN_HALT] != 0 || g.run[RUN_RUNNING] <= 0) return;
+    double ap, ape, af;
+    fold_partials(r, 0, lds, ap, ape, af);
+    if (threadIdx.x == 0) {
+        FireSys q{};
+        q.dt = snap[0]; q.alpha = snap[1]; q.peprev = snap[2]; q.ngood = isnap[0]; q.nbad = isnap[1];
+        q.dtmax = f.dtmax[0]; q.dtmin = f.dtmin[0];
+        const double kick_dt = q.dt;
+        FireSys guess = q;
+        fire_update_core(guess, 1.0);
+        fire_convergence_core(q, a.off[1] - a.off[0], ape, af, pred, pa, pb);
+        const bool stop = q.dt == 0.0;   // fire_count_stopped (the system was running)
+        fire_update_core(q, ap);
+        const bool miss = stop || q.s_dt != guess.s_dt || q.s_alpha != guess.s_alpha ||
+                          q.s_dtrev != guess.s_dtrev || q.s_vscale != guess.s_vscale ||
+                          q.s_ngood != guess.s_ngood || q.s_nbad != guess.s_nbad;
+        if (blockIdx.x == 0) {
+            r.pw_tot[0] = ap; r.pe_tot[0] = ape; r.maxf2[0] = af;
+            f.peprev[0] = q.peprev;
+            f.s_dt[0] = q.s_dt; f.s_alpha[0] = q.s_alpha; f.s_ngood[0] = q.s_ngood; f.s_nbad[0] = q.s_nbad;
+            f.s_dtrev[0] = q.s_dtrev; f.s_vscale[0] = q.s_vscale;
+            if (stop) {             // k_fire_finalize's state after a stop; no drift follows
+                f.dt[0] = q.dt; f.stopped[0] = 1;
+                f.run[anyw] = 0;
+            } else {                // what k_fire_drift commits at the start of the next step
+                f.dt[0] = q.s_dt; f.alpha[0] = q.s_alpha; f.ngood[0] = q.s_ngood; f.nbad[0] = q.s_nbad;
+            }
+            f.nsteps[0] += 1;
+            f.run[RUN_MIS] = stop ? -(g.k + 1) : (miss ? g.k + 1 : 0);
+        }
+        shi[0] = miss; shi[1] = stop;
+        sh[0] = kick_dt; sh[1] = q.s_dt; sh[2] = q.s_alpha; sh[3] = q.s_dtrev; sh[4] = q.s_vscale;
+    }
+    __syncthreads();
+    if (!shi[0]) return;
+    const bool stop = shi[1] != 0;
+    const double kick_dt = sh[0];
+    for (int c = blockIdx.x; c < r.nch; c += gridDim.x) {
+        const int i = c * DPMD_BLOCK + threadIdx.x;
+        bool flag = false;
+        if (i < a.n) {
+            double fx, fy, pe, vx, vy;
+            fire_force_kick<MODE>(a, xs, vs, i, kick_dt, fx, fy, pe, vx, vy);
+            const double2 p = xs[i];
+            if (stop) {   // the step's final state, as k_fire_force leaves it
+                a.x[i] = p.x; a.y[i] = p.y; a.vx[i] = vx; a.vy[i] = vy;
+                a.fx[i] = fx; a.fy[i] = fy; a.pe[i] = pe;
+            } else {
+                double x = p.x, y = p.y;
+                const double lx = a.cell ? a.lx[i] : 0.0, ly = a.cell ? a.ly[i] : 0.0;
+                const double th2 = a.cell ? a.thresh2[0] : 0.0;
+                flag = fire_drift_vals(x, y, vx, vy, fx, fy, a.mass[i], a.cell, lx, ly, th2, sh[1], sh[2], sh[3], sh[4]);
+                xo[i] = make_double2(x, y); vo[i] = make_double2(vx, vy);
+            }
+        }
+        if (a.cell && !stop) raise_word(&g.run[RUN_FIXW], flag);
+    }
+}
+
+// The last step of a fused batch: k_fire_force at (xs, vs), also writing positions and
+// velocities back to the context's arrays; k_fire_finalize follows.
+template <int MODE>
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_fire_force_pk(DiskArgs a, FireArgs f, RedArgs r, const double2* __restrict__ xs, const double2* __restrict__ vs,
+                Gate g) {
+    __shared__ double lds[12];
+    if (!gate_fire_fused(g)) return;
+    const int c = xcd_remap(blockIdx.x, gridDim.x);
+    const int i = c * DPMD_BLOCK + threadIdx.x;
+    double vp = 0.0, vpe = 0.0, vf2 = 0.0;
+    if (i < a.n) {
+        double fx, fy, pe, vx, vy;
+        fire_force_kick<MODE>(a, xs, vs, i, f.dt[0], fx, fy, pe, vx, vy);
+        const double2 p = xs[i];
+        a.x[i] = p.x; a.y[i] = p.y; a.vx[i] = vx; a.vy[i] = vy;
+        a.fx[i] = fx; a.fy[i] = fy; a.pe[i] = pe;
+        vp = fx * vx + fy * vy;
+        vpe = pe;
+        vf2 = fx * fx + fy * fy;
+    }
+    fire_chunk_partials(r, c, vp, vpe, vf2, lds);
+}
+
+__global__ void k_unpack_pv(int n, const double2* __restrict__ xs, const double2* __restrict__ vs,
+                            double* __restrict__ x, double* __restrict__ y, double* __restrict__ vx,
+                            double* __restrict__ vy) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 p = xs[i], v = vs[i];
+    x[i] = p.x; y[i] = p.y; vx[i] = v.x; vy[i] = v.y;
 }
 
 // Stopping test + next-step update for the step()/converged() call pattern (fire.cu:59-78).
@@ -496,6 +743,56 @@ void disk_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, doubl
 
 void fire_finalize(dpmd_ctx* c, Gate g, int do_conv, int pred, double pa, double pb) {
     k_fire_finalize<<<c->S, DPMD_BLOCK, 0, c->st>>>(c->off.p, fire_args(c), red_args(c), do_conv, 1, pred, pa, pb, g);
+    HIP_OK(hipGetLastError());
+}
+
+void disk_fire_drift_to(dpmd_ctx* c, Gate g, double* xo, double* vo, int any_word) {
+    DiskArgs a = disk_args(c);
+    a.any = c->run.p + any_word;
+    by_block(c->block, [&](auto b) {
+        constexpr int B = decltype(b)::value;
+        k_fire_drift_to<B><<<blocks_for(c->N, B), B, 0, c->st>>>(a, fire_args(c), (double2*)xo, (double2*)vo, g);
+    });
+    HIP_OK(hipGetLastError());
+}
+
+void disk_fire_fused(dpmd_ctx* c, Gate g, const double* xs, const double* vs, double* xo, double* vo, int any_word) {
+    DiskArgs a = disk_args(c);
+    a.any = c->run.p + any_word;
+    {
+        ProfScope prof(c, 3);
+        by_mode(c->fire_mode, [&](auto m) {
+            k_fire_fused<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(
+                a, fire_args(c), red_args(c), (const double2*)xs, (const double2*)vs, (double2*)xo, (double2*)vo,
+                c->f_snap.p, c->i_snap.p, g);
+        });
+    }
+    HIP_OK(hipGetLastError());
+}
+
+void disk_fire_settle(dpmd_ctx* c, Gate g, int pred, double pa, double pb, const double* xs, const double* vs,
+                      double* xo, double* vo, int any_word) {
+    const int grid = std::min(c->nchunks, kSettleGrid);
+    by_mode(c->fire_mode, [&](auto m) {
+        k_fire_settle<decltype(m)::value><<<grid, DPMD_BLOCK, 0, c->st>>>(
+            disk_args(c), fire_args(c), red_args(c), c->f_snap.p, c->i_snap.p, pred, pa, pb, (const double2*)xs,
+            (const double2*)vs, (double2*)xo, (double2*)vo, g, any_word);
+    });
+    HIP_OK(hipGetLastError());
+}
+
+void disk_fire_force_packed(dpmd_ctx* c, Gate g, const double* xs, const double* vs, int pred, double pa, double pb) {
+    by_mode(c->fire_mode, [&](auto m) {
+        k_fire_force_pk<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(
+            disk_args(c), fire_args(c), red_args(c), (const double2*)xs, (const double2*)vs, g);
+    });
+    HIP_OK(hipGetLastError());
+    fire_finalize(c, g, 1, pred, pa, pb);
+}
+
+void disk_unpack_pv(dpmd_ctx* c, const double* xs, const double* vs) {
+    k_unpack_pv<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(c->N, (const double2*)xs, (const double2*)vs, c->x.p,
+                                                            c->y.p, c->vx.p, c->vy.p);
     HIP_OK(hipGetLastError());
 }
 

@@ -49,8 +49,12 @@ struct NbrArgs {
 //  [10] gate of a device-side rebuild inside a step batch (a copy of [7], so clearing [7]
 //       cannot race with the rebuild's own gate reads)
 //  [11] number of device-side rebuilds run (batches that rebuild on the device)
+//  [12] fused FIRE batches: k + 1 when step k's guessed next-step scalars were wrong, -(k + 1)
+//       when the system stopped at step k, else 0 (written by the step's settle launch, read
+//       by the next launch's gate)
+//  [13] fused FIRE batches: rebuild word of positions from a redone (wrongly guessed) drift
 enum { RUN_MAX = 1, RUN_RUNNING = 2, RUN_ACTIVE = 3, RUN_HALT = 4, RUN_HALT_AT = 5, RUN_CONV_AT = 6, RUN_ANY = 7,
-       RUN_REC = 8, RUN_ANY2 = 9, RUN_REBUILDS = 11, RUN_WORDS = 12 };
+       RUN_REC = 8, RUN_ANY2 = 9, RUN_REBUILDS = 11, RUN_MIS = 12, RUN_FIXW = 13, RUN_WORDS = 14 };
 
 // Launch gate.  Plain form (run == nullptr): the kernel works iff *active != 0.  Batch form
 // (lean fused runs): a step works while nothing halted and some system runs; the force
@@ -211,6 +215,12 @@ struct dpmd_ctx {
     // fused velocity Verlet (disks): second position buffer and the switch
     dpmd::DVec<double> xy0, xy1;   // interleaved (x, y) ping-pong positions of fused VV batches (2N each)
     bool fuse_vv = true;
+    // fused FIRE batches (runs of one disk system): interleaved (vx, vy) ping-pong velocities
+    // (2N each; positions use xy0 / xy1), the snapshot of the FIRE scalars a step started from
+    // (dt, alpha, previous <pe>; N_good, N_bad), and the switch
+    dpmd::DVec<double> vxy0, vxy1, f_snap;
+    dpmd::DVec<int> i_snap;
+    bool fuse_fire = true;
     // workgroup size of the disk step kernels (64, 128, 256, 512) and the largest number of
     // steps captured into one hipGraph batch (a power of two, 1 .. 128)
     int block = DPMD_BLOCK;
@@ -347,6 +357,17 @@ void disk_vv_drift_to(dpmd_ctx* c, Gate g, double* xo, int any_word);
 void disk_vv_fused(dpmd_ctx* c, Gate g, const double* xs, double* xo, int any_word);
 void disk_vv_force_packed(dpmd_ctx* c, Gate g, const double* xs);
 void disk_unpack_positions(dpmd_ctx* c, const double* xs);
+// fused FIRE batches (one system): first drift into (xo, vo); step k's forces, kick and
+// partials plus step k + 1's guessed drift; the fold that checks the guess (and redoes the
+// drift when it was wrong); the batch's last forces (writing the state back); unpacking a
+// halted step's buffers
+constexpr int kSettleGrid = 128;   // workgroups of the settle launch (each folds the partials)
+void disk_fire_drift_to(dpmd_ctx* c, Gate g, double* xo, double* vo, int any_word);
+void disk_fire_fused(dpmd_ctx* c, Gate g, const double* xs, const double* vs, double* xo, double* vo, int any_word);
+void disk_fire_settle(dpmd_ctx* c, Gate g, int pred, double pa, double pb, const double* xs, const double* vs,
+                      double* xo, double* vo, int any_word);
+void disk_fire_force_packed(dpmd_ctx* c, Gate g, const double* xs, const double* vs, int pred, double pa, double pb);
+void disk_unpack_pv(dpmd_ctx* c, const double* xs, const double* vs);
 void disk_vv_force(dpmd_ctx* c, Gate g);
 void disk_damping(dpmd_ctx* c, const double* scale);
 

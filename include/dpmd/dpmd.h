@@ -356,7 +356,8 @@ int dpmd_synchronize(dpmd_ctx* ctx);
  * dpmd_profile(ctx, 1) starts recording a HIP event pair around every force-kernel launch
  * issued outside graph capture; dpmd_profile(ctx, 2) records one pair around the consecutive
  * fused velocity-Verlet launches of every directly launched batch that no rebuild halted;
- * 0 stops.  dpmd_profile_read returns the summed duration and the number of launches covered. */
+ * dpmd_profile(ctx, 3) records a pair around every fused FIRE force launch (runs of one disk
+ * system, DPMD_OPT_FUSED_FIRE) issued outside graph capture; 0 stops.  dpmd_profile_read returns the summed duration and the number of launches covered. */
 int dpmd_profile(dpmd_ctx* ctx, int enable);
 int dpmd_profile_read(dpmd_ctx* ctx, double* total_ms, int* count);
 /* The recorded event pairs one by one (ms, in order, up to capacity); *count = the number of
@@ -399,11 +400,17 @@ int dpmd_stream_copy(double* dst, const double* src, int64_t n, int repeats, dou
  *     neighbor build; 0 (default): 1 when every system fits, else 2.  Same results.
  *   DPMD_OPT_JAM_RETRIES: dpmd_jam_compress runs a round whose minimisation hit the FIRE step
  *     cap again (FIRE restarted from where it stopped, no compression) up to this many times
- *     per system before dropping the system; default 0 (drop at once). */
+ *     per system before dropping the system; default 0 (drop at once).
+ *   DPMD_OPT_FUSED_FIRE: 1 (default) = device-driven FIRE runs of one disk system compute a
+ *     step's forces, kick and power partials and the next step's drift in one launch, the
+ *     drift taken with the scalars that F.v > 0 gives and redone by a fixup launch when the
+ *     power fold shows otherwise (positions and velocities ping-pong between two buffer
+ *     pairs); 0 = drift, force and fold launches per step.  Same results. */
 enum dpmd_option {
     DPMD_OPT_COOP_REBUILD = 0, DPMD_OPT_COOP_IN_GRAPH = 1, DPMD_OPT_FUSED_VV = 2, DPMD_OPT_BLOCK = 3,
     DPMD_OPT_MAX_BATCH = 4, DPMD_OPT_PERSISTENT_FIRE = 5, DPMD_OPT_PERSIST_BLOCK = 6, DPMD_OPT_PERSIST_PHASE = 7,
-    DPMD_OPT_DEVICE_REBUILD = 8, DPMD_OPT_PERSIST_LDS = 9, DPMD_OPT_REBUILD_ENGINE = 10, DPMD_OPT_JAM_RETRIES = 11
+    DPMD_OPT_DEVICE_REBUILD = 8, DPMD_OPT_PERSIST_LDS = 9, DPMD_OPT_REBUILD_ENGINE = 10, DPMD_OPT_JAM_RETRIES = 11,
+    DPMD_OPT_FUSED_FIRE = 12
 };
 int dpmd_set_option(dpmd_ctx* ctx, int key, int value);
 int dpmd_get_option(dpmd_ctx* ctx, int key, int* value);
