@@ -830,25 +830,20 @@ def main():
     lib.dpmd_profile(P.ctx, 0)
     cap, mx, nnb = P.neighbor_info()
     k = nnb / N
-    # fused FIRE kernel (step k's forces, kick and power / energy / max|F| chunk partials, step
-    # k + 1's drift) in an untimed tail
+    # fused FIRE steps (k_fire_fused: step k's forces, kick, power / energy / max|F| chunk
+    # partials and step k + 1's drift; k_fire_settle: the fold of the partials) in an untimed
+    # tail: HIP events around the consecutive fused steps of every batch no rebuild halted
     lib.dpmd_profile(P.ctx, 3)
+    ff_ms, ff_cnt = C.c_double(), C.c_int()
     run_fire(P, 1e-2, 300, L.FIRE_PRED_MAX_FORCE, 0.0, use_graph=False)
-    P.synchronize()
-    durs = (C.c_float * 4096)()
-    nd = C.c_int()
-    L.check(lib.dpmd_profile_durations(P.ctx, durs, 4096, C.byref(nd)), P.ctx)
+    L.check(lib.dpmd_profile_read(P.ctx, C.byref(ff_ms), C.byref(ff_cnt)), P.ctx)
     lib.dpmd_profile(P.ctx, 0)
-    d = np.array(durs[:min(nd.value, 4096)], dtype=np.float64)
-    if len(d):   # launches of a batch that a rebuild halted exit at once (a few us)
-        d = d[d > 0.5 * np.percentile(d, 90)]
     kf = P.neighbor_info()[2] / N
-    # k_fire_fused per particle: x, y (16), vx, vy (16), r (8), count (4), k ids (4k), m (8),
-    # last-rebuild x, y (16) read; the next step's x, y, vx, vy (32) written -> 100 + 4k bytes
-    # (neighbor gathers and per-chunk partials excluded)
+    # one fused FIRE step per particle: x, y (16), vx, vy (16), r (8), count (4), k ids (4k),
+    # m (8), last-rebuild x, y (16) read; the next step's x, y, vx, vy (32) written -> 100 + 4k
+    # bytes (neighbor gathers, chunk partials and their fold excluded)
     ff_bytes = N * (100 + 4 * kf)
-    ff_avg = float(d.mean()) if len(d) else float("nan")
-    ff_cnt = C.c_int(len(d))
+    ff_avg = ff_ms.value / max(1, ff_cnt.value)
     ff_gbs = ff_bytes / (ff_avg * 1e-3) / 1e9
     # k_vv_fused per particle: reads x,y,rad (24), sid (4), count (4), k ids (4k), vx,vy,mass (24),
     # last-rebuild x,y (16); writes vx,vy (16) and the new x,y (16)
@@ -899,7 +894,8 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "copy_kernel_gbs": copy_gbs},
         "fire_roofline": {"bound": "hbm",
-                          "kernel": "k_fire_fused (FIRE: forces, kick, chunk partials, next step's guessed drift)",
+                          "kernel": "one fused FIRE step: k_fire_fused (forces, kick, chunk partials, next step's "
+                                    "guessed drift) + k_fire_settle (fold of the partials, FIRE update)",
                           "achieved": ff_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": ff_gbs / HBM_PEAK_GBS,
                           "traffic": (FIRE_FUSED_TRAFFIC_N1M + N * 4 * (k - FIRE_FUSED_TRAFFIC_K))
@@ -907,8 +903,8 @@ def main():
                           "traffic_source": FUSED_TRAFFIC_SOURCE.replace("neighbor-id", "k_fire_fused rows; "
                                                                          "neighbor-id")
                           if (N == 1 << 20 and FIRE_FUSED_TRAFFIC_N1M) else None,
-                                "avg_launch_us": ff_avg * 1e3,
-                                "launches": ff_cnt.value, "algorithmic_bytes_per_launch": ff_bytes},
+                          "avg_step_us": ff_avg * 1e3, "steps": ff_cnt.value,
+                          "algorithmic_bytes_per_step": ff_bytes},
     }
     del P
     if not args.no_legs:

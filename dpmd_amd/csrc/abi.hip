@@ -398,6 +398,12 @@ void lean_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b, boo
         double* P[2] = {c->xy0.p, c->xy1.p};
         double* V[2] = {c->vxy0.p, c->vxy1.p};
         disk_fire_drift_to(c, Gate{nullptr, c->run.p, 0}, P[0], V[0], any_word(0));
+        const bool prof = c->prof == 3 && !c->capturing;
+        if (prof) {   // one pair around the batch's fused steps; kept only if no rebuild halts it
+            HIP_OK(hipEventCreate(&c->pend_b)); HIP_OK(hipEventCreate(&c->pend_e));
+            HIP_OK(hipEventRecord(c->pend_b, c->st));
+            c->pend_n = m - 1;
+        }
         for (int q = 0; q + 1 < m; ++q) {
             Gate g{nullptr, c->run.p, q};
             g.any = any_word(q);
@@ -405,6 +411,7 @@ void lean_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b, boo
             disk_fire_fused(c, g, P[q % 2], V[q % 2], P[nx], V[nx], nw);
             disk_fire_settle(c, g, pred, a, b, P[q % 2], V[q % 2], P[nx], V[nx], nw);
         }
+        if (prof) HIP_OK(hipEventRecord(c->pend_e, c->st));
         Gate gf{nullptr, c->run.p, m - 1};
         gf.any = any_word(m - 1);
         disk_fire_force_packed(c, gf, P[(m - 1) % 2], V[(m - 1) % 2], pred, a, b);
@@ -598,13 +605,12 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
                 since_halt += r.halt_at;
                 if (fused)   // the halted step's positions are in the interleaved buffer P(halt_at mod 2)
                     disk_unpack_positions(c, r.halt_at % 2 == 0 ? c->xy0.p : c->xy1.p);
-                if (fused_fire) {   // and for FIRE its velocities in V(halt_at mod 2); the launches
-                    // of a fused FIRE batch raise only the run words, so the system is flagged here
+                if (fused_fire)   // and for FIRE its velocities in V(halt_at mod 2)
                     disk_unpack_pv(c, r.halt_at % 2 == 0 ? c->xy0.p : c->xy1.p,
                                    r.halt_at % 2 == 0 ? c->vxy0.p : c->vxy1.p);
-                    const unsigned one = 1u;
-                    HIP_OK(hipMemcpyAsync(c->flag.p, &one, sizeof(one), hipMemcpyHostToDevice, c->st));
-                }
+                // clear the halt and the batch words (the rebuild clears RUN_ANY); the launches of
+                // a fused FIRE batch raise only the run words, so its system is flagged here
+                halt_reset(c, fused_fire);
                 // finish the halted step: rebuild, forces, bookkeeping
                 rebuild_neighbors(c, active_one(c), false, true);
                 if (fire) {
@@ -613,12 +619,6 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
                 } else {
                     if (rigid(c)) rigid_vv_force(c, plain(c)); else disk_vv_force(c, plain(c));
                 }
-                const int clear[2] = {0, -1};
-                HIP_OK(hipMemcpyAsync(c->run.p + RUN_HALT, clear, sizeof(clear), hipMemcpyHostToDevice, c->st));
-                const int zero[5] = {0, 0, 0, 0, 0};   // the rebuild cleared RUN_ANY; fused batches also
-                // use RUN_ANY2 and (FIRE) RUN_MIS and RUN_FIXW
-                HIP_OK(hipMemcpyAsync(c->run.p + RUN_ANY2, zero, sizeof(int), hipMemcpyHostToDevice, c->st));
-                HIP_OK(hipMemcpyAsync(c->run.p + RUN_MIS, zero, 2 * sizeof(int), hipMemcpyHostToDevice, c->st));
                 r = read_run(c);
                 if (fire && r.running <= 0) done = true;
                 m = std::max(1, std::min(128, since_halt));

@@ -139,7 +139,8 @@ __global__ void __launch_bounds__(B) k_fire_drift(DiskArgs a, FireArgs f, Gate g
 
 // The fold of one system's chunk partials in a fixed order: 256 strided sequential
 // accumulators (thread t adds chunks t, t + 256, ... in turn), then the block tree.  Every
-// thread of the 256-thread workgroup gets the power, energy and max |F|^2 totals.
+// thread of the 256-thread workgroup gets the power, energy and max |F|^2 totals (lds: 12
+// doubles, not reused by the caller before a barrier).
 __device__ __forceinline__ void fold_partials(const RedArgs& r, int s, double* lds, double& ap, double& ape,
                                               double& af) {
     const int cb = r.sys_chunk[s], ce = r.sys_chunk[s + 1];
@@ -162,9 +163,15 @@ __device__ __forceinline__ void fold_partials(const RedArgs& r, int s, double* l
             af = fmax(af, q2[u]);
         }
     }
-    ap = block_sum256(ap, lds);
-    ape = block_sum256(ape, lds);
-    af = block_max256(af, lds);
+    // block_sum256 / block_max256 of the three values with one barrier: wavefront butterflies,
+    // then (w0 + w1) + (w2 + w3)
+    ap = wave_sum(ap); ape = wave_sum(ape); af = wave_max(af);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { lds[w] = ap; lds[4 + w] = ape; lds[8 + w] = af; }
+    __syncthreads();
+    ap = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+    ape = (lds[4] + lds[5]) + (lds[6] + lds[7]);
+    af = fmax(fmax(lds[8], lds[9]), fmax(lds[10], lds[11]));
 }
 
 // The fold of one system's partials, the stopping test and the next step's FIRE update, by a
@@ -247,7 +254,7 @@ k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, Gate g) {
 __global__ void __launch_bounds__(DPMD_BLOCK)
 k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_update, int pred, double pa, double pb,
                 Gate g) {
-    __shared__ double lds[4];
+    __shared__ double lds[12];
     if (!gate_open(g)) return;
     const int s = blockIdx.x;
     if (fire_frozen(f, s)) return;
@@ -398,7 +405,7 @@ __global__ void __launch_bounds__(DPMD_BLOCK)
 k_fire_settle(DiskArgs a, FireArgs f, RedArgs r, const double* __restrict__ snap, const int* __restrict__ isnap,
               int pred, double pa, double pb, const double2* __restrict__ xs, const double2* __restrict__ vs,
               double2* __restrict__ xo, double2* __restrict__ vo, Gate g, int anyw) {
-    __shared__ double lds[4];
+    __shared__ double lds[12];
     __shared__ double sh[5];
     __shared__ int shi[2];
     if (g.run[RUN_HALT] != 0 || g.run[RUN_RUNNING] <= 0) return;
@@ -590,15 +597,16 @@ k_vv_drift_to(DiskArgs a, double2* __restrict__ xo, const double* __restrict__ d
 // the skin test of the new positions.  Every force must see its neighbors' old positions,
 // hence two buffers; both hold interleaved (x, y) pairs, so each neighbor costs one 16-byte
 // gather.  Each expression is the one of k_vv_force / k_vv_drift, in the same order, so a
-// fused run equals the two-launch run bit for bit.
-template <int MODE, int B>
+// fused run equals the two-launch run bit for bit.  ONE: the context holds one system, so no
+// system id is loaded per particle.
+template <int MODE, int B, bool ONE>
 __global__ void __launch_bounds__(B)
 k_vv_fused(DiskArgs a, const double2* __restrict__ xs, double2* __restrict__ xo, const double* __restrict__ dt,
            const double* __restrict__ dth, const double* __restrict__ damp, Gate g) {
     if (!gate_force(g)) return;
     const int i = xcd_remap(blockIdx.x, gridDim.x) * B + threadIdx.x;
     const bool valid = i < a.n;
-    const int s = valid ? a.sid[i] : -1;
+    const int s = valid ? (ONE ? 0 : a.sid[i]) : -1;
     bool flag = false;
     if (valid) {
         // The forces are used here only: every fused batch ends with k_vv_force, and a batch
@@ -759,14 +767,11 @@ void disk_fire_drift_to(dpmd_ctx* c, Gate g, double* xo, double* vo, int any_wor
 void disk_fire_fused(dpmd_ctx* c, Gate g, const double* xs, const double* vs, double* xo, double* vo, int any_word) {
     DiskArgs a = disk_args(c);
     a.any = c->run.p + any_word;
-    {
-        ProfScope prof(c, 3);
-        by_mode(c->fire_mode, [&](auto m) {
-            k_fire_fused<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(
-                a, fire_args(c), red_args(c), (const double2*)xs, (const double2*)vs, (double2*)xo, (double2*)vo,
-                c->f_snap.p, c->i_snap.p, g);
-        });
-    }
+    by_mode(c->fire_mode, [&](auto m) {
+        k_fire_fused<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(
+            a, fire_args(c), red_args(c), (const double2*)xs, (const double2*)vs, (double2*)xo, (double2*)vo,
+            c->f_snap.p, c->i_snap.p, g);
+    });
     HIP_OK(hipGetLastError());
 }
 
@@ -820,8 +825,12 @@ void disk_vv_fused(dpmd_ctx* c, Gate g, const double* xs, double* xo, int any_wo
     by_mode(c->vv_mode, [&](auto m) {
         by_block(c->block, [&](auto b) {
             constexpr int B = decltype(b)::value;
-            k_vv_fused<decltype(m)::value, B><<<blocks_for(c->N, B), B, 0, c->st>>>(
-                a, (const double2*)xs, (double2*)xo, c->vv_dt.p, c->vv_dth.p, vv_damping(c), g);
+            if (c->S == 1)
+                k_vv_fused<decltype(m)::value, B, true><<<blocks_for(c->N, B), B, 0, c->st>>>(
+                    a, (const double2*)xs, (double2*)xo, c->vv_dt.p, c->vv_dth.p, vv_damping(c), g);
+            else
+                k_vv_fused<decltype(m)::value, B, false><<<blocks_for(c->N, B), B, 0, c->st>>>(
+                    a, (const double2*)xs, (double2*)xo, c->vv_dt.p, c->vv_dth.p, vv_damping(c), g);
         });
     });
     HIP_OK(hipGetLastError());

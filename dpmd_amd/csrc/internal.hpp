@@ -411,6 +411,7 @@ void sys_reduce_dot(dpmd_ctx* c, const double* a, const double* b, const double*
 void fire_apply_convergence(dpmd_ctx* c, int pred, double a, double b);
 void fire_setup(dpmd_ctx* c, const double* dt);
 void run_begin_step(dpmd_ctx* c);
+void halt_reset(dpmd_ctx* c, bool flag_one);
 void per_system_temperature(dpmd_ctx* c);
 void per_system_pressure(dpmd_ctx* c);
 void temperature_scale(dpmd_ctx* c, const double* target, double* scale_out);

@@ -129,6 +129,15 @@ __global__ void k_run_begin(int* run) {
     if (active) run[0] += 1;
 }
 
+// A halted step batch, before the host finishes the halted step: clears the halt and the
+// batch words of fused batches, and flags system 0 (fused FIRE batches of one system raise
+// only the run words).
+__global__ void k_halt_reset(int* run, unsigned* flag, int flag_one) {
+    run[RUN_HALT] = 0; run[RUN_HALT_AT] = -1;
+    run[RUN_ANY2] = 0; run[RUN_MIS] = 0; run[RUN_FIXW] = 0;
+    if (flag_one) flag[0] = 1u;
+}
+
 // save_/restore_particle_state_kernel, save_/restore_system_state_kernel (disk.cu:348-432).
 // (The reference's system kernels index g_sys.id with a system index; here the flag is
 // read per system.)
@@ -260,6 +269,11 @@ void box_inv(dpmd_ctx* c) {
 
 void scale_box_increment(dpmd_ctx* c, const double* inc, double* scale_out) {
     k_scale_box_inc<<<blocks_for(c->S), DPMD_BLOCK, 0, c->st>>>(c->S, c->Lx.p, c->Ly.p, c->phi.p, inc, scale_out);
+    HIP_OK(hipGetLastError());
+}
+
+void halt_reset(dpmd_ctx* c, bool flag_one) {
+    k_halt_reset<<<1, 1, 0, c->st>>>(c->run.p, c->flag.p, flag_one ? 1 : 0);
     HIP_OK(hipGetLastError());
 }
 

@@ -356,8 +356,9 @@ int dpmd_synchronize(dpmd_ctx* ctx);
  * dpmd_profile(ctx, 1) starts recording a HIP event pair around every force-kernel launch
  * issued outside graph capture; dpmd_profile(ctx, 2) records one pair around the consecutive
  * fused velocity-Verlet launches of every directly launched batch that no rebuild halted;
- * dpmd_profile(ctx, 3) records a pair around every fused FIRE force launch (runs of one disk
- * system, DPMD_OPT_FUSED_FIRE) issued outside graph capture; 0 stops.  dpmd_profile_read returns the summed duration and the number of launches covered. */
+ * dpmd_profile(ctx, 3) records one pair around the consecutive fused FIRE steps (fused and
+ * settle launches; runs of one disk system, DPMD_OPT_FUSED_FIRE) of every directly launched
+ * batch that no rebuild halted; 0 stops.  dpmd_profile_read returns the summed duration and the number of launches covered. */
 int dpmd_profile(dpmd_ctx* ctx, int enable);
 int dpmd_profile_read(dpmd_ctx* ctx, double* total_ms, int* count);
 /* The recorded event pairs one by one (ms, in order, up to capacity); *count = the number of
