@@ -554,7 +554,10 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
         }
         graphs = c->gcache;
     }
-    int total = 0, m = 8, since_halt = 0;
+    // interval: steps between the last two halts (0 = none seen yet); after a batch that ran to
+    // its end the next one is sized to end where the next halt is expected, since the launches
+    // of a halted batch's remaining steps still cost their dispatch
+    int total = 0, m = 8, since_halt = 0, interval = 0;
     bool done = false;
     // Rebuilds per step, smoothed over batches.  A halted batch costs a host round trip and
     // restarts with small batches; above one rebuild per 2 steps (back below 0.35 per step)
@@ -622,6 +625,7 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
                 r = read_run(c);
                 if (fire && r.running <= 0) done = true;
                 m = std::max(1, std::min(128, since_halt));
+                interval = since_halt + 1;
                 since_halt = 0;
             } else if (fire && r.running <= 0) {
                 total += r.conv_at + 1;
@@ -629,7 +633,10 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
             } else {
                 total += bm;
                 since_halt += bm;
-                m = std::min(128, 2 * bm);
+                // steps to the expected next halt; up to 7 steps past it are allowed (a halted
+                // step's tail costs less than another host round trip)
+                const int rem = interval - since_halt;
+                m = (interval > 0 && rem > 0) ? std::min(rem + 7, 2 * bm) : std::min(128, 2 * bm);
             }
         }
     } catch (...) {
