@@ -34,14 +34,15 @@ METRIC = "particle-steps/sec (batched FIRE/NVE) + %HBM roofline; adjoint/forward
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes in profiles/k_vv_fused_disk_N1M_fire_nve.md, each
 # scaled by what the counter reads for a copy kernel of known size (FETCH x 2.0, WRITE x 1.0),
 # at the profiled run's mean neighbor count.  A run reports it scaled to its own count.
-FUSED_TRAFFIC_N1M = (51455.8 * (131072.0 / 65545.5) + 32768.0) * 1024.0
+FUSED_TRAFFIC_N1M = (49443.9 * (131072.0 / 65545.5) + 32768.0) * 1024.0
 FUSED_TRAFFIC_K = 5.46
 FUSED_TRAFFIC_SOURCE = ("profile-derived: profiles/k_vv_fused_disk_N1M_fire_nve.md (FETCH_SIZE x 2.0 + "
                         "WRITE_SIZE x 1.0), neighbor-id bytes scaled to this run's mean count")
-# The same for k_fire_fused on that workload (its profiled run's mean count: 5.46), and for
+# The same for one fused FIRE step (k_fire_fused + k_fire_settle) on that workload (its
+# profiled run's mean count: 5.46), and for
 # k_vertex_force on the 256 x 256 rigid workload (profiles/k_vertex_force_rigid_256x256_fire_nve.md,
 # mean vertex neighbor count 3.77).
-FIRE_FUSED_TRAFFIC_N1M = None   # no counter pass of k_fire_fused yet
+FIRE_FUSED_TRAFFIC_N1M = ((49691.7 + 450.6) * (131072.0 / 65545.5) + 32864.1 + 0.5) * 1024.0   # fused + settle
 FIRE_FUSED_TRAFFIC_K = 5.46
 RIGID_VF_TRAFFIC = (43563.0 * (131072.0 / 65545.5) + 29164.1) * 1024.0
 RIGID_VF_TRAFFIC_K = 3.77
@@ -900,8 +901,8 @@ def main():
                           "frac": ff_gbs / HBM_PEAK_GBS,
                           "traffic": (FIRE_FUSED_TRAFFIC_N1M + N * 4 * (k - FIRE_FUSED_TRAFFIC_K))
                           if (N == 1 << 20 and FIRE_FUSED_TRAFFIC_N1M) else None,
-                          "traffic_source": FUSED_TRAFFIC_SOURCE.replace("neighbor-id", "k_fire_fused rows; "
-                                                                         "neighbor-id")
+                          "traffic_source": FUSED_TRAFFIC_SOURCE.replace("neighbor-id", "k_fire_fused + k_fire_settle "
+                                                                         "rows; neighbor-id")
                           if (N == 1 << 20 and FIRE_FUSED_TRAFFIC_N1M) else None,
                           "avg_step_us": ff_avg * 1e3, "steps": ff_cnt.value,
                           "algorithmic_bytes_per_step": ff_bytes},
