@@ -846,10 +846,11 @@ def main():
     ff_bytes = N * (100 + 4 * kf)
     ff_avg = ff_ms.value / max(1, ff_cnt.value)
     ff_gbs = ff_bytes / (ff_avg * 1e-3) / 1e9
-    # k_vv_fused per particle: reads x,y,rad (24), sid (4), count (4), k ids (4k), vx,vy,mass (24),
-    # last-rebuild x,y (16); writes vx,vy (16) and the new x,y (16)
-    # -> 104 + 4k bytes (neighbor coordinate gathers excluded; forces stay in registers).
-    bytes_per_launch = N * (104 + 4 * k)
+    # k_vv_fused per particle (a context of one system loads no system id): reads x,y,rad (24),
+    # count (4), k ids (4k), vx,vy,mass (24), last-rebuild x,y (16); writes vx,vy (16) and the
+    # new x,y (16) -> 100 + 4k bytes (neighbor coordinate gathers excluded; forces stay in
+    # registers).
+    bytes_per_launch = N * (100 + 4 * k)
     avg_ms = tot_ms.value / max(1, cnt.value)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
 
