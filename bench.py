@@ -414,8 +414,9 @@ def run_adjoint(args):
         system's rigid backbone with per-system MINRES to a 1e-10 relative residual;
     (b) dMSD_s/dv0 through 1000 velocity-Verlet steps (dt 1e-2, T = 1e-4) with checkpoints
         every 50 steps.
-    Both are checked against central finite differences (h = 1e-6) on the GPU; the timing
-    ratio is adjoint / forward."""
+    Both are checked against central finite differences (h = 1e-6) on the GPU: (a) where the
+    contact network is the same at r_i +- h, (b) through the 1000 steps as they come and through
+    a 50-step horizon where the MSD is smooth.  The timing ratio is adjoint / forward."""
     import numpy as np
 
     from dpmd_amd import _lib as L
@@ -443,27 +444,58 @@ def run_adjoint(args):
     sync()
     t_adj_a = time.perf_counter() - t0
     log(f"implicit dP/dr: {it} MINRES iterations, residual {res:.1e}, {t_adj_a:.3f} s")
-    # central differences through re-minimisation: one system at a time, FIRE from the minimum
+    # Central differences through re-minimisation, one system at a time: FIRE from the batch's
+    # minimum to max|F| < 1e-13, then a Newton polish where its Hessian solve converges.  The
+    # pressure is differentiable in r_i only while the contact network stays the same, and the
+    # differences resolve the derivative only where the response is linear over the step: a
+    # probe counts when the minima at r_i, r_i +- h and r_i +- 2h have the same contacts and
+    # the differences at h and 2h agree to 1e-7 relative.  The candidates of a system are its
+    # particles in order of |dP/dr_i|; the first that qualifies is taken.
     xm, ym = P.get("POS_X", C), P.get("POS_Y", C)
     h = 1e-6
-    errs_a = []
-    for s in (0, 37, 81, S - 1):
+
+    def minimum(q):
+        Q = Disk(); W.load_into(Q, q); Q.init_neighbors()
+        run_fire(Q, 1e-2, 300000, L.FIRE_PRED_MAX_FORCE, 1e-13, mode=mode)
+        try:   # the polish needs a Hessian solve that converges (rattlers can leave it singular)
+            A.newton_minimize(Q, mode, max_newton=20, ftol=1e-14)
+        except L.DpmdError:
+            run_fire(Q, 1e-2, 300000, L.FIRE_PRED_MAX_FORCE, 1e-13, mode=mode)
+        Q.zero_velocities(); Q.compute_forces_mode(mode); Q.compute_pressure()
+        fres = float(np.max(np.hypot(Q.get("FORCE_X"), Q.get("FORCE_Y"))))
+        x, y = Q.get("POS_X", C), Q.get("POS_Y", C)
+        dx = x[:, None] - x[None, :]; dx -= q.Lx[0] * np.rint(dx / q.Lx[0])
+        dy = y[:, None] - y[None, :]; dy -= q.Ly[0] * np.rint(dy / q.Ly[0])
+        touch = np.hypot(dx, dy) < q.rad[:, None] + q.rad[None, :] - 1e-10   # contacts of overlap > 1e-10
+        np.fill_diagonal(touch, False)
+        return Q.get("PRESSURE")[0], touch, fres
+
+    errs_a, probes_a, skipped_a, fres_a = [], [], 0, []
+    for s in (0, 37, 81, S - 1, 11, 64, 100, 23):
+        if len(errs_a) == 4:
+            break
         sl = slice(s * n, (s + 1) * n)
-        i = int(np.argmax(np.abs(dr[sl])))
         one = W.system_of(b, s)
         one.x, one.y = xm[sl].copy(), ym[sl].copy()
-
-        def pressure(delta):
-            q = W.system_of(one, 0)
-            q.rad = q.rad.copy(); q.rad[i] += delta
-            Q = Disk(); W.load_into(Q, q); Q.init_neighbors()
-            run_fire(Q, 1e-2, 300000, L.FIRE_PRED_MAX_FORCE, 1e-13, mode=mode)
-            Q.zero_velocities(); Q.compute_pressure()
-            return Q.get("PRESSURE")[0]
-
-        fd = (pressure(h) - pressure(-h)) / (2 * h)
-        errs_a.append(abs(fd - dr[s * n + i]) / abs(dr[s * n + i]))
-    log(f"dP/dr vs central differences (FIRE re-minimisation to 1e-13): rel err {['%.1e' % e for e in errs_a]}")
+        _, touch0, f0 = minimum(one)
+        fres_a.append(f0)
+        for i in np.argsort(-np.abs(dr[sl]))[:16]:
+            res_pm = []
+            for delta in (h, -h, 2 * h, -2 * h):
+                q = W.system_of(one, 0)
+                q.rad = q.rad.copy(); q.rad[i] += delta
+                res_pm.append(minimum(q))
+            fres_a += [r_[2] for r_ in res_pm]
+            fd = (res_pm[0][0] - res_pm[1][0]) / (2 * h)
+            fd2 = (res_pm[2][0] - res_pm[3][0]) / (4 * h)
+            if not all(np.array_equal(r_[1], touch0) for r_ in res_pm) or abs(fd - fd2) > 1e-7 * abs(fd):
+                skipped_a += 1
+                continue
+            errs_a.append(abs(fd - dr[s * n + i]) / abs(dr[s * n + i]))
+            probes_a.append([int(s), int(i)])
+            break
+    log(f"dP/dr vs central differences (h = {h:g}; same contacts at +-h, +-2h, linear response; {skipped_a} candidates "
+        f"skipped): rel err {['%.1e' % e for e in errs_a]}, largest residual force {max(fres_a):.1e}")
 
     # (b) NVE
     P.rand_normal_2d("VEL_X", "VEL_Y", 7, 1)
@@ -503,6 +535,8 @@ def run_adjoint(args):
     sync()
     t_adj_b = time.perf_counter() - t0
     log(f"NVE: forward {t_plain:.3f} s (checkpointed {t_fwd_b:.3f} s), backprop {t_adj_b:.3f} s")
+    # Central differences of the 1000-step MSD (h = 1e-6): contacts open and close along the
+    # trajectory, where the MSD has no second derivative, so these are reported as they come.
     errs_b = []
     for i in np.random.default_rng(1).choice(b.N, 12, replace=False):
         e = np.zeros(b.N); e[i] = h
@@ -510,7 +544,35 @@ def run_adjoint(args):
         reset(vx0 - e, vy0); VelocityVerlet(P, 1e-2, mode=mode).run(nsteps); mm = msd()[0]
         fd = (mp - mm)[sid[i]] / (2 * h)
         errs_b.append(abs(fd - gvx[i]) / abs(gvx[i]))
-    log(f"dMSD/dv0 vs central differences: rel err median {np.median(errs_b):.1e}, max {max(errs_b):.1e}")
+    log(f"dMSD/dv0 (1000 steps) vs central differences: rel err median {np.median(errs_b):.1e}, max {max(errs_b):.1e}")
+    # The same gradient through a 50-step horizon (one checkpoint segment), checked where the
+    # MSD is smooth: a component counts when its central differences at h and 2h agree to 1e-7
+    # relative (for a smooth function they differ by O(h^2); where a contact opens or closes
+    # within the perturbation they do not, and the differences themselves are not that accurate).
+    nfd = 50
+    reset(vx0, vy0)
+    A.vv_run_checkpointed(P, nfd, every)
+    sync()
+    _, ax, ay = msd()
+    _, _, gvx_s, _ = A.vv_backprop(P, 2 * (ax - xs) / n, 2 * (ay - ys) / n, None, None)
+
+    def fd_at(i, hh):
+        e = np.zeros(b.N); e[i] = hh
+        reset(vx0 + e, vy0); VelocityVerlet(P, 1e-2, mode=mode).run(nfd); mp = msd()[0]
+        reset(vx0 - e, vy0); VelocityVerlet(P, 1e-2, mode=mode).run(nfd); mm = msd()[0]
+        return (mp - mm)[sid[i]] / (2 * hh)
+
+    errs_s, skipped_s = [], 0
+    for i in np.random.default_rng(2).permutation(b.N):
+        if len(errs_s) == 12 or skipped_s == 96:
+            break
+        f1, f2 = fd_at(i, h), fd_at(i, 2 * h)
+        if abs(f1 - f2) > 1e-7 * abs(f1) or gvx_s[i] == 0.0:
+            skipped_s += 1
+            continue
+        errs_s.append(abs(f1 - gvx_s[i]) / abs(gvx_s[i]))
+    log(f"dMSD/dv0 ({nfd} steps) vs central differences where smooth ({skipped_s} skipped): rel err max "
+        f"{max(errs_s) if errs_s else float('nan'):.1e} over {len(errs_s)} components")
     ratio = t_adj_b / t_plain
     out = {
         "metric": "adjoint time / forward time (NVE reverse mode through 1000 steps, checkpoints every 50: "
@@ -522,11 +584,16 @@ def run_adjoint(args):
                    "n_particles": int(b.N),
                    "a_fire_steps": steps, "a_forward_seconds": t_fwd_a, "a_adjoint_seconds": t_adj_a,
                    "a_adjoint_over_forward": t_adj_a / t_fwd_a, "a_minres_iterations": it, "a_minres_residual": res,
-                   "a_fd_rel_err": errs_a,
+                   "a_fd_rel_err": errs_a, "a_fd_probes_system_particle": probes_a,
+                   "a_fd_candidates_skipped_contact_change": skipped_a, "a_fd_largest_residual_force": max(fres_a),
+                   "a_fd_within_1e-6": int(sum(e <= 1e-6 for e in errs_a)),
                    "b_forward_seconds": t_plain, "b_checkpointed_forward_seconds": t_fwd_b,
                    "b_backprop_seconds": t_adj_b, "b_fd_rel_err_median": float(np.median(errs_b)),
                    "b_fd_rel_err_max": float(max(errs_b)), "b_fd_within_1e-6": int(sum(e <= 1e-6 for e in errs_b)),
                    "b_fd_components": len(errs_b),
+                   "b50_fd_rel_err_max": float(max(errs_s)) if errs_s else None,
+                   "b50_fd_components": len(errs_s), "b50_fd_within_1e-6": int(sum(e <= 1e-6 for e in errs_s)),
+                   "b50_fd_skipped_not_smooth": skipped_s,
                    "b_total_gradient_over_forward": (t_fwd_b + t_adj_b) / t_plain},
         "roofline": None,
     }

@@ -890,8 +890,7 @@ void adjoint_pressure_impl(dpmd_ctx* c, int mode, double tol, int max_iter, doub
     double *b = c->adj_t0.p, *lam = c->adj_t2.p;
     adj_vjp_current(c, mode, nullptr, c->adj_u0.p, nullptr, nullptr, b, b + N, c->adj_t1.p, c->adj_u1.p, c->adj_u2.p);
     const int it = adj_cg_current(c, mode, b, b + N, lam, lam + N, tol, max_iter, resid);
-    // lambda lives on the backbone; contacts with non-backbone particles are not in H, so they
-    // are left out of dF/dq too
+    // lambda lives on the particles with contacts (the others are null directions of H)
     adj_vjp_current(c, mode, nullptr, nullptr, lam, lam + N, nullptr, nullptr, c->adj_t3.p, c->adj_u3.p, c->adj_u4.p,
                     c->adj_live.p);
     adj_add(c, N, c->adj_t1.p, c->adj_t3.p, c->adj_t3.p);

@@ -21,7 +21,7 @@ struct AdjGeom {
     const int* sid; const int* off;
     const double* Lx; const double* Ly; const double* iLx; const double* iLy; const double* e;
     const int* boundary; const int* nn; const int* nbr; int n;
-    const double* live;   // backbone mask (1 = in the rigid backbone); null = every particle
+    const double* live;   // contact mask (1 = the particle touches something); null = every particle
 };
 
 AdjGeom geom_current(dpmd_ctx* c) {
@@ -310,12 +310,16 @@ __global__ void k_mr_commit(int S, const double* mr, int* active) {
     if (s < S && active[s] && mr[(size_t)s * MR_STRIDE + MR_STOP] != 0.0) active[s] = 0;
 }
 
-// One pass of rattler removal: a particle stays in the backbone while it has at least
-// d + 1 = 3 contacts (pair contacts with backbone particles, plus wall contacts).  Repeated
-// until nothing changes, this leaves the rigid backbone (the 3-core of the contact network);
-// the removed particles' coordinates are (near-)null directions of the Hessian.
+// The contact mask: a particle is kept while it has a contact (a pair contact with a kept
+// particle, or a wall contact); the coordinates of a particle without contacts are exact null
+// directions of the Hessian.  Every contact stays, also one that carries (almost) no force:
+// in the linear response it is a spring of stiffness k, and a particle between two such
+// contacts is a strut that the response goes through.  (Dropping particles with fewer than
+// three contacts, the rigid backbone, loses those struts: in one of the 1024-disk packings it
+// changed entries of dP/dr by up to 40 %, against central differences that agree with the
+// full minimum-norm solve.)
 template <int MODE>
-__global__ void __launch_bounds__(DPMD_BLOCK) k_kcore(AdjGeom g, const double* lin, double* lout, int* changed) {
+__global__ void __launch_bounds__(DPMD_BLOCK) k_touching(AdjGeom g, const double* lin, double* lout, int* changed) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.n) return;
     if (lin[i] == 0.0) { lout[i] = 0.0; return; }
@@ -327,7 +331,7 @@ __global__ void __launch_bounds__(DPMD_BLOCK) k_kcore(AdjGeom g, const double* l
         cnt += (xi < ri) + (xi > g.Lx[s] - ri) + (yi < ri) + (yi > g.Ly[s] - ri);
     }
     for_contacts(g, i, s, wall, [&](int j, double, double, double, double, double, double) { cnt += lin[j] != 0.0; });
-    const bool keep = cnt >= 3;
+    const bool keep = cnt >= 1;
     lout[i] = keep ? 1.0 : 0.0;
     if (!keep) atomicOr(changed, 1);
 }
@@ -597,8 +601,8 @@ void adj_to_canonical(dpmd_ctx* c, const double* cur, double* canon) {
     HIP_OK(hipGetLastError());
 }
 
-// The rigid backbone of the current contact network, into c->adj_live (1 / 0 per particle).
-const double* adj_backbone(dpmd_ctx* c, int mode) {
+// The contact mask of the current contact network, into c->adj_live (1 / 0 per particle).
+const double* adj_touching(dpmd_ctx* c, int mode) {
     const int N = c->N;
     hipStream_t st = c->st;
     c->adj_live.resize(N); c->adj_live2.resize(N); c->adj_flag.resize(1);
@@ -610,7 +614,7 @@ const double* adj_backbone(dpmd_ctx* c, int mode) {
     for (int pass = 0; changed && pass < 100000; ++pass) {
         fill_i32(st, c->adj_flag.p, 1, 0);
         by_mode(mode, [&](auto m) {
-            k_kcore<decltype(m)::value><<<blocks_for(N), DPMD_BLOCK, 0, st>>>(g, a, b, c->adj_flag.p);
+            k_touching<decltype(m)::value><<<blocks_for(N), DPMD_BLOCK, 0, st>>>(g, a, b, c->adj_flag.p);
         });
         HIP_OK(hipGetLastError());
         std::swap(a, b);
@@ -621,7 +625,7 @@ const double* adj_backbone(dpmd_ctx* c, int mode) {
     return c->adj_live.p;
 }
 
-double adj_backbone_max_force(dpmd_ctx* c, const double* live) {
+double adj_touching_max_force(dpmd_ctx* c, const double* live) {
     const int N = c->N, S = c->S;
     c->adj_bx.resize(N);
     ensure(c->adj_sa, S, c->st);
@@ -672,14 +676,13 @@ void adj_vjp_current(dpmd_ctx* c, int mode, const double* gU, const double* gP, 
     }
 }
 
-// Solves (H + shift I) x = b per system by MINRES (current order).  With on_backbone the
-// Hessian is restricted to the rigid backbone (backbone particles and their mutual contacts)
-// and b and the result are projected onto the complement of its null directions
-// (non-backbone coordinates; translations of the backbone in periodic systems); the mask
-// stays in c->adj_live.  Otherwise every particle takes part (use shift > 0).  Returns the
+// Solves (H + shift I) x = b per system by MINRES (current order).  With on_touching the
+// Hessian is restricted to the particles with contacts, and b and the result are projected
+// onto the complement of its null directions (coordinates of particles without contacts;
+// uniform translations in periodic systems); the mask stays in c->adj_live.  Otherwise every particle takes part (use shift > 0).  Returns the
 // iterations taken; *rel_resid = max_s |b - H x| / |b| (true residual, recomputed at the end).
 int adj_cg_current(dpmd_ctx* c, int mode, const double* bx, const double* by, double* xx, double* xy, double tol,
-                   int max_iter, double* rel_resid, bool on_backbone, double shift) {
+                   int max_iter, double* rel_resid, bool on_touching, double shift) {
     const int N = c->N, S = c->S;
     hipStream_t st = c->st;
     ensure(c->mr_vec, 12 * (int64_t)N, st);
@@ -689,12 +692,12 @@ int adj_cg_current(dpmd_ctx* c, int mode, const double* bx, const double* by, do
     c->cg_active.resize(S);
     const int G = blocks_for(N), GS = blocks_for(S);
     auto vec = [&](int k) { return V2{c->mr_vec.p + (size_t)(2 * k) * N, c->mr_vec.p + (size_t)(2 * k + 1) * N}; };
-    // null directions: coordinates outside the rigid backbone and (periodic systems)
-    // translations of the backbone
+    // null directions: coordinates of particles without contacts and (periodic systems)
+    // uniform translations
     ensure(c->cg_beta, S, st);
     const double* live = nullptr;
-    if (on_backbone) {
-        live = adj_backbone(c, mode);
+    if (on_touching) {
+        live = adj_touching(c, mode);
     } else {   // every particle: with shift > 0 the shifted matrix is definite
         c->adj_live.resize(N);
         fill_f64(st, c->adj_live.p, N, 1.0);

@@ -205,7 +205,7 @@ struct dpmd_ctx {
     dpmd::DVec<double> cg_rr, cg_rrn, cg_pq, cg_bb, cg_beta;
     dpmd::DVec<int> cg_active;
     dpmd::DVec<double> mr_vec, mr_s;   // MINRES work vectors (6 x 2N) and per-system scalars
-    dpmd::DVec<double> adj_live, adj_live2;   // rigid-backbone mask (and its double buffer)
+    dpmd::DVec<double> adj_live, adj_live2;   // contact mask (and its double buffer)
     dpmd::DVec<int> adj_flag;
     dpmd::DVec<double> ck_state;     // n_ck x 4N: x, y, vx, vy (canonical) at every checkpoint
     dpmd::DVec<double> ck_seg;       // (every + 1) x 2N positions of one replayed segment
@@ -324,10 +324,10 @@ void adj_hvp_current(dpmd_ctx* c, int mode, const double* wx, const double* wy, 
                      const double* live = nullptr, double shift = 0.0);
 void adj_vjp_current(dpmd_ctx* c, int mode, const double* gU, const double* gP, const double* gfx, const double* gfy,
                      double* dx, double* dy, double* dr, double* dLx, double* dLy, const double* live = nullptr);
-const double* adj_backbone(dpmd_ctx* c, int mode);
-double adj_backbone_max_force(dpmd_ctx* c, const double* live);
+const double* adj_touching(dpmd_ctx* c, int mode);
+double adj_touching_max_force(dpmd_ctx* c, const double* live);
 int adj_cg_current(dpmd_ctx* c, int mode, const double* bx, const double* by, double* xx, double* xy, double tol,
-                   int max_iter, double* rel_resid, bool on_backbone = true, double shift = 0.0);
+                   int max_iter, double* rel_resid, bool on_touching = true, double shift = 0.0);
 void adj_record_positions(dpmd_ctx* c, Gate g);
 void adj_list_canonical(dpmd_ctx* c, int* inv_scratch);
 void adj_seg_disp2(dpmd_ctx* c, int base, int len, double* d2);

@@ -70,9 +70,9 @@ def pressure(x, y, r, m, vx, vy, Lx, Ly, k, wall):
     return (np.sum(m * (vx * vx + vy * vy)) + W) / (2.0 * Lx * Ly)
 
 
-def backbone(x, y, r, Lx, Ly, wall):
-    """Rigid backbone: repeatedly drop particles with fewer than d + 1 = 3 contacts (pair
-    contacts with kept particles plus wall contacts); returns the boolean mask."""
+def touching(x, y, r, Lx, Ly, wall):
+    """Contact mask: the particles with at least one contact (pair or wall); the coordinates
+    of the others are exact null directions of the Hessian.  Returns the boolean mask."""
     n = len(x)
     I, J, *_ = contacts(x, y, r, Lx, Ly, wall)
     wc = np.zeros(n, int)
@@ -81,7 +81,7 @@ def backbone(x, y, r, Lx, Ly, wall):
     live = np.ones(n, bool)
     while True:
         cnt = wc + np.bincount(I[live[J]], minlength=n) + np.bincount(J[live[I]], minlength=n)
-        new = live & (cnt >= 3)
+        new = live & (cnt >= 1)
         if np.array_equal(new, live):
             return live
         live = new
@@ -170,11 +170,11 @@ def grads(x, y, r, m, vx, vy, Lx, Ly, k, wall, live=None):
 
 
 def implicit_pressure_grad(x, y, r, m, vx, vy, Lx, Ly, k, wall, live=None):
-    """dP/dr and dP/dL through the minimum: dP/dq + lambda^T dF/dq, H lambda = dP/dx, on the
-    rigid backbone (or the given mask; minimum-norm least squares, which removes translations
-    and the coordinates outside the mask)."""
+    """dP/dr and dP/dL through the minimum: dP/dq + lambda^T dF/dq, H lambda = dP/dx, over the
+    particles with contacts (or the given mask; minimum-norm least squares, which removes
+    translations and the coordinates outside the mask)."""
     if live is None:
-        live = backbone(x, y, r, Lx, Ly, wall)
+        live = touching(x, y, r, Lx, Ly, wall)
     g = grads(x, y, r, m, vx, vy, Lx, Ly, k, wall, live)
     H = hessian(x, y, r, Lx, Ly, k, wall, live)
     lam = np.linalg.lstsq(H, g["dPdx"], rcond=1e-13)[0]

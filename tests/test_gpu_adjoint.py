@@ -98,8 +98,8 @@ def test_cg_solution_and_pressure_gradient_match_host(minimum):
 
 def test_pressure_gradient_full_size_systems_match_host():
     """Systems of the differentiation workload's size (1024 disks at phi = 0.86, one periodic,
-    one walled): the GPU implicit gradient (MINRES on the rigid backbone) against the dense
-    host reference (least squares on the same backbone)."""
+    one walled): the GPU implicit gradient (MINRES over the particles with contacts) against
+    the dense host reference (least squares over the same particles)."""
     b = W.disk_batch([1024, 1024], [0.86, 0.86], [0, 1], seed=1)
     P = make(b)
     print(f"adjoint full-size S={b.S} N={b.N}", flush=True)

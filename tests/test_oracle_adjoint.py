@@ -111,8 +111,9 @@ def test_ring_implicit_pressure_closed_form():
     T = k (2 sum r - Lx) / 3, and the virial W = sum k delta d = T Lx, so
     P = k (2 sum r - Lx) / (6 Ly):  dP/dr_m = k / (3 Ly) for every disk (relaxation included;
     the unrelaxed partial k (d_left + d_right) / (2 Lx Ly) differs for the middle disk),
-    dP/dLx = -k / (6 Ly) and dP/dLy = -P / Ly.  (A ring is a one-dimensional network: the 2-D
-    backbone rule, three contacts, would drop it, so the full network is used.)"""
+    dP/dLx = -k / (6 Ly) and dP/dLy = -P / Ly.  Every disk has two contacts, so the contact
+    mask keeps them all (a rule that drops particles with fewer than three contacts, the 2-D
+    rigid backbone, would lose the whole ring)."""
     Lx, Ly, k = 3.0, 10.0, 1.0
     r = np.array([0.55, 0.6, 0.5])
     x = np.array([0.5, 1.55, 2.55]); y = np.full(3, 5.0); m = np.ones(3); z = np.zeros(3)
@@ -120,8 +121,8 @@ def test_ring_implicit_pressure_closed_form():
     assert np.max(np.abs(fx)) < 1e-14 and np.max(np.abs(fy)) == 0.0
     P = R.pressure(x, y, r, m, z, z, Lx, Ly, k, False)
     assert P == pytest.approx(k * (2 * r.sum() - Lx) / (6 * Ly), rel=1e-13)
-    assert not R.backbone(x, y, r, Lx, Ly, False).any()
-    dr, dL, lam = R.implicit_pressure_grad(x, y, r, m, z, z, Lx, Ly, k, False, live=np.ones(3, bool))
+    assert R.touching(x, y, r, Lx, Ly, False).all()
+    dr, dL, lam = R.implicit_pressure_grad(x, y, r, m, z, z, Lx, Ly, k, False)
     np.testing.assert_allclose(dr, [k / (3 * Ly)] * 3, rtol=1e-12)
     np.testing.assert_allclose(dL, [-k / (6 * Ly), -P / Ly], rtol=1e-12)
     g = R.grads(x, y, r, m, z, z, Lx, Ly, k, False)
@@ -163,6 +164,56 @@ def test_implicit_pressure_matches_fd_through_reminimisation(wall):
         fd = (P(LL=LL + d) - P(LL=LL - d)) / (2 * h)
         assert fd == pytest.approx(dL[ax], rel=1e-6)
     assert checked == 4
+
+
+def test_implicit_pressure_through_struts_matches_fd():
+    """A packing of the differentiation workload (system 11 of the 128 x 1024 batch at
+    phi = 0.86, FIRE to max|F| < 1e-12 on the oracle) holds struts: particles between two
+    contacts that carry almost no force.  In the linear response such a contact is a spring of
+    stiffness k, so dP/dr_i goes through them.  The implicit derivative over the particles
+    with contacts matches central differences through re-minimisation (h = 1e-6, 1e-6
+    relative) where the contact network (contacts of overlap above 1e-10) is unchanged at +-h;
+    dropping every particle with
+    fewer than three contacts (the rigid backbone) misses the struts and is 2 % off here."""
+    b = W.system_of(W.disk_batch([1024] * 128, [0.86] * 128, [0] * 128, seed=1), 11)
+    o = orc.OracleDisk(b)
+    assert o.fire_minimize(1e-2, 300000, pred=1, ftol=1e-12)[1]
+    si = o.static_index()
+    x1, y1 = o.get("x")[si], o.get("y")[si]
+    r, m, Lx, Ly, k = b.rad, b.mass, b.Lx[0], b.Ly[0], b.e[0]
+    z = np.zeros(b.N)
+    i, h = 960, 1e-6
+
+    def minimum(rr):   # FIRE to max|F| < 1e-13 from the packing
+        q = W.system_of(b, 0)
+        q.rad, q.x, q.y = rr.copy(), x1.copy(), y1.copy()
+        oq = orc.OracleDisk(q)
+        oq.fire_minimize(1e-2, 300000, pred=1, ftol=1e-13)
+        s2 = oq.static_index()
+        xq, yq = oq.get("x")[s2], oq.get("y")[s2]
+        I, J, _, _, _, de = R.contacts(xq, yq, rr, Lx, Ly, False)
+        firm = de > 1e-10   # contacts that do not open within the minimiser's tolerance
+        return xq, yq, R.pressure(xq, yq, rr, m, z, z, Lx, Ly, k, False), set(zip(I[firm].tolist(), J[firm].tolist()))
+
+    x, y, _, c0 = minimum(r)
+    dr, _, _ = R.implicit_pressure_grad(x, y, r, m, z, z, Lx, Ly, k, False)
+    e = np.zeros(b.N); e[i] = h
+    (_, _, pp, cp), (_, _, pm, cm) = minimum(r + e), minimum(r - e)
+    assert cp == cm == c0
+    fd = (pp - pm) / (2 * h)
+    assert fd == pytest.approx(dr[i], rel=1e-6)
+    I0, J0, *_ = R.contacts(x, y, r, Lx, Ly, False)
+    cnt = np.bincount(I0, minlength=b.N) + np.bincount(J0, minlength=b.N)
+    live3 = np.ones(b.N, bool)
+    while True:   # the rigid backbone (3-core), for contrast
+        c3 = np.bincount(I0[live3[J0]], minlength=b.N) + np.bincount(J0[live3[I0]], minlength=b.N)
+        nxt = live3 & (c3 >= 3)
+        if np.array_equal(nxt, live3):
+            break
+        live3 = nxt
+    assert (cnt > 0).sum() > live3.sum()
+    dr3, _, _ = R.implicit_pressure_grad(x, y, r, m, z, z, Lx, Ly, k, False, live=live3)
+    assert abs(dr3[i] - fd) > 1e-2 * abs(fd)
 
 
 def test_vv_backprop_matches_fd():
