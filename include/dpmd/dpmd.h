@@ -315,18 +315,16 @@ int dpmd_adjoint_hvp_noalloc(dpmd_ctx* ctx, int mode, const double* wx, const do
  * sum_s gU_s U_s + gP_s P_s + sum_i gF_i . F_i (any output may be NULL). */
 int dpmd_adjoint_vjp(dpmd_ctx* ctx, int mode, const double* gU, const double* gP, const double* gfx,
                      const double* gfy, double* dx, double* dy, double* dr, double* dLx, double* dLy);
-/* Solves H x = b per system by MINRES to |b - Hx| <= tol |b| on the rigid backbone: particles
- * are dropped repeatedly while they have fewer than 3 contacts (pair contacts with kept
- * particles plus wall contacts); H is restricted to the kept particles and their mutual
- * contacts, and b and x are projected off its null directions (dropped coordinates, and
- * uniform translations in periodic systems).  rel_resid: max over systems of the true
- * relative residual. */
+/* Solves H x = b per system by MINRES to |b - Hx| <= tol |b| over the particles with a
+ * contact (pair or wall): b and x are projected off H's null directions (the coordinates of
+ * particles without contacts, and uniform translations in periodic systems).  rel_resid: max
+ * over systems of the true relative residual. */
 int dpmd_adjoint_solve(dpmd_ctx* ctx, int mode, const double* bx, const double* by, double* xx, double* xy,
                        double tol, int max_iter, int* iters, double* rel_resid);
 /* Implicit derivative of each system's pressure through its energy minimum (call at a
  * FIRE / Newton minimum): dP_s/dr_i and dP_s/dL_s = dP/dq + lambda^T dF/dq with
- * H lambda = dP/dx solved to tol on the rigid backbone (as dpmd_adjoint_solve; dF/dq keeps
- * the backbone's mutual contacts).  dr: [N]; dLx, dLy: [S]; any may be NULL. */
+ * H lambda = dP/dx solved to tol as dpmd_adjoint_solve does.  dr: [N]; dLx, dLy: [S]; any
+ * may be NULL. */
 int dpmd_adjoint_pressure(dpmd_ctx* ctx, int mode, double tol, int max_iter, double* dr, double* dLx,
                           double* dLy, int* iters, double* rel_resid);
 /* Levenberg-Marquardt Newton iterations x <- x + (H + 1e-10 k I)^-1 F from near a minimum
