@@ -1,6 +1,8 @@
 // C ABI (include/dpmd/dpmd.h): context lifecycle, fields, validation, and the host-side
 // orchestration of the particle / integrator / routine calls.
 #include "internal.hpp"
+#include <array>
+#include <map>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -374,7 +376,7 @@ RunCtl read_run(dpmd_ctx* c) {
 // W(q) (RUN_ANY for even q, RUN_ANY2 for odd q) and gate the launch that computes forces at
 // x_q; a halted batch copies x_q back to A before the host finishes the step.
 bool fused_vv_batch(dpmd_ctx* c, int m, bool fire) {
-    return !fire && !rigid(c) && c->fuse_vv && !c->rec_seg && m >= 2 && m % 2 == 0;
+    return !fire && !rigid(c) && c->fuse_vv && m >= 2 && m % 2 == 0;
 }
 int any_word(int q) { return q % 2 == 0 ? RUN_ANY : RUN_ANY2; }
 
@@ -420,6 +422,8 @@ void lean_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b, boo
     if (!dev && fused_vv_batch(c, m, fire)) {
         double* P[2] = {c->xy0.p, c->xy1.p};
         disk_vv_drift_to(c, Gate{nullptr, c->run.p, 0}, P[0], any_word(0));
+        // a recording replay (reverse mode) stores the positions after every step's drift
+        if (c->rec_seg) adj_record_positions(c, Gate{nullptr, c->run.p, 0}, P[0]);
         const bool prof = c->prof == 2 && !c->capturing;
         if (prof) {   // one pair around the batch's fused launches; kept only if no rebuild halts it
             HIP_OK(hipEventCreate(&c->pend_b)); HIP_OK(hipEventCreate(&c->pend_e));
@@ -430,6 +434,7 @@ void lean_batch(dpmd_ctx* c, int m, bool fire, int pred, double a, double b, boo
             Gate g{nullptr, c->run.p, q};
             g.any = any_word(q);
             disk_vv_fused(c, g, P[q % 2], P[(q + 1) % 2], any_word(q + 1));
+            if (c->rec_seg) adj_record_positions(c, Gate{nullptr, c->run.p, q + 1}, P[(q + 1) % 2]);
         }
         if (prof) HIP_OK(hipEventRecord(c->pend_e, c->st));
         Gate gf{nullptr, c->run.p, m - 1};
@@ -781,19 +786,21 @@ void vv_backprop_impl(dpmd_ctx* c, const double* gx, const double* gy, const dou
     const int mode = c->vv_mode;
     hipStream_t st = c->st;
     c->adj_t0.resize(2 * (int64_t)N); c->adj_t1.resize(2 * (int64_t)N); c->adj_t2.resize(2 * (int64_t)N);
-    c->adj_t3.resize(2 * (int64_t)N); c->adj_t4.resize(N); c->adj_t5.resize(N); c->adj_bx.resize(N);
-    c->adj_t6.resize(2 * (int64_t)N);
+    c->adj_bx.resize(N);
     c->adj_u0.resize(S);
-    double *xb = c->adj_t0.p, *vb = c->adj_t1.p, *w_c = c->adj_t2.p, *w_n = c->adj_t6.p;
-    double *rad_c = c->adj_t4.p, *m_c = c->adj_t5.p;
+    // x-bar, v-bar and w in canonical order between windows
+    double *xb = c->adj_t0.p, *vb = c->adj_t1.p, *w_c = c->adj_t2.p;
     auto in = [&](const double* src, double* dst) {
         if (src) HIP_OK(hipMemcpyAsync(dst, src, (size_t)N * 8, hipMemcpyDeviceToDevice, st));
         else fill_f64(st, dst, N, 0.0);
     };
     in(gx, xb); in(gy, xb + N); in(gvx, vb); in(gvy, vb + N);
-    adj_to_canonical(c, c->rad.p, rad_c);
-    adj_to_canonical(c, c->mass.p, m_c);
     c->ck_seg.resize((int64_t)(every + 1) * 2 * N);
+    c->ck_cur.resize((int64_t)(every + 1) * 2 * N);
+    c->adj_cur.resize(8 * (int64_t)N);
+    c->ck_nn.resize(N);
+    c->ck_nbr.resize((int64_t)N * c->K);
+    double *xc = c->adj_cur.p, *vc = xc + 2 * (size_t)N, *wA = xc + 4 * (size_t)N, *wB = xc + 6 * (size_t)N;
     std::vector<double> cut0(S), d2(S), cut(S), Lx(S), Ly(S), rmax(S, 0.0);
     HIP_OK(hipMemcpyAsync(cut0.data(), c->cutoff.p, (size_t)S * 8, hipMemcpyDeviceToHost, st));
     HIP_OK(hipMemcpyAsync(Lx.data(), c->Lx.p, (size_t)S * 8, hipMemcpyDeviceToHost, st));
@@ -803,6 +810,15 @@ void vv_backprop_impl(dpmd_ctx* c, const double* gx, const double* gy, const dou
         for (int s = 0; s < S; ++s)
             for (int i = c->h_off[s]; i < c->h_off[s + 1]; ++i) rmax[s] = std::max(rmax[s], rad[i]);
     }
+    // captured reverse windows, destroyed once the stream has drained
+    struct RevGraphs {
+        hipStream_t st;
+        std::map<std::array<int64_t, 6>, hipGraphExec_t> g;
+        ~RevGraphs() {
+            (void)hipStreamSynchronize(st);
+            for (auto& kv : g) (void)hipGraphExecDestroy(kv.second);
+        }
+    } rev_graphs{st, {}};
     bool first = true;
     for (int j = J - 1; j >= 0; --j) {
         const int k0 = j * every, k1 = std::min(k0 + every, n), len = k1 - k0;
@@ -849,24 +865,78 @@ void vv_backprop_impl(dpmd_ctx* c, const double* gx, const double* gy, const dou
             }
             restore_positions(c, c->ck_seg.p + (size_t)(a - k0) * 2 * N);
             check_overflow(c);
-            adj_list_canonical(c, c->tmp_i.p);
+            // The window runs in the context's current order (the cells of x_a): its contact
+            // list as built, and the recorded positions of slots a .. hi and the adjoint
+            // vectors permuted into that order, so that a particle's neighbors lie near it in
+            // memory.  Every per-particle sum runs over the same list in the same order as in
+            // canonical order, so the results are the same bits.
+            HIP_OK(hipMemcpyAsync(c->ck_nn.p, c->nn.p, (size_t)N * 4, hipMemcpyDeviceToDevice, st));
+            HIP_OK(hipMemcpyAsync(c->ck_nbr.p, c->nbr.p, (size_t)N * c->K * 4, hipMemcpyDeviceToDevice, st));
+            adj_rows_to_current(c, c->ck_seg.p + (size_t)(a - k0) * 2 * N, c->ck_cur.p + (size_t)(a - k0) * 2 * N,
+                                2 * (hi - a + 1));
+            adj_rows_to_current(c, xb, xc, 2);
+            adj_rows_to_current(c, vb, vc, 2);
+            adj_rows_to_current(c, w_c, wA, 2);
             if (enlarge) {
                 c->cutoff.upload(cut0.data(), S, st);
                 setup_cells(c, CELLS_BOX_CHANGED);
             }
+            const double* rad_w = c->rad.p;
+            const double* m_w = c->mass.p;
+            double* wc = wA;
+            double* wn = wB;
             // Each launch is one reverse step (Hessian product, xbar update) plus the next
             // step's vbar and w, which need only per-particle data; w ping-pongs between
             // two buffers because the product reads the neighbors' w.
             if (first) {
-                adj_rev_w(c, m_c, vb, w_c);
-                adj_rev_step(c, mode, hi - k0, rad_c, m_c, w_c, 0.5, xb, vb, w_n, n > 0);
-                std::swap(w_c, w_n);
+                adj_rev_w(c, m_w, vc, wc);
+                adj_rev_step(c, mode, hi - k0, rad_w, m_w, wc, 0.5, xc, vc, wn, n > 0, c->ck_cur.p);
+                std::swap(wc, wn);
                 first = false;
             }
-            for (int k = hi - 1; k >= a; --k) {
-                adj_rev_step(c, mode, k - k0, rad_c, m_c, w_c, k > 0 ? 1.0 : 0.5, xb, vb, w_n, k > 0);
-                std::swap(w_c, w_n);
+            auto steps = [&](double* w0, double* w1) {
+                for (int k = hi - 1; k >= a; --k) {
+                    adj_rev_step(c, mode, k - k0, rad_w, m_w, w0, k > 0 ? 1.0 : 0.5, xc, vc, w1, k > 0, c->ck_cur.p);
+                    std::swap(w0, w1);
+                }
+            };
+            if (use_graph && hi - a >= 4) {
+                // Every argument of a window's launches is fixed by its first and last slot in
+                // the segment, whether it ends at step 0 and the w buffer it starts from (the
+                // other buffers are the context's), so windows of one shape (the full segments)
+                // replay one captured graph instead of launching step by step.
+                const std::array<int64_t, 6> key = {hi - k0, a - k0, a == 0 ? 1 : 0, (int64_t)(intptr_t)wc,
+                                                    (int64_t)(intptr_t)c->ck_nn.p, (int64_t)(intptr_t)c->ck_cur.p};
+                auto it = rev_graphs.g.find(key);
+                if (it == rev_graphs.g.end()) {
+                    hipGraph_t gr = nullptr;
+                    HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+                    c->capturing = true;
+                    try {
+                        steps(wc, wn);
+                    } catch (...) {
+                        c->capturing = false;
+                        (void)hipStreamEndCapture(st, &gr);
+                        if (gr) (void)hipGraphDestroy(gr);
+                        throw;
+                    }
+                    c->capturing = false;
+                    HIP_OK(hipStreamEndCapture(st, &gr));
+                    hipGraphExec_t ex = nullptr;
+                    const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(gr);
+                    if (e != hipSuccess) throw Error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+                    it = rev_graphs.g.emplace(key, ex).first;
+                }
+                HIP_OK(hipGraphLaunch(it->second, st));
+            } else {
+                steps(wc, wn);
             }
+            if ((hi - a) % 2) std::swap(wc, wn);
+            // back to canonical order for the next window (another cell order)
+            adj_rows_to_canonical(c, xc, xb, 2);
+            adj_rows_to_canonical(c, vc, vb, 2);
+            adj_rows_to_canonical(c, wc, w_c, 2);
             hi = a;
         }
     }

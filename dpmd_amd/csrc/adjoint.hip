@@ -210,6 +210,22 @@ __global__ void k_to_current(int n, const int* __restrict__ si, const double* __
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < n) cur[si[c]] = canon[c];
 }
+// rows of a row-major array of n-vectors from canonical into current order, and back
+__global__ void k_rows_to_current(int n, const int* __restrict__ si, const double* __restrict__ src,
+                                  double* __restrict__ dst) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const size_t r = (size_t)blockIdx.y * n;
+    dst[r + si[c]] = src[r + c];
+}
+__global__ void k_rows_to_canonical(int n, const int* __restrict__ si, const double* __restrict__ src,
+                                    double* __restrict__ dst) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const size_t r = (size_t)blockIdx.y * n;
+    dst[r + c] = src[r + si[c]];
+}
+
 __global__ void k_to_canonical(int n, const int* __restrict__ si, const double* __restrict__ cur, double* canon) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < n) canon[c] = cur[si[c]];
@@ -487,14 +503,16 @@ __global__ void k_seg_disp2(int n, int base, int len, const double* seg, double*
 }
 
 // Positions of the step just drifted, in canonical order, into slot run[RUN_REC] + k + 1.
-__global__ void k_record_pos(int n, const int* si, const double* x, const double* y, double* seg, Gate g) {
+__global__ void k_record_pos(int n, const int* si, const double* x, const double* y, const double2* xy, double* seg,
+                             Gate g) {
     if (!gate_open(g)) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const size_t slot = (size_t)(g.run[RUN_REC] + g.k + 1);
     const int i = si[c];
-    seg[slot * 2 * n + c] = x[i];
-    seg[slot * 2 * n + n + c] = y[i];
+    const double2 p = xy ? xy[i] : make_double2(x[i], y[i]);
+    seg[slot * 2 * n + c] = p.x;
+    seg[slot * 2 * n + n + c] = p.y;
 }
 
 template <class F>
@@ -513,8 +531,9 @@ void ensure(dpmd::DVec<double>& v, int64_t n, hipStream_t st) {
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
-void adj_record_positions(dpmd_ctx* c, Gate g) {
-    k_record_pos<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(c->N, c->static_index.p, c->x.p, c->y.p, c->rec_seg, g);
+void adj_record_positions(dpmd_ctx* c, Gate g, const double* xy) {
+    k_record_pos<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(c->N, c->static_index.p, c->x.p, c->y.p,
+                                                            (const double2*)xy, c->rec_seg, g);
     HIP_OK(hipGetLastError());
 }
 
@@ -564,10 +583,10 @@ void adj_hvp_segment(dpmd_ctx* c, int mode, int slot, const double* rad_c, const
 }
 
 void adj_rev_step(dpmd_ctx* c, int mode, int slot, const double* rad_c, const double* m, const double* w, double coef,
-                  double* xb, double* vb, double* w_next, bool next) {
+                  double* xb, double* vb, double* w_next, bool next, const double* seg) {
     const int N = c->N;
     AdjGeom g = geom_current(c);
-    g.x = c->ck_seg.p + (size_t)slot * 2 * N;
+    g.x = seg + (size_t)slot * 2 * N;
     g.y = g.x + N;
     g.rad = rad_c;
     g.nn = c->ck_nn.p;
@@ -589,6 +608,17 @@ void adj_norm2(dpmd_ctx* c, const double* ax, const double* ay, double* out) {
 }
 void adj_axpy2(dpmd_ctx* c, double a, const double* xx, const double* xy, double* yx, double* yy) {
     k_axpy2<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(c->N, a, xx, xy, yx, yy);
+    HIP_OK(hipGetLastError());
+}
+
+void adj_rows_to_current(dpmd_ctx* c, const double* canon, double* cur, int rows) {
+    if (rows <= 0) return;
+    k_rows_to_current<<<dim3(blocks_for(c->N), rows), DPMD_BLOCK, 0, c->st>>>(c->N, c->static_index.p, canon, cur);
+    HIP_OK(hipGetLastError());
+}
+void adj_rows_to_canonical(dpmd_ctx* c, const double* cur, double* canon, int rows) {
+    if (rows <= 0) return;
+    k_rows_to_canonical<<<dim3(blocks_for(c->N), rows), DPMD_BLOCK, 0, c->st>>>(c->N, c->static_index.p, cur, canon);
     HIP_OK(hipGetLastError());
 }
 

@@ -209,6 +209,8 @@ struct dpmd_ctx {
     dpmd::DVec<int> adj_flag;
     dpmd::DVec<double> ck_state;     // n_ck x 4N: x, y, vx, vy (canonical) at every checkpoint
     dpmd::DVec<double> ck_seg;       // (every + 1) x 2N positions of one replayed segment
+    dpmd::DVec<double> ck_cur;       // the same slots of a reverse window, in the window's current order
+    dpmd::DVec<double> adj_cur;      // 8N: x-bar, v-bar and the two w buffers of a reverse window
     dpmd::DVec<int> ck_nn, ck_nbr;   // canonical-order contact list for a segment
     int ck_steps = 0, ck_every = 0, ck_count = 0;
     double* rec_seg = nullptr;       // when set, lean VV batches record positions per step
@@ -319,6 +321,9 @@ void check_overflow(dpmd_ctx* c);
 
 // adjoint.hip (disks; particle vectors in current order unless named canonical)
 void adj_to_current(dpmd_ctx* c, const double* canon, double* cur);
+// `rows` consecutive N-vectors from canonical into current order, and back
+void adj_rows_to_current(dpmd_ctx* c, const double* canon, double* cur, int rows);
+void adj_rows_to_canonical(dpmd_ctx* c, const double* cur, double* canon, int rows);
 void adj_to_canonical(dpmd_ctx* c, const double* cur, double* canon);
 void adj_hvp_current(dpmd_ctx* c, int mode, const double* wx, const double* wy, double* yx, double* yy,
                      const double* live = nullptr, double shift = 0.0);
@@ -328,15 +333,18 @@ const double* adj_touching(dpmd_ctx* c, int mode);
 double adj_touching_max_force(dpmd_ctx* c, const double* live);
 int adj_cg_current(dpmd_ctx* c, int mode, const double* bx, const double* by, double* xx, double* xy, double tol,
                    int max_iter, double* rel_resid, bool on_touching = true, double shift = 0.0);
-void adj_record_positions(dpmd_ctx* c, Gate g);
+// positions after a step's drift into the recording segment (canonical order); xy: read them
+// from an interleaved (x, y) buffer instead of the context's arrays
+void adj_record_positions(dpmd_ctx* c, Gate g, const double* xy = nullptr);
 void adj_list_canonical(dpmd_ctx* c, int* inv_scratch);
 void adj_seg_disp2(dpmd_ctx* c, int base, int len, double* d2);
 void adj_rev_vw(dpmd_ctx* c, const double* m, double* xb, double* vb, double* w);
 void adj_rev_w(dpmd_ctx* c, const double* m, const double* vb, double* w);
 void adj_rev_x(dpmd_ctx* c, double coef, const double* z, double* xb);
 void adj_hvp_segment(dpmd_ctx* c, int mode, int slot, const double* rad_c, const double* w, double* z);
+// one reverse step with positions from slot `slot` of the segment array `seg` (2N per slot)
 void adj_rev_step(dpmd_ctx* c, int mode, int slot, const double* rad_c, const double* m, const double* w, double coef,
-                  double* xb, double* vb, double* w_next, bool next);
+                  double* xb, double* vb, double* w_next, bool next, const double* seg);
 void adj_add(dpmd_ctx* c, int64_t n, const double* a, const double* b, double* out);
 void adj_axpy2(dpmd_ctx* c, double a, const double* xx, const double* xy, double* yx, double* yy);
 void adj_norm2(dpmd_ctx* c, const double* ax, const double* ay, double* out);
