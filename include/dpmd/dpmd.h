@@ -402,8 +402,8 @@ int dpmd_stream_copy(double* dst, const double* src, int64_t n, int repeats, dou
  *     per system before dropping the system; default 0 (drop at once).
  *   DPMD_OPT_FUSED_FIRE: 1 (default) = device-driven FIRE runs of one disk system compute a
  *     step's forces, kick and power partials and the next step's drift in one launch, the
- *     drift taken with the scalars that F.v > 0 gives and redone by a fixup launch when the
- *     power fold shows otherwise (positions and velocities ping-pong between two buffer
+ *     drift taken with the scalars that F.v > 0 gives and redone by the fold's launch when
+ *     the power shows otherwise (positions and velocities ping-pong between two buffer
  *     pairs); 0 = drift, force and fold launches per step.  Same results. */
 enum dpmd_option {
     DPMD_OPT_COOP_REBUILD = 0, DPMD_OPT_COOP_IN_GRAPH = 1, DPMD_OPT_FUSED_VV = 2, DPMD_OPT_BLOCK = 3,

@@ -2,8 +2,9 @@
 batches and the oracle, bit for bit.
 
 A fused launch computes step k's forces, kick and power partials and then drifts step k + 1
-with the scalars that F.v > 0 would give; the fold checks that guess and a fixup launch redoes
-the drift (or, when the system stopped, writes step k's final state) when it was wrong.  The
+with the scalars that F.v > 0 would give; the settle launch folds the partials, checks that
+guess and redoes the drift (or, when the system stopped, writes step k's final state) when it
+was wrong.  The
 systems below start from random positions, so FIRE meets F.v <= 0 many times (counted on the
 oracle, which must see at least one such step per case so the fixup path is exercised), and
 rebuild halts land on steps of both parities.
