@@ -1992,12 +1992,16 @@ int dpmd_jam_compress(dpmd_ctx* c, const double* dt, double dphi, double p_targe
                 if (dts[s] == 0.0) continue;
                 nsteps[s] += ns[s];
                 total += (int64_t)(c->h_off[s + 1] - c->h_off[s]) * ns[s];
-                if (!stopped[s]) {
+                if (!stopped[s] && retries[s] < c->jam_retries) {
                     // a round that hit the step cap is run again from where it stopped (FIRE
-                    // restarted, no compression), up to jam_retries times, then the system is dropped
-                    if (retries[s] < c->jam_retries) { retries[s] += 1; retry = true; }
-                    else state[s] = -1;
+                    // restarted, no compression), up to jam_retries times
+                    retries[s] += 1;
+                    retry = true;
                 }
+                // the reference's rule (jammers.cuh:85-89, jammers.cu:7-17): a system whose
+                // initial minimisation does not stop is dropped; after a compression the
+                // state the minimisation reached decides, whether it stopped or not
+                else if (!stopped[s] && ncomp[s] == 0) state[s] = -1;
                 else if (P[s] >= p_target) state[s] = 1;
                 else if (ncomp[s] >= max_comp) state[s] = 2;
                 else { inc[s] = dphi; ncomp[s] += 1; grow = true; }

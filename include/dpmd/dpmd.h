@@ -267,8 +267,10 @@ int dpmd_fire_minimize(dpmd_ctx* ctx, const double* dt, int max_steps, int predi
  * P_s >= p_target is jammed; the others grow phi_s by dphi and go on, up to
  * max_compressions times.  jam_state (host, length S, may be NULL):
  *   1 jammed; 0 never run (dt[s] == 0) or still below p_target after max_compressions;
- *  -1 a minimisation did not stop within fire_max_steps (dropped, as jammers.cu:7-17 drops
- *     systems whose minimisation failed).
+ *  -1 the initial minimisation (before any compression) did not stop within fire_max_steps
+ *     (dropped, as jammers.cu:7-17 drops systems whose initial minimisation failed).  After a
+ *     compression the state a minimisation reached decides, whether it stopped or not, as
+ *     the reference's jammer decides after each minimisation (jammers.cuh:100-110).
  * n_compressions (host, S, may be NULL): compressions applied per system.  work (host, may
  * be NULL): sum over rounds and systems of N_s x (FIRE steps that did work).
  * DPMD_F_FIRE_STEPS then holds each system's FIRE steps summed over all rounds. */
@@ -399,7 +401,8 @@ int dpmd_stream_copy(double* dst, const double* src, int64_t n, int repeats, dou
  *     neighbor build; 0 (default): 1 when every system fits, else 2.  Same results.
  *   DPMD_OPT_JAM_RETRIES: dpmd_jam_compress runs a round whose minimisation hit the FIRE step
  *     cap again (FIRE restarted from where it stopped, no compression) up to this many times
- *     per system before dropping the system; default 0 (drop at once).
+ *     per system before the system is dropped (initial minimisation) or decided on the state
+ *     reached (later rounds); default 0.
  *   DPMD_OPT_FUSED_FIRE: 1 (default) = device-driven FIRE runs of one disk system compute a
  *     step's forces, kick and power partials and the next step's drift in one launch, the
  *     drift taken with the scalars that F.v > 0 gives and redone by the fold's launch when

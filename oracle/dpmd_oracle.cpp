@@ -1424,10 +1424,12 @@ static int fire_minimize_counted(Disk* d, const double* dt, int max_steps, int p
 // dpmd_jam_compress: minimise the systems still compressing (dt 0 for the rest), P_s;
 // jammed if P_s >= p_target, else phi_s += dphi (increment_packing_fraction,
 // base_particle.hpp:505-529), until max_comp compressions.  jam_state: 1 jammed, 0 never
-// run or budget used up, -1 minimisation did not stop.
+// run or budget used up, -1 the initial minimisation did not stop (dropped, as
+// remove_badly_initialized_systems does, jammers.cu:7-17; after a compression the state a
+// minimisation reached decides whether it stopped or not, as jammers.cuh:100-110 does).
 // retries: a round whose minimisation hit fire_max is run again from where it stopped (FIRE
 // restarted, no compression; every system is rebuilt at the round boundary) up to `retries`
-// times per system before the system is dropped.
+// times per system before that rule applies.
 long long or_disk_jam_compress(void* h, const double* dt0, double dphi, double p_target, int max_comp, int fire_max,
                                int pred, double a, double b, int mode, int* jam_state, int* n_comp, int* nsteps_out,
                                int retries) {
@@ -1449,10 +1451,8 @@ long long or_disk_jam_compress(void* h, const double* dt0, double dphi, double p
             if (dts[s] == 0.0) continue;
             nsteps[s] += ns[s];
             work += (long long)(d->off[s + 1] - d->off[s]) * ns[s];
-            if (!stopped[s]) {
-                if (nretry[s] < retries) { nretry[s] += 1; retry = true; }
-                else state[s] = -1;
-            }
+            if (!stopped[s] && nretry[s] < retries) { nretry[s] += 1; retry = true; }
+            else if (!stopped[s] && ncomp[s] == 0) state[s] = -1;
             else if (P[s] >= p_target) state[s] = 1;
             else if (ncomp[s] >= max_comp) state[s] = 2;
             else { inc[s] = dphi; ncomp[s] += 1; grow = true; }
