@@ -150,12 +150,30 @@ def settled(P, off, ftol: float, pe_target: float):
     return (fmax < ftol) | (pe < pe_target)
 
 
-def compare_jam(st, ok, pe_b, pr_b, s, Q, acc):
-    """Energy and pressure of system s on the plain path against the batched run: relative for
-    jammed systems (kept apart by whether the decided state met the stop test), absolute for
-    the others (P ~ 0 from cancelling terms)."""
-    Q.compute_pe_total(); Q.compute_pressure()
-    for got, ref in ((Q.get("PE_TOTAL")[0], pe_b[s]), (Q.get("PRESSURE")[0], pr_b[s])):
+def jam_end_state(P):
+    """Canonical positions, PE_s and the static pressure (velocities zeroed first) of every
+    system after a jamming run.  The static pressure is compared because a batch's later
+    rounds restart FIRE on every system, which zeroes the velocities of systems decided
+    earlier (as the reference's FIRE init does, fire.cuh:90-95), while a system run alone
+    keeps the velocities of its last round; the stress tensor's kinetic part then differs."""
+    from dpmd_amd import _lib as L
+    C = L.ORDER_CANONICAL
+    P.compute_pe_total()
+    P.zero_velocities()
+    P.compute_pressure()
+    return P.get("POS_X", C), P.get("POS_Y", C), P.get("PE_TOTAL"), P.get("PRESSURE")
+
+
+def compare_jam(st, ok, end_b, off, s, Q, acc):
+    """System s on the plain path against the batched run: canonical positions bit for bit;
+    energy and static pressure relative for jammed systems (kept apart by whether the
+    decided state met the stop test), absolute for the others (P ~ 0 from cancelling terms)."""
+    import numpy as np
+    xb, yb, pe_b, pr_b = end_b
+    x, y, pe, pr = jam_end_state(Q)
+    acc["positions_bit_identical"] &= bool(np.array_equal(x, xb[off[s]:off[s + 1]])
+                                           and np.array_equal(y, yb[off[s]:off[s + 1]]))
+    for got, ref in ((pe[0], pe_b[s]), (pr[0], pr_b[s])):
         if st[s] == 1:
             key = "rel_jammed_settled" if ok[s] else "rel_jammed_at_cap"
             acc[key] = max(acc[key], abs(got - ref) / max(abs(ref), 1e-300))
@@ -205,8 +223,7 @@ def run_jam512(args):
     t_b = time.perf_counter() - t0
     steps = P.get("FIRE_STEPS")
     ok = settled(P, b.off, JAM["ftol"], JAM["pe_target"])
-    P.compute_pe_total(); P.compute_pressure()
-    pe_b, pr_b = P.get("PE_TOTAL"), P.get("PRESSURE")
+    end_b = jam_end_state(P)
     rate_b = work / t_b
     log(f"batched done: {t_b:.2f} s, work {work:.3e} particle-steps, {rate_b:.3e}/s; "
         f"jammed {int((st == 1).sum())} ({int(((st == 1) & ~ok).sum())} on a capped round), "
@@ -216,7 +233,7 @@ def run_jam512(args):
     order = np.argsort(sizes, kind="stable")
     sample = sorted(order[np.linspace(0, b.S - 1, args.plain_sample).astype(int)].tolist())
     work_p, t_p, same = 0, 0.0, True
-    acc = dict(rel_jammed_settled=0.0, rel_jammed_at_cap=0.0, abs_unjammed=0.0)
+    acc = dict(rel_jammed_settled=0.0, rel_jammed_at_cap=0.0, abs_unjammed=0.0, positions_bit_identical=True)
     for s in sample:
         Q = load(W.system_of(b, s))
         # the plain method: one system per launch sequence (per-step launches, no persistent
@@ -231,7 +248,7 @@ def run_jam512(args):
         Q.synchronize()
         t_p += time.perf_counter() - t1
         work_p += qwork
-        compare_jam(st, ok, pe_b, pr_b, s, Q, acc)
+        compare_jam(st, ok, end_b, b.off, s, Q, acc)
         same = same and qst[0] == st[s] and qnc[0] == nc[s]
     rate_p = work_p / t_p
     log(f"plain sample: {len(sample)} systems, {t_p:.2f} s, {rate_p:.3e}/s; same decisions {same}; {acc}")
@@ -249,6 +266,7 @@ def run_jam512(args):
                    "plain_sample_systems": len(sample), "plain_seconds": t_p, "plain_rate": rate_p,
                    "batched_over_plain": rate_b / rate_p,
                    "jammed_on_capped_round": int(((st == 1) & ~ok).sum()), "same_decisions": bool(same),
+                   "positions_bit_identical": acc["positions_bit_identical"],
                    "plain_vs_batched_max_rel_diff_jammed_settled": acc["rel_jammed_settled"],
                    "plain_vs_batched_max_rel_diff_jammed_at_cap": acc["rel_jammed_at_cap"],
                    "plain_vs_batched_max_abs_diff_unjammed": acc["abs_unjammed"],
@@ -639,8 +657,7 @@ def leg_batched(args):
     max|F| < 1e-10 or <pe> < 1e-16 (the reference's unjammed test).  Batched (one context,
     persistent per-system FIRE) against the plain path (one context per system, per-step
     launches) on 32 systems spread over the size range, with the same autotuned configuration;
-    energies and pressures compared (jammed systems relative, kept apart by whether the state
-    they were decided on met the stop test; the others absolute)."""
+    canonical positions compared bit for bit, energies and static pressures to rounding."""
     import numpy as np
 
     from dpmd_amd import _lib as L
@@ -672,8 +689,7 @@ def leg_batched(args):
     t_b = time.perf_counter() - t0
     steps = P.get("FIRE_STEPS")
     ok = settled(P, b.off, JAM_B["ftol"], JAM_B["pe_target"])
-    P.compute_pe_total(); P.compute_pressure()
-    pe_b, pr_b = P.get("PE_TOTAL"), P.get("PRESSURE")
+    end_b = jam_end_state(P)
     rate_b = work / t_b
     log(f"batched: {t_b:.2f} s, {rate_b:.3e} particle-steps/s; jammed {int((st == 1).sum())} "
         f"({int(((st == 1) & ~ok).sum())} on a capped round), dropped {int((st == -1).sum())}, "
@@ -681,7 +697,7 @@ def leg_batched(args):
     order = np.argsort(sizes, kind="stable")
     sample = sorted(order[np.linspace(0, b.S - 1, 32).astype(int)].tolist())
     work_p, t_p, same_decisions = 0, 0.0, True
-    acc = dict(rel_jammed_settled=0.0, rel_jammed_at_cap=0.0, abs_unjammed=0.0)
+    acc = dict(rel_jammed_settled=0.0, rel_jammed_at_cap=0.0, abs_unjammed=0.0, positions_bit_identical=True)
     for s_ in sample:
         Q = load(W.system_of(b, s_))
         L.check(L.lib().dpmd_set_option(Q.ctx, L.OPT_PERSISTENT_FIRE, 0), Q.ctx)
@@ -692,7 +708,7 @@ def leg_batched(args):
         t_p += time.perf_counter() - t1
         work_p += qwork
         same_decisions = same_decisions and qst[0] == st[s_] and qnc[0] == nc[s_]
-        compare_jam(st, ok, pe_b, pr_b, s_, Q, acc)
+        compare_jam(st, ok, end_b, b.off, s_, Q, acc)
     rate_p = work_p / t_p
     log(f"plain: 32 systems {t_p:.2f} s, {rate_p:.3e}/s; batched/plain {rate_b / rate_p:.1f}x")
     return {"workload": "512 uneven disk systems (N_s 64..4096, phi 0.80..0.86, half PBC half walls), compressions "
@@ -705,6 +721,7 @@ def leg_batched(args):
             "plain_sample_systems": len(sample), "plain_value": rate_p, "plain_seconds": t_p,
             "batched_over_plain": rate_b / rate_p, "same_decisions": bool(same_decisions),
             "jammed_on_capped_round": int(((st == 1) & ~ok).sum()),
+            "positions_bit_identical": acc["positions_bit_identical"],
             "max_rel_energy_pressure_diff_jammed_settled": acc["rel_jammed_settled"],
             "max_rel_energy_pressure_diff_jammed_at_cap": acc["rel_jammed_at_cap"],
             "max_abs_diff_unjammed": acc["abs_unjammed"],
