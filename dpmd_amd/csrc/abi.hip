@@ -568,15 +568,21 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
     // restarts with small batches; above one rebuild per 2 steps (back below 0.35 per step)
     // the batches rebuild on the device instead.  A halt costs about 95 us and the gated
     // device chain about 21 us per step, but the smoothed rate lags FIRE's early burst of
-    // rebuilds, so the switch sits above the 0.2 break-even.
+    // rebuilds, so the switch sits above the 0.2 break-even.  FIRE of one disk system halts
+    // in fused batches (one fused step about 40 us) but rebuilds on the device in
+    // three-launch steps (about 60 us), so there a device batch pays about 41 us more per
+    // step and the switch moves to 0.8 (back below 0.6): at 2^20 disks 2000 FIRE steps from
+    // random positions took 0.126-0.127 s so against 0.139 s with the general thresholds.
     double rate = 0.0;
     bool dev = false;
+    const bool fused_halts = fused_fire_batch(c, 2, fire);
+    const double dev_up = fused_halts ? 0.8 : 0.5, dev_down = fused_halts ? 0.6 : 0.35;
     int rebuilds = read_run(c).rebuilds;
     try {
         while (!done && total < max_steps) {
             int lg = 0;
             const bool cell = c->method == DPMD_NEIGHBOR_CELL;
-            dev = cell && (c->dev_rebuild == 2 || (c->dev_rebuild == 0 && rate > (dev ? 0.35 : 0.5)));
+            dev = cell && (c->dev_rebuild == 2 || (c->dev_rebuild == 0 && rate > (dev ? dev_down : dev_up)));
             if (dev) m = std::max(m, 8);
             while ((2 << lg) <= std::min(std::min(m, c->max_batch), max_steps - total) && lg < 7) ++lg;
             const int bm = 1 << lg;

@@ -391,7 +391,8 @@ int dpmd_stream_copy(double* dst, const double* src, int64_t n, int repeats, dou
  *     batch halts and the host finishes the step (cheapest when rebuilds are rare); 2: every
  *     step runs the rebuild chain on the device, gated by the rebuild flags, and batches never
  *     halt; 0 (default): 1 while rebuilds are rarer than one per 2 steps, 2 above (back below
- *     0.35 per step).  Same results.
+ *     0.35 per step); for FIRE runs of one disk system with DPMD_OPT_FUSED_FIRE, whose halting
+ *     batches take the fused step, the switch is at 0.8 (back below 0.6).  Same results.
  *   DPMD_OPT_PERSIST_LDS: 1 (default) runs persistent FIRE launches whose systems all have at
  *     most 1536 particles with each system's particle state held in LDS (one 512-thread
  *     workgroup per system); 0 keeps it in global memory.  Same results.
