@@ -8,7 +8,9 @@ Default line (one GPU's share at its real shape):
     steps) / wall time of the timed region (inputs resident in HBM).
   * roofline: HIP events around the fused velocity-Verlet launches of an untimed tail of at
     least 256 launches after the timed region (whatever --steps is); fire_roofline: events
-    around the fused FIRE launches of 300 untimed FIRE steps.
+    around the fused FIRE launches of 300 untimed FIRE steps; rebuild_roofline: events
+    around the cell and neighbor rebuild of every halted batch in 3000 untimed NVE steps at
+    T = 1e-3.
   * batched / rigid / adjoint: bounded runs of the 512-system uneven jamming batch (batched
     against one context per system), the 256 x 256 rigid bumpy batch (with the vertex-force
     roofline) and the 128 x 1024 differentiation workload (adjoint / forward time).
@@ -646,6 +648,16 @@ def run_adjoint(args):
     print(json.dumps(out))
 
 
+# The rebuild chain's algorithmic bytes per particle (in-step rebuild of one system, 7 live
+# arrays of 8 bytes): cell counts zeroed and scanned (~8); hash and rank: sid, x, y read, cell
+# id and rank written, one atomic per run of equal cells (30); rank scatter (20); per-cell sort
+# (10) and the gather of the 7 arrays and the cell id into scratch with the inverse order
+# (132); tiled neighbor build: own scratch arrays, cell id, static_index and its inverse-order
+# lookup, halo candidates (x, y, r; ~1.5 reads per object) read, the 7 arrays, static_index,
+# the skin reference (x, y) and the count written (156), plus 4 bytes per list entry.
+REBUILD_BYTES_PER_PARTICLE = 356
+REBUILD_TAIL_STEPS, REBUILD_TAIL_T = 3000, 1e-3
+
 # ---- bounded legs of the default run: the metric's batched, rigid and adjoint parts ---------
 JAM_B = dict(dt=1e-2, dphi=1e-3, p_target=1e-7, max_comp=6, fire_max=20000, ftol=1e-10, pe_target=1e-16, retries=3)
 
@@ -954,6 +966,24 @@ def main():
     L.check(lib.dpmd_profile_read(P.ctx, C.byref(ff_ms), C.byref(ff_cnt)), P.ctx)
     lib.dpmd_profile(P.ctx, 0)
     kf = P.neighbor_info()[2] / N
+    # the rebuild chain (cell hash, scan, scatter, per-cell sort with the gather of the 7 live
+    # arrays, tiled neighbor build with the copy-back) in an untimed NVE tail: HIP events around
+    # the rebuild that finishes every halted batch
+    # the FIRE tail left the system at rest; at T = 1e-3 rebuilds come about ten times as often
+    # as at the workload's 1e-4 (the chain's work does not depend on T)
+    P.rand_normal_2d("VEL_X", "VEL_Y", 1, 2)
+    P.set_average_velocity(0.0, 0.0)
+    P.set_temperature(REBUILD_TAIL_T)
+    lib.dpmd_profile(P.ctx, 4)
+    vv.init()
+    vv.run(REBUILD_TAIL_STEPS, use_graph=False)
+    rb_ms, rb_cnt = C.c_double(), C.c_int()
+    L.check(lib.dpmd_profile_read(P.ctx, C.byref(rb_ms), C.byref(rb_cnt)), P.ctx)
+    lib.dpmd_profile(P.ctx, 0)
+    kr = P.neighbor_info()[2] / N
+    rb_bytes = N * (REBUILD_BYTES_PER_PARTICLE + 4 * kr)
+    rb_avg = rb_ms.value / max(1, rb_cnt.value)
+    rb_gbs = rb_bytes / (rb_avg * 1e-3) / 1e9 if rb_cnt.value else None
     # one fused FIRE step per particle: x, y (16), vx, vy (16), r (8), count (4), k ids (4k),
     # m (8), last-rebuild x, y (16) read; the next step's x, y, vx, vy (32) written -> 100 + 4k
     # bytes (neighbor gathers, chunk partials and their fold excluded)
@@ -1021,6 +1051,15 @@ def main():
                           if (N == 1 << 20 and FIRE_FUSED_TRAFFIC_N1M) else None,
                           "avg_step_us": ff_avg * 1e3, "steps": ff_cnt.value,
                           "algorithmic_bytes_per_step": ff_bytes},
+        "rebuild_roofline": {"bound": "hbm",
+                             "kernel": "the in-step rebuild chain: k_zero, k_hash_rank, k_tile_sum, k_tile_down_self, "
+                                       "k_scatter_rank, k_sort_gather, k_neighbors_tiled, k_clear_flags",
+                             "achieved": rb_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": rb_gbs / HBM_PEAK_GBS if rb_gbs else None,
+                             "traffic": None,
+                             "avg_rebuild_us": rb_avg * 1e3, "rebuilds": rb_cnt.value,
+                             "tail_nve_steps": REBUILD_TAIL_STEPS, "tail_temperature": REBUILD_TAIL_T,
+                             "algorithmic_bytes_per_rebuild": rb_bytes},
     }
     del P
     if not args.no_legs:

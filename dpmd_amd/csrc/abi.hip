@@ -626,7 +626,16 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
                 // a fused FIRE batch raise only the run words, so its system is flagged here
                 halt_reset(c, fused_fire);
                 // finish the halted step: rebuild, forces, bookkeeping
+                hipEvent_t rb0 = nullptr, rb1 = nullptr;
+                if (c->prof == 4 && !c->capturing) {
+                    HIP_OK(hipEventCreate(&rb0)); HIP_OK(hipEventCreate(&rb1));
+                    HIP_OK(hipEventRecord(rb0, c->st));
+                }
                 rebuild_neighbors(c, active_one(c), false, true);
+                if (rb0) {
+                    HIP_OK(hipEventRecord(rb1, c->st));
+                    c->prof_ev.push_back(rb0); c->prof_ev.push_back(rb1); c->prof_n.push_back(1);
+                }
                 if (fire) {
                     if (rigid(c)) rigid_fire_force(c, plain(c), 1, 1, pred, a, b);
                     else disk_fire_force(c, plain(c), 1, 1, pred, a, b);
@@ -2140,7 +2149,7 @@ int dpmd_profile(dpmd_ctx* c, int enable) {
         for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
         c->prof_ev.clear();
         c->prof_n.clear();
-        need(enable >= 0 && enable <= 3, "dpmd_profile: enable must be 0, 1, 2 or 3");
+        need(enable >= 0 && enable <= 4, "dpmd_profile: enable must be 0 .. 4");
         c->prof = enable;
     });
 }

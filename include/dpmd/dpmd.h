@@ -358,7 +358,10 @@ int dpmd_synchronize(dpmd_ctx* ctx);
  * fused velocity-Verlet launches of every directly launched batch that no rebuild halted;
  * dpmd_profile(ctx, 3) records one pair around the consecutive fused FIRE steps (fused and
  * settle launches; runs of one disk system, DPMD_OPT_FUSED_FIRE) of every directly launched
- * batch that no rebuild halted; 0 stops.  dpmd_profile_read returns the summed duration and the number of launches covered. */
+ * batch that no rebuild halted; dpmd_profile(ctx, 4) records one pair around the rebuild of
+ * cells and lists that finishes every halted step batch (the in-step rebuild, whichever
+ * engine runs it); 0 stops.  dpmd_profile_read returns the summed duration and the number of
+ * launches (kind 4: rebuilds) covered. */
 int dpmd_profile(dpmd_ctx* ctx, int enable);
 int dpmd_profile_read(dpmd_ctx* ctx, double* total_ms, int* count);
 /* The recorded event pairs one by one (ms, in order, up to capacity); *count = the number of
