@@ -572,7 +572,7 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
     // in fused batches (one fused step about 40 us) but rebuilds on the device in
     // three-launch steps (about 60 us), so there a device batch pays about 41 us more per
     // step and the switch moves to 0.8 (back below 0.6): at 2^20 disks 2000 FIRE steps from
-    // random positions took 0.126-0.127 s so against 0.139 s with the general thresholds.
+    // random positions took 0.126-0.127 s against 0.139 s with the general thresholds.
     double rate = 0.0;
     bool dev = false;
     const bool fused_halts = fused_fire_batch(c, 2, fire);
@@ -631,7 +631,12 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
                     HIP_OK(hipEventCreate(&rb0)); HIP_OK(hipEventCreate(&rb1));
                     HIP_OK(hipEventRecord(rb0, c->st));
                 }
-                rebuild_neighbors(c, active_one(c), false, true);
+                try {
+                    rebuild_neighbors(c, active_one(c), false, true);
+                } catch (...) {
+                    if (rb0) { (void)hipEventDestroy(rb0); (void)hipEventDestroy(rb1); }
+                    throw;
+                }
                 if (rb0) {
                     HIP_OK(hipEventRecord(rb1, c->st));
                     c->prof_ev.push_back(rb0); c->prof_ev.push_back(rb1); c->prof_n.push_back(1);
