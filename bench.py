@@ -873,7 +873,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                                   "(systems shard across GPUs with no communication, see DESIGN.md)"}))
+                                   "(systems shard across GPUs with no communication; this run measures one GPU's share)"}))
         sys.exit(2)
     if args.workload == "jam512":
         run_jam512(args)
