@@ -517,7 +517,7 @@ std::vector<int64_t> batch_key(dpmd_ctx* c, bool fire, int pred, double a, doubl
             P(c->order.p), P(c->order_inv.p), P(c->tmp_d.p), P(c->tmp_i.p), P(c->tiles.p), P(c->overflow.p),
             P(c->Ly.p), P(c->iLx.p), P(c->iLy.p), P(c->cutoff.p), P(c->vsid.p), P(c->voff.p), P(c->vpid.p),
             P(c->poff.p), P(c->vrad.p), P(c->vmass.p), P(c->inertia.p), P(c->area.p), P(c->sys_tile.p),
-            P(c->sys_list.p), P(c->sys_list_n.p)};
+            P(c->sys_list.p), P(c->sys_list_n.p), c->n_act_chunk, c->n_act_sys, P(c->act_chunk.p), P(c->act_sys.p)};
 }
 
 void drop_cached_graphs(dpmd_ctx* c) {

@@ -137,6 +137,19 @@ __global__ void __launch_bounds__(B) k_fire_drift(DiskArgs a, FireArgs f, Gate g
     if (a.cell) raise_flags(a.flag, a.any, s, flag);
 }
 
+// k_fire_drift over the chunks of the systems still running (one workgroup per chunk).
+__global__ void __launch_bounds__(DPMD_BLOCK)
+k_fire_drift_act(DiskArgs a, FireArgs f, RedArgs r, const int* __restrict__ act, Gate g) {
+    if (!gate_open(g)) return;
+    const int c = act[blockIdx.x];
+    const int s = r.chunk_sys[c];
+    const int i = r.chunk_beg[c] + threadIdx.x;
+    const bool valid = i < a.off[s + 1];
+    bool flag = false;
+    if (valid && !fire_frozen(f, s)) flag = fire_drift_one(a, f, i, s);
+    if (a.cell) raise_flags(a.flag, a.any, valid ? s : -1, flag);
+}
+
 // The fold of one system's chunk partials in a fixed order: 256 strided sequential
 // accumulators (thread t adds chunks t, t + 256, ... in turn), then the block tree.  Every
 // thread of the 256-thread workgroup gets the power, energy and max |F|^2 totals (lds: 12
@@ -213,10 +226,11 @@ __device__ __forceinline__ void fire_fold_system(const int* off, const FireArgs&
 
 template <int MODE>
 __global__ void __launch_bounds__(DPMD_BLOCK)
-k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, Gate g) {
+k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, const int* __restrict__ act, Gate g) {
     __shared__ double lds[12];
     if (!gate_force(g)) return;
-    const int c = xcd_remap(blockIdx.x, gridDim.x);
+    // act: the chunks of the systems still running, when the launch covers only those
+    const int c = act ? act[blockIdx.x] : xcd_remap(blockIdx.x, gridDim.x);
     const int s = r.chunk_sys[c];
     if (fire_frozen(f, s)) return;   // whole workgroup: chunks never straddle systems
     const int beg = r.chunk_beg[c];
@@ -253,10 +267,10 @@ k_fire_force(DiskArgs a, FireArgs f, RedArgs r, int kick, Gate g) {
 // One workgroup per system folds the chunk partials (fire_fold_system).
 __global__ void __launch_bounds__(DPMD_BLOCK)
 k_fire_finalize(const int* off, FireArgs f, RedArgs r, int do_conv, int do_update, int pred, double pa, double pb,
-                Gate g) {
+                const int* __restrict__ act, Gate g) {
     __shared__ double lds[12];
     if (!gate_open(g)) return;
-    const int s = blockIdx.x;
+    const int s = act ? act[blockIdx.x] : blockIdx.x;   // act: the systems still running
     if (fire_frozen(f, s)) return;
     fire_fold_system(off, f, r, s, do_conv, do_update, pred, pa, pb, g, lds);
 }
@@ -728,6 +742,12 @@ void disk_stress(dpmd_ctx* c) {
 }
 
 void disk_fire_drift(dpmd_ctx* c, Gate g) {
+    if (c->n_act_chunk > 0) {
+        k_fire_drift_act<<<c->n_act_chunk, DPMD_BLOCK, 0, c->st>>>(disk_args(c), fire_args(c), red_args(c),
+                                                                     c->act_chunk.p, g);
+        HIP_OK(hipGetLastError());
+        return;
+    }
     by_block(c->block, [&](auto b) {
         constexpr int B = decltype(b)::value;
         k_fire_drift<B><<<blocks_for(c->N, B), B, 0, c->st>>>(disk_args(c), fire_args(c), g);
@@ -742,7 +762,9 @@ void disk_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, doubl
     {
         ProfScope prof(c);
         by_mode(c->fire_mode, [&](auto m) {
-            k_fire_force<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(a, f, r, kick, g);
+            const bool act = c->n_act_chunk > 0;
+            k_fire_force<decltype(m)::value><<<act ? c->n_act_chunk : c->nchunks, DPMD_BLOCK, 0, c->st>>>(
+                a, f, r, kick, act ? c->act_chunk.p : nullptr, g);
         });
     }
     HIP_OK(hipGetLastError());
@@ -750,7 +772,10 @@ void disk_fire_force(dpmd_ctx* c, Gate g, int kick, int do_conv, int pred, doubl
 }
 
 void fire_finalize(dpmd_ctx* c, Gate g, int do_conv, int pred, double pa, double pb) {
-    k_fire_finalize<<<c->S, DPMD_BLOCK, 0, c->st>>>(c->off.p, fire_args(c), red_args(c), do_conv, 1, pred, pa, pb, g);
+    const bool act = c->n_act_sys > 0;
+    k_fire_finalize<<<act ? c->n_act_sys : c->S, DPMD_BLOCK, 0, c->st>>>(c->off.p, fire_args(c), red_args(c), do_conv,
+                                                                         1, pred, pa, pb,
+                                                                         act ? c->act_sys.p : nullptr, g);
     HIP_OK(hipGetLastError());
 }
 

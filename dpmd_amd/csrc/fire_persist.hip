@@ -373,12 +373,19 @@ bool fire_persist_ok(dpmd_ctx* c) {
 // Per-step cost models (microseconds) of the two engines on the MI355X, from timed runs of
 // FIRE on single systems and on the 512-system batch: a persistent workgroup is bound by one
 // CU's memory and FP64 throughput (about 9 ns per particle; two systems share a CU when more
-// than 256 run), while a step of per-step launches costs about 18 us of launch latency plus
-// about 0.18 ns per particle of the batch.
+// than 256 run).  A step of per-step launches over the chunks of the running systems costs a
+// few us of launch latency plus a share of their particles (5 us + 0.08 ns per particle, set
+// by the time of the 512-system jamming workload: 47 s, against 61 s with 18 us + 0.18 ns).
+// Against the LDS-resident engine the per-step launches also pay the rebuild halts that the
+// engine's in-workgroup rebuilds avoid, so while the running systems fit that engine the
+// launches are charged 18 us + 0.18 ns per particle (a 128 x 1024 FIRE run to 1e-12 took
+// 1.15 s with the lower charge, 0.86 s with this one).
 double persist_step_us(int nmax, int nrun) {
     return (3.2 + 0.009 * nmax) * std::max(1, (nrun + 255) / 256);
 }
-double launch_step_us(int64_t ntotal) { return 18.0 + 0.00018 * (double)ntotal; }
+double launch_step_us(int64_t ntotal, bool vs_lds) {
+    return vs_lds ? 18.0 + 0.00018 * (double)ntotal : 5.0 + 0.00008 * (double)ntotal;
+}
 
 // The device-driven FIRE run of lean_run with the same results.  Running systems advance in
 // persistent phases (persist_phase steps), each system in its own workgroup until its
@@ -407,11 +414,29 @@ int fire_persistent_run(dpmd_ctx* c, int max_steps, int pred, double a, double b
         int nmax = 0;
         int64_t ntot = 0;
         for (int s : running) { nmax = std::max(nmax, size(s)); ntot += size(s); }
-        if (c->persist_fire == 1 && launch_step_us(ntot) < persist_step_us(nmax, (int)running.size())) {
-            // hand the rest of the run to the per-step launches, in lockstep from step `done`
+        const bool lds_run = c->persist_lds && nmax <= kLdsMaxN && lds_engine_fits();
+        if (c->persist_fire == 1 && launch_step_us(ntot, lds_run) < persist_step_us(nmax, (int)running.size())) {
+            // hand the rest of the run to the per-step launches, in lockstep from step `done`;
+            // their launches cover only the chunks of the systems still running (the others
+            // would return at once, but a launch over every chunk of a large batch costs its
+            // dispatch)
             const int nrun = (int)running.size();
             int run[RUN_WORDS] = {0, 0, nrun, 1, 0, -1, -1, 0, 0, 0};
             c->run.upload(run, RUN_WORDS, c->st);
+            std::vector<int> first(S + 1, 0), chunks;
+            for (int s = 0; s < S; ++s) first[s + 1] = first[s] + (size(s) + DPMD_BLOCK - 1) / DPMD_BLOCK;
+            std::vector<int> sys(running);
+            std::sort(sys.begin(), sys.end());
+            for (int s : sys)
+                for (int k = first[s]; k < first[s + 1]; ++k) chunks.push_back(k);
+            c->act_chunk.upload(chunks.data(), (int64_t)chunks.size(), c->st);
+            c->act_sys.upload(sys.data(), (int64_t)sys.size(), c->st);
+            struct ActScope {
+                dpmd_ctx* c;
+                ~ActScope() { c->n_act_chunk = 0; c->n_act_sys = 0; }
+            } scope{c};
+            c->n_act_chunk = (int)chunks.size();
+            c->n_act_sys = (int)sys.size();
             int lconv = 0;
             const int steps = lean_run_batches(c, max_steps - done, true, pred, a, b, use_graph, &lconv);
             if (conv) *conv = lconv;

@@ -231,6 +231,10 @@ struct dpmd_ctx {
     // the switch, its workgroup size, and the per-run work lists
     int persist_fire = 1;        // 0 off, 1 with hand-off to per-step launches, 2 persistent only
     int persist_block = 512;
+    // FIRE step launches restricted to the systems still running (set while a persistent run
+    // hands its last systems to the per-step launches; 0 = launches over every chunk / system)
+    dpmd::DVec<int> act_chunk, act_sys;
+    int n_act_chunk = 0, n_act_sys = 0;
     int persist_lds = 1;         // systems of up to 1536 particles run with their state in LDS
     int persist_phase = 2000;    // steps per persistent phase
     dpmd::DVec<int> persist_order, persist_todo, persist_stop;
