@@ -389,7 +389,7 @@ int any_word(int q) { return q % 2 == 0 ? RUN_ANY : RUN_ANY2; }
 // the parity words as in fused velocity-Verlet batches (a redone drift to RUN_FIXW); a halted
 // batch unpacks the halted step's pair and flags the system before the rebuild.
 bool fused_fire_batch(dpmd_ctx* c, int m, bool fire) {
-    return fire && !rigid(c) && c->fuse_fire && c->S == 1 && m >= 2 && m % 2 == 0;
+    return fire && !rigid(c) && c->fuse_fire && (c->S == 1 || c->fsys >= 0) && m >= 2 && m % 2 == 0;
 }
 
 // dev: every step rebuilds the flagged systems on the device between its drift and its
@@ -517,7 +517,8 @@ std::vector<int64_t> batch_key(dpmd_ctx* c, bool fire, int pred, double a, doubl
             P(c->order.p), P(c->order_inv.p), P(c->tmp_d.p), P(c->tmp_i.p), P(c->tiles.p), P(c->overflow.p),
             P(c->Ly.p), P(c->iLx.p), P(c->iLy.p), P(c->cutoff.p), P(c->vsid.p), P(c->voff.p), P(c->vpid.p),
             P(c->poff.p), P(c->vrad.p), P(c->vmass.p), P(c->inertia.p), P(c->area.p), P(c->sys_tile.p),
-            P(c->sys_list.p), P(c->sys_list_n.p), c->n_act_chunk, c->n_act_sys, P(c->act_chunk.p), P(c->act_sys.p)};
+            P(c->sys_list.p), P(c->sys_list_n.p), c->n_act_chunk, c->n_act_sys, P(c->act_chunk.p), P(c->act_sys.p),
+            c->fsys, c->fsys_ch0};
 }
 
 void drop_cached_graphs(dpmd_ctx* c) {
@@ -542,6 +543,36 @@ struct GraphScope {
     }
 };
 
+}  // namespace
+
+namespace {
+// A FIRE run whose launches cover only the running systems (act_chunk / act_sys): drops the
+// systems that are frozen now (once at most half of the set is left, or one system) and,
+// when one system is left and the fused FIRE step applies, makes it the system of fused
+// batches.  Returns whether the launches changed.
+bool shrink_active(dpmd_ctx* c) {
+    const int S = c->S;
+    std::vector<int> frozen = dl(c, c->f_frozen), sys = dl(c, c->act_sys);
+    sys.resize(c->n_act_sys);
+    std::vector<int> keep;
+    for (int s : sys) if (!frozen[s]) keep.push_back(s);
+    // each change costs the capture of new graphs: shrink when the set has halved, or to the
+    // last system (which then takes the fused step)
+    if (keep.empty() || (keep.size() != 1 && 2 * (int)keep.size() > c->n_act_sys)) return false;
+    std::vector<int> first(S + 1, 0), chunks;
+    for (int s = 0; s < S; ++s) first[s + 1] = first[s] + (c->h_off[s + 1] - c->h_off[s] + DPMD_BLOCK - 1) / DPMD_BLOCK;
+    for (int s : keep)
+        for (int k = first[s]; k < first[s + 1]; ++k) chunks.push_back(k);
+    c->act_chunk.upload(chunks.data(), (int64_t)chunks.size(), c->st);
+    c->act_sys.upload(keep.data(), (int64_t)keep.size(), c->st);
+    c->n_act_chunk = (int)chunks.size();
+    c->n_act_sys = (int)keep.size();
+    if (keep.size() == 1 && c->fuse_fire && !rigid(c)) {
+        c->fsys = keep[0];
+        c->fsys_ch0 = first[keep[0]];
+    }
+    return true;
+}
 }  // namespace
 
 // Runs up to max_steps steps as graph-captured batches of per-step launches; returns the
@@ -598,6 +629,18 @@ int dpmd::lean_run_batches(dpmd_ctx* c, int max_steps, bool fire, int pred, doub
                 lean_batch(c, bm, fire, pred, a, b, dev);
             }
             RunCtl r = read_run(c);
+            if (fire && c->n_act_sys > 1 && c->mask_frozen && r.running < c->n_act_sys && !r.halt) {
+                // systems of a restricted run have stopped: once frozen they leave the launches,
+                // and the last one left runs the fused FIRE step
+                if (shrink_active(c)) {
+                    if (cached) {
+                        drop_cached_graphs(c);
+                        c->gcache_key = batch_key(c, fire, pred, a, b);
+                    } else {
+                        for (auto& g : local) if (g) { HIP_OK(hipGraphExecDestroy(g)); g = nullptr; }
+                    }
+                }
+            }
             {
                 const int dn = r.rebuilds - rebuilds;
                 rebuilds = r.rebuilds;

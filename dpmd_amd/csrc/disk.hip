@@ -326,10 +326,10 @@ __device__ __forceinline__ bool gate_fire_fused(const Gate& g) {
 // buffer pair (xo, vo).
 template <int B>
 __global__ void __launch_bounds__(B)
-k_fire_drift_to(DiskArgs a, FireArgs f, double2* __restrict__ xo, double2* __restrict__ vo, Gate g) {
+k_fire_drift_to(DiskArgs a, FireArgs f, double2* __restrict__ xo, double2* __restrict__ vo, FusedSys fs, Gate g) {
     if (!gate_open(g)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < a.n;
+    const int i = fs.b0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < fs.b0 + fs.n;
     const int s = valid ? a.sid[i] : -1;
     bool flag = false;
     if (valid && !fire_frozen(f, s)) {
@@ -350,9 +350,9 @@ k_fire_drift_to(DiskArgs a, FireArgs f, double2* __restrict__ xo, double2* __res
 // returns the kicked velocity, the force and the energy.
 template <int MODE>
 __device__ __forceinline__ void fire_force_kick(const DiskArgs& a, const double2* __restrict__ xs,
-                                                const double2* __restrict__ vs, int i, double dt, double& fx,
+                                                const double2* __restrict__ vs, int i, int s, double dt, double& fx,
                                                 double& fy, double& pe, double& vx, double& vy) {
-    disk_force_mode_pos<MODE>(a, PackedPos{xs}, i, 0, fx, fy, pe);
+    disk_force_mode_pos<MODE>(a, PackedPos{xs}, i, s, fx, fy, pe);
     const double2 v = vs[i];
     vx = v.x; vy = v.y;
     const double smi = dt * 0.5 / a.mass[i];
@@ -373,43 +373,44 @@ __device__ __forceinline__ void fire_chunk_partials(const RedArgs& r, int c, dou
     }
 }
 
-// One system: chunk c covers particles [256 c, 256 c + 256).
+// One system (fs): its chunk c covers its particles b0 + [256 c, 256 c + 256).
 template <int MODE>
 __global__ void __launch_bounds__(DPMD_BLOCK)
 k_fire_fused(DiskArgs a, FireArgs f, RedArgs r, const double2* __restrict__ xs, const double2* __restrict__ vs,
              double2* __restrict__ xo, double2* __restrict__ vo, double* __restrict__ snap, int* __restrict__ isnap,
-             Gate g) {
+             FusedSys fs, Gate g) {
     __shared__ double lds[12];
     if (!gate_fire_fused(g)) return;
     const int c = xcd_remap(blockIdx.x, gridDim.x);
-    const int i = c * DPMD_BLOCK + threadIdx.x;
+    const int i = fs.b0 + c * DPMD_BLOCK + threadIdx.x;
+    const int s = fs.s;
     // this step's dt (kick) and the next step's scalars as F.v > 0 makes them
     FireSys q{};
-    q.dt = f.dt[0]; q.alpha = f.alpha[0]; q.ngood = f.ngood[0]; q.nbad = f.nbad[0];
-    q.dtmax = f.dtmax[0]; q.dtmin = f.dtmin[0];
+    q.dt = f.dt[s]; q.alpha = f.alpha[s]; q.ngood = f.ngood[s]; q.nbad = f.nbad[s];
+    q.dtmax = f.dtmax[s]; q.dtmin = f.dtmin[s];
     if (c == 0 && threadIdx.x == 0) {
-        snap[0] = q.dt; snap[1] = q.alpha; snap[2] = f.peprev[0];
+        snap[0] = q.dt; snap[1] = q.alpha; snap[2] = f.peprev[s];
         isnap[0] = q.ngood; isnap[1] = q.nbad;
     }
     const double dt = q.dt;
     fire_update_core(q, 1.0);
     double vp = 0.0, vpe = 0.0, vf2 = 0.0;
     bool flag = false;
-    if (i < a.n) {
+    if (i < fs.b0 + fs.n) {
         double fx, fy, pe, vx, vy;
-        fire_force_kick<MODE>(a, xs, vs, i, dt, fx, fy, pe, vx, vy);
+        fire_force_kick<MODE>(a, xs, vs, i, s, dt, fx, fy, pe, vx, vy);
         vp = fx * vx + fy * vy;
         vpe = pe;
         vf2 = fx * fx + fy * fy;
         const double2 p = xs[i];
         double x = p.x, y = p.y;
-        const double lx = a.cell ? a.lx[i] : 0.0, ly = a.cell ? a.ly[i] : 0.0, th2 = a.cell ? a.thresh2[0] : 0.0;
+        const double lx = a.cell ? a.lx[i] : 0.0, ly = a.cell ? a.ly[i] : 0.0, th2 = a.cell ? a.thresh2[s] : 0.0;
         flag = fire_drift_vals(x, y, vx, vy, fx, fy, a.mass[i], a.cell, lx, ly, th2, q.s_dt, q.s_alpha, q.s_dtrev,
                                q.s_vscale);
         xo[i] = make_double2(x, y); vo[i] = make_double2(vx, vy);
     }
     if (a.cell) raise_word(a.any, flag);
-    fire_chunk_partials(r, c, vp, vpe, vf2, lds);
+    fire_chunk_partials(r, fs.ch0 + c, vp, vpe, vf2, lds);
 }
 
 // After k_fire_fused of step k (gate: nothing halted, the system running, as k_fire_fused saw
@@ -418,38 +419,39 @@ template <int MODE>
 __global__ void __launch_bounds__(DPMD_BLOCK)
 k_fire_settle(DiskArgs a, FireArgs f, RedArgs r, const double* __restrict__ snap, const int* __restrict__ isnap,
               int pred, double pa, double pb, const double2* __restrict__ xs, const double2* __restrict__ vs,
-              double2* __restrict__ xo, double2* __restrict__ vo, Gate g, int anyw) {
+              double2* __restrict__ xo, double2* __restrict__ vo, FusedSys fs, Gate g, int anyw) {
     __shared__ double lds[12];
     __shared__ double sh[5];
     __shared__ int shi[2];
     if (g.run[RUN_HALT] != 0 || g.run[RUN_RUNNING] <= 0) return;
+    const int s = fs.s;
     double ap, ape, af;
-    fold_partials(r, 0, lds, ap, ape, af);
+    fold_partials(r, s, lds, ap, ape, af);
     if (threadIdx.x == 0) {
         FireSys q{};
         q.dt = snap[0]; q.alpha = snap[1]; q.peprev = snap[2]; q.ngood = isnap[0]; q.nbad = isnap[1];
-        q.dtmax = f.dtmax[0]; q.dtmin = f.dtmin[0];
+        q.dtmax = f.dtmax[s]; q.dtmin = f.dtmin[s];
         const double kick_dt = q.dt;
         FireSys guess = q;
         fire_update_core(guess, 1.0);
-        fire_convergence_core(q, a.off[1] - a.off[0], ape, af, pred, pa, pb);
+        fire_convergence_core(q, fs.n, ape, af, pred, pa, pb);
         const bool stop = q.dt == 0.0;   // fire_count_stopped (the system was running)
         fire_update_core(q, ap);
         const bool miss = stop || q.s_dt != guess.s_dt || q.s_alpha != guess.s_alpha ||
                           q.s_dtrev != guess.s_dtrev || q.s_vscale != guess.s_vscale ||
                           q.s_ngood != guess.s_ngood || q.s_nbad != guess.s_nbad;
         if (blockIdx.x == 0) {
-            r.pw_tot[0] = ap; r.pe_tot[0] = ape; r.maxf2[0] = af;
-            f.peprev[0] = q.peprev;
-            f.s_dt[0] = q.s_dt; f.s_alpha[0] = q.s_alpha; f.s_ngood[0] = q.s_ngood; f.s_nbad[0] = q.s_nbad;
-            f.s_dtrev[0] = q.s_dtrev; f.s_vscale[0] = q.s_vscale;
+            r.pw_tot[s] = ap; r.pe_tot[s] = ape; r.maxf2[s] = af;
+            f.peprev[s] = q.peprev;
+            f.s_dt[s] = q.s_dt; f.s_alpha[s] = q.s_alpha; f.s_ngood[s] = q.s_ngood; f.s_nbad[s] = q.s_nbad;
+            f.s_dtrev[s] = q.s_dtrev; f.s_vscale[s] = q.s_vscale;
             if (stop) {             // k_fire_finalize's state after a stop; no drift follows
-                f.dt[0] = q.dt; f.stopped[0] = 1;
+                f.dt[s] = q.dt; f.stopped[s] = 1;
                 f.run[anyw] = 0;
             } else {                // what k_fire_drift commits at the start of the next step
-                f.dt[0] = q.s_dt; f.alpha[0] = q.s_alpha; f.ngood[0] = q.s_ngood; f.nbad[0] = q.s_nbad;
+                f.dt[s] = q.s_dt; f.alpha[s] = q.s_alpha; f.ngood[s] = q.s_ngood; f.nbad[s] = q.s_nbad;
             }
-            f.nsteps[0] += 1;
+            f.nsteps[s] += 1;
             f.run[RUN_MIS] = stop ? -(g.k + 1) : (miss ? g.k + 1 : 0);
         }
         shi[0] = miss; shi[1] = stop;
@@ -459,12 +461,12 @@ k_fire_settle(DiskArgs a, FireArgs f, RedArgs r, const double* __restrict__ snap
     if (!shi[0]) return;
     const bool stop = shi[1] != 0;
     const double kick_dt = sh[0];
-    for (int c = blockIdx.x; c < r.nch; c += gridDim.x) {
-        const int i = c * DPMD_BLOCK + threadIdx.x;
+    for (int c = blockIdx.x; c < fs.nch; c += gridDim.x) {
+        const int i = fs.b0 + c * DPMD_BLOCK + threadIdx.x;
         bool flag = false;
-        if (i < a.n) {
+        if (i < fs.b0 + fs.n) {
             double fx, fy, pe, vx, vy;
-            fire_force_kick<MODE>(a, xs, vs, i, kick_dt, fx, fy, pe, vx, vy);
+            fire_force_kick<MODE>(a, xs, vs, i, s, kick_dt, fx, fy, pe, vx, vy);
             const double2 p = xs[i];
             if (stop) {   // the step's final state, as k_fire_force leaves it
                 a.x[i] = p.x; a.y[i] = p.y; a.vx[i] = vx; a.vy[i] = vy;
@@ -472,7 +474,7 @@ k_fire_settle(DiskArgs a, FireArgs f, RedArgs r, const double* __restrict__ snap
             } else {
                 double x = p.x, y = p.y;
                 const double lx = a.cell ? a.lx[i] : 0.0, ly = a.cell ? a.ly[i] : 0.0;
-                const double th2 = a.cell ? a.thresh2[0] : 0.0;
+                const double th2 = a.cell ? a.thresh2[s] : 0.0;
                 flag = fire_drift_vals(x, y, vx, vy, fx, fy, a.mass[i], a.cell, lx, ly, th2, sh[1], sh[2], sh[3], sh[4]);
                 xo[i] = make_double2(x, y); vo[i] = make_double2(vx, vy);
             }
@@ -486,15 +488,15 @@ k_fire_settle(DiskArgs a, FireArgs f, RedArgs r, const double* __restrict__ snap
 template <int MODE>
 __global__ void __launch_bounds__(DPMD_BLOCK)
 k_fire_force_pk(DiskArgs a, FireArgs f, RedArgs r, const double2* __restrict__ xs, const double2* __restrict__ vs,
-                Gate g) {
+                FusedSys fs, Gate g) {
     __shared__ double lds[12];
     if (!gate_fire_fused(g)) return;
     const int c = xcd_remap(blockIdx.x, gridDim.x);
-    const int i = c * DPMD_BLOCK + threadIdx.x;
+    const int i = fs.b0 + c * DPMD_BLOCK + threadIdx.x;
     double vp = 0.0, vpe = 0.0, vf2 = 0.0;
-    if (i < a.n) {
+    if (i < fs.b0 + fs.n) {
         double fx, fy, pe, vx, vy;
-        fire_force_kick<MODE>(a, xs, vs, i, f.dt[0], fx, fy, pe, vx, vy);
+        fire_force_kick<MODE>(a, xs, vs, i, fs.s, f.dt[fs.s], fx, fy, pe, vx, vy);
         const double2 p = xs[i];
         a.x[i] = p.x; a.y[i] = p.y; a.vx[i] = vx; a.vy[i] = vy;
         a.fx[i] = fx; a.fy[i] = fy; a.pe[i] = pe;
@@ -502,14 +504,15 @@ k_fire_force_pk(DiskArgs a, FireArgs f, RedArgs r, const double2* __restrict__ x
         vpe = pe;
         vf2 = fx * fx + fy * fy;
     }
-    fire_chunk_partials(r, c, vp, vpe, vf2, lds);
+    fire_chunk_partials(r, fs.ch0 + c, vp, vpe, vf2, lds);
 }
 
-__global__ void k_unpack_pv(int n, const double2* __restrict__ xs, const double2* __restrict__ vs,
+// Particles b0 .. b0 + n - 1 from the interleaved pairs to the context's arrays.
+__global__ void k_unpack_pv(int b0, int n, const double2* __restrict__ xs, const double2* __restrict__ vs,
                             double* __restrict__ x, double* __restrict__ y, double* __restrict__ vx,
                             double* __restrict__ vy) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int i = b0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b0 + n) return;
     const double2 p = xs[i], v = vs[i];
     x[i] = p.x; y[i] = p.y; vx[i] = v.x; vy[i] = v.y;
 }
@@ -779,12 +782,20 @@ void fire_finalize(dpmd_ctx* c, Gate g, int do_conv, int pred, double pa, double
     HIP_OK(hipGetLastError());
 }
 
+FusedSys fused_sys(dpmd_ctx* c) {
+    if (c->S == 1) return FusedSys{0, 0, c->N, 0, c->nchunks};
+    need_state(c->fsys >= 0 && c->fsys < c->S, "fused FIRE: no single running system");
+    const int s = c->fsys, b0 = c->h_off[s], n = c->h_off[s + 1] - b0;
+    return FusedSys{s, b0, n, c->fsys_ch0, (n + DPMD_BLOCK - 1) / DPMD_BLOCK};
+}
+
 void disk_fire_drift_to(dpmd_ctx* c, Gate g, double* xo, double* vo, int any_word) {
     DiskArgs a = disk_args(c);
     a.any = c->run.p + any_word;
     by_block(c->block, [&](auto b) {
         constexpr int B = decltype(b)::value;
-        k_fire_drift_to<B><<<blocks_for(c->N, B), B, 0, c->st>>>(a, fire_args(c), (double2*)xo, (double2*)vo, g);
+        const FusedSys fs = fused_sys(c);
+        k_fire_drift_to<B><<<blocks_for(fs.n, B), B, 0, c->st>>>(a, fire_args(c), (double2*)xo, (double2*)vo, fs, g);
     });
     HIP_OK(hipGetLastError());
 }
@@ -793,36 +804,40 @@ void disk_fire_fused(dpmd_ctx* c, Gate g, const double* xs, const double* vs, do
     DiskArgs a = disk_args(c);
     a.any = c->run.p + any_word;
     by_mode(c->fire_mode, [&](auto m) {
-        k_fire_fused<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(
+        const FusedSys fs = fused_sys(c);
+        k_fire_fused<decltype(m)::value><<<fs.nch, DPMD_BLOCK, 0, c->st>>>(
             a, fire_args(c), red_args(c), (const double2*)xs, (const double2*)vs, (double2*)xo, (double2*)vo,
-            c->f_snap.p, c->i_snap.p, g);
+            c->f_snap.p, c->i_snap.p, fs, g);
     });
     HIP_OK(hipGetLastError());
 }
 
 void disk_fire_settle(dpmd_ctx* c, Gate g, int pred, double pa, double pb, const double* xs, const double* vs,
                       double* xo, double* vo, int any_word) {
-    const int grid = std::min(c->nchunks, kSettleGrid);
+    const FusedSys fs = fused_sys(c);
+    const int grid = std::min(fs.nch, kSettleGrid);
     by_mode(c->fire_mode, [&](auto m) {
         k_fire_settle<decltype(m)::value><<<grid, DPMD_BLOCK, 0, c->st>>>(
             disk_args(c), fire_args(c), red_args(c), c->f_snap.p, c->i_snap.p, pred, pa, pb, (const double2*)xs,
-            (const double2*)vs, (double2*)xo, (double2*)vo, g, any_word);
+            (const double2*)vs, (double2*)xo, (double2*)vo, fs, g, any_word);
     });
     HIP_OK(hipGetLastError());
 }
 
 void disk_fire_force_packed(dpmd_ctx* c, Gate g, const double* xs, const double* vs, int pred, double pa, double pb) {
     by_mode(c->fire_mode, [&](auto m) {
-        k_fire_force_pk<decltype(m)::value><<<c->nchunks, DPMD_BLOCK, 0, c->st>>>(
-            disk_args(c), fire_args(c), red_args(c), (const double2*)xs, (const double2*)vs, g);
+        const FusedSys fs = fused_sys(c);
+        k_fire_force_pk<decltype(m)::value><<<fs.nch, DPMD_BLOCK, 0, c->st>>>(
+            disk_args(c), fire_args(c), red_args(c), (const double2*)xs, (const double2*)vs, fs, g);
     });
     HIP_OK(hipGetLastError());
     fire_finalize(c, g, 1, pred, pa, pb);
 }
 
 void disk_unpack_pv(dpmd_ctx* c, const double* xs, const double* vs) {
-    k_unpack_pv<<<blocks_for(c->N), DPMD_BLOCK, 0, c->st>>>(c->N, (const double2*)xs, (const double2*)vs, c->x.p,
-                                                            c->y.p, c->vx.p, c->vy.p);
+    const FusedSys fs = fused_sys(c);
+    k_unpack_pv<<<blocks_for(fs.n), DPMD_BLOCK, 0, c->st>>>(fs.b0, fs.n, (const double2*)xs, (const double2*)vs,
+                                                            c->x.p, c->y.p, c->vx.p, c->vy.p);
     HIP_OK(hipGetLastError());
 }
 

@@ -419,7 +419,8 @@ int fire_persistent_run(dpmd_ctx* c, int max_steps, int pred, double a, double b
             // hand the rest of the run to the per-step launches, in lockstep from step `done`;
             // their launches cover only the chunks of the systems still running (the others
             // would return at once, but a launch over every chunk of a large batch costs its
-            // dispatch)
+            // dispatch), the set shrinks as systems freeze, and the last system left takes the
+            // fused FIRE step
             const int nrun = (int)running.size();
             int run[RUN_WORDS] = {0, 0, nrun, 1, 0, -1, -1, 0, 0, 0};
             c->run.upload(run, RUN_WORDS, c->st);
@@ -433,10 +434,14 @@ int fire_persistent_run(dpmd_ctx* c, int max_steps, int pred, double a, double b
             c->act_sys.upload(sys.data(), (int64_t)sys.size(), c->st);
             struct ActScope {
                 dpmd_ctx* c;
-                ~ActScope() { c->n_act_chunk = 0; c->n_act_sys = 0; }
+                ~ActScope() { c->n_act_chunk = 0; c->n_act_sys = 0; c->fsys = -1; }
             } scope{c};
             c->n_act_chunk = (int)chunks.size();
             c->n_act_sys = (int)sys.size();
+            if (sys.size() == 1 && c->fuse_fire) {
+                c->fsys = sys[0];
+                c->fsys_ch0 = first[sys[0]];
+            }
             int lconv = 0;
             const int steps = lean_run_batches(c, max_steps - done, true, pred, a, b, use_graph, &lconv);
             if (conv) *conv = lconv;

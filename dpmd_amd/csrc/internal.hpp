@@ -235,6 +235,9 @@ struct dpmd_ctx {
     // hands its last systems to the per-step launches; 0 = launches over every chunk / system)
     dpmd::DVec<int> act_chunk, act_sys;
     int n_act_chunk = 0, n_act_sys = 0;
+    // the one system of fused FIRE batches in a context of several (the last system running
+    // in such launches; -1: none) and its first chunk
+    int fsys = -1, fsys_ch0 = 0;
     int persist_lds = 1;         // systems of up to 1536 particles run with their state in LDS
     int persist_phase = 2000;    // steps per persistent phase
     dpmd::DVec<int> persist_order, persist_todo, persist_stop;
@@ -374,6 +377,12 @@ void disk_unpack_positions(dpmd_ctx* c, const double* xs);
 // drift when it was wrong); the batch's last forces (writing the state back); unpacking a
 // halted step's buffers
 constexpr int kSettleGrid = 128;   // workgroups of the settle launch (each folds the partials)
+// The system of fused FIRE batches: its index, first particle, size, first chunk and number of
+// chunks (chunk c of it covers particles b0 + 256 c .. b0 + 256 c + 255).
+struct FusedSys {
+    int s, b0, n, ch0, nch;
+};
+FusedSys fused_sys(dpmd_ctx* c);
 void disk_fire_drift_to(dpmd_ctx* c, Gate g, double* xo, double* vo, int any_word);
 void disk_fire_fused(dpmd_ctx* c, Gate g, const double* xs, const double* vs, double* xo, double* vo, int any_word);
 void disk_fire_settle(dpmd_ctx* c, Gate g, int pred, double pa, double pb, const double* xs, const double* vs,
@@ -423,6 +432,8 @@ void sys_reduce_dot(dpmd_ctx* c, const double* a, const double* b, const double*
 void fire_apply_convergence(dpmd_ctx* c, int pred, double a, double b);
 void fire_setup(dpmd_ctx* c, const double* dt);
 void run_begin_step(dpmd_ctx* c);
+// Clears the halt and the batch words; flag_one: also flags the system of fused FIRE batches
+// (fused_sys) for the rebuild (their launches raise only run words).
 void halt_reset(dpmd_ctx* c, bool flag_one);
 void per_system_temperature(dpmd_ctx* c);
 void per_system_pressure(dpmd_ctx* c);

@@ -132,10 +132,10 @@ __global__ void k_run_begin(int* run) {
 // A halted step batch, before the host finishes the halted step: clears the halt and the
 // batch words of fused batches, and flags system 0 (fused FIRE batches of one system raise
 // only the run words).
-__global__ void k_halt_reset(int* run, unsigned* flag, int flag_one) {
+__global__ void k_halt_reset(int* run, unsigned* flag, int flag_sys) {
     run[RUN_HALT] = 0; run[RUN_HALT_AT] = -1;
     run[RUN_ANY2] = 0; run[RUN_MIS] = 0; run[RUN_FIXW] = 0;
-    if (flag_one) flag[0] = 1u;
+    if (flag_sys >= 0) flag[flag_sys] = 1u;
 }
 
 // save_/restore_particle_state_kernel, save_/restore_system_state_kernel (disk.cu:348-432).
@@ -273,7 +273,7 @@ void scale_box_increment(dpmd_ctx* c, const double* inc, double* scale_out) {
 }
 
 void halt_reset(dpmd_ctx* c, bool flag_one) {
-    k_halt_reset<<<1, 1, 0, c->st>>>(c->run.p, c->flag.p, flag_one ? 1 : 0);
+    k_halt_reset<<<1, 1, 0, c->st>>>(c->run.p, c->flag.p, flag_one ? fused_sys(c).s : -1);
     HIP_OK(hipGetLastError());
 }
 
