@@ -160,3 +160,32 @@ def test_hand_off_to_per_step_launches_mid_run():
             assert np.array_equal(P.get(f), ref[0].get(f)), (persist, f)
     ns = ref[0].get("FIRE_STEPS")
     assert ns[:200].max() < ns[200]   # the large system outlasts the small ones
+
+
+def test_jamming_hand_off_shrinks_to_one_fused_system():
+    """jam_compress (frozen systems masked) on 60 systems of 256 disks and one of 2048: the
+    rounds are handed to per-step launches over the running systems, whose set shrinks as
+    they freeze, and a system left running alone takes the fused FIRE step on its slice of
+    the batch (counted with HIP events).  Equal, bit for bit, to the same jamming with
+    per-step launches over the whole batch."""
+    b = W.disk_batch([256] * 60 + [2048], [0.80] * 60 + [0.845], [0] * 30 + [1] * 30 + [0], seed=5)
+    out = []
+    for persist in (0, 1):
+        P = Disk()
+        W.load_into(P, b)
+        P.init_neighbors()
+        L.check(L.lib().dpmd_set_option(P.ctx, L.OPT_PERSISTENT_FIRE, persist), P.ctx)
+        print(f"jam hand-off persist={persist} S={b.S} N={b.N}", flush=True)
+        # the hand-off run is launched directly, with HIP events around its fused FIRE steps
+        L.check(L.lib().dpmd_profile(P.ctx, 3 if persist else 0), P.ctx)
+        st, nc, work = jam_compress(P, 1e-2, 4e-3, 1e-7, 3, 30000, L.FIRE_PRED_MAX_FORCE, 1e-9,
+                                    mode=L.FORCE_PER_SYSTEM, use_graph=not persist)
+        ms, cnt = C.c_double(), C.c_int()
+        L.check(L.lib().dpmd_profile_read(P.ctx, C.byref(ms), C.byref(cnt)), P.ctx)
+        out.append((st, nc, work, {f: P.get(f) for f in FIELDS}, cnt.value))
+    (s0, n0, w0, f0, _), (s1, n1, w1, f1, fused_steps) = out
+    print(f"fused FIRE steps of the last system: {fused_steps}", flush=True)
+    assert fused_steps > 0, "the last running system should take the fused FIRE step"
+    assert np.array_equal(s0, s1) and np.array_equal(n0, n1) and w0 == w1
+    for f in FIELDS:
+        assert np.array_equal(f0[f], f1[f]), f
