@@ -46,7 +46,9 @@ RigidArgs rigid_args(dpmd_ctx* c) {
 
 // compute_pair_forces_kernel (rigid_bumpy.cu:164-221) / compute_wall_forces_kernel (:224-307);
 // in walled boxes the pair displacement is xi - xj (repulsive), the reference writes xj - xi.
-template <bool WALL>
+// PE: also the vertex energy (the steps inside a fused velocity-Verlet batch skip it: their
+// energies are never read, and the batch's last step and a halted step compute them).
+template <bool WALL, bool PE = true>
 __device__ __forceinline__ void vertex_force_at(const RigidArgs& a, int i, int s) {
     const double ei = a.e[s];
     const double xi = a.vpx[i], yi = a.vpy[i], ri = a.vrad[i];
@@ -89,20 +91,21 @@ __device__ __forceinline__ void vertex_force_at(const RigidArgs& a, int i, int s
             pei += (0.5 * ei * delta * delta) * 0.5;
         }
     }
-    a.vfx[i] = fxi; a.vfy[i] = fyi; a.vpe[i] = pei;
+    a.vfx[i] = fxi; a.vfy[i] = fyi;
+    if (PE) a.vpe[i] = pei;
 }
 
-template <int MODE>
+template <int MODE, bool PE = true>
 __global__ void __launch_bounds__(DPMD_BLOCK) k_vertex_force(RigidArgs a, Gate g) {
     if (!gate_force(g)) return;
     const int i = xcd_remap(blockIdx.x, gridDim.x) * DPMD_BLOCK + threadIdx.x;
     if (i >= a.nv) return;
     const int s = a.vsid[i];
     if (a.frozen && a.frozen[s]) return;
-    if (MODE == DPMD_FORCE_PBC) vertex_force_at<false>(a, i, s);
-    else if (MODE == DPMD_FORCE_WALL) vertex_force_at<true>(a, i, s);
-    else if (a.boundary[s]) vertex_force_at<true>(a, i, s);
-    else vertex_force_at<false>(a, i, s);
+    if (MODE == DPMD_FORCE_PBC) vertex_force_at<false, PE>(a, i, s);
+    else if (MODE == DPMD_FORCE_WALL) vertex_force_at<true, PE>(a, i, s);
+    else if (a.boundary[s]) vertex_force_at<true, PE>(a, i, s);
+    else vertex_force_at<false, PE>(a, i, s);
 }
 
 // compute_particle_forces_kernel (rigid_bumpy.cu:331-370): canonical vertex order.
@@ -509,14 +512,16 @@ k_particle_v(RigidArgs a, const double* __restrict__ dth, const double* __restri
 
 // Velocity Verlet across a step boundary for G particles: k_particle_v's sums and closing half
 // kick, then k_vv_drift_v's opening half kick, drift, vertex rotation and skin test, each in
-// the same expressions and order, so a fused run equals the two-launch run bit for bit.  Each
+// the same expressions and order, so a fused run equals the two-launch run bit for bit.  The
+// particle energies of these steps are not formed (nothing reads them: a batch's last step
+// and a halted step run k_particle_v, which does).  Each
 // vertex position is loaded once, for its torque term and its rotation; a workgroup reads and
 // writes only its own particles' vertices, so they are updated in place.
 __global__ void __launch_bounds__(DPMD_BLOCK)
 k_vv_fused_v(RigidArgs a, const double* __restrict__ dt, const double* __restrict__ dth,
              const double* __restrict__ damp, int G, Gate g) {
     if (!gate_open(g)) return;
-    __shared__ double l_fx[DPMD_BLOCK], l_fy[DPMD_BLOCK], l_pe[DPMD_BLOCK], l_tq[DPMD_BLOCK];
+    __shared__ double l_fx[DPMD_BLOCK], l_fy[DPMD_BLOCK], l_tq[DPMD_BLOCK];
     __shared__ double l_x0[kRigidGroupMax], l_y0[kRigidGroupMax], l_nx[kRigidGroupMax], l_ny[kRigidGroupMax];
     __shared__ double l_cs[kRigidGroupMax], l_sn[kRigidGroupMax];
     __shared__ int l_off[kRigidGroupMax + 1];
@@ -536,18 +541,17 @@ k_vv_fused_v(RigidArgs a, const double* __restrict__ dt, const double* __restric
         const double fvx = a.vfx[v], fvy = a.vfy[v];
         vx0 = a.vpx[v]; vy0 = a.vpy[v];
         const double dx = vx0 - a.x[i], dy = vy0 - a.y[i];
-        l_fx[t] = fvx; l_fy[t] = fvy; l_pe[t] = a.vpe[v];
+        l_fx[t] = fvx; l_fy[t] = fvy;
         l_tq[t] = fvy * dx - fvx * dy;
     }
     __syncthreads();
     if (t < np) {
         const int i = p0 + t;
         const int s = a.sid[i];
-        double fx = 0.0, fy = 0.0, tq = 0.0, pe = 0.0;
+        double fx = 0.0, fy = 0.0, tq = 0.0;
         for (int k = l_off[t] - l_off[0]; k < l_off[t + 1] - l_off[0]; ++k) {
             fx += l_fx[k];
             fy += l_fy[k];
-            pe += l_pe[k];
             tq += l_tq[k];
         }
         double vx = a.vx[i], vy = a.vy[i], w = a.w[i];
@@ -557,7 +561,7 @@ k_vv_fused_v(RigidArgs a, const double* __restrict__ dt, const double* __restric
             fy += -vy * sc;
             tq += -w * sc;
         }
-        a.fx[i] = fx; a.fy[i] = fy; a.tq[i] = tq; a.pe[i] = pe;
+        a.fx[i] = fx; a.fy[i] = fy; a.tq[i] = tq;   // no energy: see rigid_vv_fused
         kick_one(a, i, dth[s] * 1.0, fx, fy, tq, vx, vy, w);     // end of this step
         kick_one(a, i, dth[s] * 1.0, fx, fy, tq, vx, vy, w);     // next step: update_velocities(dt_half)
         const double sc = dt[s] * 1.0;                           // update_positions(dt)
@@ -675,15 +679,19 @@ int rigid_group(dpmd_ctx* c) {
     return std::min(DPMD_BLOCK - 1, DPMD_BLOCK / c->nv_max);
 }
 
-void launch_vertex_force(dpmd_ctx* c, int mode, Gate g) {
+void launch_vertex_force(dpmd_ctx* c, int mode, Gate g, bool pe = true) {
     RigidArgs a = rigid_args(c);
     const int G = blocks_for(c->Nv);
     ProfScope prof(c);
-    switch (mode) {
-        case DPMD_FORCE_PBC: k_vertex_force<DPMD_FORCE_PBC><<<G, DPMD_BLOCK, 0, c->st>>>(a, g); break;
-        case DPMD_FORCE_WALL: k_vertex_force<DPMD_FORCE_WALL><<<G, DPMD_BLOCK, 0, c->st>>>(a, g); break;
-        default: k_vertex_force<DPMD_FORCE_PER_SYSTEM><<<G, DPMD_BLOCK, 0, c->st>>>(a, g); break;
-    }
+    auto go = [&](auto pv) {
+        constexpr bool PE = decltype(pv)::value;
+        switch (mode) {
+            case DPMD_FORCE_PBC: k_vertex_force<DPMD_FORCE_PBC, PE><<<G, DPMD_BLOCK, 0, c->st>>>(a, g); break;
+            case DPMD_FORCE_WALL: k_vertex_force<DPMD_FORCE_WALL, PE><<<G, DPMD_BLOCK, 0, c->st>>>(a, g); break;
+            default: k_vertex_force<DPMD_FORCE_PER_SYSTEM, PE><<<G, DPMD_BLOCK, 0, c->st>>>(a, g); break;
+        }
+    };
+    if (pe) go(std::true_type{}); else go(std::false_type{});
     HIP_OK(hipGetLastError());
 }
 
@@ -759,7 +767,7 @@ bool rigid_vv_fusable(dpmd_ctx* c) { return rigid_group(c) != 0; }
 void rigid_vv_fused(dpmd_ctx* c, Gate g) {
     const int G = rigid_group(c);
     need(G != 0, "rigid fused velocity Verlet: a particle has more vertices than a workgroup holds");
-    launch_vertex_force(c, c->vv_mode, g);
+    launch_vertex_force(c, c->vv_mode, g, false);   // k_vv_fused_v reads no vertex energy
     k_vv_fused_v<<<(c->N + G - 1) / G, DPMD_BLOCK, 0, c->st>>>(rigid_args(c), c->vv_dt.p, c->vv_dth.p, vv_damping(c),
                                                               G, g);
     HIP_OK(hipGetLastError());
