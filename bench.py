@@ -402,10 +402,14 @@ def run_rigid(args):
     # launches directly; its batches halt for rebuilds (the host finishes those steps) instead
     # of running the device rebuild chain each step, whose many short launches would leave
     # the stream waiting on the host between an event and its kernel.
+    # Its steps are two-launch steps, so every timed launch is the full kernel (with the vertex
+    # energy, whose bytes are counted below); inside fused batches the steps skip that energy.
     L.check(lib.dpmd_set_option(P.ctx, L.OPT_DEVICE_REBUILD, 1), P.ctx)
+    L.check(lib.dpmd_set_option(P.ctx, L.OPT_FUSED_VV, 0), P.ctx)
     lib.dpmd_profile(P.ctx, 1)                        # HIP events around every vertex-force launch
     vv.run(n_prof, use_graph=False)
     L.check(lib.dpmd_set_option(P.ctx, L.OPT_DEVICE_REBUILD, 0), P.ctx)
+    L.check(lib.dpmd_set_option(P.ctx, L.OPT_FUSED_VV, 1), P.ctx)
     P.synchronize()
     t_nve = time.perf_counter() - t1
     tot_ms, cnt = C.c_double(), C.c_int()
@@ -777,11 +781,15 @@ def leg_rigid(args):
     vv.run(n_nve, use_graph=not args.no_graph)
     P.synchronize()
     t_nve = time.perf_counter() - t1
+    # the tail runs two-launch steps, so every timed launch is the full kernel (with the
+    # vertex energy) whose bytes are counted below; inside fused batches the steps skip it
     L.check(lib.dpmd_set_option(P.ctx, L.OPT_DEVICE_REBUILD, 1), P.ctx)
+    L.check(lib.dpmd_set_option(P.ctx, L.OPT_FUSED_VV, 0), P.ctx)
     lib.dpmd_profile(P.ctx, 1)
     vv.run(300, use_graph=False)
     P.synchronize()
     L.check(lib.dpmd_set_option(P.ctx, L.OPT_DEVICE_REBUILD, 0), P.ctx)
+    L.check(lib.dpmd_set_option(P.ctx, L.OPT_FUSED_VV, 1), P.ctx)
     tot_ms, cnt = C.c_double(), C.c_int()
     L.check(lib.dpmd_profile_read(P.ctx, C.byref(tot_ms), C.byref(cnt)), P.ctx)
     lib.dpmd_profile(P.ctx, 0)
